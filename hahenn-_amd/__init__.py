@@ -1,0 +1,282 @@
+"""suiri (MI355X) — Python host binding of the liquid-step C ABI (include/suiri.h).
+
+The product is the HIP library libsuiri_amd.so (built in-tree from csrc/ for gfx950). This module
+binds it with ctypes and mirrors the reference's JavaScript driver interface (class FluidSim,
+src/js/50-sim.js) for Python callers (tests, bench, the batch and serving front-ends). The Node host
+(js/fluid_sim.js through the N-API addon) is the drop-in for the reference's own callers.
+
+There is no fallback: loading fails loudly when the library is missing.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libsuiri_amd.so")
+
+SL_OK = 0
+F_PARTICLES, F_WW, F_COUNTERS, F_ACC, F_CELLCOUNT, F_VEL, F_VELOLD, F_FRAC, F_SVEL, F_CELLTYPE, \
+    F_DIVG, F_PRES, F_OBJS, F_RSRC, F_RSTATE, F_RHAT, F_RFIELD, F_RMASK, F_RTEX, F_TOPH, F_RTEX16 = range(21)
+S_GEOM, S_P2G, S_NORMFORCE, S_DIVERGENCE, S_PRESSURE, S_PROJECT, S_G2P, S_COMPACT, S_WW, S_RIPPLE, \
+    S_SOR = range(11)
+T_NAMES = ["emit", "clear", "geom", "p2g", "normforce", "divergence", "sor", "project", "g2p", "compact",
+           "ww", "ripple"]
+
+FIELD_DTYPE = {F_PARTICLES: (np.float32, 8), F_WW: (np.float32, 8), F_COUNTERS: (np.uint32, 1),
+               F_ACC: (np.int32, 6), F_CELLCOUNT: (np.uint32, 1), F_VEL: (np.float32, 4),
+               F_VELOLD: (np.float32, 4), F_FRAC: (np.float32, 4), F_SVEL: (np.float32, 4),
+               F_CELLTYPE: (np.uint32, 1), F_DIVG: (np.float32, 1), F_PRES: (np.float32, 1),
+               F_OBJS: (np.float32, 28), F_RSRC: (np.int32, 1), F_RSTATE: (np.float32, 2),
+               F_RHAT: (np.float32, 2), F_RFIELD: (np.float32, 2), F_RMASK: (np.float32, 1),
+               F_RTEX: (np.float32, 4), F_TOPH: (np.float32, 1), F_RTEX16: (np.float16, 4)}
+
+
+class SlConfig(ctypes.Structure):
+    _fields_ = [
+        ("tank_w", ctypes.c_double), ("tank_h", ctypes.c_double), ("tank_d", ctypes.c_double),
+        ("h", ctypes.c_double), ("cap", ctypes.c_uint32), ("ww_cap", ctypes.c_uint32),
+        ("iters", ctypes.c_uint32), ("ripple_R", ctypes.c_uint32), ("rc_factor", ctypes.c_double),
+        ("gravity", ctypes.c_float), ("flip", ctypes.c_float), ("ppc", ctypes.c_float),
+        ("drift", ctypes.c_float), ("omega", ctypes.c_float), ("friction", ctypes.c_float),
+        ("ww_rate", ctypes.c_float), ("r_sigma_rho", ctypes.c_float), ("r_nu", ctypes.c_float),
+        ("r_film", ctypes.c_float), ("r_hp_k", ctypes.c_float), ("r_rho_kick", ctypes.c_float),
+        ("wmap", ctypes.c_uint32), ("whitewater", ctypes.c_uint32), ("device", ctypes.c_int32),
+        ("reorder_every", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4),
+    ]
+
+
+class SlEmit(ctypes.Structure):
+    _fields_ = [("a", ctypes.c_float * 4), ("b", ctypes.c_float * 4), ("c", ctypes.c_float * 4),
+                ("d", ctypes.c_float * 4), ("shape", ctypes.c_uint32), ("count", ctypes.c_uint32),
+                ("seed", ctypes.c_uint32), ("pad", ctypes.c_uint32)]
+
+
+class SlGrid(ctypes.Structure):
+    _fields_ = [("nx", ctypes.c_int32), ("ny", ctypes.c_int32), ("nz", ctypes.c_int32),
+                ("nodes", ctypes.c_uint32), ("cells", ctypes.c_uint32), ("cap", ctypes.c_uint32),
+                ("ww_cap", ctypes.c_uint32), ("R", ctypes.c_uint32), ("M", ctypes.c_uint32),
+                ("iters", ctypes.c_uint32), ("h", ctypes.c_double), ("origin", ctypes.c_double * 3),
+                ("rim", ctypes.c_double), ("vp", ctypes.c_double), ("radius", ctypes.c_double)]
+
+
+class SlStats(ctypes.Structure):
+    _fields_ = [("count", ctypes.c_uint32), ("ww_count", ctypes.c_uint32), ("impacts", ctypes.c_uint32),
+                ("frames", ctypes.c_uint32), ("full", ctypes.c_uint32), ("gpu_errors", ctypes.c_uint32),
+                ("substeps_last", ctypes.c_uint32), ("reserved", ctypes.c_uint32),
+                ("vmax", ctypes.c_double), ("time", ctypes.c_double), ("liters", ctypes.c_double),
+                ("grid", ctypes.c_int32 * 3)]
+
+
+# every entry point declared in include/suiri.h
+EXPORTS = ["sl_config_default", "sl_create", "sl_destroy", "sl_last_error", "sl_grid_info", "sl_set_objects",
+           "sl_emit_now", "sl_step_begin", "sl_substep", "sl_step_end", "sl_step", "sl_set_substeps",
+           "sl_clear_water", "sl_save_water", "sl_has_water_copy", "sl_restore_water", "sl_poll_stats",
+           "sl_stats", "sl_sync", "sl_set_ripple_mask", "sl_field_bytes", "sl_upload", "sl_readback",
+           "sl_set_scalar", "sl_get_scalar", "sl_run_stage", "sl_timing_enable", "sl_timing_read",
+           "sl_version", "sl_build_info"]
+
+_lib = None
+
+
+class SuiriError(RuntimeError):
+    pass
+
+
+def load():
+    """Load libsuiri_amd.so (raises when it is missing: there is no CPU fallback)."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIB_PATH):
+        raise SuiriError(f"suiri HIP library not built: {LIB_PATH} (run __graft_entry__.build())")
+    L = ctypes.CDLL(LIB_PATH)
+    P, I, U, D = ctypes.c_void_p, ctypes.c_int, ctypes.c_uint32, ctypes.c_double
+    sig = {
+        "sl_config_default": (I, [ctypes.POINTER(SlConfig), ctypes.c_char_p]),
+        "sl_create": (I, [ctypes.POINTER(SlConfig), ctypes.POINTER(P)]),
+        "sl_destroy": (None, [P]),
+        "sl_last_error": (ctypes.c_char_p, [P]),
+        "sl_grid_info": (I, [P, ctypes.POINTER(SlGrid)]),
+        "sl_set_objects": (I, [P, P, U, ctypes.POINTER(I)]),
+        "sl_emit_now": (I, [P, P, U, I, ctypes.POINTER(U)]),
+        "sl_step_begin": (I, [P, D, ctypes.POINTER(U), ctypes.POINTER(D)]),
+        "sl_substep": (I, [P, D, P, U, ctypes.POINTER(U)]),
+        "sl_step_end": (I, [P, D]),
+        "sl_step": (I, [P, D, P, P]),
+        "sl_set_substeps": (I, [P, U]),
+        "sl_clear_water": (I, [P]),
+        "sl_save_water": (I, [P, ctypes.POINTER(U)]),
+        "sl_has_water_copy": (I, [P, U]),
+        "sl_restore_water": (I, [P, U]),
+        "sl_poll_stats": (I, [P, ctypes.POINTER(I)]),
+        "sl_stats": (I, [P, ctypes.POINTER(SlStats)]),
+        "sl_sync": (I, [P]),
+        "sl_set_ripple_mask": (I, [P, I]),
+        "sl_field_bytes": (ctypes.c_size_t, [P, I]),
+        "sl_upload": (I, [P, I, P, ctypes.c_size_t]),
+        "sl_readback": (I, [P, I, P, ctypes.c_size_t]),
+        "sl_set_scalar": (I, [P, ctypes.c_char_p, D]),
+        "sl_get_scalar": (I, [P, ctypes.c_char_p, ctypes.POINTER(D)]),
+        "sl_run_stage": (I, [P, I, U, D]),
+        "sl_timing_enable": (I, [P, I]),
+        "sl_timing_read": (I, [P, ctypes.POINTER(D), ctypes.POINTER(U), I, I]),
+        "sl_version": (I, []),
+        "sl_build_info": (ctypes.c_char_p, []),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(L, name)
+        fn.restype = res
+        fn.argtypes = args
+    _lib = L
+    return L
+
+
+def config(quality="standard", **kw):
+    c = SlConfig()
+    L = load()
+    r = L.sl_config_default(ctypes.byref(c), quality.encode())
+    if r != SL_OK:
+        raise SuiriError(L.sl_last_error(None).decode())
+    for k, v in kw.items():
+        setattr(c, k, v)
+    return c
+
+
+def emit_records(records):
+    n = len(records)
+    arr = (SlEmit * max(n, 1))()
+    for i, r in enumerate(records):
+        for key in "abcd":
+            v = list(r.get(key) or [0, 0, 0, 0]) + [0, 0, 0, 0]
+            getattr(arr[i], key)[:] = [float(x) for x in v[:4]]
+        arr[i].shape = int(r["shape"])
+        arr[i].count = int(r["count"])
+        arr[i].seed = int(r.get("seed", 0)) & 0xFFFFFFFF
+    return arr
+
+
+class Sim:
+    """One simulation on the GPU through the C ABI (mirrors FluidSim, src/js/50-sim.js:28-376)."""
+
+    def __init__(self, cfg):
+        self.L = load()
+        self.ptr = ctypes.c_void_p()
+        self._check(self.L.sl_create(ctypes.byref(cfg), ctypes.byref(self.ptr)), None)
+        g = SlGrid()
+        self._check(self.L.sl_grid_info(self.ptr, ctypes.byref(g)))
+        self.grid = g
+        self.nx, self.ny, self.nz, self.h = g.nx, g.ny, g.nz, g.h
+        self.origin = list(g.origin)
+        self.vp, self.rim, self.cap = g.vp, g.rim, g.cap
+
+    def _check(self, r, ptr=0):
+        if r != SL_OK:
+            msg = self.L.sl_last_error(self.ptr if ptr is not None else None)
+            raise SuiriError(f"suiri error {r}: {msg.decode() if msg else ''}")
+
+    def close(self):
+        if self.ptr:
+            self.L.sl_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # FluidSim interface
+    def set_objects_rows(self, rows):
+        rows = np.ascontiguousarray(rows, dtype=np.float32).reshape(-1, 28)
+        ch = ctypes.c_int(0)
+        self._check(self.L.sl_set_objects(self.ptr, rows.ctypes.data, rows.shape[0], ctypes.byref(ch)))
+        return bool(ch.value)
+
+    def emit_now(self, records, fill=False):
+        arr = emit_records(records)
+        out = ctypes.c_uint32(0)
+        self._check(self.L.sl_emit_now(self.ptr, ctypes.byref(arr), len(records), int(fill), ctypes.byref(out)))
+        return out.value
+
+    def set_substeps(self, n):
+        self._check(self.L.sl_set_substeps(self.ptr, int(n)))
+
+    def step(self, frame_dt, emit_fn=None):
+        n = ctypes.c_uint32(0)
+        dt = ctypes.c_double(0)
+        self._check(self.L.sl_step_begin(self.ptr, float(frame_dt), ctypes.byref(n), ctypes.byref(dt)))
+        for _ in range(n.value):
+            recs = emit_fn(dt.value, self.scalar("time")) if emit_fn else []
+            arr = emit_records(recs)
+            self._check(self.L.sl_substep(self.ptr, dt.value, ctypes.byref(arr), len(recs), None))
+        self._check(self.L.sl_step_end(self.ptr, float(frame_dt)))
+        return n.value
+
+    def clear_water(self):
+        self._check(self.L.sl_clear_water(self.ptr))
+
+    def save_water(self):
+        i = ctypes.c_uint32(0)
+        self._check(self.L.sl_save_water(self.ptr, ctypes.byref(i)))
+        return i.value
+
+    def has_water_copy(self, i):
+        return bool(self.L.sl_has_water_copy(self.ptr, int(i)))
+
+    def restore_water(self, i):
+        return bool(self.L.sl_restore_water(self.ptr, int(i)))
+
+    def poll_stats(self):
+        u = ctypes.c_int(0)
+        self._check(self.L.sl_poll_stats(self.ptr, ctypes.byref(u)))
+        return bool(u.value)
+
+    def stats(self):
+        st = SlStats()
+        self._check(self.L.sl_stats(self.ptr, ctypes.byref(st)))
+        return {"count": st.count, "ww": st.ww_count, "impacts": st.impacts, "frames": st.frames,
+                "full": bool(st.full), "gpuErrors": st.gpu_errors, "vmax": st.vmax, "time": st.time,
+                "liters": st.liters, "grid": list(st.grid), "substeps": st.substeps_last}
+
+    def sync(self):
+        self._check(self.L.sl_sync(self.ptr))
+
+    def set_ripple_mask(self, bound):
+        self._check(self.L.sl_set_ripple_mask(self.ptr, int(bound)))
+
+    def scalar(self, name):
+        v = ctypes.c_double(0)
+        self._check(self.L.sl_get_scalar(self.ptr, name.encode(), ctypes.byref(v)))
+        return v.value
+
+    def set_scalar(self, name, value):
+        self._check(self.L.sl_set_scalar(self.ptr, name.encode(), float(value)))
+
+    # state transfer (reference layouts)
+    def readback(self, field):
+        nb = self.L.sl_field_bytes(self.ptr, field)
+        dt, w = FIELD_DTYPE[field]
+        out = np.empty(nb // np.dtype(dt).itemsize, dtype=dt)
+        self._check(self.L.sl_readback(self.ptr, field, out.ctypes.data, nb))
+        return out.reshape(-1, w) if w > 1 else out
+
+    def upload(self, field, arr):
+        nb = self.L.sl_field_bytes(self.ptr, field)
+        a = np.ascontiguousarray(arr)
+        if a.nbytes < nb:
+            raise SuiriError(f"upload of field {field}: {a.nbytes} bytes < {nb}")
+        self._check(self.L.sl_upload(self.ptr, field, a.ctypes.data, nb))
+
+    def run_stage(self, stage, arg=0, dt=1.0 / 60.0):
+        self._check(self.L.sl_run_stage(self.ptr, int(stage), int(arg), float(dt)))
+
+    # timing
+    def timing(self, enable=True):
+        self._check(self.L.sl_timing_enable(self.ptr, int(enable)))
+
+    def timing_read(self, reset=True):
+        ms = (ctypes.c_double * len(T_NAMES))()
+        ln = (ctypes.c_uint32 * len(T_NAMES))()
+        self._check(self.L.sl_timing_read(self.ptr, ms, ln, len(T_NAMES), int(reset)))
+        return {n: (ms[i], ln[i]) for i, n in enumerate(T_NAMES)}

@@ -1,0 +1,1060 @@
+// Host driver of the suiri library: the C ABI of include/suiri.h.
+//
+// Restates the orchestration of the reference's FluidSim (src/js/50-sim.js) and RippleLayer
+// (src/js/64-ripple.js:149-267) on one HIP stream per simulation. The grid sizing and the parameter
+// values follow the reference's double-precision JavaScript arithmetic, rounded to f32 where the
+// reference writes them into its uniform buffer.
+#include <hip/hip_runtime.h>
+#include <math.h>
+#include <stdio.h>
+#include <string.h>
+#include <string>
+#include <vector>
+#include "../../include/suiri.h"
+#include "sl_internal.h"
+
+static thread_local std::string g_err;
+
+// dynamic per-substep fields of a tank, written on the device by k_update (kernel arguments are
+// captured at launch, so the host can change them freely while earlier work is in flight)
+struct SlTankDyn {
+  float dt, gdt, time_f, rLx, rLz, rdt, rmd;
+  uint32_t frame, obj_count, emit_k, emit_total, iters;
+  int32_t mask_bound;
+};
+__global__ void k_update(SlTank *T, SlTankDyn d) {
+  if (threadIdx.x) return;
+  T->dt = d.dt; T->gdt = d.gdt; T->time_f = d.time_f;
+  T->rLx = d.rLx; T->rLz = d.rLz; T->rdt = d.rdt; T->rmd = d.rmd;
+  T->frame = d.frame; T->obj_count = d.obj_count; T->emit_k = d.emit_k; T->emit_total = d.emit_total;
+  T->iters = d.iters; T->mask_bound = d.mask_bound;
+}
+
+// pinned staging for host->device uploads issued on the stream: two halves, each reused only after
+// the event recorded when the stream last left it has completed
+struct Staging {
+  uint8_t *buf = nullptr;
+  size_t half = 0, off = 0;
+  int cur = 0;
+  hipEvent_t ev[2] = {nullptr, nullptr};
+  bool ev_valid[2] = {false, false};
+};
+
+struct sl_sim {
+  sl_config cfg;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::string err;
+  uint32_t gpu_errors = 0;
+  // host-side FluidSim state (50-sim.js:29-58)
+  int nx = 0, ny = 0, nz = 0;
+  double h = 0, origin[3] = {0, 0, 0}, rim = 0, vp = 0, radius = 0, time = 0, vmax = 1.0;
+  uint32_t nodes = 0, cells = 0, hw = 0, hcells = 0, cap = 0, ww_cap = 0, R = 0, M = 0, iters = 0;
+  uint32_t frame = 0, obj_count = 0, count = 0, ww_count = 0, impacts = 0, frames = 0;
+  bool full = false, geom_dirty = true;
+  int cur = 0, ww_cur = 0;
+  uint32_t forced_substeps = 0, substeps_last = 0;
+  double cur_dt = 1.0 / 60.0;
+  int mask_bound = 1;
+  float objs[SL_MAX_OBJ * 28];
+  // particle count upper bound for launch sizes
+  uint64_t emitted_total = 0, emitted_at_copy = 0;
+  uint32_t p_ub = 0;
+  // device
+  SlTank ht;             // host mirror of the device record
+  SlTank *d_tank = nullptr;
+  std::vector<void *> allocs;
+  SlCounters *h_counters = nullptr;  // pinned readback slot
+  hipEvent_t stats_ev = nullptr;
+  bool stats_pending = false;
+  Staging stg;
+  // saved water (50-sim.js:111-141)
+  struct Saved {
+    bool valid = false;
+    uint32_t id = 0;
+    void *parts = nullptr, *ww = nullptr, *counters = nullptr, *ripple = nullptr;
+    double time = 0;
+    uint32_t count = 0, ww_count = 0, impacts = 0;
+    bool full = false;
+    int cur = 0, ww_cur = 0;
+  } saved;
+  // timing
+  bool timing = false;
+  struct TEv { int stage; hipEvent_t a, b; };
+  std::vector<TEv> tev;
+  std::vector<hipEvent_t> ev_pool;
+  double t_ms[SL_T_COUNT_] = {0};
+  uint32_t t_launch[SL_T_COUNT_] = {0};
+};
+
+static uint32_t g_copies = 0;
+
+#define FAIL(sim, code, ...)                                               \
+  do {                                                                     \
+    char _b[512];                                                          \
+    snprintf(_b, sizeof _b, __VA_ARGS__);                                  \
+    if (sim) (sim)->err = _b; else g_err = _b;                             \
+    return code;                                                           \
+  } while (0)
+
+#define HIPCHK(sim, call)                                                                  \
+  do {                                                                                     \
+    hipError_t _e = (call);                                                                \
+    if (_e != hipSuccess) {                                                                \
+      if (sim) (sim)->gpu_errors++;                                                        \
+      FAIL(sim, SL_EHIP, "%s failed: %s", #call, hipGetErrorString(_e));                   \
+    }                                                                                      \
+  } while (0)
+
+static int check_launch(sl_sim *s, const char *what) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) {
+    s->gpu_errors++;
+    FAIL(s, SL_EHIP, "launch of %s failed: %s", what, hipGetErrorString(e));
+  }
+  return SL_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// timing
+// ------------------------------------------------------------------------------------------------
+static hipEvent_t ev_get(sl_sim *s) {
+  if (!s->ev_pool.empty()) { hipEvent_t e = s->ev_pool.back(); s->ev_pool.pop_back(); return e; }
+  hipEvent_t e = nullptr;
+  (void)hipEventCreate(&e);
+  return e;
+}
+struct StageScope {
+  sl_sim *s; int stage; hipEvent_t a = nullptr;
+  StageScope(sl_sim *s_, int st, uint32_t launches) : s(s_), stage(st) {
+    if (!s->timing) return;
+    a = ev_get(s);
+    (void)hipEventRecord(a, s->stream);
+    s->t_launch[st] += launches;
+  }
+  ~StageScope() {
+    if (!s->timing || !a) return;
+    hipEvent_t b = ev_get(s);
+    (void)hipEventRecord(b, s->stream);
+    s->tev.push_back({stage, a, b});
+  }
+};
+static void timing_collect(sl_sim *s) {
+  if (s->tev.empty()) return;
+  (void)hipStreamSynchronize(s->stream);
+  for (auto &t : s->tev) {
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, t.a, t.b) == hipSuccess) s->t_ms[t.stage] += ms;
+    s->ev_pool.push_back(t.a);
+    s->ev_pool.push_back(t.b);
+  }
+  s->tev.clear();
+}
+
+// ------------------------------------------------------------------------------------------------
+// staging
+// ------------------------------------------------------------------------------------------------
+static int stage_upload(sl_sim *s, void *dst, const void *src, size_t bytes) {
+  Staging &g = s->stg;
+  if (bytes > g.half) {  // large: synchronous
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+    return SL_OK;
+  }
+  if (g.off + bytes > g.half) {
+    HIPCHK(s, hipEventRecord(g.ev[g.cur], s->stream));
+    g.ev_valid[g.cur] = true;
+    g.cur ^= 1;
+    g.off = 0;
+    if (g.ev_valid[g.cur]) HIPCHK(s, hipEventSynchronize(g.ev[g.cur]));
+  }
+  uint8_t *p = g.buf + (size_t)g.cur * g.half + g.off;
+  memcpy(p, src, bytes);
+  g.off += (bytes + 255) & ~(size_t)255;
+  HIPCHK(s, hipMemcpyAsync(dst, p, bytes, hipMemcpyHostToDevice, s->stream));
+  return SL_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// work maps of a single tank
+// ------------------------------------------------------------------------------------------------
+static inline uint32_t cdiv(uint64_t a, uint64_t b) { return (uint32_t)((a + b - 1) / b); }
+static SlLaunch launch_of(sl_sim *s, uint64_t items, uint32_t per_wg = SL_WG) {
+  SlLaunch L;
+  L.w.tanks = s->d_tank;
+  L.w.start = nullptr;
+  L.w.ntanks = 1;
+  L.w.total = cdiv(items, per_wg);
+  L.stream = s->stream;
+  return L;
+}
+
+static int push_dyn(sl_sim *s, uint32_t emit_k = 0, uint32_t emit_total = 0) {
+  SlTankDyn d;
+  d.dt = s->ht.dt; d.gdt = s->ht.gdt; d.time_f = s->ht.time_f;
+  d.rLx = s->ht.rLx; d.rLz = s->ht.rLz; d.rdt = s->ht.rdt; d.rmd = s->ht.rmd;
+  d.frame = s->frame; d.obj_count = s->obj_count; d.emit_k = emit_k; d.emit_total = emit_total;
+  d.iters = s->iters; d.mask_bound = s->mask_bound;
+  hipLaunchKernelGGL(k_update, dim3(1), dim3(64), 0, s->stream, s->d_tank, d);
+  return check_launch(s, "update");
+}
+
+// writeParams, 50-sim.js:151-162 (the per-substep part; the static part is set at creation)
+static void write_params(sl_sim *s, double dt) {
+  s->cur_dt = dt;
+  s->ht.dt = (float)dt;
+  s->ht.gdt = s->ht.g * s->ht.dt;
+  s->ht.time_f = (float)s->time;
+  s->ht.frame = s->frame;
+  s->ht.obj_count = s->obj_count;
+}
+
+// ------------------------------------------------------------------------------------------------
+// config + creation
+// ------------------------------------------------------------------------------------------------
+extern "C" int sl_config_default(sl_config *c, const char *quality) {
+  if (!c) FAIL((sl_sim *)nullptr, SL_EARG, "sl_config_default: null config");
+  memset(c, 0, sizeof *c);
+  c->tank_w = 0.6; c->tank_h = 0.32; c->tank_d = 0.4;  // scene.tank default, 10-scene.js:246
+  // QUALITY, 50-sim.js:4-8
+  c->h = 0.0085; c->cap = 320000; c->ww_cap = 140000; c->iters = 40; c->ripple_R = 256;
+  if (quality && !strcmp(quality, "low")) { c->h = 0.012; c->cap = 140000; c->ww_cap = 60000; c->iters = 28; c->ripple_R = 128; }
+  else if (quality && !strcmp(quality, "high")) { c->h = 0.0065; c->cap = 700000; c->ww_cap = 260000; c->iters = 56; c->ripple_R = 512; }
+  else if (quality && strcmp(quality, "standard")) FAIL((sl_sim *)nullptr, SL_EARG, "unknown quality '%s'", quality);
+  // writeParams constants, 50-sim.js:11, :157-160
+  c->rc_factor = 0.32;
+  c->gravity = 9.81f; c->flip = 0.96f; c->ppc = 8.0f; c->drift = 6.0f; c->omega = 1.6f;
+  c->friction = 0.03f; c->ww_rate = 7.0f;
+  // ripple constants, 64-ripple.js:225, :268, :88
+  c->r_sigma_rho = (float)(0.0728 / 998.0); c->r_nu = 1.0e-6f; c->r_film = 0.3f;
+  c->r_hp_k = (float)(2.0 * M_PI / 0.12); c->r_rho_kick = 998.0f;
+  c->wmap = 256;
+  c->whitewater = 1;
+  c->device = 0;
+  c->reorder_every = 0;
+  return SL_OK;
+}
+
+static int alloc(sl_sim *s, void **p, size_t bytes) {
+  if (!bytes) bytes = 256;
+  hipError_t e = hipMalloc(p, bytes);
+  if (e != hipSuccess) FAIL(s, SL_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
+  s->allocs.push_back(*p);
+  e = hipMemset(*p, 0, bytes);
+  if (e != hipSuccess) FAIL(s, SL_EHIP, "hipMemset failed: %s", hipGetErrorString(e));
+  return SL_OK;
+}
+#define ALLOC(ptr, bytes)                                           \
+  do {                                                              \
+    void *_p = nullptr;                                             \
+    int _r = alloc(s, &_p, (bytes));                                \
+    if (_r) return _r;                                              \
+    (ptr) = (decltype(ptr))_p;                                      \
+  } while (0)
+
+static int create_impl(sl_sim *s) {
+  const sl_config &c = s->cfg;
+  if (!(c.h > 0) || !(c.tank_w > 0) || !(c.tank_h > 0) || !(c.tank_d > 0))
+    FAIL(s, SL_EARG, "tank size and cell size must be positive");
+  if (c.ripple_R && (c.ripple_R & (c.ripple_R - 1))) FAIL(s, SL_EARG, "ripple_R must be a power of two");
+  if (c.ripple_R > 2048) FAIL(s, SL_ECAP, "ripple_R above 2048");
+  if (c.cap == 0) FAIL(s, SL_EARG, "capacity must be positive");
+  HIPCHK(s, hipSetDevice(c.device));
+  s->device = c.device;
+  HIPCHK(s, hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  // grid sizing, 50-sim.js:33-45
+  s->nx = (int)fmax(8.0, round(c.tank_w / c.h));
+  s->ny = (int)fmax(8.0, round(c.tank_h / c.h));
+  s->nz = (int)fmax(8.0, round(c.tank_d / c.h));
+  s->h = fmin(fmin(c.tank_w / s->nx, c.tank_h / s->ny), c.tank_d / s->nz);
+  s->origin[0] = -s->nx * s->h / 2; s->origin[1] = 0; s->origin[2] = -s->nz * s->h / 2;
+  s->rim = s->ny * s->h;
+  uint64_t nodes = (uint64_t)(s->nx + 1) * (s->ny + 1) * (s->nz + 1);
+  if (nodes > 0x7fffffffull) FAIL(s, SL_ECAP, "grid too large");
+  s->nodes = (uint32_t)nodes;
+  s->cells = (uint32_t)s->nx * s->ny * s->nz;
+  s->hw = (uint32_t)(s->nx + 1) / 2;
+  s->hcells = s->hw * (uint32_t)s->ny * (uint32_t)s->nz;
+  s->cap = c.cap; s->ww_cap = c.ww_cap; s->iters = c.iters;
+  s->R = c.ripple_R; s->M = 2 * c.ripple_R;
+  s->vp = pow(s->h / 2, 3);
+  s->radius = 0.5 * s->h * 1.05;
+  s->time = 0; s->vmax = 1.0; s->frame = 0;
+  memset(s->objs, 0, sizeof s->objs);
+
+  SlTank &T = s->ht;
+  memset(&T, 0, sizeof T);
+  T.nx = s->nx; T.ny = s->ny; T.nz = s->nz; T.hw = (int32_t)s->hw;
+  T.nodes = s->nodes; T.cells = s->cells; T.hcells = s->hcells;
+  T.cap = s->cap; T.ww_cap = s->ww_cap; T.R = s->R; T.M = s->M; T.iters = s->iters;
+  uint32_t lm = 0;
+  while (s->M && (1u << lm) < s->M) lm++;
+  T.logM = lm;
+  T.wmap = c.wmap ? c.wmap : 256;
+  T.mask_bound = s->R ? 1 : 0;
+  s->mask_bound = T.mask_bound;
+  T.ox = (float)s->origin[0]; T.oy = (float)s->origin[1]; T.oz = (float)s->origin[2];
+  T.h = (float)s->h; T.invh = (float)(1.0 / s->h); T.hh = T.h * T.h;
+  T.g = c.gravity; T.flip = c.flip; T.ppc = c.ppc; T.drift = c.drift; T.omega = c.omega;
+  T.omega1 = 1.0f - T.omega;
+  T.rc = (float)(c.rc_factor * s->h);
+  T.rim = (float)s->rim;
+  T.ww_rate = c.whitewater ? c.ww_rate : 0.0f;
+  T.whitewater_on = c.whitewater ? 1 : 0;
+  T.ww_cap_f = (float)s->ww_cap;
+  T.friction = c.friction;
+  T.radius = (float)s->radius;
+  T.R_f = (float)s->R;
+  T.rsr = c.r_sigma_rho; T.rnu = c.r_nu; T.rfilm = c.r_film; T.rhpk = c.r_hp_k; T.rrho = c.r_rho_kick;
+  // buffers (50-sim.js:59-80, 64-ripple.js:152-165), structure-of-arrays
+  const size_t N = s->nodes, C = s->cells, H = s->hcells;
+  for (int b = 0; b < 2; b++) {
+    ALLOC(T.px[b], (size_t)s->cap * 4); ALLOC(T.py[b], (size_t)s->cap * 4); ALLOC(T.pz[b], (size_t)s->cap * 4);
+    ALLOC(T.vx[b], (size_t)s->cap * 4); ALLOC(T.vy[b], (size_t)s->cap * 4); ALLOC(T.vz[b], (size_t)s->cap * 4);
+    ALLOC(T.pf[b], (size_t)s->cap);
+    ALLOC(T.wp[b], (size_t)(s->ww_cap ? s->ww_cap : 1) * 16);
+    ALLOC(T.wv[b], (size_t)(s->ww_cap ? s->ww_cap : 1) * 16);
+  }
+  ALLOC(T.C, sizeof(SlCounters));
+  ALLOC(T.acc, N * 24);
+  ALLOC(T.cellCount, C * 4);
+  ALLOC(T.vel, N * 12); ALLOC(T.velOld, N * 12); ALLOC(T.vmask, N);
+  ALLOC(T.frac, N * 12); ALLOC(T.svel, N * 12);
+  ALLOC(T.divg, C * 4);
+  ALLOC(T.pres, H * 8); ALLOC(T.rhs, H * 8); ALLOC(T.sflag, H * 4);
+  ALLOC(T.objs, sizeof(SlObj) * SL_MAX_OBJ);
+  ALLOC(T.emits, sizeof(SlEmitDev) * SL_MAX_EMIT);
+  ALLOC(T.scanBlocks, (size_t)cdiv(s->cap, SL_COMPACT_ITEMS) * 4 + 64);
+  if (s->R) {
+    const size_t M2 = (size_t)s->M * s->M, R1 = (size_t)(s->R + 1) * (s->R + 1);
+    ALLOC(T.rsrc, R1 * 4);
+    ALLOC(T.rstate, M2 * 8); ALLOC(T.ra, M2 * 8); ALLOC(T.rb, M2 * 8); ALLOC(T.rhat, M2 * 8);
+    ALLOC(T.tw, (size_t)s->M * 8);
+    ALLOC(T.rmask, R1 * 4);
+    ALLOC(T.rtex, R1 * 16);
+    ALLOC(T.topH, (size_t)T.wmap * T.wmap * 4);
+    // twiddles e^(2 pi i t/M) in double, rounded to f32 (64-ripple.js:175-178)
+    std::vector<float> tw(2 * (size_t)s->M);
+    for (uint32_t t = 0; t < s->M; t++) {
+      tw[2 * t] = (float)cos((2 * M_PI * t) / s->M);
+      tw[2 * t + 1] = (float)sin((2 * M_PI * t) / s->M);
+    }
+    HIPCHK(s, hipMemcpy(T.tw, tw.data(), tw.size() * 4, hipMemcpyHostToDevice));
+  }
+  ALLOC(s->d_tank, sizeof(SlTank));
+  // pinned host memory
+  HIPCHK(s, hipHostMalloc((void **)&s->h_counters, sizeof(SlCounters), hipHostMallocDefault));
+  memset(s->h_counters, 0, sizeof(SlCounters));
+  HIPCHK(s, hipEventCreateWithFlags(&s->stats_ev, hipEventDisableTiming));
+  s->stg.half = 1 << 20;
+  HIPCHK(s, hipHostMalloc((void **)&s->stg.buf, 2 * s->stg.half, hipHostMallocDefault));
+  HIPCHK(s, hipEventCreateWithFlags(&s->stg.ev[0], hipEventDisableTiming));
+  HIPCHK(s, hipEventCreateWithFlags(&s->stg.ev[1], hipEventDisableTiming));
+  // writeParams(1/60) + COUNT(4), 50-sim.js:85-88
+  write_params(s, 1.0 / 60.0);
+  HIPCHK(s, hipMemcpy(s->d_tank, &s->ht, sizeof(SlTank), hipMemcpyHostToDevice));
+  HIPCHK(s, hipDeviceSynchronize());
+  return SL_OK;
+}
+
+extern "C" int sl_create(const sl_config *cfg, sl_sim **out) {
+  if (!cfg || !out) FAIL((sl_sim *)nullptr, SL_EARG, "sl_create: null argument");
+  *out = nullptr;
+  sl_sim *s = new (std::nothrow) sl_sim();
+  if (!s) FAIL((sl_sim *)nullptr, SL_ENOMEM, "out of host memory");
+  s->cfg = *cfg;
+  int r = create_impl(s);
+  if (r) {
+    g_err = s->err;
+    sl_destroy(s);
+    return r;
+  }
+  *out = s;
+  return SL_OK;
+}
+
+extern "C" void sl_destroy(sl_sim *s) {
+  if (!s) return;
+  if (s->stream) (void)hipStreamSynchronize(s->stream);
+  for (void *p : s->allocs) (void)hipFree(p);
+  if (s->saved.parts) { (void)hipFree(s->saved.parts); (void)hipFree(s->saved.ww); (void)hipFree(s->saved.counters); }
+  if (s->saved.ripple) (void)hipFree(s->saved.ripple);
+  if (s->h_counters) (void)hipHostFree(s->h_counters);
+  if (s->stg.buf) (void)hipHostFree(s->stg.buf);
+  for (int i = 0; i < 2; i++) if (s->stg.ev[i]) (void)hipEventDestroy(s->stg.ev[i]);
+  if (s->stats_ev) (void)hipEventDestroy(s->stats_ev);
+  for (auto &t : s->tev) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
+  for (auto e : s->ev_pool) (void)hipEventDestroy(e);
+  if (s->stream) (void)hipStreamDestroy(s->stream);
+  delete s;
+}
+
+extern "C" const char *sl_last_error(const sl_sim *s) { return s ? s->err.c_str() : g_err.c_str(); }
+
+extern "C" int sl_grid_info(const sl_sim *s, sl_grid_t *o) {
+  if (!s || !o) FAIL((sl_sim *)nullptr, SL_EARG, "sl_grid_info: null argument");
+  o->nx = s->nx; o->ny = s->ny; o->nz = s->nz; o->nodes = s->nodes; o->cells = s->cells;
+  o->cap = s->cap; o->ww_cap = s->ww_cap; o->R = s->R; o->M = s->M; o->iters = s->iters;
+  o->h = s->h; o->origin[0] = s->origin[0]; o->origin[1] = s->origin[1]; o->origin[2] = s->origin[2];
+  o->rim = s->rim; o->vp = s->vp; o->radius = s->radius;
+  return SL_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// objects, 50-sim.js:194-210
+// ------------------------------------------------------------------------------------------------
+static bool same_f(float a, float b) { return a == b || (a != a && b != b); }
+extern "C" int sl_set_objects(sl_sim *s, const float *rows, uint32_t n, int *changed) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  if (n && !rows) FAIL(s, SL_EARG, "sl_set_objects: null rows");
+  if (n > SL_MAX_OBJ) n = SL_MAX_OBJ;
+  bool ch = n != s->obj_count;
+  for (uint32_t i = 0; !ch && i < n * 28; i++) ch = !same_f(s->objs[i], rows[i]);
+  if (changed) *changed = ch ? 1 : 0;
+  if (!ch) return SL_OK;
+  memset(s->objs, 0, sizeof s->objs);
+  if (n) memcpy(s->objs, rows, (size_t)n * 28 * 4);
+  s->obj_count = n;
+  s->geom_dirty = true;
+  return stage_upload(s, s->ht.objs, s->objs, sizeof(SlObj) * (n ? n : 1));
+}
+
+// ------------------------------------------------------------------------------------------------
+// emission: writeEmits (50-sim.js:222-245) + EMIT + COUNT(2) (50-sim.js:267-274)
+// ------------------------------------------------------------------------------------------------
+static int write_emits(sl_sim *s, const sl_emit *list, uint32_t n, uint32_t *total_out, uint32_t *k_out) {
+  SlEmitDev em[SL_MAX_EMIT];
+  uint32_t off = 0, k = 0;
+  int64_t room = (int64_t)s->cap - (int64_t)s->count - 64;
+  int64_t budget = room > 0 ? room : 0;
+  bool any = false;
+  for (uint32_t i = 0; i < n; i++) if (list[i].count > 0) any = true;
+  for (uint32_t i = 0; i < n; i++) {
+    if (k >= SL_MAX_EMIT || budget <= 0) break;
+    int64_t cnt = (int64_t)list[i].count < budget ? (int64_t)list[i].count : budget;
+    if (cnt <= 0) continue;
+    memcpy(em[k].a, list[i].a, 16); memcpy(em[k].b, list[i].b, 16);
+    memcpy(em[k].c, list[i].c, 16); memcpy(em[k].d, list[i].d, 16);
+    em[k].info[0] = off; em[k].info[1] = (uint32_t)cnt; em[k].info[2] = list[i].seed; em[k].info[3] = list[i].shape;
+    off += (uint32_t)cnt; budget -= cnt; k++;
+  }
+  if (any && budget <= 0) s->full = true;
+  *total_out = off; *k_out = k;
+  if (off > 0) {
+    int r = stage_upload(s, s->ht.emits, em, sizeof(SlEmitDev) * k);
+    if (r) return r;
+    s->count += off;
+  }
+  return SL_OK;
+}
+static int encode_emit(sl_sim *s, uint32_t k, uint32_t total) {
+  int r = push_dyn(s, k, total);
+  if (r) return r;
+  {
+    StageScope sc(s, SL_T_EMIT, 2);
+    sl_k_emit(launch_of(s, total));
+    sl_k_count(launch_of(s, 1), 2, total);
+  }
+  s->emitted_total += total;
+  uint64_t ub = (uint64_t)s->p_ub + total;
+  s->p_ub = (uint32_t)(ub < s->cap ? ub : s->cap);
+  return check_launch(s, "emit");
+}
+
+extern "C" int sl_emit_now(sl_sim *s, const sl_emit *list, uint32_t n, int fill, uint32_t *emitted) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  if (n && !list) FAIL(s, SL_EARG, "sl_emit_now: null list");
+  uint32_t sum = 0;
+  for (uint32_t i = 0; i < n; i += SL_MAX_EMIT) {
+    uint32_t m = n - i < SL_MAX_EMIT ? n - i : SL_MAX_EMIT;
+    uint32_t total = 0, k = 0;
+    int r = write_emits(s, list + i, m, &total, &k);
+    if (r) return r;
+    if (!total) continue;
+    if (fill) {
+      // fresh per-cell counts so that the pour only fills cells without water (50-sim.js:253-261)
+      sl_k_fill_u32(s->stream, s->ht.cellCount, 0, s->cells);
+      sl_k_p2g(launch_of(s, s->p_ub), 1);
+    }
+    r = encode_emit(s, k, total);
+    if (r) return r;
+    sum += total;
+  }
+  if (emitted) *emitted = sum;
+  return check_launch(s, "emit_now");
+}
+
+// ------------------------------------------------------------------------------------------------
+// step, 50-sim.js:300-336
+// ------------------------------------------------------------------------------------------------
+extern "C" int sl_set_substeps(sl_sim *s, uint32_t n) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  s->forced_substeps = n;
+  return SL_OK;
+}
+
+extern "C" int sl_step_begin(sl_sim *s, double frame_dt, uint32_t *nsub, double *dt) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  if (!(frame_dt > 0)) { if (nsub) *nsub = 0; if (dt) *dt = 0; return SL_OK; }
+  uint32_t n;
+  if (s->forced_substeps) n = s->forced_substeps;
+  else {
+    double vmax = s->vmax > 0.4 ? s->vmax : 0.4;
+    double q = ceil(frame_dt / (1.4 * s->h / vmax));
+    n = (uint32_t)(q < 1 ? 1 : (q > 6 ? 6 : q));
+  }
+  s->substeps_last = n;
+  if (nsub) *nsub = n;
+  if (dt) *dt = frame_dt / n;
+  sl_k_count(launch_of(s, 1), 5, 0);  // counters.vmax = 0
+  return check_launch(s, "step_begin");
+}
+
+static int encode_substep(sl_sim *s) {
+  const uint32_t pb = s->p_ub;
+  if (s->geom_dirty) {
+    StageScope sc(s, SL_T_GEOM, 1);
+    sl_k_geom(launch_of(s, s->nodes));
+    s->geom_dirty = false;
+  }
+  {
+    StageScope sc(s, SL_T_P2G, 2);
+    sl_k_fill_u32(s->stream, s->ht.cellCount, 0, s->cells);
+    sl_k_p2g(launch_of(s, pb), 0);
+  }
+  { StageScope sc(s, SL_T_NORMFORCE, 1); sl_k_normforce(launch_of(s, s->nodes)); }
+  { StageScope sc(s, SL_T_DIVERGENCE, 1); sl_k_divsetup(launch_of(s, s->cells)); }
+  {
+    StageScope sc(s, SL_T_SOR, 2 * s->iters);
+    SlLaunch L = launch_of(s, s->hcells);
+    for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor(L, 0, 2 * i); sl_k_sor(L, 1, 2 * i + 1); }
+  }
+  { StageScope sc(s, SL_T_PROJECT, 1); sl_k_project(launch_of(s, s->nodes)); }
+  { StageScope sc(s, SL_T_G2P, 1); sl_k_g2p(launch_of(s, pb)); }
+  return check_launch(s, "substep");
+}
+
+extern "C" int sl_substep(sl_sim *s, double dt, const sl_emit *list, uint32_t n, uint32_t *emitted) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  if (n && !list) FAIL(s, SL_EARG, "sl_substep: null list");
+  s->frame++;
+  write_params(s, dt);
+  uint32_t total = 0, k = 0;
+  int r = write_emits(s, list, n, &total, &k);
+  if (r) return r;
+  if (total > 0) { r = encode_emit(s, k, total); if (r) return r; }
+  r = push_dyn(s);
+  if (r) return r;
+  r = encode_substep(s);
+  if (r) return r;
+  s->time += dt;
+  if (emitted) *emitted = total;
+  return SL_OK;
+}
+
+static int encode_ripple(sl_sim *s, double frame_dt, double mean_depth) {
+  if (!s->R) return SL_OK;
+  s->ht.rLx = (float)(s->nx * s->h); s->ht.rLz = (float)(s->nz * s->h);
+  s->ht.rdt = (float)frame_dt; s->ht.rmd = (float)mean_depth;
+  int r = push_dyn(s);
+  if (r) return r;
+  StageScope sc(s, SL_T_RIPPLE, s->mask_bound ? 10 : 6);
+  const uint32_t M = s->M, R1 = s->R + 1;
+  uint32_t CW = 4096 / M; if (CW > M) CW = M; if (CW < 1) CW = 1;
+  sl_k_ripple_frame_ex(launch_of(s, M, 1).w, launch_of(s, M, CW).w, launch_of(s, (uint64_t)M * M).w,
+                       launch_of(s, (uint64_t)R1 * R1).w, s->stream, s->mask_bound);
+  return check_launch(s, "ripple");
+}
+
+static int encode_compact(sl_sim *s) {
+  StageScope sc(s, SL_T_COMPACT, 4);
+  SlLaunch L = launch_of(s, s->p_ub, SL_COMPACT_ITEMS);
+  sl_k_compact_count(L);
+  sl_k_compact_scan(L, 1);
+  sl_k_compact_scatter(L);
+  sl_k_count(launch_of(s, 1), 1, 0);
+  s->cur ^= 1;
+  return check_launch(s, "compact");
+}
+
+static int encode_stats_copy(sl_sim *s) {
+  if (s->stats_pending) return SL_OK;  // one readback in flight, like the staging ring being busy
+  HIPCHK(s, hipMemcpyAsync(s->h_counters, s->ht.C, sizeof(SlCounters), hipMemcpyDeviceToHost, s->stream));
+  HIPCHK(s, hipEventRecord(s->stats_ev, s->stream));
+  s->stats_pending = true;
+  s->emitted_at_copy = s->emitted_total;
+  return SL_OK;
+}
+
+extern "C" int sl_step_end(sl_sim *s, double frame_dt) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  if (!(frame_dt > 0)) return SL_OK;
+  int r = encode_compact(s);
+  if (r) return r;
+  {
+    StageScope sc(s, SL_T_WW, 2);
+    sl_k_ww(launch_of(s, s->ww_cap), (float)frame_dt);
+    sl_k_count(launch_of(s, 1), 3, 0);
+    s->ww_cur ^= 1;
+  }
+  double liters = s->count * s->vp * 1000;
+  double md = liters / 1000 / (s->nx * s->nz * s->h * s->h);
+  md = md < 0.003 ? 0.003 : (md > s->rim ? s->rim : md);
+  r = encode_ripple(s, frame_dt, md);
+  if (r) return r;
+  s->frames++;
+  return encode_stats_copy(s);
+}
+
+extern "C" int sl_step(sl_sim *s, double frame_dt, sl_emit_fn fn, void *user) {
+  uint32_t n = 0;
+  double dt = 0;
+  int r = sl_step_begin(s, frame_dt, &n, &dt);
+  if (r || n == 0) return r;
+  sl_emit list[SL_MAX_EMIT];
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t m = fn ? fn(user, s, dt, s->time, list, SL_MAX_EMIT) : 0;
+    if (m > SL_MAX_EMIT) m = SL_MAX_EMIT;
+    r = sl_substep(s, dt, list, m, nullptr);
+    if (r) return r;
+  }
+  return sl_step_end(s, frame_dt);
+}
+
+// ------------------------------------------------------------------------------------------------
+// stats, 50-sim.js:338-363
+// ------------------------------------------------------------------------------------------------
+static void apply_stats(sl_sim *s) {
+  const SlCounters &u = *s->h_counters;
+  s->count = u.count;
+  s->impacts = u.impacts;
+  s->ww_count = u.wwCount < s->ww_cap ? u.wwCount : s->ww_cap;
+  float vm;
+  memcpy(&vm, &u.vmax, 4);
+  if (isfinite(vm)) s->vmax = vm;
+  if (s->count < s->cap * 0.9) s->full = false;
+  uint64_t ub = (uint64_t)u.count + (s->emitted_total - s->emitted_at_copy);
+  s->p_ub = (uint32_t)(ub < s->cap ? ub : s->cap);
+}
+extern "C" int sl_poll_stats(sl_sim *s, int *updated) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  if (updated) *updated = 0;
+  if (!s->stats_pending) return SL_OK;
+  hipError_t e = hipEventQuery(s->stats_ev);
+  if (e == hipErrorNotReady) return SL_OK;
+  if (e != hipSuccess) { s->gpu_errors++; FAIL(s, SL_EHIP, "stats readback: %s", hipGetErrorString(e)); }
+  s->stats_pending = false;
+  apply_stats(s);
+  if (updated) *updated = 1;
+  return SL_OK;
+}
+extern "C" int sl_sync(sl_sim *s) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return SL_OK;
+}
+extern "C" int sl_stats(sl_sim *s, sl_stats_t *o) {
+  if (!s || !o) FAIL((sl_sim *)nullptr, SL_EARG, "sl_stats: null argument");
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  int r = sl_poll_stats(s, nullptr);
+  if (r) return r;
+  // a fresh readback of the counters of the last finished work
+  HIPCHK(s, hipMemcpy(s->h_counters, s->ht.C, sizeof(SlCounters), hipMemcpyDeviceToHost));
+  s->emitted_at_copy = s->emitted_total;
+  apply_stats(s);
+  memset(o, 0, sizeof *o);
+  o->count = s->count; o->ww_count = s->ww_count; o->impacts = s->impacts; o->frames = s->frames;
+  o->full = s->full; o->gpu_errors = s->gpu_errors; o->substeps_last = s->substeps_last;
+  o->vmax = s->vmax; o->time = s->time; o->liters = s->count * s->vp * 1000;
+  o->grid[0] = s->nx; o->grid[1] = s->ny; o->grid[2] = s->nz;
+  return SL_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// clear / save / restore, 50-sim.js:111-141, :365-373
+// ------------------------------------------------------------------------------------------------
+extern "C" int sl_clear_water(sl_sim *s) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  sl_k_count(launch_of(s, 1), 4, 0);
+  sl_k_fill_u32(s->stream, (uint32_t *)s->ht.pres, 0, (size_t)s->hcells * 2);
+  if (s->R) {
+    sl_k_fill_u32(s->stream, (uint32_t *)s->ht.rstate, 0, (size_t)s->M * s->M * 2);
+    sl_k_fill_u32(s->stream, (uint32_t *)s->ht.rsrc, 0, (size_t)(s->R + 1) * (s->R + 1));
+  }
+  s->count = 0; s->ww_count = 0; s->impacts = 0; s->full = false; s->time = 0; s->vmax = 1;
+  s->p_ub = 0;
+  return check_launch(s, "clear_water");
+}
+
+extern "C" int sl_save_water(sl_sim *s, uint32_t *id) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  auto &v = s->saved;
+  const size_t pbytes = (size_t)s->cap * 25, wbytes = (size_t)s->ww_cap * 32;
+  const size_t rbytes = (size_t)s->M * s->M * 8;
+  if (!v.parts) {
+    HIPCHK(s, hipMalloc(&v.parts, pbytes));
+    HIPCHK(s, hipMalloc(&v.ww, wbytes ? wbytes : 32));
+    HIPCHK(s, hipMalloc(&v.counters, 32));
+    if (rbytes) HIPCHK(s, hipMalloc(&v.ripple, rbytes));
+  }
+  const SlTank &T = s->ht;
+  const int c = s->cur, w = s->ww_cur;
+  uint8_t *p = (uint8_t *)v.parts;
+  const size_t cf = (size_t)s->cap * 4;
+  const void *src[6] = {T.px[c], T.py[c], T.pz[c], T.vx[c], T.vy[c], T.vz[c]};
+  for (int i = 0; i < 6; i++) HIPCHK(s, hipMemcpyAsync(p + i * cf, src[i], cf, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(s, hipMemcpyAsync(p + 6 * cf, T.pf[c], s->cap, hipMemcpyDeviceToDevice, s->stream));
+  if (wbytes) {
+    HIPCHK(s, hipMemcpyAsync(v.ww, T.wp[w], wbytes / 2, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync((uint8_t *)v.ww + wbytes / 2, T.wv[w], wbytes / 2, hipMemcpyDeviceToDevice, s->stream));
+  }
+  HIPCHK(s, hipMemcpyAsync(v.counters, T.C, 32, hipMemcpyDeviceToDevice, s->stream));
+  if (rbytes) HIPCHK(s, hipMemcpyAsync(v.ripple, T.rstate, rbytes, hipMemcpyDeviceToDevice, s->stream));
+  v.time = s->time; v.count = s->count; v.ww_count = s->ww_count; v.impacts = s->impacts; v.full = s->full;
+  v.cur = c; v.ww_cur = w;
+  v.valid = true;
+  v.id = ++g_copies;
+  if (id) *id = v.id;
+  return SL_OK;
+}
+extern "C" int sl_has_water_copy(const sl_sim *s, uint32_t id) { return s && s->saved.valid && s->saved.id == id; }
+extern "C" int sl_restore_water(sl_sim *s, uint32_t id) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  if (!sl_has_water_copy(s, id)) return 0;
+  auto &v = s->saved;
+  const SlTank &T = s->ht;
+  const int c = s->cur, w = s->ww_cur;
+  const size_t cf = (size_t)s->cap * 4, wbytes = (size_t)s->ww_cap * 32, rbytes = (size_t)s->M * s->M * 8;
+  uint8_t *p = (uint8_t *)v.parts;
+  void *dst[6] = {T.px[c], T.py[c], T.pz[c], T.vx[c], T.vy[c], T.vz[c]};
+  for (int i = 0; i < 6; i++) HIPCHK(s, hipMemcpyAsync(dst[i], p + i * cf, cf, hipMemcpyDeviceToDevice, s->stream));
+  HIPCHK(s, hipMemcpyAsync(T.pf[c], p + 6 * cf, s->cap, hipMemcpyDeviceToDevice, s->stream));
+  if (wbytes) {
+    HIPCHK(s, hipMemcpyAsync(T.wp[w], v.ww, wbytes / 2, hipMemcpyDeviceToDevice, s->stream));
+    HIPCHK(s, hipMemcpyAsync(T.wv[w], (uint8_t *)v.ww + wbytes / 2, wbytes / 2, hipMemcpyDeviceToDevice, s->stream));
+  }
+  HIPCHK(s, hipMemcpyAsync(T.C, v.counters, 32, hipMemcpyDeviceToDevice, s->stream));
+  if (rbytes) HIPCHK(s, hipMemcpyAsync(T.rstate, v.ripple, rbytes, hipMemcpyDeviceToDevice, s->stream));
+  s->time = v.time; s->count = v.count; s->ww_count = v.ww_count; s->impacts = v.impacts; s->full = v.full;
+  s->p_ub = s->cap;  // the device count is restored; bound it by the capacity until the next readback
+  v.valid = false;
+  return 1;
+}
+
+extern "C" int sl_set_ripple_mask(sl_sim *s, int bound) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  s->mask_bound = (bound && s->R) ? 1 : 0;
+  return SL_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// state transfer in the reference layouts
+// ------------------------------------------------------------------------------------------------
+extern "C" size_t sl_field_bytes(const sl_sim *s, int f) {
+  if (!s) return 0;
+  const size_t N = s->nodes, C = s->cells, M2 = (size_t)s->M * s->M, R1 = (size_t)(s->R + 1) * (s->R + 1);
+  switch (f) {
+    case SL_F_PARTICLES: return (size_t)s->cap * 32;
+    case SL_F_WW: return (size_t)s->ww_cap * 32;
+    case SL_F_COUNTERS: return 32;
+    case SL_F_ACC: return N * 24;
+    case SL_F_CELLCOUNT: case SL_F_CELLTYPE: case SL_F_DIVG: case SL_F_PRES: return C * 4;
+    case SL_F_VEL: case SL_F_VELOLD: case SL_F_FRAC: case SL_F_SVEL: return N * 16;
+    case SL_F_OBJS: return SL_MAX_OBJ * 112;
+    case SL_F_RSRC: case SL_F_RMASK: return s->R ? R1 * 4 : 0;
+    case SL_F_RSTATE: case SL_F_RHAT: case SL_F_RFIELD: return s->R ? M2 * 8 : 0;
+    case SL_F_RTEX: return s->R ? R1 * 16 : 0;
+    case SL_F_RTEX16: return s->R ? R1 * 8 : 0;
+    case SL_F_TOPH: return s->R ? (size_t)s->ht.wmap * s->ht.wmap * 4 : 0;
+  }
+  return 0;
+}
+
+static int d2h(sl_sim *s, void *dst, const void *src, size_t bytes) {
+  HIPCHK(s, hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost));
+  return SL_OK;
+}
+static int h2d(sl_sim *s, void *dst, const void *src, size_t bytes) {
+  HIPCHK(s, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+  return SL_OK;
+}
+static inline uint32_t packed_slot(const sl_sim *s, uint32_t c) {
+  uint32_t i = c % s->nx, j = (c / s->nx) % s->ny, k = c / (s->nx * s->ny);
+  return (((i + j + k) & 1) ? s->hcells : 0) + (i >> 1) + s->hw * (j + s->ny * k);
+}
+static uint16_t f32_to_f16(float f) {
+  uint32_t x;
+  memcpy(&x, &f, 4);
+  uint32_t sign = (x >> 16) & 0x8000u, e = (x >> 23) & 0xffu, m = x & 0x7fffffu;
+  if (e == 0xff) return (uint16_t)(sign | 0x7c00u | (m ? 0x200u : 0));
+  int32_t ee = (int32_t)e - 127 + 15;
+  if (ee >= 31) return (uint16_t)(sign | 0x7c00u);
+  if (ee <= 0) {
+    if (ee < -10) return (uint16_t)sign;
+    m |= 0x800000u;
+    uint32_t shift = (uint32_t)(14 - ee);
+    uint32_t hm = m >> shift, rem = m & ((1u << shift) - 1), half = 1u << (shift - 1);
+    if (rem > half || (rem == half && (hm & 1u))) hm++;
+    return (uint16_t)(sign | hm);
+  }
+  uint32_t hm = m >> 13, rem = m & 0x1fffu;
+  uint32_t out = sign | ((uint32_t)ee << 10) | hm;
+  if (rem > 0x1000u || (rem == 0x1000u && (hm & 1u))) out++;
+  return (uint16_t)out;
+}
+
+extern "C" int sl_readback(sl_sim *s, int f, void *dst, size_t bytes) {
+  if (!s || !dst) FAIL((sl_sim *)nullptr, SL_EARG, "sl_readback: null argument");
+  size_t need = sl_field_bytes(s, f);
+  if (!need || bytes < need) FAIL(s, SL_EARG, "sl_readback: field %d needs %zu bytes, got %zu", f, need, bytes);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  const SlTank &T = s->ht;
+  const size_t N = s->nodes, C = s->cells;
+  int r = SL_OK;
+  float *o = (float *)dst;
+  switch (f) {
+    case SL_F_PARTICLES: {
+      const size_t P = s->cap;
+      std::vector<float> a(P * 6);
+      std::vector<uint8_t> fl(P);
+      const int c = s->cur;
+      const float *srcs[6] = {T.px[c], T.py[c], T.pz[c], T.vx[c], T.vy[c], T.vz[c]};
+      for (int q = 0; q < 6; q++) if ((r = d2h(s, a.data() + q * P, srcs[q], P * 4))) return r;
+      if ((r = d2h(s, fl.data(), T.pf[c], P))) return r;
+      for (size_t i = 0; i < P; i++) {
+        float *p = o + i * 8;
+        p[0] = a[i]; p[1] = a[P + i]; p[2] = a[2 * P + i]; p[3] = (fl[i] & 1) ? 1.0f : 0.0f;
+        p[4] = a[3 * P + i]; p[5] = a[4 * P + i]; p[6] = a[5 * P + i]; p[7] = (fl[i] & 2) ? 1.0f : 0.0f;
+      }
+      return SL_OK;
+    }
+    case SL_F_WW: {
+      const size_t W = s->ww_cap;
+      std::vector<float> a(W * 4), b(W * 4);
+      if ((r = d2h(s, a.data(), T.wp[s->ww_cur], W * 16))) return r;
+      if ((r = d2h(s, b.data(), T.wv[s->ww_cur], W * 16))) return r;
+      for (size_t i = 0; i < W; i++) { memcpy(o + i * 8, &a[i * 4], 16); memcpy(o + i * 8 + 4, &b[i * 4], 16); }
+      return SL_OK;
+    }
+    case SL_F_COUNTERS: return d2h(s, dst, T.C, 32);
+    case SL_F_ACC: {
+      std::vector<int32_t> a(N * 6);
+      if ((r = d2h(s, a.data(), T.acc, N * 24))) return r;
+      int32_t *oi = (int32_t *)dst;
+      for (size_t n = 0; n < N; n++) for (int q = 0; q < 6; q++) oi[n * 6 + q] = a[q * N + n];
+      return SL_OK;
+    }
+    case SL_F_CELLCOUNT: return d2h(s, dst, T.cellCount, C * 4);
+    case SL_F_CELLTYPE: {
+      std::vector<uint32_t> a(C);
+      if ((r = d2h(s, a.data(), T.cellCount, C * 4))) return r;
+      uint32_t *ou = (uint32_t *)dst;
+      for (size_t c = 0; c < C; c++) ou[c] = a[c] > 0 ? 1u : 0u;
+      return SL_OK;
+    }
+    case SL_F_VEL: case SL_F_VELOLD: case SL_F_FRAC: case SL_F_SVEL: {
+      const float *src = f == SL_F_VEL ? T.vel : f == SL_F_VELOLD ? T.velOld : f == SL_F_FRAC ? T.frac : T.svel;
+      std::vector<float> a(N * 3);
+      std::vector<uint8_t> m(N, 0);
+      if ((r = d2h(s, a.data(), src, N * 12))) return r;
+      if (f == SL_F_VELOLD && (r = d2h(s, m.data(), T.vmask, N))) return r;
+      for (size_t n = 0; n < N; n++) {
+        o[n * 4] = a[n]; o[n * 4 + 1] = a[N + n]; o[n * 4 + 2] = a[2 * N + n]; o[n * 4 + 3] = (float)m[n];
+      }
+      return SL_OK;
+    }
+    case SL_F_DIVG: return d2h(s, dst, T.divg, C * 4);
+    case SL_F_PRES: {
+      std::vector<float> a((size_t)s->hcells * 2);
+      if ((r = d2h(s, a.data(), T.pres, a.size() * 4))) return r;
+      for (uint32_t c = 0; c < C; c++) o[c] = a[packed_slot(s, c)];
+      return SL_OK;
+    }
+    case SL_F_OBJS: memcpy(dst, s->objs, sizeof s->objs); return SL_OK;
+    case SL_F_RSRC: return d2h(s, dst, T.rsrc, need);
+    case SL_F_RSTATE: return d2h(s, dst, T.rstate, need);
+    case SL_F_RHAT: return d2h(s, dst, T.rhat, need);
+    case SL_F_RFIELD: return d2h(s, dst, T.rb, need);
+    case SL_F_RMASK: return d2h(s, dst, T.rmask, need);
+    case SL_F_RTEX: return d2h(s, dst, T.rtex, need);
+    case SL_F_TOPH: return d2h(s, dst, T.topH, need);
+    case SL_F_RTEX16: {
+      size_t R1 = (size_t)(s->R + 1) * (s->R + 1);
+      std::vector<float> a(R1 * 4);
+      if ((r = d2h(s, a.data(), T.rtex, R1 * 16))) return r;
+      uint16_t *oh = (uint16_t *)dst;
+      for (size_t i = 0; i < R1 * 4; i++) oh[i] = f32_to_f16(a[i]);
+      return SL_OK;
+    }
+  }
+  FAIL(s, SL_EARG, "sl_readback: unknown field %d", f);
+}
+
+extern "C" int sl_upload(sl_sim *s, int f, const void *src, size_t bytes) {
+  if (!s || !src) FAIL((sl_sim *)nullptr, SL_EARG, "sl_upload: null argument");
+  size_t need = sl_field_bytes(s, f);
+  if (!need || bytes < need) FAIL(s, SL_EARG, "sl_upload: field %d needs %zu bytes, got %zu", f, need, bytes);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  const SlTank &T = s->ht;
+  const size_t N = s->nodes, C = s->cells;
+  const float *in = (const float *)src;
+  int r = SL_OK;
+  switch (f) {
+    case SL_F_PARTICLES: {
+      const size_t P = s->cap;
+      std::vector<float> a(P * 6);
+      std::vector<uint8_t> fl(P);
+      for (size_t i = 0; i < P; i++) {
+        const float *p = in + i * 8;
+        a[i] = p[0]; a[P + i] = p[1]; a[2 * P + i] = p[2];
+        a[3 * P + i] = p[4]; a[4 * P + i] = p[5]; a[5 * P + i] = p[6];
+        fl[i] = (uint8_t)((p[3] < 0.5f ? 0u : 1u) | (p[7] > 0.5f ? 2u : 0u));
+      }
+      const int c = s->cur;
+      float *dsts[6] = {T.px[c], T.py[c], T.pz[c], T.vx[c], T.vy[c], T.vz[c]};
+      for (int q = 0; q < 6; q++) if ((r = h2d(s, dsts[q], a.data() + q * P, P * 4))) return r;
+      return h2d(s, T.pf[c], fl.data(), P);
+    }
+    case SL_F_WW: {
+      const size_t W = s->ww_cap;
+      std::vector<float> a(W * 4), b(W * 4);
+      for (size_t i = 0; i < W; i++) { memcpy(&a[i * 4], in + i * 8, 16); memcpy(&b[i * 4], in + i * 8 + 4, 16); }
+      if ((r = h2d(s, T.wp[s->ww_cur], a.data(), W * 16))) return r;
+      return h2d(s, T.wv[s->ww_cur], b.data(), W * 16);
+    }
+    case SL_F_COUNTERS: {
+      r = h2d(s, T.C, src, 32);
+      if (r) return r;
+      const SlCounters *cc = (const SlCounters *)src;
+      s->p_ub = cc->count < s->cap ? cc->count : s->cap;
+      return SL_OK;
+    }
+    case SL_F_ACC: {
+      std::vector<int32_t> a(N * 6);
+      const int32_t *ii = (const int32_t *)src;
+      for (size_t n = 0; n < N; n++) for (int q = 0; q < 6; q++) a[q * N + n] = ii[n * 6 + q];
+      return h2d(s, T.acc, a.data(), N * 24);
+    }
+    case SL_F_CELLCOUNT: return h2d(s, T.cellCount, src, C * 4);
+    case SL_F_VEL: case SL_F_VELOLD: case SL_F_FRAC: case SL_F_SVEL: {
+      float *d = f == SL_F_VEL ? T.vel : f == SL_F_VELOLD ? T.velOld : f == SL_F_FRAC ? T.frac : T.svel;
+      std::vector<float> a(N * 3);
+      std::vector<uint8_t> m(N);
+      for (size_t n = 0; n < N; n++) {
+        a[n] = in[n * 4]; a[N + n] = in[n * 4 + 1]; a[2 * N + n] = in[n * 4 + 2];
+        float w = in[n * 4 + 3];
+        m[n] = (uint8_t)(w > 0 ? (uint32_t)w & 7u : 0u);
+      }
+      if ((r = h2d(s, d, a.data(), N * 12))) return r;
+      if (f == SL_F_VELOLD) return h2d(s, T.vmask, m.data(), N);
+      if (f == SL_F_FRAC || f == SL_F_SVEL) s->geom_dirty = false;  // explicit geometry replaces GEOM
+      return SL_OK;
+    }
+    case SL_F_DIVG: return h2d(s, T.divg, src, C * 4);
+    case SL_F_PRES: {
+      std::vector<float> a((size_t)s->hcells * 2, 0.0f);
+      for (uint32_t c = 0; c < C; c++) a[packed_slot(s, c)] = in[c];
+      return h2d(s, T.pres, a.data(), a.size() * 4);
+    }
+    case SL_F_RSRC: return h2d(s, T.rsrc, src, need);
+    case SL_F_RSTATE: return h2d(s, T.rstate, src, need);
+    case SL_F_TOPH: return h2d(s, T.topH, src, need);
+  }
+  FAIL(s, SL_EARG, "sl_upload: field %d is read-only", f);
+}
+
+extern "C" int sl_set_scalar(sl_sim *s, const char *name, double v) {
+  if (!s || !name) FAIL((sl_sim *)nullptr, SL_EARG, "sl_set_scalar: null argument");
+  if (!strcmp(name, "count")) s->count = (uint32_t)v;
+  else if (!strcmp(name, "time")) s->time = v;
+  else if (!strcmp(name, "frame")) s->frame = (uint32_t)v;
+  else if (!strcmp(name, "vmax")) s->vmax = v;
+  else if (!strcmp(name, "full")) s->full = v != 0;
+  else if (!strcmp(name, "p_ub")) s->p_ub = (uint32_t)v;
+  else if (!strcmp(name, "geom_dirty")) s->geom_dirty = v != 0;
+  else FAIL(s, SL_EARG, "unknown scalar '%s'", name);
+  return SL_OK;
+}
+extern "C" int sl_get_scalar(const sl_sim *s, const char *name, double *v) {
+  if (!s || !name || !v) FAIL((sl_sim *)nullptr, SL_EARG, "sl_get_scalar: null argument");
+  if (!strcmp(name, "count")) *v = s->count;
+  else if (!strcmp(name, "time")) *v = s->time;
+  else if (!strcmp(name, "frame")) *v = s->frame;
+  else if (!strcmp(name, "vmax")) *v = s->vmax;
+  else if (!strcmp(name, "full")) *v = s->full;
+  else if (!strcmp(name, "p_ub")) *v = s->p_ub;
+  else if (!strcmp(name, "cur")) *v = s->cur;
+  else if (!strcmp(name, "ww_cur")) *v = s->ww_cur;
+  else if (!strcmp(name, "impacts")) *v = s->impacts;
+  else if (!strcmp(name, "ww_count")) *v = s->ww_count;
+  else if (!strcmp(name, "frames")) *v = s->frames;
+  else { ((sl_sim *)s)->err = std::string("unknown scalar '") + name + "'"; return SL_EARG; }
+  return SL_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// single stages (stage-level parity)
+// ------------------------------------------------------------------------------------------------
+extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  write_params(s, dt);
+  int r = push_dyn(s);
+  if (r) return r;
+  switch (stage) {
+    case SL_S_GEOM: sl_k_geom(launch_of(s, s->nodes)); s->geom_dirty = false; break;
+    case SL_S_P2G:
+      sl_k_fill_u32(s->stream, (uint32_t *)s->ht.acc, 0, (size_t)s->nodes * 6);
+      sl_k_fill_u32(s->stream, s->ht.cellCount, 0, s->cells);
+      sl_k_p2g(launch_of(s, s->p_ub), 0);
+      break;
+    case SL_S_NORMFORCE: sl_k_normforce(launch_of(s, s->nodes)); break;
+    case SL_S_DIVERGENCE: sl_k_divsetup(launch_of(s, s->cells)); break;
+    case SL_S_PRESSURE: sl_k_sor(launch_of(s, s->hcells), arg & 1u, 0); break;
+    case SL_S_SOR: {
+      SlLaunch L = launch_of(s, s->hcells);
+      for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor(L, 0, 2 * i); sl_k_sor(L, 1, 2 * i + 1); }
+      break;
+    }
+    case SL_S_PROJECT: sl_k_project(launch_of(s, s->nodes)); break;
+    case SL_S_G2P: sl_k_g2p(launch_of(s, s->p_ub)); break;
+    case SL_S_COMPACT: return encode_compact(s);
+    case SL_S_WW:
+      sl_k_ww(launch_of(s, s->ww_cap), (float)dt);
+      sl_k_count(launch_of(s, 1), 3, 0);
+      s->ww_cur ^= 1;
+      break;
+    case SL_S_RIPPLE: {
+      double liters = s->count * s->vp * 1000;
+      double md = liters / 1000 / (s->nx * s->nz * s->h * s->h);
+      md = md < 0.003 ? 0.003 : (md > s->rim ? s->rim : md);
+      return encode_ripple(s, dt, md);
+    }
+    default: FAIL(s, SL_EARG, "unknown stage %d", stage);
+  }
+  return check_launch(s, "stage");
+}
+
+// ------------------------------------------------------------------------------------------------
+// timing
+// ------------------------------------------------------------------------------------------------
+extern "C" int sl_timing_enable(sl_sim *s, int enable) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  timing_collect(s);
+  s->timing = enable != 0;
+  return SL_OK;
+}
+extern "C" int sl_timing_read(sl_sim *s, double *ms, uint32_t *launches, int n, int reset) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  timing_collect(s);
+  for (int i = 0; i < n && i < SL_T_COUNT_; i++) {
+    if (ms) ms[i] = s->t_ms[i];
+    if (launches) launches[i] = s->t_launch[i];
+  }
+  if (reset) { memset(s->t_ms, 0, sizeof s->t_ms); memset(s->t_launch, 0, sizeof s->t_launch); }
+  return SL_OK;
+}
+
+extern "C" int sl_version(void) { return SL_ABI_VERSION; }
+extern "C" const char *sl_build_info(void) {
+  return "suiri gfx950 HIP (" __DATE__ ") fp32, -ffp-contract=off";
+}

@@ -1,0 +1,345 @@
+// Grid-stage kernels of the liquid step: face fractions, particle-to-grid transfer, velocity
+// normalisation + body forces, divergence + SOR set-up, red-black SOR, pressure projection.
+// Each kernel follows one (or two fused) WGSL kernels of the reference; the arithmetic of every
+// output value is evaluated in the same order as the WGSL expression it restates.
+#include "sl_internal.h"
+
+// Resolve (tank, local workgroup) of workgroup b of a launch.
+#define SL_TANK_OF(W, b, t, lb)                               \
+  int t = 0;                                                  \
+  uint32_t lb = (b);                                          \
+  if ((W).ntanks > 1) { t = tank_of((W).start, (W).ntanks, b); lb = (b) - (W).start[t]; }
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) p[i] = v;
+}
+void sl_k_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n) {
+  if (!n) return;
+  size_t blocks = (n + SL_WG - 1) / SL_WG;
+  if (blocks > 8192) blocks = 8192;
+  hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(SL_WG), 0, s, p, v, n);
+}
+
+// ------------------------------------------------------------------------------------------------
+// GEOM: open fraction and solid velocity of every staggered face, 30-wgsl-sim.js:118-157
+// ------------------------------------------------------------------------------------------------
+__device__ float face_open(const SlTank &T, f3 c, int axis) {
+  float a = 0.5f * T.h;
+  f3 o1 = F3(0, 0, 0), o2 = F3(0, 0, 0);
+  if (axis == 0) { o1 = F3(0.0f, a, 0.0f); o2 = F3(0.0f, 0.0f, a); }
+  else if (axis == 1) { o1 = F3(a, 0.0f, 0.0f); o2 = F3(0.0f, 0.0f, a); }
+  else { o1 = F3(a, 0.0f, 0.0f); o2 = F3(0.0f, a, 0.0f); }
+  float s0 = scene_sdf(T, sub3(sub3(c, o1), o2));
+  float s1 = scene_sdf(T, sub3(add3(c, o1), o2));
+  float s2 = scene_sdf(T, add3(add3(c, o1), o2));
+  float s3 = scene_sdf(T, add3(sub3(c, o1), o2));
+  float sc = scene_sdf(T, c);
+  float pos = fmaxf(s0, 0.0f) + fmaxf(s1, 0.0f) + fmaxf(s2, 0.0f) + fmaxf(s3, 0.0f) + 2.0f * fmaxf(sc, 0.0f);
+  float tot = fabsf(s0) + fabsf(s1) + fabsf(s2) + fabsf(s3) + 2.0f * fabsf(sc);
+  if (tot < 1e-12f) return 1.0f;
+  return clampf(pos / tot, 0.0f, 1.0f);
+}
+
+__global__ __launch_bounds__(SL_WG) void k_geom(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  uint32_t n = lb * SL_WG + threadIdx.x;
+  if (n >= T.nodes) return;
+  uint32_t sx = T.nx + 1, sy = T.ny + 1;
+  int32_t cx = (int32_t)(n % sx), cy = (int32_t)((n / sx) % sy), cz = (int32_t)(n / (sx * sy));
+  const float h = T.h;
+  const f3 o = F3(T.ox, T.oy, T.oz);
+  float fr[3] = {0.0f, 0.0f, 0.0f}, sv[3] = {0.0f, 0.0f, 0.0f};
+  if (cy < T.ny && cz < T.nz && cx > 0 && cx < T.nx) {
+    f3 p = add3(o, mul3(F3((float)cx, (float)cy + 0.5f, (float)cz + 0.5f), h));
+    fr[0] = face_open(T, p, 0); sv[0] = solid_vel_at(T, p).x;
+  }
+  if (cx < T.nx && cz < T.nz && cy > 0) {
+    if (cy >= T.ny) fr[1] = 1.0f;
+    else {
+      f3 p = add3(o, mul3(F3((float)cx + 0.5f, (float)cy, (float)cz + 0.5f), h));
+      fr[1] = face_open(T, p, 1); sv[1] = solid_vel_at(T, p).y;
+    }
+  }
+  if (cx < T.nx && cy < T.ny && cz > 0 && cz < T.nz) {
+    f3 p = add3(o, mul3(F3((float)cx + 0.5f, (float)cy + 0.5f, (float)cz), h));
+    fr[2] = face_open(T, p, 2); sv[2] = solid_vel_at(T, p).z;
+  }
+  const size_t N = T.nodes;
+  for (int c = 0; c < 3; c++) { T.frac[c * N + n] = fr[c]; T.svel[c * N + n] = sv[c]; }
+}
+void sl_k_geom(const SlLaunch &L) {
+  if (!L.w.total) return;
+  hipLaunchKernelGGL(k_geom, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+}
+
+// ------------------------------------------------------------------------------------------------
+// P2G: trilinear splat of each velocity component to its 8 face nodes in 2^18 fixed point,
+// 30-wgsl-sim.js:159-200. Integer atomics make the sums independent of the particle order.
+// ------------------------------------------------------------------------------------------------
+#define VSCALE 262144.0f
+__device__ __forceinline__ void splat(const SlTank &T, f3 q, float val, int comp) {
+  int32_t bx = f2i(floorf(q.x)), by = f2i(floorf(q.y)), bz = f2i(floorf(q.z));
+  float fx = q.x - (float)bx, fy = q.y - (float)by, fz = q.z - (float)bz;
+  const size_t N = T.nodes;
+  int32_t *mom = T.acc + (size_t)(2 * comp) * N;
+  int32_t *wt = T.acc + (size_t)(2 * comp + 1) * N;
+#pragma unroll
+  for (int dz = 0; dz < 2; dz++) {
+    int32_t k = bz + dz;
+    if (k < 0 || k > T.nz) continue;
+    float wz = dz == 1 ? fz : 1.0f - fz;
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++) {
+      int32_t j = by + dy;
+      if (j < 0 || j > T.ny) continue;
+      float wy = dy == 1 ? fy : 1.0f - fy;
+#pragma unroll
+      for (int dx = 0; dx < 2; dx++) {
+        int32_t i = bx + dx;
+        if (i < 0 || i > T.nx) continue;
+        float w = (dx == 1 ? fx : 1.0f - fx) * wy * wz;
+        if (w <= 0.0f) continue;
+        uint32_t n = node_idx(T, i, j, k);
+        atomicAdd(&mom[n], f2i(rintf(w * val * VSCALE)));
+        atomicAdd(&wt[n], f2i(rintf(w * VSCALE)));
+      }
+    }
+  }
+}
+template <int COUNT_ONLY>
+__global__ __launch_bounds__(SL_WG) void k_p2g(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  uint32_t id = lb * SL_WG + threadIdx.x;
+  if (id >= T.C->count) return;
+  const int cur = T.cur;
+  if (!(T.pf[cur][id] & 1u)) return;
+  f3 x = F3(T.px[cur][id], T.py[cur][id], T.pz[cur][id]);
+  f3 g = grid_pos(T, x);
+  if (!in_grid(T, g)) return;
+  i3 c = toi3(g);
+  atomicAdd(&T.cellCount[cell_idx(T, c.x, c.y, c.z)], 1u);
+  if (COUNT_ONLY) return;
+  float vx = clampf(T.vx[cur][id], -30.0f, 30.0f);
+  float vy = clampf(T.vy[cur][id], -30.0f, 30.0f);
+  float vz = clampf(T.vz[cur][id], -30.0f, 30.0f);
+  splat(T, sub3(g, F3(0.0f, 0.5f, 0.5f)), vx, 0);
+  splat(T, sub3(g, F3(0.5f, 0.0f, 0.5f)), vy, 1);
+  splat(T, sub3(g, F3(0.5f, 0.5f, 0.0f)), vz, 2);
+}
+void sl_k_p2g(const SlLaunch &L, int count_only) {
+  if (!L.w.total) return;
+  if (count_only) hipLaunchKernelGGL(k_p2g<1>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  else hipLaunchKernelGGL(k_p2g<0>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+}
+
+// ------------------------------------------------------------------------------------------------
+// NORMALIZE (30-wgsl-sim.js:203-222) fused with FORCES (30-wgsl-sim.js:224-238), per node.
+// The accumulators are cleared after they are read, ready for the next substep's P2G.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SL_WG) void k_normforce(SlWork W) {
+  SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
+  const SlTank &T = W.tanks[t];
+  uint32_t n = lb * SL_WG + threadIdx.x;
+  if (n >= T.nodes) return;
+  const size_t N = T.nodes;
+  float v[3] = {0.0f, 0.0f, 0.0f};
+  uint32_t m = 0;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    int32_t mo = T.acc[(2 * c) * N + n], wi = T.acc[(2 * c + 1) * N + n];
+    float wt = (float)wi;
+    if (wt > 1.0f) { v[c] = (float)mo / wt; m |= 1u << c; }
+    T.velOld[c * N + n] = v[c];
+    T.acc[(2 * c) * N + n] = 0;
+    T.acc[(2 * c + 1) * N + n] = 0;
+  }
+  T.vmask[n] = (uint8_t)m;
+  v[1] -= T.gdt;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    if (T.frac[c * N + n] <= 0.0f) v[c] = T.svel[c * N + n];
+    T.vel[c * N + n] = v[c];
+  }
+}
+void sl_k_normforce(const SlLaunch &L) {
+  if (!L.w.total) return;
+  hipLaunchKernelGGL(k_normforce, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+}
+
+// ------------------------------------------------------------------------------------------------
+// DIVERGENCE (30-wgsl-sim.js:240-264) fused with the per-substep SOR set-up: the right-hand side
+// h*h*div, the 6-face stencil flags of every cell (open faces, contributing fluid neighbours) and
+// p = 0 for non-fluid cells (what every colour pass of 30-wgsl-sim.js:283 writes), in the
+// red/black split layout.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ float flux(float f, float v, float s) { return f * v + (1.0f - f) * s; }
+
+__global__ __launch_bounds__(SL_WG) void k_divsetup(SlWork W) {
+  SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
+  const SlTank &T = W.tanks[t];
+  uint32_t c = lb * SL_WG + threadIdx.x;
+  if (c >= T.cells) return;
+  const int32_t nx = T.nx, ny = T.ny, nz = T.nz;
+  int32_t i = (int32_t)(c % (uint32_t)nx), j = (int32_t)((c / (uint32_t)nx) % (uint32_t)ny);
+  int32_t k = (int32_t)(c / (uint32_t)(nx * ny));
+  const uint32_t slot = colour_base(T, i, j, k) + packed_idx(T, i, j, k);
+  const uint32_t cnt = T.cellCount[c];
+  if (cnt == 0u) {
+    T.divg[c] = 0.0f;
+    T.rhs[slot] = 0.0f;
+    T.sflag[slot] = 0;
+    if (T.iters > 0) T.pres[slot] = 0.0f;
+    return;
+  }
+  const size_t N = T.nodes;
+  const float *fx = T.frac, *fy = T.frac + N, *fz = T.frac + 2 * N;
+  const float *vx = T.vel, *vy = T.vel + N, *vz = T.vel + 2 * N;
+  const float *sx = T.svel, *sy = T.svel + N, *sz = T.svel + 2 * N;
+  uint32_t n0 = node_idx(T, i, j, k), nxp = n0 + 1, nyp = n0 + (uint32_t)(nx + 1);
+  uint32_t nzp = n0 + (uint32_t)((nx + 1) * (ny + 1));
+  float f0x = fx[n0], f0y = fy[n0], f0z = fz[n0], f1x = fx[nxp], f1y = fy[nyp], f1z = fz[nzp];
+  float d = flux(f1x, vx[nxp], sx[nxp]) - flux(f0x, vx[n0], sx[n0]);
+  d += flux(f1y, vy[nyp], sy[nyp]) - flux(f0y, vy[n0], sy[n0]);
+  d += flux(f1z, vz[nzp], sz[nzp]) - flux(f0z, vz[n0], sz[n0]);
+  d *= T.invh;
+  float crowd = (float)cnt / T.ppc - 1.0f;
+  float dv = d - T.drift * fmaxf(crowd, 0.0f);
+  T.divg[c] = dv;
+  T.rhs[slot] = T.hh * dv;
+  // stencil flags in the face order of 30-wgsl-sim.js:286-291
+  const float f[6] = {f0x, f1x, f0y, f1y, f0z, f1z};
+  const bool ok[6] = {i > 0, i < nx - 1, j > 0, j < ny - 1, k > 0, k < nz - 1};
+  const int32_t off[6] = {-1, 1, -nx, nx, -nx * ny, nx * ny};
+  uint32_t fl = SF_FLUID;
+#pragma unroll
+  for (int q = 0; q < 6; q++) {
+    if (f[q] <= 0.0f) continue;
+    fl |= 1u << (SF_OPEN0 + q);
+    if (f[q] != 1.0f) fl |= SF_SLOW;
+    if (ok[q] && T.cellCount[(int64_t)c + off[q]] > 0u) fl |= 1u << (SF_NUM0 + q);
+  }
+  T.sflag[slot] = (uint16_t)fl;
+}
+void sl_k_divsetup(const SlLaunch &L) {
+  if (!L.w.total) return;
+  hipLaunchKernelGGL(k_divsetup, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+}
+
+// ------------------------------------------------------------------------------------------------
+// One colour pass of red-black SOR, 30-wgsl-sim.js:267-297, over the slots of colour `parity`.
+// Neighbours of slot m in row (j,k) are slots m-1+r, m+r (x) of the same row and slot m of rows
+// j+-1 and planes k+-1 in the other colour's array, with r = (parity+j+k)&1 (i = 2m + r).
+// Fast path (all open faces have f = 1): num = sum of contributing neighbour pressures in face
+// order, den = number of open faces -- bit-identical to the reference's f*p and f sums.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SL_WG) void k_sor(SlWork W, uint32_t parity, uint32_t pass) {
+  SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
+  const SlTank &T = W.tanks[t];
+  if (pass >= 2u * T.iters) return;
+  uint32_t s = lb * SL_WG + threadIdx.x;
+  if (s >= T.hcells) return;
+  const uint32_t hw = (uint32_t)T.hw, rowl = hw * (uint32_t)T.ny;
+  uint32_t k = s / rowl, rem = s - k * rowl, j = rem / hw, m = rem - j * hw;
+  uint32_t r = (parity + j + k) & 1u;
+  int32_t i = (int32_t)(2u * m + r);
+  if (i >= T.nx) return;
+  const uint32_t me = parity * T.hcells + s;
+  const uint32_t fl = T.sflag[me];
+  if (!(fl & SF_FLUID)) return;
+  const float *po = T.pres + (1u - parity) * T.hcells;
+  const uint32_t H = T.hcells;
+  // neighbour slots (clamped into the array; unused ones are masked by the flags)
+  uint32_t sxm = s + r - 1u, sxp = s + r;
+  uint32_t sym = s - hw, syp = s + hw, szm = s - rowl, szp = s + rowl;
+  sxm = sxm < H ? sxm : s; sxp = sxp < H ? sxp : s;
+  sym = sym < H ? sym : s; syp = syp < H ? syp : s;
+  szm = szm < H ? szm : s; szp = szp < H ? szp : s;
+  const float p_n[6] = {po[sxm], po[sxp], po[sym], po[syp], po[szm], po[szp]};
+  float num = 0.0f, den = 0.0f;
+  if (!(fl & SF_SLOW)) {
+#pragma unroll
+    for (int q = 0; q < 6; q++)
+      if (fl & (1u << (SF_NUM0 + q))) num += p_n[q];
+    den = (float)__builtin_popcount(fl & 0x3fu);
+  } else {
+    const size_t N = T.nodes;
+    const int32_t jj = (int32_t)j, kk = (int32_t)k;
+    uint32_t n0 = node_idx(T, i, jj, kk);
+    const float f[6] = {T.frac[n0], T.frac[n0 + 1], T.frac[N + n0], T.frac[N + n0 + (uint32_t)(T.nx + 1)],
+                        T.frac[2 * N + n0], T.frac[2 * N + n0 + (uint32_t)((T.nx + 1) * (T.ny + 1))]};
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      if (f[q] <= 0.0f) continue;
+      den += f[q];
+      if (fl & (1u << (SF_NUM0 + q))) num += f[q] * p_n[q];
+    }
+  }
+  if (den > 0.0f) {
+    float pn = (num - T.rhs[me]) / den;
+    T.pres[me] = T.pres[me] * T.omega1 + pn * T.omega;
+  }
+}
+void sl_k_sor(const SlLaunch &L, uint32_t parity, uint32_t pass) {
+  if (!L.w.total) return;
+  hipLaunchKernelGGL(k_sor, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w, parity, pass);
+}
+
+// ------------------------------------------------------------------------------------------------
+// PROJECT, 30-wgsl-sim.js:299-341: outside the side walls and floor is solid, above the grid is air.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t ctype(const SlTank &T, int32_t i, int32_t j, int32_t k) {
+  if (j >= T.ny) return 0u;
+  if (i < 0 || j < 0 || k < 0 || i >= T.nx || k >= T.nz) return 2u;
+  return T.cellCount[cell_idx(T, i, j, k)] > 0u ? 1u : 0u;
+}
+__device__ __forceinline__ float cpres(const SlTank &T, int32_t i, int32_t j, int32_t k, uint32_t ct) {
+  if (ct != 1u) return 0.0f;
+  return T.pres[colour_base(T, i, j, k) + packed_idx(T, i, j, k)];
+}
+__global__ __launch_bounds__(SL_WG) void k_project(SlWork W) {
+  SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
+  const SlTank &T = W.tanks[t];
+  uint32_t n = lb * SL_WG + threadIdx.x;
+  if (n >= T.nodes) return;
+  uint32_t sxn = T.nx + 1, syn = T.ny + 1;
+  int32_t cx = (int32_t)(n % sxn), cy = (int32_t)((n / sxn) % syn), cz = (int32_t)(n / (sxn * syn));
+  const size_t N = T.nodes;
+  const float ih = T.invh;
+  const uint32_t t0 = ctype(T, cx, cy, cz);
+  const float p0 = cpres(T, cx, cy, cz, t0);
+  if (cy < T.ny && cz < T.nz) {
+    float v = T.vel[n];
+    if (T.frac[n] <= 0.0f) v = T.svel[n];
+    else {
+      uint32_t tm = ctype(T, cx - 1, cy, cz);
+      if (tm == 1u || t0 == 1u) v -= (p0 - cpres(T, cx - 1, cy, cz, tm)) * ih;
+    }
+    T.vel[n] = v;
+  }
+  if (cx < T.nx && cz < T.nz) {
+    float v = T.vel[N + n];
+    if (T.frac[N + n] <= 0.0f) v = T.svel[N + n];
+    else {
+      uint32_t tm = ctype(T, cx, cy - 1, cz);
+      if (tm == 1u || t0 == 1u) v -= (p0 - cpres(T, cx, cy - 1, cz, tm)) * ih;
+    }
+    T.vel[N + n] = v;
+  }
+  if (cx < T.nx && cy < T.ny) {
+    float v = T.vel[2 * N + n];
+    if (T.frac[2 * N + n] <= 0.0f) v = T.svel[2 * N + n];
+    else {
+      uint32_t tm = ctype(T, cx, cy, cz - 1);
+      if (tm == 1u || t0 == 1u) v -= (p0 - cpres(T, cx, cy, cz - 1, tm)) * ih;
+    }
+    T.vel[2 * N + n] = v;
+  }
+}
+void sl_k_project(const SlLaunch &L) {
+  if (!L.w.total) return;
+  hipLaunchKernelGGL(k_project, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+}

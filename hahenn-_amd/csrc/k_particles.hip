@@ -1,0 +1,543 @@
+// Particle-stage kernels: grid-to-particle transfer + advection + collisions + drop merging +
+// whitewater spawning (G2P), emission (EMIT), counter commits (COUNT), compaction, whitewater.
+#include "sl_internal.h"
+
+#define SL_TANK_OF(W, b, t, lb)                               \
+  int t = 0;                                                  \
+  uint32_t lb = (b);                                          \
+  if ((W).ntanks > 1) { t = tank_of((W).start, (W).ntanks, b); lb = (b) - (W).start[t]; }
+
+// ------------------------------------------------------------------------------------------------
+// masked PIC/FLIP gather, 31-wgsl-sim2.js:9-36 -> (PIC, FLIP, found)
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ f3 transfer1(const SlTank &T, f3 q, int comp, float vp) {
+  int32_t bx = f2i(floorf(q.x)), by = f2i(floorf(q.y)), bz = f2i(floorf(q.z));
+  float fx = q.x - (float)bx, fy = q.y - (float)by, fz = q.z - (float)bz;
+  const size_t N = T.nodes;
+  const float *vn = T.vel + comp * N, *vo = T.velOld + comp * N;
+  const uint32_t bit = 1u << comp;
+  float sn = 0.0f, so = 0.0f, sw = 0.0f;
+#pragma unroll
+  for (int dz = 0; dz < 2; dz++) {
+    int32_t k = bz + dz;
+    if (k < 0 || k > T.nz) continue;
+    float wz = dz == 1 ? fz : 1.0f - fz;
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++) {
+      int32_t j = by + dy;
+      if (j < 0 || j > T.ny) continue;
+      float wy = dy == 1 ? fy : 1.0f - fy;
+#pragma unroll
+      for (int dx = 0; dx < 2; dx++) {
+        int32_t i = bx + dx;
+        if (i < 0 || i > T.nx) continue;
+        uint32_t n = node_idx(T, i, j, k);
+        if ((T.vmask[n] & bit) == 0u) continue;
+        float w = (dx == 1 ? fx : 1.0f - fx) * wy * wz;
+        sn += w * vn[n];
+        so += w * vo[n];
+        sw += w;
+      }
+    }
+  }
+  if (sw < 1e-6f) return F3(vp, vp, 0.0f);
+  return F3(sn / sw, vp + (sn - so) / sw, 1.0f);
+}
+// unmasked clamped sample, 31-wgsl-sim2.js:39-59
+__device__ __forceinline__ float sample1(const SlTank &T, f3 q, int comp) {
+  int32_t bx = f2i(floorf(q.x)), by = f2i(floorf(q.y)), bz = f2i(floorf(q.z));
+  float fx = q.x - (float)bx, fy = q.y - (float)by, fz = q.z - (float)bz;
+  const float *vn = T.vel + comp * (size_t)T.nodes;
+  float s = 0.0f;
+  for (int dz = 0; dz < 2; dz++) {
+    int32_t k = min(max(bz + dz, 0), T.nz);
+    float wz = dz == 1 ? fz : 1.0f - fz;
+    for (int dy = 0; dy < 2; dy++) {
+      int32_t j = min(max(by + dy, 0), T.ny);
+      float wy = dy == 1 ? fy : 1.0f - fy;
+      for (int dx = 0; dx < 2; dx++) {
+        int32_t i = min(max(bx + dx, 0), T.nx);
+        s += (dx == 1 ? fx : 1.0f - fx) * wy * wz * vn[node_idx(T, i, j, k)];
+      }
+    }
+  }
+  return s;
+}
+// tank footprint + glass-wall clamp, 31-wgsl-sim2.js:62-80
+__device__ __forceinline__ void fp_max(const SlTank &T, float &hx, float &hz) {
+  hx = T.ox + (float)T.nx * T.h;
+  hz = T.oz + (float)T.nz * T.h;
+}
+__device__ __forceinline__ bool in_footprint(const SlTank &T, f3 x) {
+  float hx, hz;
+  fp_max(T, hx, hz);
+  return x.x > T.ox && x.z > T.oz && x.x < hx && x.z < hz;
+}
+__device__ __forceinline__ void tank_clamp(const SlTank &T, f3 x0, f3 &x, f3 &v, float r) {
+  float hx, hz;
+  fp_max(T, hx, hz);
+  const float lx = T.ox, lz = T.oz;
+  bool wasIn = in_footprint(T, x0);
+  if (wasIn && x.y < T.rim) {
+    if (x.x < lx + r) { x.x = lx + r; v.x = fmaxf(v.x, 0.0f); }
+    if (x.x > hx - r) { x.x = hx - r; v.x = fminf(v.x, 0.0f); }
+    if (x.z < lz + r) { x.z = lz + r; v.z = fmaxf(v.z, 0.0f); }
+    if (x.z > hz - r) { x.z = hz - r; v.z = fminf(v.z, 0.0f); }
+  }
+  if (wasIn && x.y < T.oy + r) { x.y = T.oy + r; v.y = fmaxf(v.y, 0.0f); }
+  if (!wasIn && x.y < T.rim && in_footprint(T, x)) {
+    x = F3(x0.x, x.y, x0.z);
+    v = F3(-0.2f * v.x, v.y, -0.2f * v.z);
+  }
+}
+// 31-wgsl-sim2.js:93-109
+__device__ __forceinline__ uint32_t count_at(const SlTank &T, int32_t x, int32_t y, int32_t z) {
+  if (y >= T.ny || x < 0 || y < 0 || z < 0 || x >= T.nx || z >= T.nz) return 0u;
+  return T.cellCount[cell_idx(T, x, y, z)];
+}
+__device__ __forceinline__ bool air_at(const SlTank &T, int32_t x, int32_t y, int32_t z) {
+  if (y >= T.ny) return true;
+  if (x < 0 || y < 0 || z < 0 || x >= T.nx || z >= T.nz) return false;
+  return T.cellCount[cell_idx(T, x, y, z)] == 0u;
+}
+__device__ __forceinline__ bool near_air(const SlTank &T, i3 c) {
+  return air_at(T, wadd(c.x, -1), c.y, c.z) || air_at(T, wadd(c.x, 1), c.y, c.z) ||
+         air_at(T, c.x, wadd(c.y, -1), c.z) || air_at(T, c.x, wadd(c.y, 1), c.z) ||
+         air_at(T, c.x, c.y, wadd(c.z, -1)) || air_at(T, c.x, c.y, wadd(c.z, 1));
+}
+// 31-wgsl-sim2.js:110-114
+__device__ __forceinline__ void deposit_ripple(const SlTank &T, float x, float z, float impulse) {
+  float hx, hz;
+  fp_max(T, hx, hz);
+  const uint32_t R = T.R;
+  float u = (x - T.ox) / (hx - T.ox), w = (z - T.oz) / (hz - T.oz);
+  uint32_t i = f2u(clampf(rintf(u * (float)R), 0.0f, (float)R));
+  uint32_t j = f2u(clampf(rintf(w * (float)R), 0.0f, (float)R));
+  atomicAdd(&T.rsrc[j * (R + 1u) + i], f2i(clampf(impulse, 0.0f, 2e-3f) * 1e9f));
+}
+__device__ __forceinline__ void ww_store(const SlTank &T, uint32_t k, f3 p, float life, f3 v, float kind) {
+  const int wc = T.ww_cur;
+  T.wp[wc][k] = make_float4(p.x, p.y, p.z, life);
+  T.wv[wc][k] = make_float4(v.x, v.y, v.z, kind);
+}
+// splash crown, 31-wgsl-sim2.js:119-148
+__device__ void splash_crown(const SlTank &T, f3 x, float U, float m, uint32_t id) {
+  if (T.ww_rate <= 0.0f) return;
+  const float PI998 = (float)(3.14159265 * 998.0);
+  const float RS = (float)(998.0 * 0.0728);
+  float D = powf(6.0f * m / PI998, (float)(1.0 / 3.0));
+  float We = 998.0f * U * U * D / 0.0728f;
+  float Oh = 0.001f / sqrtf(RS * D);
+  float K = We * powf(Oh, -0.4f);
+  uint32_t n = f2u(clampf(rintf(4.0f * (K / 2100.0f - 1.0f)), 0.0f, 24.0f));
+  uint32_t nj = K > 700.0f ? (K > 2100.0f ? 2u : 1u) : 0u;
+  if (n + nj == 0u) return;
+  uint32_t s = hashU(id * 2654435761u + T.frame * 97u + 3u);
+  uint32_t base = atomicAdd(&T.C->wwCount, n + nj);
+  uint32_t cap = f2u(T.ww_cap_f);
+  float r0 = 0.6f * D;
+  float ph0 = rndu(s) * 6.2831853f;
+  for (uint32_t i = 0; i < n; i++) {
+    uint32_t k = base + i;
+    if (k >= cap) return;
+    float ph = ph0 + 6.2831853f * ((float)i + 0.3f * rndu(s)) / (float)n;
+    float el = 0.9f + 0.45f * rndu(s);
+    float sp = U * (0.15f + 0.2f * rndu(s));
+    f3 d = F3(cosf(ph), 0.0f, sinf(ph));
+    f3 p = add3(x, mul3(d, r0));
+    float life = 0.5f + 0.4f * rndu(s);
+    f3 v = add3(mul3(mul3(d, cosf(el)), sp), F3(0.0f, sinf(el) * sp, 0.0f));
+    ww_store(T, k, p, life, v, 0.0f);
+  }
+  for (uint32_t j = 0; j < nj; j++) {
+    uint32_t k = base + n + j;
+    if (k >= cap) return;
+    float sp = U * (0.22f + 0.04f * (float)j) * (0.9f + 0.2f * rndu(s));
+    f3 p = F3(x.x + 0.0f, x.y + 0.001f * (float)j, x.z + 0.0f);
+    float a = (rndu(s) - 0.5f) * 0.05f;
+    float c = (rndu(s) - 0.5f) * 0.05f;
+    ww_store(T, k, p, 0.7f, F3(a, sp, c), 0.0f);
+  }
+}
+
+// ------------------------------------------------------------------------------------------------
+// G2P, 31-wgsl-sim2.js:149-251
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SL_WG) void k_g2p(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  uint32_t id = lb * SL_WG + threadIdx.x;
+  if (id >= T.C->count) return;
+  const int cur = T.cur;
+  const uint32_t pf = T.pf[cur][id];
+  if (!(pf & 1u)) return;
+  const float dt = T.dt, h = T.h;
+  const f3 x0 = F3(T.px[cur][id], T.py[cur][id], T.pz[cur][id]);
+  const f3 v0 = F3(T.vx[cur][id], T.vy[cur][id], T.vz[cur][id]);
+  f3 v = v0;
+  const f3 g = grid_pos(T, x0);
+  const bool ing = in_grid(T, g);
+  if (ing) {
+    f3 tx = transfer1(T, sub3(g, F3(0.0f, 0.5f, 0.5f)), 0, v.x);
+    f3 ty = transfer1(T, sub3(g, F3(0.5f, 0.0f, 0.5f)), 1, v.y);
+    f3 tz = transfer1(T, sub3(g, F3(0.5f, 0.5f, 0.0f)), 2, v.z);
+    v = F3(mixf(tx.x, tx.y, T.flip), mixf(ty.x, ty.y, T.flip), mixf(tz.x, tz.y, T.flip));
+  } else {
+    v.y -= T.g * dt;
+  }
+  f3 x = add3(x0, mul3(v, dt));
+  const float rc = T.rc;
+  float d = scene_sdf(T, x);
+  if (d < rc) {
+    f3 nrm = scene_normal(T, x);
+    x = add3(x, mul3(nrm, rc - d));
+    f3 vs = solid_vel_at(T, x);
+    f3 rel = sub3(v, vs);
+    float vn = dot3(rel, nrm);
+    if (vn < 0.0f) v = add3(vs, mul3(sub3(rel, mul3(nrm, vn)), 1.0f - T.friction));
+  }
+  tank_clamp(T, x0, x, v, rc);
+  bool alive = true;
+  if (in_drain(T, x)) alive = false;
+  if (!in_footprint(T, x) && x.y < 0.002f) alive = false;
+  if (fabsf(x.x) > 20.0f || fabsf(x.z) > 20.0f || x.y < -1.0f || x.y > 30.0f) alive = false;
+  const i3 c0 = toi3(g);
+  bool airborne = (pf & 2u) != 0u;
+  if (ing) {
+    uint32_t n0 = count_at(T, c0.x, c0.y, c0.z);
+    if (n0 >= 4u) airborne = false;
+    else if (n0 <= 2u && wadd(c0.y, 2) < T.ny) {
+      uint32_t around = 0u, above = 0u;
+      for (int dz = -1; dz <= 1; dz++)
+        for (int dx = -1; dx <= 1; dx++) {
+          int32_t xx = wadd(c0.x, dx), zz = wadd(c0.z, dz);
+          for (int dy = -1; dy <= 1; dy++) around += count_at(T, xx, wadd(c0.y, dy), zz);
+          above += count_at(T, xx, wadd(c0.y, 1), zz) + count_at(T, xx, wadd(c0.y, 2), zz);
+        }
+      if (around <= 4u && above == 0u) airborne = true;
+    }
+  }
+  const f3 g1 = grid_pos(T, x);
+  const i3 c1 = toi3(g1);
+  if (airborne && v0.y < -0.3f && in_grid(T, g1)) {
+    uint32_t others = count_at(T, c1.x, c1.y, c1.z) - ((c1.x == c0.x && c1.y == c0.y && c1.z == c0.z) ? 1u : 0u);
+    if (others >= 3u) {
+      f3 vl = F3(transfer1(T, sub3(g1, F3(0.0f, 0.5f, 0.5f)), 0, v.x).x,
+                 transfer1(T, sub3(g1, F3(0.5f, 0.0f, 0.5f)), 1, v.y).x,
+                 transfer1(T, sub3(g1, F3(0.5f, 0.5f, 0.0f)), 2, v.z).x);
+      float dv = vl.y - v0.y;
+      if (dv > 0.3f) {
+        v = vl;
+        airborne = false;
+        atomicAdd(&T.C->impacts, 1u);
+        float m = 998.0f * powf(0.5f * h, 3.0f);
+        if (T.R > 0u) deposit_ripple(T, x.x, x.z, 2.0f * m * fminf(dv, 8.0f));
+        float level = T.oy + ((float)c1.y + fminf((float)count_at(T, c1.x, c1.y, c1.z) / T.ppc, 1.0f)) * h;
+        splash_crown(T, F3(x.x, level + 0.25f * h, x.z), -v0.y, m, id);
+      }
+    }
+  }
+  if (ing) atomicMax(&T.C->vmax, __float_as_uint(fminf(len3(v), 60.0f)));
+  if (ing && T.ww_rate > 0.0f) {
+    float sp = len3(v);
+    if (sp > 0.75f && near_air(T, c0)) {
+      uint32_t s = hashU(id * 9781u + T.frame * 6271u + 17u);
+      float e = clampf((sp - 0.75f) / 1.6f, 0.0f, 1.0f);
+      if (rndu(s) < T.ww_rate * e * e * dt) {
+        uint32_t k = atomicAdd(&T.C->wwCount, 1u);
+        if (k < f2u(T.ww_cap_f)) {
+          float j0 = rndu(s), j1 = rndu(s), j2 = rndu(s);
+          f3 j = sub3(F3(j0, j1, j2), F3(0.5f, 0.5f, 0.5f));
+          f3 p = add3(x, mul3(mul3(j, h), 0.6f));
+          float life = 1.0f + 2.5f * rndu(s);
+          f3 wv = add3(mul3(v, 0.8f + 0.35f * rndu(s)), mul3(j, 0.35f));
+          ww_store(T, k, p, life, wv, 0.0f);
+        }
+      }
+    }
+  }
+  T.px[cur][id] = x.x; T.py[cur][id] = x.y; T.pz[cur][id] = x.z;
+  T.vx[cur][id] = v.x; T.vy[cur][id] = v.y; T.vz[cur][id] = v.z;
+  T.pf[cur][id] = (uint8_t)((alive ? 1u : 0u) | (airborne ? 2u : 0u));
+}
+void sl_k_g2p(const SlLaunch &L) {
+  if (!L.w.total) return;
+  hipLaunchKernelGGL(k_g2p, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+}
+
+// ------------------------------------------------------------------------------------------------
+// EMIT, 31-wgsl-sim2.js:262-315 (one thread per emitted particle, `emit_total` of them)
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SL_WG) void k_emit(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, tk, lb);
+  const SlTank &T = W.tanks[tk];
+  uint32_t t = lb * SL_WG + threadIdx.x;
+  if (t >= T.emit_total) return;
+  uint32_t e = 0;
+  for (uint32_t i = 0; i < T.emit_k; i++) if (t >= T.emits[i].info[0]) e = i;
+  const SlEmitDev &em = T.emits[e];
+  uint32_t local = t - em.info[0];
+  uint32_t idx = T.C->count + t;
+  if (idx >= T.cap) return;
+  uint32_t s = hashU(t * 747796405u + em.info[2] * 2891336453u + 1u);
+  f3 pos = F3(0, 0, 0), vel = F3(0, 0, 0);
+  bool mark = false;
+  const uint32_t shape = em.info[3];
+  const int cur = T.cur;
+  if (shape == 0u) {
+    float rr = em.a[3] * sqrtf(rndu(s));
+    float th = 6.2831853f * rndu(s);
+    f3 u = F3(em.c[0], em.c[1], em.c[2]), w = F3(em.d[0], em.d[1], em.d[2]), dir = F3(em.b[0], em.b[1], em.b[2]);
+    f3 ring = mul3(add3(mul3(u, cosf(th)), mul3(w, sinf(th))), rr);
+    float r3 = rndu(s);
+    pos = add3(add3(F3(em.a[0], em.a[1], em.a[2]), ring), mul3(dir, r3 * em.d[3]));
+    f3 dv = dir;
+    if (em.c[3] > 0.0f) {
+      float ca = 1.0f - rndu(s) * (1.0f - cosf(em.c[3]));
+      float sa = sqrtf(fmaxf(1.0f - ca * ca, 0.0f));
+      float ph = 6.2831853f * rndu(s);
+      dv = add3(mul3(dir, ca), mul3(add3(mul3(u, cosf(ph)), mul3(w, sinf(ph))), sa));
+    }
+    float j0 = rndu(s), j1 = rndu(s), j2 = rndu(s);
+    vel = add3(mul3(dv, em.b[3]), mul3(sub3(F3(j0, j1, j2), F3(0.5f, 0.5f, 0.5f)), 0.02f * em.b[3]));
+  } else if (shape == 1u) {
+    float j0 = rndu(s), j1 = rndu(s), j2 = rndu(s);
+    f3 dd = sub3(mul3(F3(j0, j1, j2), 2.0f), F3(1.0f, 1.0f, 1.0f));
+    dd = add3(dd, F3(1e-4f, 1e-4f, 1e-4f));
+    float l = len3(dd);
+    dd = F3(dd.x / l, dd.y / l, dd.z / l);
+    float r = powf(rndu(s), 0.3333f);
+    pos = add3(F3(em.a[0], em.a[1], em.a[2]), mul3(mul3(dd, em.a[3]), r));
+    vel = F3(em.b[0], em.b[1], em.b[2]);
+    mark = em.b[3] > 0.5f;
+  } else if (shape == 2u) {
+    uint32_t nx = f2u(em.b[0]), ny = f2u(em.b[1]);
+    uint32_t i = local % nx, j = (local / nx) % ny, k = local / (nx * ny);
+    float j0 = rndu(s), j1 = rndu(s), j2 = rndu(s);
+    f3 jit = mul3(sub3(F3(j0, j1, j2), F3(0.5f, 0.5f, 0.5f)), 0.3f);
+    pos = add3(F3(em.a[0], em.a[1], em.a[2]),
+               mul3(add3(add3(F3((float)i, (float)j, (float)k), F3(0.5f, 0.5f, 0.5f)), jit), em.a[3]));
+    vel = F3(em.c[0], em.c[1], em.c[2]);
+    f3 g = grid_pos(T, pos);
+    if (em.b[3] > 0.5f && in_grid(T, g)) {
+      i3 c = toi3(g);
+      if ((float)T.cellCount[cell_idx(T, c.x, c.y, c.z)] >= 0.5f * T.ppc) {
+        T.px[cur][idx] = pos.x; T.py[cur][idx] = pos.y; T.pz[cur][idx] = pos.z;
+        T.vx[cur][idx] = 0.0f; T.vy[cur][idx] = 0.0f; T.vz[cur][idx] = 0.0f;
+        T.pf[cur][idx] = 0;
+        return;
+      }
+    }
+  } else {
+    float r0 = rndu(s), r1 = rndu(s), r2 = rndu(s);
+    pos = add3(F3(em.a[0], em.a[1], em.a[2]), F3((r0 * 2.0f - 1.0f) * em.a[3], -r1 * em.d[3], (r2 * 2.0f - 1.0f) * em.a[3]));
+    vel = F3(em.b[0], em.b[1], em.b[2]);
+    mark = true;
+  }
+  bool alive = !(scene_sdf(T, pos) < 0.0f);
+  T.px[cur][idx] = pos.x; T.py[cur][idx] = pos.y; T.pz[cur][idx] = pos.z;
+  T.vx[cur][idx] = vel.x; T.vy[cur][idx] = vel.y; T.vz[cur][idx] = vel.z;
+  T.pf[cur][idx] = (uint8_t)((alive ? 1u : 0u) | (mark ? 2u : 0u));
+}
+void sl_k_emit(const SlLaunch &L) {
+  if (!L.w.total) return;
+  hipLaunchKernelGGL(k_emit, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+}
+
+// ------------------------------------------------------------------------------------------------
+// COUNT, 31-wgsl-sim2.js:332-349 (the indirect arguments are not needed: kernels read the counters)
+// mode 1: count = newCount; 2: count = min(count + y, cap); 3: wwCount = min(wwNew, wcap);
+// 4: clear; 5: vmax = 0 (the writeBuffer of 50-sim.js:306). One thread per tank.
+// ------------------------------------------------------------------------------------------------
+__global__ void k_count(SlTank *tanks, int ntanks, uint32_t mode, uint32_t y) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= ntanks) return;
+  SlTank &T = tanks[t];
+  SlCounters *C = T.C;
+  uint32_t cap = T.cap, wcap = f2u(T.ww_cap_f);
+  if (mode == 1) { C->count = C->newCount; C->newCount = 0; T.cur = 1 - T.cur; }
+  if (mode == 2) { uint32_t yy = y ? y : T.emit_total; uint32_t s = C->count + yy; C->count = s < cap ? s : cap; }
+  if (mode == 3) { C->wwCount = C->wwNew < wcap ? C->wwNew : wcap; C->wwNew = 0; T.ww_cur = 1 - T.ww_cur; }
+  if (mode == 4) { C->count = 0; C->newCount = 0; C->wwCount = 0; C->wwNew = 0; C->impacts = 0; }
+  if (mode == 5) C->vmax = 0;
+}
+void sl_k_count(const SlLaunch &L, uint32_t mode, uint32_t y) {
+  int nt = L.w.ntanks;
+  hipLaunchKernelGGL(k_count, dim3((nt + 63) / 64), dim3(64), 0, L.stream, L.w.tanks, nt, mode, y);
+}
+
+// ------------------------------------------------------------------------------------------------
+// COMPACT, 31-wgsl-sim2.js:317-328, order-preserving: (1) live count per workgroup of
+// SL_COMPACT_ITEMS particles, (2) exclusive scan of the workgroup counts per tank (+ newCount),
+// (3) scatter of the live particles to the other buffer at offset + rank within the workgroup.
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SL_WG) void k_compact_count(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t lim = min(T.C->count, T.cap);
+  const uint32_t base = lb * SL_COMPACT_ITEMS;
+  const uint8_t *pf = T.pf[T.cur];
+  uint32_t c = 0;
+  for (uint32_t q = 0; q < SL_COMPACT_ITEMS / SL_WG; q++) {
+    uint32_t id = base + q * SL_WG + threadIdx.x;
+    if (id < lim && (pf[id] & 1u)) c++;
+  }
+  __shared__ uint32_t red[SL_WG / 64];
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int w = 0; w < SL_WG / 64; w++) s += red[w];
+    T.scanBlocks[lb] = s;
+  }
+}
+void sl_k_compact_count(const SlLaunch &L) {
+  if (!L.w.total) return;
+  hipLaunchKernelGGL(k_compact_count, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+}
+// one workgroup per tank scans its per-workgroup counts in place (exclusive) and sets newCount
+__global__ __launch_bounds__(1024) void k_compact_scan(SlWork W) {
+  const int t = blockIdx.x;
+  const SlTank &T = W.tanks[t];
+  const uint32_t lim = min(T.C->count, T.cap);
+  const uint32_t nb = (lim + SL_COMPACT_ITEMS - 1) / SL_COMPACT_ITEMS;
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  for (uint32_t base = 0; base < nb; base += 1024) {
+    uint32_t i = base + threadIdx.x;
+    uint32_t v = i < nb ? T.scanBlocks[i] : 0u;
+    // inclusive wave scan
+    uint32_t x = v;
+    const int lane = threadIdx.x & 63;
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+      uint32_t ws = wsum[threadIdx.x], acc = ws;
+      for (int o = 1; o < 16; o <<= 1) {
+        uint32_t y = __shfl_up(acc, o, 16);
+        if ((int)threadIdx.x >= o) acc += y;
+      }
+      wsum[threadIdx.x] = acc - ws;  // exclusive prefix of wave sums
+    }
+    __syncthreads();
+    uint32_t excl = carry + wsum[threadIdx.x >> 6] + x - v;
+    if (i < nb) T.scanBlocks[i] = excl;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = excl + v;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) T.C->newCount = carry;
+}
+void sl_k_compact_scan(const SlLaunch &L, int ntanks) {
+  hipLaunchKernelGGL(k_compact_scan, dim3(ntanks), dim3(1024), 0, L.stream, L.w);
+}
+__global__ __launch_bounds__(SL_WG) void k_compact_scatter(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t lim = min(T.C->count, T.cap);
+  const uint32_t base = lb * SL_COMPACT_ITEMS;
+  if (base >= lim) return;
+  const int cur = T.cur, oth = 1 - cur;
+  const uint8_t *pf = T.pf[cur];
+  __shared__ uint32_t wtot[SL_COMPACT_ITEMS / 64];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  // pass 1: per-wave (64-item chunk) counts
+  bool live[SL_COMPACT_ITEMS / SL_WG];
+  uint64_t masks[SL_COMPACT_ITEMS / SL_WG];
+  for (uint32_t q = 0; q < SL_COMPACT_ITEMS / SL_WG; q++) {
+    uint32_t id = base + q * SL_WG + threadIdx.x;
+    live[q] = id < lim && (pf[id] & 1u);
+    masks[q] = __ballot(live[q]);
+    if (lane == 0) wtot[q * (SL_WG / 64) + wv] = (uint32_t)__popcll(masks[q]);
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t s = 0;
+    for (int c = 0; c < SL_COMPACT_ITEMS / 64; c++) { uint32_t v = wtot[c]; wtot[c] = s; s += v; }
+  }
+  __syncthreads();
+  const uint32_t off0 = T.scanBlocks[lb];
+  for (uint32_t q = 0; q < SL_COMPACT_ITEMS / SL_WG; q++) {
+    if (!live[q]) continue;
+    uint32_t id = base + q * SL_WG + threadIdx.x;
+    uint64_t below = masks[q] & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
+    uint32_t d = off0 + wtot[q * (SL_WG / 64) + wv] + (uint32_t)__popcll(below);
+    T.px[oth][d] = T.px[cur][id]; T.py[oth][d] = T.py[cur][id]; T.pz[oth][d] = T.pz[cur][id];
+    T.vx[oth][d] = T.vx[cur][id]; T.vy[oth][d] = T.vy[cur][id]; T.vz[oth][d] = T.vz[cur][id];
+    T.pf[oth][d] = pf[id];
+  }
+}
+void sl_k_compact_scatter(const SlLaunch &L) {
+  if (!L.w.total) return;
+  hipLaunchKernelGGL(k_compact_scatter, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+}
+
+// ------------------------------------------------------------------------------------------------
+// WW, 31-wgsl-sim2.js:361-409: spray / foam / bubbles; survivors appended to the other buffer
+// ------------------------------------------------------------------------------------------------
+__global__ __launch_bounds__(SL_WG) void k_ww(SlWork W, float dt) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t wcap = f2u(T.ww_cap_f);
+  uint32_t id = lb * SL_WG + threadIdx.x;
+  if (id >= min(T.C->wwCount, wcap)) return;
+  const int wc = T.ww_cur;
+  const float4 P = T.wp[wc][id], Vv = T.wv[wc][id];
+  f3 x = F3(P.x, P.y, P.z), v = F3(Vv.x, Vv.y, Vv.z);
+  float life = P.w;
+  const f3 g = grid_pos(T, x);
+  float kind = 0.0f;
+  if (in_grid(T, g)) {
+    i3 c = toi3(g);
+    uint32_t n = T.cellCount[cell_idx(T, c.x, c.y, c.z)];
+    if (n > 0u) {
+      uint32_t up = 0u;
+      if (c.y + 1 < T.ny) up = T.cellCount[cell_idx(T, c.x, c.y + 1, c.z)];
+      if (up > 0u) kind = 2.0f;
+      else if (g.y - (float)c.y < fminf((float)n / T.ppc, 1.0f)) kind = 1.0f;
+    }
+  }
+  if (Vv.w < 0.5f && kind > 0.5f && T.R > 0u && Vv.y < -0.3f) {
+    float hx, hz;
+    fp_max(T, hx, hz);
+    const uint32_t R = T.R;
+    float u = (x.x - T.ox) / (hx - T.ox), w = (x.z - T.oz) / (hz - T.oz);
+    uint32_t i = f2u(clampf(rintf(u * (float)R), 0.0f, (float)R));
+    uint32_t j = f2u(clampf(rintf(w * (float)R), 0.0f, (float)R));
+    atomicAdd(&T.rsrc[j * (R + 1u) + i], f2i(clampf(4.2e-6f * -Vv.y, 0.0f, 1e-4f) * 1e9f));
+  }
+  if (kind == 0.0f) {
+    v.y -= T.g * dt;
+    v = mul3(v, 1.0f / (1.0f + 0.6f * dt));
+    life -= dt * 0.7f;
+  } else {
+    f3 vf = F3(sample1(T, sub3(g, F3(0.0f, 0.5f, 0.5f)), 0), sample1(T, sub3(g, F3(0.5f, 0.0f, 0.5f)), 1),
+               sample1(T, sub3(g, F3(0.5f, 0.5f, 0.0f)), 2));
+    if (kind == 1.0f) { v = F3(vf.x, 0.2f * vf.y, vf.z); life -= dt * 0.5f; }
+    else { v = add3(vf, F3(0.0f, 0.16f, 0.0f)); life -= dt * 0.3f; }
+  }
+  const f3 xb = x;
+  x = add3(x, mul3(v, dt));
+  float d = scene_sdf(T, x);
+  if (d < 0.0f) x = sub3(x, mul3(scene_normal(T, x), d));
+  tank_clamp(T, xb, x, v, 0.001f);
+  bool outside = !in_footprint(T, x) && x.y < 0.001f;
+  if (life > 0.0f && !outside && x.y > -0.5f && !in_drain(T, x)) {
+    uint32_t k = atomicAdd(&T.C->wwNew, 1u);
+    if (k < wcap) {
+      T.wp[1 - wc][k] = make_float4(x.x, x.y, x.z, life);
+      T.wv[1 - wc][k] = make_float4(v.x, v.y, v.z, kind);
+    }
+  }
+}
+void sl_k_ww(const SlLaunch &L, float dt) {
+  if (!L.w.total) return;
+  hipLaunchKernelGGL(k_ww, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w, dt);
+}

@@ -1,0 +1,216 @@
+// Capillary-gravity ripple layer, src/js/64-ripple.js:8-253.
+//
+// The reference runs each 2-D FFT as 2*log2(M) global-memory radix-2 Stockham passes (horizontal
+// first, then vertical). Here one workgroup keeps a whole row (or a tile of CW columns) in LDS and
+// runs all log2(M) passes of that direction there, so each 2-D transform reads and writes the field
+// once per direction. Each output is the same butterfly a + w*b with the same f32 twiddle table, so
+// the result equals the pass-by-pass transform. The elementwise kernels RSRC, RMASK and RSCALE are
+// folded into the loads and stores of the transforms they feed.
+#include "sl_internal.h"
+
+#define SL_TANK_OF(W, b, t, lb)                               \
+  int t = 0;                                                  \
+  uint32_t lb = (b);                                          \
+  if ((W).ntanks > 1) { t = tank_of((W).start, (W).ntanks, b); lb = (b) - (W).start[t]; }
+
+#define FFT_LDS_COMPLEX 4096  // complex values per LDS buffer (two buffers = 64 KiB)
+
+enum { IN_BUF = 0, IN_SRC = 1, IN_MASK = 2 };
+enum { OUT_BUF = 0, OUT_SCALE = 1 };
+
+// butterfly of one output of a Stockham pass (64-ripple.js:44-55)
+__device__ __forceinline__ float2 bfly(const float2 *A, uint32_t idx, uint32_t ns, uint32_t M, uint32_t stride,
+                                       const float2 *tw, float sign) {
+  uint32_t hs = ns >> 1;
+  uint32_t e = (idx / ns) * hs + (idx % hs);
+  uint32_t o = e + (M >> 1);
+  float2 a = A[e * stride], b = A[o * stride];
+  float2 t = tw[(idx % ns) * (M / ns)];
+  float wx = t.x, wy = sign * t.y;
+  return make_float2(a.x + (wx * b.x - wy * b.y), a.y + (wx * b.y + wy * b.x));
+}
+
+__device__ __forceinline__ float2 fft_input(const SlTank &T, int mode, const float2 *in, uint32_t x, uint32_t y) {
+  const uint32_t M = T.M, R = T.R;
+  if (mode == IN_BUF) return in[y * M + x];
+  uint32_t pi = x <= R ? x : M - x, pj = y <= R ? y : M - y;
+  if (mode == IN_SRC) {  // RSRC, 64-ripple.js:21-30
+    float dx = T.rLx / (float)R, dz = T.rLz / (float)R;
+    float v = (float)T.rsrc[pj * (R + 1u) + pi] / 1e9f / (dx * dz);
+    return make_float2(v, 0.0f);
+  }
+  float m = T.rmask[pj * (R + 1u) + pi];  // RMASK, 64-ripple.js:95-103
+  float2 f = in[y * M + x];
+  return make_float2(f.x * m, f.y * m);
+}
+
+// all horizontal passes of one row
+__global__ __launch_bounds__(256) void k_fft_rows(SlWork W, int which_in, int which_out, int mode, float sign) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t M = T.M, y = lb;
+  if (y >= M) return;
+  __shared__ float2 lds[2][FFT_LDS_COMPLEX];
+  const float2 *bufs[5] = {T.rstate, T.ra, T.rb, T.rhat, nullptr};
+  float2 *obufs[5] = {T.rstate, T.ra, T.rb, T.rhat, nullptr};
+  const float2 *in = bufs[which_in];
+  float2 *out = obufs[which_out];
+  for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) lds[0][x] = fft_input(T, mode, in, x, y);
+  __syncthreads();
+  int cur = 0;
+  for (uint32_t ns = 2; ns <= M; ns <<= 1) {
+    for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) lds[cur ^ 1][x] = bfly(lds[cur], x, ns, M, 1, T.tw, sign);
+    __syncthreads();
+    cur ^= 1;
+  }
+  for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) out[y * M + x] = lds[cur][x];
+}
+
+// all vertical passes of a tile of CW = 4096/M adjacent columns
+__global__ __launch_bounds__(256) void k_fft_cols(SlWork W, int which_in, int which_out, int omode, float sign) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t M = T.M;
+  uint32_t CW = FFT_LDS_COMPLEX / M;
+  if (CW > M) CW = M;
+  const uint32_t x0 = lb * CW;
+  if (x0 >= M) return;
+  __shared__ float2 lds[2][FFT_LDS_COMPLEX];
+  const float2 *bufs[4] = {T.rstate, T.ra, T.rb, T.rhat};
+  float2 *obufs[4] = {T.rstate, T.ra, T.rb, T.rhat};
+  const float2 *in = bufs[which_in];
+  float2 *out = obufs[which_out];
+  const uint32_t n = CW * M;
+  // LDS layout: column c at offset c, row y at y*CW (stride CW)
+  for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
+    uint32_t c = q % CW, y = q / CW;
+    lds[0][q] = in[y * M + x0 + c];
+  }
+  __syncthreads();
+  int cur = 0;
+  for (uint32_t ns = 2; ns <= M; ns <<= 1) {
+    for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
+      uint32_t c = q % CW, y = q / CW;
+      lds[cur ^ 1][q] = bfly(lds[cur] + c, y, ns, M, CW, T.tw, sign);
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  const float sc = (float)(M * M);
+  for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
+    uint32_t c = q % CW, y = q / CW;
+    float2 v = lds[cur][q];
+    if (omode == OUT_SCALE) v = make_float2(v.x / sc, v.y / sc);  // RSCALE, 64-ripple.js:127-133
+    out[y * M + x0 + c] = v;
+  }
+}
+
+// REVOLVE, 64-ripple.js:58-90
+__global__ __launch_bounds__(256) void k_revolve(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t M = T.M, R = T.R;
+  uint32_t id = lb * 256 + threadIdx.x;
+  if (id >= M * M) return;
+  int32_t i = (int32_t)(id % M), j = (int32_t)(id / M);
+  int32_t mx = i < (int32_t)R ? i : i - (int32_t)M;
+  int32_t mz = j < (int32_t)R ? j : j - (int32_t)M;
+  if ((mx == 0 && mz == 0) || mx == -(int32_t)R || mz == -(int32_t)R) { T.rstate[id] = make_float2(0.0f, 0.0f); return; }
+  const float PI = (float)3.141592653589793;
+  float kx = (float)mx * PI / T.rLx, kz = (float)mz * PI / T.rLz;
+  float kk = sqrtf(kx * kx + kz * kz);
+  float D = fmaxf(T.rmd, 0.002f);
+  float kd = fminf(kk * D, 15.0f);
+  float th = 1.0f - 2.0f / (expf(2.0f * kd) + 1.0f);
+  float w2 = (T.g * kk + T.rsr * kk * kk * kk) * th;
+  float w = sqrtf(w2);
+  float gam = 2.0f * T.rnu * kk * kk + T.rfilm * kk * sqrtf(T.rnu * w * 0.125f) +
+              kk * sqrtf(T.rnu * w * 0.5f) / fmaxf(sinhf(fminf(2.0f * kd, 30.0f)), 1e-4f);
+  float wd = sqrtf(fmaxf(w2 - gam * gam, 1e-6f));
+  float2 s = T.rstate[id];
+  float sx = s.x, sy = s.y;
+  if (T.rdt > 0.0f) {
+    float c = cosf(wd * T.rdt), sn = sinf(wd * T.rdt), e = expf(-gam * T.rdt);
+    float nx = sx * c + (sy + gam * sx) / wd * sn;
+    float ny = sy * c - (w2 * sx + gam * sy) / wd * sn;
+    sx = e * nx; sy = e * ny;
+  }
+  float hp = 1.0f - expf(-(kk * kk) / (T.rhpk * T.rhpk));
+  float sw = fmaxf(1.2f * T.rLx / (float)R, 0.003f);
+  float spread = expf(-0.5f * kk * kk * sw * sw);
+  sy += -(kk * th / T.rrho) * T.rhat[id].x * hp * spread;
+  T.rstate[id] = make_float2(sx, sy);
+}
+
+// REXTRACT, 64-ripple.js:134-147 (f32 values; the f16 texture is produced on readback)
+__global__ __launch_bounds__(256) void k_rextract(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t M = T.M, R = T.R, R1 = R + 1;
+  uint32_t id = lb * 256 + threadIdx.x;
+  if (id >= R1 * R1) return;
+  uint32_t i = id % R1, j = id / R1;
+  const float2 *f = T.rb;
+  const float s = 1.0f / (float)(M * M);
+  float h = f[j * M + i].x * s;
+  float hl = f[j * M + (i + M - 1u) % M].x * s, hr = f[j * M + (i + 1u) % M].x * s;
+  float hd = f[((j + M - 1u) % M) * M + i].x * s, hu = f[((j + 1u) % M) * M + i].x * s;
+  float dx = T.rLx / (float)R, dz = T.rLz / (float)R;
+  T.rtex[id] = make_float4(h, (hr - hl) / (2.0f * dx), (hu - hd) / (2.0f * dz), 0.0f);
+}
+
+// RMASKBUILD, 64-ripple.js:106-126
+__global__ __launch_bounds__(256) void k_rmaskbuild(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t R = T.R, R1 = R + 1;
+  uint32_t id = lb * 256 + threadIdx.x;
+  if (id >= R1 * R1) return;
+  uint32_t gx = id % R1, gy = id / R1;
+  float u = (float)gx / (float)R, w = (float)gy / (float)R;
+  float hix = T.ox + (float)T.nx * T.h, hiz = T.oz + (float)T.nz * T.h;
+  float xx = T.ox + u * (hix - T.ox), zz = T.oz + w * (hiz - T.oz);
+  float dw = (float)T.wmap;
+  int32_t tx = f2i(clampf(u * dw, 0.0f, dw - 1.0f)), ty = f2i(clampf(w * dw, 0.0f, dw - 1.0f));
+  float hh = T.topH[(size_t)ty * T.wmap + tx];
+  float m;
+  if (hh >= 0.0f) {
+    float d = scene_sdf(T, F3(xx, hh - 0.002f, zz));
+    float tt = clampf((d - -0.003f) / (0.006f - -0.003f), 0.0f, 1.0f);
+    m = tt * tt * (3.0f - 2.0f * tt);
+  } else {
+    m = 0.75f;
+  }
+  T.rmask[id] = m;
+}
+
+__global__ void k_clear_rsrc(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  uint32_t id = lb * 256 + threadIdx.x;
+  uint32_t R1 = T.R + 1;
+  if (id < R1 * R1) T.rsrc[id] = 0;
+}
+
+// buffers: 0 state, 1 a, 2 b, 3 hat
+void sl_k_ripple_frame_ex(const SlWork &rows, const SlWork &cols, const SlWork &elems, const SlWork &tex,
+                          hipStream_t s, int mask) {
+  if (!rows.total) return;
+  // forward transform of the mirrored source -> hat (64-ripple.js:229-232)
+  hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), 0, s, rows, 0, 1, (int)IN_SRC, -1.0f);
+  hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), 0, s, cols, 1, 3, (int)OUT_BUF, -1.0f);
+  // evolve the modes and add the kick (64-ripple.js:233)
+  hipLaunchKernelGGL(k_revolve, dim3(elems.total), dim3(256), 0, s, elems);
+  // inverse transform of the state -> b (64-ripple.js:234-235)
+  hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), 0, s, rows, 0, 1, (int)IN_BUF, 1.0f);
+  hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), 0, s, cols, 1, 2, (int)OUT_BUF, 1.0f);
+  // height and slopes (64-ripple.js:236-238)
+  hipLaunchKernelGGL(k_rextract, dim3(tex.total), dim3(256), 0, s, tex);
+  if (mask) {
+    // absorb: mask the spatial field, transform back into the state, rescale (64-ripple.js:239-250)
+    hipLaunchKernelGGL(k_rmaskbuild, dim3(tex.total), dim3(256), 0, s, tex);
+    hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), 0, s, rows, 2, 1, (int)IN_MASK, -1.0f);
+    hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), 0, s, cols, 1, 0, (int)OUT_SCALE, -1.0f);
+  }
+  hipLaunchKernelGGL(k_clear_rsrc, dim3(tex.total), dim3(256), 0, s, tex);  // 64-ripple.js:252
+}

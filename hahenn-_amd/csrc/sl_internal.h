@@ -1,0 +1,271 @@
+// Internal definitions shared by the HIP kernels and the host driver of the suiri library.
+//
+// Device layout (structure-of-arrays, one tank = one simulation):
+//   particles : px,py,pz,vx,vy,vz (f32) + pf (u8: bit0 alive, bit1 free-flight mark), ping-pong x2
+//   whitewater: wp (float4 pos.xyz, life), wv (float4 vel.xyz, kind), ping-pong x2
+//   MAC nodes : N = (nx+1)(ny+1)(nz+1), x-fastest; per component planes of N floats:
+//               acc (6N i32: momX,wtX,momY,wtY,momZ,wtZ), vel (3N), velOld (3N) + vmask (N u8),
+//               frac (3N), svel (3N)
+//   cells     : C = nx*ny*nz, x-fastest: cellCount (u32), divg (f32)
+//   SOR arrays: red/black split ("packed"): cell (i,j,k) has colour (i+j+k)&1 and slot
+//               (i>>1) + hw*(j + ny*k) with hw = (nx+1)/2, in array `colour`; pres, rhs (f32) and
+//               sflag (u16 per-cell stencil flags) each hold 2*hw*ny*nz entries.
+// A batch is a table of SlTank records in device memory; every stage kernel maps its workgroups onto
+// the tanks through a prefix table of workgroup counts, so one launch per stage serves any number of
+// tanks (a plain simulation is a batch of one).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define SL_MAX_OBJ 64
+#define SL_MAX_EMIT 48
+#define SL_WG 256
+
+// per-cell SOR stencil flags (sflag)
+#define SF_OPEN0 0       // bits 0..5: face open (f > 0), faces -x,+x,-y,+y,-z,+z
+#define SF_NUM0 6        // bits 6..11: face contributes to the numerator (open, in range, fluid neighbour)
+#define SF_FLUID (1u << 12)
+#define SF_SLOW (1u << 13)  // an open face has 0 < f < 1: the kernel re-reads the fractions
+
+struct SlCounters { uint32_t count, newCount, wwCount, wwNew, vmax, impacts, pad0, pad1; };
+struct SlObj { float r0[4], r1[4], r2[4], pos[4], dims[4], vel[4], angv[4]; };
+struct SlEmitDev { float a[4], b[4], c[4], d[4]; uint32_t info[4]; };
+
+struct SlTank {
+  // sizes
+  int32_t nx, ny, nz, hw;
+  uint32_t nodes, cells, hcells;  // hcells = hw*ny*nz
+  uint32_t cap, ww_cap, R, M, logM, iters, obj_count, frame, emit_k, emit_total, cur, ww_cur, wmap;
+  int32_t mask_bound, whitewater_on, pad_i[2];
+  // params as the reference uniform (f32)
+  float ox, oy, oz, h, invh, hh, dt, g, gdt, flip, ppc, drift, omega, omega1, rc, rim;
+  float ww_rate, ww_cap_f, friction, time_f, radius, R_f;
+  // ripple uniform (64-ripple.js:223-226)
+  float rLx, rLz, rdt, rmd, rsr, rnu, rfilm, rhpk, rrho;
+  float pad_f[3];
+  // particles
+  float *px[2], *py[2], *pz[2], *vx[2], *vy[2], *vz[2];
+  uint8_t *pf[2];
+  float4 *wp[2], *wv[2];
+  SlCounters *C;
+  // grid
+  int32_t *acc;
+  uint32_t *cellCount;
+  float *vel, *velOld, *frac, *svel;
+  uint8_t *vmask;
+  float *divg, *pres, *rhs;
+  uint16_t *sflag;
+  SlObj *objs;
+  SlEmitDev *emits;
+  uint32_t *scanBlocks;   // compaction per-workgroup counts / offsets
+  // ripple
+  int32_t *rsrc;
+  float2 *rstate, *ra, *rb, *rhat, *tw;
+  float *rmask, *topH;
+  float4 *rtex;
+};
+
+// Work map of one launch: workgroup b belongs to tank t with start[t] <= b < start[t+1].
+struct SlWork {
+  SlTank *tanks;
+  const uint32_t *start;  // ntanks+1 prefix of workgroup counts (device)
+  int32_t ntanks;
+  uint32_t total;         // start[ntanks]
+};
+
+#ifdef __HIPCC__
+// ----------------------------------------------------------------------------------------------
+// device helpers with WGSL semantics
+// ----------------------------------------------------------------------------------------------
+struct f3 { float x, y, z; };
+struct i3 { int32_t x, y, z; };
+__device__ __forceinline__ f3 F3(float x, float y, float z) { f3 r; r.x = x; r.y = y; r.z = z; return r; }
+__device__ __forceinline__ f3 add3(f3 a, f3 b) { return F3(a.x + b.x, a.y + b.y, a.z + b.z); }
+__device__ __forceinline__ f3 sub3(f3 a, f3 b) { return F3(a.x - b.x, a.y - b.y, a.z - b.z); }
+__device__ __forceinline__ f3 mul3(f3 a, float s) { return F3(a.x * s, a.y * s, a.z * s); }
+__device__ __forceinline__ float dot3(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+__device__ __forceinline__ float len3(f3 a) { return sqrtf(a.x * a.x + a.y * a.y + a.z * a.z); }
+__device__ __forceinline__ float len2f(float x, float y) { return sqrtf(x * x + y * y); }
+__device__ __forceinline__ f3 cross3(f3 a, f3 b) {
+  return F3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
+__device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+__device__ __forceinline__ int32_t f2i(float f) {
+  if (f != f) return 0;
+  if (f >= 2147483648.0f) return INT32_MAX;
+  if (f <= -2147483648.0f) return INT32_MIN;
+  return (int32_t)f;
+}
+__device__ __forceinline__ uint32_t f2u(float f) {
+  if (f != f || f <= 0.0f) return 0u;
+  if (f >= 4294967296.0f) return UINT32_MAX;
+  return (uint32_t)f;
+}
+__device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+__device__ __forceinline__ i3 toi3(f3 g) { i3 r; r.x = f2i(g.x); r.y = f2i(g.y); r.z = f2i(g.z); return r; }
+__device__ __forceinline__ uint32_t hashU(uint32_t x) {
+  x ^= x >> 16; x *= 0x7feb352du; x ^= x >> 15; x *= 0x846ca68bu; x ^= x >> 16;
+  return x;
+}
+__device__ __forceinline__ float rndu(uint32_t &s) { s = hashU(s); return (float)(s >> 8) / 16777216.0f; }
+
+__device__ __forceinline__ uint32_t node_idx(const SlTank &T, int32_t i, int32_t j, int32_t k) {
+  return (uint32_t)(i + (j + k * (T.ny + 1)) * (T.nx + 1));
+}
+__device__ __forceinline__ uint32_t cell_idx(const SlTank &T, int32_t i, int32_t j, int32_t k) {
+  return (uint32_t)(i + (j + k * T.ny) * T.nx);
+}
+// slot of cell (i,j,k) in its colour array
+__device__ __forceinline__ uint32_t packed_idx(const SlTank &T, int32_t i, int32_t j, int32_t k) {
+  return (uint32_t)((i >> 1) + T.hw * (j + T.ny * k));
+}
+__device__ __forceinline__ uint32_t colour_base(const SlTank &T, int32_t i, int32_t j, int32_t k) {
+  return ((i + j + k) & 1) ? T.hcells : 0u;
+}
+__device__ __forceinline__ f3 grid_pos(const SlTank &T, f3 x) {
+  return mul3(sub3(x, F3(T.ox, T.oy, T.oz)), T.invh);
+}
+__device__ __forceinline__ bool in_grid(const SlTank &T, f3 g) {
+  return g.x >= 0.0f && g.y >= 0.0f && g.z >= 0.0f && g.x < (float)T.nx && g.y < (float)T.ny &&
+         g.z < (float)T.nz;
+}
+
+// largest t with start[t] <= b
+__device__ __forceinline__ int tank_of(const uint32_t *start, int ntanks, uint32_t b) {
+  int lo = 0, hi = ntanks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (start[mid] <= b) lo = mid; else hi = mid - 1;
+  }
+  return lo;
+}
+// Workgroup i is dispatched to XCD i % 8. Re-number workgroups so that each XCD works on one
+// contiguous range (its L2 then holds the halos its neighbours in that range re-read).
+__device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t G) {
+  const uint32_t X = 8;
+  uint32_t xcd = i % X, pos = i / X, q = G / X, r = G % X;
+  return xcd < r ? xcd * (q + 1) + pos : r * (q + 1) + (xcd - r) * q + pos;
+}
+
+// ----------------------------------------------------------------------------------------------
+// signed distances (30-wgsl-sim.js:40-114)
+// ----------------------------------------------------------------------------------------------
+__device__ __forceinline__ f3 obj_pos(const SlObj &o) { return F3(o.pos[0], o.pos[1], o.pos[2]); }
+__device__ __forceinline__ f3 to_local(const SlObj &o, f3 p) {
+  f3 d = sub3(p, obj_pos(o));
+  return F3(dot3(F3(o.r0[0], o.r0[1], o.r0[2]), d), dot3(F3(o.r1[0], o.r1[1], o.r1[2]), d),
+            dot3(F3(o.r2[0], o.r2[1], o.r2[2]), d));
+}
+__device__ __forceinline__ float sd_round_box(f3 p, f3 b, float r) {
+  f3 q = F3(fabsf(p.x) - b.x + r, fabsf(p.y) - b.y + r, fabsf(p.z) - b.z + r);
+  float l = len3(F3(fmaxf(q.x, 0.0f), fmaxf(q.y, 0.0f), fmaxf(q.z, 0.0f)));
+  return l + fminf(fmaxf(q.x, fmaxf(q.y, q.z)), 0.0f) - r;
+}
+__device__ __forceinline__ float sd_cyl(f3 p, float r, float hh) {
+  float dx = len2f(p.x, p.z) - r, dy = fabsf(p.y) - hh;
+  return fminf(fmaxf(dx, dy), 0.0f) + len2f(fmaxf(dx, 0.0f), fmaxf(dy, 0.0f));
+}
+__device__ __forceinline__ float sd_cone(f3 p, float r, float hh) {
+  float qx = len2f(p.x, p.z), qy = p.y;
+  float k2x = -r, k2y = 2.0f * hh;
+  float cax = qx - fminf(qx, qy < 0.0f ? r : 0.0f), cay = fabsf(qy) - hh;
+  float t = clampf(((0.0f - qx) * k2x + (hh - qy) * k2y) / (k2x * k2x + k2y * k2y), 0.0f, 1.0f);
+  float cbx = qx - 0.0f + k2x * t, cby = qy - hh + k2y * t;
+  float sg = (cbx < 0.0f && cay < 0.0f) ? -1.0f : 1.0f;
+  return sg * sqrtf(fminf(cax * cax + cay * cay, cbx * cbx + cby * cby));
+}
+__device__ __forceinline__ float shape_sdf(const SlObj &o, f3 p) {
+  f3 q = to_local(o, p);
+  const float *d = o.dims;
+  uint32_t t = f2u(o.pos[3] + 0.5f);
+  float r = 1e9f;
+  switch (t) {
+    case 0: r = sd_round_box(q, F3(d[0], d[1], d[2]), d[3]); break;
+    case 1: r = len3(q) - d[0]; break;
+    case 2: r = sd_cyl(q, d[0], d[1]); break;
+    case 3: r = sd_cone(q, d[0], d[1]); break;
+    case 4: r = len2f(len2f(q.x, q.z) - d[0], q.y) - d[1]; break;
+    case 5: r = fmaxf(sd_cyl(q, d[0], d[1]), -sd_cyl(sub3(q, F3(0.0f, d[2], 0.0f)), d[0] - d[2], d[1])); break;
+    case 6:
+      r = fmaxf(sd_round_box(q, F3(d[0], d[1], d[2]), 0.002f),
+                -sd_round_box(sub3(q, F3(0.0f, d[3], 0.0f)), F3(d[0] - d[3], d[1] - 0.0f, d[2] - d[3]), 0.001f));
+      break;
+    case 7: r = fmaxf(fmaxf(fabsf(len2f(q.y, q.z) - d[1]) - 0.5f * d[2], q.y), fabsf(q.x) - d[0]); break;
+    default: break;
+  }
+  return r;
+}
+__device__ __forceinline__ float scene_sdf(const SlTank &T, f3 p) {
+  float best = 1e9f;
+  for (uint32_t i = 0; i < T.obj_count; i++) {
+    const SlObj &o = T.objs[i];
+    if (o.angv[3] != 1.0f) continue;
+    if (len3(sub3(p, obj_pos(o))) - o.vel[3] > best) continue;
+    best = fminf(best, shape_sdf(o, p));
+  }
+  return best;
+}
+__device__ __forceinline__ f3 scene_normal(const SlTank &T, f3 p) {
+  const float e = 0.0006f;
+  f3 n = F3(scene_sdf(T, add3(p, F3(e, 0, 0))) - scene_sdf(T, sub3(p, F3(e, 0, 0))),
+            scene_sdf(T, add3(p, F3(0, e, 0))) - scene_sdf(T, sub3(p, F3(0, e, 0))),
+            scene_sdf(T, add3(p, F3(0, 0, e))) - scene_sdf(T, sub3(p, F3(0, 0, e))));
+  float l = len3(n);
+  return l > 1e-9f ? F3(n.x / l, n.y / l, n.z / l) : F3(0.0f, 1.0f, 0.0f);
+}
+__device__ __forceinline__ f3 solid_vel_at(const SlTank &T, f3 p) {
+  float best = 1e9f;
+  f3 v = F3(0, 0, 0);
+  for (uint32_t i = 0; i < T.obj_count; i++) {
+    const SlObj &o = T.objs[i];
+    if (o.angv[3] != 1.0f) continue;
+    if (len3(sub3(p, obj_pos(o))) - o.vel[3] > best) continue;
+    float d = shape_sdf(o, p);
+    if (d < best) {
+      best = d;
+      v = add3(F3(o.vel[0], o.vel[1], o.vel[2]), cross3(F3(o.angv[0], o.angv[1], o.angv[2]), sub3(p, obj_pos(o))));
+    }
+  }
+  return v;
+}
+__device__ __forceinline__ bool in_drain(const SlTank &T, f3 p) {
+  for (uint32_t i = 0; i < T.obj_count; i++) {
+    const SlObj &o = T.objs[i];
+    if (o.angv[3] != 2.0f) continue;
+    if (sd_cyl(to_local(o, p), o.dims[0], o.dims[1]) < 0.0f) return true;
+  }
+  return false;
+}
+#endif  // __HIPCC__
+
+// ----------------------------------------------------------------------------------------------
+// host-side launchers (defined in the kernel translation units)
+// ----------------------------------------------------------------------------------------------
+struct SlLaunch {
+  SlWork w;         // tanks + work map
+  hipStream_t stream;
+};
+// grid stages (work map in units of SL_WG-wide workgroups over the stage's index space)
+void sl_k_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
+void sl_k_geom(const SlLaunch &L);
+void sl_k_p2g(const SlLaunch &L, int count_only);
+void sl_k_normforce(const SlLaunch &L);
+void sl_k_divsetup(const SlLaunch &L);
+void sl_k_sor(const SlLaunch &L, uint32_t parity, uint32_t pass);
+void sl_k_project(const SlLaunch &L);
+// particle stages
+void sl_k_g2p(const SlLaunch &L);
+void sl_k_emit(const SlLaunch &L);
+void sl_k_count(const SlLaunch &L, uint32_t mode, uint32_t y);
+void sl_k_vmax_reset(const SlLaunch &L);
+void sl_k_compact_count(const SlLaunch &L);
+void sl_k_compact_scan(const SlLaunch &L, int ntanks);
+void sl_k_compact_scatter(const SlLaunch &L);
+void sl_k_ww(const SlLaunch &L, float dt);
+void sl_k_clear_i32(const SlLaunch &L, int which);
+// ripple: rows = M workgroups per tank, cols = M/CW, elems = M*M/256, tex = (R+1)^2/256
+void sl_k_ripple_frame_ex(const SlWork &rows, const SlWork &cols, const SlWork &elems, const SlWork &tex,
+                          hipStream_t s, int mask);
+// workgroups of compaction per tank (each compaction workgroup covers SL_COMPACT_ITEMS particles)
+#define SL_COMPACT_ITEMS 1024

@@ -1,0 +1,207 @@
+/*
+ * suiri — MI355X-native liquid step (FLIP on a MAC grid with cut-cell face fractions, red-black SOR,
+ * projection, advection, capillary-gravity ripple layer, whitewater), C ABI.
+ *
+ * Every entry point below replaces one method of the reference's JavaScript simulation driver
+ * (`class FluidSim`, src/js/50-sim.js, and `class RippleLayer`, src/js/64-ripple.js). The reference has
+ * no FFI; its callers are the main loop (src/js/90-main.js:5-30, :72-104), the UI
+ * (src/js/80-ui.js:27-93, :334-380) and the automation API (src/js/90-main.js:145-205). A Node N-API
+ * addon (hahenn-_amd/csrc/napi_addon.cpp) binds these functions for the JavaScript host.
+ *
+ * Conventions
+ *   - Status: every int-returning function returns SL_OK (0) or a negative SL_E* code; the message is
+ *     available from sl_last_error(sim) (sl_last_error(NULL) for failures before a sim exists).
+ *     The reference never throws from step() and counts GPU errors (src/js/40-gpu.js:22-26); here
+ *     device errors are sticky per sim and reported by the next call and in sl_stats.gpu_errors.
+ *   - Ownership: the sim owns all device memory. Host inputs are copied at call time (like
+ *     GPUQueue.writeBuffer); readbacks copy into caller-owned memory.
+ *   - Threading: one sim = one HIP stream. Calls on one sim must not overlap. Work is enqueued and
+ *     the call returns (like queue.submit); sl_sync / sl_stats / sl_readback wait for the stream.
+ *   - Layouts on upload/readback are the reference's buffer layouts (array-of-structs particles of
+ *     2 x vec4f, vec4f per MAC node, x-fastest indices); the device layout is structure-of-arrays.
+ */
+#ifndef SUIRI_H
+#define SUIRI_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SL_ABI_VERSION 1
+
+#define SL_OK 0
+#define SL_EARG -1     /* bad argument */
+#define SL_EHIP -2     /* HIP runtime / device error (sticky) */
+#define SL_ENOMEM -3   /* device or host allocation failed */
+#define SL_ESTATE -4   /* call not valid in the current state */
+#define SL_EBUDGET -5  /* admission refused: HBM budget */
+#define SL_ECAP -6     /* size above a fixed maximum */
+
+typedef struct sl_sim sl_sim;
+
+/* Quality preset + physics (QUALITY, src/js/50-sim.js:4-8; constants of writeParams, :151-162;
+   ripple constants, src/js/64-ripple.js:225, :268, :88). sl_config_default fills the reference values. */
+typedef struct sl_config {
+  double tank_w, tank_h, tank_d; /* tank size [m] (scene.tank, src/js/10-scene.js:246) */
+  double h;                      /* requested cell size [m] */
+  uint32_t cap, ww_cap;          /* particle / whitewater capacity */
+  uint32_t iters;                /* SOR iterations per substep */
+  uint32_t ripple_R;             /* ripple physical cells, M = 2R (power of two); 0 = ripple off */
+  double rc_factor;              /* collision radius in cells (0.32) */
+  float gravity, flip, ppc, drift, omega, friction, ww_rate;
+  float r_sigma_rho, r_nu, r_film, r_hp_k, r_rho_kick;
+  uint32_t wmap;                 /* top-height map size for the ripple mask (256) */
+  uint32_t whitewater;           /* settings.whitewater: 0 sets the whitewater rate to 0 (50-sim.js:159) */
+  int32_t device;                /* HIP device ordinal */
+  uint32_t reorder_every;        /* spatial particle re-ordering period in frames (0 = never) */
+  uint32_t reserved[4];
+} sl_config;
+
+/* One emission record, as produced by sourceEmits / burstEmit / blockEmits (src/js/55-emit.js:13-106)
+   and packed by writeEmits (src/js/50-sim.js:222-245). `seed` replaces Math.random() (50-sim.js:234). */
+typedef struct sl_emit {
+  float a[4], b[4], c[4], d[4];
+  uint32_t shape;   /* 0 disc outlet, 1 sphere, 2 jittered lattice, 3 rain box */
+  uint32_t count;   /* requested particles */
+  uint32_t seed;
+  uint32_t pad;
+} sl_emit;
+
+/* FluidSim grid sizing (src/js/50-sim.js:33-45) */
+typedef struct sl_grid_t {
+  int32_t nx, ny, nz;
+  uint32_t nodes, cells, cap, ww_cap, R, M, iters;
+  double h, origin[3], rim, vp, radius;
+} sl_grid_t;
+
+/* suiriLab.stats() (src/js/90-main.js:196) + counters (31-wgsl-sim2.js:5) */
+typedef struct sl_stats_t {
+  uint32_t count, ww_count, impacts, frames;
+  uint32_t full, gpu_errors, substeps_last, reserved;
+  double vmax, time, liters;
+  int32_t grid[3];
+} sl_stats_t;
+
+/* Buffers for sl_upload / sl_readback (reference layouts). */
+enum sl_field {
+  SL_F_PARTICLES = 0, /* cap x {pos.xyz, alive, vel.xyz, mark} f32 (parts[cur], 50-sim.js:65) */
+  SL_F_WW = 1,        /* ww_cap x {pos.xyz, life, vel.xyz, kind} f32 (ww[wwCur], 50-sim.js:66) */
+  SL_F_COUNTERS = 2,  /* 8 x u32 {count,newCount,wwCount,wwNew,vmax,impacts,0,0} */
+  SL_F_ACC = 3,       /* N x 6 i32 (mom,wt per axis) */
+  SL_F_CELLCOUNT = 4, /* C x u32 */
+  SL_F_VEL = 5,       /* N x vec4f */
+  SL_F_VELOLD = 6,    /* N x vec4f, .w = participation mask */
+  SL_F_FRAC = 7,      /* N x vec4f face open fractions */
+  SL_F_SVEL = 8,      /* N x vec4f solid velocity at faces */
+  SL_F_CELLTYPE = 9,  /* C x u32 (readback only; = cellCount > 0) */
+  SL_F_DIVG = 10,     /* C x f32 */
+  SL_F_PRES = 11,     /* C x f32 */
+  SL_F_OBJS = 12,     /* 64 x 28 f32 (readback only) */
+  SL_F_RSRC = 13,     /* (R+1)^2 i32 ripple impulses (1e9 fixed point) */
+  SL_F_RSTATE = 14,   /* M^2 x vec2f ripple spectral state */
+  SL_F_RHAT = 15,     /* M^2 x vec2f forward transform of the source (readback only) */
+  SL_F_RFIELD = 16,   /* M^2 x vec2f inverse transform of the state (height, velocity) (readback only) */
+  SL_F_RMASK = 17,    /* (R+1)^2 f32 absorption mask (readback only) */
+  SL_F_RTEX = 18,     /* (R+1)^2 x 4 f32: height, slope x, slope z, 0 (readback only) */
+  SL_F_TOPH = 19,     /* wmap^2 f32 top-surface height map, -1 dry (input of the ripple mask) */
+  SL_F_RTEX16 = 20,   /* (R+1)^2 x 4 f16 rgba16float texture of the reference (64-ripple.js:160) */
+  SL_F_COUNT_ = 21
+};
+
+/* Single stages, for stage-level parity checks. One call = one dispatch of the reference kernel(s). */
+enum sl_stage {
+  SL_S_GEOM = 0,       /* 30-wgsl-sim.js:118-157 */
+  SL_S_P2G = 1,        /* clear acc + cellCount, then 30-wgsl-sim.js:159-200 */
+  SL_S_NORMFORCE = 2,  /* 30-wgsl-sim.js:203-238 (NORMALIZE then FORCES) */
+  SL_S_DIVERGENCE = 3, /* 30-wgsl-sim.js:240-264 */
+  SL_S_PRESSURE = 4,   /* one colour pass (arg = parity), 30-wgsl-sim.js:267-297 */
+  SL_S_PROJECT = 5,    /* 30-wgsl-sim.js:299-341 */
+  SL_S_G2P = 6,        /* 31-wgsl-sim2.js:149-251 */
+  SL_S_COMPACT = 7,    /* COMPACT + COUNT(1), 31-wgsl-sim2.js:317-339 */
+  SL_S_WW = 8,         /* WW + COUNT(3) with dt = arg-th part... see sl_run_stage_f */
+  SL_S_RIPPLE = 9,     /* RippleLayer.encodeFrame, 64-ripple.js:222-253 */
+  SL_S_SOR = 10,       /* the full SOR loop (iters x red, black) */
+  SL_S_COUNT_ = 11
+};
+
+int sl_config_default(sl_config *cfg, const char *quality); /* "low" | "standard" | "high" */
+
+/* new FluidSim(tank, quality) — src/js/50-sim.js:29-89 (sizing, buffers, COUNT(4)) */
+int sl_create(const sl_config *cfg, sl_sim **out);
+/* FluidSim.destroy() — src/js/50-sim.js:143-149 */
+void sl_destroy(sl_sim *sim);
+const char *sl_last_error(const sl_sim *sim);
+int sl_grid_info(const sl_sim *sim, sl_grid_t *out);
+
+/* The object upload of FluidSim.setObjects(items, dtMove) — src/js/50-sim.js:194-210: `rows` are n
+   packed 28-float Obj records (rows of R^T, pos+type, dims, lin+bound, ang+flag). The host computes the
+   packing and the moving-object velocities (50-sim.js:171-201); an unchanged set is not re-uploaded
+   and does not mark the geometry dirty. *changed (optional) receives 1 when it was. */
+int sl_set_objects(sl_sim *sim, const float *rows, uint32_t n, int *changed);
+
+/* FluidSim.emitNow(list, fill) — src/js/50-sim.js:248-265 (writeEmits budget cap-count-64, `full`) */
+int sl_emit_now(sl_sim *sim, const sl_emit *list, uint32_t n, int fill, uint32_t *emitted);
+
+/* FluidSim.step(frameDt, emitFn) — src/js/50-sim.js:300-336, split so that a host language can call
+   its own emitter between substeps:
+     sl_step_begin: substep count n = clamp(ceil(frameDt / (1.4 h / max(vmax, 0.4))), 1, 6) (or the
+                    forced count of sl_set_substeps) and counters.vmax = 0;
+     sl_substep:    frame++, params(dt), writeEmits(list) + EMIT + COUNT(2), COUNT(0), encodeSubstep;
+     sl_step_end:   COMPACT, COUNT(1), WW, COUNT(3), ripple frame, stats readback. */
+int sl_step_begin(sl_sim *sim, double frame_dt, uint32_t *nsub, double *dt);
+int sl_substep(sl_sim *sim, double dt, const sl_emit *list, uint32_t n, uint32_t *emitted);
+int sl_step_end(sl_sim *sim, double frame_dt);
+/* whole step with a C emitter callback (returns the number of records written to `out`) */
+typedef uint32_t (*sl_emit_fn)(void *user, sl_sim *sim, double dt, double t, sl_emit *out, uint32_t max);
+int sl_step(sl_sim *sim, double frame_dt, sl_emit_fn fn, void *user);
+/* 0 = adaptive substeps as the reference; n > 0 pins the substep count (fixed-dt configurations) */
+int sl_set_substeps(sl_sim *sim, uint32_t n);
+
+/* FluidSim.clearWater() — src/js/50-sim.js:365-373 */
+int sl_clear_water(sl_sim *sim);
+/* FluidSim.saveWater / hasWaterCopy / restoreWater — src/js/50-sim.js:111-141 */
+int sl_save_water(sl_sim *sim, uint32_t *id);
+int sl_has_water_copy(const sl_sim *sim, uint32_t id);
+int sl_restore_water(sl_sim *sim, uint32_t id);
+
+/* FluidSim.pollStats() — src/js/50-sim.js:346-363: non-blocking; *updated = 1 when a readback landed */
+int sl_poll_stats(sl_sim *sim, int *updated);
+/* suiriLab.stats() after onSubmittedWorkDone — src/js/90-main.js:159-165, :196 (waits, then polls) */
+int sl_stats(sl_sim *sim, sl_stats_t *out);
+/* device.queue.onSubmittedWorkDone() */
+int sl_sync(sl_sim *sim);
+
+/* Ripple mask on/off: suiriLab.noRippleMask() — src/js/90-main.js:195 */
+int sl_set_ripple_mask(sl_sim *sim, int bound);
+
+/* State transfer in the reference layouts (new; needed for parity). */
+size_t sl_field_bytes(const sl_sim *sim, int field);
+int sl_upload(sl_sim *sim, int field, const void *src, size_t bytes);
+int sl_readback(sl_sim *sim, int field, void *dst, size_t bytes);
+/* Host-side FluidSim scalars that the reference keeps in JS (count, time, frame, vmax, full):
+   names "count", "time", "frame", "vmax", "full", "cur_dt" */
+int sl_set_scalar(sl_sim *sim, const char *name, double value);
+int sl_get_scalar(const sl_sim *sim, const char *name, double *value);
+
+/* One stage with the current params (stage-level parity). `dt` sets params.dt for the call. */
+int sl_run_stage(sl_sim *sim, int stage, uint32_t arg, double dt);
+
+/* Per-stage device timing with HIP events on the sim's stream. */
+enum sl_tstage {
+  SL_T_EMIT = 0, SL_T_CLEAR, SL_T_GEOM, SL_T_P2G, SL_T_NORMFORCE, SL_T_DIVERGENCE, SL_T_SOR,
+  SL_T_PROJECT, SL_T_G2P, SL_T_COMPACT, SL_T_WW, SL_T_RIPPLE, SL_T_COUNT_
+};
+int sl_timing_enable(sl_sim *sim, int enable);
+/* ms[i] = accumulated milliseconds of stage i since the last reset, launches[i] = kernel launches */
+int sl_timing_read(sl_sim *sim, double *ms, uint32_t *launches, int n, int reset);
+
+/* ABI version + build description */
+int sl_version(void);
+const char *sl_build_info(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,521 @@
+"""Pins the oracle (oracle/suiri_oracle.c) against what the mathematics fixes: published hash
+inverse, closed-form signed distances, face-fraction invariants, P2G partition of unity, a direct
+sparse solve of the SOR system, a divergence-free projection, numpy's FFT, the damped-oscillator
+ODE, and conservation/ordering properties of emission and compaction.
+
+The reference ships no tests, golden vectors or fixtures for this path (its WGSL needs a WebGPU
+runtime that this environment lacks), so these checks are the pins.
+"""
+import ctypes
+import math
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def empty_sim(tank=(0.25, 0.125, 0.125), h=1 / 32, iters=20, R=0, cap=20000, **kw):
+    q = O.quality("standard", h=h, iters=iters, ripple_R=R, cap=cap, ww_cap=4000, **kw)
+    return O.Oracle(tank, q)
+
+
+def obj_row(kind_type, pos, dims, R=None, flag=1, bound=None, lin=(0, 0, 0), ang=(0, 0, 0)):
+    R = R if R is not None else np.eye(3)
+    RT = np.asarray(R).T
+    row = [RT[0, 0], RT[0, 1], RT[0, 2], 0, RT[1, 0], RT[1, 1], RT[1, 2], 0, RT[2, 0], RT[2, 1], RT[2, 2], 0,
+           pos[0], pos[1], pos[2], kind_type, dims[0], dims[1], dims[2], dims[3],
+           lin[0], lin[1], lin[2], bound if bound is not None else 1.0, ang[0], ang[1], ang[2], flag]
+    return np.array(row, dtype=np.float32)
+
+
+def sdf(o, p):
+    return o.L.or_scene_sdf(o.ptr, (ctypes.c_float * 3)(*p))
+
+
+# ------------------------------------------------------------------------------------------------
+def test_hash_is_lowbias32_bijection():
+    """The published lowbias32 hash is inverted by the published inverse constants."""
+    assert (0x846ca68b * 0x43021123) % 2**32 == 1 and (0x7feb352d * 0x1d69e2a5) % 2**32 == 1
+    L = O.lib()
+    rng = np.random.default_rng(1)
+
+    def inv(x):
+        # published inverse (lowbias32_r): x ^= x>>16; x *= 0x43021123; x ^= x>>15 ^ x>>30;
+        # x *= 0x1d69e2a5; x ^= x>>16
+        x ^= x >> 16
+        x = (x * 0x43021123) & 0xFFFFFFFF
+        x ^= (x >> 15) ^ (x >> 30)
+        x = (x * 0x1d69e2a5) & 0xFFFFFFFF
+        x ^= x >> 16
+        return x
+
+    assert L.or_hashU(0) == 0
+    for x in rng.integers(0, 2**32, 2000, dtype=np.uint64):
+        assert inv(L.or_hashU(int(x))) == int(x)
+
+
+def test_sdf_closed_forms():
+    o = empty_sim()
+    rows = [obj_row(1, (0.0, 0.05, 0.0), (0.02, 0, 0, 0), bound=0.02)]
+    o.set_objects(np.array(rows))
+    o.write_params(1 / 60)
+    rng = np.random.default_rng(2)
+    for p in rng.uniform(-0.1, 0.1, (200, 3)):
+        p = p + [0, 0.05, 0]
+        assert sdf(o, p) == pytest.approx(np.linalg.norm(p - [0, 0.05, 0]) - 0.02, abs=2e-6)
+    # box half extents (0.03, 0.02, 0.04), rounding radius 0: exact distance outside faces / inside
+    o.set_objects(np.array([obj_row(0, (0.0, 0.05, 0.0), (0.03, 0.02, 0.04, 0.0), bound=0.06)]))
+    o.write_params(1 / 60)
+    b = np.array([0.03, 0.02, 0.04])
+    for p in rng.uniform(-0.1, 0.1, (300, 3)):
+        q = np.abs(p) - b
+        exact = np.linalg.norm(np.maximum(q, 0)) + min(q.max(), 0.0)
+        assert sdf(o, p + [0, 0.05, 0]) == pytest.approx(exact, abs=3e-6)
+    # torus R=0.05, r=0.01 in the xz plane
+    o.set_objects(np.array([obj_row(4, (0.0, 0.05, 0.0), (0.05, 0.01, 0, 0), bound=0.06)]))
+    o.write_params(1 / 60)
+    for p in rng.uniform(-0.1, 0.1, (300, 3)):
+        exact = math.hypot(math.hypot(p[0], p[2]) - 0.05, p[1]) - 0.01
+        assert sdf(o, p + [0, 0.05, 0]) == pytest.approx(exact, abs=3e-6)
+    # cylinder r=0.02, half height 0.03: closed form outside the side and caps
+    o.set_objects(np.array([obj_row(2, (0.0, 0.05, 0.0), (0.02, 0.03, 0, 0), bound=0.04)]))
+    o.write_params(1 / 60)
+    for p in rng.uniform(-0.1, 0.1, (300, 3)):
+        dr, dy = math.hypot(p[0], p[2]) - 0.02, abs(p[1]) - 0.03
+        exact = min(max(dr, dy), 0) + math.hypot(max(dr, 0), max(dy, 0))
+        assert sdf(o, p + [0, 0.05, 0]) == pytest.approx(exact, abs=3e-6)
+    # rotated box: the SDF is the local-frame distance (rows of R^T)
+    th = 0.7
+    Rz = np.array([[math.cos(th), -math.sin(th), 0], [math.sin(th), math.cos(th), 0], [0, 0, 1]])
+    o.set_objects(np.array([obj_row(0, (0.0, 0.05, 0.0), (0.03, 0.02, 0.04, 0.0), R=Rz, bound=0.06)]))
+    o.write_params(1 / 60)
+    for p in rng.uniform(-0.1, 0.1, (200, 3)):
+        ql = Rz.T @ p
+        q = np.abs(ql) - b
+        exact = np.linalg.norm(np.maximum(q, 0)) + min(q.max(), 0.0)
+        assert sdf(o, p + [0, 0.05, 0]) == pytest.approx(exact, abs=3e-6)
+
+
+def test_sdf_cone_against_brute_force():
+    """Cone (type 3) distance equals the nearest point of a dense sampling of its surface."""
+    o = empty_sim()
+    r, hh = 0.03, 0.025
+    o.set_objects(np.array([obj_row(3, (0.0, 0.05, 0.0), (r, hh, 0, 0), bound=0.05)]))
+    o.write_params(1 / 60)
+    # surface in the (rho, y) half plane: base disc y=-hh, rho<=r; slanted side from (r,-hh) to (0,hh)
+    t = np.linspace(0, 1, 20001)
+    side = np.stack([r * (1 - t), -hh + 2 * hh * t], 1)
+    base = np.stack([r * t, -hh * np.ones_like(t)], 1)
+    surf = np.concatenate([side, base])
+    rng = np.random.default_rng(3)
+    for p in rng.uniform(-0.06, 0.06, (150, 3)):
+        rho, y = math.hypot(p[0], p[2]), p[1]
+        dist = np.min(np.hypot(surf[:, 0] - rho, surf[:, 1] - y))
+        inside = y > -hh and y < hh and rho < r * (hh - y) / (2 * hh)
+        exact = -dist if inside else dist
+        assert sdf(o, p + [0, 0.05, 0]) == pytest.approx(exact, abs=2e-5)
+
+
+def test_face_fractions_empty_and_plane():
+    o = empty_sim()
+    o.set_objects(np.zeros((0, 28), np.float32))
+    o.write_params(1 / 60)
+    o.kernel("geom")
+    inf = o.info()
+    nx, ny, nz = int(inf["nx"]), int(inf["ny"]), int(inf["nz"])
+    fr = o.field("frac").reshape(nz + 1, ny + 1, nx + 1, 4)
+    # x faces: open in the interior, closed at the side walls and outside the cell range
+    assert np.all(fr[:nz, :ny, 1:nx, 0] == 1.0)
+    assert np.all(fr[:, :, 0, 0] == 0) and np.all(fr[:, :, nx, 0] == 0)
+    # y faces: floor closed, top row open (air), interior open
+    assert np.all(fr[:nz, 0, :nx, 1] == 0) and np.all(fr[:nz, ny, :nx, 1] == 1.0)
+    assert np.all(fr[:nz, 1:ny, :nx, 1] == 1.0)
+    assert np.all(fr[:, :, :, 3] == 0)
+    # plane cuts: a box filling everything below y = y0 and its complement (a box filling everything
+    # above): the construction gives complementary open fractions (1 - f), 1/2 where the plane passes
+    # through the face centre, 0 / 1 when the face is entirely inside / outside, monotone in between
+    h = inf["h"]
+    rng = np.random.default_rng(12)
+    for y0 in list(rng.uniform(0.2 * h, (ny - 0.2) * h, 6)) + [2.5 * h]:
+        below = obj_row(0, (0.0, y0 - 0.5, 0.0), (1.0, 0.5, 1.0, 0.0), bound=2.0)
+        above = obj_row(0, (0.0, y0 + 0.5, 0.0), (1.0, 0.5, 1.0, 0.0), bound=2.0)
+        got = []
+        for row in (below, above):
+            o.set_objects(np.array([row]))
+            o.write_params(1 / 60)
+            o.kernel("geom")
+            fr = o.field("frac").reshape(nz + 1, ny + 1, nx + 1, 4)
+            got.append(fr[nz // 2, :ny, nx // 2, 0].astype(np.float64).copy())
+        fb, fa = got
+        assert np.allclose(fb + fa, 1.0, atol=2e-6)
+        yc = (np.arange(ny) + 0.5) * h
+        d = yc - y0
+        assert np.all(fb[d >= h / 2 + 1e-6] == 1.0) and np.all(fb[d <= -h / 2 - 1e-6] == 0.0)
+        assert np.all(np.diff(fb) >= 0)
+        if abs(y0 - 2.5 * h) < 1e-12:
+            assert fb[2] == pytest.approx(0.5, abs=1e-6)
+
+
+def place_particles(o, pts, vel):
+    parts = o.field("parts")
+    n = len(pts)
+    parts[:] = 0
+    parts[:n, :3] = pts
+    parts[:n, 3] = 1.0
+    parts[:n, 4:7] = vel
+    o.field("counters")[0] = n
+
+
+def test_p2g_partition_of_unity_and_order_independence():
+    o = empty_sim()
+    o.write_params(1 / 60)
+    inf = o.info()
+    rng = np.random.default_rng(4)
+    lo = np.array([inf["ox"], inf["oy"], inf["oz"]])
+    ext = np.array([inf["nx"], inf["ny"], inf["nz"]]) * inf["h"]
+    # at least one cell from the walls, where all 8 nodes of every component exist
+    h = inf["h"]
+    pts = lo + h + (ext - 2 * h) * rng.uniform(0, 1, (3000, 3))
+    vel = rng.uniform(-2, 2, (3000, 3))
+    place_particles(o, pts, vel)
+    o.field("acc")[:] = 0
+    o.field("cellCount")[:] = 0
+    o.kernel("p2g")
+    acc = o.field("acc").astype(np.int64).copy()
+    cnt = o.field("cellCount").copy()
+    assert cnt.sum() == 3000
+    for c in range(3):
+        total_w = acc[:, 2 * c + 1].sum()
+        assert abs(total_w - 3000 * 262144) <= 3000 * 8   # per-weight rounding of 8 splats
+    # permuted particle order -> bit-identical accumulators (integer atomics)
+    perm = rng.permutation(3000)
+    place_particles(o, pts[perm], vel[perm])
+    o.field("acc")[:] = 0
+    o.field("cellCount")[:] = 0
+    o.kernel("p2g")
+    assert np.array_equal(o.field("acc"), acc) and np.array_equal(o.field("cellCount"), cnt)
+    # particles at cell quarter points carry dyadic weights (products of 1/4 and 3/4), so with a dyadic
+    # velocity every fixed-point term is exact and NORMALIZE recovers the velocity exactly
+    q4 = np.array([0.25, 0.75])
+    cells = np.argwhere(np.ones((int(inf["nx"]) - 2, int(inf["ny"]) - 2, int(inf["nz"]) - 2), bool)) + 1
+    sub = np.array(np.meshgrid(q4, q4, q4, indexing="ij")).reshape(3, -1).T
+    g = (cells[:, None, :] + sub[None, :, :]).reshape(-1, 3)
+    pts = (lo + g * h).astype(np.float32)
+    place_particles(o, pts, np.tile([0.75, -0.5, 0.25], (len(pts), 1)))
+    o.field("acc")[:] = 0
+    o.field("cellCount")[:] = 0
+    o.kernel("p2g")
+    o.kernel("normalize")
+    vo = o.field("velOld")
+    for c, v in enumerate([0.75, -0.5, 0.25]):
+        m = (vo[:, 3].astype(np.int32) >> c) & 1 == 1
+        assert m.sum() > 50
+        assert np.all(vo[m, c] == np.float32(v))
+
+
+def build_sor_system(o):
+    """The 7-point system of 30-wgsl-sim.js:267-297 assembled from the oracle's buffers in float64."""
+    inf = o.info()
+    nx, ny, nz = int(inf["nx"]), int(inf["ny"]), int(inf["nz"])
+    h = inf["h"]
+    fr = o.field("frac").reshape(nz + 1, ny + 1, nx + 1, 4).astype(np.float64)
+    ct = o.field("cellType").reshape(nz, ny, nx)
+    dv = o.field("divg").reshape(nz, ny, nx).astype(np.float64)
+    idx = -np.ones((nz, ny, nx), np.int64)
+    fluid = np.argwhere(ct == 1)
+    idx[tuple(fluid.T)] = np.arange(len(fluid))
+    import scipy.sparse as sp
+    rows, cols, vals = [], [], []
+    rhs = np.zeros(len(fluid))
+    for n, (k, j, i) in enumerate(fluid):
+        faces = [(fr[k, j, i, 0], i > 0, (k, j, i - 1)), (fr[k, j, i + 1, 0], i < nx - 1, (k, j, i + 1)),
+                 (fr[k, j, i, 1], j > 0, (k, j - 1, i)), (fr[k, j + 1, i, 1], j < ny - 1, (k, j + 1, i)),
+                 (fr[k, j, i, 2], k > 0, (k - 1, j, i)), (fr[k + 1, j, i, 2], k < nz - 1, (k + 1, j, i))]
+        den = 0.0
+        for f, ok, nb in faces:
+            if f <= 0:
+                continue
+            den += f
+            if ok and ct[nb] == 1:
+                rows.append(n); cols.append(idx[nb]); vals.append(-f)
+        if den == 0:   # closed cell: the reference leaves p unchanged (0 here)
+            rows.append(n); cols.append(n); vals.append(1.0)
+            continue
+        rows.append(n); cols.append(n); vals.append(den)
+        rhs[n] = -h * h * dv[k, j, i]
+    A = sp.csr_matrix((vals, (rows, cols)), shape=(len(fluid), len(fluid)))
+    return A, rhs, fluid
+
+
+def test_sor_converges_to_direct_solve():
+    """Many red-black SOR sweeps reach the solution of the same sparse system solved directly."""
+    import scipy.sparse.linalg as spla
+    o = empty_sim(tank=(0.1875, 0.125, 0.125), h=1 / 32, iters=1)
+    # an obstacle so that some faces are fractional
+    o.set_objects(np.array([obj_row(1, (0.02, 0.05, 0.0), (0.025, 0, 0, 0), bound=0.03)]))
+    o.write_params(1 / 60)
+    o.kernel("geom")
+    inf = o.info()
+    nx, ny, nz = int(inf["nx"]), int(inf["ny"]), int(inf["nz"])
+    rng = np.random.default_rng(5)
+    cc = o.field("cellCount")
+    cc[:] = 0
+    cc.reshape(nz, ny, nx)[:, : ny * 3 // 4, :] = 8     # fluid in the lower three quarters
+    o.field("vel")[:] = rng.uniform(-0.3, 0.3, o.field("vel").shape).astype(np.float32)
+    o.kernel("normalize")  # cellType from cellCount (acc is zero -> vel cleared)
+    o.field("vel")[:] = rng.uniform(-0.3, 0.3, o.field("vel").shape).astype(np.float32)
+    o.kernel("divergence")
+    A, rhs, fluid = build_sor_system(o)
+    p_direct = spla.spsolve(A.tocsc(), rhs)
+    o.field("pres")[:] = 0
+    for _ in range(4000):
+        o.kernel("pressure", 0)
+        o.kernel("pressure", 1)
+    p = o.field("pres").reshape(nz, ny, nx)[tuple(fluid.T)].astype(np.float64)
+    scale = np.abs(p_direct).max()
+    assert np.abs(p - p_direct).max() <= 2e-4 * scale
+    # non-fluid cells hold 0
+    ct = o.field("cellType")
+    assert np.all(o.field("pres")[ct == 0] == 0)
+
+
+def test_sor_colouring():
+    """A red pass changes only red cells; a black pass only black cells."""
+    o = empty_sim(iters=1)
+    o.write_params(1 / 60)
+    o.kernel("geom")
+    inf = o.info()
+    nx, ny, nz = int(inf["nx"]), int(inf["ny"]), int(inf["nz"])
+    o.field("cellCount")[:] = 8
+    o.kernel("normalize")
+    rng = np.random.default_rng(6)
+    o.field("divg")[:] = rng.normal(0, 1, o.field("divg").shape).astype(np.float32)
+    o.field("pres")[:] = rng.normal(0, 1e-3, o.field("pres").shape).astype(np.float32)
+    k, j, i = np.meshgrid(np.arange(nz), np.arange(ny), np.arange(nx), indexing="ij")
+    par = ((i + j + k) & 1).ravel()
+    for colour in (0, 1):
+        before = o.field("pres").copy()
+        o.kernel("pressure", colour)
+        after = o.field("pres")
+        assert np.array_equal(before[par != colour], after[par != colour])
+        assert np.any(before[par == colour] != after[par == colour])
+
+
+def test_projection_removes_divergence():
+    """DIVERGENCE -> converged SOR -> PROJECT leaves max |div| (minus the drift target) ~ 0."""
+    o = empty_sim(tank=(0.1875, 0.125, 0.125), h=1 / 32, iters=1)
+    o.set_objects(np.array([obj_row(0, (0.0, 0.03, 0.0), (0.03, 0.02, 0.03, 0.003), bound=0.05)]))
+    o.write_params(1 / 120)
+    o.kernel("geom")
+    inf = o.info()
+    nx, ny, nz = int(inf["nx"]), int(inf["ny"]), int(inf["nz"])
+    rng = np.random.default_rng(7)
+    cc = o.field("cellCount")
+    cc[:] = 0
+    cc.reshape(nz, ny, nx)[:, : ny // 2, :] = 8
+    o.kernel("normalize")
+    o.field("vel")[:] = rng.uniform(-0.5, 0.5, o.field("vel").shape).astype(np.float32)
+    o.kernel("forces")
+    o.kernel("divergence")
+    d0 = np.abs(o.field("divg")).max()
+    for _ in range(6000):
+        o.kernel("pressure", 0)
+        o.kernel("pressure", 1)
+    o.kernel("project")
+    o.kernel("divergence")
+    d1 = np.abs(o.field("divg")).max()
+    assert d0 > 10
+    assert d1 < 1e-4 * 60, (d0, d1)   # 1e-4 of the initial scale per unit of velocity
+
+
+def test_fft_matches_numpy():
+    o = empty_sim(R=32)
+    inf = o.info()
+    M = int(inf["M"])
+    rng = np.random.default_rng(8)
+    x = (rng.normal(size=(M, M)) + 1j * rng.normal(size=(M, M))).astype(np.complex64)
+    a = o.field("ra")
+    a[:, 0] = x.real.ravel()
+    a[:, 1] = x.imag.ravel()
+    out = o.field("rhat")
+    o.L.or_rfft2d(o.ptr, a.ctypes.data, ctypes.c_float(-1.0), out.ctypes.data)
+    y = out[:, 0] + 1j * out[:, 1]
+    ref = np.fft.fft2(x.astype(np.complex128)).ravel()
+    assert np.abs(y - ref).max() <= 1e-5 * np.abs(ref).max()
+    # sign +1 is the unnormalised inverse
+    a[:, 0] = x.real.ravel()
+    a[:, 1] = x.imag.ravel()
+    o.L.or_rfft2d(o.ptr, a.ctypes.data, ctypes.c_float(1.0), out.ctypes.data)
+    y = out[:, 0] + 1j * out[:, 1]
+    ref = (np.fft.ifft2(x.astype(np.complex128)) * M * M).ravel()
+    assert np.abs(y - ref).max() <= 1e-5 * np.abs(ref).max()
+
+
+def ripple_sim(R=32, depth=0.05):
+    o = empty_sim(tank=(0.25, 0.125, 0.125), R=R)
+    o.write_params(1 / 60)
+    return o
+
+
+def revolve_once(o, dt, depth, state, hat):
+    o.field("rstate")[:] = state
+    o.field("rhat")[:] = hat
+    L = o.L
+    # set the ripple uniform through a frame-less call: mimic RippleLayer's uniform
+    inf = o.info()
+    lx, lz = inf["nx"] * inf["h"], inf["nz"] * inf["h"]
+    sim = ctypes.cast(o.ptr, ctypes.POINTER(ctypes.c_byte))
+    # rU_l lives in the struct; use or_ripple_frame's first steps by hand instead
+    return lx, lz
+
+
+def test_revolve_is_the_damped_oscillator():
+    """REVOLVE advances (H, V) by the exact solution of H' = V, V' = -w^2 H - 2 g V with
+    w^2 = (g k + sigma/rho k^3) tanh(k D) (capillary-gravity dispersion); checked against RK4."""
+    o = empty_sim(tank=(0.25, 0.125, 0.125), R=16)
+    o.write_params(1 / 60)
+    inf = o.info()
+    M, R = int(inf["M"]), int(inf["R"])
+    Lx, Lz = inf["nx"] * inf["h"], inf["nz"] * inf["h"]
+    rng = np.random.default_rng(9)
+    st0 = rng.normal(0, 1e-3, (M * M, 2)).astype(np.float32)
+    o.field("rsrc")[:] = 0
+    # run a frame with zero source and no mask: the state is only revolved (+ inverse FFT)
+    o.set("mask_bound", 0)
+    o.field("rstate")[:] = st0
+    dt, D = 1 / 60, 0.05
+    o.L.or_ripple_frame(o.ptr, dt, D)
+    st1 = o.field("rstate").astype(np.float64)
+    q = o.q
+    g, sr, nu, film = 9.81, q.r_sigma_rho, q.r_nu, q.r_film
+    checked = 0
+    for idx in rng.choice(M * M, 40, replace=False):
+        i, j = idx % M, idx // M
+        mx = i if i < R else i - M
+        mz = j if j < R else j - M
+        if (mx == 0 and mz == 0) or mx == -R or mz == -R:
+            assert st1[idx, 0] == 0 and st1[idx, 1] == 0
+            continue
+        k = math.hypot(mx * math.pi / Lx, mz * math.pi / Lz)
+        kd = min(k * D, 15)
+        w2 = (g * k + sr * k ** 3) * math.tanh(kd)
+        w = math.sqrt(w2)
+        gam = 2 * nu * k * k + film * k * math.sqrt(nu * w / 8) + k * math.sqrt(nu * w / 2) / max(math.sinh(min(2 * kd, 30)), 1e-4)
+        y = np.array(st0[idx], np.float64)
+        f = lambda s: np.array([s[1], -w2 * s[0] - 2 * gam * s[1]])
+        nstep = 4000
+        hh = dt / nstep
+        for _ in range(nstep):
+            k1 = f(y); k2 = f(y + hh / 2 * k1); k3 = f(y + hh / 2 * k2); k4 = f(y + hh * k3)
+            y = y + hh / 6 * (k1 + 2 * k2 + 2 * k3 + k4)
+        scale = np.abs(st0[idx]).max() * max(1.0, w)
+        assert abs(st1[idx, 0] - y[0]) <= 2e-4 * scale and abs(st1[idx, 1] - y[1]) <= 2e-4 * scale * w
+        checked += 1
+    assert checked > 20
+
+
+def test_ripple_mask_one_is_identity_round_trip():
+    """With a mask of 1 the masked path (inverse FFT, x1, forward FFT, /M^2) returns the state."""
+    o = empty_sim(tank=(0.25, 0.125, 0.125), R=16)
+    o.write_params(1 / 60)
+    inf = o.info()
+    M = int(inf["M"])
+    rng = np.random.default_rng(10)
+    st0 = rng.normal(0, 1e-3, (M * M, 2)).astype(np.float32)
+    o.field("rstate")[:] = st0
+    o.field("mask")[:] = 1.0
+    L = o.L
+    out = L.or_rfft2d(o.ptr, o.field("rstate").ctypes.data, ctypes.c_float(1.0), o.field("rb").ctypes.data)
+    L.or_rmask(o.ptr, o.field("rb").ctypes.data, o.field("ra").ctypes.data)
+    L.or_rfft2d(o.ptr, o.field("ra").ctypes.data, ctypes.c_float(-1.0), o.field("rstate").ctypes.data)
+    L.or_rscale(o.ptr)
+    assert np.abs(o.field("rstate") - st0).max() <= 1e-5 * np.abs(st0).max()
+
+
+def test_lattice_emit_and_stable_compaction():
+    o = empty_sim(cap=50000)
+    o.write_params(1 / 60)
+    inf = o.info()
+    h = inf["h"]
+    sp = h / 2
+    start = [inf["ox"] + 2 * h, 0.5 * h, inf["oz"] + h]
+    nx, ny, nz = 6, 4, 3
+    rec = {"shape": 2, "a": start + [sp], "b": [nx, ny, nz, 0], "c": [0.1, -0.2, 0.3, 0], "count": nx * ny * nz, "seed": 77}
+    o.emit_now([rec])
+    n = nx * ny * nz
+    parts = o.field("parts")[:n].copy()
+    assert o.field("counters")[0] == n
+    ii, jj, kk = np.meshgrid(np.arange(nx), np.arange(ny), np.arange(nz), indexing="ij")
+    lat = np.stack([ii.T.ravel(), jj.T.ravel(), kk.T.ravel()], 1)  # local = i + nx*(j + ny*k)
+    lat = np.stack([np.arange(n) % nx, (np.arange(n) // nx) % ny, np.arange(n) // (nx * ny)], 1)
+    rel = (parts[:, :3] - np.array(start, np.float32)) / sp - (lat + 0.5)
+    assert np.all(np.abs(rel) <= 0.15 + 1e-5)          # jitter of +-0.15 lattice spacings
+    assert np.all(parts[:, 3] == 1) and np.allclose(parts[:, 4:7], [0.1, -0.2, 0.3])
+    # kill every third particle, compact: survivors keep their order
+    parts_all = o.field("parts")
+    parts_all[:n:3, 3] = 0
+    keep = parts_all[:n][parts_all[:n, 3] >= 0.5].copy()
+    o.kernel("compact")
+    o.set("cur", 1 - o.info()["cur"])
+    o.kernel("count", 1, 0)
+    assert o.field("counters")[0] == len(keep)
+    assert np.array_equal(o.field("parts")[: len(keep)], keep)
+
+
+def test_g2p_uniform_field_and_ballistic():
+    o = empty_sim(iters=1)
+    o.write_params(1 / 120)
+    o.kernel("geom")
+    inf = o.info()
+    nx, ny, nz = int(inf["nx"]), int(inf["ny"]), int(inf["nz"])
+    v = np.array([0.3, -0.2, 0.1], np.float32)
+    vel = o.field("vel")
+    vel[:, :3] = v
+    vo = o.field("velOld")
+    vo[:, :3] = v
+    vo[:, 3] = 7
+    o.field("cellCount")[:] = 8
+    lo = np.array([inf["ox"], inf["oy"], inf["oz"]])
+    ext = np.array([nx, ny, nz]) * inf["h"]
+    rng = np.random.default_rng(11)
+    pts = lo + ext * rng.uniform(0.3, 0.7, (500, 3))
+    vp = rng.uniform(-1, 1, (500, 3)).astype(np.float32)
+    vp[:250] = v
+    place_particles(o, pts, vp)
+    # one ballistic particle above the grid
+    parts = o.field("parts")
+    parts[500, :3] = [0.0, inf["rim"] + 0.05, 0.0]
+    parts[500, 3] = 1
+    parts[500, 4:7] = [0.1, 0.0, 0.0]
+    o.field("counters")[0] = 501
+    o.kernel("g2p")
+    out = o.field("parts")
+    # unchanged grid: PIC = v, FLIP = particle velocity + 0, blended 0.96 (the reference's default)
+    assert np.allclose(out[:250, 4:7], v, atol=1e-6)
+    assert np.allclose(out[250:500, 4:7], 0.04 * v + 0.96 * vp[250:], atol=1e-6)
+    assert np.allclose(out[:250, :3], pts[:250] + v * (1 / 120), atol=1e-6)
+    assert out[500, 5] == pytest.approx(-9.81 / 120, rel=1e-6)   # gravity only
+
+
+def test_dam_break_steps_conserve_and_are_deterministic():
+    def run():
+        q = O.quality("standard", h=1 / 32, iters=30, ripple_R=0, cap=20000, ww_cap=2000, flip=0.95, omega=1.7)
+        q.ww_rate = 0.0
+        o = O.Oracle((0.5, 0.25, 0.25), q)
+        inf = o.info()
+        h = inf["h"]
+        rec = {"shape": 2, "a": [inf["ox"], 0.0, inf["oz"], h / 2], "b": [2 * 6, 2 * 5, 2 * 8, 0],
+               "c": [0, 0, 0, 0], "count": 12 * 10 * 16, "seed": 5}
+        o.emit_now([rec])
+        o.poll = None
+        for _ in range(20):
+            o.step(1 / 120, nsub=1)
+        return o
+    a, b = run(), run()
+    n = a.info()["count"]
+    assert n == 12 * 10 * 16                              # nothing leaves a closed tank
+    assert np.array_equal(a.field("parts")[: int(n)], b.field("parts")[: int(n)])
+    p = a.field("parts")[: int(n)]
+    assert np.all(p[:, 3] == 1)
+    assert np.isfinite(p).all() and p[:, 4:7].std() > 0.01
