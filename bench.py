@@ -1,0 +1,189 @@
+#!/usr/bin/env python3
+"""Benchmark of the liquid step on one MI355X.
+
+Workload (default): faucet into an obstacle field, 256x128x128 MAC grid (h = 1/256 m, 1.0 x 0.5 x
+0.5 m tank) initially half full at 8 particles per cell (16.8 M particles), a sphere (r 5 cm), a
+cylinder (r 3 cm, h 20 cm), a torus (R 6 cm, r 1.5 cm) and a drain (r 2 cm); a 0.5 L/s faucet at
+1.5 m/s angled 30 degrees down; red-black SOR omega 1.8 with 60 iterations; dt = 1/240 s, one
+substep per step; FLIP 0.95; whitewater on; ripple layer M = 1024 (R = 512) with its mask.
+
+A step = one FluidSim.step (emission, P2G, normalise+forces, divergence, 60 SOR iterations,
+projection, G2P, compaction, whitewater, ripple frame). Inputs are resident in HBM before timing.
+Prints one JSON line: particle-steps/s, the SOR kernel's HBM roofline and a CPU baseline (the fp32
+C restatement of the reference, OpenMP on the host cores, on a bounded sample of the same workload).
+"""
+import argparse
+import importlib
+import json
+import math
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+suiri = importlib.import_module("hahenn-_amd")
+scene = importlib.import_module("hahenn-_amd.scene")
+
+HBM_PEAK_GBS = 8000.0
+
+
+def faucet_workload(grid=(256, 128, 128), fill_layers=None, iters=60, R=512, ww=True):
+    nx, ny, nz = grid
+    h = 1.0 / 256 * (256 / nx)
+    tank = (nx * h, ny * h, nz * h)
+    fill = fill_layers if fill_layers is not None else ny // 2
+    p_fill = nx * fill * nz * 8
+    items = [
+        scene.make_obstacle("sphere", pos=[-0.25 * tank[0], 0.1 * tank[1] * 2, -0.2 * tank[2]], dims={"r": 0.05}),
+        scene.make_obstacle("cylinder", pos=[0.0, 0.1, 0.2 * tank[2]], dims={"r": 0.03, "h": 0.2}),
+        scene.make_obstacle("torus", pos=[0.25 * tank[0], 0.12, -0.1 * tank[2]], dims={"R": 0.06, "r": 0.015}),
+        scene.make_drain(pos=[0.4 * tank[0], 0.0, 0.3 * tank[2]], r=0.02),
+        scene.make_source(pos=[-0.42 * tank[0], 0.9 * tank[1], 0.0], mode="stream", flow=30.0, speed=1.5,
+                          pitch=-30, yaw=90),
+    ]
+    cap = p_fill + 400_000
+    cfg = dict(tank_w=tank[0], tank_h=tank[1], tank_d=tank[2], h=h, iters=iters, ripple_R=R, cap=cap,
+               ww_cap=260_000, flip=0.95, omega=1.8, whitewater=1 if ww else 0, reorder_every=16)
+    return dict(cfg=cfg, items=items, fill=fill, frame_dt=1.0 / 240, grid=grid, name="faucet-obstacles")
+
+
+def build_sim(wl):
+    g = suiri.Sim(suiri.config("high", **wl["cfg"]))
+    rows, _ = scene.ObjectPacker().pack(wl["items"], 0)
+    g.set_objects_rows(rows)
+    rec = scene.cell_block_emit(g.h, g.origin, 0, g.nx, 0, wl["fill"], 0, g.nz, seed=12345)
+    g.emit_now([rec])
+    g.reorder()
+    g.set_substeps(1)
+    em = scene.Emitters(wl["items"], g.vp, seed=99)
+    return g, em
+
+
+def sor_bytes_per_launch(nx, ny, nz):
+    """Algorithmic bytes of one colour pass: 4C (p read) + 2C (p write, half) + 12N (face
+    fractions) + 2C (rhs, half) + 1C (flags) = 12N + 9C."""
+    N = (nx + 1) * (ny + 1) * (nz + 1)
+    C = nx * ny * nz
+    return 12 * N + 9 * C
+
+
+def step_bytes(nx, ny, nz, P, iters):
+    N = (nx + 1) * (ny + 1) * (nz + 1)
+    C = nx * ny * nz
+    return 92 * P + 252 * N + 40 * C + iters * (24 * N + 18 * C)
+
+
+def cpu_baseline(wl, steps, threads):
+    """The oracle (fp32 C restatement, OpenMP) on the same workload for `steps` steps."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    c = wl["cfg"]
+    q = O.quality("high", h=c["h"], iters=c["iters"], ripple_R=c["ripple_R"], cap=c["cap"], ww_cap=c["ww_cap"],
+                  flip=c["flip"], omega=c["omega"])
+    o = O.Oracle((c["tank_w"], c["tank_h"], c["tank_d"]), q, threads=threads)
+    rows, _ = scene.ObjectPacker().pack(wl["items"], 0)
+    o.set_objects(rows)
+    inf = o.info()
+    rec = scene.cell_block_emit(inf["h"], [inf["ox"], inf["oy"], inf["oz"]], 0, int(inf["nx"]), 0, wl["fill"], 0,
+                                int(inf["nz"]), seed=12345)
+    o.emit_now([rec])
+    em = scene.Emitters(wl["items"], inf["vp"], seed=99)
+    o.step(wl["frame_dt"], nsub=1, emit_fn=em)   # first step includes GEOM: untimed
+    t0 = time.perf_counter()
+    psteps = 0
+    for _ in range(steps):
+        o.step(wl["frame_dt"], nsub=1, emit_fn=em)
+        psteps += o.info()["count"]
+    dt = time.perf_counter() - t0
+    o.close()
+    return psteps / dt, dt
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--grid", default="256x128x128")
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-ripple", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+    grid = tuple(int(x) for x in args.grid.split("x"))
+    wl = faucet_workload(grid=grid, R=0 if args.no_ripple else 512)
+    g, em = build_sim(wl)
+    for _ in range(args.warmup):
+        g.step(wl["frame_dt"], em)
+    st0 = g.stats()
+    g.timing(True)
+    g.timing_read(reset=True)
+    g.sync()
+    t0 = time.perf_counter()
+    counts = []
+    for i in range(args.steps):
+        g.step(wl["frame_dt"], em)
+        if i % 25 == 24:
+            counts.append(g.stats()["count"])
+    g.sync()
+    elapsed = time.perf_counter() - t0
+    st1 = g.stats()
+    tm = g.timing_read(reset=True)
+    g.timing(False)
+    # timed region without event overhead
+    g.sync()
+    t1 = time.perf_counter()
+    for _ in range(args.steps):
+        g.step(wl["frame_dt"], em)
+    g.sync()
+    elapsed_plain = time.perf_counter() - t1
+    st2 = g.stats()
+    mean_p = (st1["count"] + st2["count"]) / 2
+    ms_per_step = elapsed_plain / args.steps * 1e3
+    value = mean_p * args.steps / elapsed_plain
+    nx, ny, nz = g.nx, g.ny, g.nz
+    iters = wl["cfg"]["iters"]
+    sor_ms, sor_launches = tm["sor"]
+    sor_launch_s = sor_ms / 1e3 / max(sor_launches, 1)
+    bpl = sor_bytes_per_launch(nx, ny, nz)
+    achieved = bpl / sor_launch_s / 1e9
+    out = {
+        "metric": "particle-steps/s at 256x128x128; SOR HBM GB/s; batched p99 step latency",
+        "value": value,
+        "unit": "particle-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "steps_per_s": 1e3 / ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (half-full tank from the reference's lattice emitter, seeded)",
+        "config": {"workload": wl["name"], "grid": list(grid), "particles": int(mean_p), "sor_iters": iters,
+                   "omega": 1.8, "dt": wl["frame_dt"], "ripple_M": 2 * wl["cfg"]["ripple_R"],
+                   "whitewater": True},
+        "roofline": {"bound": "hbm", "kernel": "k_sor (one colour pass)", "achieved": achieved,
+                     "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": None, "bytes_per_launch": bpl, "avg_launch_us": sor_launch_s * 1e6},
+        "sor_GBps": iters * 2 * bpl / (sor_ms / 1e3 / args.steps) / 1e9,
+        "stage_ms_per_step": {k: v[0] / args.steps for k, v in tm.items()},
+        "step_algorithmic_GB": step_bytes(nx, ny, nz, mean_p, iters) / 1e9,
+        "gpu_errors": st2["gpuErrors"],
+    }
+    if not args.no_cpu:
+        threads = os.cpu_count() or 1
+        v, secs = cpu_baseline(wl, args.cpu_steps, threads)
+        out["cpu_baseline"] = {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+                               "sample": f"{args.cpu_steps} steps of the same workload after 1 untimed step "
+                                         f"({secs:.1f} s), oracle/suiri_oracle.c with OpenMP"}
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -74,7 +74,7 @@ EXPORTS = ["sl_config_default", "sl_create", "sl_destroy", "sl_last_error", "sl_
            "sl_clear_water", "sl_save_water", "sl_has_water_copy", "sl_restore_water", "sl_poll_stats",
            "sl_stats", "sl_sync", "sl_set_ripple_mask", "sl_field_bytes", "sl_upload", "sl_readback",
            "sl_set_scalar", "sl_get_scalar", "sl_run_stage", "sl_timing_enable", "sl_timing_read",
-           "sl_version", "sl_build_info"]
+           "sl_version", "sl_build_info", "sl_reorder"]
 
 _lib = None
 
@@ -106,6 +106,7 @@ def load():
         "sl_step": (I, [P, D, P, P]),
         "sl_set_substeps": (I, [P, U]),
         "sl_clear_water": (I, [P]),
+        "sl_reorder": (I, [P]),
         "sl_save_water": (I, [P, ctypes.POINTER(U)]),
         "sl_has_water_copy": (I, [P, U]),
         "sl_restore_water": (I, [P, U]),
@@ -212,6 +213,9 @@ class Sim:
             self._check(self.L.sl_substep(self.ptr, dt.value, ctypes.byref(arr), len(recs), None))
         self._check(self.L.sl_step_end(self.ptr, float(frame_dt)))
         return n.value
+
+    def reorder(self):
+        self._check(self.L.sl_reorder(self.ptr))
 
     def clear_water(self):
         self._check(self.L.sl_clear_water(self.ptr))

@@ -10,6 +10,7 @@
 #include <string.h>
 #include <string>
 #include <vector>
+#include <algorithm>
 #include "../../include/suiri.h"
 #include "sl_internal.h"
 
@@ -54,6 +55,12 @@ struct sl_sim {
   bool full = false, geom_dirty = true;
   int cur = 0, ww_cur = 0;
   uint32_t forced_substeps = 0, substeps_last = 0;
+  // spatial re-ordering
+  uint32_t *sort_keys = nullptr, *sort_vals = nullptr;
+  void *sort_temp = nullptr;
+  size_t sort_temp_bytes = 0;
+  int sort_end_bit = 1;
+  uint32_t reorder_clock = 0;
   double cur_dt = 1.0 / 60.0;
   int mask_bound = 1;
   float objs[SL_MAX_OBJ * 28];
@@ -341,6 +348,18 @@ static int create_impl(sl_sim *s) {
     }
     HIPCHK(s, hipMemcpy(T.tw, tw.data(), tw.size() * 4, hipMemcpyHostToDevice));
   }
+  // brick-Morton keys: 3 bits per level of the largest brick coordinate, + 1 for the outside key
+  {
+    uint32_t mb = (uint32_t)std::max(std::max((s->nx + 3) / 4, (s->ny + 3) / 4), (s->nz + 3) / 4);
+    int bits = 0;
+    while ((1u << bits) < mb) bits++;
+    T.brick_bits = bits;
+    s->sort_end_bit = 3 * bits + 1;
+    ALLOC(s->sort_keys, (size_t)s->cap * 8);
+    ALLOC(s->sort_vals, (size_t)s->cap * 8);
+    s->sort_temp_bytes = sl_sort_temp_bytes(s->cap, s->sort_end_bit);
+    ALLOC(s->sort_temp, s->sort_temp_bytes);
+  }
   ALLOC(s->d_tank, sizeof(SlTank));
   // pinned host memory
   HIPCHK(s, hipHostMalloc((void **)&s->h_counters, sizeof(SlCounters), hipHostMallocDefault));
@@ -474,7 +493,7 @@ extern "C" int sl_emit_now(sl_sim *s, const sl_emit *list, uint32_t n, int fill,
     if (fill) {
       // fresh per-cell counts so that the pour only fills cells without water (50-sim.js:253-261)
       sl_k_fill_u32(s->stream, s->ht.cellCount, 0, s->cells);
-      sl_k_p2g(launch_of(s, s->p_ub), 1);
+      sl_k_p2g(launch_of(s, s->p_ub, SL_P2G_CHUNK), 1);
     }
     r = encode_emit(s, k, total);
     if (r) return r;
@@ -520,7 +539,7 @@ static int encode_substep(sl_sim *s) {
   {
     StageScope sc(s, SL_T_P2G, 2);
     sl_k_fill_u32(s->stream, s->ht.cellCount, 0, s->cells);
-    sl_k_p2g(launch_of(s, pb), 0);
+    sl_k_p2g(launch_of(s, pb, SL_P2G_CHUNK), 0);
   }
   { StageScope sc(s, SL_T_NORMFORCE, 1); sl_k_normforce(launch_of(s, s->nodes)); }
   { StageScope sc(s, SL_T_DIVERGENCE, 1); sl_k_divsetup(launch_of(s, s->cells)); }
@@ -586,11 +605,31 @@ static int encode_stats_copy(sl_sim *s) {
   return SL_OK;
 }
 
+static int encode_reorder(sl_sim *s) {
+  StageScope sc(s, SL_T_COMPACT, 5);
+  int e = sl_k_reorder(launch_of(s, 1), s->p_ub, s->sort_keys, s->sort_vals, s->sort_temp, s->sort_temp_bytes,
+                       s->sort_end_bit);
+  if (e) { s->gpu_errors++; FAIL(s, SL_EHIP, "particle re-ordering failed: %s", hipGetErrorString((hipError_t)e)); }
+  sl_k_count(launch_of(s, 1), 6, 0);
+  s->cur ^= 1;
+  s->reorder_clock = 0;
+  return check_launch(s, "reorder");
+}
+
+extern "C" int sl_reorder(sl_sim *s) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  return encode_reorder(s);
+}
+
 extern "C" int sl_step_end(sl_sim *s, double frame_dt) {
   if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
   if (!(frame_dt > 0)) return SL_OK;
   int r = encode_compact(s);
   if (r) return r;
+  if (s->cfg.reorder_every && ++s->reorder_clock >= s->cfg.reorder_every) {
+    r = encode_reorder(s);
+    if (r) return r;
+  }
   {
     StageScope sc(s, SL_T_WW, 2);
     sl_k_ww(launch_of(s, s->ww_cap), (float)frame_dt);
@@ -1005,7 +1044,7 @@ extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
     case SL_S_P2G:
       sl_k_fill_u32(s->stream, (uint32_t *)s->ht.acc, 0, (size_t)s->nodes * 6);
       sl_k_fill_u32(s->stream, s->ht.cellCount, 0, s->cells);
-      sl_k_p2g(launch_of(s, s->p_ub), 0);
+      sl_k_p2g(launch_of(s, s->p_ub, SL_P2G_CHUNK), 0);
       break;
     case SL_S_NORMFORCE: sl_k_normforce(launch_of(s, s->nodes)); break;
     case SL_S_DIVERGENCE: sl_k_divsetup(launch_of(s, s->cells)); break;

@@ -109,26 +109,140 @@ __device__ __forceinline__ void splat(const SlTank &T, f3 q, float val, int comp
     }
   }
 }
+// Workgroup-privatised P2G: a workgroup takes a chunk of P2G_CHUNK consecutive particles, finds the
+// box of MAC nodes they touch, accumulates all their fixed-point contributions (and cell counts) in
+// LDS with LDS atomics, then adds the box to the global accumulators with one global atomic per
+// non-zero entry. Integer sums are associative, so the result is identical to per-particle global
+// atomics. Chunks whose box does not fit in LDS (scattered particles) use global atomics directly.
+#define P2G_CHUNK SL_P2G_CHUNK
+#define P2G_PER_THREAD (P2G_CHUNK / SL_WG)
+#define P2G_LDS_INTS 11264  // 44 KiB
+
+__device__ __forceinline__ void splat_lds(int32_t *box, int32_t vol, int32_t i0, int32_t j0, int32_t k0, int32_t ni,
+                                          int32_t nj, int32_t imax, int32_t jmax, int32_t kmax, f3 q, float val,
+                                          int comp) {
+  int32_t bx = f2i(floorf(q.x)), by = f2i(floorf(q.y)), bz = f2i(floorf(q.z));
+  float fx = q.x - (float)bx, fy = q.y - (float)by, fz = q.z - (float)bz;
+  int32_t *mom = box + (2 * comp) * vol, *wt = box + (2 * comp + 1) * vol;
+#pragma unroll
+  for (int dz = 0; dz < 2; dz++) {
+    int32_t k = bz + dz;
+    if (k < 0 || k > kmax) continue;
+    float wz = dz == 1 ? fz : 1.0f - fz;
+#pragma unroll
+    for (int dy = 0; dy < 2; dy++) {
+      int32_t j = by + dy;
+      if (j < 0 || j > jmax) continue;
+      float wy = dy == 1 ? fy : 1.0f - fy;
+#pragma unroll
+      for (int dx = 0; dx < 2; dx++) {
+        int32_t i = bx + dx;
+        if (i < 0 || i > imax) continue;
+        float w = (dx == 1 ? fx : 1.0f - fx) * wy * wz;
+        if (w <= 0.0f) continue;
+        int32_t e = ((k - k0) * nj + (j - j0)) * ni + (i - i0);
+        atomicAdd(&mom[e], f2i(rintf(w * val * VSCALE)));
+        atomicAdd(&wt[e], f2i(rintf(w * VSCALE)));
+      }
+    }
+  }
+}
+
 template <int COUNT_ONLY>
 __global__ __launch_bounds__(SL_WG) void k_p2g(SlWork W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
-  uint32_t id = lb * SL_WG + threadIdx.x;
-  if (id >= T.C->count) return;
+  __shared__ int32_t box[P2G_LDS_INTS];
+  __shared__ int32_t bb[6];
+  const uint32_t cnt = T.C->count;
+  const uint32_t base = lb * P2G_CHUNK;
+  if (base >= cnt) return;
   const int cur = T.cur;
-  if (!(T.pf[cur][id] & 1u)) return;
-  f3 x = F3(T.px[cur][id], T.py[cur][id], T.pz[cur][id]);
-  f3 g = grid_pos(T, x);
-  if (!in_grid(T, g)) return;
-  i3 c = toi3(g);
-  atomicAdd(&T.cellCount[cell_idx(T, c.x, c.y, c.z)], 1u);
-  if (COUNT_ONLY) return;
-  float vx = clampf(T.vx[cur][id], -30.0f, 30.0f);
-  float vy = clampf(T.vy[cur][id], -30.0f, 30.0f);
-  float vz = clampf(T.vz[cur][id], -30.0f, 30.0f);
-  splat(T, sub3(g, F3(0.0f, 0.5f, 0.5f)), vx, 0);
-  splat(T, sub3(g, F3(0.5f, 0.0f, 0.5f)), vy, 1);
-  splat(T, sub3(g, F3(0.5f, 0.5f, 0.0f)), vz, 2);
+  const int32_t nx = T.nx, ny = T.ny, nz = T.nz;
+  f3 g[P2G_PER_THREAD];
+  bool ok[P2G_PER_THREAD];
+  int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
+#pragma unroll
+  for (int q = 0; q < P2G_PER_THREAD; q++) {
+    uint32_t id = base + q * SL_WG + threadIdx.x;
+    ok[q] = false;
+    if (id < cnt && (T.pf[cur][id] & 1u)) {
+      g[q] = grid_pos(T, F3(T.px[cur][id], T.py[cur][id], T.pz[cur][id]));
+      if (in_grid(T, g[q])) {
+        ok[q] = true;
+        i3 c = toi3(g[q]);
+        mn[0] = min(mn[0], c.x); mn[1] = min(mn[1], c.y); mn[2] = min(mn[2], c.z);
+        mx[0] = max(mx[0], c.x); mx[1] = max(mx[1], c.y); mx[2] = max(mx[2], c.z);
+      }
+    }
+  }
+  if (threadIdx.x < 6) bb[threadIdx.x] = threadIdx.x < 3 ? INT32_MAX : INT32_MIN;
+  __syncthreads();
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    int32_t lo = mn[a], hi = mx[a];
+    for (int o = 32; o > 0; o >>= 1) {
+      lo = min(lo, __shfl_xor(lo, o, 64));
+      hi = max(hi, __shfl_xor(hi, o, 64));
+    }
+    if ((threadIdx.x & 63) == 0) { atomicMin(&bb[a], lo); atomicMax(&bb[3 + a], hi); }
+  }
+  __syncthreads();
+  if (bb[0] > bb[3]) return;  // no particle of the chunk is in the grid (uniform across the workgroup)
+  // node box [c-1, c+1] around the cell box, clamped to the node range; cell box for the counts
+  const int32_t i0 = max(bb[0] - 1, 0), j0 = max(bb[1] - 1, 0), k0 = max(bb[2] - 1, 0);
+  const int32_t i1 = min(bb[3] + 1, nx), j1 = min(bb[4] + 1, ny), k1 = min(bb[5] + 1, nz);
+  const int32_t ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
+  const int32_t vol = ni * nj * nk;
+  const int32_t ci = bb[3] - bb[0] + 1, cj = bb[4] - bb[1] + 1, ck = bb[5] - bb[2] + 1;
+  const int32_t cvol = ci * cj * ck;
+  const int32_t need = (COUNT_ONLY ? 0 : 6 * vol) + cvol;
+  const bool use_lds = (int64_t)need <= P2G_LDS_INTS;
+  const size_t N = T.nodes;
+  int32_t *cbox = box + (COUNT_ONLY ? 0 : 6 * vol);
+  if (use_lds) {
+    for (int32_t e = threadIdx.x; e < need; e += SL_WG) box[e] = 0;
+    __syncthreads();
+  }
+#pragma unroll
+  for (int q = 0; q < P2G_PER_THREAD; q++) {
+    if (!ok[q]) continue;
+    uint32_t id = base + q * SL_WG + threadIdx.x;
+    i3 c = toi3(g[q]);
+    if (use_lds) atomicAdd(&cbox[((c.z - bb[2]) * cj + (c.y - bb[1])) * ci + (c.x - bb[0])], 1);
+    else atomicAdd(&T.cellCount[cell_idx(T, c.x, c.y, c.z)], 1u);
+    if (COUNT_ONLY) continue;
+    float vx = clampf(T.vx[cur][id], -30.0f, 30.0f);
+    float vy = clampf(T.vy[cur][id], -30.0f, 30.0f);
+    float vz = clampf(T.vz[cur][id], -30.0f, 30.0f);
+    f3 qx = sub3(g[q], F3(0.0f, 0.5f, 0.5f)), qy = sub3(g[q], F3(0.5f, 0.0f, 0.5f)), qz = sub3(g[q], F3(0.5f, 0.5f, 0.0f));
+    if (use_lds) {
+      splat_lds(box, vol, i0, j0, k0, ni, nj, nx, ny, nz, qx, vx, 0);
+      splat_lds(box, vol, i0, j0, k0, ni, nj, nx, ny, nz, qy, vy, 1);
+      splat_lds(box, vol, i0, j0, k0, ni, nj, nx, ny, nz, qz, vz, 2);
+    } else {
+      splat(T, qx, vx, 0);
+      splat(T, qy, vy, 1);
+      splat(T, qz, vz, 2);
+    }
+  }
+  if (!use_lds) return;
+  __syncthreads();
+  if (!COUNT_ONLY) {
+    for (int32_t e = threadIdx.x; e < 6 * vol; e += SL_WG) {
+      int32_t v = box[e];
+      if (v == 0) continue;
+      int32_t comp2 = e / vol, l = e - comp2 * vol;
+      int32_t kk = l / (ni * nj), r = l - kk * ni * nj, jj = r / ni, ii = r - jj * ni;
+      atomicAdd(&T.acc[(size_t)comp2 * N + node_idx(T, i0 + ii, j0 + jj, k0 + kk)], v);
+    }
+  }
+  for (int32_t e = threadIdx.x; e < cvol; e += SL_WG) {
+    int32_t v = cbox[e];
+    if (v == 0) continue;
+    int32_t kk = e / (ci * cj), r = e - kk * ci * cj, jj = r / ci, ii = r - jj * ci;
+    atomicAdd(&T.cellCount[cell_idx(T, bb[0] + ii, bb[1] + jj, bb[2] + kk)], (uint32_t)v);
+  }
 }
 void sl_k_p2g(const SlLaunch &L, int count_only) {
   if (!L.w.total) return;

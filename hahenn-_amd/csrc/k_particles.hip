@@ -163,14 +163,11 @@ __device__ void splash_crown(const SlTank &T, f3 x, float U, float m, uint32_t i
 // ------------------------------------------------------------------------------------------------
 // G2P, 31-wgsl-sim2.js:149-251
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(SL_WG) void k_g2p(SlWork W) {
-  SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
-  uint32_t id = lb * SL_WG + threadIdx.x;
-  if (id >= T.C->count) return;
+// one particle; returns the bit pattern of its speed candidate for vmax (0 when it has none)
+__device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
   const int cur = T.cur;
   const uint32_t pf = T.pf[cur][id];
-  if (!(pf & 1u)) return;
+  if (!(pf & 1u)) return 0u;
   const float dt = T.dt, h = T.h;
   const f3 x0 = F3(T.px[cur][id], T.py[cur][id], T.pz[cur][id]);
   const f3 v0 = F3(T.vx[cur][id], T.vy[cur][id], T.vz[cur][id]);
@@ -237,7 +234,7 @@ __global__ __launch_bounds__(SL_WG) void k_g2p(SlWork W) {
       }
     }
   }
-  if (ing) atomicMax(&T.C->vmax, __float_as_uint(fminf(len3(v), 60.0f)));
+  const uint32_t vm = ing ? __float_as_uint(fminf(len3(v), 60.0f)) : 0u;
   if (ing && T.ww_rate > 0.0f) {
     float sp = len3(v);
     if (sp > 0.75f && near_air(T, c0)) {
@@ -259,6 +256,24 @@ __global__ __launch_bounds__(SL_WG) void k_g2p(SlWork W) {
   T.px[cur][id] = x.x; T.py[cur][id] = x.y; T.pz[cur][id] = x.z;
   T.vx[cur][id] = v.x; T.vy[cur][id] = v.y; T.vz[cur][id] = v.z;
   T.pf[cur][id] = (uint8_t)((alive ? 1u : 0u) | (airborne ? 2u : 0u));
+  return vm;
+}
+// vmax: atomicMax on the bit pattern of a non-negative float (31-wgsl-sim2.js:233), reduced over
+// the workgroup first so that one atomic per workgroup reaches the counter
+__global__ __launch_bounds__(SL_WG) void k_g2p(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  uint32_t id = lb * SL_WG + threadIdx.x;
+  uint32_t vm = id < T.C->count ? g2p_one(T, id) : 0u;
+  for (int o = 32; o > 0; o >>= 1) vm = max(vm, (uint32_t)__shfl_xor((int)vm, o, 64));
+  __shared__ uint32_t red[SL_WG / 64];
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vm;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t m = red[0];
+    for (int w = 1; w < SL_WG / 64; w++) m = max(m, red[w]);
+    if (m) atomicMax(&T.C->vmax, m);
+  }
 }
 void sl_k_g2p(const SlLaunch &L) {
   if (!L.w.total) return;
@@ -360,6 +375,7 @@ __global__ void k_count(SlTank *tanks, int ntanks, uint32_t mode, uint32_t y) {
   if (mode == 3) { C->wwCount = C->wwNew < wcap ? C->wwNew : wcap; C->wwNew = 0; T.ww_cur = 1 - T.ww_cur; }
   if (mode == 4) { C->count = 0; C->newCount = 0; C->wwCount = 0; C->wwNew = 0; C->impacts = 0; }
   if (mode == 5) C->vmax = 0;
+  if (mode == 6) T.cur = 1 - T.cur;
 }
 void sl_k_count(const SlLaunch &L, uint32_t mode, uint32_t y) {
   int nt = L.w.ntanks;

@@ -1,0 +1,68 @@
+// Spatial re-ordering of the particle arrays: a stable sort of the live particles by the Morton
+// code of the 4x4x4-cell brick that holds them (particles outside the grid last, in their order).
+//
+// The reference appends survivors with an atomic counter (31-wgsl-sim2.js:317-328), so its particle
+// order is arbitrary; any permutation is a valid state. A brick-Morton order keeps the particles of
+// one workgroup in a compact block of cells, which P2G's LDS privatisation and G2P's gathers need.
+#include <string.h>
+#include <hip/hip_runtime.h>
+#include <rocprim/rocprim.hpp>
+#include "sl_internal.h"
+
+#define SL_TANK_OF(W, b, t, lb)                               \
+  int t = 0;                                                  \
+  uint32_t lb = (b);                                          \
+  if ((W).ntanks > 1) { t = tank_of((W).start, (W).ntanks, b); lb = (b) - (W).start[t]; }
+
+__global__ __launch_bounds__(SL_WG) void k_sort_keys(SlWork W, uint32_t *keys, uint32_t *vals, uint32_t n_ub) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  uint32_t id = lb * SL_WG + threadIdx.x;
+  if (id >= n_ub) return;
+  uint32_t key = sl_brick_sentinel(T);
+  if (id < T.C->count) {
+    const int cur = T.cur;
+    f3 g = grid_pos(T, F3(T.px[cur][id], T.py[cur][id], T.pz[cur][id]));
+    if (in_grid(T, g)) {
+      i3 c = toi3(g);
+      key = sl_morton3((uint32_t)c.x >> 2, (uint32_t)c.y >> 2, (uint32_t)c.z >> 2);
+    }
+  }
+  keys[id] = key;
+  vals[id] = id;
+}
+
+__global__ __launch_bounds__(SL_WG) void k_sort_gather(SlWork W, const uint32_t *vals) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  uint32_t d = lb * SL_WG + threadIdx.x;
+  if (d >= T.C->count) return;
+  const int cur = T.cur, oth = 1 - cur;
+  uint32_t s = vals[d];
+  T.px[oth][d] = T.px[cur][s]; T.py[oth][d] = T.py[cur][s]; T.pz[oth][d] = T.pz[cur][s];
+  T.vx[oth][d] = T.vx[cur][s]; T.vy[oth][d] = T.vy[cur][s]; T.vz[oth][d] = T.vz[cur][s];
+  T.pf[oth][d] = T.pf[cur][s];
+}
+
+size_t sl_sort_temp_bytes(uint32_t n, int end_bit) {
+  size_t bytes = 0;
+  (void)rocprim::radix_sort_pairs(nullptr, bytes, (uint32_t *)nullptr, (uint32_t *)nullptr, (uint32_t *)nullptr,
+                                  (uint32_t *)nullptr, n, 0, end_bit, (hipStream_t)0);
+  return bytes;
+}
+
+// keys/vals: 2 x n_ub each; after the call the particle arrays are re-ordered into the other
+// buffer (the caller flips `cur`)
+int sl_k_reorder(const SlLaunch &L, uint32_t n_ub, uint32_t *keys, uint32_t *vals, void *temp, size_t temp_bytes,
+                 int end_bit) {
+  if (!n_ub) return 0;
+  SlLaunch K = L;
+  K.w.total = (n_ub + SL_WG - 1) / SL_WG;
+  hipLaunchKernelGGL(k_sort_keys, dim3(K.w.total), dim3(SL_WG), 0, L.stream, K.w, keys, vals, n_ub);
+  size_t tb = temp_bytes;
+  hipError_t e = rocprim::radix_sort_pairs(temp, tb, keys, keys + n_ub, vals, vals + n_ub, n_ub, 0, end_bit,
+                                           L.stream);
+  if (e != hipSuccess) return (int)e;
+  hipLaunchKernelGGL(k_sort_gather, dim3(K.w.total), dim3(SL_WG), 0, L.stream, K.w, vals + n_ub);
+  return 0;
+}

@@ -36,7 +36,7 @@ struct SlTank {
   int32_t nx, ny, nz, hw;
   uint32_t nodes, cells, hcells;  // hcells = hw*ny*nz
   uint32_t cap, ww_cap, R, M, logM, iters, obj_count, frame, emit_k, emit_total, cur, ww_cur, wmap;
-  int32_t mask_bound, whitewater_on, pad_i[2];
+  int32_t mask_bound, whitewater_on, brick_bits, pad_i;
   // params as the reference uniform (f32)
   float ox, oy, oz, h, invh, hh, dt, g, gdt, flip, ppc, drift, omega, omega1, rc, rim;
   float ww_rate, ww_cap_f, friction, time_f, radius, R_f;
@@ -130,6 +130,21 @@ __device__ __forceinline__ bool in_grid(const SlTank &T, f3 g) {
   return g.x >= 0.0f && g.y >= 0.0f && g.z >= 0.0f && g.x < (float)T.nx && g.y < (float)T.ny &&
          g.z < (float)T.nz;
 }
+
+// Morton code (x in bit 0, y in bit 1, z in bit 2) of brick coordinates (< 1024 each)
+__device__ __forceinline__ uint32_t sl_spread3(uint32_t v) {
+  v &= 0x3ffu;
+  v = (v | (v << 16)) & 0x030000ffu;
+  v = (v | (v << 8)) & 0x0300f00fu;
+  v = (v | (v << 4)) & 0x030c30c3u;
+  v = (v | (v << 2)) & 0x09249249u;
+  return v;
+}
+__device__ __forceinline__ uint32_t sl_morton3(uint32_t x, uint32_t y, uint32_t z) {
+  return sl_spread3(x) | (sl_spread3(y) << 1) | (sl_spread3(z) << 2);
+}
+// key of particles outside the grid: above every brick key
+__device__ __forceinline__ uint32_t sl_brick_sentinel(const SlTank &T) { return 1u << (3 * T.brick_bits); }
 
 // largest t with start[t] <= b
 __device__ __forceinline__ int tank_of(const uint32_t *start, int ntanks, uint32_t b) {
@@ -264,8 +279,13 @@ void sl_k_compact_scan(const SlLaunch &L, int ntanks);
 void sl_k_compact_scatter(const SlLaunch &L);
 void sl_k_ww(const SlLaunch &L, float dt);
 void sl_k_clear_i32(const SlLaunch &L, int which);
+// spatial re-ordering (rocPRIM radix sort of brick-Morton keys)
+size_t sl_sort_temp_bytes(uint32_t n, int end_bit);
+int sl_k_reorder(const SlLaunch &L, uint32_t n_ub, uint32_t *keys, uint32_t *vals, void *temp, size_t temp_bytes,
+                 int end_bit);
 // ripple: rows = M workgroups per tank, cols = M/CW, elems = M*M/256, tex = (R+1)^2/256
 void sl_k_ripple_frame_ex(const SlWork &rows, const SlWork &cols, const SlWork &elems, const SlWork &tex,
                           hipStream_t s, int mask);
 // workgroups of compaction per tank (each compaction workgroup covers SL_COMPACT_ITEMS particles)
 #define SL_COMPACT_ITEMS 1024
+#define SL_P2G_CHUNK 512

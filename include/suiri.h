@@ -159,6 +159,12 @@ int sl_step(sl_sim *sim, double frame_dt, sl_emit_fn fn, void *user);
 /* 0 = adaptive substeps as the reference; n > 0 pins the substep count (fixed-dt configurations) */
 int sl_set_substeps(sl_sim *sim, uint32_t n);
 
+/* Spatial re-ordering of the particles (stable sort by the Morton code of their 4x4x4-cell brick).
+   No counterpart in the reference: its compaction leaves the particle order arbitrary
+   (src/js/31-wgsl-sim2.js:317-328), so any permutation is a valid state. Also run automatically every
+   cfg.reorder_every frames at the end of sl_step_end. */
+int sl_reorder(sl_sim *sim);
+
 /* FluidSim.clearWater() — src/js/50-sim.js:365-373 */
 int sl_clear_water(sl_sim *sim);
 /* FluidSim.saveWater / hasWaterCopy / restoreWater — src/js/50-sim.js:111-141 */

@@ -748,6 +748,41 @@ void or_compact(or_sim *s) {
   s->C.newCount = j;
 }
 
+/* spatial re-ordering: stable counting sort by brick-Morton key */
+static uint32_t spread3(uint32_t v) {
+  uint32_t r = 0;
+  for (int b = 0; b < 10; b++) r |= ((v >> b) & 1u) << (3 * b);
+  return r;
+}
+void or_reorder(or_sim *s) {
+  const uint32_t n = s->C.count;
+  const or_particle *src = s->parts[s->cur];
+  or_particle *dst = s->parts[1 - s->cur];
+  int a = (s->nx + 3) / 4, b = (s->ny + 3) / 4, c = (s->nz + 3) / 4;
+  int mb = a > b ? a : b;
+  mb = mb > c ? mb : c;
+  int bits = 0;
+  while ((1 << bits) < mb) bits++;
+  const uint32_t sentinel = 1u << (3 * bits);
+  uint32_t *key = (uint32_t *)malloc((size_t)(n ? n : 1) * 4);
+  uint32_t *hist = (uint32_t *)calloc((size_t)sentinel + 2, 4);
+  for (uint32_t i = 0; i < n; i++) {
+    v3 g = gridPos(s, pos3(src[i].pos));
+    uint32_t k = sentinel;
+    if (inGrid(s, g)) {
+      v3i q = v2i(g);
+      k = spread3((uint32_t)q.x >> 2) | (spread3((uint32_t)q.y >> 2) << 1) | (spread3((uint32_t)q.z >> 2) << 2);
+    }
+    key[i] = k;
+    hist[k + 1]++;
+  }
+  for (uint32_t k = 1; k <= sentinel + 1; k++) hist[k] += hist[k - 1];
+  for (uint32_t i = 0; i < n; i++) dst[hist[key[i]]++] = src[i];
+  free(key);
+  free(hist);
+  s->cur = 1 - s->cur;
+}
+
 /* COUNT, src/js/31-wgsl-sim2.js:332-349 */
 void or_count(or_sim *s, uint32_t mode, uint32_t y) {
   or_counters *C = &s->C;
@@ -1181,6 +1216,10 @@ void or_frame_tail(or_sim *s, double frame_dt) {
   or_compact(s);
   s->cur = 1 - s->cur;
   or_count(s, 1, 0);
+  if (s->q.reorder_every && ++s->reorder_clock >= s->q.reorder_every) {
+    or_reorder(s);
+    s->reorder_clock = 0;
+  }
   or_ww(s, (float)frame_dt);
   s->ww_cur = 1 - s->ww_cur;
   or_count(s, 3, 0);

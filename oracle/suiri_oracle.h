@@ -60,6 +60,7 @@ typedef struct {
   /* ripple physics: sigma/rho, nu, film damping, high-pass k0, kick density */
   float r_sigma_rho, r_nu, r_film, r_hp_k, r_rho_kick;
   uint32_t wmap;     /* top-height map size used by the ripple mask (256 in the reference) */
+  uint32_t reorder_every; /* spatial re-ordering period in frames (0 = never) */
 } or_quality;
 
 typedef struct or_sim {
@@ -94,6 +95,7 @@ typedef struct or_sim {
   int mask_bound;
   float rU_l[4];       /* last ripple uniform l = (Lx, Lz, dt, meanDepth) */
   int nthreads;
+  uint32_t reorder_clock;
 } or_sim;
 
 /* sizing + allocation, src/js/50-sim.js:29-89 */
@@ -115,6 +117,9 @@ void or_project(or_sim *s);                /* 30-wgsl-sim.js:299-341 */
 void or_g2p(or_sim *s);                    /* 31-wgsl-sim2.js:149-251 */
 void or_emit_kernel(or_sim *s, uint32_t k, uint32_t total); /* 31-wgsl-sim2.js:262-315 */
 void or_compact(or_sim *s);                /* 31-wgsl-sim2.js:317-328 (stable order) */
+/* stable sort of the live particles by the Morton code of their 4x4x4-cell brick (outside the grid
+   last); no counterpart in the reference, whose particle order is arbitrary after compaction */
+void or_reorder(or_sim *s);
 void or_count(or_sim *s, uint32_t mode, uint32_t y); /* 31-wgsl-sim2.js:332-349 */
 void or_ww(or_sim *s, float dt);           /* 31-wgsl-sim2.js:361-409 */
 
