@@ -109,7 +109,19 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ripple", action="store_true")
+    ap.add_argument("--calib-copy-mb", type=int, default=0,
+                    help="only run a plain device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
+    if args.calib_copy_mb:
+        import torch
+        n = args.calib_copy_mb * (1 << 20) // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"calib_copy_bytes_read": n * 4, "calib_copy_bytes_written": n * 4, "launches": 3}))
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
                           "n_gpus": args.gpus}))
@@ -119,6 +131,7 @@ def main():
     g, em = build_sim(wl)
     for _ in range(args.warmup):
         g.step(wl["frame_dt"], em)
+    tuned = g.autotune(reps=2)
     st0 = g.stats()
     g.timing(True)
     g.timing_read(reset=True)
@@ -175,6 +188,7 @@ def main():
         "stage_ms_per_step": {k: v[0] / args.steps for k, v in tm.items()},
         "step_algorithmic_GB": step_bytes(nx, ny, nz, mean_p, iters) / 1e9,
         "gpu_errors": st2["gpuErrors"],
+        "tuned_variants": tuned,
     }
     if not args.no_cpu:
         threads = os.cpu_count() or 1

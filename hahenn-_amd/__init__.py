@@ -74,7 +74,7 @@ EXPORTS = ["sl_config_default", "sl_create", "sl_destroy", "sl_last_error", "sl_
            "sl_clear_water", "sl_save_water", "sl_has_water_copy", "sl_restore_water", "sl_poll_stats",
            "sl_stats", "sl_sync", "sl_set_ripple_mask", "sl_field_bytes", "sl_upload", "sl_readback",
            "sl_set_scalar", "sl_get_scalar", "sl_run_stage", "sl_timing_enable", "sl_timing_read",
-           "sl_version", "sl_build_info", "sl_reorder"]
+           "sl_version", "sl_build_info", "sl_reorder", "sl_autotune"]
 
 _lib = None
 
@@ -107,6 +107,7 @@ def load():
         "sl_set_substeps": (I, [P, U]),
         "sl_clear_water": (I, [P]),
         "sl_reorder": (I, [P]),
+        "sl_autotune": (I, [P, U, ctypes.POINTER(U)]),
         "sl_save_water": (I, [P, ctypes.POINTER(U)]),
         "sl_has_water_copy": (I, [P, U]),
         "sl_restore_water": (I, [P, U]),
@@ -213,6 +214,11 @@ class Sim:
             self._check(self.L.sl_substep(self.ptr, dt.value, ctypes.byref(arr), len(recs), None))
         self._check(self.L.sl_step_end(self.ptr, float(frame_dt)))
         return n.value
+
+    def autotune(self, reps=3):
+        ch = (ctypes.c_uint32 * 2)()
+        self._check(self.L.sl_autotune(self.ptr, int(reps), ch))
+        return {"p2g_variant": ch[0], "g2p_variant": ch[1]}
 
     def reorder(self):
         self._check(self.L.sl_reorder(self.ptr))

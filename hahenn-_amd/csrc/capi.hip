@@ -61,6 +61,8 @@ struct sl_sim {
   size_t sort_temp_bytes = 0;
   int sort_end_bit = 1;
   uint32_t reorder_clock = 0;
+  int p2g_variant = 0;  // kernel variant (chunk size, LDS word size, lane mapping); see k_grid.hip
+  int g2p_variant = 1;  // occupancy target of G2P's register allocation; see k_particles.hip
   double cur_dt = 1.0 / 60.0;
   int mask_bound = 1;
   float objs[SL_MAX_OBJ * 28];
@@ -493,7 +495,7 @@ extern "C" int sl_emit_now(sl_sim *s, const sl_emit *list, uint32_t n, int fill,
     if (fill) {
       // fresh per-cell counts so that the pour only fills cells without water (50-sim.js:253-261)
       sl_k_fill_u32(s->stream, s->ht.cellCount, 0, s->cells);
-      sl_k_p2g(launch_of(s, s->p_ub, SL_P2G_CHUNK), 1);
+      sl_k_p2g(launch_of(s, s->p_ub, 512), 1, 0);
     }
     r = encode_emit(s, k, total);
     if (r) return r;
@@ -539,7 +541,7 @@ static int encode_substep(sl_sim *s) {
   {
     StageScope sc(s, SL_T_P2G, 2);
     sl_k_fill_u32(s->stream, s->ht.cellCount, 0, s->cells);
-    sl_k_p2g(launch_of(s, pb, SL_P2G_CHUNK), 0);
+    sl_k_p2g(launch_of(s, pb, sl_p2g_chunk(s->p2g_variant)), 0, s->p2g_variant);
   }
   { StageScope sc(s, SL_T_NORMFORCE, 1); sl_k_normforce(launch_of(s, s->nodes)); }
   { StageScope sc(s, SL_T_DIVERGENCE, 1); sl_k_divsetup(launch_of(s, s->cells)); }
@@ -549,7 +551,7 @@ static int encode_substep(sl_sim *s) {
     for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor(L, 0, 2 * i); sl_k_sor(L, 1, 2 * i + 1); }
   }
   { StageScope sc(s, SL_T_PROJECT, 1); sl_k_project(launch_of(s, s->nodes)); }
-  { StageScope sc(s, SL_T_G2P, 1); sl_k_g2p(launch_of(s, pb)); }
+  { StageScope sc(s, SL_T_G2P, 1); sl_k_g2p(launch_of(s, pb), s->g2p_variant); }
   return check_launch(s, "substep");
 }
 
@@ -876,11 +878,16 @@ extern "C" int sl_readback(sl_sim *s, int f, void *dst, size_t bytes) {
       return SL_OK;
     }
     case SL_F_COUNTERS: return d2h(s, dst, T.C, 32);
-    case SL_F_ACC: {
-      std::vector<int32_t> a(N * 6);
+    case SL_F_ACC: {  // device: 3 u64 planes (mom << 32 | wt); reference: 6 i32 per node
+      std::vector<uint64_t> a(N * 3);
       if ((r = d2h(s, a.data(), T.acc, N * 24))) return r;
       int32_t *oi = (int32_t *)dst;
-      for (size_t n = 0; n < N; n++) for (int q = 0; q < 6; q++) oi[n * 6 + q] = a[q * N + n];
+      for (size_t n = 0; n < N; n++)
+        for (int c = 0; c < 3; c++) {
+          uint64_t v = a[c * N + n];
+          oi[n * 6 + 2 * c] = (int32_t)(uint32_t)(v >> 32);
+          oi[n * 6 + 2 * c + 1] = (int32_t)(uint32_t)v;
+        }
       return SL_OK;
     }
     case SL_F_CELLCOUNT: return d2h(s, dst, T.cellCount, C * 4);
@@ -969,9 +976,11 @@ extern "C" int sl_upload(sl_sim *s, int f, const void *src, size_t bytes) {
       return SL_OK;
     }
     case SL_F_ACC: {
-      std::vector<int32_t> a(N * 6);
+      std::vector<uint64_t> a(N * 3);
       const int32_t *ii = (const int32_t *)src;
-      for (size_t n = 0; n < N; n++) for (int q = 0; q < 6; q++) a[q * N + n] = ii[n * 6 + q];
+      for (size_t n = 0; n < N; n++)
+        for (int c = 0; c < 3; c++)
+          a[c * N + n] = ((uint64_t)(uint32_t)ii[n * 6 + 2 * c] << 32) + (uint64_t)(uint32_t)ii[n * 6 + 2 * c + 1];
       return h2d(s, T.acc, a.data(), N * 24);
     }
     case SL_F_CELLCOUNT: return h2d(s, T.cellCount, src, C * 4);
@@ -1011,6 +1020,9 @@ extern "C" int sl_set_scalar(sl_sim *s, const char *name, double v) {
   else if (!strcmp(name, "full")) s->full = v != 0;
   else if (!strcmp(name, "p_ub")) s->p_ub = (uint32_t)v;
   else if (!strcmp(name, "geom_dirty")) s->geom_dirty = v != 0;
+  else if (!strcmp(name, "p2g_variant")) s->p2g_variant = (int)v & 7;
+  else if (!strcmp(name, "g2p_variant")) s->g2p_variant = (int)v & 3;
+  else if (!strcmp(name, "reorder_every")) s->cfg.reorder_every = (uint32_t)v;
   else FAIL(s, SL_EARG, "unknown scalar '%s'", name);
   return SL_OK;
 }
@@ -1027,6 +1039,8 @@ extern "C" int sl_get_scalar(const sl_sim *s, const char *name, double *v) {
   else if (!strcmp(name, "impacts")) *v = s->impacts;
   else if (!strcmp(name, "ww_count")) *v = s->ww_count;
   else if (!strcmp(name, "frames")) *v = s->frames;
+  else if (!strcmp(name, "p2g_variant")) *v = s->p2g_variant;
+  else if (!strcmp(name, "g2p_variant")) *v = s->g2p_variant;
   else { ((sl_sim *)s)->err = std::string("unknown scalar '") + name + "'"; return SL_EARG; }
   return SL_OK;
 }
@@ -1044,7 +1058,7 @@ extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
     case SL_S_P2G:
       sl_k_fill_u32(s->stream, (uint32_t *)s->ht.acc, 0, (size_t)s->nodes * 6);
       sl_k_fill_u32(s->stream, s->ht.cellCount, 0, s->cells);
-      sl_k_p2g(launch_of(s, s->p_ub, SL_P2G_CHUNK), 0);
+      sl_k_p2g(launch_of(s, s->p_ub, sl_p2g_chunk(s->p2g_variant)), 0, s->p2g_variant);
       break;
     case SL_S_NORMFORCE: sl_k_normforce(launch_of(s, s->nodes)); break;
     case SL_S_DIVERGENCE: sl_k_divsetup(launch_of(s, s->cells)); break;
@@ -1055,7 +1069,7 @@ extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
       break;
     }
     case SL_S_PROJECT: sl_k_project(launch_of(s, s->nodes)); break;
-    case SL_S_G2P: sl_k_g2p(launch_of(s, s->p_ub)); break;
+    case SL_S_G2P: sl_k_g2p(launch_of(s, s->p_ub), s->g2p_variant); break;
     case SL_S_COMPACT: return encode_compact(s);
     case SL_S_WW:
       sl_k_ww(launch_of(s, s->ww_cap), (float)dt);
@@ -1091,6 +1105,100 @@ extern "C" int sl_timing_read(sl_sim *s, double *ms, uint32_t *launches, int n, 
   }
   if (reset) { memset(s->t_ms, 0, sizeof s->t_ms); memset(s->t_launch, 0, sizeof s->t_launch); }
   return SL_OK;
+}
+
+// ------------------------------------------------------------------------------------------------
+// autotuning of kernel variants on the live state
+// ------------------------------------------------------------------------------------------------
+// Every variant of a kernel computes the same result bit for bit (integer sums, identical
+// arithmetic); only the work decomposition and the register budget differ. The tuner times each
+// variant with HIP events on the simulation's own state between two frames, restores everything the
+// trials touched, and keeps the fastest. It needs a state produced by at least one completed step.
+static float time_region(hipEvent_t a, hipEvent_t b) {
+  float ms = 0;
+  (void)hipEventSynchronize(b);
+  (void)hipEventElapsedTime(&ms, a, b);
+  return ms;
+}
+extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  if (!reps) reps = 3;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  const SlTank &T = s->ht;
+  const int c = s->cur;
+  const size_t P = s->p_ub, C = s->cells;
+  const size_t R1 = s->R ? (size_t)(s->R + 1) * (s->R + 1) : 0;
+  // snapshot of what P2G and G2P write: particles, counters, cell counts, ripple impulses
+  std::vector<void *> snap;
+  bool ok = true;
+  auto dup = [&](const void *src, size_t bytes) -> void * {
+    void *p = nullptr;
+    if (!bytes) return nullptr;
+    if (hipMalloc(&p, bytes) != hipSuccess) { ok = false; return nullptr; }
+    (void)hipMemcpyAsync(p, src, bytes, hipMemcpyDeviceToDevice, s->stream);
+    snap.push_back(p);
+    return p;
+  };
+  void *psrc[7] = {T.px[c], T.py[c], T.pz[c], T.vx[c], T.vy[c], T.vz[c], T.pf[c]};
+  void *pcpy[7];
+  for (int i = 0; i < 7; i++) pcpy[i] = dup(psrc[i], i < 6 ? P * 4 : P);
+  void *ccpy = dup(T.C, sizeof(SlCounters));
+  void *cccpy = dup(T.cellCount, C * 4);
+  void *rcpy = R1 ? dup(T.rsrc, R1 * 4) : nullptr;
+  if (!ok) {
+    (void)hipStreamSynchronize(s->stream);
+    for (void *p : snap) (void)hipFree(p);
+    FAIL(s, SL_ENOMEM, "sl_autotune: no memory for the state snapshot");
+  }
+  auto restore = [&]() {
+    for (int i = 0; i < 7; i++)
+      if (pcpy[i]) (void)hipMemcpyAsync(psrc[i], pcpy[i], i < 6 ? P * 4 : P, hipMemcpyDeviceToDevice, s->stream);
+    if (ccpy) (void)hipMemcpyAsync(T.C, ccpy, sizeof(SlCounters), hipMemcpyDeviceToDevice, s->stream);
+    if (cccpy) (void)hipMemcpyAsync(T.cellCount, cccpy, C * 4, hipMemcpyDeviceToDevice, s->stream);
+    if (rcpy) (void)hipMemcpyAsync(T.rsrc, rcpy, R1 * 4, hipMemcpyDeviceToDevice, s->stream);
+  };
+  hipEvent_t a = ev_get(s), b = ev_get(s);
+  // P2G variants: clear, accumulate, time; the accumulators are cleared again afterwards
+  float best = 1e30f;
+  int bestv = s->p2g_variant;
+  for (int v = 0; v < 8; v++) {
+    float tot = 0;
+    for (uint32_t r = 0; r <= reps; r++) {
+      sl_k_fill_u32(s->stream, (uint32_t *)T.acc, 0, (size_t)s->nodes * 6);
+      sl_k_fill_u32(s->stream, T.cellCount, 0, s->cells);
+      (void)hipEventRecord(a, s->stream);
+      sl_k_p2g(launch_of(s, P, sl_p2g_chunk(v)), 0, v);
+      (void)hipEventRecord(b, s->stream);
+      float ms = time_region(a, b);
+      if (r) tot += ms;  // the first run warms the caches
+    }
+    if (tot < best) { best = tot; bestv = v; }
+  }
+  s->p2g_variant = bestv;
+  sl_k_fill_u32(s->stream, (uint32_t *)T.acc, 0, (size_t)s->nodes * 6);
+  // G2P variants: restore the particles before every run
+  best = 1e30f;
+  int bestg = s->g2p_variant;
+  for (int v = 0; v < 4; v++) {
+    float tot = 0;
+    for (uint32_t r = 0; r <= reps; r++) {
+      restore();
+      (void)hipEventRecord(a, s->stream);
+      sl_k_g2p(launch_of(s, P), v);
+      (void)hipEventRecord(b, s->stream);
+      float ms = time_region(a, b);
+      if (r) tot += ms;
+    }
+    if (tot < best) { best = tot; bestg = v; }
+  }
+  s->g2p_variant = bestg;
+  restore();
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  for (void *p : snap) (void)hipFree(p);
+  s->ev_pool.push_back(a);
+  s->ev_pool.push_back(b);
+  if (choice) { choice[0] = (uint32_t)s->p2g_variant; choice[1] = (uint32_t)s->g2p_variant; }
+  return check_launch(s, "autotune");
 }
 
 extern "C" int sl_version(void) { return SL_ABI_VERSION; }

@@ -80,12 +80,18 @@ void sl_k_geom(const SlLaunch &L) {
 // 30-wgsl-sim.js:159-200. Integer atomics make the sums independent of the particle order.
 // ------------------------------------------------------------------------------------------------
 #define VSCALE 262144.0f
-__device__ __forceinline__ void splat(const SlTank &T, f3 q, float val, int comp) {
+// One accumulator per (node, component): a u64 holding the momentum sum in the high word and the
+// weight sum in the low word. Weight terms are non-negative, so the low word never carries into the
+// high one (until 2^32, far past the i32 range the reference itself keeps); the high word then wraps
+// exactly like the reference's i32 momentum sum. One 64-bit atomic replaces the reference's two.
+__device__ __forceinline__ unsigned long long pack_mw(int32_t mom, int32_t wt) {
+  return ((unsigned long long)(uint32_t)mom << 32) + (unsigned long long)(uint32_t)wt;
+}
+template <bool LDS>
+__device__ __forceinline__ void splat(unsigned long long *acc, const SlTank &T, int32_t i0, int32_t j0, int32_t k0,
+                                      int32_t ni, int32_t nj, f3 q, float val) {
   int32_t bx = f2i(floorf(q.x)), by = f2i(floorf(q.y)), bz = f2i(floorf(q.z));
   float fx = q.x - (float)bx, fy = q.y - (float)by, fz = q.z - (float)bz;
-  const size_t N = T.nodes;
-  int32_t *mom = T.acc + (size_t)(2 * comp) * N;
-  int32_t *wt = T.acc + (size_t)(2 * comp + 1) * N;
 #pragma unroll
   for (int dz = 0; dz < 2; dz++) {
     int32_t k = bz + dz;
@@ -102,69 +108,77 @@ __device__ __forceinline__ void splat(const SlTank &T, f3 q, float val, int comp
         if (i < 0 || i > T.nx) continue;
         float w = (dx == 1 ? fx : 1.0f - fx) * wy * wz;
         if (w <= 0.0f) continue;
-        uint32_t n = node_idx(T, i, j, k);
-        atomicAdd(&mom[n], f2i(rintf(w * val * VSCALE)));
-        atomicAdd(&wt[n], f2i(rintf(w * VSCALE)));
+        unsigned long long v = pack_mw(f2i(rintf(w * val * VSCALE)), f2i(rintf(w * VSCALE)));
+        uint32_t e = LDS ? (uint32_t)(((k - k0) * nj + (j - j0)) * ni + (i - i0)) : node_idx(T, i, j, k);
+        atomicAdd(&acc[e], v);
       }
     }
   }
 }
-// Workgroup-privatised P2G: a workgroup takes a chunk of P2G_CHUNK consecutive particles, finds the
-// box of MAC nodes they touch, accumulates all their fixed-point contributions (and cell counts) in
-// LDS with LDS atomics, then adds the box to the global accumulators with one global atomic per
-// non-zero entry. Integer sums are associative, so the result is identical to per-particle global
-// atomics. Chunks whose box does not fit in LDS (scattered particles) use global atomics directly.
-#define P2G_CHUNK SL_P2G_CHUNK
-#define P2G_PER_THREAD (P2G_CHUNK / SL_WG)
-#define P2G_LDS_INTS 11264  // 44 KiB
-
-__device__ __forceinline__ void splat_lds(int32_t *box, int32_t vol, int32_t i0, int32_t j0, int32_t k0, int32_t ni,
-                                          int32_t nj, int32_t imax, int32_t jmax, int32_t kmax, f3 q, float val,
-                                          int comp) {
+// Workgroup-privatised P2G (30-wgsl-sim.js:159-200): a workgroup takes a chunk of CHUNK consecutive
+// particles, finds the box of MAC nodes they touch, accumulates all their fixed-point contributions
+// (and cell counts) in LDS with LDS atomics, then adds the box to the global accumulators with one
+// global 64-bit atomic per non-zero (node, component). Integer sums are associative, so the result
+// equals per-particle global atomics. Chunks whose box does not fit in LDS (scattered particles) use
+// global atomics directly. Variants (chosen by measurement per workload, see sl_p2g_variant):
+//   CHUNK     particles per workgroup (512, 1024, 2048)
+//   U64       LDS accumulates packed u64 (one LDS atomic per node) or two i32 words (momentum, weight)
+//   STRIDED   lane t takes particles base + t*PER + q (spread over the chunk) instead of base + q*256 + t
+template <bool LDS64>
+__device__ __forceinline__ void splat_box(void *boxp, int32_t vol, int32_t i0, int32_t j0, int32_t k0, int32_t ni,
+                                          int32_t nj, const SlTank &T, f3 q, float val, int comp) {
   int32_t bx = f2i(floorf(q.x)), by = f2i(floorf(q.y)), bz = f2i(floorf(q.z));
   float fx = q.x - (float)bx, fy = q.y - (float)by, fz = q.z - (float)bz;
-  int32_t *mom = box + (2 * comp) * vol, *wt = box + (2 * comp + 1) * vol;
 #pragma unroll
   for (int dz = 0; dz < 2; dz++) {
     int32_t k = bz + dz;
-    if (k < 0 || k > kmax) continue;
+    if (k < 0 || k > T.nz) continue;
     float wz = dz == 1 ? fz : 1.0f - fz;
 #pragma unroll
     for (int dy = 0; dy < 2; dy++) {
       int32_t j = by + dy;
-      if (j < 0 || j > jmax) continue;
+      if (j < 0 || j > T.ny) continue;
       float wy = dy == 1 ? fy : 1.0f - fy;
 #pragma unroll
       for (int dx = 0; dx < 2; dx++) {
         int32_t i = bx + dx;
-        if (i < 0 || i > imax) continue;
+        if (i < 0 || i > T.nx) continue;
         float w = (dx == 1 ? fx : 1.0f - fx) * wy * wz;
         if (w <= 0.0f) continue;
+        int32_t mo = f2i(rintf(w * val * VSCALE)), wi = f2i(rintf(w * VSCALE));
         int32_t e = ((k - k0) * nj + (j - j0)) * ni + (i - i0);
-        atomicAdd(&mom[e], f2i(rintf(w * val * VSCALE)));
-        atomicAdd(&wt[e], f2i(rintf(w * VSCALE)));
+        if (LDS64) {
+          atomicAdd(&((unsigned long long *)boxp)[comp * vol + e], pack_mw(mo, wi));
+        } else {
+          atomicAdd(&((int32_t *)boxp)[(2 * comp) * vol + e], mo);
+          atomicAdd(&((int32_t *)boxp)[(2 * comp + 1) * vol + e], wi);
+        }
       }
     }
   }
 }
 
-template <int COUNT_ONLY>
+#define P2G_LDS_BYTES (60 * 1024)
+
+template <int COUNT_ONLY, int CHUNK, bool LDS64, bool STRIDED>
 __global__ __launch_bounds__(SL_WG) void k_p2g(SlWork W) {
+  constexpr int PER = CHUNK / SL_WG;
+  constexpr int LDS_BYTES = CHUNK <= 512 ? 44 * 1024 : P2G_LDS_BYTES;
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
-  __shared__ int32_t box[P2G_LDS_INTS];
+  __shared__ unsigned long long box[LDS_BYTES / 8];
   __shared__ int32_t bb[6];
   const uint32_t cnt = T.C->count;
-  const uint32_t base = lb * P2G_CHUNK;
+  const uint32_t base = lb * CHUNK;
   if (base >= cnt) return;
   const int cur = T.cur;
   const int32_t nx = T.nx, ny = T.ny, nz = T.nz;
-  f3 g[P2G_PER_THREAD];
-  bool ok[P2G_PER_THREAD];
+  f3 g[PER];
+  bool ok[PER];
   int32_t mn[3] = {INT32_MAX, INT32_MAX, INT32_MAX}, mx[3] = {INT32_MIN, INT32_MIN, INT32_MIN};
 #pragma unroll
-  for (int q = 0; q < P2G_PER_THREAD; q++) {
-    uint32_t id = base + q * SL_WG + threadIdx.x;
+  for (int q = 0; q < PER; q++) {
+    uint32_t id = STRIDED ? base + threadIdx.x * PER + q : base + q * SL_WG + threadIdx.x;
     ok[q] = false;
     if (id < cnt && (T.pf[cur][id] & 1u)) {
       g[q] = grid_pos(T, F3(T.px[cur][id], T.py[cur][id], T.pz[cur][id]));
@@ -193,23 +207,24 @@ __global__ __launch_bounds__(SL_WG) void k_p2g(SlWork W) {
   const int32_t i0 = max(bb[0] - 1, 0), j0 = max(bb[1] - 1, 0), k0 = max(bb[2] - 1, 0);
   const int32_t i1 = min(bb[3] + 1, nx), j1 = min(bb[4] + 1, ny), k1 = min(bb[5] + 1, nz);
   const int32_t ni = i1 - i0 + 1, nj = j1 - j0 + 1, nk = k1 - k0 + 1;
-  const int32_t vol = ni * nj * nk;
+  const int64_t vol = (int64_t)ni * nj * nk;
   const int32_t ci = bb[3] - bb[0] + 1, cj = bb[4] - bb[1] + 1, ck = bb[5] - bb[2] + 1;
-  const int32_t cvol = ci * cj * ck;
-  const int32_t need = (COUNT_ONLY ? 0 : 6 * vol) + cvol;
-  const bool use_lds = (int64_t)need <= P2G_LDS_INTS;
+  const int64_t cvol = (int64_t)ci * cj * ck;
+  const int64_t need = (COUNT_ONLY ? 0 : 3 * vol * 8) + cvol * 4;
+  const bool use_lds = need <= LDS_BYTES;
   const size_t N = T.nodes;
-  int32_t *cbox = box + (COUNT_ONLY ? 0 : 6 * vol);
+  unsigned long long *gacc = (unsigned long long *)T.acc;
+  uint32_t *cbox = (uint32_t *)box + (COUNT_ONLY ? 0 : 6 * vol);
   if (use_lds) {
-    for (int32_t e = threadIdx.x; e < need; e += SL_WG) box[e] = 0;
+    for (int32_t e = threadIdx.x; e < (int32_t)(need / 4); e += SL_WG) ((uint32_t *)box)[e] = 0;
     __syncthreads();
   }
 #pragma unroll
-  for (int q = 0; q < P2G_PER_THREAD; q++) {
+  for (int q = 0; q < PER; q++) {
     if (!ok[q]) continue;
-    uint32_t id = base + q * SL_WG + threadIdx.x;
+    uint32_t id = STRIDED ? base + threadIdx.x * PER + q : base + q * SL_WG + threadIdx.x;
     i3 c = toi3(g[q]);
-    if (use_lds) atomicAdd(&cbox[((c.z - bb[2]) * cj + (c.y - bb[1])) * ci + (c.x - bb[0])], 1);
+    if (use_lds) atomicAdd(&cbox[((c.z - bb[2]) * cj + (c.y - bb[1])) * ci + (c.x - bb[0])], 1u);
     else atomicAdd(&T.cellCount[cell_idx(T, c.x, c.y, c.z)], 1u);
     if (COUNT_ONLY) continue;
     float vx = clampf(T.vx[cur][id], -30.0f, 30.0f);
@@ -217,37 +232,54 @@ __global__ __launch_bounds__(SL_WG) void k_p2g(SlWork W) {
     float vz = clampf(T.vz[cur][id], -30.0f, 30.0f);
     f3 qx = sub3(g[q], F3(0.0f, 0.5f, 0.5f)), qy = sub3(g[q], F3(0.5f, 0.0f, 0.5f)), qz = sub3(g[q], F3(0.5f, 0.5f, 0.0f));
     if (use_lds) {
-      splat_lds(box, vol, i0, j0, k0, ni, nj, nx, ny, nz, qx, vx, 0);
-      splat_lds(box, vol, i0, j0, k0, ni, nj, nx, ny, nz, qy, vy, 1);
-      splat_lds(box, vol, i0, j0, k0, ni, nj, nx, ny, nz, qz, vz, 2);
+      splat_box<LDS64>(box, (int32_t)vol, i0, j0, k0, ni, nj, T, qx, vx, 0);
+      splat_box<LDS64>(box, (int32_t)vol, i0, j0, k0, ni, nj, T, qy, vy, 1);
+      splat_box<LDS64>(box, (int32_t)vol, i0, j0, k0, ni, nj, T, qz, vz, 2);
     } else {
-      splat(T, qx, vx, 0);
-      splat(T, qy, vy, 1);
-      splat(T, qz, vz, 2);
+      splat<false>(gacc, T, 0, 0, 0, 0, 0, qx, vx);
+      splat<false>(gacc + N, T, 0, 0, 0, 0, 0, qy, vy);
+      splat<false>(gacc + 2 * N, T, 0, 0, 0, 0, 0, qz, vz);
     }
   }
   if (!use_lds) return;
   __syncthreads();
   if (!COUNT_ONLY) {
-    for (int32_t e = threadIdx.x; e < 6 * vol; e += SL_WG) {
-      int32_t v = box[e];
-      if (v == 0) continue;
-      int32_t comp2 = e / vol, l = e - comp2 * vol;
+    for (int32_t e = threadIdx.x; e < (int32_t)(3 * vol); e += SL_WG) {
+      int32_t comp = e / (int32_t)vol, l = e - comp * (int32_t)vol;
+      unsigned long long v;
+      if (LDS64) v = box[e];
+      else v = pack_mw(((int32_t *)box)[(2 * comp) * vol + l], ((int32_t *)box)[(2 * comp + 1) * vol + l]);
+      if (v == 0ull) continue;
       int32_t kk = l / (ni * nj), r = l - kk * ni * nj, jj = r / ni, ii = r - jj * ni;
-      atomicAdd(&T.acc[(size_t)comp2 * N + node_idx(T, i0 + ii, j0 + jj, k0 + kk)], v);
+      atomicAdd(&gacc[(size_t)comp * N + node_idx(T, i0 + ii, j0 + jj, k0 + kk)], v);
     }
   }
-  for (int32_t e = threadIdx.x; e < cvol; e += SL_WG) {
-    int32_t v = cbox[e];
+  for (int32_t e = threadIdx.x; e < (int32_t)cvol; e += SL_WG) {
+    uint32_t v = cbox[e];
     if (v == 0) continue;
     int32_t kk = e / (ci * cj), r = e - kk * ci * cj, jj = r / ci, ii = r - jj * ci;
-    atomicAdd(&T.cellCount[cell_idx(T, bb[0] + ii, bb[1] + jj, bb[2] + kk)], (uint32_t)v);
+    atomicAdd(&T.cellCount[cell_idx(T, bb[0] + ii, bb[1] + jj, bb[2] + kk)], v);
   }
 }
-void sl_k_p2g(const SlLaunch &L, int count_only) {
+
+// variant table: id -> (chunk, u64 LDS, strided)
+static const int kP2gChunk[8] = {512, 512, 1024, 1024, 2048, 2048, 1024, 512};
+int sl_p2g_chunk(int variant) { return kP2gChunk[variant & 7]; }
+void sl_k_p2g(const SlLaunch &L, int count_only, int variant) {
   if (!L.w.total) return;
-  if (count_only) hipLaunchKernelGGL(k_p2g<1>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
-  else hipLaunchKernelGGL(k_p2g<0>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+#define P2G_LAUNCH(C, CH, U, S) hipLaunchKernelGGL((k_p2g<C, CH, U, S>), dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w)
+  if (count_only) { P2G_LAUNCH(1, 512, false, false); return; }
+  switch (variant & 7) {
+    case 0: P2G_LAUNCH(0, 512, false, false); break;
+    case 1: P2G_LAUNCH(0, 512, true, false); break;
+    case 2: P2G_LAUNCH(0, 1024, false, false); break;
+    case 3: P2G_LAUNCH(0, 1024, true, false); break;
+    case 4: P2G_LAUNCH(0, 2048, false, false); break;
+    case 5: P2G_LAUNCH(0, 2048, true, false); break;
+    case 6: P2G_LAUNCH(0, 1024, false, true); break;
+    default: P2G_LAUNCH(0, 512, false, true); break;
+  }
+#undef P2G_LAUNCH
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -262,14 +294,15 @@ __global__ __launch_bounds__(SL_WG) void k_normforce(SlWork W) {
   const size_t N = T.nodes;
   float v[3] = {0.0f, 0.0f, 0.0f};
   uint32_t m = 0;
+  unsigned long long *acc = (unsigned long long *)T.acc;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
-    int32_t mo = T.acc[(2 * c) * N + n], wi = T.acc[(2 * c + 1) * N + n];
+    unsigned long long a = acc[c * N + n];
+    int32_t mo = (int32_t)(uint32_t)(a >> 32), wi = (int32_t)(uint32_t)a;
     float wt = (float)wi;
     if (wt > 1.0f) { v[c] = (float)mo / wt; m |= 1u << c; }
     T.velOld[c * N + n] = v[c];
-    T.acc[(2 * c) * N + n] = 0;
-    T.acc[(2 * c + 1) * N + n] = 0;
+    acc[c * N + n] = 0ull;
   }
   T.vmask[n] = (uint8_t)m;
   v[1] -= T.gdt;

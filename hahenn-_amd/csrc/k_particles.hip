@@ -125,10 +125,10 @@ __device__ void splash_crown(const SlTank &T, f3 x, float U, float m, uint32_t i
   if (T.ww_rate <= 0.0f) return;
   const float PI998 = (float)(3.14159265 * 998.0);
   const float RS = (float)(998.0 * 0.0728);
-  float D = powf(6.0f * m / PI998, (float)(1.0 / 3.0));
+  float D = sl_pow(6.0f * m / PI998, (float)(1.0 / 3.0));
   float We = 998.0f * U * U * D / 0.0728f;
   float Oh = 0.001f / sqrtf(RS * D);
-  float K = We * powf(Oh, -0.4f);
+  float K = We * sl_pow(Oh, -0.4f);
   uint32_t n = f2u(clampf(rintf(4.0f * (K / 2100.0f - 1.0f)), 0.0f, 24.0f));
   uint32_t nj = K > 700.0f ? (K > 2100.0f ? 2u : 1u) : 0u;
   if (n + nj == 0u) return;
@@ -143,10 +143,10 @@ __device__ void splash_crown(const SlTank &T, f3 x, float U, float m, uint32_t i
     float ph = ph0 + 6.2831853f * ((float)i + 0.3f * rndu(s)) / (float)n;
     float el = 0.9f + 0.45f * rndu(s);
     float sp = U * (0.15f + 0.2f * rndu(s));
-    f3 d = F3(cosf(ph), 0.0f, sinf(ph));
+    f3 d = F3(sl_cos(ph), 0.0f, sl_sin(ph));
     f3 p = add3(x, mul3(d, r0));
     float life = 0.5f + 0.4f * rndu(s);
-    f3 v = add3(mul3(mul3(d, cosf(el)), sp), F3(0.0f, sinf(el) * sp, 0.0f));
+    f3 v = add3(mul3(mul3(d, sl_cos(el)), sp), F3(0.0f, sl_sin(el) * sp, 0.0f));
     ww_store(T, k, p, life, v, 0.0f);
   }
   for (uint32_t j = 0; j < nj; j++) {
@@ -227,7 +227,7 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
         v = vl;
         airborne = false;
         atomicAdd(&T.C->impacts, 1u);
-        float m = 998.0f * powf(0.5f * h, 3.0f);
+        float m = 998.0f * sl_pow(0.5f * h, 3.0f);
         if (T.R > 0u) deposit_ripple(T, x.x, x.z, 2.0f * m * fminf(dv, 8.0f));
         float level = T.oy + ((float)c1.y + fminf((float)count_at(T, c1.x, c1.y, c1.z) / T.ppc, 1.0f)) * h;
         splash_crown(T, F3(x.x, level + 0.25f * h, x.z), -v0.y, m, id);
@@ -259,8 +259,10 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
   return vm;
 }
 // vmax: atomicMax on the bit pattern of a non-negative float (31-wgsl-sim2.js:233), reduced over
-// the workgroup first so that one atomic per workgroup reaches the counter
-__global__ __launch_bounds__(SL_WG) void k_g2p(SlWork W) {
+// the workgroup first so that one atomic per workgroup reaches the counter. MINW is the occupancy
+// target handed to the register allocator (waves per SIMD); the variants compute identical results.
+template <int MINW>
+__global__ __launch_bounds__(SL_WG, MINW) void k_g2p(SlWork W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
   uint32_t id = lb * SL_WG + threadIdx.x;
@@ -275,9 +277,14 @@ __global__ __launch_bounds__(SL_WG) void k_g2p(SlWork W) {
     if (m) atomicMax(&T.C->vmax, m);
   }
 }
-void sl_k_g2p(const SlLaunch &L) {
+void sl_k_g2p(const SlLaunch &L, int variant) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_g2p, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  switch (variant & 3) {
+    case 0: hipLaunchKernelGGL(k_g2p<1>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w); break;
+    case 1: hipLaunchKernelGGL(k_g2p<5>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w); break;
+    case 2: hipLaunchKernelGGL(k_g2p<6>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w); break;
+    default: hipLaunchKernelGGL(k_g2p<8>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w); break;
+  }
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -303,15 +310,15 @@ __global__ __launch_bounds__(SL_WG) void k_emit(SlWork W) {
     float rr = em.a[3] * sqrtf(rndu(s));
     float th = 6.2831853f * rndu(s);
     f3 u = F3(em.c[0], em.c[1], em.c[2]), w = F3(em.d[0], em.d[1], em.d[2]), dir = F3(em.b[0], em.b[1], em.b[2]);
-    f3 ring = mul3(add3(mul3(u, cosf(th)), mul3(w, sinf(th))), rr);
+    f3 ring = mul3(add3(mul3(u, sl_cos(th)), mul3(w, sl_sin(th))), rr);
     float r3 = rndu(s);
     pos = add3(add3(F3(em.a[0], em.a[1], em.a[2]), ring), mul3(dir, r3 * em.d[3]));
     f3 dv = dir;
     if (em.c[3] > 0.0f) {
-      float ca = 1.0f - rndu(s) * (1.0f - cosf(em.c[3]));
+      float ca = 1.0f - rndu(s) * (1.0f - sl_cos(em.c[3]));
       float sa = sqrtf(fmaxf(1.0f - ca * ca, 0.0f));
       float ph = 6.2831853f * rndu(s);
-      dv = add3(mul3(dir, ca), mul3(add3(mul3(u, cosf(ph)), mul3(w, sinf(ph))), sa));
+      dv = add3(mul3(dir, ca), mul3(add3(mul3(u, sl_cos(ph)), mul3(w, sl_sin(ph))), sa));
     }
     float j0 = rndu(s), j1 = rndu(s), j2 = rndu(s);
     vel = add3(mul3(dv, em.b[3]), mul3(sub3(F3(j0, j1, j2), F3(0.5f, 0.5f, 0.5f)), 0.02f * em.b[3]));
@@ -321,7 +328,7 @@ __global__ __launch_bounds__(SL_WG) void k_emit(SlWork W) {
     dd = add3(dd, F3(1e-4f, 1e-4f, 1e-4f));
     float l = len3(dd);
     dd = F3(dd.x / l, dd.y / l, dd.z / l);
-    float r = powf(rndu(s), 0.3333f);
+    float r = sl_pow(rndu(s), 0.3333f);
     pos = add3(F3(em.a[0], em.a[1], em.a[2]), mul3(mul3(dd, em.a[3]), r));
     vel = F3(em.b[0], em.b[1], em.b[2]);
     mark = em.b[3] > 0.5f;

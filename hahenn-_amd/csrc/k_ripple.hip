@@ -121,23 +121,23 @@ __global__ __launch_bounds__(256) void k_revolve(SlWork W) {
   float kk = sqrtf(kx * kx + kz * kz);
   float D = fmaxf(T.rmd, 0.002f);
   float kd = fminf(kk * D, 15.0f);
-  float th = 1.0f - 2.0f / (expf(2.0f * kd) + 1.0f);
+  float th = 1.0f - 2.0f / (sl_exp(2.0f * kd) + 1.0f);
   float w2 = (T.g * kk + T.rsr * kk * kk * kk) * th;
   float w = sqrtf(w2);
   float gam = 2.0f * T.rnu * kk * kk + T.rfilm * kk * sqrtf(T.rnu * w * 0.125f) +
-              kk * sqrtf(T.rnu * w * 0.5f) / fmaxf(sinhf(fminf(2.0f * kd, 30.0f)), 1e-4f);
+              kk * sqrtf(T.rnu * w * 0.5f) / fmaxf(sl_sinh(fminf(2.0f * kd, 30.0f)), 1e-4f);
   float wd = sqrtf(fmaxf(w2 - gam * gam, 1e-6f));
   float2 s = T.rstate[id];
   float sx = s.x, sy = s.y;
   if (T.rdt > 0.0f) {
-    float c = cosf(wd * T.rdt), sn = sinf(wd * T.rdt), e = expf(-gam * T.rdt);
+    float c = sl_cos(wd * T.rdt), sn = sl_sin(wd * T.rdt), e = sl_exp(-gam * T.rdt);
     float nx = sx * c + (sy + gam * sx) / wd * sn;
     float ny = sy * c - (w2 * sx + gam * sy) / wd * sn;
     sx = e * nx; sy = e * ny;
   }
-  float hp = 1.0f - expf(-(kk * kk) / (T.rhpk * T.rhpk));
+  float hp = 1.0f - sl_exp(-(kk * kk) / (T.rhpk * T.rhpk));
   float sw = fmaxf(1.2f * T.rLx / (float)R, 0.003f);
-  float spread = expf(-0.5f * kk * kk * sw * sw);
+  float spread = sl_exp(-0.5f * kk * kk * sw * sw);
   sy += -(kk * th / T.rrho) * T.rhat[id].x * hp * spread;
   T.rstate[id] = make_float2(sx, sy);
 }

@@ -1,0 +1,527 @@
+// Node N-API binding of the suiri C ABI (include/suiri.h), so that a JavaScript host keeps the
+// reference's FluidSim interface (src/js/50-sim.js) with the HIP library underneath.
+//
+// Every function runs on the JS thread and calls the C ABI synchronously; the C ABI itself only
+// enqueues GPU work (like GPUQueue.submit), so step() returns without waiting for the device.
+// Status codes other than SL_OK become JS exceptions carrying sl_last_error().
+#include <node_api.h>
+#include <stdint.h>
+#include <string.h>
+#include <string>
+#include <vector>
+
+#include "../../include/suiri.h"
+
+#define NAPI_CALL(env, call)                                             \
+  do {                                                                   \
+    if ((call) != napi_ok) {                                             \
+      napi_throw_error((env), nullptr, "N-API call failed: " #call);     \
+      return nullptr;                                                    \
+    }                                                                    \
+  } while (0)
+
+static napi_value throw_sl(napi_env env, const sl_sim *s, int code) {
+  std::string m = "suiri error " + std::to_string(code) + ": " + (sl_last_error(s) ? sl_last_error(s) : "");
+  napi_throw_error(env, nullptr, m.c_str());
+  return nullptr;
+}
+
+static napi_value num(napi_env env, double v) {
+  napi_value r;
+  napi_create_double(env, v, &r);
+  return r;
+}
+static napi_value boolean(napi_env env, bool v) {
+  napi_value r;
+  napi_get_boolean(env, v, &r);
+  return r;
+}
+static double get_num(napi_env env, napi_value obj, const char *key, double dflt, bool *found = nullptr) {
+  bool has = false;
+  napi_has_named_property(env, obj, key, &has);
+  if (found) *found = has;
+  if (!has) return dflt;
+  napi_value v;
+  napi_get_named_property(env, obj, key, &v);
+  napi_valuetype t;
+  napi_typeof(env, v, &t);
+  if (t != napi_number) return dflt;
+  double d;
+  napi_get_value_double(env, v, &d);
+  return d;
+}
+static void set_num(napi_env env, napi_value obj, const char *key, double v) {
+  napi_set_named_property(env, obj, key, num(env, v));
+}
+
+struct Handle {
+  sl_sim *sim = nullptr;
+};
+static void finalize(napi_env, void *data, void *) {
+  Handle *h = (Handle *)data;
+  if (h->sim) sl_destroy(h->sim);
+  delete h;
+}
+static sl_sim *get_sim(napi_env env, napi_value v) {
+  Handle *h = nullptr;
+  if (napi_get_value_external(env, v, (void **)&h) != napi_ok || !h || !h->sim) {
+    napi_throw_error(env, nullptr, "suiri: invalid or destroyed simulation handle");
+    return nullptr;
+  }
+  return h->sim;
+}
+static bool get_args(napi_env env, napi_callback_info info, size_t want, napi_value *argv) {
+  size_t argc = want;
+  napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr);
+  if (argc < want) {
+    napi_throw_type_error(env, nullptr, "suiri: missing arguments");
+    return false;
+  }
+  return true;
+}
+
+// ---- config ----
+#define CFG_FIELDS(X)                                                                           \
+  X(tank_w) X(tank_h) X(tank_d) X(h) X(cap) X(ww_cap) X(iters) X(ripple_R) X(rc_factor) X(gravity) \
+  X(flip) X(ppc) X(drift) X(omega) X(friction) X(ww_rate) X(r_sigma_rho) X(r_nu) X(r_film)         \
+  X(r_hp_k) X(r_rho_kick) X(wmap) X(whitewater) X(device) X(reorder_every)
+
+static napi_value config_to_js(napi_env env, const sl_config &c) {
+  napi_value o;
+  napi_create_object(env, &o);
+#define X(f) set_num(env, o, #f, (double)c.f);
+  CFG_FIELDS(X)
+#undef X
+  return o;
+}
+static void config_from_js(napi_env env, napi_value o, sl_config &c) {
+#define X(f) c.f = (decltype(c.f))get_num(env, o, #f, (double)c.f);
+  CFG_FIELDS(X)
+#undef X
+}
+
+// configDefault(quality) -> object
+static napi_value ConfigDefault(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  size_t argc = 1;
+  napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr);
+  char q[32] = "standard";
+  if (argc >= 1) {
+    size_t len = 0;
+    napi_get_value_string_utf8(env, argv[0], q, sizeof q, &len);
+  }
+  sl_config c;
+  int r = sl_config_default(&c, q);
+  if (r) return throw_sl(env, nullptr, r);
+  return config_to_js(env, c);
+}
+
+// create(config) -> handle
+static napi_value Create(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  sl_config c;
+  sl_config_default(&c, "standard");
+  config_from_js(env, argv[0], c);
+  sl_sim *s = nullptr;
+  int r = sl_create(&c, &s);
+  if (r) return throw_sl(env, nullptr, r);
+  Handle *h = new Handle();
+  h->sim = s;
+  napi_value ext;
+  NAPI_CALL(env, napi_create_external(env, h, finalize, nullptr, &ext));
+  return ext;
+}
+
+static napi_value Destroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  Handle *h = nullptr;
+  if (napi_get_value_external(env, argv[0], (void **)&h) == napi_ok && h && h->sim) {
+    sl_destroy(h->sim);
+    h->sim = nullptr;
+  }
+  return nullptr;
+}
+
+static napi_value GridInfo(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  sl_grid_t g;
+  int r = sl_grid_info(s, &g);
+  if (r) return throw_sl(env, s, r);
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "nx", g.nx); set_num(env, o, "ny", g.ny); set_num(env, o, "nz", g.nz);
+  set_num(env, o, "nodes", g.nodes); set_num(env, o, "cells", g.cells); set_num(env, o, "cap", g.cap);
+  set_num(env, o, "wwCap", g.ww_cap); set_num(env, o, "R", g.R); set_num(env, o, "M", g.M);
+  set_num(env, o, "iters", g.iters); set_num(env, o, "h", g.h); set_num(env, o, "rim", g.rim);
+  set_num(env, o, "vp", g.vp); set_num(env, o, "radius", g.radius);
+  napi_value org;
+  napi_create_array_with_length(env, 3, &org);
+  for (uint32_t i = 0; i < 3; i++) napi_set_element(env, org, i, num(env, g.origin[i]));
+  napi_set_named_property(env, o, "origin", org);
+  return o;
+}
+
+// bytes of a typed array / array buffer / buffer
+static bool get_bytes(napi_env env, napi_value v, void **data, size_t *len) {
+  bool is = false;
+  napi_is_typedarray(env, v, &is);
+  if (is) {
+    napi_typedarray_type t;
+    size_t n, off;
+    napi_value ab;
+    if (napi_get_typedarray_info(env, v, &t, &n, data, &ab, &off) != napi_ok) return false;
+    size_t es = 1;
+    switch (t) {
+      case napi_int16_array: case napi_uint16_array: es = 2; break;
+      case napi_int32_array: case napi_uint32_array: case napi_float32_array: es = 4; break;
+      case napi_float64_array: es = 8; break;
+      default: es = 1;
+    }
+    *len = n * es;
+    return true;
+  }
+  napi_is_arraybuffer(env, v, &is);
+  if (is) return napi_get_arraybuffer_info(env, v, data, len) == napi_ok;
+  napi_is_buffer(env, v, &is);
+  if (is) return napi_get_buffer_info(env, v, data, len) == napi_ok;
+  return false;
+}
+
+// setObjects(h, Float32Array rows) -> changed
+static napi_value SetObjects(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  void *data = nullptr;
+  size_t len = 0;
+  if (!get_bytes(env, argv[1], &data, &len)) {
+    napi_throw_type_error(env, nullptr, "setObjects: rows must be a Float32Array");
+    return nullptr;
+  }
+  int changed = 0;
+  int r = sl_set_objects(s, (const float *)data, (uint32_t)(len / (28 * 4)), &changed);
+  if (r) return throw_sl(env, s, r);
+  return boolean(env, changed != 0);
+}
+
+// emission records: [{shape, a:[4], b:[4], c:[4]?, d:[4]?, count, seed}]
+static bool read_vec4(napi_env env, napi_value rec, const char *key, float out[4]) {
+  out[0] = out[1] = out[2] = out[3] = 0.0f;
+  bool has = false;
+  napi_has_named_property(env, rec, key, &has);
+  if (!has) return true;
+  napi_value arr;
+  napi_get_named_property(env, rec, key, &arr);
+  napi_valuetype t;
+  napi_typeof(env, arr, &t);
+  if (t == napi_undefined || t == napi_null) return true;
+  bool isarr = false;
+  napi_is_array(env, arr, &isarr);
+  if (!isarr) return false;
+  uint32_t n = 0;
+  napi_get_array_length(env, arr, &n);
+  for (uint32_t i = 0; i < n && i < 4; i++) {
+    napi_value e;
+    napi_get_element(env, arr, i, &e);
+    double d = 0;
+    napi_get_value_double(env, e, &d);
+    out[i] = (float)d;
+  }
+  return true;
+}
+static bool read_emits(napi_env env, napi_value list, std::vector<sl_emit> &out) {
+  bool isarr = false;
+  napi_is_array(env, list, &isarr);
+  if (!isarr) return false;
+  uint32_t n = 0;
+  napi_get_array_length(env, list, &n);
+  out.resize(n);
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value rec;
+    napi_get_element(env, list, i, &rec);
+    sl_emit &e = out[i];
+    memset(&e, 0, sizeof e);
+    if (!read_vec4(env, rec, "a", e.a) || !read_vec4(env, rec, "b", e.b) || !read_vec4(env, rec, "c", e.c) ||
+        !read_vec4(env, rec, "d", e.d))
+      return false;
+    e.shape = (uint32_t)get_num(env, rec, "shape", 0);
+    double cnt = get_num(env, rec, "count", 0);
+    e.count = cnt > 0 ? (uint32_t)cnt : 0u;
+    e.seed = (uint32_t)(int64_t)get_num(env, rec, "seed", 0);
+  }
+  return true;
+}
+
+// emitNow(h, list, fill) -> emitted
+static napi_value EmitNow(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  size_t argc = 3;
+  napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr);
+  if (argc < 2) { napi_throw_type_error(env, nullptr, "emitNow(h, list, fill)"); return nullptr; }
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  std::vector<sl_emit> list;
+  if (!read_emits(env, argv[1], list)) { napi_throw_type_error(env, nullptr, "emitNow: bad list"); return nullptr; }
+  bool fill = false;
+  if (argc >= 3) napi_get_value_bool(env, argv[2], &fill);
+  uint32_t emitted = 0;
+  int r = sl_emit_now(s, list.data(), (uint32_t)list.size(), fill ? 1 : 0, &emitted);
+  if (r) return throw_sl(env, s, r);
+  return num(env, emitted);
+}
+
+// stepBegin(h, frameDt) -> {n, dt}
+static napi_value StepBegin(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  double fdt = 0;
+  napi_get_value_double(env, argv[1], &fdt);
+  uint32_t n = 0;
+  double dt = 0;
+  int r = sl_step_begin(s, fdt, &n, &dt);
+  if (r) return throw_sl(env, s, r);
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "n", n);
+  set_num(env, o, "dt", dt);
+  return o;
+}
+
+// substep(h, dt, list) -> emitted
+static napi_value Substep(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  double dt = 0;
+  napi_get_value_double(env, argv[1], &dt);
+  std::vector<sl_emit> list;
+  if (!read_emits(env, argv[2], list)) { napi_throw_type_error(env, nullptr, "substep: bad list"); return nullptr; }
+  uint32_t emitted = 0;
+  int r = sl_substep(s, dt, list.data(), (uint32_t)list.size(), &emitted);
+  if (r) return throw_sl(env, s, r);
+  return num(env, emitted);
+}
+
+static napi_value StepEnd(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  double fdt = 0;
+  napi_get_value_double(env, argv[1], &fdt);
+  int r = sl_step_end(s, fdt);
+  if (r) return throw_sl(env, s, r);
+  return nullptr;
+}
+
+// simple (h) -> status calls
+#define SIMPLE(NAME, CALL)                                        \
+  static napi_value NAME(napi_env env, napi_callback_info info) { \
+    napi_value argv[1];                                           \
+    if (!get_args(env, info, 1, argv)) return nullptr;            \
+    sl_sim *s = get_sim(env, argv[0]);                            \
+    if (!s) return nullptr;                                       \
+    int r = CALL(s);                                              \
+    if (r) return throw_sl(env, s, r);                            \
+    return nullptr;                                               \
+  }
+SIMPLE(ClearWater, sl_clear_water)
+SIMPLE(Sync, sl_sync)
+SIMPLE(Reorder, sl_reorder)
+
+static napi_value SetSubsteps(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  uint32_t n = 0;
+  napi_get_value_uint32(env, argv[1], &n);
+  int r = sl_set_substeps(s, n);
+  if (r) return throw_sl(env, s, r);
+  return nullptr;
+}
+static napi_value SetRippleMask(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  bool b = false;
+  napi_get_value_bool(env, argv[1], &b);
+  int r = sl_set_ripple_mask(s, b ? 1 : 0);
+  if (r) return throw_sl(env, s, r);
+  return nullptr;
+}
+static napi_value SaveWater(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  uint32_t id = 0;
+  int r = sl_save_water(s, &id);
+  if (r) return throw_sl(env, s, r);
+  return num(env, id);
+}
+static napi_value HasWaterCopy(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  uint32_t id = 0;
+  napi_get_value_uint32(env, argv[1], &id);
+  return boolean(env, sl_has_water_copy(s, id) != 0);
+}
+static napi_value RestoreWater(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  uint32_t id = 0;
+  napi_get_value_uint32(env, argv[1], &id);
+  int r = sl_restore_water(s, id);
+  if (r < 0) return throw_sl(env, s, r);
+  return boolean(env, r == 1);
+}
+static napi_value PollStats(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  int u = 0;
+  int r = sl_poll_stats(s, &u);
+  if (r) return throw_sl(env, s, r);
+  return boolean(env, u != 0);
+}
+static napi_value Stats(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  sl_stats_t st;
+  int r = sl_stats(s, &st);
+  if (r) return throw_sl(env, s, r);
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "count", st.count); set_num(env, o, "ww", st.ww_count); set_num(env, o, "impacts", st.impacts);
+  set_num(env, o, "frames", st.frames); set_num(env, o, "vmax", st.vmax); set_num(env, o, "time", st.time);
+  set_num(env, o, "liters", st.liters); set_num(env, o, "gpuErrors", st.gpu_errors);
+  set_num(env, o, "substeps", st.substeps_last);
+  napi_set_named_property(env, o, "full", boolean(env, st.full != 0));
+  napi_value grid;
+  napi_create_array_with_length(env, 3, &grid);
+  for (uint32_t i = 0; i < 3; i++) napi_set_element(env, grid, i, num(env, st.grid[i]));
+  napi_set_named_property(env, o, "grid", grid);
+  return o;
+}
+// hostState(h) -> {count, time, frame, vmax, full, impacts, wwCount}: no device wait
+static napi_value HostState(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  napi_value o;
+  napi_create_object(env, &o);
+  const char *names[] = {"count", "time", "frame", "vmax", "full", "impacts", "ww_count"};
+  const char *keys[] = {"count", "time", "frame", "vmax", "full", "impacts", "wwCount"};
+  for (int i = 0; i < 7; i++) {
+    double v = 0;
+    sl_get_scalar(s, names[i], &v);
+    set_num(env, o, keys[i], v);
+  }
+  return o;
+}
+static napi_value SetScalar(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  char name[32];
+  size_t len = 0;
+  napi_get_value_string_utf8(env, argv[1], name, sizeof name, &len);
+  double v = 0;
+  napi_get_value_double(env, argv[2], &v);
+  int r = sl_set_scalar(s, name, v);
+  if (r) return throw_sl(env, s, r);
+  return nullptr;
+}
+
+// readback(h, field) -> ArrayBuffer (reference layout)
+static napi_value Readback(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  int32_t f = 0;
+  napi_get_value_int32(env, argv[1], &f);
+  size_t n = sl_field_bytes(s, f);
+  if (!n) { napi_throw_range_error(env, nullptr, "readback: field not present"); return nullptr; }
+  void *data = nullptr;
+  napi_value ab;
+  NAPI_CALL(env, napi_create_arraybuffer(env, n, &data, &ab));
+  int r = sl_readback(s, f, data, n);
+  if (r) return throw_sl(env, s, r);
+  return ab;
+}
+// upload(h, field, bytes)
+static napi_value Upload(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  int32_t f = 0;
+  napi_get_value_int32(env, argv[1], &f);
+  void *data = nullptr;
+  size_t len = 0;
+  if (!get_bytes(env, argv[2], &data, &len)) { napi_throw_type_error(env, nullptr, "upload: bytes"); return nullptr; }
+  int r = sl_upload(s, f, data, len);
+  if (r) return throw_sl(env, s, r);
+  return nullptr;
+}
+static napi_value Version(napi_env env, napi_callback_info) { return num(env, sl_version()); }
+static napi_value BuildInfo(napi_env env, napi_callback_info) {
+  napi_value v;
+  napi_create_string_utf8(env, sl_build_info(), NAPI_AUTO_LENGTH, &v);
+  return v;
+}
+
+static napi_value Init(napi_env env, napi_value exports) {
+  napi_property_descriptor d[] = {
+      {"configDefault", 0, ConfigDefault, 0, 0, 0, napi_default, 0},
+      {"create", 0, Create, 0, 0, 0, napi_default, 0},
+      {"destroy", 0, Destroy, 0, 0, 0, napi_default, 0},
+      {"gridInfo", 0, GridInfo, 0, 0, 0, napi_default, 0},
+      {"setObjects", 0, SetObjects, 0, 0, 0, napi_default, 0},
+      {"emitNow", 0, EmitNow, 0, 0, 0, napi_default, 0},
+      {"stepBegin", 0, StepBegin, 0, 0, 0, napi_default, 0},
+      {"substep", 0, Substep, 0, 0, 0, napi_default, 0},
+      {"stepEnd", 0, StepEnd, 0, 0, 0, napi_default, 0},
+      {"setSubsteps", 0, SetSubsteps, 0, 0, 0, napi_default, 0},
+      {"clearWater", 0, ClearWater, 0, 0, 0, napi_default, 0},
+      {"saveWater", 0, SaveWater, 0, 0, 0, napi_default, 0},
+      {"hasWaterCopy", 0, HasWaterCopy, 0, 0, 0, napi_default, 0},
+      {"restoreWater", 0, RestoreWater, 0, 0, 0, napi_default, 0},
+      {"pollStats", 0, PollStats, 0, 0, 0, napi_default, 0},
+      {"stats", 0, Stats, 0, 0, 0, napi_default, 0},
+      {"hostState", 0, HostState, 0, 0, 0, napi_default, 0},
+      {"setScalar", 0, SetScalar, 0, 0, 0, napi_default, 0},
+      {"sync", 0, Sync, 0, 0, 0, napi_default, 0},
+      {"reorder", 0, Reorder, 0, 0, 0, napi_default, 0},
+      {"setRippleMask", 0, SetRippleMask, 0, 0, 0, napi_default, 0},
+      {"readback", 0, Readback, 0, 0, 0, napi_default, 0},
+      {"upload", 0, Upload, 0, 0, 0, napi_default, 0},
+      {"version", 0, Version, 0, 0, 0, napi_default, 0},
+      {"buildInfo", 0, BuildInfo, 0, 0, 0, napi_default, 0},
+  };
+  napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
+  return exports;
+}
+
+NAPI_MODULE(NODE_GYP_MODULE_NAME, Init)

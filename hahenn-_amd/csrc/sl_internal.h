@@ -4,7 +4,7 @@
 //   particles : px,py,pz,vx,vy,vz (f32) + pf (u8: bit0 alive, bit1 free-flight mark), ping-pong x2
 //   whitewater: wp (float4 pos.xyz, life), wv (float4 vel.xyz, kind), ping-pong x2
 //   MAC nodes : N = (nx+1)(ny+1)(nz+1), x-fastest; per component planes of N floats:
-//               acc (6N i32: momX,wtX,momY,wtY,momZ,wtZ), vel (3N), velOld (3N) + vmask (N u8),
+//               acc (3N u64: momentum sum << 32 | weight sum, per axis), vel (3N), velOld (3N) + vmask (N u8),
 //               frac (3N), svel (3N)
 //   cells     : C = nx*ny*nz, x-fastest: cellCount (u32), divg (f32)
 //   SOR arrays: red/black split ("packed"): cell (i,j,k) has colour (i+j+k)&1 and slot
@@ -91,6 +91,14 @@ __device__ __forceinline__ f3 cross3(f3 a, f3 b) {
 }
 __device__ __forceinline__ float clampf(float x, float lo, float hi) { return fminf(fmaxf(x, lo), hi); }
 __device__ __forceinline__ float mixf(float a, float b, float t) { return a * (1.0f - t) + b * t; }
+// WGSL transcendentals (exp, pow, sin, cos, sinh) evaluated as the correctly rounded f32 value: the
+// double-precision function rounded once to f32. That is the most accurate result the WGSL error
+// bounds allow, and it does not depend on a math library's last-ulp behaviour.
+__device__ __forceinline__ float sl_exp(float x) { return (float)exp((double)x); }
+__device__ __forceinline__ float sl_pow(float x, float y) { return (float)pow((double)x, (double)y); }
+__device__ __forceinline__ float sl_sin(float x) { return (float)sin((double)x); }
+__device__ __forceinline__ float sl_cos(float x) { return (float)cos((double)x); }
+__device__ __forceinline__ float sl_sinh(float x) { return (float)sinh((double)x); }
 __device__ __forceinline__ int32_t f2i(float f) {
   if (f != f) return 0;
   if (f >= 2147483648.0f) return INT32_MAX;
@@ -264,13 +272,14 @@ struct SlLaunch {
 // grid stages (work map in units of SL_WG-wide workgroups over the stage's index space)
 void sl_k_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 void sl_k_geom(const SlLaunch &L);
-void sl_k_p2g(const SlLaunch &L, int count_only);
+void sl_k_p2g(const SlLaunch &L, int count_only, int variant);
+int sl_p2g_chunk(int variant);
 void sl_k_normforce(const SlLaunch &L);
 void sl_k_divsetup(const SlLaunch &L);
 void sl_k_sor(const SlLaunch &L, uint32_t parity, uint32_t pass);
 void sl_k_project(const SlLaunch &L);
 // particle stages
-void sl_k_g2p(const SlLaunch &L);
+void sl_k_g2p(const SlLaunch &L, int variant);
 void sl_k_emit(const SlLaunch &L);
 void sl_k_count(const SlLaunch &L, uint32_t mode, uint32_t y);
 void sl_k_vmax_reset(const SlLaunch &L);
@@ -288,4 +297,3 @@ void sl_k_ripple_frame_ex(const SlWork &rows, const SlWork &cols, const SlWork &
                           hipStream_t s, int mask);
 // workgroups of compaction per tank (each compaction workgroup covers SL_COMPACT_ITEMS particles)
 #define SL_COMPACT_ITEMS 1024
-#define SL_P2G_CHUNK 512

@@ -203,6 +203,12 @@ int sl_timing_enable(sl_sim *sim, int enable);
 /* ms[i] = accumulated milliseconds of stage i since the last reset, launches[i] = kernel launches */
 int sl_timing_read(sl_sim *sim, double *ms, uint32_t *launches, int n, int reset);
 
+/* Chooses the fastest work decomposition of P2G and G2P for this simulation by timing each variant
+   on its live state (between two frames, after at least one step) and restoring everything the
+   trials touched. All variants compute bit-identical results. choice (optional, 2 entries) receives
+   the selected P2G and G2P variants. No counterpart in the reference. */
+int sl_autotune(sl_sim *sim, uint32_t reps, uint32_t *choice);
+
 /* ABI version + build description */
 int sl_version(void);
 const char *sl_build_info(void);

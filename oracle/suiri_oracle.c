@@ -45,6 +45,13 @@ static inline uint32_t f2u(float f) {
   if (f >= 4294967296.0f) return UINT32_MAX;
   return (uint32_t)f;
 }
+/* WGSL transcendentals as correctly rounded f32 values: the double function rounded once. This is
+   the most accurate result the WGSL error bounds allow and is independent of a library's last ulp. */
+static inline float cr_expf(float x) { return (float)exp((double)x); }
+static inline float cr_powf(float x, float y) { return (float)pow((double)x, (double)y); }
+static inline float cr_sinf(float x) { return (float)sin((double)x); }
+static inline float cr_cosf(float x) { return (float)cos((double)x); }
+static inline float cr_sinhf(float x) { return (float)sinh((double)x); }
 static inline int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
 static inline v3i v2i(v3 g) { v3i r = {f2i(g.x), f2i(g.y), f2i(g.z)}; return r; }
 static inline float ld4(const float *a, uint32_t n, int c) { return a[(size_t)n * 4 + c]; }
@@ -512,10 +519,10 @@ static void splashCrown(or_sim *s, v3 x, float U, float m, uint32_t id, or_parti
   if (s->P.tank[1] <= 0.0f) return;
   const float PI998 = (float)(3.14159265 * 998.0);
   const float RS = (float)(998.0 * 0.0728);
-  float D = powf(6.0f * m / PI998, (float)(1.0 / 3.0));
+  float D = cr_powf(6.0f * m / PI998, (float)(1.0 / 3.0));
   float We = 998.0f * U * U * D / 0.0728f;
   float Oh = 0.001f / sqrtf(RS * D);
-  float K = We * powf(Oh, -0.4f);
+  float K = We * cr_powf(Oh, -0.4f);
   uint32_t n = f2u(clampw(rintf(4.0f * (K / 2100.0f - 1.0f)), 0.0f, 24.0f));
   uint32_t nj = K > 700.0f ? (K > 2100.0f ? 2u : 1u) : 0u;
   if (n + nj == 0u) return;
@@ -530,11 +537,11 @@ static void splashCrown(or_sim *s, v3 x, float U, float m, uint32_t id, or_parti
     float ph = ph0 + 6.2831853f * ((float)i + 0.3f * rnd(&st)) / (float)n;
     float el = 0.9f + 0.45f * rnd(&st);
     float sp = U * (0.15f + 0.2f * rnd(&st));
-    v3 d = V(cosf(ph), 0.0f, sinf(ph));
+    v3 d = V(cr_cosf(ph), 0.0f, cr_sinf(ph));
     or_particle q;
     v3 pp = vadd(x, vmul(d, r0));
     q.pos[0] = pp.x; q.pos[1] = pp.y; q.pos[2] = pp.z; q.pos[3] = 0.5f + 0.4f * rnd(&st);
-    float ce = cosf(el), se = sinf(el);
+    float ce = cr_cosf(el), se = cr_sinf(el);
     v3 vv = vadd(vmul(vmul(d, ce), sp), V(0.0f, se * sp, 0.0f));
     q.vel[0] = vv.x; q.vel[1] = vv.y; q.vel[2] = vv.z; q.vel[3] = 0.0f;
     ww[k] = q;
@@ -624,7 +631,7 @@ void or_g2p(or_sim *s) {
           v = vl;
           airborne = 0.0f;
           __atomic_fetch_add(&s->C.impacts, 1u, __ATOMIC_RELAXED);
-          float m = 998.0f * powf(0.5f * h, 3.0f);
+          float m = 998.0f * cr_powf(0.5f * h, 3.0f);
           uint32_t R = f2u(s->P.misc[2]);
           if (R > 0u) depositRipple(s, x.x, x.z, 2.0f * m * fminw(dv, 8.0f), R);
           float level = s->P.origin[1] +
@@ -684,15 +691,15 @@ void or_emit_kernel(or_sim *s, uint32_t k, uint32_t total) {
       float rr = em->a[3] * sqrtf(rnd(&st));
       float th = 6.2831853f * rnd(&st);
       v3 u = pos3(em->c), w = pos3(em->d), dir = pos3(em->b);
-      v3 ring = vmul(vadd(vmul(u, cosf(th)), vmul(w, sinf(th))), rr);
+      v3 ring = vmul(vadd(vmul(u, cr_cosf(th)), vmul(w, cr_sinf(th))), rr);
       float r3 = rnd(&st);
       pos = vadd(vadd(pos3(em->a), ring), vmul(dir, r3 * em->d[3]));
       v3 dv = dir;
       if (em->c[3] > 0.0f) {
-        float ca = 1.0f - rnd(&st) * (1.0f - cosf(em->c[3]));
+        float ca = 1.0f - rnd(&st) * (1.0f - cr_cosf(em->c[3]));
         float sa = sqrtf(fmaxw(1.0f - ca * ca, 0.0f));
         float ph = 6.2831853f * rnd(&st);
-        dv = vadd(vmul(dir, ca), vmul(vadd(vmul(u, cosf(ph)), vmul(w, sinf(ph))), sa));
+        dv = vadd(vmul(dir, ca), vmul(vadd(vmul(u, cr_cosf(ph)), vmul(w, cr_sinf(ph))), sa));
       }
       float j0 = rnd(&st), j1 = rnd(&st), j2 = rnd(&st);
       vel = vadd(vmul(dv, em->b[3]), vmul(vsub(V(j0, j1, j2), V(0.5f, 0.5f, 0.5f)), 0.02f * em->b[3]));
@@ -702,7 +709,7 @@ void or_emit_kernel(or_sim *s, uint32_t k, uint32_t total) {
       dd = vadd(dd, V(1e-4f, 1e-4f, 1e-4f));
       float l = vlen(dd);
       dd = V(dd.x / l, dd.y / l, dd.z / l);
-      float r = powf(rnd(&st), 0.3333f);
+      float r = cr_powf(rnd(&st), 0.3333f);
       pos = vadd(pos3(em->a), vmul(vmul(dd, em->a[3]), r));
       vel = pos3(em->b);
       mark = em->b[3];
@@ -921,22 +928,22 @@ void or_revolve(or_sim *s) {
     float kk = sqrtf(kx * kx + kz * kz);
     float D = fmaxw(md, 0.002f);
     float kd = fminw(kk * D, 15.0f);
-    float th = 1.0f - 2.0f / (expf(2.0f * kd) + 1.0f);
+    float th = 1.0f - 2.0f / (cr_expf(2.0f * kd) + 1.0f);
     float w2 = (g * kk + sr * kk * kk * kk) * th;
     float w = sqrtf(w2);
     float gam = 2.0f * nu * kk * kk + film * kk * sqrtf(nu * w * 0.125f) +
-                kk * sqrtf(nu * w * 0.5f) / fmaxw(sinhf(fminw(2.0f * kd, 30.0f)), 1e-4f);
+                kk * sqrtf(nu * w * 0.5f) / fmaxw(cr_sinhf(fminw(2.0f * kd, 30.0f)), 1e-4f);
     float wd = sqrtf(fmaxw(w2 - gam * gam, 1e-6f));
     float sx = st[0], sy = st[1];
     if (dt > 0.0f) {
-      float c = cosf(wd * dt), sn = sinf(wd * dt), e = expf(-gam * dt);
+      float c = cr_cosf(wd * dt), sn = cr_sinf(wd * dt), e = cr_expf(-gam * dt);
       float nx = sx * c + (sy + gam * sx) / wd * sn;
       float ny = sy * c - (w2 * sx + gam * sy) / wd * sn;
       sx = e * nx; sy = e * ny;
     }
-    float hp = 1.0f - expf(-(kk * kk) / (hpk * hpk));
+    float hp = 1.0f - cr_expf(-(kk * kk) / (hpk * hpk));
     float sw = fmaxw(1.2f * Lx / (float)R, 0.003f);
-    float spread = expf(-0.5f * kk * kk * sw * sw);
+    float spread = cr_expf(-0.5f * kk * kk * sw * sw);
     sy += -(kk * th / s->q.r_rho_kick) * s->rhat[2 * id] * hp * spread;
     st[0] = sx; st[1] = sy;
   }

@@ -12,7 +12,9 @@
  *
  * Numeric conventions (WGSL semantics): round() is half-to-even (rintf); i32()/u32() truncate and
  * saturate; mix(a,b,t) = a*(1-t) + b*t; u32 arithmetic wraps; constant sub-expressions of abstract
- * floats are folded in double precision and rounded once to f32. Build with -ffp-contract=off.
+ * floats are folded in double precision and rounded once to f32; transcendentals (exp, pow, sin, cos,
+ * sinh) are the correctly rounded f32 values (double evaluation rounded once), the most accurate
+ * results the WGSL error bounds allow. Build with -ffp-contract=off.
  *
  * One deliberate, documented choice where the reference is nondeterministic: particle compaction
  * (src/js/31-wgsl-sim2.js:317-328 appends survivors with an atomic counter, so the order changes from

@@ -137,6 +137,24 @@ def test_steps_with_periodic_reorder():
     assert np.array_equal(gp[:, 3], op[:, 3])
 
 
+def test_autotune_keeps_results_bitwise():
+    """The tuner times every kernel variant on the live state and restores it; the variants compute
+    identical results, so the run continues exactly like the oracle."""
+    o, g = twin.make_twin(twin.small_scene(whitewater=False))
+    for _ in range(4):
+        twin.oracle_step(o, 1 / 120)
+        twin.gpu_step(g, 1 / 120)
+    ch = g.autotune(reps=1)
+    assert 0 <= ch["p2g_variant"] < 8 and 0 <= ch["g2p_variant"] < 4
+    for v in range(8):   # and every P2G variant on its own
+        g.set_scalar("p2g_variant", v)
+        twin.oracle_step(o, 1 / 120)
+        twin.gpu_step(g, 1 / 120)
+    n = int(o.field("counters")[0])
+    assert g.stats()["count"] == n
+    assert np.array_equal(g.readback(suiri.F_PARTICLES)[:n], o.field("parts")[:n])
+
+
 def test_multi_step_dam_break():
     """50 steps: identical particle counts; fields within tolerance; volume and kinetic energy
     within 1%."""
