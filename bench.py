@@ -100,6 +100,47 @@ def cpu_baseline(wl, steps, threads):
     return psteps / dt, dt
 
 
+def pmc_traffic(out_dir, timeout=150):
+    """HBM bytes of one k_sor launch from two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE) of a short
+    bench, each scaled by what the same counter reads for a plain 256 MiB device copy. Child processes;
+    returns None when rocprofv3 is missing or a pass fails."""
+    import csv
+    import shutil
+    import statistics
+    import subprocess
+    prof = shutil.which("rocprofv3")
+    if not prof:
+        return None
+    env = dict(os.environ, TMPDIR="/tmp")
+    short = [sys.executable, os.path.join(ROOT, "bench.py"), "--steps", "3", "--warmup", "1", "--no-cpu", "--no-pmc"]
+    calib = [sys.executable, os.path.join(ROOT, "bench.py"), "--calib-copy-mb", "256"]
+    runs = {"fetch": ("FETCH_SIZE", short, "k_sor"), "write": ("WRITE_SIZE", short, "k_sor"),
+            "cfetch": ("FETCH_SIZE", calib, None), "cwrite": ("WRITE_SIZE", calib, None)}
+    vals = {}
+    for key, (counter, cmd, regex) in runs.items():
+        d = os.path.join(out_dir, "pmc_" + key)
+        args = [prof, "--pmc", counter, "--output-format", "csv", "-d", d, "-o", "run"]
+        if regex:
+            args += ["--kernel-include-regex", regex]
+        try:
+            subprocess.run(args + ["--"] + cmd, env=env, cwd="/tmp", timeout=timeout, capture_output=True)
+            rows = list(csv.DictReader(open(os.path.join(d, "run_counter_collection.csv"))))
+        except Exception:
+            return None
+        if regex:
+            v = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"].startswith("k_sor(")]
+        else:
+            v = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
+        if not v:
+            return None
+        vals[key] = statistics.mean(v)
+    kib = 256 * 1024
+    fetch = vals["fetch"] * 1024 * kib / vals["cfetch"]
+    write = vals["write"] * 1024 * kib / vals["cwrite"]
+    return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
+            "fetch_factor": kib / vals["cfetch"], "write_factor": kib / vals["cwrite"]}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -109,6 +150,7 @@ def main():
     ap.add_argument("--cpu-steps", type=int, default=2)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ripple", action="store_true")
+    ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter runs for `traffic`")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -190,6 +232,12 @@ def main():
         "gpu_errors": st2["gpuErrors"],
         "tuned_variants": tuned,
     }
+    g.close()
+    if not args.no_pmc:
+        t = pmc_traffic(os.path.join(ROOT, "gpurun_out", "bench_pmc"))
+        if t:
+            out["roofline"]["traffic"] = t["bytes"]
+            out["roofline"]["traffic_detail"] = t
     if not args.no_cpu:
         threads = os.cpu_count() or 1
         v, secs = cpu_baseline(wl, args.cpu_steps, threads)
