@@ -74,7 +74,10 @@ EXPORTS = ["sl_config_default", "sl_create", "sl_destroy", "sl_last_error", "sl_
            "sl_clear_water", "sl_save_water", "sl_has_water_copy", "sl_restore_water", "sl_poll_stats",
            "sl_stats", "sl_sync", "sl_set_ripple_mask", "sl_field_bytes", "sl_upload", "sl_readback",
            "sl_set_scalar", "sl_get_scalar", "sl_run_stage", "sl_timing_enable", "sl_timing_read",
-           "sl_version", "sl_build_info", "sl_reorder", "sl_autotune"]
+           "sl_version", "sl_build_info", "sl_reorder", "sl_autotune",
+           "sl_batch_create", "sl_batch_destroy", "sl_batch_last_error", "sl_batch_size", "sl_batch_sim",
+           "sl_batch_step", "sl_batch_capture", "sl_batch_sync", "sl_batch_reorder", "sl_batch_autotune",
+           "sl_batch_set_variants"]
 
 _lib = None
 
@@ -123,6 +126,17 @@ def load():
         "sl_run_stage": (I, [P, I, U, D]),
         "sl_timing_enable": (I, [P, I]),
         "sl_timing_read": (I, [P, ctypes.POINTER(D), ctypes.POINTER(U), I, I]),
+        "sl_batch_create": (I, [ctypes.POINTER(SlConfig), U, ctypes.POINTER(P)]),
+        "sl_batch_destroy": (None, [P]),
+        "sl_batch_last_error": (ctypes.c_char_p, [P]),
+        "sl_batch_size": (U, [P]),
+        "sl_batch_sim": (P, [P, U]),
+        "sl_batch_step": (I, [P, D]),
+        "sl_batch_capture": (I, [P, D]),
+        "sl_batch_sync": (I, [P]),
+        "sl_batch_reorder": (I, [P]),
+        "sl_batch_autotune": (I, [P, U, ctypes.POINTER(U)]),
+        "sl_batch_set_variants": (I, [P, ctypes.POINTER(U)]),
         "sl_version": (I, []),
         "sl_build_info": (ctypes.c_char_p, []),
     }
@@ -161,10 +175,14 @@ def emit_records(records):
 class Sim:
     """One simulation on the GPU through the C ABI (mirrors FluidSim, src/js/50-sim.js:28-376)."""
 
-    def __init__(self, cfg):
+    def __init__(self, cfg, _handle=None, _owner=None):
         self.L = load()
         self.ptr = ctypes.c_void_p()
-        self._check(self.L.sl_create(ctypes.byref(cfg), ctypes.byref(self.ptr)), None)
+        self._owner = _owner  # a Batch owns its tanks: they are destroyed with it
+        if _handle is not None:
+            self.ptr = ctypes.c_void_p(_handle)
+        else:
+            self._check(self.L.sl_create(ctypes.byref(cfg), ctypes.byref(self.ptr)), None)
         g = SlGrid()
         self._check(self.L.sl_grid_info(self.ptr, ctypes.byref(g)))
         self.grid = g
@@ -178,9 +196,9 @@ class Sim:
             raise SuiriError(f"suiri error {r}: {msg.decode() if msg else ''}")
 
     def close(self):
-        if self.ptr:
+        if self.ptr and self._owner is None:
             self.L.sl_destroy(self.ptr)
-            self.ptr = ctypes.c_void_p()
+        self.ptr = ctypes.c_void_p()
 
     def __del__(self):
         try:
@@ -290,3 +308,61 @@ class Sim:
         ln = (ctypes.c_uint32 * len(T_NAMES))()
         self._check(self.L.sl_timing_read(self.ptr, ms, ln, len(T_NAMES), int(reset)))
         return {n: (ms[i], ln[i]) for i, n in enumerate(T_NAMES)}
+
+
+class Batch:
+    """Many independent tanks stepped together, one kernel launch per stage for all of them
+    (sl_batch_*). Per-tank set-up, emission between frames and readback go through `tanks[i]`, a
+    Sim on the batch's stream."""
+
+    def __init__(self, cfgs):
+        self.L = load()
+        self.ptr = ctypes.c_void_p()
+        arr = (SlConfig * len(cfgs))(*cfgs)
+        r = self.L.sl_batch_create(arr, len(cfgs), ctypes.byref(self.ptr))
+        if r != SL_OK:
+            msg = self.L.sl_batch_last_error(None)
+            raise SuiriError(f"suiri error {r}: {msg.decode() if msg else ''}")
+        self.tanks = [Sim(None, _handle=self.L.sl_batch_sim(self.ptr, i), _owner=self) for i in range(len(cfgs))]
+
+    def _check(self, r):
+        if r != SL_OK:
+            msg = self.L.sl_batch_last_error(self.ptr)
+            raise SuiriError(f"suiri error {r}: {msg.decode() if msg else ''}")
+
+    def __len__(self):
+        return len(self.tanks)
+
+    def step(self, dt):
+        self._check(self.L.sl_batch_step(self.ptr, float(dt)))
+
+    def capture(self, dt):
+        self._check(self.L.sl_batch_capture(self.ptr, float(dt)))
+
+    def sync(self):
+        self._check(self.L.sl_batch_sync(self.ptr))
+
+    def reorder(self):
+        self._check(self.L.sl_batch_reorder(self.ptr))
+
+    def autotune(self, reps=2):
+        ch = (ctypes.c_uint32 * 6)()
+        self._check(self.L.sl_batch_autotune(self.ptr, int(reps), ch))
+        return [(ch[2 * k], ch[2 * k + 1]) for k in range(3)]
+
+    def set_variants(self, pairs):
+        ch = (ctypes.c_uint32 * 6)(*[int(v) for pr in pairs for v in pr])
+        self._check(self.L.sl_batch_set_variants(self.ptr, ch))
+
+    def close(self):
+        for t in getattr(self, "tanks", []):
+            t.ptr = ctypes.c_void_p()
+        if self.ptr:
+            self.L.sl_batch_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -19,6 +19,7 @@ static thread_local std::string g_err;
 // dynamic per-substep fields of a tank, written on the device by k_update (kernel arguments are
 // captured at launch, so the host can change them freely while earlier work is in flight)
 struct SlTankDyn {
+  double time_d;
   float dt, gdt, time_f, rLx, rLz, rdt, rmd;
   uint32_t frame, obj_count, emit_k, emit_total, iters;
   int32_t mask_bound;
@@ -29,6 +30,7 @@ __global__ void k_update(SlTank *T, SlTankDyn d) {
   T->rLx = d.rLx; T->rLz = d.rLz; T->rdt = d.rdt; T->rmd = d.rmd;
   T->frame = d.frame; T->obj_count = d.obj_count; T->emit_k = d.emit_k; T->emit_total = d.emit_total;
   T->iters = d.iters; T->mask_bound = d.mask_bound;
+  T->time_d = d.time_d;
 }
 
 // pinned staging for host->device uploads issued on the stream: two halves, each reused only after
@@ -43,6 +45,11 @@ struct Staging {
 
 struct sl_sim {
   sl_config cfg;
+  // set when the simulation is a tank of a batch: the batch owns the stream, the table slot of the
+  // tank record and the counters slot
+  hipStream_t ext_stream = nullptr;
+  SlTank *ext_tank = nullptr;
+  SlCounters *ext_counters = nullptr;
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -204,6 +211,7 @@ static int push_dyn(sl_sim *s, uint32_t emit_k = 0, uint32_t emit_total = 0) {
   d.rLx = s->ht.rLx; d.rLz = s->ht.rLz; d.rdt = s->ht.rdt; d.rmd = s->ht.rmd;
   d.frame = s->frame; d.obj_count = s->obj_count; d.emit_k = emit_k; d.emit_total = emit_total;
   d.iters = s->iters; d.mask_bound = s->mask_bound;
+  d.time_d = s->time;
   hipLaunchKernelGGL(k_update, dim3(1), dim3(64), 0, s->stream, s->d_tank, d);
   return check_launch(s, "update");
 }
@@ -270,7 +278,8 @@ static int create_impl(sl_sim *s) {
   if (c.cap == 0) FAIL(s, SL_EARG, "capacity must be positive");
   HIPCHK(s, hipSetDevice(c.device));
   s->device = c.device;
-  HIPCHK(s, hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  if (s->ext_stream) s->stream = s->ext_stream;
+  else HIPCHK(s, hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   // grid sizing, 50-sim.js:33-45
   s->nx = (int)fmax(8.0, round(c.tank_w / c.h));
   s->ny = (int)fmax(8.0, round(c.tank_h / c.h));
@@ -314,6 +323,7 @@ static int create_impl(sl_sim *s) {
   T.friction = c.friction;
   T.radius = (float)s->radius;
   T.R_f = (float)s->R;
+  T.h_d = s->h; T.vp_d = s->vp; T.rim_d = s->rim; T.time_d = 0;
   T.rsr = c.r_sigma_rho; T.rnu = c.r_nu; T.rfilm = c.r_film; T.rhpk = c.r_hp_k; T.rrho = c.r_rho_kick;
   // buffers (50-sim.js:59-80, 64-ripple.js:152-165), structure-of-arrays
   const size_t N = s->nodes, C = s->cells, H = s->hcells;
@@ -324,7 +334,8 @@ static int create_impl(sl_sim *s) {
     ALLOC(T.wp[b], (size_t)(s->ww_cap ? s->ww_cap : 1) * 16);
     ALLOC(T.wv[b], (size_t)(s->ww_cap ? s->ww_cap : 1) * 16);
   }
-  ALLOC(T.C, sizeof(SlCounters));
+  if (s->ext_counters) T.C = s->ext_counters;
+  else ALLOC(T.C, sizeof(SlCounters));
   ALLOC(T.acc, N * 24);
   ALLOC(T.cellCount, C * 4);
   ALLOC(T.vel, N * 12); ALLOC(T.velOld, N * 12); ALLOC(T.vmask, N);
@@ -362,7 +373,8 @@ static int create_impl(sl_sim *s) {
     s->sort_temp_bytes = sl_sort_temp_bytes(s->cap, s->sort_end_bit);
     ALLOC(s->sort_temp, s->sort_temp_bytes);
   }
-  ALLOC(s->d_tank, sizeof(SlTank));
+  if (s->ext_tank) s->d_tank = s->ext_tank;
+  else ALLOC(s->d_tank, sizeof(SlTank));
   // pinned host memory
   HIPCHK(s, hipHostMalloc((void **)&s->h_counters, sizeof(SlCounters), hipHostMallocDefault));
   memset(s->h_counters, 0, sizeof(SlCounters));
@@ -406,7 +418,7 @@ extern "C" void sl_destroy(sl_sim *s) {
   if (s->stats_ev) (void)hipEventDestroy(s->stats_ev);
   for (auto &t : s->tev) { (void)hipEventDestroy(t.a); (void)hipEventDestroy(t.b); }
   for (auto e : s->ev_pool) (void)hipEventDestroy(e);
-  if (s->stream) (void)hipStreamDestroy(s->stream);
+  if (s->stream && !s->ext_stream) (void)hipStreamDestroy(s->stream);
   delete s;
 }
 
@@ -1199,6 +1211,461 @@ extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
   s->ev_pool.push_back(b);
   if (choice) { choice[0] = (uint32_t)s->p2g_variant; choice[1] = (uint32_t)s->g2p_variant; }
   return check_launch(s, "autotune");
+}
+
+
+// ================================================================================================
+// Batched tanks: many independent simulations stepped together with one launch per stage for all
+// of them, optionally replayed from a captured hipGraph. No counterpart in the reference (it runs
+// one tank per page). Each tank follows the single-simulation semantics of one substep of dt per
+// frame; every kernel finds its tank through the work map and reads the tank's buffer index,
+// counts and parameters from the device table, so a captured frame stays valid across frames,
+// re-orderings and emissions.
+// ================================================================================================
+#define SL_BUCKETS 3
+
+// frame++, the per-substep uniform (50-sim.js:151-162, :308-317) and the ripple uniform of the frame
+// (64-ripple.js:223-226, mean depth as 50-sim.js:330-333 computes it, from the count at frame start)
+__global__ void k_batch_advance(SlTank *tanks, int n, float dt, double dtd) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  SlTank &T = tanks[t];
+  T.frame += 1;
+  T.dt = dt;
+  T.gdt = T.g * T.dt;
+  T.time_f = (float)T.time_d;
+  T.time_d += dtd;
+  T.emit_k = 0;
+  T.emit_total = 0;
+  if (T.R) {
+    double liters = (double)T.C->count * T.vp_d * 1000;
+    double md = liters / 1000 / ((double)(T.nx * T.nz) * T.h_d * T.h_d);
+    md = md < 0.003 ? 0.003 : (md > T.rim_d ? T.rim_d : md);
+    T.rLx = (float)(T.nx * T.h_d);
+    T.rLz = (float)(T.nz * T.h_d);
+    T.rdt = dt;
+    T.rmd = (float)md;
+  }
+}
+__global__ __launch_bounds__(SL_WG) void k_batch_clear_cells(SlWork W) {
+  int t = 0;
+  uint32_t lb = blockIdx.x;
+  if (W.ntanks > 1) { t = tank_of(W.start, W.ntanks, lb); lb -= W.start[t]; }
+  const SlTank &T = W.tanks[t];
+  uint32_t c = lb * SL_WG + threadIdx.x;
+  if (c < T.cells) T.cellCount[c] = 0u;
+}
+
+struct sl_batch {
+  std::vector<sl_sim *> sims;
+  hipStream_t stream = nullptr;
+  SlTank *d_tanks = nullptr;
+  SlCounters *d_counters = nullptr, *h_counters = nullptr;
+  std::vector<void *> allocs;
+  std::string err;
+  struct Map { uint32_t *d = nullptr; uint32_t total = 0; int variant = -1; };
+  Map m_nodes, m_cells, m_hcells, m_compact, m_ww, m_elems, m_tex, m_rows, m_cols;
+  Map m_p2g[SL_BUCKETS], m_g2p[SL_BUCKETS];  // one per distinct chosen variant (variant -1: unused)
+  int p2g_variant[SL_BUCKETS] = {0, 0, 0}, g2p_variant[SL_BUCKETS] = {1, 1, 1};
+  std::vector<int> bucket;                   // size bucket of each tank
+  uint32_t max_iters = 0;
+  bool any_ripple = false;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+  double graph_dt = 0;
+  hipEvent_t ev_a = nullptr, ev_b = nullptr;
+  uint64_t steps = 0;
+};
+
+static int batch_fail(sl_batch *b, int code, const std::string &msg) {
+  if (b) b->err = msg;
+  g_err = msg;
+  return code;
+}
+
+// size buckets for the variant choice: <= 64Ki cells, <= 256Ki cells, larger
+static int bucket_of(const sl_sim *s) { return s->cells <= 65536 ? 0 : (s->cells <= 262144 ? 1 : 2); }
+
+// device prefix of workgroup counts per tank; tanks with include[t] == 0 get none
+static int build_map(sl_batch *b, sl_batch::Map &m, const std::vector<uint64_t> &items, uint32_t per_wg,
+                     const std::vector<char> *include = nullptr) {
+  const size_t n = b->sims.size();
+  std::vector<uint32_t> start(n + 1, 0);
+  for (size_t t = 0; t < n; t++) {
+    uint64_t wg = (include && !(*include)[t]) ? 0 : (items[t] + per_wg - 1) / per_wg;
+    if (start[t] + wg > 0x7fffffffull) return batch_fail(b, SL_ECAP, "batch: launch too large");
+    start[t + 1] = start[t] + (uint32_t)wg;
+  }
+  if (!m.d) {
+    if (hipMalloc((void **)&m.d, (n + 1) * 4) != hipSuccess) return batch_fail(b, SL_ENOMEM, "batch: map alloc");
+    b->allocs.push_back(m.d);
+  }
+  if (hipMemcpy(m.d, start.data(), (n + 1) * 4, hipMemcpyHostToDevice) != hipSuccess)
+    return batch_fail(b, SL_EHIP, "batch: map upload");
+  m.total = start[n];
+  return SL_OK;
+}
+static SlLaunch batch_launch(const sl_batch *b, const sl_batch::Map &m) {
+  SlLaunch L;
+  L.w.tanks = b->d_tanks;
+  L.w.start = m.d;
+  L.w.ntanks = (int32_t)b->sims.size();
+  L.w.total = m.total;
+  L.stream = b->stream;
+  return L;
+}
+
+// P2G / G2P maps: one per distinct variant chosen by the buckets that hold tanks
+static int batch_particle_maps(sl_batch *b) {
+  const size_t n = b->sims.size();
+  std::vector<uint64_t> cap(n), ww(n);
+  for (size_t t = 0; t < n; t++) { cap[t] = b->sims[t]->cap; ww[t] = b->sims[t]->ww_cap; }
+  for (int pass = 0; pass < 2; pass++) {
+    const int *var = pass ? b->g2p_variant : b->p2g_variant;
+    sl_batch::Map *maps = pass ? b->m_g2p : b->m_p2g;
+    for (int k = 0; k < SL_BUCKETS; k++) { maps[k].variant = -1; maps[k].total = 0; }
+    int used = 0;
+    for (int k = 0; k < SL_BUCKETS; k++) {
+      bool has = false, seen = false;
+      for (size_t t = 0; t < n; t++) has |= b->bucket[t] == k;
+      for (int u = 0; u < used; u++) seen |= maps[u].variant == var[k];
+      if (!has || seen) continue;
+      std::vector<char> inc(n);
+      for (size_t t = 0; t < n; t++) inc[t] = var[b->bucket[t]] == var[k];
+      int r = build_map(b, maps[used], cap, pass ? SL_WG : sl_p2g_chunk(var[k]), &inc);
+      if (r) return r;
+      maps[used++].variant = var[k];
+    }
+  }
+  int r = build_map(b, b->m_compact, cap, SL_COMPACT_ITEMS);
+  return r ? r : build_map(b, b->m_ww, ww, SL_WG);
+}
+
+extern "C" void sl_batch_destroy(sl_batch *b) {
+  if (!b) return;
+  if (b->stream) (void)hipStreamSynchronize(b->stream);
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->graph) (void)hipGraphDestroy(b->graph);
+  for (sl_sim *s : b->sims) sl_destroy(s);
+  for (void *p : b->allocs) (void)hipFree(p);
+  if (b->h_counters) (void)hipHostFree(b->h_counters);
+  if (b->ev_a) (void)hipEventDestroy(b->ev_a);
+  if (b->ev_b) (void)hipEventDestroy(b->ev_b);
+  if (b->stream) (void)hipStreamDestroy(b->stream);
+  delete b;
+}
+
+extern "C" int sl_batch_create(const sl_config *cfgs, uint32_t n, sl_batch **out) {
+  if (out) *out = nullptr;
+  if (!cfgs || !out || !n) return batch_fail(nullptr, SL_EARG, "sl_batch_create: bad arguments");
+  for (uint32_t t = 1; t < n; t++)
+    if (cfgs[t].device != cfgs[0].device) return batch_fail(nullptr, SL_EARG, "sl_batch_create: tanks on several devices");
+  sl_batch *b = new (std::nothrow) sl_batch();
+  if (!b) return batch_fail(nullptr, SL_ENOMEM, "out of host memory");
+  auto fail = [&](int code, const std::string &msg) {
+    sl_batch_destroy(b);
+    return batch_fail(nullptr, code, msg);
+  };
+  if (hipSetDevice(cfgs[0].device) != hipSuccess) return fail(SL_EHIP, "sl_batch_create: no such device");
+  if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(SL_EHIP, "stream");
+  if (hipMalloc((void **)&b->d_tanks, sizeof(SlTank) * n) != hipSuccess) return fail(SL_ENOMEM, "tank table");
+  b->allocs.push_back(b->d_tanks);
+  if (hipMalloc((void **)&b->d_counters, sizeof(SlCounters) * n) != hipSuccess) return fail(SL_ENOMEM, "counters");
+  b->allocs.push_back(b->d_counters);
+  if (hipMemset(b->d_counters, 0, sizeof(SlCounters) * n) != hipSuccess) return fail(SL_EHIP, "counters");
+  if (hipHostMalloc((void **)&b->h_counters, sizeof(SlCounters) * n, hipHostMallocDefault) != hipSuccess)
+    return fail(SL_ENOMEM, "pinned counters");
+  memset(b->h_counters, 0, sizeof(SlCounters) * n);
+  for (uint32_t t = 0; t < n; t++) {
+    sl_sim *s = new (std::nothrow) sl_sim();
+    if (!s) return fail(SL_ENOMEM, "out of host memory");
+    s->cfg = cfgs[t];
+    s->ext_stream = b->stream;
+    s->ext_tank = b->d_tanks + t;
+    s->ext_counters = b->d_counters + t;
+    int r = create_impl(s);
+    if (r) {
+      std::string m = "tank " + std::to_string(t) + ": " + s->err;
+      sl_destroy(s);
+      return fail(r, m);
+    }
+    b->sims.push_back(s);
+    b->bucket.push_back(bucket_of(s));
+    b->max_iters = std::max(b->max_iters, s->iters);
+    if (s->R) b->any_ripple = true;
+  }
+  std::vector<uint64_t> nodes(n), cells(n), hcells(n), rows(n), cols(n), elems(n), tex(n);
+  for (uint32_t t = 0; t < n; t++) {
+    const sl_sim *s = b->sims[t];
+    nodes[t] = s->nodes; cells[t] = s->cells; hcells[t] = s->hcells;
+    const uint32_t M = s->M;
+    uint32_t CW = M ? 4096 / M : 1;
+    if (CW > M) CW = M;
+    if (CW < 1) CW = 1;
+    rows[t] = M; cols[t] = M ? (M + CW - 1) / CW : 0;
+    elems[t] = (uint64_t)M * M;
+    tex[t] = s->R ? (uint64_t)(s->R + 1) * (s->R + 1) : 0;
+  }
+  int r;
+  if ((r = build_map(b, b->m_nodes, nodes, SL_WG)) || (r = build_map(b, b->m_cells, cells, SL_WG)) ||
+      (r = build_map(b, b->m_hcells, hcells, SL_WG)) || (r = build_map(b, b->m_rows, rows, 1)) ||
+      (r = build_map(b, b->m_cols, cols, 1)) || (r = build_map(b, b->m_elems, elems, SL_WG)) ||
+      (r = build_map(b, b->m_tex, tex, SL_WG)) || (r = batch_particle_maps(b)))
+    return fail(r, b->err);
+  if (hipEventCreate(&b->ev_a) != hipSuccess || hipEventCreate(&b->ev_b) != hipSuccess)
+    return fail(SL_EHIP, "events");
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(SL_EHIP, "sl_batch_create: sync");
+  *out = b;
+  return SL_OK;
+}
+
+extern "C" const char *sl_batch_last_error(const sl_batch *b) { return b ? b->err.c_str() : g_err.c_str(); }
+extern "C" uint32_t sl_batch_size(const sl_batch *b) { return b ? (uint32_t)b->sims.size() : 0; }
+extern "C" sl_sim *sl_batch_sim(sl_batch *b, uint32_t i) { return b && i < b->sims.size() ? b->sims[i] : nullptr; }
+
+// the kernels of one frame of every tank, in stream order (no host synchronisation, no memset: the
+// sequence can be captured)
+static int batch_encode_frame(sl_batch *b, double dt) {
+  const int n = (int)b->sims.size();
+  const SlLaunch one = batch_launch(b, b->m_nodes);  // k_count runs one thread per tank
+  hipLaunchKernelGGL(k_batch_advance, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_tanks, n, (float)dt, dt);
+  sl_k_count(one, 5, 0);  // vmax = 0 (50-sim.js:306)
+  hipLaunchKernelGGL(k_batch_clear_cells, dim3(b->m_cells.total), dim3(SL_WG), 0, b->stream,
+                     batch_launch(b, b->m_cells).w);
+  for (const auto &m : b->m_p2g)
+    if (m.variant >= 0 && m.total) sl_k_p2g(batch_launch(b, m), 0, m.variant);
+  sl_k_normforce(batch_launch(b, b->m_nodes));
+  sl_k_divsetup(batch_launch(b, b->m_cells));
+  const SlLaunch Ls = batch_launch(b, b->m_hcells);  // k_sor returns for passes beyond a tank's iters
+  for (uint32_t i = 0; i < b->max_iters; i++) { sl_k_sor(Ls, 0, 2 * i); sl_k_sor(Ls, 1, 2 * i + 1); }
+  sl_k_project(batch_launch(b, b->m_nodes));
+  for (const auto &m : b->m_g2p)
+    if (m.variant >= 0 && m.total) sl_k_g2p(batch_launch(b, m), m.variant);
+  const SlLaunch Lc = batch_launch(b, b->m_compact);
+  sl_k_compact_count(Lc);
+  sl_k_compact_scan(Lc, n);
+  sl_k_compact_scatter(Lc);
+  sl_k_count(one, 1, 0);
+  sl_k_ww(batch_launch(b, b->m_ww), (float)dt);
+  sl_k_count(one, 3, 0);
+  if (b->any_ripple)  // the mask stages return for tanks whose mask is off
+    sl_k_ripple_frame_ex(batch_launch(b, b->m_rows).w, batch_launch(b, b->m_cols).w, batch_launch(b, b->m_elems).w,
+                         batch_launch(b, b->m_tex).w, b->stream, 1);
+  hipError_t e = hipMemcpyAsync(b->h_counters, b->d_counters, sizeof(SlCounters) * n, hipMemcpyDeviceToHost,
+                                b->stream);
+  if (e == hipSuccess) e = hipGetLastError();
+  return e == hipSuccess ? SL_OK : batch_fail(b, SL_EHIP, std::string("batch frame: ") + hipGetErrorString(e));
+}
+
+// objects that changed since the last frame: their GEOM pass (outside any captured graph)
+static int batch_prepare(sl_batch *b) {
+  for (sl_sim *s : b->sims) {
+    if (!s->geom_dirty) continue;
+    int r = push_dyn(s);
+    if (r) return batch_fail(b, r, s->err);
+    sl_k_geom(launch_of(s, s->nodes));
+    s->geom_dirty = false;
+  }
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SL_OK : batch_fail(b, SL_EHIP, hipGetErrorString(e));
+}
+
+// host mirror of what the device did in one frame, plus each tank's re-ordering cadence
+static int batch_host_frame(sl_batch *b, double dt) {
+  for (sl_sim *s : b->sims) {
+    s->frame++;
+    write_params(s, dt);
+    s->time += dt;
+    s->cur ^= 1;
+    s->ww_cur ^= 1;
+    s->frames++;
+    s->substeps_last = 1;
+    s->emitted_at_copy = s->emitted_total;  // the frame's counters copy covers every earlier emission
+    if (s->cfg.reorder_every && ++s->reorder_clock >= s->cfg.reorder_every) {
+      int r = encode_reorder(s);
+      if (r) return batch_fail(b, r, s->err);
+    }
+  }
+  return SL_OK;
+}
+
+extern "C" int sl_batch_step(sl_batch *b, double dt) {
+  if (!b) return batch_fail(nullptr, SL_EARG, "null batch");
+  if (!(dt > 0)) return batch_fail(b, SL_EARG, "sl_batch_step: dt must be positive");
+  int r = batch_prepare(b);
+  if (r) return r;
+  if (b->exec && b->graph_dt == dt) {
+    hipError_t e = hipGraphLaunch(b->exec, b->stream);
+    if (e != hipSuccess) return batch_fail(b, SL_EHIP, std::string("graph launch: ") + hipGetErrorString(e));
+  } else if ((r = batch_encode_frame(b, dt))) {
+    return r;
+  }
+  b->steps++;
+  return batch_host_frame(b, dt);
+}
+
+// records one frame of every tank into a hipGraph; sl_batch_step with the same dt replays it (one
+// graph launch per frame instead of 2*iters + ~25 kernel launches). Records nothing else: the
+// stream is idle before the capture.
+extern "C" int sl_batch_capture(sl_batch *b, double dt) {
+  if (!b) return batch_fail(nullptr, SL_EARG, "null batch");
+  if (!(dt > 0)) return batch_fail(b, SL_EARG, "sl_batch_capture: dt must be positive");
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, SL_EHIP, "sync before capture");
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+    return batch_fail(b, SL_EHIP, "begin capture failed");
+  int r = batch_encode_frame(b, dt);
+  hipGraph_t g = nullptr;
+  hipError_t e = hipStreamEndCapture(b->stream, &g);
+  if (r || e != hipSuccess || !g) {
+    if (g) (void)hipGraphDestroy(g);
+    (void)hipGetLastError();
+    return r ? r : batch_fail(b, SL_EHIP, "end capture failed");
+  }
+  b->graph = g;
+  if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
+    b->exec = nullptr;
+    return batch_fail(b, SL_EHIP, "graph instantiate failed");
+  }
+  b->graph_dt = dt;
+  return SL_OK;
+}
+
+extern "C" int sl_batch_sync(sl_batch *b) {
+  if (!b) return batch_fail(nullptr, SL_EARG, "null batch");
+  hipError_t e = hipStreamSynchronize(b->stream);
+  if (e != hipSuccess) {
+    for (sl_sim *s : b->sims) s->gpu_errors++;
+    return batch_fail(b, SL_EHIP, std::string("batch sync: ") + hipGetErrorString(e));
+  }
+  if (!b->steps) return SL_OK;
+  // counters of the last frame are in pinned memory: refresh every tank's host state
+  for (size_t t = 0; t < b->sims.size(); t++) {
+    sl_sim *s = b->sims[t];
+    *s->h_counters = b->h_counters[t];
+    apply_stats(s);
+  }
+  return SL_OK;
+}
+
+extern "C" int sl_batch_reorder(sl_batch *b) {
+  if (!b) return batch_fail(nullptr, SL_EARG, "null batch");
+  for (sl_sim *s : b->sims) {
+    int r = encode_reorder(s);
+    if (r) return batch_fail(b, r, s->err);
+  }
+  return SL_OK;
+}
+
+// Per size bucket, the fastest P2G and G2P variant on the live state of that bucket's tanks (all
+// variants give bit-identical results). Particles, counters, cell counts and ripple impulses are
+// snapshotted and restored, as sl_autotune does for one simulation.
+extern "C" int sl_batch_autotune(sl_batch *b, uint32_t reps, uint32_t *choice) {
+  if (!b) return batch_fail(nullptr, SL_EARG, "null batch");
+  if (!reps) reps = 2;
+  int r = sl_batch_sync(b);
+  if (r) return r;
+  struct Saved { void *live, *copy; size_t bytes; };
+  std::vector<Saved> items;
+  auto release = [&]() { for (auto &it : items) (void)hipFree(it.copy); };
+  for (sl_sim *s : b->sims) {
+    const SlTank &T = s->ht;
+    const int c = s->cur;
+    const size_t P = s->cap;
+    void *live[10] = {T.px[c], T.py[c], T.pz[c], T.vx[c], T.vy[c], T.vz[c], T.pf[c], T.C, T.cellCount, T.rsrc};
+    size_t by[10] = {P * 4, P * 4, P * 4, P * 4, P * 4, P * 4, P, sizeof(SlCounters), (size_t)s->cells * 4,
+                     s->R ? (size_t)(s->R + 1) * (s->R + 1) * 4 : 0};
+    for (int i = 0; i < 10; i++) {
+      if (!by[i]) continue;
+      void *p = nullptr;
+      if (hipMalloc(&p, by[i]) != hipSuccess) { release(); return batch_fail(b, SL_ENOMEM, "batch autotune: snapshot"); }
+      items.push_back({live[i], p, by[i]});
+      (void)hipMemcpyAsync(p, live[i], by[i], hipMemcpyDeviceToDevice, b->stream);
+    }
+  }
+  auto restore = [&]() {
+    for (auto &it : items) (void)hipMemcpyAsync(it.live, it.copy, it.bytes, hipMemcpyDeviceToDevice, b->stream);
+  };
+  auto clear_grids = [&]() {
+    for (sl_sim *s : b->sims) sl_k_fill_u32(b->stream, (uint32_t *)s->ht.acc, 0, (size_t)s->nodes * 6);
+    hipLaunchKernelGGL(k_batch_clear_cells, dim3(b->m_cells.total), dim3(SL_WG), 0, b->stream,
+                       batch_launch(b, b->m_cells).w);
+  };
+  auto timed = [&](auto fn) {
+    (void)hipEventRecord(b->ev_a, b->stream);
+    fn();
+    (void)hipEventRecord(b->ev_b, b->stream);
+    (void)hipEventSynchronize(b->ev_b);
+    float ms = 0;
+    (void)hipEventElapsedTime(&ms, b->ev_a, b->ev_b);
+    return ms;
+  };
+  const size_t n = b->sims.size();
+  std::vector<uint64_t> cap(n);
+  for (size_t t = 0; t < n; t++) cap[t] = b->sims[t]->cap;
+  sl_batch::Map m;  // scratch map, reused for every (bucket, variant)
+  for (int k = 0; k < SL_BUCKETS && !r; k++) {
+    std::vector<char> inc(n);
+    bool has = false;
+    for (size_t t = 0; t < n; t++) { inc[t] = b->bucket[t] == k; has |= inc[t] != 0; }
+    if (!has) continue;
+    float best = 1e30f;
+    for (int v = 0; v < 8 && !r; v++) {
+      if ((r = build_map(b, m, cap, sl_p2g_chunk(v), &inc))) break;
+      float tot = 0;
+      for (uint32_t rep = 0; rep <= reps; rep++) {
+        clear_grids();
+        float ms = timed([&] { sl_k_p2g(batch_launch(b, m), 0, v); });
+        if (rep) tot += ms;
+      }
+      if (tot < best) { best = tot; b->p2g_variant[k] = v; }
+    }
+    if (r) break;
+    if ((r = build_map(b, m, cap, SL_WG, &inc))) break;
+    best = 1e30f;
+    for (int v = 0; v < 4; v++) {
+      float tot = 0;
+      for (uint32_t rep = 0; rep <= reps; rep++) {
+        restore();
+        float ms = timed([&] { sl_k_g2p(batch_launch(b, m), v); });
+        if (rep) tot += ms;
+      }
+      if (tot < best) { best = tot; b->g2p_variant[k] = v; }
+    }
+  }
+  // accumulators cleared (the next P2G adds into them), everything else as it was
+  for (sl_sim *s : b->sims) sl_k_fill_u32(b->stream, (uint32_t *)s->ht.acc, 0, (size_t)s->nodes * 6);
+  restore();
+  hipError_t e = hipStreamSynchronize(b->stream);
+  release();
+  if (m.d) {
+    b->allocs.erase(std::find(b->allocs.begin(), b->allocs.end(), (void *)m.d));
+    (void)hipFree(m.d);
+  }
+  if (r) return r;
+  if (e != hipSuccess || (e = hipGetLastError()) != hipSuccess) return batch_fail(b, SL_EHIP, hipGetErrorString(e));
+  if ((r = batch_particle_maps(b))) return r;
+  // a captured frame holds the old launch shapes
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  if (choice)
+    for (int k = 0; k < SL_BUCKETS; k++) { choice[2 * k] = (uint32_t)b->p2g_variant[k]; choice[2 * k + 1] = (uint32_t)b->g2p_variant[k]; }
+  return SL_OK;
+}
+
+// sets every bucket's variants by hand (values as sl_batch_autotune reports them)
+extern "C" int sl_batch_set_variants(sl_batch *b, const uint32_t *choice) {
+  if (!b || !choice) return batch_fail(b, SL_EARG, "sl_batch_set_variants: bad arguments");
+  for (int k = 0; k < SL_BUCKETS; k++)
+    if (choice[2 * k] > 7 || choice[2 * k + 1] > 3) return batch_fail(b, SL_EARG, "variant out of range");
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, SL_EHIP, "sync");
+  for (int k = 0; k < SL_BUCKETS; k++) { b->p2g_variant[k] = (int)choice[2 * k]; b->g2p_variant[k] = (int)choice[2 * k + 1]; }
+  int r = batch_particle_maps(b);
+  if (r) return r;
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  return SL_OK;
 }
 
 extern "C" int sl_version(void) { return SL_ABI_VERSION; }

@@ -50,6 +50,7 @@ __global__ __launch_bounds__(256) void k_fft_rows(SlWork W, int which_in, int wh
   const SlTank &T = W.tanks[t];
   const uint32_t M = T.M, y = lb;
   if (y >= M) return;
+  if (mode == IN_MASK && !T.mask_bound) return;  // tanks of a batch without the absorbing mask
   __shared__ float2 lds[2][FFT_LDS_COMPLEX];
   const float2 *bufs[5] = {T.rstate, T.ra, T.rb, T.rhat, nullptr};
   float2 *obufs[5] = {T.rstate, T.ra, T.rb, T.rhat, nullptr};
@@ -75,6 +76,7 @@ __global__ __launch_bounds__(256) void k_fft_cols(SlWork W, int which_in, int wh
   if (CW > M) CW = M;
   const uint32_t x0 = lb * CW;
   if (x0 >= M) return;
+  if (omode == OUT_SCALE && !T.mask_bound) return;  // tanks of a batch without the absorbing mask
   __shared__ float2 lds[2][FFT_LDS_COMPLEX];
   const float2 *bufs[4] = {T.rstate, T.ra, T.rb, T.rhat};
   float2 *obufs[4] = {T.rstate, T.ra, T.rb, T.rhat};
@@ -163,6 +165,7 @@ __global__ __launch_bounds__(256) void k_rextract(SlWork W) {
 __global__ __launch_bounds__(256) void k_rmaskbuild(SlWork W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
+  if (!T.mask_bound) return;
   const uint32_t R = T.R, R1 = R + 1;
   uint32_t id = lb * 256 + threadIdx.x;
   if (id >= R1 * R1) return;

@@ -43,6 +43,8 @@ struct SlTank {
   // ripple uniform (64-ripple.js:223-226)
   float rLx, rLz, rdt, rmd, rsr, rnu, rfilm, rhpk, rrho;
   float pad_f[3];
+  // host-precision values for the device-side frame advance of batched steps
+  double time_d, h_d, vp_d, rim_d;
   // particles
   float *px[2], *py[2], *pz[2], *vx[2], *vy[2], *vz[2];
   uint8_t *pf[2];

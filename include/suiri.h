@@ -209,6 +209,31 @@ int sl_timing_read(sl_sim *sim, double *ms, uint32_t *launches, int n, int reset
    the selected P2G and G2P variants. No counterpart in the reference. */
 int sl_autotune(sl_sim *sim, uint32_t reps, uint32_t *choice);
 
+/* Batched tanks (no counterpart in the reference, which runs one tank). A batch steps many
+   independent simulations with one kernel launch per stage for all of them; each tank keeps the
+   single-simulation semantics (one substep of dt per frame, no emission inside sl_batch_step).
+   Per-tank set-up, emission between frames, readback and stats go through sl_batch_sim(). */
+typedef struct sl_batch sl_batch;
+int sl_batch_create(const sl_config *cfgs, uint32_t n, sl_batch **out);
+void sl_batch_destroy(sl_batch *batch);
+const char *sl_batch_last_error(const sl_batch *batch);
+uint32_t sl_batch_size(const sl_batch *batch);
+sl_sim *sl_batch_sim(sl_batch *batch, uint32_t i);
+/* one frame of every tank (frame++, P2G .. G2P, compaction, whitewater, ripple, counters copy);
+   the ripple mean depth comes from each tank's count at the start of the frame */
+int sl_batch_step(sl_batch *batch, double dt);
+/* records one frame into a hipGraph; sl_batch_step with the same dt then replays it */
+int sl_batch_capture(sl_batch *batch, double dt);
+/* waits for the batch and refreshes every tank's counters */
+int sl_batch_sync(sl_batch *batch);
+/* spatial re-ordering of every tank (drops a captured graph) */
+int sl_batch_reorder(sl_batch *batch);
+/* per size bucket (<= 64Ki cells, <= 256Ki, larger): fastest P2G / G2P variant on the live state;
+   choice (optional, 6 entries) receives (p2g, g2p) per bucket */
+int sl_batch_autotune(sl_batch *batch, uint32_t reps, uint32_t *choice);
+/* sets the (p2g, g2p) variants of the three buckets (6 entries, as sl_batch_autotune reports) */
+int sl_batch_set_variants(sl_batch *batch, const uint32_t *choice);
+
 /* ABI version + build description */
 int sl_version(void);
 const char *sl_build_info(void);
