@@ -77,7 +77,8 @@ EXPORTS = ["sl_config_default", "sl_create", "sl_destroy", "sl_last_error", "sl_
            "sl_version", "sl_build_info", "sl_reorder", "sl_autotune",
            "sl_batch_create", "sl_batch_destroy", "sl_batch_last_error", "sl_batch_size", "sl_batch_sim",
            "sl_batch_step", "sl_batch_capture", "sl_batch_sync", "sl_batch_reorder", "sl_batch_autotune",
-           "sl_batch_set_variants"]
+           "sl_batch_set_variants", "sl_batch_timing_enable", "sl_batch_timing_read", "sl_batch_open",
+           "sl_batch_add", "sl_batch_remove", "sl_batch_set_stepped", "sl_batch_emit", "sl_batch_captured"]
 
 _lib = None
 
@@ -137,6 +138,14 @@ def load():
         "sl_batch_reorder": (I, [P]),
         "sl_batch_autotune": (I, [P, U, ctypes.POINTER(U)]),
         "sl_batch_set_variants": (I, [P, ctypes.POINTER(U)]),
+        "sl_batch_timing_enable": (I, [P, I]),
+        "sl_batch_open": (I, [U, I, ctypes.POINTER(P)]),
+        "sl_batch_add": (I, [P, ctypes.POINTER(SlConfig), ctypes.POINTER(U)]),
+        "sl_batch_remove": (I, [P, U]),
+        "sl_batch_set_stepped": (I, [P, U, I]),
+        "sl_batch_emit": (I, [P, P, ctypes.POINTER(U)]),
+        "sl_batch_captured": (I, [P]),
+        "sl_batch_timing_read": (I, [P, ctypes.POINTER(D), U, ctypes.POINTER(U)]),
         "sl_version": (I, []),
         "sl_build_info": (ctypes.c_char_p, []),
     }
@@ -311,19 +320,28 @@ class Sim:
 
 
 class Batch:
-    """Many independent tanks stepped together, one kernel launch per stage for all of them
-    (sl_batch_*). Per-tank set-up, emission between frames and readback go through `tanks[i]`, a
-    Sim on the batch's stream."""
+    """A table of tank slots stepped together, one kernel launch per stage for all stepped tanks
+    (sl_batch_*). `Batch(cfgs)` fills one slot per configuration; `Batch(capacity=n)` starts empty.
+    Per-tank set-up, single emissions and readback go through `tanks[slot]`, a Sim on the batch's
+    stream (None for an empty slot)."""
 
-    def __init__(self, cfgs):
+    def __init__(self, cfgs=None, capacity=None, device=0):
         self.L = load()
         self.ptr = ctypes.c_void_p()
-        arr = (SlConfig * len(cfgs))(*cfgs)
-        r = self.L.sl_batch_create(arr, len(cfgs), ctypes.byref(self.ptr))
+        if cfgs is not None:
+            arr = (SlConfig * max(len(cfgs), 1))(*cfgs)
+            r = self.L.sl_batch_create(arr, len(cfgs), ctypes.byref(self.ptr))
+        else:
+            r = self.L.sl_batch_open(int(capacity), int(device), ctypes.byref(self.ptr))
         if r != SL_OK:
             msg = self.L.sl_batch_last_error(None)
             raise SuiriError(f"suiri error {r}: {msg.decode() if msg else ''}")
-        self.tanks = [Sim(None, _handle=self.L.sl_batch_sim(self.ptr, i), _owner=self) for i in range(len(cfgs))]
+        self.capacity = self.L.sl_batch_size(self.ptr)
+        self.tanks = [None] * self.capacity
+        for i in range(self.capacity):
+            h = self.L.sl_batch_sim(self.ptr, i)
+            if h:
+                self.tanks[i] = Sim(None, _handle=h, _owner=self)
 
     def _check(self, r):
         if r != SL_OK:
@@ -331,13 +349,43 @@ class Batch:
             raise SuiriError(f"suiri error {r}: {msg.decode() if msg else ''}")
 
     def __len__(self):
-        return len(self.tanks)
+        return self.capacity
+
+    def add(self, cfg):
+        slot = ctypes.c_uint32(0)
+        self._check(self.L.sl_batch_add(self.ptr, ctypes.byref(cfg), ctypes.byref(slot)))
+        self.tanks[slot.value] = Sim(None, _handle=self.L.sl_batch_sim(self.ptr, slot.value), _owner=self)
+        return slot.value
+
+    def remove(self, slot):
+        t = self.tanks[slot]
+        if t is not None:
+            t.ptr = ctypes.c_void_p()
+        self.tanks[slot] = None
+        self._check(self.L.sl_batch_remove(self.ptr, int(slot)))
+
+    def set_stepped(self, slot, on):
+        self._check(self.L.sl_batch_set_stepped(self.ptr, int(slot), int(bool(on))))
+
+    def emit(self, per_slot):
+        """per_slot: {slot: [emission records]} for one emission into many tanks."""
+        counts = (ctypes.c_uint32 * self.capacity)()
+        recs = []
+        for slot in sorted(per_slot):
+            counts[slot] = len(per_slot[slot])
+            recs.extend(per_slot[slot])
+        arr = emit_records(recs)
+        self._check(self.L.sl_batch_emit(self.ptr, ctypes.byref(arr), counts))
 
     def step(self, dt):
         self._check(self.L.sl_batch_step(self.ptr, float(dt)))
 
     def capture(self, dt):
         self._check(self.L.sl_batch_capture(self.ptr, float(dt)))
+
+    @property
+    def captured(self):
+        return bool(self.L.sl_batch_captured(self.ptr))
 
     def sync(self):
         self._check(self.L.sl_batch_sync(self.ptr))
@@ -354,9 +402,20 @@ class Batch:
         ch = (ctypes.c_uint32 * 6)(*[int(v) for pr in pairs for v in pr])
         self._check(self.L.sl_batch_set_variants(self.ptr, ch))
 
+    def timing(self, enable=True):
+        self._check(self.L.sl_batch_timing_enable(self.ptr, int(enable)))
+
+    def step_times(self, max_steps=100000):
+        """Device time [ms] of each step since timing() or the last call, oldest first."""
+        buf = (ctypes.c_double * max_steps)()
+        n = ctypes.c_uint32(0)
+        self._check(self.L.sl_batch_timing_read(self.ptr, buf, max_steps, ctypes.byref(n)))
+        return np.array(buf[:n.value])
+
     def close(self):
         for t in getattr(self, "tanks", []):
-            t.ptr = ctypes.c_void_p()
+            if t is not None:
+                t.ptr = ctypes.c_void_p()
         if self.ptr:
             self.L.sl_batch_destroy(self.ptr)
             self.ptr = ctypes.c_void_p()

@@ -50,6 +50,7 @@ struct sl_sim {
   hipStream_t ext_stream = nullptr;
   SlTank *ext_tank = nullptr;
   SlCounters *ext_counters = nullptr;
+  SlEmitDev *ext_emits = nullptr;
   int device = 0;
   hipStream_t stream = nullptr;
   std::string err;
@@ -201,6 +202,7 @@ static SlLaunch launch_of(sl_sim *s, uint64_t items, uint32_t per_wg = SL_WG) {
   L.w.start = nullptr;
   L.w.ntanks = 1;
   L.w.total = cdiv(items, per_wg);
+  L.w.list = nullptr;
   L.stream = s->stream;
   return L;
 }
@@ -343,7 +345,8 @@ static int create_impl(sl_sim *s) {
   ALLOC(T.divg, C * 4);
   ALLOC(T.pres, H * 8); ALLOC(T.rhs, H * 8); ALLOC(T.sflag, H * 4);
   ALLOC(T.objs, sizeof(SlObj) * SL_MAX_OBJ);
-  ALLOC(T.emits, sizeof(SlEmitDev) * SL_MAX_EMIT);
+  if (s->ext_emits) T.emits = s->ext_emits;
+  else ALLOC(T.emits, sizeof(SlEmitDev) * SL_MAX_EMIT);
   ALLOC(T.scanBlocks, (size_t)cdiv(s->cap, SL_COMPACT_ITEMS) * 4 + 64);
   if (s->R) {
     const size_t M2 = (size_t)s->M * s->M, R1 = (size_t)(s->R + 1) * (s->R + 1);
@@ -1215,21 +1218,27 @@ extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
 
 
 // ================================================================================================
-// Batched tanks: many independent simulations stepped together with one launch per stage for all
-// of them, optionally replayed from a captured hipGraph. No counterpart in the reference (it runs
-// one tank per page). Each tank follows the single-simulation semantics of one substep of dt per
-// frame; every kernel finds its tank through the work map and reads the tank's buffer index,
-// counts and parameters from the device table, so a captured frame stays valid across frames,
-// re-orderings and emissions.
+// Batched tanks: many independent simulations in one tank table, stepped with one launch per stage
+// for all of them and optionally replayed from a captured hipGraph. No counterpart in the reference
+// (it runs one tank per page). A tank of a batch keeps the single-simulation semantics of one
+// substep of dt per frame. Kernels find their tank through work maps (prefix tables of workgroup
+// counts per table slot) and read the tank's buffer index, counts and parameters from the device
+// table, so a captured frame stays valid across frames, emissions, object moves and re-orderings.
+// Slots are added and removed while the batch runs (continuous batching); per-tank kernels walk
+// the list of stepped slots.
 // ================================================================================================
 #define SL_BUCKETS 3
+enum {
+  BM_NODES, BM_CELLS, BM_HCELLS, BM_COMPACT, BM_WW, BM_ELEMS, BM_TEX, BM_ROWS, BM_COLS,
+  BM_P2G0, BM_G2P0 = BM_P2G0 + SL_BUCKETS, BM_SCRATCH = BM_G2P0 + SL_BUCKETS, BM_COUNT
+};
 
-// frame++, the per-substep uniform (50-sim.js:151-162, :308-317) and the ripple uniform of the frame
-// (64-ripple.js:223-226, mean depth as 50-sim.js:330-333 computes it, from the count at frame start)
-__global__ void k_batch_advance(SlTank *tanks, int n, float dt, double dtd) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= n) return;
-  SlTank &T = tanks[t];
+// frame++ and the per-substep uniform (50-sim.js:151-162, :308-317), and the frame's ripple uniform
+// (64-ripple.js:223-226) with the mean depth of 50-sim.js:330-333 from the count at frame start
+__global__ void k_batch_advance(SlTank *tanks, const uint32_t *list, int n, float dt, double dtd) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  SlTank &T = tanks[list[i]];
   T.frame += 1;
   T.dt = dt;
   T.gdt = T.g * T.dt;
@@ -1255,26 +1264,50 @@ __global__ __launch_bounds__(SL_WG) void k_batch_clear_cells(SlWork W) {
   uint32_t c = lb * SL_WG + threadIdx.x;
   if (c < T.cells) T.cellCount[c] = 0u;
 }
+// kt[2 slot] = emission records, kt[2 slot + 1] = particles requested (0: the slot does not emit)
+__global__ void k_batch_emit_setup(SlTank *tanks, const uint32_t *kt, int n) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n || !kt[2 * t + 1]) return;
+  tanks[t].emit_k = kt[2 * t];
+  tanks[t].emit_total = kt[2 * t + 1];
+}
 
 struct sl_batch {
-  std::vector<sl_sim *> sims;
+  uint32_t cap = 0;  // table slots
+  int device = 0;
+  std::vector<sl_sim *> sims;   // per slot (nullptr: empty)
+  std::vector<char> stepped;    // per slot: advanced by sl_batch_step
+  std::vector<char> copy_valid; // per slot: the last counters copy holds this tank
+  std::vector<int> bucket;      // per slot: size bucket for the variant choice
   hipStream_t stream = nullptr;
   SlTank *d_tanks = nullptr;
   SlCounters *d_counters = nullptr, *h_counters = nullptr;
-  std::vector<void *> allocs;
-  std::string err;
-  struct Map { uint32_t *d = nullptr; uint32_t total = 0; int variant = -1; };
-  Map m_nodes, m_cells, m_hcells, m_compact, m_ww, m_elems, m_tex, m_rows, m_cols;
-  Map m_p2g[SL_BUCKETS], m_g2p[SL_BUCKETS];  // one per distinct chosen variant (variant -1: unused)
+  // a device block with two pinned host copies, written by stream-ordered uploads (kernels of
+  // earlier frames keep reading the previous contents)
+  struct Ring {
+    uint8_t *d = nullptr, *h[2] = {nullptr, nullptr};
+    hipEvent_t ev[2] = {nullptr, nullptr};
+    bool valid[2] = {false, false};
+    int cur = 0;
+    size_t bytes = 0;
+  } maps, emit;
+  // maps block: BM_COUNT prefix tables of cap+1 entries, then the stepped-slot list (cap entries)
+  std::vector<uint32_t> maps_host;
+  uint32_t total[BM_COUNT] = {};
+  int map_variant[BM_COUNT];
+  uint32_t n_stepped = 0, max_iters = 0;
+  bool any_ripple = false, maps_dirty = true;
   int p2g_variant[SL_BUCKETS] = {0, 0, 0}, g2p_variant[SL_BUCKETS] = {1, 1, 1};
-  std::vector<int> bucket;                   // size bucket of each tank
-  uint32_t max_iters = 0;
-  bool any_ripple = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double graph_dt = 0;
   hipEvent_t ev_a = nullptr, ev_b = nullptr;
   uint64_t steps = 0;
+  bool timing = false;  // per-step events while enabled
+  std::vector<hipEvent_t> tev;
+  size_t tev_used = 0;
+  std::vector<void *> allocs;
+  std::string err;
 };
 
 static int batch_fail(sl_batch *b, int code, const std::string &msg) {
@@ -1286,193 +1319,407 @@ static int batch_fail(sl_batch *b, int code, const std::string &msg) {
 // size buckets for the variant choice: <= 64Ki cells, <= 256Ki cells, larger
 static int bucket_of(const sl_sim *s) { return s->cells <= 65536 ? 0 : (s->cells <= 262144 ? 1 : 2); }
 
-// device prefix of workgroup counts per tank; tanks with include[t] == 0 get none
-static int build_map(sl_batch *b, sl_batch::Map &m, const std::vector<uint64_t> &items, uint32_t per_wg,
-                     const std::vector<char> *include = nullptr) {
-  const size_t n = b->sims.size();
-  std::vector<uint32_t> start(n + 1, 0);
-  for (size_t t = 0; t < n; t++) {
-    uint64_t wg = (include && !(*include)[t]) ? 0 : (items[t] + per_wg - 1) / per_wg;
-    if (start[t] + wg > 0x7fffffffull) return batch_fail(b, SL_ECAP, "batch: launch too large");
-    start[t + 1] = start[t] + (uint32_t)wg;
+static int ring_init(sl_batch *b, sl_batch::Ring &r, size_t bytes) {
+  r.bytes = bytes;
+  if (hipMalloc((void **)&r.d, bytes) != hipSuccess) return batch_fail(b, SL_ENOMEM, "batch: device block");
+  b->allocs.push_back(r.d);
+  if (hipMemset(r.d, 0, bytes) != hipSuccess) return batch_fail(b, SL_EHIP, "batch: clear");
+  for (int i = 0; i < 2; i++) {
+    if (hipHostMalloc((void **)&r.h[i], bytes, hipHostMallocDefault) != hipSuccess)
+      return batch_fail(b, SL_ENOMEM, "batch: pinned block");
+    memset(r.h[i], 0, bytes);
+    if (hipEventCreateWithFlags(&r.ev[i], hipEventDisableTiming) != hipSuccess) return batch_fail(b, SL_EHIP, "event");
   }
-  if (!m.d) {
-    if (hipMalloc((void **)&m.d, (n + 1) * 4) != hipSuccess) return batch_fail(b, SL_ENOMEM, "batch: map alloc");
-    b->allocs.push_back(m.d);
-  }
-  if (hipMemcpy(m.d, start.data(), (n + 1) * 4, hipMemcpyHostToDevice) != hipSuccess)
-    return batch_fail(b, SL_EHIP, "batch: map upload");
-  m.total = start[n];
   return SL_OK;
 }
-static SlLaunch batch_launch(const sl_batch *b, const sl_batch::Map &m) {
+static void ring_free(sl_batch::Ring &r) {
+  for (int i = 0; i < 2; i++) {
+    if (r.h[i]) (void)hipHostFree(r.h[i]);
+    if (r.ev[i]) (void)hipEventDestroy(r.ev[i]);
+  }
+}
+// the host copy that may be written now (its previous upload has completed)
+static uint8_t *ring_host(sl_batch::Ring &r) {
+  if (r.valid[r.cur]) { (void)hipEventSynchronize(r.ev[r.cur]); r.valid[r.cur] = false; }
+  return r.h[r.cur];
+}
+static int ring_push(sl_batch *b, sl_batch::Ring &r, size_t bytes) {
+  hipError_t e = hipMemcpyAsync(r.d, r.h[r.cur], bytes, hipMemcpyHostToDevice, b->stream);
+  if (e == hipSuccess) e = hipEventRecord(r.ev[r.cur], b->stream);
+  if (e != hipSuccess) return batch_fail(b, SL_EHIP, std::string("batch upload: ") + hipGetErrorString(e));
+  r.valid[r.cur] = true;
+  r.cur ^= 1;
+  return SL_OK;
+}
+
+static uint32_t *map_dev(const sl_batch *b, int which) { return (uint32_t *)b->maps.d + (size_t)which * (b->cap + 1); }
+static const uint32_t *list_dev(const sl_batch *b) { return map_dev(b, BM_COUNT); }
+static SlLaunch batch_launch(const sl_batch *b, int which) {
   SlLaunch L;
   L.w.tanks = b->d_tanks;
-  L.w.start = m.d;
-  L.w.ntanks = (int32_t)b->sims.size();
-  L.w.total = m.total;
+  L.w.start = map_dev(b, which);
+  L.w.ntanks = (int32_t)b->cap;
+  L.w.total = b->total[which];
+  L.w.list = nullptr;
   L.stream = b->stream;
   return L;
 }
+// per-tank kernels: one thread / workgroup per stepped slot
+static SlLaunch batch_tanks(const sl_batch *b) {
+  SlLaunch L;
+  L.w.tanks = b->d_tanks;
+  L.w.start = nullptr;
+  L.w.ntanks = (int32_t)b->n_stepped;
+  L.w.total = 0;
+  L.w.list = list_dev(b);
+  L.stream = b->stream;
+  return L;
+}
+static bool live(const sl_batch *b, uint32_t t) { return b->sims[t] && b->stepped[t]; }
 
-// P2G / G2P maps: one per distinct variant chosen by the buckets that hold tanks
-static int batch_particle_maps(sl_batch *b) {
-  const size_t n = b->sims.size();
-  std::vector<uint64_t> cap(n), ww(n);
-  for (size_t t = 0; t < n; t++) { cap[t] = b->sims[t]->cap; ww[t] = b->sims[t]->ww_cap; }
+// one prefix table in the host copy of the maps block; `inc` selects the slots that get workgroups
+template <class Items, class Inc>
+static int put_map(sl_batch *b, int which, Items items, uint32_t per_wg, Inc inc) {
+  uint32_t *m = b->maps_host.data() + (size_t)which * (b->cap + 1);
+  uint64_t acc = 0;
+  for (uint32_t t = 0; t < b->cap; t++) {
+    m[t] = (uint32_t)acc;
+    if (live(b, t) && inc(t)) acc += ((uint64_t)items(b->sims[t]) + per_wg - 1) / per_wg;
+    if (acc > 0x7fffffffull) return batch_fail(b, SL_ECAP, "batch: launch too large");
+  }
+  m[b->cap] = (uint32_t)acc;
+  b->total[which] = (uint32_t)acc;
+  return SL_OK;
+}
+static int maps_upload(sl_batch *b) {
+  const size_t bytes = b->maps_host.size() * 4;
+  memcpy(ring_host(b->maps), b->maps_host.data(), bytes);
+  return ring_push(b, b->maps, bytes);
+}
+static void drop_graph(sl_batch *b) {
+  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
+  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+}
+static uint32_t cols_of(const sl_sim *s) {
+  const uint32_t M = s->M;
+  if (!M) return 0;
+  uint32_t CW = 4096 / M;
+  if (CW > M) CW = M;
+  if (CW < 1) CW = 1;
+  return (M + CW - 1) / CW;
+}
+
+// every work map and the stepped-slot list from the current membership and variant choice
+static int batch_rebuild(sl_batch *b) {
+  std::fill(b->maps_host.begin(), b->maps_host.end(), 0u);
+  uint32_t *list = b->maps_host.data() + (size_t)BM_COUNT * (b->cap + 1);
+  b->n_stepped = 0;
+  b->max_iters = 0;
+  b->any_ripple = false;
+  for (uint32_t t = 0; t < b->cap; t++) {
+    if (!live(b, t)) continue;
+    list[b->n_stepped++] = t;
+    b->max_iters = std::max(b->max_iters, b->sims[t]->iters);
+    if (b->sims[t]->R) b->any_ripple = true;
+  }
+  auto all = [](uint32_t) { return true; };
+  int r;
+  if ((r = put_map(b, BM_NODES, [](const sl_sim *s) { return (uint64_t)s->nodes; }, SL_WG, all)) ||
+      (r = put_map(b, BM_CELLS, [](const sl_sim *s) { return (uint64_t)s->cells; }, SL_WG, all)) ||
+      (r = put_map(b, BM_HCELLS, [](const sl_sim *s) { return (uint64_t)s->hcells; }, SL_WG, all)) ||
+      (r = put_map(b, BM_COMPACT, [](const sl_sim *s) { return (uint64_t)s->cap; }, SL_COMPACT_ITEMS, all)) ||
+      (r = put_map(b, BM_WW, [](const sl_sim *s) { return (uint64_t)s->ww_cap; }, SL_WG, all)) ||
+      (r = put_map(b, BM_ELEMS, [](const sl_sim *s) { return (uint64_t)s->M * s->M; }, SL_WG, all)) ||
+      (r = put_map(b, BM_TEX, [](const sl_sim *s) { return s->R ? (uint64_t)(s->R + 1) * (s->R + 1) : 0; }, SL_WG, all)) ||
+      (r = put_map(b, BM_ROWS, [](const sl_sim *s) { return (uint64_t)s->M; }, 1, all)) ||
+      (r = put_map(b, BM_COLS, [](const sl_sim *s) { return (uint64_t)cols_of(s); }, 1, all)))
+    return r;
+  // P2G / G2P: one map per distinct variant among the buckets that hold stepped tanks
   for (int pass = 0; pass < 2; pass++) {
     const int *var = pass ? b->g2p_variant : b->p2g_variant;
-    sl_batch::Map *maps = pass ? b->m_g2p : b->m_p2g;
-    for (int k = 0; k < SL_BUCKETS; k++) { maps[k].variant = -1; maps[k].total = 0; }
+    const int base = pass ? BM_G2P0 : BM_P2G0;
     int used = 0;
+    for (int k = 0; k < SL_BUCKETS; k++) { b->map_variant[base + k] = -1; b->total[base + k] = 0; }
     for (int k = 0; k < SL_BUCKETS; k++) {
       bool has = false, seen = false;
-      for (size_t t = 0; t < n; t++) has |= b->bucket[t] == k;
-      for (int u = 0; u < used; u++) seen |= maps[u].variant == var[k];
+      for (uint32_t t = 0; t < b->cap; t++) has |= live(b, t) && b->bucket[t] == k;
+      for (int u = 0; u < used; u++) seen |= b->map_variant[base + u] == var[k];
       if (!has || seen) continue;
-      std::vector<char> inc(n);
-      for (size_t t = 0; t < n; t++) inc[t] = var[b->bucket[t]] == var[k];
-      int r = build_map(b, maps[used], cap, pass ? SL_WG : sl_p2g_chunk(var[k]), &inc);
+      const int v = var[k];
+      r = put_map(b, base + used, [](const sl_sim *s) { return (uint64_t)s->cap; }, pass ? SL_WG : sl_p2g_chunk(v),
+                  [&](uint32_t t) { return var[b->bucket[t]] == v; });
       if (r) return r;
-      maps[used++].variant = var[k];
+      b->map_variant[base + used++] = v;
     }
   }
-  int r = build_map(b, b->m_compact, cap, SL_COMPACT_ITEMS);
-  return r ? r : build_map(b, b->m_ww, ww, SL_WG);
+  b->total[BM_SCRATCH] = 0;
+  drop_graph(b);  // a captured frame holds the old launch shapes
+  b->maps_dirty = false;
+  return maps_upload(b);
 }
 
 extern "C" void sl_batch_destroy(sl_batch *b) {
   if (!b) return;
   if (b->stream) (void)hipStreamSynchronize(b->stream);
-  if (b->exec) (void)hipGraphExecDestroy(b->exec);
-  if (b->graph) (void)hipGraphDestroy(b->graph);
-  for (sl_sim *s : b->sims) sl_destroy(s);
+  drop_graph(b);
+  for (sl_sim *s : b->sims) if (s) sl_destroy(s);
   for (void *p : b->allocs) (void)hipFree(p);
+  ring_free(b->maps);
+  ring_free(b->emit);
   if (b->h_counters) (void)hipHostFree(b->h_counters);
+  for (hipEvent_t e : b->tev) (void)hipEventDestroy(e);
   if (b->ev_a) (void)hipEventDestroy(b->ev_a);
   if (b->ev_b) (void)hipEventDestroy(b->ev_b);
   if (b->stream) (void)hipStreamDestroy(b->stream);
   delete b;
 }
 
-extern "C" int sl_batch_create(const sl_config *cfgs, uint32_t n, sl_batch **out) {
+extern "C" int sl_batch_open(uint32_t capacity, int device, sl_batch **out) {
   if (out) *out = nullptr;
-  if (!cfgs || !out || !n) return batch_fail(nullptr, SL_EARG, "sl_batch_create: bad arguments");
-  for (uint32_t t = 1; t < n; t++)
-    if (cfgs[t].device != cfgs[0].device) return batch_fail(nullptr, SL_EARG, "sl_batch_create: tanks on several devices");
+  if (!out || !capacity || capacity > 65536) return batch_fail(nullptr, SL_EARG, "sl_batch_open: bad arguments");
   sl_batch *b = new (std::nothrow) sl_batch();
   if (!b) return batch_fail(nullptr, SL_ENOMEM, "out of host memory");
   auto fail = [&](int code, const std::string &msg) {
     sl_batch_destroy(b);
     return batch_fail(nullptr, code, msg);
   };
-  if (hipSetDevice(cfgs[0].device) != hipSuccess) return fail(SL_EHIP, "sl_batch_create: no such device");
+  const uint32_t n = capacity;
+  b->cap = n;
+  b->device = device;
+  b->sims.assign(n, nullptr);
+  b->stepped.assign(n, 0);
+  b->copy_valid.assign(n, 0);
+  b->bucket.assign(n, 0);
+  for (int i = 0; i < BM_COUNT; i++) b->map_variant[i] = -1;
+  if (hipSetDevice(device) != hipSuccess) return fail(SL_EHIP, "sl_batch_open: no such device");
   if (hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking) != hipSuccess) return fail(SL_EHIP, "stream");
   if (hipMalloc((void **)&b->d_tanks, sizeof(SlTank) * n) != hipSuccess) return fail(SL_ENOMEM, "tank table");
   b->allocs.push_back(b->d_tanks);
+  if (hipMemset(b->d_tanks, 0, sizeof(SlTank) * n) != hipSuccess) return fail(SL_EHIP, "tank table");
   if (hipMalloc((void **)&b->d_counters, sizeof(SlCounters) * n) != hipSuccess) return fail(SL_ENOMEM, "counters");
   b->allocs.push_back(b->d_counters);
   if (hipMemset(b->d_counters, 0, sizeof(SlCounters) * n) != hipSuccess) return fail(SL_EHIP, "counters");
   if (hipHostMalloc((void **)&b->h_counters, sizeof(SlCounters) * n, hipHostMallocDefault) != hipSuccess)
     return fail(SL_ENOMEM, "pinned counters");
   memset(b->h_counters, 0, sizeof(SlCounters) * n);
-  for (uint32_t t = 0; t < n; t++) {
-    sl_sim *s = new (std::nothrow) sl_sim();
-    if (!s) return fail(SL_ENOMEM, "out of host memory");
-    s->cfg = cfgs[t];
-    s->ext_stream = b->stream;
-    s->ext_tank = b->d_tanks + t;
-    s->ext_counters = b->d_counters + t;
-    int r = create_impl(s);
-    if (r) {
-      std::string m = "tank " + std::to_string(t) + ": " + s->err;
-      sl_destroy(s);
-      return fail(r, m);
-    }
-    b->sims.push_back(s);
-    b->bucket.push_back(bucket_of(s));
-    b->max_iters = std::max(b->max_iters, s->iters);
-    if (s->R) b->any_ripple = true;
-  }
-  std::vector<uint64_t> nodes(n), cells(n), hcells(n), rows(n), cols(n), elems(n), tex(n);
-  for (uint32_t t = 0; t < n; t++) {
-    const sl_sim *s = b->sims[t];
-    nodes[t] = s->nodes; cells[t] = s->cells; hcells[t] = s->hcells;
-    const uint32_t M = s->M;
-    uint32_t CW = M ? 4096 / M : 1;
-    if (CW > M) CW = M;
-    if (CW < 1) CW = 1;
-    rows[t] = M; cols[t] = M ? (M + CW - 1) / CW : 0;
-    elems[t] = (uint64_t)M * M;
-    tex[t] = s->R ? (uint64_t)(s->R + 1) * (s->R + 1) : 0;
-  }
+  b->maps_host.assign((size_t)BM_COUNT * (n + 1) + n, 0u);
   int r;
-  if ((r = build_map(b, b->m_nodes, nodes, SL_WG)) || (r = build_map(b, b->m_cells, cells, SL_WG)) ||
-      (r = build_map(b, b->m_hcells, hcells, SL_WG)) || (r = build_map(b, b->m_rows, rows, 1)) ||
-      (r = build_map(b, b->m_cols, cols, 1)) || (r = build_map(b, b->m_elems, elems, SL_WG)) ||
-      (r = build_map(b, b->m_tex, tex, SL_WG)) || (r = batch_particle_maps(b)))
-    return fail(r, b->err);
-  if (hipEventCreate(&b->ev_a) != hipSuccess || hipEventCreate(&b->ev_b) != hipSuccess)
-    return fail(SL_EHIP, "events");
-  if (hipStreamSynchronize(b->stream) != hipSuccess) return fail(SL_EHIP, "sl_batch_create: sync");
+  if ((r = ring_init(b, b->maps, b->maps_host.size() * 4))) return fail(r, b->err);
+  // emission block: SL_MAX_EMIT records per slot, then kt[2n], the map [n+1] and the emitting slots [n]
+  if ((r = ring_init(b, b->emit, sizeof(SlEmitDev) * SL_MAX_EMIT * n + 4 * (4 * (size_t)n + 1)))) return fail(r, b->err);
+  if (hipEventCreate(&b->ev_a) != hipSuccess || hipEventCreate(&b->ev_b) != hipSuccess) return fail(SL_EHIP, "events");
+  *out = b;
+  return SL_OK;
+}
+
+extern "C" int sl_batch_add(sl_batch *b, const sl_config *cfg, uint32_t *slot) {
+  if (!b || !cfg) return batch_fail(b, SL_EARG, "sl_batch_add: bad arguments");
+  if (cfg->device != b->device) return batch_fail(b, SL_EARG, "sl_batch_add: tank on another device");
+  uint32_t t = 0;
+  while (t < b->cap && b->sims[t]) t++;
+  if (t == b->cap) return batch_fail(b, SL_ECAP, "sl_batch_add: every slot is taken");
+  // work in flight reads the table: the new record is written with the stream idle
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, SL_EHIP, "sl_batch_add: sync");
+  if (hipMemset(b->d_counters + t, 0, sizeof(SlCounters)) != hipSuccess) return batch_fail(b, SL_EHIP, "counters");
+  sl_sim *s = new (std::nothrow) sl_sim();
+  if (!s) return batch_fail(b, SL_ENOMEM, "out of host memory");
+  s->cfg = *cfg;
+  s->ext_stream = b->stream;
+  s->ext_tank = b->d_tanks + t;
+  s->ext_counters = b->d_counters + t;
+  s->ext_emits = (SlEmitDev *)b->emit.d + (size_t)t * SL_MAX_EMIT;
+  int r = create_impl(s);
+  if (r) {
+    std::string m = s->err;
+    sl_destroy(s);
+    return batch_fail(b, r, m);
+  }
+  b->sims[t] = s;
+  b->stepped[t] = 1;
+  b->copy_valid[t] = 0;
+  b->bucket[t] = bucket_of(s);
+  b->maps_dirty = true;
+  if (slot) *slot = t;
+  return SL_OK;
+}
+
+extern "C" int sl_batch_remove(sl_batch *b, uint32_t slot) {
+  if (!b || slot >= b->cap || !b->sims[slot]) return batch_fail(b, SL_EARG, "sl_batch_remove: no tank in that slot");
+  sl_destroy(b->sims[slot]);  // waits for the stream first
+  b->sims[slot] = nullptr;
+  b->stepped[slot] = 0;
+  b->copy_valid[slot] = 0;
+  b->maps_dirty = true;
+  return SL_OK;
+}
+
+extern "C" int sl_batch_set_stepped(sl_batch *b, uint32_t slot, int on) {
+  if (!b || slot >= b->cap || !b->sims[slot]) return batch_fail(b, SL_EARG, "sl_batch_set_stepped: no tank in that slot");
+  if ((b->stepped[slot] != 0) != (on != 0)) {
+    b->stepped[slot] = on ? 1 : 0;
+    b->maps_dirty = true;
+  }
+  return SL_OK;
+}
+
+extern "C" int sl_batch_create(const sl_config *cfgs, uint32_t n, sl_batch **out) {
+  if (out) *out = nullptr;
+  if (!cfgs || !out || !n) return batch_fail(nullptr, SL_EARG, "sl_batch_create: bad arguments");
+  sl_batch *b = nullptr;
+  int r = sl_batch_open(n, cfgs[0].device, &b);
+  if (r) return r;
+  for (uint32_t t = 0; t < n; t++) {
+    if ((r = sl_batch_add(b, cfgs + t, nullptr))) {
+      std::string m = "tank " + std::to_string(t) + ": " + b->err;
+      sl_batch_destroy(b);
+      return batch_fail(nullptr, r, m);
+    }
+  }
   *out = b;
   return SL_OK;
 }
 
 extern "C" const char *sl_batch_last_error(const sl_batch *b) { return b ? b->err.c_str() : g_err.c_str(); }
-extern "C" uint32_t sl_batch_size(const sl_batch *b) { return b ? (uint32_t)b->sims.size() : 0; }
-extern "C" sl_sim *sl_batch_sim(sl_batch *b, uint32_t i) { return b && i < b->sims.size() ? b->sims[i] : nullptr; }
+extern "C" uint32_t sl_batch_size(const sl_batch *b) { return b ? b->cap : 0; }
+extern "C" sl_sim *sl_batch_sim(sl_batch *b, uint32_t slot) { return b && slot < b->cap ? b->sims[slot] : nullptr; }
 
-// the kernels of one frame of every tank, in stream order (no host synchronisation, no memset: the
-// sequence can be captured)
+// Emission into many tanks with one upload and three launches: counts[slot] records per slot
+// (recs holds them concatenated in slot order). The budget of writeEmits (cap - count - 64,
+// 50-sim.js:222-245) is applied on the device count, so the result does not depend on when the
+// host last read the counters.
+extern "C" int sl_batch_emit(sl_batch *b, const sl_emit *recs, const uint32_t *counts) {
+  if (!b || !counts) return batch_fail(b, SL_EARG, "sl_batch_emit: bad arguments");
+  const uint32_t n = b->cap;
+  size_t nrec = 0;
+  for (uint32_t t = 0; t < n; t++) {
+    if (counts[t] && !b->sims[t]) return batch_fail(b, SL_EARG, "sl_batch_emit: records for an empty slot");
+    nrec += counts[t];
+  }
+  if (nrec && !recs) return batch_fail(b, SL_EARG, "sl_batch_emit: null records");
+  if (!nrec) return SL_OK;
+  uint8_t *h = ring_host(b->emit);
+  const size_t eb = sizeof(SlEmitDev) * SL_MAX_EMIT * n;
+  SlEmitDev *he = (SlEmitDev *)h;
+  uint32_t *kt = (uint32_t *)(h + eb), *emap = kt + 2 * (size_t)n, *elist = emap + n + 1;
+  uint32_t nlist = 0;
+  uint64_t wg = 0;
+  size_t off = 0;
+  for (uint32_t t = 0; t < n; t++) {
+    emap[t] = (uint32_t)wg;
+    kt[2 * t] = kt[2 * t + 1] = 0;
+    const sl_emit *L = recs + off;
+    const uint32_t m = counts[t];
+    off += m;
+    if (!m) continue;
+    sl_sim *s = b->sims[t];
+    uint32_t k = 0, tot = 0;
+    for (uint32_t i = 0; i < m && k < SL_MAX_EMIT && tot < s->cap; i++) {
+      if (!L[i].count) continue;
+      uint32_t cnt = std::min(L[i].count, s->cap - tot);
+      SlEmitDev &e = he[(size_t)t * SL_MAX_EMIT + k];
+      memcpy(e.a, L[i].a, 16); memcpy(e.b, L[i].b, 16); memcpy(e.c, L[i].c, 16); memcpy(e.d, L[i].d, 16);
+      e.info[0] = tot; e.info[1] = cnt; e.info[2] = L[i].seed; e.info[3] = L[i].shape;
+      tot += cnt;
+      k++;
+    }
+    if (!tot) continue;
+    kt[2 * t] = k;
+    kt[2 * t + 1] = tot;
+    elist[nlist++] = t;
+    wg += (tot + SL_WG - 1) / SL_WG;
+    // host estimates until the next counters copy
+    if ((uint64_t)s->count + tot + 64 > s->cap) s->full = true;
+    s->count = (uint32_t)std::min<uint64_t>((uint64_t)s->count + tot, s->cap);
+    s->emitted_total += tot;
+    s->p_ub = (uint32_t)std::min<uint64_t>((uint64_t)s->p_ub + tot, s->cap);
+  }
+  emap[n] = (uint32_t)wg;
+  if (!nlist) return SL_OK;
+  if (wg > 0x7fffffffull) return batch_fail(b, SL_ECAP, "sl_batch_emit: too many particles");
+  int r = ring_push(b, b->emit, b->emit.bytes);
+  if (r) return r;
+  const uint32_t *d_kt = (const uint32_t *)(b->emit.d + eb);
+  hipLaunchKernelGGL(k_batch_emit_setup, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_tanks, d_kt, (int)n);
+  SlLaunch L;
+  L.w.tanks = b->d_tanks;
+  L.w.start = d_kt + 2 * (size_t)n;
+  L.w.ntanks = (int32_t)n;
+  L.w.total = (uint32_t)wg;
+  L.w.list = nullptr;
+  L.stream = b->stream;
+  sl_k_emit(L, 64);
+  SlLaunch P = L;
+  P.w.start = nullptr;
+  P.w.ntanks = (int32_t)nlist;
+  P.w.list = d_kt + 3 * (size_t)n + 1;
+  sl_k_count(P, 7, 0);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SL_OK : batch_fail(b, SL_EHIP, std::string("batch emit: ") + hipGetErrorString(e));
+}
+
+// the kernels of one frame of every stepped tank, in stream order (no host synchronisation and no
+// memset, so the sequence can be captured)
 static int batch_encode_frame(sl_batch *b, double dt) {
-  const int n = (int)b->sims.size();
-  const SlLaunch one = batch_launch(b, b->m_nodes);  // k_count runs one thread per tank
-  hipLaunchKernelGGL(k_batch_advance, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_tanks, n, (float)dt, dt);
-  sl_k_count(one, 5, 0);  // vmax = 0 (50-sim.js:306)
-  hipLaunchKernelGGL(k_batch_clear_cells, dim3(b->m_cells.total), dim3(SL_WG), 0, b->stream,
-                     batch_launch(b, b->m_cells).w);
-  for (const auto &m : b->m_p2g)
-    if (m.variant >= 0 && m.total) sl_k_p2g(batch_launch(b, m), 0, m.variant);
-  sl_k_normforce(batch_launch(b, b->m_nodes));
-  sl_k_divsetup(batch_launch(b, b->m_cells));
-  const SlLaunch Ls = batch_launch(b, b->m_hcells);  // k_sor returns for passes beyond a tank's iters
+  const int n = (int)b->n_stepped;
+  if (!n) return SL_OK;
+  const SlLaunch P = batch_tanks(b);
+  hipLaunchKernelGGL(k_batch_advance, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_tanks, list_dev(b), n,
+                     (float)dt, dt);
+  sl_k_count(P, 5, 0);  // vmax = 0 (50-sim.js:306)
+  hipLaunchKernelGGL(k_batch_clear_cells, dim3(b->total[BM_CELLS]), dim3(SL_WG), 0, b->stream,
+                     batch_launch(b, BM_CELLS).w);
+  for (int u = 0; u < SL_BUCKETS; u++)
+    if (b->map_variant[BM_P2G0 + u] >= 0) sl_k_p2g(batch_launch(b, BM_P2G0 + u), 0, b->map_variant[BM_P2G0 + u]);
+  sl_k_normforce(batch_launch(b, BM_NODES));
+  sl_k_divsetup(batch_launch(b, BM_CELLS));
+  const SlLaunch Ls = batch_launch(b, BM_HCELLS);  // k_sor returns for passes beyond a tank's iters
   for (uint32_t i = 0; i < b->max_iters; i++) { sl_k_sor(Ls, 0, 2 * i); sl_k_sor(Ls, 1, 2 * i + 1); }
-  sl_k_project(batch_launch(b, b->m_nodes));
-  for (const auto &m : b->m_g2p)
-    if (m.variant >= 0 && m.total) sl_k_g2p(batch_launch(b, m), m.variant);
-  const SlLaunch Lc = batch_launch(b, b->m_compact);
+  sl_k_project(batch_launch(b, BM_NODES));
+  for (int u = 0; u < SL_BUCKETS; u++)
+    if (b->map_variant[BM_G2P0 + u] >= 0) sl_k_g2p(batch_launch(b, BM_G2P0 + u), b->map_variant[BM_G2P0 + u]);
+  const SlLaunch Lc = batch_launch(b, BM_COMPACT);
   sl_k_compact_count(Lc);
-  sl_k_compact_scan(Lc, n);
+  SlLaunch Lscan = Lc;
+  Lscan.w.list = list_dev(b);
+  sl_k_compact_scan(Lscan, n);
   sl_k_compact_scatter(Lc);
-  sl_k_count(one, 1, 0);
-  sl_k_ww(batch_launch(b, b->m_ww), (float)dt);
-  sl_k_count(one, 3, 0);
+  sl_k_count(P, 1, 0);
+  sl_k_ww(batch_launch(b, BM_WW), (float)dt);
+  sl_k_count(P, 3, 0);
   if (b->any_ripple)  // the mask stages return for tanks whose mask is off
-    sl_k_ripple_frame_ex(batch_launch(b, b->m_rows).w, batch_launch(b, b->m_cols).w, batch_launch(b, b->m_elems).w,
-                         batch_launch(b, b->m_tex).w, b->stream, 1);
-  hipError_t e = hipMemcpyAsync(b->h_counters, b->d_counters, sizeof(SlCounters) * n, hipMemcpyDeviceToHost,
+    sl_k_ripple_frame_ex(batch_launch(b, BM_ROWS).w, batch_launch(b, BM_COLS).w, batch_launch(b, BM_ELEMS).w,
+                         batch_launch(b, BM_TEX).w, b->stream, 1);
+  hipError_t e = hipMemcpyAsync(b->h_counters, b->d_counters, sizeof(SlCounters) * b->cap, hipMemcpyDeviceToHost,
                                 b->stream);
   if (e == hipSuccess) e = hipGetLastError();
   return e == hipSuccess ? SL_OK : batch_fail(b, SL_EHIP, std::string("batch frame: ") + hipGetErrorString(e));
 }
 
-// objects that changed since the last frame: their GEOM pass (outside any captured graph)
+// GEOM for tanks whose objects changed (outside any captured graph), and the work maps after a
+// membership change
 static int batch_prepare(sl_batch *b) {
-  for (sl_sim *s : b->sims) {
-    if (!s->geom_dirty) continue;
+  for (uint32_t t = 0; t < b->cap; t++) {
+    sl_sim *s = b->sims[t];
+    if (!s || !b->stepped[t] || !s->geom_dirty) continue;
     int r = push_dyn(s);
     if (r) return batch_fail(b, r, s->err);
     sl_k_geom(launch_of(s, s->nodes));
     s->geom_dirty = false;
   }
+  if (b->maps_dirty) {
+    int r = batch_rebuild(b);
+    if (r) return r;
+  }
   hipError_t e = hipGetLastError();
   return e == hipSuccess ? SL_OK : batch_fail(b, SL_EHIP, hipGetErrorString(e));
 }
 
-// host mirror of what the device did in one frame, plus each tank's re-ordering cadence
+// host mirror of one frame of every stepped tank, plus each tank's re-ordering cadence
 static int batch_host_frame(sl_batch *b, double dt) {
-  for (sl_sim *s : b->sims) {
+  for (uint32_t t = 0; t < b->cap; t++) {
+    sl_sim *s = b->sims[t];
+    if (s) b->copy_valid[t] = 1;  // the frame's counters copy covers every tank in the table
+    if (!s || !b->stepped[t]) continue;
     s->frame++;
     write_params(s, dt);
     s->time += dt;
@@ -1480,7 +1727,7 @@ static int batch_host_frame(sl_batch *b, double dt) {
     s->ww_cur ^= 1;
     s->frames++;
     s->substeps_last = 1;
-    s->emitted_at_copy = s->emitted_total;  // the frame's counters copy covers every earlier emission
+    s->emitted_at_copy = s->emitted_total;
     if (s->cfg.reorder_every && ++s->reorder_clock >= s->cfg.reorder_every) {
       int r = encode_reorder(s);
       if (r) return batch_fail(b, r, s->err);
@@ -1494,28 +1741,43 @@ extern "C" int sl_batch_step(sl_batch *b, double dt) {
   if (!(dt > 0)) return batch_fail(b, SL_EARG, "sl_batch_step: dt must be positive");
   int r = batch_prepare(b);
   if (r) return r;
+  if (!b->n_stepped) return SL_OK;
+  hipEvent_t t1 = nullptr;
+  if (b->timing) {
+    while (b->tev.size() < b->tev_used + 2) {
+      hipEvent_t e;
+      if (hipEventCreate(&e) != hipSuccess) return batch_fail(b, SL_EHIP, "timing event");
+      b->tev.push_back(e);
+    }
+    (void)hipEventRecord(b->tev[b->tev_used], b->stream);
+    t1 = b->tev[b->tev_used + 1];
+    b->tev_used += 2;
+  }
   if (b->exec && b->graph_dt == dt) {
     hipError_t e = hipGraphLaunch(b->exec, b->stream);
     if (e != hipSuccess) return batch_fail(b, SL_EHIP, std::string("graph launch: ") + hipGetErrorString(e));
   } else if ((r = batch_encode_frame(b, dt))) {
     return r;
   }
+  if (t1) (void)hipEventRecord(t1, b->stream);
   b->steps++;
   return batch_host_frame(b, dt);
 }
 
-// records one frame of every tank into a hipGraph; sl_batch_step with the same dt replays it (one
-// graph launch per frame instead of 2*iters + ~25 kernel launches). Records nothing else: the
-// stream is idle before the capture.
+// Records one frame of every stepped tank into a hipGraph; sl_batch_step with the same dt replays it
+// (one graph launch per frame instead of 2*iters + ~25 kernel launches) until the membership or the
+// variant choice changes.
 extern "C" int sl_batch_capture(sl_batch *b, double dt) {
   if (!b) return batch_fail(nullptr, SL_EARG, "null batch");
   if (!(dt > 0)) return batch_fail(b, SL_EARG, "sl_batch_capture: dt must be positive");
+  int r = batch_prepare(b);
+  if (r) return r;
   if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, SL_EHIP, "sync before capture");
-  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
-  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  drop_graph(b);
+  if (!b->n_stepped) return SL_OK;
   if (hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
     return batch_fail(b, SL_EHIP, "begin capture failed");
-  int r = batch_encode_frame(b, dt);
+  r = batch_encode_frame(b, dt);
   hipGraph_t g = nullptr;
   hipError_t e = hipStreamEndCapture(b->stream, &g);
   if (r || e != hipSuccess || !g) {
@@ -1526,71 +1788,99 @@ extern "C" int sl_batch_capture(sl_batch *b, double dt) {
   b->graph = g;
   if (hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0) != hipSuccess) {
     b->exec = nullptr;
+    drop_graph(b);
     return batch_fail(b, SL_EHIP, "graph instantiate failed");
   }
   b->graph_dt = dt;
   return SL_OK;
 }
+extern "C" int sl_batch_captured(const sl_batch *b) { return b && b->exec ? 1 : 0; }
 
 extern "C" int sl_batch_sync(sl_batch *b) {
   if (!b) return batch_fail(nullptr, SL_EARG, "null batch");
   hipError_t e = hipStreamSynchronize(b->stream);
   if (e != hipSuccess) {
-    for (sl_sim *s : b->sims) s->gpu_errors++;
+    for (sl_sim *s : b->sims) if (s) s->gpu_errors++;
     return batch_fail(b, SL_EHIP, std::string("batch sync: ") + hipGetErrorString(e));
   }
-  if (!b->steps) return SL_OK;
-  // counters of the last frame are in pinned memory: refresh every tank's host state
-  for (size_t t = 0; t < b->sims.size(); t++) {
+  // the counters of the last frame are in pinned memory: refresh the host state of the tanks
+  for (uint32_t t = 0; t < b->cap; t++) {
     sl_sim *s = b->sims[t];
+    if (!s || !b->copy_valid[t]) continue;
     *s->h_counters = b->h_counters[t];
     apply_stats(s);
+    if (s->count + 64 >= s->cap) s->full = true;
   }
   return SL_OK;
 }
 
 extern "C" int sl_batch_reorder(sl_batch *b) {
   if (!b) return batch_fail(nullptr, SL_EARG, "null batch");
-  for (sl_sim *s : b->sims) {
-    int r = encode_reorder(s);
-    if (r) return batch_fail(b, r, s->err);
+  for (uint32_t t = 0; t < b->cap; t++) {
+    if (!live(b, t)) continue;
+    int r = encode_reorder(b->sims[t]);
+    if (r) return batch_fail(b, r, b->sims[t]->err);
   }
   return SL_OK;
 }
 
-// Per size bucket, the fastest P2G and G2P variant on the live state of that bucket's tanks (all
-// variants give bit-identical results). Particles, counters, cell counts and ripple impulses are
-// snapshotted and restored, as sl_autotune does for one simulation.
+extern "C" int sl_batch_timing_enable(sl_batch *b, int enable) {
+  if (!b) return batch_fail(nullptr, SL_EARG, "null batch");
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, SL_EHIP, "sync");
+  b->timing = enable != 0;
+  b->tev_used = 0;
+  return SL_OK;
+}
+
+// device time of each step since timing was enabled or last read (oldest first), then starts over
+extern "C" int sl_batch_timing_read(sl_batch *b, double *ms, uint32_t max, uint32_t *n) {
+  if (!b || (max && !ms)) return batch_fail(b, SL_EARG, "sl_batch_timing_read: bad arguments");
+  if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, SL_EHIP, "sync");
+  uint32_t k = 0;
+  for (size_t i = 0; i + 1 < b->tev_used && k < max; i += 2) {
+    float t = 0;
+    (void)hipEventElapsedTime(&t, b->tev[i], b->tev[i + 1]);
+    ms[k++] = t;
+  }
+  if (n) *n = k;
+  b->tev_used = 0;
+  return SL_OK;
+}
+
+// Per size bucket, the fastest P2G and G2P variant on the live state of that bucket's stepped tanks
+// (all variants give bit-identical results). Particles, counters, cell counts and ripple impulses
+// are snapshotted and restored, as sl_autotune does for one simulation.
 extern "C" int sl_batch_autotune(sl_batch *b, uint32_t reps, uint32_t *choice) {
   if (!b) return batch_fail(nullptr, SL_EARG, "null batch");
   if (!reps) reps = 2;
-  int r = sl_batch_sync(b);
-  if (r) return r;
+  int r = batch_prepare(b);
+  if (r || (r = sl_batch_sync(b))) return r;
   struct Saved { void *live, *copy; size_t bytes; };
   std::vector<Saved> items;
   auto release = [&]() { for (auto &it : items) (void)hipFree(it.copy); };
-  for (sl_sim *s : b->sims) {
+  for (uint32_t t = 0; t < b->cap; t++) {
+    if (!live(b, t)) continue;
+    sl_sim *s = b->sims[t];
     const SlTank &T = s->ht;
     const int c = s->cur;
     const size_t P = s->cap;
-    void *live[10] = {T.px[c], T.py[c], T.pz[c], T.vx[c], T.vy[c], T.vz[c], T.pf[c], T.C, T.cellCount, T.rsrc};
+    void *src[10] = {T.px[c], T.py[c], T.pz[c], T.vx[c], T.vy[c], T.vz[c], T.pf[c], T.C, T.cellCount, T.rsrc};
     size_t by[10] = {P * 4, P * 4, P * 4, P * 4, P * 4, P * 4, P, sizeof(SlCounters), (size_t)s->cells * 4,
                      s->R ? (size_t)(s->R + 1) * (s->R + 1) * 4 : 0};
     for (int i = 0; i < 10; i++) {
       if (!by[i]) continue;
       void *p = nullptr;
       if (hipMalloc(&p, by[i]) != hipSuccess) { release(); return batch_fail(b, SL_ENOMEM, "batch autotune: snapshot"); }
-      items.push_back({live[i], p, by[i]});
-      (void)hipMemcpyAsync(p, live[i], by[i], hipMemcpyDeviceToDevice, b->stream);
+      items.push_back({src[i], p, by[i]});
+      (void)hipMemcpyAsync(p, src[i], by[i], hipMemcpyDeviceToDevice, b->stream);
     }
   }
   auto restore = [&]() {
     for (auto &it : items) (void)hipMemcpyAsync(it.live, it.copy, it.bytes, hipMemcpyDeviceToDevice, b->stream);
   };
-  auto clear_grids = [&]() {
-    for (sl_sim *s : b->sims) sl_k_fill_u32(b->stream, (uint32_t *)s->ht.acc, 0, (size_t)s->nodes * 6);
-    hipLaunchKernelGGL(k_batch_clear_cells, dim3(b->m_cells.total), dim3(SL_WG), 0, b->stream,
-                       batch_launch(b, b->m_cells).w);
+  auto clear_acc = [&]() {
+    for (uint32_t t = 0; t < b->cap; t++)
+      if (live(b, t)) sl_k_fill_u32(b->stream, (uint32_t *)b->sims[t]->ht.acc, 0, (size_t)b->sims[t]->nodes * 6);
   };
   auto timed = [&](auto fn) {
     (void)hipEventRecord(b->ev_a, b->stream);
@@ -1601,70 +1891,60 @@ extern "C" int sl_batch_autotune(sl_batch *b, uint32_t reps, uint32_t *choice) {
     (void)hipEventElapsedTime(&ms, b->ev_a, b->ev_b);
     return ms;
   };
-  const size_t n = b->sims.size();
-  std::vector<uint64_t> cap(n);
-  for (size_t t = 0; t < n; t++) cap[t] = b->sims[t]->cap;
-  sl_batch::Map m;  // scratch map, reused for every (bucket, variant)
   for (int k = 0; k < SL_BUCKETS && !r; k++) {
-    std::vector<char> inc(n);
     bool has = false;
-    for (size_t t = 0; t < n; t++) { inc[t] = b->bucket[t] == k; has |= inc[t] != 0; }
+    for (uint32_t t = 0; t < b->cap; t++) has |= live(b, t) && b->bucket[t] == k;
     if (!has) continue;
+    auto in_k = [&](uint32_t t) { return b->bucket[t] == k; };
     float best = 1e30f;
-    for (int v = 0; v < 8 && !r; v++) {
-      if ((r = build_map(b, m, cap, sl_p2g_chunk(v), &inc))) break;
+    for (int v = 0; v < 8; v++) {
+      if ((r = put_map(b, BM_SCRATCH, [](const sl_sim *s) { return (uint64_t)s->cap; }, sl_p2g_chunk(v), in_k)) ||
+          (r = maps_upload(b)))
+        break;
       float tot = 0;
       for (uint32_t rep = 0; rep <= reps; rep++) {
-        clear_grids();
-        float ms = timed([&] { sl_k_p2g(batch_launch(b, m), 0, v); });
+        clear_acc();
+        hipLaunchKernelGGL(k_batch_clear_cells, dim3(b->total[BM_CELLS]), dim3(SL_WG), 0, b->stream,
+                           batch_launch(b, BM_CELLS).w);
+        float ms = timed([&] { sl_k_p2g(batch_launch(b, BM_SCRATCH), 0, v); });
         if (rep) tot += ms;
       }
       if (tot < best) { best = tot; b->p2g_variant[k] = v; }
     }
     if (r) break;
-    if ((r = build_map(b, m, cap, SL_WG, &inc))) break;
+    if ((r = put_map(b, BM_SCRATCH, [](const sl_sim *s) { return (uint64_t)s->cap; }, SL_WG, in_k)) ||
+        (r = maps_upload(b)))
+      break;
     best = 1e30f;
     for (int v = 0; v < 4; v++) {
       float tot = 0;
       for (uint32_t rep = 0; rep <= reps; rep++) {
         restore();
-        float ms = timed([&] { sl_k_g2p(batch_launch(b, m), v); });
+        float ms = timed([&] { sl_k_g2p(batch_launch(b, BM_SCRATCH), v); });
         if (rep) tot += ms;
       }
       if (tot < best) { best = tot; b->g2p_variant[k] = v; }
     }
   }
-  // accumulators cleared (the next P2G adds into them), everything else as it was
-  for (sl_sim *s : b->sims) sl_k_fill_u32(b->stream, (uint32_t *)s->ht.acc, 0, (size_t)s->nodes * 6);
+  clear_acc();  // the next P2G adds into cleared accumulators
   restore();
   hipError_t e = hipStreamSynchronize(b->stream);
   release();
-  if (m.d) {
-    b->allocs.erase(std::find(b->allocs.begin(), b->allocs.end(), (void *)m.d));
-    (void)hipFree(m.d);
-  }
   if (r) return r;
   if (e != hipSuccess || (e = hipGetLastError()) != hipSuccess) return batch_fail(b, SL_EHIP, hipGetErrorString(e));
-  if ((r = batch_particle_maps(b))) return r;
-  // a captured frame holds the old launch shapes
-  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
-  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  if ((r = batch_rebuild(b))) return r;
   if (choice)
     for (int k = 0; k < SL_BUCKETS; k++) { choice[2 * k] = (uint32_t)b->p2g_variant[k]; choice[2 * k + 1] = (uint32_t)b->g2p_variant[k]; }
   return SL_OK;
 }
 
-// sets every bucket's variants by hand (values as sl_batch_autotune reports them)
+// sets every bucket's variants (values as sl_batch_autotune reports them)
 extern "C" int sl_batch_set_variants(sl_batch *b, const uint32_t *choice) {
   if (!b || !choice) return batch_fail(b, SL_EARG, "sl_batch_set_variants: bad arguments");
   for (int k = 0; k < SL_BUCKETS; k++)
     if (choice[2 * k] > 7 || choice[2 * k + 1] > 3) return batch_fail(b, SL_EARG, "variant out of range");
-  if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, SL_EHIP, "sync");
   for (int k = 0; k < SL_BUCKETS; k++) { b->p2g_variant[k] = (int)choice[2 * k]; b->g2p_variant[k] = (int)choice[2 * k + 1]; }
-  int r = batch_particle_maps(b);
-  if (r) return r;
-  if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
-  if (b->graph) { (void)hipGraphDestroy(b->graph); b->graph = nullptr; }
+  b->maps_dirty = true;
   return SL_OK;
 }
 

@@ -290,7 +290,9 @@ void sl_k_g2p(const SlLaunch &L, int variant) {
 // ------------------------------------------------------------------------------------------------
 // EMIT, 31-wgsl-sim2.js:262-315 (one thread per emitted particle, `emit_total` of them)
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(SL_WG) void k_emit(SlWork W) {
+// `reserve`: slots kept free below the capacity (0 here; a batch passes the 64 that the host-side
+// budget of writeEmits keeps, 50-sim.js:226, so that the budget is applied on the device count)
+__global__ __launch_bounds__(SL_WG) void k_emit(SlWork W, uint32_t reserve) {
   SL_TANK_OF(W, blockIdx.x, tk, lb);
   const SlTank &T = W.tanks[tk];
   uint32_t t = lb * SL_WG + threadIdx.x;
@@ -300,7 +302,7 @@ __global__ __launch_bounds__(SL_WG) void k_emit(SlWork W) {
   const SlEmitDev &em = T.emits[e];
   uint32_t local = t - em.info[0];
   uint32_t idx = T.C->count + t;
-  if (idx >= T.cap) return;
+  if (idx >= T.cap || idx + reserve >= T.cap) return;
   uint32_t s = hashU(t * 747796405u + em.info[2] * 2891336453u + 1u);
   f3 pos = F3(0, 0, 0), vel = F3(0, 0, 0);
   bool mark = false;
@@ -361,20 +363,22 @@ __global__ __launch_bounds__(SL_WG) void k_emit(SlWork W) {
   T.vx[cur][idx] = vel.x; T.vy[cur][idx] = vel.y; T.vz[cur][idx] = vel.z;
   T.pf[cur][idx] = (uint8_t)((alive ? 1u : 0u) | (mark ? 2u : 0u));
 }
-void sl_k_emit(const SlLaunch &L) {
+void sl_k_emit(const SlLaunch &L, uint32_t reserve) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_emit, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  hipLaunchKernelGGL(k_emit, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w, reserve);
 }
 
 // ------------------------------------------------------------------------------------------------
 // COUNT, 31-wgsl-sim2.js:332-349 (the indirect arguments are not needed: kernels read the counters)
 // mode 1: count = newCount; 2: count = min(count + y, cap); 3: wwCount = min(wwNew, wcap);
-// 4: clear; 5: vmax = 0 (the writeBuffer of 50-sim.js:306). One thread per tank.
+// 4: clear; 5: vmax = 0 (the writeBuffer of 50-sim.js:306); 6: flip the particle buffer;
+// 7: count += the emitted particles that fit below cap - 64 (a batch's emission, whose budget is
+// applied on the device count). One thread per tank (per listed slot when `list` is given).
 // ------------------------------------------------------------------------------------------------
-__global__ void k_count(SlTank *tanks, int ntanks, uint32_t mode, uint32_t y) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= ntanks) return;
-  SlTank &T = tanks[t];
+__global__ void k_count(SlTank *tanks, const uint32_t *list, int ntanks, uint32_t mode, uint32_t y) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntanks) return;
+  SlTank &T = tanks[list ? list[i] : i];
   SlCounters *C = T.C;
   uint32_t cap = T.cap, wcap = f2u(T.ww_cap_f);
   if (mode == 1) { C->count = C->newCount; C->newCount = 0; T.cur = 1 - T.cur; }
@@ -383,10 +387,15 @@ __global__ void k_count(SlTank *tanks, int ntanks, uint32_t mode, uint32_t y) {
   if (mode == 4) { C->count = 0; C->newCount = 0; C->wwCount = 0; C->wwNew = 0; C->impacts = 0; }
   if (mode == 5) C->vmax = 0;
   if (mode == 6) T.cur = 1 - T.cur;
+  if (mode == 7 && T.emit_total) {
+    const uint32_t lim = cap > 64u ? cap - 64u : 0u, c = C->count;
+    if (c < lim) C->count = c + min(T.emit_total, lim - c);
+  }
 }
 void sl_k_count(const SlLaunch &L, uint32_t mode, uint32_t y) {
   int nt = L.w.ntanks;
-  hipLaunchKernelGGL(k_count, dim3((nt + 63) / 64), dim3(64), 0, L.stream, L.w.tanks, nt, mode, y);
+  if (nt <= 0) return;
+  hipLaunchKernelGGL(k_count, dim3((nt + 63) / 64), dim3(64), 0, L.stream, L.w.tanks, L.w.list, nt, mode, y);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -421,7 +430,7 @@ void sl_k_compact_count(const SlLaunch &L) {
 }
 // one workgroup per tank scans its per-workgroup counts in place (exclusive) and sets newCount
 __global__ __launch_bounds__(1024) void k_compact_scan(SlWork W) {
-  const int t = blockIdx.x;
+  const int t = W.list ? (int)W.list[blockIdx.x] : (int)blockIdx.x;
   const SlTank &T = W.tanks[t];
   const uint32_t lim = min(T.C->count, T.cap);
   const uint32_t nb = (lim + SL_COMPACT_ITEMS - 1) / SL_COMPACT_ITEMS;
@@ -459,6 +468,7 @@ __global__ __launch_bounds__(1024) void k_compact_scan(SlWork W) {
   if (threadIdx.x == 0) T.C->newCount = carry;
 }
 void sl_k_compact_scan(const SlLaunch &L, int ntanks) {
+  if (ntanks <= 0) return;
   hipLaunchKernelGGL(k_compact_scan, dim3(ntanks), dim3(1024), 0, L.stream, L.w);
 }
 __global__ __launch_bounds__(SL_WG) void k_compact_scatter(SlWork W) {

@@ -73,6 +73,9 @@ struct SlWork {
   const uint32_t *start;  // ntanks+1 prefix of workgroup counts (device)
   int32_t ntanks;
   uint32_t total;         // start[ntanks]
+  // per-tank kernels (one thread or workgroup per tank): the table slots to process, in order;
+  // nullptr means slots 0 .. ntanks-1
+  const uint32_t *list = nullptr;
 };
 
 #ifdef __HIPCC__
@@ -282,7 +285,7 @@ void sl_k_sor(const SlLaunch &L, uint32_t parity, uint32_t pass);
 void sl_k_project(const SlLaunch &L);
 // particle stages
 void sl_k_g2p(const SlLaunch &L, int variant);
-void sl_k_emit(const SlLaunch &L);
+void sl_k_emit(const SlLaunch &L, uint32_t reserve = 0);
 void sl_k_count(const SlLaunch &L, uint32_t mode, uint32_t y);
 void sl_k_vmax_reset(const SlLaunch &L);
 void sl_k_compact_count(const SlLaunch &L);

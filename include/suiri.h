@@ -209,30 +209,50 @@ int sl_timing_read(sl_sim *sim, double *ms, uint32_t *launches, int n, int reset
    the selected P2G and G2P variants. No counterpart in the reference. */
 int sl_autotune(sl_sim *sim, uint32_t reps, uint32_t *choice);
 
-/* Batched tanks (no counterpart in the reference, which runs one tank). A batch steps many
-   independent simulations with one kernel launch per stage for all of them; each tank keeps the
-   single-simulation semantics (one substep of dt per frame, no emission inside sl_batch_step).
-   Per-tank set-up, emission between frames, readback and stats go through sl_batch_sim(). */
+/* Batched tanks (no counterpart in the reference, which runs one tank per page). A batch is a table
+   of tank slots stepped with one kernel launch per stage for all stepped tanks; each tank keeps the
+   single-simulation semantics (one substep of dt per frame). Tanks are added and removed while the
+   batch runs (continuous batching). Per-tank set-up, single emissions, readback and stats go
+   through the tank's sl_sim (sl_batch_sim), which shares the batch's stream. */
 typedef struct sl_batch sl_batch;
+/* an empty batch of `capacity` slots on `device` */
+int sl_batch_open(uint32_t capacity, int device, sl_batch **out);
+/* sl_batch_open(n) + sl_batch_add of each configuration, in slot order */
 int sl_batch_create(const sl_config *cfgs, uint32_t n, sl_batch **out);
 void sl_batch_destroy(sl_batch *batch);
 const char *sl_batch_last_error(const sl_batch *batch);
-uint32_t sl_batch_size(const sl_batch *batch);
-sl_sim *sl_batch_sim(sl_batch *batch, uint32_t i);
-/* one frame of every tank (frame++, P2G .. G2P, compaction, whitewater, ripple, counters copy);
-   the ripple mean depth comes from each tank's count at the start of the frame */
+uint32_t sl_batch_size(const sl_batch *batch);                /* slots */
+sl_sim *sl_batch_sim(sl_batch *batch, uint32_t slot);         /* NULL for an empty slot */
+/* creates a tank in a free slot (stepped from the next sl_batch_step on); SL_ECAP when full */
+int sl_batch_add(sl_batch *batch, const sl_config *cfg, uint32_t *slot);
+/* destroys the tank of a slot and frees its device memory */
+int sl_batch_remove(sl_batch *batch, uint32_t slot);
+/* pauses (0) or resumes (1) stepping a tank; a paused tank stays resident for readback */
+int sl_batch_set_stepped(sl_batch *batch, uint32_t slot, int on);
+/* emission into many tanks at once: counts[slot] records per slot (capacity entries), records
+   concatenated in slot order. writeEmits' budget (cap - count - 64, 50-sim.js:222-245) is applied
+   on the device count. No `fill` pours here (sl_emit_now on the tank does them). */
+int sl_batch_emit(sl_batch *batch, const sl_emit *recs, const uint32_t *counts);
+/* one frame of every stepped tank (frame++, P2G .. G2P, compaction, whitewater, ripple, counters
+   copy); the ripple mean depth comes from each tank's count at the start of the frame */
 int sl_batch_step(sl_batch *batch, double dt);
-/* records one frame into a hipGraph; sl_batch_step with the same dt then replays it */
+/* records one frame into a hipGraph; sl_batch_step with the same dt then replays it until the
+   membership or the variant choice changes */
 int sl_batch_capture(sl_batch *batch, double dt);
+int sl_batch_captured(const sl_batch *batch);
 /* waits for the batch and refreshes every tank's counters */
 int sl_batch_sync(sl_batch *batch);
-/* spatial re-ordering of every tank (drops a captured graph) */
+/* spatial re-ordering of every stepped tank */
 int sl_batch_reorder(sl_batch *batch);
 /* per size bucket (<= 64Ki cells, <= 256Ki, larger): fastest P2G / G2P variant on the live state;
    choice (optional, 6 entries) receives (p2g, g2p) per bucket */
 int sl_batch_autotune(sl_batch *batch, uint32_t reps, uint32_t *choice);
 /* sets the (p2g, g2p) variants of the three buckets (6 entries, as sl_batch_autotune reports) */
 int sl_batch_set_variants(sl_batch *batch, const uint32_t *choice);
+/* per-step device time: while enabled, each sl_batch_step is bracketed by events; read returns the
+   durations [ms] of the steps since enable or the last read, oldest first */
+int sl_batch_timing_enable(sl_batch *batch, int enable);
+int sl_batch_timing_read(sl_batch *batch, double *ms, uint32_t max, uint32_t *n);
 
 /* ABI version + build description */
 int sl_version(void);

@@ -71,13 +71,24 @@ class Driver:
             self.emit(nx - 4, nx - 1, 3, 5, 1, 4)
 
 
-def single_run(sp, seed, frames):
+def source_records(sim, frame, seed):
+    """A falling ball of water (emitter shape 1) every other frame."""
+    if frame % 2:
+        return []
+    x = sim.origin[0] + (0.3 + 0.05 * (frame % 5)) * sim.nx * sim.h
+    z = sim.origin[2] + 0.5 * sim.nz * sim.h
+    y = 0.8 * sim.ny * sim.h
+    return [dict(a=[x, y, z, 2.5 * sim.h], b=[0.1, -0.8, 0.0, 1.0], shape=1, count=300, seed=seed * 1000 + frame)]
+
+
+def single_run(sp, seed, frames, sources=False):
     s = suiri.Sim(cfg_of(sp))
     d = Driver(s, sp, seed)
     s.set_substeps(1)
     for f in range(frames):
         d.event(f)
-        s.step(DT)
+        recs = source_records(s, f, seed) if sources else []
+        s.step(DT, (lambda dt, t: recs) if recs else None)
         s.stats()   # waits and refreshes the counts (the next frame's ripple depth reads them)
     return s
 
@@ -165,3 +176,47 @@ def test_batch_errors():
     bad.ripple_R = 48   # not a power of two
     with pytest.raises(suiri.SuiriError, match="tank 1"):
         suiri.Batch([cfg_of(SPECS[0]), bad])
+
+
+def test_batch_membership_changes_and_batched_emission():
+    """Continuous batching: tanks join at different frames, one pauses and resumes, one leaves, every
+    stepped tank gets source emission through one sl_batch_emit per frame, and the last frames replay
+    a captured graph. Each tank still equals its own single run."""
+    print("batch membership: 5 slots, 8 frames, sources", flush=True)
+    plan = [dict(spec=0, add=0, pause=(), remove=None), dict(spec=1, add=2, pause=(), remove=None),
+            dict(spec=2, add=0, pause=(3, 4), remove=None), dict(spec=3, add=1, pause=(), remove=None),
+            dict(spec=0, add=0, pause=(), remove=4)]
+    b = suiri.Batch(capacity=5)
+    live = {}   # plan index -> [slot, driver, frames stepped]
+    for f in range(8):
+        for i, p in enumerate(plan):
+            if f == p["add"]:
+                slot = b.add(cfg_of(SPECS[p["spec"]]))
+                live[i] = [slot, Driver(b.tanks[slot], SPECS[p["spec"]], 50 + i), 0]
+            if f == p["remove"] and i in live:
+                b.remove(live.pop(i)[0])
+        stepped = {}
+        for i, (slot, d, n) in live.items():
+            on = f not in plan[i]["pause"]
+            b.set_stepped(slot, on)
+            if on:
+                d.event(n)
+                recs = source_records(b.tanks[slot], n, 50 + i)
+                if recs:
+                    stepped[slot] = recs
+        if stepped:
+            b.emit(stepped)
+        if f == 6:
+            b.capture(DT)
+            assert b.captured
+        b.step(DT)
+        b.sync()
+        for i, (slot, d, n) in live.items():
+            if f not in plan[i]["pause"]:
+                live[i][2] += 1
+    assert sorted(live) == [0, 1, 2, 3]
+    for i, (slot, d, n) in live.items():
+        want = single_run(SPECS[plan[i]["spec"]], 50 + i, n, sources=True)
+        assert_same_tank(b.tanks[slot], want, f"plan {i} ({n} frames)")
+        want.close()
+    b.close()
