@@ -20,40 +20,20 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 suiri = importlib.import_module("hahenn-_amd")
 scene = importlib.import_module("hahenn-_amd.scene")
+workloads = importlib.import_module("hahenn-_amd.workloads")
 
 HBM_PEAK_GBS = 8000.0
 
 
-def faucet_workload(grid=(256, 128, 128), fill_layers=None, iters=60, R=512, ww=True):
-    nx, ny, nz = grid
-    h = 1.0 / 256 * (256 / nx)
-    tank = (nx * h, ny * h, nz * h)
-    fill = fill_layers if fill_layers is not None else ny // 2
-    p_fill = nx * fill * nz * 8
-    items = [
-        scene.make_obstacle("sphere", pos=[-0.25 * tank[0], 0.1 * tank[1] * 2, -0.2 * tank[2]], dims={"r": 0.05}),
-        scene.make_obstacle("cylinder", pos=[0.0, 0.1, 0.2 * tank[2]], dims={"r": 0.03, "h": 0.2}),
-        scene.make_obstacle("torus", pos=[0.25 * tank[0], 0.12, -0.1 * tank[2]], dims={"R": 0.06, "r": 0.015}),
-        scene.make_drain(pos=[0.4 * tank[0], 0.0, 0.3 * tank[2]], r=0.02),
-        scene.make_source(pos=[-0.42 * tank[0], 0.9 * tank[1], 0.0], mode="stream", flow=30.0, speed=1.5,
-                          pitch=-30, yaw=90),
-    ]
-    cap = p_fill + 400_000
-    cfg = dict(tank_w=tank[0], tank_h=tank[1], tank_d=tank[2], h=h, iters=iters, ripple_R=R, cap=cap,
-               ww_cap=260_000, flip=0.95, omega=1.8, whitewater=1 if ww else 0, reorder_every=16)
-    return dict(cfg=cfg, items=items, fill=fill, frame_dt=1.0 / 240, grid=grid, name="faucet-obstacles")
-
-
 def build_sim(wl):
     g = suiri.Sim(suiri.config("high", **wl["cfg"]))
-    rows, _ = scene.ObjectPacker().pack(wl["items"], 0)
-    g.set_objects_rows(rows)
-    rec = scene.cell_block_emit(g.h, g.origin, 0, g.nx, 0, wl["fill"], 0, g.nz, seed=12345)
-    g.emit_now([rec])
+    workloads.setup(g, wl)
     g.reorder()
     g.set_substeps(1)
     em = scene.Emitters(wl["items"], g.vp, seed=99)
@@ -68,12 +48,6 @@ def sor_bytes_per_launch(nx, ny, nz):
     return 12 * N + 9 * C
 
 
-def step_bytes(nx, ny, nz, P, iters):
-    N = (nx + 1) * (ny + 1) * (nz + 1)
-    C = nx * ny * nz
-    return 92 * P + 252 * N + 40 * C + iters * (24 * N + 18 * C)
-
-
 def cpu_baseline(wl, steps, threads):
     """The oracle (fp32 C restatement, OpenMP) on the same workload for `steps` steps."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
@@ -85,9 +59,8 @@ def cpu_baseline(wl, steps, threads):
     rows, _ = scene.ObjectPacker().pack(wl["items"], 0)
     o.set_objects(rows)
     inf = o.info()
-    rec = scene.cell_block_emit(inf["h"], [inf["ox"], inf["oy"], inf["oz"]], 0, int(inf["nx"]), 0, wl["fill"], 0,
-                                int(inf["nz"]), seed=12345)
-    o.emit_now([rec])
+    o.emit_now([scene.cell_block_emit(inf["h"], [inf["ox"], inf["oy"], inf["oz"]], *blk, seed=wl["seed"] + i)
+                for i, blk in enumerate(wl["blocks"])])
     em = scene.Emitters(wl["items"], inf["vp"], seed=99)
     o.step(wl["frame_dt"], nsub=1, emit_fn=em)   # first step includes GEOM: untimed
     t0 = time.perf_counter()
@@ -112,7 +85,8 @@ def pmc_traffic(out_dir, timeout=150):
     if not prof:
         return None
     env = dict(os.environ, TMPDIR="/tmp")
-    short = [sys.executable, os.path.join(ROOT, "bench.py"), "--steps", "3", "--warmup", "1", "--no-cpu", "--no-pmc"]
+    short = [sys.executable, os.path.join(ROOT, "bench.py"), "--steps", "3", "--warmup", "1", "--no-cpu", "--no-pmc",
+             "--only", "headline"]
     calib = [sys.executable, os.path.join(ROOT, "bench.py"), "--calib-copy-mb", "256"]
     runs = {"fetch": ("FETCH_SIZE", short, "k_sor"), "write": ("WRITE_SIZE", short, "k_sor"),
             "cfetch": ("FETCH_SIZE", calib, None), "cwrite": ("WRITE_SIZE", calib, None)}
@@ -141,6 +115,87 @@ def pmc_traffic(out_dir, timeout=150):
             "fetch_factor": kib / vals["cfetch"], "write_factor": kib / vals["cwrite"]}
 
 
+def batched_leg(n_tanks, steps, warmup, seq_steps, graph_steps=50, seed=7):
+    """64 independent tanks (grids 32^3 .. 128x64x64, fill 0.2 .. 0.7, 0 .. 3 primitives, 40 .. 80
+    SOR iterations, dt = 1/120 s, re-orderings staggered over 16 frames) stepped as one batch, one
+    launch per stage for all tanks, against the same tanks solved one after another on the plain
+    path. Per-step device latency from hipEvent pairs. A block replayed from the captured hipGraph
+    is measured after the eager block and reported beside it."""
+    mix = workloads.tank_mix(n_tanks, seed=seed)
+    dt = 1.0 / 120
+    b = suiri.Batch([suiri.config("standard", **wl["cfg"]) for wl in mix])
+    for t, wl in zip(b.tanks, mix):
+        workloads.setup(t, wl)
+    b.reorder()
+    for _ in range(warmup):
+        b.step(dt)
+    b.sync()
+    tuned = b.autotune(reps=1)
+
+    def block(k):
+        b.sync()
+        p0 = sum(t.stats()["count"] for t in b.tanks)
+        b.timing(True)
+        t0 = time.perf_counter()
+        for _ in range(k):
+            b.step(dt)
+        b.sync()
+        el = time.perf_counter() - t0
+        lat = b.step_times()
+        b.timing(False)
+        p1 = sum(t.stats()["count"] for t in b.tanks)
+        return el, lat, (p0 + p1) / 2
+
+    elapsed, lat, parts = block(steps)
+    gbytes = sum(workloads.step_bytes(*wl["grid"], t.stats()["count"], wl["cfg"]["iters"])
+                 for t, wl in zip(b.tanks, mix)) / 1e9
+    graph = None
+    if graph_steps:
+        b.capture(dt)
+        g_el, g_lat, g_parts = block(graph_steps)
+        graph = {"steps": graph_steps, "ms_per_step": g_el / graph_steps * 1e3,
+                 "step_latency_ms_p50": float(np.percentile(g_lat, 50)),
+                 "value": g_parts * graph_steps / g_el,
+                 "note": "measured after the eager block (later frames of the same scenes)"}
+    b.close()
+    value = parts * steps / elapsed
+    # the same tanks, one plain solve after another (each tank: warm-up, then seq_steps timed steps)
+    seq_time, seq_ps = 0.0, 0.0
+    for wl in (mix if seq_steps else []):
+        g = suiri.Sim(suiri.config("standard", **wl["cfg"]))
+        workloads.setup(g, wl)
+        g.reorder()
+        g.set_substeps(1)
+        for _ in range(2):
+            g.step(dt)
+        g.sync()
+        c0 = g.stats()["count"]
+        t1 = time.perf_counter()
+        for _ in range(seq_steps):
+            g.step(dt)
+        g.sync()
+        seq_time += time.perf_counter() - t1
+        seq_ps += (c0 + g.stats()["count"]) / 2 * seq_steps
+        g.close()
+    seq_value = seq_ps / seq_time if seq_steps else None
+    return {
+        "workload": f"{n_tanks} tanks, grids 32^3..128x64x64, fill 0.2-0.7, 0-3 primitives, SOR 40-80 iters, "
+                    f"dt 1/120, whitewater on, ripple M=256",
+        "tanks": n_tanks, "particles": int(parts), "steps": steps, "launch": "eager, one launch per stage",
+        "value": value, "unit": "particle-steps/s",
+        "ms_per_step": elapsed / steps * 1e3,
+        "step_latency_ms": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                            "max": float(lat.max()), "source": "hipEvent pairs around each batch step"},
+        "algorithmic_GB_per_step": gbytes,
+        "achieved_GBps": gbytes / (elapsed / steps),
+        "graph_replay": graph,
+        "sequential": {"value": seq_value, "unit": "particle-steps/s", "steps_per_tank": seq_steps,
+                       "method": "each tank alone on the plain path, one after another"},
+        "speedup_vs_sequential": value / seq_value if seq_value else None,
+        "tuned_variants": tuned,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -151,6 +206,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ripple", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter runs for `traffic`")
+    ap.add_argument("--only", choices=["all", "headline", "batched"], default="all",
+                    help="run one leg (its JSON object alone) instead of the full line")
+    ap.add_argument("--batch-tanks", type=int, default=64)
+    ap.add_argument("--batch-steps", type=int, default=500)
+    ap.add_argument("--seq-steps", type=int, default=10)
+    ap.add_argument("--graph-steps", type=int, default=50)
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -168,8 +229,11 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.only == "batched":
+        print(json.dumps(batched_leg(args.batch_tanks, args.batch_steps, 5, args.seq_steps, args.graph_steps)))
+        return
     grid = tuple(int(x) for x in args.grid.split("x"))
-    wl = faucet_workload(grid=grid, R=0 if args.no_ripple else 512)
+    wl = workloads.faucet(grid=grid, R=0 if args.no_ripple else 512)
     g, em = build_sim(wl)
     for _ in range(args.warmup):
         g.step(wl["frame_dt"], em)
@@ -228,18 +292,21 @@ def main():
                      "traffic": None, "bytes_per_launch": bpl, "avg_launch_us": sor_launch_s * 1e6},
         "sor_GBps": iters * 2 * bpl / (sor_ms / 1e3 / args.steps) / 1e9,
         "stage_ms_per_step": {k: v[0] / args.steps for k, v in tm.items()},
-        "step_algorithmic_GB": step_bytes(nx, ny, nz, mean_p, iters) / 1e9,
+        "step_algorithmic_GB": workloads.step_bytes(nx, ny, nz, mean_p, iters) / 1e9,
         "gpu_errors": st2["gpuErrors"],
         "tuned_variants": tuned,
     }
     g.close()
+    if args.only == "all":
+        out["batched"] = batched_leg(args.batch_tanks, args.batch_steps, 5, args.seq_steps, args.graph_steps)
+        out["batched_p99_step_ms"] = out["batched"]["step_latency_ms"]["p99"]
     if not args.no_pmc:
         t = pmc_traffic(os.path.join(ROOT, "gpurun_out", "bench_pmc"))
         if t:
             out["roofline"]["traffic"] = t["bytes"]
             out["roofline"]["traffic_detail"] = t
     if not args.no_cpu:
-        threads = os.cpu_count() or 1
+        threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
         v, secs = cpu_baseline(wl, args.cpu_steps, threads)
         out["cpu_baseline"] = {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port",
                                "sample": f"{args.cpu_steps} steps of the same workload after 1 untimed step "

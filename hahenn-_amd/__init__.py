@@ -16,6 +16,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(HERE, "libsuiri_amd.so")
 
 SL_OK = 0
+SL_EBUDGET = -5
 F_PARTICLES, F_WW, F_COUNTERS, F_ACC, F_CELLCOUNT, F_VEL, F_VELOLD, F_FRAC, F_SVEL, F_CELLTYPE, \
     F_DIVG, F_PRES, F_OBJS, F_RSRC, F_RSTATE, F_RHAT, F_RFIELD, F_RMASK, F_RTEX, F_TOPH, F_RTEX16 = range(21)
 S_GEOM, S_P2G, S_NORMFORCE, S_DIVERGENCE, S_PRESSURE, S_PROJECT, S_G2P, S_COMPACT, S_WW, S_RIPPLE, \
@@ -42,7 +43,8 @@ class SlConfig(ctypes.Structure):
         ("ww_rate", ctypes.c_float), ("r_sigma_rho", ctypes.c_float), ("r_nu", ctypes.c_float),
         ("r_film", ctypes.c_float), ("r_hp_k", ctypes.c_float), ("r_rho_kick", ctypes.c_float),
         ("wmap", ctypes.c_uint32), ("whitewater", ctypes.c_uint32), ("device", ctypes.c_int32),
-        ("reorder_every", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 4),
+        ("reorder_every", ctypes.c_uint32), ("reorder_phase", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 3),
     ]
 
 
@@ -78,7 +80,44 @@ EXPORTS = ["sl_config_default", "sl_create", "sl_destroy", "sl_last_error", "sl_
            "sl_batch_create", "sl_batch_destroy", "sl_batch_last_error", "sl_batch_size", "sl_batch_sim",
            "sl_batch_step", "sl_batch_capture", "sl_batch_sync", "sl_batch_reorder", "sl_batch_autotune",
            "sl_batch_set_variants", "sl_batch_timing_enable", "sl_batch_timing_read", "sl_batch_open",
-           "sl_batch_add", "sl_batch_remove", "sl_batch_set_stepped", "sl_batch_emit", "sl_batch_captured"]
+           "sl_batch_add", "sl_batch_remove", "sl_batch_set_stepped", "sl_batch_emit", "sl_batch_captured",
+           "sl_batch_set_dt", "sl_job_bytes", "sl_server_config_default", "sl_server_create", "sl_server_destroy",
+           "sl_server_last_error", "sl_server_submit", "sl_server_cancel", "sl_server_tick", "sl_server_poll",
+           "sl_server_info", "sl_server_job_sim", "sl_server_release", "sl_server_stats"]
+
+EMIT_FN = ctypes.CFUNCTYPE(ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
+                           ctypes.POINTER(SlEmit), ctypes.c_uint32)
+
+
+class SlServerConfig(ctypes.Structure):
+    _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_jobs", ctypes.c_uint32), ("device", ctypes.c_int32),
+                ("autotune_after", ctypes.c_uint32), ("graph_after", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 4)]
+
+
+class SlJob(ctypes.Structure):
+    _fields_ = [("cfg", SlConfig), ("objects", ctypes.POINTER(ctypes.c_float)), ("n_objects", ctypes.c_uint32),
+                ("fill", ctypes.POINTER(SlEmit)), ("n_fill", ctypes.c_uint32), ("steps", ctypes.c_uint32),
+                ("dt", ctypes.c_double), ("source", EMIT_FN), ("user", ctypes.c_void_p), ("keep", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32 * 3)]
+
+
+class SlJobInfo(ctypes.Structure):
+    _fields_ = [("id", ctypes.c_uint64), ("status", ctypes.c_int32), ("frames", ctypes.c_uint32),
+                ("steps", ctypes.c_uint32), ("slot", ctypes.c_uint32), ("t_submit", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double), ("particle_steps", ctypes.c_double),
+                ("bytes", ctypes.c_uint64), ("count", ctypes.c_uint32), ("ww_count", ctypes.c_uint32)]
+
+
+class SlServerStats(ctypes.Structure):
+    _fields_ = [("budget", ctypes.c_uint64), ("used", ctypes.c_uint64), ("queued", ctypes.c_uint32),
+                ("running", ctypes.c_uint32), ("resident", ctypes.c_uint32), ("done", ctypes.c_uint32),
+                ("ticks", ctypes.c_uint64), ("frames", ctypes.c_uint64), ("particle_steps", ctypes.c_double),
+                ("uptime", ctypes.c_double), ("step_ms_p50", ctypes.c_double), ("step_ms_p99", ctypes.c_double),
+                ("step_ms_sum", ctypes.c_double), ("latency_p50", ctypes.c_double), ("latency_p99", ctypes.c_double)]
+
+
+JOB_STATUS = {0: "queued", 1: "running", 2: "done", 3: "cancelled", 4: "refused"}
 
 _lib = None
 
@@ -145,6 +184,20 @@ def load():
         "sl_batch_set_stepped": (I, [P, U, I]),
         "sl_batch_emit": (I, [P, P, ctypes.POINTER(U)]),
         "sl_batch_captured": (I, [P]),
+        "sl_batch_set_dt": (I, [P, U, D]),
+        "sl_job_bytes": (ctypes.c_size_t, [ctypes.POINTER(SlConfig)]),
+        "sl_server_config_default": (I, [ctypes.POINTER(SlServerConfig)]),
+        "sl_server_create": (I, [ctypes.POINTER(SlServerConfig), ctypes.POINTER(P)]),
+        "sl_server_destroy": (None, [P]),
+        "sl_server_last_error": (ctypes.c_char_p, [P]),
+        "sl_server_submit": (I, [P, ctypes.POINTER(SlJob), ctypes.POINTER(ctypes.c_uint64)]),
+        "sl_server_cancel": (I, [P, ctypes.c_uint64]),
+        "sl_server_tick": (I, [P, ctypes.POINTER(U)]),
+        "sl_server_poll": (I, [P, ctypes.POINTER(SlJobInfo), U, ctypes.POINTER(U)]),
+        "sl_server_info": (I, [P, ctypes.c_uint64, ctypes.POINTER(SlJobInfo)]),
+        "sl_server_job_sim": (P, [P, ctypes.c_uint64]),
+        "sl_server_release": (I, [P, ctypes.c_uint64]),
+        "sl_server_stats": (I, [P, ctypes.POINTER(SlServerStats)]),
         "sl_batch_timing_read": (I, [P, ctypes.POINTER(D), U, ctypes.POINTER(U)]),
         "sl_version": (I, []),
         "sl_build_info": (ctypes.c_char_p, []),
@@ -418,6 +471,117 @@ class Batch:
                 t.ptr = ctypes.c_void_p()
         if self.ptr:
             self.L.sl_batch_destroy(self.ptr)
+            self.ptr = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def job_bytes(cfg):
+    """Device bytes a simulation of this configuration holds (host-side computation)."""
+    return int(load().sl_job_bytes(ctypes.byref(cfg)))
+
+
+def _info_dict(i):
+    return {"id": i.id, "status": JOB_STATUS.get(i.status, i.status), "frames": i.frames, "steps": i.steps,
+            "t_submit": i.t_submit, "t_admit": i.t_admit, "t_done": i.t_done, "particle_steps": i.particle_steps,
+            "bytes": i.bytes, "count": i.count, "ww_count": i.ww_count}
+
+
+class Server:
+    """The serving loop (sl_server_*): jobs admitted against a device-memory budget, stepped together
+    as a continuous batch, one frame per tick. `source(dt, t)` returns a job's per-frame emission
+    records (like the emitter callback of Sim.step)."""
+
+    def __init__(self, budget=200 << 30, max_jobs=128, device=0, autotune_after=8, graph_after=0):
+        self.L = load()
+        c = SlServerConfig()
+        self.L.sl_server_config_default(ctypes.byref(c))
+        c.hbm_budget, c.max_jobs, c.device = int(budget), int(max_jobs), int(device)
+        c.autotune_after, c.graph_after = int(autotune_after), int(graph_after)
+        self.ptr = ctypes.c_void_p()
+        r = self.L.sl_server_create(ctypes.byref(c), ctypes.byref(self.ptr))
+        if r != SL_OK:
+            raise SuiriError(f"suiri error {r}: sl_server_create")
+        self._keep = {}   # per job: ctypes objects the library points at (callbacks)
+
+    def _check(self, r):
+        if r != SL_OK:
+            msg = self.L.sl_server_last_error(self.ptr)
+            raise SuiriError(f"suiri error {r}: {msg.decode() if msg else ''}")
+
+    def submit(self, cfg, rows=None, fill=(), steps=100, dt=1.0 / 120, source=None, keep=False):
+        j = SlJob()
+        j.cfg = cfg
+        rows_a = np.ascontiguousarray(rows if rows is not None else np.zeros((0, 28)), dtype=np.float32).reshape(-1, 28)
+        j.objects = rows_a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+        j.n_objects = rows_a.shape[0]
+        fill_a = emit_records(list(fill))
+        j.fill = ctypes.cast(fill_a, ctypes.POINTER(SlEmit))
+        j.n_fill = len(fill)
+        j.steps, j.dt, j.keep = int(steps), float(dt), int(bool(keep))
+        cb = None
+        if source is not None:
+            def _cb(user, sim, d, t, out, mx):
+                recs = source(d, t)[:mx]
+                arr = emit_records(recs)
+                for k in range(len(recs)):
+                    out[k] = arr[k]
+                return len(recs)
+            cb = EMIT_FN(_cb)
+            j.source = cb
+        jid = ctypes.c_uint64(0)
+        r = self.L.sl_server_submit(self.ptr, ctypes.byref(j), ctypes.byref(jid))
+        if cb is not None:
+            self._keep[jid.value] = cb
+        if r == SL_EBUDGET:
+            return jid.value   # refused: reported by poll() / info()
+        self._check(r)
+        return jid.value
+
+    def cancel(self, jid):
+        self._check(self.L.sl_server_cancel(self.ptr, ctypes.c_uint64(jid)))
+
+    def tick(self):
+        n = ctypes.c_uint32(0)
+        self._check(self.L.sl_server_tick(self.ptr, ctypes.byref(n)))
+        return n.value
+
+    def poll(self):
+        buf = (SlJobInfo * 256)()
+        out = []
+        while True:
+            n = ctypes.c_uint32(0)
+            self._check(self.L.sl_server_poll(self.ptr, buf, 256, ctypes.byref(n)))
+            out.extend(_info_dict(buf[i]) for i in range(n.value))
+            for i in range(n.value):
+                self._keep.pop(buf[i].id, None)
+            if n.value < 256:
+                return out
+
+    def info(self, jid):
+        i = SlJobInfo()
+        self._check(self.L.sl_server_info(self.ptr, ctypes.c_uint64(jid), ctypes.byref(i)))
+        return _info_dict(i)
+
+    def job_sim(self, jid):
+        h = self.L.sl_server_job_sim(self.ptr, ctypes.c_uint64(jid))
+        return Sim(None, _handle=h, _owner=self) if h else None
+
+    def release(self, jid):
+        self._check(self.L.sl_server_release(self.ptr, ctypes.c_uint64(jid)))
+
+    def stats(self):
+        st = SlServerStats()
+        self._check(self.L.sl_server_stats(self.ptr, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in SlServerStats._fields_}
+
+    def close(self):
+        if self.ptr:
+            self.L.sl_server_destroy(self.ptr)
             self.ptr = ctypes.c_void_p()
 
     def __del__(self):

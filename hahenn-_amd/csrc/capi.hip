@@ -19,7 +19,7 @@ static thread_local std::string g_err;
 // dynamic per-substep fields of a tank, written on the device by k_update (kernel arguments are
 // captured at launch, so the host can change them freely while earlier work is in flight)
 struct SlTankDyn {
-  double time_d;
+  double time_d, dt_frame;
   float dt, gdt, time_f, rLx, rLz, rdt, rmd;
   uint32_t frame, obj_count, emit_k, emit_total, iters;
   int32_t mask_bound;
@@ -31,6 +31,7 @@ __global__ void k_update(SlTank *T, SlTankDyn d) {
   T->frame = d.frame; T->obj_count = d.obj_count; T->emit_k = d.emit_k; T->emit_total = d.emit_total;
   T->iters = d.iters; T->mask_bound = d.mask_bound;
   T->time_d = d.time_d;
+  T->dt_frame = d.dt_frame;
 }
 
 // pinned staging for host->device uploads issued on the stream: two halves, each reused only after
@@ -69,6 +70,8 @@ struct sl_sim {
   size_t sort_temp_bytes = 0;
   int sort_end_bit = 1;
   uint32_t reorder_clock = 0;
+  double batch_dt = 0;    // frame length of a batched tank stepped with dt = 0
+  size_t bytes = 0;       // device memory held (sum of the allocation sizes)
   int p2g_variant = 0;  // kernel variant (chunk size, LDS word size, lane mapping); see k_grid.hip
   int g2p_variant = 1;  // occupancy target of G2P's register allocation; see k_particles.hip
   double cur_dt = 1.0 / 60.0;
@@ -214,8 +217,19 @@ static int push_dyn(sl_sim *s, uint32_t emit_k = 0, uint32_t emit_total = 0) {
   d.frame = s->frame; d.obj_count = s->obj_count; d.emit_k = emit_k; d.emit_total = emit_total;
   d.iters = s->iters; d.mask_bound = s->mask_bound;
   d.time_d = s->time;
+  d.dt_frame = s->batch_dt;
   hipLaunchKernelGGL(k_update, dim3(1), dim3(64), 0, s->stream, s->d_tank, d);
   return check_launch(s, "update");
+}
+
+// The particle buffer index lives on the device (compaction flips it only when a particle died):
+// host code that copies particle buffers reads it at a synchronisation point.
+static int dev_cur(sl_sim *s) {
+  int32_t c = 0;
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipMemcpy(&c, (const uint8_t *)s->d_tank + offsetof(SlTank, cur), 4, hipMemcpyDeviceToHost));
+  s->cur = c & 1;
+  return SL_OK;
 }
 
 // writeParams, 50-sim.js:151-162 (the per-substep part; the static part is set at creation)
@@ -256,6 +270,7 @@ extern "C" int sl_config_default(sl_config *c, const char *quality) {
 
 static int alloc(sl_sim *s, void **p, size_t bytes) {
   if (!bytes) bytes = 256;
+  s->bytes += bytes;
   hipError_t e = hipMalloc(p, bytes);
   if (e != hipSuccess) FAIL(s, SL_ENOMEM, "hipMalloc(%zu) failed: %s", bytes, hipGetErrorString(e));
   s->allocs.push_back(*p);
@@ -271,17 +286,14 @@ static int alloc(sl_sim *s, void **p, size_t bytes) {
     (ptr) = (decltype(ptr))_p;                                      \
   } while (0)
 
-static int create_impl(sl_sim *s) {
+// configuration checks and grid sizing (50-sim.js:33-45); host only
+static int size_grid(sl_sim *s) {
   const sl_config &c = s->cfg;
   if (!(c.h > 0) || !(c.tank_w > 0) || !(c.tank_h > 0) || !(c.tank_d > 0))
     FAIL(s, SL_EARG, "tank size and cell size must be positive");
   if (c.ripple_R && (c.ripple_R & (c.ripple_R - 1))) FAIL(s, SL_EARG, "ripple_R must be a power of two");
   if (c.ripple_R > 2048) FAIL(s, SL_ECAP, "ripple_R above 2048");
   if (c.cap == 0) FAIL(s, SL_EARG, "capacity must be positive");
-  HIPCHK(s, hipSetDevice(c.device));
-  s->device = c.device;
-  if (s->ext_stream) s->stream = s->ext_stream;
-  else HIPCHK(s, hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   // grid sizing, 50-sim.js:33-45
   s->nx = (int)fmax(8.0, round(c.tank_w / c.h));
   s->ny = (int)fmax(8.0, round(c.tank_h / c.h));
@@ -299,7 +311,81 @@ static int create_impl(sl_sim *s) {
   s->R = c.ripple_R; s->M = 2 * c.ripple_R;
   s->vp = pow(s->h / 2, 3);
   s->radius = 0.5 * s->h * 1.05;
+  return SL_OK;
+}
+
+// Every device buffer of a tank, in allocation order (50-sim.js:59-80, 64-ripple.js:152-165,
+// structure-of-arrays). `take(&ptr, bytes)` allocates, or only counts for sl_job_bytes; the sizes
+// come from the grid of size_grid.
+#define TAKE(ptr, bytes)                                            \
+  do {                                                              \
+    void *_p = nullptr;                                             \
+    int _r = take(&_p, (size_t)(bytes));                            \
+    if (_r) return _r;                                              \
+    (ptr) = (decltype(ptr))_p;                                      \
+  } while (0)
+template <class Take>
+static int tank_buffers(sl_sim *s, SlTank &T, Take take) {
+  const size_t N = s->nodes, C = s->cells, H = s->hcells;
+  for (int b = 0; b < 2; b++) {
+    TAKE(T.px[b], (size_t)s->cap * 4); TAKE(T.py[b], (size_t)s->cap * 4); TAKE(T.pz[b], (size_t)s->cap * 4);
+    TAKE(T.vx[b], (size_t)s->cap * 4); TAKE(T.vy[b], (size_t)s->cap * 4); TAKE(T.vz[b], (size_t)s->cap * 4);
+    TAKE(T.pf[b], (size_t)s->cap);
+    TAKE(T.wp[b], (size_t)(s->ww_cap ? s->ww_cap : 1) * 16);
+    TAKE(T.wv[b], (size_t)(s->ww_cap ? s->ww_cap : 1) * 16);
+  }
+  if (s->ext_counters) T.C = s->ext_counters;
+  else TAKE(T.C, sizeof(SlCounters));
+  TAKE(T.acc, N * 24);
+  TAKE(T.cellCount, C * 4);
+  TAKE(T.vel, N * 12); TAKE(T.velOld, N * 12); TAKE(T.vmask, N);
+  TAKE(T.frac, N * 12); TAKE(T.svel, N * 12);
+  TAKE(T.divg, C * 4);
+  TAKE(T.pres, H * 8); TAKE(T.rhs, H * 8); TAKE(T.sflag, H * 4);
+  TAKE(T.objs, sizeof(SlObj) * SL_MAX_OBJ);
+  if (s->ext_emits) T.emits = s->ext_emits;
+  else TAKE(T.emits, sizeof(SlEmitDev) * SL_MAX_EMIT);
+  TAKE(T.scanBlocks, (size_t)cdiv(s->cap, SL_COMPACT_ITEMS) * 4 + 64);
+  if (s->R) {
+    const size_t M2 = (size_t)s->M * s->M, R1 = (size_t)(s->R + 1) * (s->R + 1);
+    TAKE(T.rsrc, R1 * 4);
+    TAKE(T.rstate, M2 * 8); TAKE(T.ra, M2 * 8); TAKE(T.rb, M2 * 8); TAKE(T.rhat, M2 * 8);
+    TAKE(T.tw, (size_t)s->M * 8);
+    TAKE(T.rmask, R1 * 4);
+    TAKE(T.rtex, R1 * 16);
+    const size_t wm = s->cfg.wmap ? s->cfg.wmap : 256;
+    TAKE(T.topH, wm * wm * 4);
+  }
+  // brick-Morton keys: 3 bits per level of the largest brick coordinate, + 1 for the outside key
+  {
+    uint32_t mb = (uint32_t)std::max(std::max((s->nx + 3) / 4, (s->ny + 3) / 4), (s->nz + 3) / 4);
+    int bits = 0;
+    while ((1u << bits) < mb) bits++;
+    T.brick_bits = bits;
+    s->sort_end_bit = 3 * bits + 1;
+    TAKE(s->sort_keys, (size_t)s->cap * 8);
+    TAKE(s->sort_vals, (size_t)s->cap * 8);
+    s->sort_temp_bytes = sl_sort_temp_bytes(s->cap, s->sort_end_bit);
+    TAKE(s->sort_temp, s->sort_temp_bytes);
+  }
+  if (s->ext_tank) s->d_tank = s->ext_tank;
+  else TAKE(s->d_tank, sizeof(SlTank));
+  return SL_OK;
+}
+#undef TAKE
+
+static int create_impl(sl_sim *s) {
+  const sl_config &c = s->cfg;
+  {
+    int r = size_grid(s);
+    if (r) return r;
+  }
+  HIPCHK(s, hipSetDevice(c.device));
+  s->device = c.device;
+  if (s->ext_stream) s->stream = s->ext_stream;
+  else HIPCHK(s, hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   s->time = 0; s->vmax = 1.0; s->frame = 0;
+  s->reorder_clock = c.reorder_every ? c.reorder_phase % c.reorder_every : 0;
   memset(s->objs, 0, sizeof s->objs);
 
   SlTank &T = s->ht;
@@ -327,35 +413,11 @@ static int create_impl(sl_sim *s) {
   T.R_f = (float)s->R;
   T.h_d = s->h; T.vp_d = s->vp; T.rim_d = s->rim; T.time_d = 0;
   T.rsr = c.r_sigma_rho; T.rnu = c.r_nu; T.rfilm = c.r_film; T.rhpk = c.r_hp_k; T.rrho = c.r_rho_kick;
-  // buffers (50-sim.js:59-80, 64-ripple.js:152-165), structure-of-arrays
-  const size_t N = s->nodes, C = s->cells, H = s->hcells;
-  for (int b = 0; b < 2; b++) {
-    ALLOC(T.px[b], (size_t)s->cap * 4); ALLOC(T.py[b], (size_t)s->cap * 4); ALLOC(T.pz[b], (size_t)s->cap * 4);
-    ALLOC(T.vx[b], (size_t)s->cap * 4); ALLOC(T.vy[b], (size_t)s->cap * 4); ALLOC(T.vz[b], (size_t)s->cap * 4);
-    ALLOC(T.pf[b], (size_t)s->cap);
-    ALLOC(T.wp[b], (size_t)(s->ww_cap ? s->ww_cap : 1) * 16);
-    ALLOC(T.wv[b], (size_t)(s->ww_cap ? s->ww_cap : 1) * 16);
+  {
+    int r = tank_buffers(s, T, [s](void **p, size_t bytes) { return alloc(s, p, bytes); });
+    if (r) return r;
   }
-  if (s->ext_counters) T.C = s->ext_counters;
-  else ALLOC(T.C, sizeof(SlCounters));
-  ALLOC(T.acc, N * 24);
-  ALLOC(T.cellCount, C * 4);
-  ALLOC(T.vel, N * 12); ALLOC(T.velOld, N * 12); ALLOC(T.vmask, N);
-  ALLOC(T.frac, N * 12); ALLOC(T.svel, N * 12);
-  ALLOC(T.divg, C * 4);
-  ALLOC(T.pres, H * 8); ALLOC(T.rhs, H * 8); ALLOC(T.sflag, H * 4);
-  ALLOC(T.objs, sizeof(SlObj) * SL_MAX_OBJ);
-  if (s->ext_emits) T.emits = s->ext_emits;
-  else ALLOC(T.emits, sizeof(SlEmitDev) * SL_MAX_EMIT);
-  ALLOC(T.scanBlocks, (size_t)cdiv(s->cap, SL_COMPACT_ITEMS) * 4 + 64);
   if (s->R) {
-    const size_t M2 = (size_t)s->M * s->M, R1 = (size_t)(s->R + 1) * (s->R + 1);
-    ALLOC(T.rsrc, R1 * 4);
-    ALLOC(T.rstate, M2 * 8); ALLOC(T.ra, M2 * 8); ALLOC(T.rb, M2 * 8); ALLOC(T.rhat, M2 * 8);
-    ALLOC(T.tw, (size_t)s->M * 8);
-    ALLOC(T.rmask, R1 * 4);
-    ALLOC(T.rtex, R1 * 16);
-    ALLOC(T.topH, (size_t)T.wmap * T.wmap * 4);
     // twiddles e^(2 pi i t/M) in double, rounded to f32 (64-ripple.js:175-178)
     std::vector<float> tw(2 * (size_t)s->M);
     for (uint32_t t = 0; t < s->M; t++) {
@@ -364,20 +426,6 @@ static int create_impl(sl_sim *s) {
     }
     HIPCHK(s, hipMemcpy(T.tw, tw.data(), tw.size() * 4, hipMemcpyHostToDevice));
   }
-  // brick-Morton keys: 3 bits per level of the largest brick coordinate, + 1 for the outside key
-  {
-    uint32_t mb = (uint32_t)std::max(std::max((s->nx + 3) / 4, (s->ny + 3) / 4), (s->nz + 3) / 4);
-    int bits = 0;
-    while ((1u << bits) < mb) bits++;
-    T.brick_bits = bits;
-    s->sort_end_bit = 3 * bits + 1;
-    ALLOC(s->sort_keys, (size_t)s->cap * 8);
-    ALLOC(s->sort_vals, (size_t)s->cap * 8);
-    s->sort_temp_bytes = sl_sort_temp_bytes(s->cap, s->sort_end_bit);
-    ALLOC(s->sort_temp, s->sort_temp_bytes);
-  }
-  if (s->ext_tank) s->d_tank = s->ext_tank;
-  else ALLOC(s->d_tank, sizeof(SlTank));
   // pinned host memory
   HIPCHK(s, hipHostMalloc((void **)&s->h_counters, sizeof(SlCounters), hipHostMallocDefault));
   memset(s->h_counters, 0, sizeof(SlCounters));
@@ -391,6 +439,23 @@ static int create_impl(sl_sim *s) {
   HIPCHK(s, hipMemcpy(s->d_tank, &s->ht, sizeof(SlTank), hipMemcpyHostToDevice));
   HIPCHK(s, hipDeviceSynchronize());
   return SL_OK;
+}
+
+// device bytes a simulation of this configuration holds (the sum of its allocation sizes); host only
+extern "C" size_t sl_job_bytes(const sl_config *cfg) {
+  if (!cfg) return 0;
+  sl_sim tmp;
+  tmp.cfg = *cfg;
+  if (size_grid(&tmp)) return 0;
+  SlTank T;
+  memset(&T, 0, sizeof T);
+  size_t sum = 0;
+  tank_buffers(&tmp, T, [&sum](void **p, size_t bytes) {
+    *p = nullptr;
+    sum += bytes ? bytes : 256;
+    return SL_OK;
+  });
+  return sum;
 }
 
 extern "C" int sl_create(const sl_config *cfg, sl_sim **out) {
@@ -609,7 +674,6 @@ static int encode_compact(sl_sim *s) {
   sl_k_compact_scan(L, 1);
   sl_k_compact_scatter(L);
   sl_k_count(launch_of(s, 1), 1, 0);
-  s->cur ^= 1;
   return check_launch(s, "compact");
 }
 
@@ -628,7 +692,6 @@ static int encode_reorder(sl_sim *s) {
                        s->sort_end_bit);
   if (e) { s->gpu_errors++; FAIL(s, SL_EHIP, "particle re-ordering failed: %s", hipGetErrorString((hipError_t)e)); }
   sl_k_count(launch_of(s, 1), 6, 0);
-  s->cur ^= 1;
   s->reorder_clock = 0;
   return check_launch(s, "reorder");
 }
@@ -754,6 +817,8 @@ extern "C" int sl_save_water(sl_sim *s, uint32_t *id) {
     if (rbytes) HIPCHK(s, hipMalloc(&v.ripple, rbytes));
   }
   const SlTank &T = s->ht;
+  int r = dev_cur(s);
+  if (r) return r;
   const int c = s->cur, w = s->ww_cur;
   uint8_t *p = (uint8_t *)v.parts;
   const size_t cf = (size_t)s->cap * 4;
@@ -779,6 +844,7 @@ extern "C" int sl_restore_water(sl_sim *s, uint32_t id) {
   if (!sl_has_water_copy(s, id)) return 0;
   auto &v = s->saved;
   const SlTank &T = s->ht;
+  if (dev_cur(s)) return 0;
   const int c = s->cur, w = s->ww_cur;
   const size_t cf = (size_t)s->cap * 4, wbytes = (size_t)s->ww_cap * 32, rbytes = (size_t)s->M * s->M * 8;
   uint8_t *p = (uint8_t *)v.parts;
@@ -873,6 +939,7 @@ extern "C" int sl_readback(sl_sim *s, int f, void *dst, size_t bytes) {
       const size_t P = s->cap;
       std::vector<float> a(P * 6);
       std::vector<uint8_t> fl(P);
+      if ((r = dev_cur(s))) return r;
       const int c = s->cur;
       const float *srcs[6] = {T.px[c], T.py[c], T.pz[c], T.vx[c], T.vy[c], T.vz[c]};
       for (int q = 0; q < 6; q++) if ((r = d2h(s, a.data() + q * P, srcs[q], P * 4))) return r;
@@ -971,6 +1038,7 @@ extern "C" int sl_upload(sl_sim *s, int f, const void *src, size_t bytes) {
         a[3 * P + i] = p[4]; a[4 * P + i] = p[5]; a[5 * P + i] = p[6];
         fl[i] = (uint8_t)((p[3] < 0.5f ? 0u : 1u) | (p[7] > 0.5f ? 2u : 0u));
       }
+      if ((r = dev_cur(s))) return r;
       const int c = s->cur;
       float *dsts[6] = {T.px[c], T.py[c], T.pz[c], T.vx[c], T.vy[c], T.vz[c]};
       for (int q = 0; q < 6; q++) if ((r = h2d(s, dsts[q], a.data() + q * P, P * 4))) return r;
@@ -1049,9 +1117,14 @@ extern "C" int sl_get_scalar(const sl_sim *s, const char *name, double *v) {
   else if (!strcmp(name, "vmax")) *v = s->vmax;
   else if (!strcmp(name, "full")) *v = s->full;
   else if (!strcmp(name, "p_ub")) *v = s->p_ub;
-  else if (!strcmp(name, "cur")) *v = s->cur;
+  else if (!strcmp(name, "cur")) {
+    int r = dev_cur(const_cast<sl_sim *>(s));
+    if (r) return r;
+    *v = s->cur;
+  }
   else if (!strcmp(name, "ww_cur")) *v = s->ww_cur;
   else if (!strcmp(name, "impacts")) *v = s->impacts;
+  else if (!strcmp(name, "bytes")) *v = (double)s->bytes;
   else if (!strcmp(name, "ww_count")) *v = s->ww_count;
   else if (!strcmp(name, "frames")) *v = s->frames;
   else if (!strcmp(name, "p2g_variant")) *v = s->p2g_variant;
@@ -1138,7 +1211,8 @@ static float time_region(hipEvent_t a, hipEvent_t b) {
 extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
   if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
   if (!reps) reps = 3;
-  HIPCHK(s, hipStreamSynchronize(s->stream));
+  int rc = dev_cur(s);
+  if (rc) return rc;
   const SlTank &T = s->ht;
   const int c = s->cur;
   const size_t P = s->p_ub, C = s->cells;
@@ -1228,17 +1302,21 @@ extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
 // the list of stepped slots.
 // ================================================================================================
 #define SL_BUCKETS 3
+#define SL_SOR_BANDS 16  // SOR launch maps: band j covers the tanks with more than band_start[j] iterations
 enum {
   BM_NODES, BM_CELLS, BM_HCELLS, BM_COMPACT, BM_WW, BM_ELEMS, BM_TEX, BM_ROWS, BM_COLS,
-  BM_P2G0, BM_G2P0 = BM_P2G0 + SL_BUCKETS, BM_SCRATCH = BM_G2P0 + SL_BUCKETS, BM_COUNT
+  BM_P2G0, BM_G2P0 = BM_P2G0 + SL_BUCKETS, BM_SCRATCH = BM_G2P0 + SL_BUCKETS,
+  BM_SOR0, BM_COUNT = BM_SOR0 + SL_SOR_BANDS
 };
 
 // frame++ and the per-substep uniform (50-sim.js:151-162, :308-317), and the frame's ripple uniform
 // (64-ripple.js:223-226) with the mean depth of 50-sim.js:330-333 from the count at frame start
-__global__ void k_batch_advance(SlTank *tanks, const uint32_t *list, int n, float dt, double dtd) {
+__global__ void k_batch_advance(SlTank *tanks, const uint32_t *list, int n, double dtd) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   SlTank &T = tanks[list[i]];
+  if (!(dtd > 0.0)) dtd = T.dt_frame;  // each tank's own frame length
+  const float dt = (float)dtd;
   T.frame += 1;
   T.dt = dt;
   T.gdt = T.g * T.dt;
@@ -1296,6 +1374,7 @@ struct sl_batch {
   uint32_t total[BM_COUNT] = {};
   int map_variant[BM_COUNT];
   uint32_t n_stepped = 0, max_iters = 0;
+  uint32_t sor_bands = 0, band_start[SL_SOR_BANDS + 1] = {};
   bool any_ripple = false, maps_dirty = true;
   int p2g_variant[SL_BUCKETS] = {0, 0, 0}, g2p_variant[SL_BUCKETS] = {1, 1, 1};
   hipGraph_t graph = nullptr;
@@ -1453,6 +1532,32 @@ static int batch_rebuild(sl_batch *b) {
     }
   }
   b->total[BM_SCRATCH] = 0;
+  // SOR bands: iteration i runs over the tanks with iters > i. Band j (iterations band_start[j] ..
+  // band_start[j+1]-1) covers the tanks with iters > band_start[j], a superset (k_sor returns for
+  // passes beyond a tank's own count); with more distinct counts than bands, bands are merged.
+  {
+    std::vector<uint32_t> its;
+    for (uint32_t t = 0; t < b->cap; t++)
+      if (live(b, t)) its.push_back(b->sims[t]->iters);
+    std::sort(its.begin(), its.end());
+    its.erase(std::unique(its.begin(), its.end()), its.end());
+    std::vector<uint32_t> starts{0};
+    for (size_t q = 0; q + 1 < its.size(); q++) starts.push_back(its[q]);  // a tank set shrinks after each count
+    if (starts.size() > SL_SOR_BANDS) {
+      std::vector<uint32_t> keep;
+      for (uint32_t j = 0; j < SL_SOR_BANDS; j++) keep.push_back(starts[(size_t)j * starts.size() / SL_SOR_BANDS]);
+      starts = keep;
+    }
+    b->sor_bands = (uint32_t)starts.size();
+    for (uint32_t j = 0; j < b->sor_bands; j++) {
+      const uint32_t a = starts[j];
+      b->band_start[j] = a;
+      r = put_map(b, BM_SOR0 + (int)j, [](const sl_sim *s) { return (uint64_t)s->hcells; }, SL_WG,
+                  [&](uint32_t t) { return b->sims[t]->iters > a; });
+      if (r) return r;
+    }
+    b->band_start[b->sor_bands] = b->max_iters;
+  }
   drop_graph(b);  // a captured frame holds the old launch shapes
   b->maps_dirty = false;
   return maps_upload(b);
@@ -1663,8 +1768,7 @@ static int batch_encode_frame(sl_batch *b, double dt) {
   const int n = (int)b->n_stepped;
   if (!n) return SL_OK;
   const SlLaunch P = batch_tanks(b);
-  hipLaunchKernelGGL(k_batch_advance, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_tanks, list_dev(b), n,
-                     (float)dt, dt);
+  hipLaunchKernelGGL(k_batch_advance, dim3((n + 63) / 64), dim3(64), 0, b->stream, b->d_tanks, list_dev(b), n, dt);
   sl_k_count(P, 5, 0);  // vmax = 0 (50-sim.js:306)
   hipLaunchKernelGGL(k_batch_clear_cells, dim3(b->total[BM_CELLS]), dim3(SL_WG), 0, b->stream,
                      batch_launch(b, BM_CELLS).w);
@@ -1672,8 +1776,10 @@ static int batch_encode_frame(sl_batch *b, double dt) {
     if (b->map_variant[BM_P2G0 + u] >= 0) sl_k_p2g(batch_launch(b, BM_P2G0 + u), 0, b->map_variant[BM_P2G0 + u]);
   sl_k_normforce(batch_launch(b, BM_NODES));
   sl_k_divsetup(batch_launch(b, BM_CELLS));
-  const SlLaunch Ls = batch_launch(b, BM_HCELLS);  // k_sor returns for passes beyond a tank's iters
-  for (uint32_t i = 0; i < b->max_iters; i++) { sl_k_sor(Ls, 0, 2 * i); sl_k_sor(Ls, 1, 2 * i + 1); }
+  for (uint32_t j = 0; j < b->sor_bands; j++) {  // only the tanks that still iterate
+    const SlLaunch Ls = batch_launch(b, BM_SOR0 + (int)j);
+    for (uint32_t i = b->band_start[j]; i < b->band_start[j + 1]; i++) { sl_k_sor(Ls, 0, 2 * i); sl_k_sor(Ls, 1, 2 * i + 1); }
+  }
   sl_k_project(batch_launch(b, BM_NODES));
   for (int u = 0; u < SL_BUCKETS; u++)
     if (b->map_variant[BM_G2P0 + u] >= 0) sl_k_g2p(batch_launch(b, BM_G2P0 + u), b->map_variant[BM_G2P0 + u]);
@@ -1684,7 +1790,7 @@ static int batch_encode_frame(sl_batch *b, double dt) {
   sl_k_compact_scan(Lscan, n);
   sl_k_compact_scatter(Lc);
   sl_k_count(P, 1, 0);
-  sl_k_ww(batch_launch(b, BM_WW), (float)dt);
+  sl_k_ww(batch_launch(b, BM_WW), (float)dt);  // dt = 0: each tank's own (its substep dt)
   sl_k_count(P, 3, 0);
   if (b->any_ripple)  // the mask stages return for tanks whose mask is off
     sl_k_ripple_frame_ex(batch_launch(b, BM_ROWS).w, batch_launch(b, BM_COLS).w, batch_launch(b, BM_ELEMS).w,
@@ -1720,10 +1826,10 @@ static int batch_host_frame(sl_batch *b, double dt) {
     sl_sim *s = b->sims[t];
     if (s) b->copy_valid[t] = 1;  // the frame's counters copy covers every tank in the table
     if (!s || !b->stepped[t]) continue;
+    const double d = dt > 0 ? dt : s->batch_dt;
     s->frame++;
-    write_params(s, dt);
-    s->time += dt;
-    s->cur ^= 1;
+    write_params(s, d);
+    s->time += d;
     s->ww_cur ^= 1;
     s->frames++;
     s->substeps_last = 1;
@@ -1736,9 +1842,27 @@ static int batch_host_frame(sl_batch *b, double dt) {
   return SL_OK;
 }
 
+// dt > 0: every stepped tank advances by dt; dt = 0: each by its own frame length (sl_batch_set_dt)
+static int check_dt(sl_batch *b, double dt, const char *who) {
+  if (dt > 0) return SL_OK;
+  if (dt != 0) return batch_fail(b, SL_EARG, std::string(who) + ": dt must be positive, or 0 for per-tank dt");
+  for (uint32_t t = 0; t < b->cap; t++)
+    if (live(b, t) && !(b->sims[t]->batch_dt > 0))
+      return batch_fail(b, SL_EARG, std::string(who) + ": a stepped tank has no frame length (sl_batch_set_dt)");
+  return SL_OK;
+}
+
+extern "C" int sl_batch_set_dt(sl_batch *b, uint32_t slot, double dt) {
+  if (!b || slot >= b->cap || !b->sims[slot] || !(dt > 0)) return batch_fail(b, SL_EARG, "sl_batch_set_dt: bad arguments");
+  b->sims[slot]->batch_dt = dt;
+  int r = push_dyn(b->sims[slot]);
+  return r ? batch_fail(b, r, b->sims[slot]->err) : SL_OK;
+}
+
 extern "C" int sl_batch_step(sl_batch *b, double dt) {
   if (!b) return batch_fail(nullptr, SL_EARG, "null batch");
-  if (!(dt > 0)) return batch_fail(b, SL_EARG, "sl_batch_step: dt must be positive");
+  int rc = check_dt(b, dt, "sl_batch_step");
+  if (rc) return rc;
   int r = batch_prepare(b);
   if (r) return r;
   if (!b->n_stepped) return SL_OK;
@@ -1769,7 +1893,8 @@ extern "C" int sl_batch_step(sl_batch *b, double dt) {
 // variant choice changes.
 extern "C" int sl_batch_capture(sl_batch *b, double dt) {
   if (!b) return batch_fail(nullptr, SL_EARG, "null batch");
-  if (!(dt > 0)) return batch_fail(b, SL_EARG, "sl_batch_capture: dt must be positive");
+  int rc = check_dt(b, dt, "sl_batch_capture");
+  if (rc) return rc;
   int r = batch_prepare(b);
   if (r) return r;
   if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, SL_EHIP, "sync before capture");
@@ -1861,6 +1986,7 @@ extern "C" int sl_batch_autotune(sl_batch *b, uint32_t reps, uint32_t *choice) {
   for (uint32_t t = 0; t < b->cap; t++) {
     if (!live(b, t)) continue;
     sl_sim *s = b->sims[t];
+    if ((r = dev_cur(s))) return batch_fail(b, r, s->err);
     const SlTank &T = s->ht;
     const int c = s->cur;
     const size_t P = s->cap;

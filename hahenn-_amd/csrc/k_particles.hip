@@ -381,7 +381,11 @@ __global__ void k_count(SlTank *tanks, const uint32_t *list, int ntanks, uint32_
   SlTank &T = tanks[list ? list[i] : i];
   SlCounters *C = T.C;
   uint32_t cap = T.cap, wcap = f2u(T.ww_cap_f);
-  if (mode == 1) { C->count = C->newCount; C->newCount = 0; T.cur = 1 - T.cur; }
+  if (mode == 1) {  // the scatter ran only when a particle died; otherwise the buffer stays current
+    if (C->newCount != min(C->count, cap)) T.cur = 1 - T.cur;
+    C->count = C->newCount;
+    C->newCount = 0;
+  }
   if (mode == 2) { uint32_t yy = y ? y : T.emit_total; uint32_t s = C->count + yy; C->count = s < cap ? s : cap; }
   if (mode == 3) { C->wwCount = C->wwNew < wcap ? C->wwNew : wcap; C->wwNew = 0; T.ww_cur = 1 - T.ww_cur; }
   if (mode == 4) { C->count = 0; C->newCount = 0; C->wwCount = 0; C->wwNew = 0; C->impacts = 0; }
@@ -401,7 +405,8 @@ void sl_k_count(const SlLaunch &L, uint32_t mode, uint32_t y) {
 // ------------------------------------------------------------------------------------------------
 // COMPACT, 31-wgsl-sim2.js:317-328, order-preserving: (1) live count per workgroup of
 // SL_COMPACT_ITEMS particles, (2) exclusive scan of the workgroup counts per tank (+ newCount),
-// (3) scatter of the live particles to the other buffer at offset + rank within the workgroup.
+// (3) scatter of the live particles to the other buffer at offset + rank within the workgroup,
+// skipped (and the buffers not flipped, COUNT mode 1) when every particle is alive.
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(SL_WG) void k_compact_count(SlWork W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
@@ -476,7 +481,7 @@ __global__ __launch_bounds__(SL_WG) void k_compact_scatter(SlWork W) {
   const SlTank &T = W.tanks[t];
   const uint32_t lim = min(T.C->count, T.cap);
   const uint32_t base = lb * SL_COMPACT_ITEMS;
-  if (base >= lim) return;
+  if (base >= lim || T.C->newCount == lim) return;  // nothing died: the order is already compact
   const int cur = T.cur, oth = 1 - cur;
   const uint8_t *pf = T.pf[cur];
   __shared__ uint32_t wtot[SL_COMPACT_ITEMS / 64];
@@ -515,9 +520,11 @@ void sl_k_compact_scatter(const SlLaunch &L) {
 // ------------------------------------------------------------------------------------------------
 // WW, 31-wgsl-sim2.js:361-409: spray / foam / bubbles; survivors appended to the other buffer
 // ------------------------------------------------------------------------------------------------
+// dt = 0: the tank's substep dt (batched steps, where every tank runs one substep per frame)
 __global__ __launch_bounds__(SL_WG) void k_ww(SlWork W, float dt) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
+  if (!(dt > 0.0f)) dt = T.dt;
   const uint32_t wcap = f2u(T.ww_cap_f);
   uint32_t id = lb * SL_WG + threadIdx.x;
   if (id >= min(T.C->wwCount, wcap)) return;
@@ -562,12 +569,11 @@ __global__ __launch_bounds__(SL_WG) void k_ww(SlWork W, float dt) {
   if (d < 0.0f) x = sub3(x, mul3(scene_normal(T, x), d));
   tank_clamp(T, xb, x, v, 0.001f);
   bool outside = !in_footprint(T, x) && x.y < 0.001f;
-  if (life > 0.0f && !outside && x.y > -0.5f && !in_drain(T, x)) {
-    uint32_t k = atomicAdd(&T.C->wwNew, 1u);
-    if (k < wcap) {
-      T.wp[1 - wc][k] = make_float4(x.x, x.y, x.z, life);
-      T.wv[1 - wc][k] = make_float4(v.x, v.y, v.z, kind);
-    }
+  const bool keep = life > 0.0f && !outside && x.y > -0.5f && !in_drain(T, x);
+  const uint32_t k = wave_append(&T.C->wwNew, keep);  // the survivors of the wave, one atomic
+  if (keep && k < wcap) {
+    T.wp[1 - wc][k] = make_float4(x.x, x.y, x.z, life);
+    T.wv[1 - wc][k] = make_float4(v.x, v.y, v.z, kind);
   }
 }
 void sl_k_ww(const SlLaunch &L, float dt) {

@@ -17,8 +17,10 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
-#define SL_MAX_OBJ 64
+#ifndef SL_MAX_OBJ
+#define SL_MAX_OBJ 64   // as include/suiri.h
 #define SL_MAX_EMIT 48
+#endif
 #define SL_WG 256
 
 // per-cell SOR stencil flags (sflag)
@@ -45,6 +47,7 @@ struct SlTank {
   float pad_f[3];
   // host-precision values for the device-side frame advance of batched steps
   double time_d, h_d, vp_d, rim_d;
+  double dt_frame;  // a batched tank's own frame length (used when the batch step passes dt = 0)
   // particles
   float *px[2], *py[2], *pz[2], *vx[2], *vy[2], *vz[2];
   uint8_t *pf[2];
@@ -158,6 +161,20 @@ __device__ __forceinline__ uint32_t sl_morton3(uint32_t x, uint32_t y, uint32_t 
 }
 // key of particles outside the grid: above every brick key
 __device__ __forceinline__ uint32_t sl_brick_sentinel(const SlTank &T) { return 1u << (3 * T.brick_bits); }
+
+// Append with one atomic per wavefront: the lanes with `keep` get consecutive slots in lane order
+// after the counter's old value (the reference appends with one atomicAdd per item; the order of
+// appended items is unspecified there too). Every active lane of the wave must call it.
+__device__ __forceinline__ uint32_t wave_append(uint32_t *counter, bool keep) {
+  const uint64_t m = __ballot(keep);
+  if (!m) return 0xffffffffu;
+  const int lane = __lane_id();
+  const int leader = __ffsll((unsigned long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(m));
+  base = __shfl(base, leader, 64);
+  return keep ? base + (uint32_t)__popcll(m & ((1ull << lane) - 1ull)) : 0xffffffffu;
+}
 
 // largest t with start[t] <= b
 __device__ __forceinline__ int tank_of(const uint32_t *start, int ntanks, uint32_t b) {

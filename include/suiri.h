@@ -39,6 +39,9 @@ extern "C" {
 #define SL_EBUDGET -5  /* admission refused: HBM budget */
 #define SL_ECAP -6     /* size above a fixed maximum */
 
+#define SL_MAX_OBJ 64   /* obstacles and drains per tank (MAX_OBJ, 50-sim.js:9) */
+#define SL_MAX_EMIT 48  /* emission records per substep (MAX_EMIT, 50-sim.js:10) */
+
 typedef struct sl_sim sl_sim;
 
 /* Quality preset + physics (QUALITY, src/js/50-sim.js:4-8; constants of writeParams, :151-162;
@@ -56,7 +59,9 @@ typedef struct sl_config {
   uint32_t whitewater;           /* settings.whitewater: 0 sets the whitewater rate to 0 (50-sim.js:159) */
   int32_t device;                /* HIP device ordinal */
   uint32_t reorder_every;        /* spatial particle re-ordering period in frames (0 = never) */
-  uint32_t reserved[4];
+  uint32_t reorder_phase;        /* frames already counted towards the first re-ordering (staggers
+                                    the re-orderings of many tanks) */
+  uint32_t reserved[3];
 } sl_config;
 
 /* One emission record, as produced by sourceEmits / burstEmit / blockEmits (src/js/55-emit.js:13-106)
@@ -253,6 +258,76 @@ int sl_batch_set_variants(sl_batch *batch, const uint32_t *choice);
    durations [ms] of the steps since enable or the last read, oldest first */
 int sl_batch_timing_enable(sl_batch *batch, int enable);
 int sl_batch_timing_read(sl_batch *batch, double *ms, uint32_t max, uint32_t *n);
+/* a tank's own frame length, used by sl_batch_step(batch, 0) (tanks of one batch may differ) */
+int sl_batch_set_dt(sl_batch *batch, uint32_t slot, double dt);
+
+/* device bytes a simulation of this configuration holds (sum of its allocation sizes; host only) */
+size_t sl_job_bytes(const sl_config *cfg);
+
+/* Serving loop (no counterpart in the reference, whose bake.py runs one scene at a time).
+   Scene-bake jobs are submitted at any time; each tick admits queued jobs (first fit in arrival
+   order) while a batch slot and the device-memory budget allow, advances every running job by one
+   frame of its own dt in one batched step, and retires the jobs that reached their frame count.
+   State stays resident on the device from admission to retirement (or to sl_server_release for
+   jobs submitted with keep = 1). A job's result does not depend on the other jobs. */
+typedef struct sl_server sl_server;
+typedef struct {
+  uint64_t hbm_budget;      /* device bytes the resident jobs may hold (sl_job_bytes each) */
+  uint32_t max_jobs;        /* batch slots */
+  int32_t device;
+  uint32_t autotune_after;  /* ticks of unchanged membership before the one variant tuning (0 = never) */
+  uint32_t graph_after;     /* capture the frame after this many unchanged ticks (0 = eager launches) */
+  uint32_t reserved[4];
+} sl_server_config;
+typedef struct {
+  sl_config cfg;            /* the tank */
+  const float *objects;     /* obstacles and drains, 28 floats each (as sl_set_objects); copied */
+  uint32_t n_objects;
+  const sl_emit *fill;      /* emitted at admission (water blocks); copied */
+  uint32_t n_fill;
+  uint32_t steps;           /* frames to run */
+  double dt;                /* frame length, one substep per frame */
+  sl_emit_fn source;        /* per-frame emission (sources), or NULL */
+  void *user;
+  uint32_t keep;            /* keep the final state resident until sl_server_release */
+  uint32_t reserved[3];
+} sl_job;
+enum { SL_JOB_QUEUED = 0, SL_JOB_RUNNING = 1, SL_JOB_DONE = 2, SL_JOB_CANCELLED = 3, SL_JOB_REFUSED = 4 };
+typedef struct {
+  uint64_t id;
+  int32_t status;
+  uint32_t frames, steps, slot;
+  double t_submit, t_admit, t_done;   /* seconds on the server clock; -1 when not reached */
+  double particle_steps;
+  uint64_t bytes;
+  uint32_t count, ww_count;
+} sl_job_info;
+typedef struct {
+  uint64_t budget, used;
+  uint32_t queued, running, resident, done;
+  uint64_t ticks, frames;
+  double particle_steps, uptime;      /* uptime [s] */
+  double step_ms_p50, step_ms_p99, step_ms_sum;   /* device time per batched frame */
+  double latency_p50, latency_p99;    /* submit -> done of finished jobs [s] */
+} sl_server_stats_t;
+int sl_server_config_default(sl_server_config *cfg);
+int sl_server_create(const sl_server_config *cfg, sl_server **out);
+void sl_server_destroy(sl_server *srv);
+const char *sl_server_last_error(const sl_server *srv);
+/* queues a job; SL_EBUDGET (job refused, *id still set) when it alone exceeds the budget */
+int sl_server_submit(sl_server *srv, const sl_job *job, uint64_t *id);
+/* cancels a queued or running job at the next tick (its memory is freed then) */
+int sl_server_cancel(sl_server *srv, uint64_t id);
+/* one scheduling round and one frame of every running job; *running = jobs stepped */
+int sl_server_tick(sl_server *srv, uint32_t *running);
+/* jobs retired (done, cancelled, refused) since the last poll */
+int sl_server_poll(sl_server *srv, sl_job_info *out, uint32_t max, uint32_t *n);
+int sl_server_info(const sl_server *srv, uint64_t id, sl_job_info *out);
+/* the resident state of a running or kept job (readback through the sl_sim entry points) */
+sl_sim *sl_server_job_sim(sl_server *srv, uint64_t id);
+/* frees a finished job kept resident */
+int sl_server_release(sl_server *srv, uint64_t id);
+int sl_server_stats(const sl_server *srv, sl_server_stats_t *out);
 
 /* ABI version + build description */
 int sl_version(void);

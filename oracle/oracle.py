@@ -23,7 +23,7 @@ class OrQuality(ctypes.Structure):
         ("drift", ctypes.c_float), ("omega", ctypes.c_float), ("friction", ctypes.c_float),
         ("ww_rate", ctypes.c_float), ("r_sigma_rho", ctypes.c_float), ("r_nu", ctypes.c_float),
         ("r_film", ctypes.c_float), ("r_hp_k", ctypes.c_float), ("r_rho_kick", ctypes.c_float),
-        ("wmap", ctypes.c_uint32), ("reorder_every", ctypes.c_uint32),
+        ("wmap", ctypes.c_uint32), ("reorder_every", ctypes.c_uint32), ("reorder_phase", ctypes.c_uint32),
     ]
 
 

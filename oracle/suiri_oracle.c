@@ -1045,6 +1045,7 @@ static void *xcalloc(size_t n, size_t sz) { void *p = calloc(n ? n : 1, sz); if 
 or_sim *or_create(double tank_w, double tank_h, double tank_d, const or_quality *q) {
   or_sim *s = (or_sim *)xcalloc(1, sizeof(or_sim));
   s->q = *q;
+  s->reorder_clock = q->reorder_every ? q->reorder_phase % q->reorder_every : 0;
   s->nthreads = 1;
   /* grid sizing, 50-sim.js:33-45 */
   s->h = q->h;

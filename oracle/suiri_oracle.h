@@ -63,6 +63,7 @@ typedef struct {
   float r_sigma_rho, r_nu, r_film, r_hp_k, r_rho_kick;
   uint32_t wmap;     /* top-height map size used by the ripple mask (256 in the reference) */
   uint32_t reorder_every; /* spatial re-ordering period in frames (0 = never) */
+  uint32_t reorder_phase; /* frames already counted towards the first re-ordering */
 } or_quality;
 
 typedef struct or_sim {
