@@ -196,6 +196,106 @@ def batched_leg(n_tanks, steps, warmup, seq_steps, graph_steps=50, seed=7):
     }
 
 
+def serve_leg(rate=20.0, window=3.0, limit=45.0, steps=(100, 2000), kinds=(0, 1, 3), seed=11, budget_gb=200,
+              max_jobs=128):
+    """Serving stream: scene-bake requests arrive as a Poisson process (`rate` per second for `window`
+    seconds), each scene drawn from the 64x32x32 dam break (0), the 256x128x128 faucet (1) and the
+    batched-tank mix (3), each with 100..2000 frames; admitted against a `budget_gb` device-memory
+    budget, stepped together as a continuous batch. The run stops when the stream drains or at
+    `limit` seconds. Request latency = submit -> done on the server clock; per-frame latency from
+    hipEvents around each batched frame."""
+    rng = np.random.default_rng(seed)
+    arrivals, t = [], 0.0
+    while True:
+        t += rng.exponential(1.0 / rate)
+        if t > window:
+            break
+        arrivals.append(t)
+    reqs = []
+    for i, ta in enumerate(arrivals):
+        kind = int(rng.choice(kinds))
+        n = int(rng.integers(steps[0], steps[1] + 1))
+        wl = workloads.dam_break() if kind == 0 else (workloads.faucet() if kind == 1 else
+                                                        workloads.mix_tank(seed * 7919 + i))
+        reqs.append(dict(t=ta, kind=kind, steps=n, wl=wl))
+    srv = suiri.Server(budget=int(budget_gb * (1 << 30)), max_jobs=max_jobs, autotune_after=8)
+    ids = {}
+    t0 = time.perf_counter()
+    i = 0
+    while True:
+        now = time.perf_counter() - t0
+        while i < len(reqs) and reqs[i]["t"] <= now:
+            r = reqs[i]
+            rows, fill, src = workloads.job_inputs(r["wl"])
+            q = "high" if r["kind"] == 1 else "standard"
+            ids[srv.submit(suiri.config(q, **r["wl"]["cfg"]), rows=rows, fill=fill, steps=r["steps"],
+                           dt=r["wl"]["frame_dt"], source=src)] = i
+            i += 1
+        running = srv.tick()
+        now = time.perf_counter() - t0
+        st = srv.stats()
+        if now > limit or (i == len(reqs) and st["queued"] == 0 and running == 0):
+            break
+        if running == 0 and st["queued"] == 0 and i < len(reqs):
+            time.sleep(max(0.0, reqs[i]["t"] - now))
+    elapsed = time.perf_counter() - t0
+    st = srv.stats()
+    infos = [srv.info(j) for j in ids]
+    srv.close()
+    done = [x for x in infos if x["status"] == "done"]
+    lat = np.array([x["t_done"] - x["t_submit"] for x in done]) if done else np.zeros(1)
+    # sequential plain solves of the same requests: per-kind time per frame measured alone on the plain
+    # path (faucet and dam break at their sizes, batched-mix tanks on a sample), applied to each request
+    def plain_rate(wl, q, k=8):
+        g = suiri.Sim(suiri.config(q, **wl["cfg"]))
+        workloads.setup(g, wl)
+        g.reorder()
+        g.set_substeps(1)
+        em = scene.Emitters(wl["items"], g.vp, seed=99) if wl["sources"] else None
+        for _ in range(2):
+            g.step(wl["frame_dt"], em)
+        g.sync()
+        c0 = g.stats()["count"]
+        t1 = time.perf_counter()
+        for _ in range(k):
+            g.step(wl["frame_dt"], em)
+        g.sync()
+        dtm = (time.perf_counter() - t1) / k
+        c1 = g.stats()["count"]
+        g.close()
+        return dtm, (c0 + c1) / 2
+    per_kind = {}
+    if 0 in kinds:
+        per_kind[0] = plain_rate(workloads.dam_break(), "standard", 20)
+    if 1 in kinds:
+        per_kind[1] = plain_rate(workloads.faucet(), "high", 8)
+    if 3 in kinds:
+        sample = [plain_rate(workloads.mix_tank(seed * 7919 + j), "standard", 10) for j in range(6)]
+        per_kind[3] = (sum(x[0] for x in sample) / len(sample), sum(x[1] for x in sample) / len(sample))
+    seq_t = sum(r["steps"] * per_kind[r["kind"]][0] for r in reqs)
+    seq_ps = sum(r["steps"] * per_kind[r["kind"]][1] for r in reqs)
+    served = st["particle_steps"] / elapsed
+    return {
+        "workload": f"Poisson {rate}/s for {window} s, scenes from dam-break 64x32x32 / faucet 256x128x128 / "
+                    f"batched-mix tanks, {steps[0]}-{steps[1]} frames each, {budget_gb} GB budget, "
+                    f"stopped at {limit} s or when drained",
+        "requests": len(reqs), "completed": len(done),
+        "unfinished": len(reqs) - len(done) - sum(x["status"] in ("cancelled", "refused") for x in infos),
+        "refused": sum(x["status"] == "refused" for x in infos),
+        "elapsed_s": elapsed,
+        "value": served, "unit": "particle-steps/s",
+        "frames": st["frames"],
+        "frame_latency_ms": {"p50": st["step_ms_p50"], "p99": st["step_ms_p99"]},
+        "request_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
+                              "of": "completed requests"},
+        "sequential_estimate": {"value": seq_ps / seq_t, "unit": "particle-steps/s",
+                                "method": "each request's frames at the per-frame time of its scene run alone "
+                                          "on the plain path"},
+        "speedup_vs_sequential": served / (seq_ps / seq_t),
+        "offered_load_x_capacity": (seq_ps / max(served, 1.0)) / window,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -206,12 +306,16 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ripple", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter runs for `traffic`")
-    ap.add_argument("--only", choices=["all", "headline", "batched"], default="all",
+    ap.add_argument("--only", choices=["all", "headline", "batched", "serve"], default="all",
                     help="run one leg (its JSON object alone) instead of the full line")
     ap.add_argument("--batch-tanks", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--seq-steps", type=int, default=10)
     ap.add_argument("--graph-steps", type=int, default=50)
+    ap.add_argument("--serve-rate", type=float, default=20.0)
+    ap.add_argument("--serve-window", type=float, default=3.0)
+    ap.add_argument("--serve-limit", type=float, default=45.0)
+    ap.add_argument("--serve-kinds", default="0,1,3")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -229,6 +333,10 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
                           "n_gpus": args.gpus}))
         sys.exit(2)
+    if args.only == "serve":
+        print(json.dumps(serve_leg(args.serve_rate, args.serve_window, args.serve_limit,
+                                   kinds=tuple(int(k) for k in args.serve_kinds.split(",")))))
+        return
     if args.only == "batched":
         print(json.dumps(batched_leg(args.batch_tanks, args.batch_steps, 5, args.seq_steps, args.graph_steps)))
         return
@@ -300,6 +408,8 @@ def main():
     if args.only == "all":
         out["batched"] = batched_leg(args.batch_tanks, args.batch_steps, 5, args.seq_steps, args.graph_steps)
         out["batched_p99_step_ms"] = out["batched"]["step_latency_ms"]["p99"]
+        out["served"] = serve_leg(args.serve_rate, args.serve_window, args.serve_limit,
+                                  kinds=tuple(int(k) for k in args.serve_kinds.split(",")))
     if not args.no_pmc:
         t = pmc_traffic(os.path.join(ROOT, "gpurun_out", "bench_pmc"))
         if t:

@@ -14,6 +14,8 @@ emitters), as configurations + object lists + fill records for the library.
 Particles come from the reference's jittered half-cell lattice emitter (shape 2,
 src/js/31-wgsl-sim2.js:295-306) with recorded seeds; velocities start at zero.
 """
+import math
+
 import numpy as np
 
 from . import scene
@@ -98,6 +100,25 @@ def tank_mix(n=64, seed=7, steps=500):
     for i, wl in enumerate(mix):
         wl["cfg"]["reorder_phase"] = i % 16
     return mix
+
+
+def grid_of(cfg):
+    """Grid of a tank configuration as the library sizes it (50-sim.js:33-45): (nx, ny, nz, h, origin)."""
+    def rnd(x):  # C round(): halves away from zero
+        return int(math.floor(x + 0.5))
+    h = cfg["h"]
+    nx, ny, nz = (max(8, rnd(cfg[k] / h)) for k in ("tank_w", "tank_h", "tank_d"))
+    hh = min(cfg["tank_w"] / nx, cfg["tank_h"] / ny, cfg["tank_d"] / nz)
+    return nx, ny, nz, hh, [-nx * hh / 2, 0.0, -nz * hh / 2]
+
+
+def job_inputs(wl):
+    """(object rows, fill records, source emitter or None) of a workload, without a simulation."""
+    nx, ny, nz, hh, origin = grid_of(wl["cfg"])
+    rows, _ = scene.ObjectPacker().pack(wl["items"], 0)
+    fill = [scene.cell_block_emit(hh, origin, *blk, seed=wl["seed"] + i) for i, blk in enumerate(wl["blocks"])]
+    src = scene.Emitters(wl["items"], (hh / 2) ** 3, seed=wl["seed"] + 99) if wl["sources"] else None
+    return rows, fill, src
 
 
 def setup(sim, wl):
