@@ -346,6 +346,9 @@ static int tank_buffers(sl_sim *s, SlTank &T, Take take) {
   if (s->ext_emits) T.emits = s->ext_emits;
   else TAKE(T.emits, sizeof(SlEmitDev) * SL_MAX_EMIT);
   TAKE(T.scanBlocks, (size_t)cdiv(s->cap, SL_COMPACT_ITEMS) * 4 + 64);
+  TAKE(T.binCell, (size_t)s->cap * 4); TAKE(T.binRank, (size_t)s->cap * 4); TAKE(T.perm, (size_t)s->cap * 4);
+  TAKE(T.cellStart, C * 4 + 64);
+  TAKE(T.binTop, (size_t)sl_bin_blocks(s->cells) * 4 + 64);
   if (s->R) {
     const size_t M2 = (size_t)s->M * s->M, R1 = (size_t)(s->R + 1) * (s->R + 1);
     TAKE(T.rsrc, R1 * 4);
@@ -611,6 +614,16 @@ extern "C" int sl_step_begin(sl_sim *s, double frame_dt, uint32_t *nsub, double 
   return check_launch(s, "step_begin");
 }
 
+// P2G of one tank (cell counts cleared by the caller): a chunked LDS variant or the binned one
+static void encode_p2g(sl_sim *s, int variant) {
+  const uint32_t pb = s->p_ub;
+  if (variant == SL_P2G_BINNED)
+    sl_k_p2g_binned(launch_of(s, pb), launch_of(s, sl_bin_blocks(s->cells), 1), launch_of(s, 1, 1),
+                    launch_of(s, sl_bin_tiles(s->nx, s->ny, s->nz), 1));
+  else
+    sl_k_p2g(launch_of(s, pb, sl_p2g_chunk(variant)), 0, variant);
+}
+
 static int encode_substep(sl_sim *s) {
   const uint32_t pb = s->p_ub;
   if (s->geom_dirty) {
@@ -621,7 +634,7 @@ static int encode_substep(sl_sim *s) {
   {
     StageScope sc(s, SL_T_P2G, 2);
     sl_k_fill_u32(s->stream, s->ht.cellCount, 0, s->cells);
-    sl_k_p2g(launch_of(s, pb, sl_p2g_chunk(s->p2g_variant)), 0, s->p2g_variant);
+    encode_p2g(s, s->p2g_variant);
   }
   { StageScope sc(s, SL_T_NORMFORCE, 1); sl_k_normforce(launch_of(s, s->nodes)); }
   { StageScope sc(s, SL_T_DIVERGENCE, 1); sl_k_divsetup(launch_of(s, s->cells)); }
@@ -1103,7 +1116,10 @@ extern "C" int sl_set_scalar(sl_sim *s, const char *name, double v) {
   else if (!strcmp(name, "full")) s->full = v != 0;
   else if (!strcmp(name, "p_ub")) s->p_ub = (uint32_t)v;
   else if (!strcmp(name, "geom_dirty")) s->geom_dirty = v != 0;
-  else if (!strcmp(name, "p2g_variant")) s->p2g_variant = (int)v & 7;
+  else if (!strcmp(name, "p2g_variant")) {
+    if (!(v >= 0 && v <= SL_P2G_BINNED)) FAIL(s, SL_EARG, "p2g_variant must be 0..%d", SL_P2G_BINNED);
+    s->p2g_variant = (int)v;
+  }
   else if (!strcmp(name, "g2p_variant")) s->g2p_variant = (int)v & 3;
   else if (!strcmp(name, "reorder_every")) s->cfg.reorder_every = (uint32_t)v;
   else FAIL(s, SL_EARG, "unknown scalar '%s'", name);
@@ -1146,7 +1162,7 @@ extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
     case SL_S_P2G:
       sl_k_fill_u32(s->stream, (uint32_t *)s->ht.acc, 0, (size_t)s->nodes * 6);
       sl_k_fill_u32(s->stream, s->ht.cellCount, 0, s->cells);
-      sl_k_p2g(launch_of(s, s->p_ub, sl_p2g_chunk(s->p2g_variant)), 0, s->p2g_variant);
+      encode_p2g(s, s->p2g_variant);
       break;
     case SL_S_NORMFORCE: sl_k_normforce(launch_of(s, s->nodes)); break;
     case SL_S_DIVERGENCE: sl_k_divsetup(launch_of(s, s->cells)); break;
@@ -1250,13 +1266,13 @@ extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
   // P2G variants: clear, accumulate, time; the accumulators are cleared again afterwards
   float best = 1e30f;
   int bestv = s->p2g_variant;
-  for (int v = 0; v < 8; v++) {
+  for (int v = 0; v <= SL_P2G_BINNED; v++) {
     float tot = 0;
     for (uint32_t r = 0; r <= reps; r++) {
       sl_k_fill_u32(s->stream, (uint32_t *)T.acc, 0, (size_t)s->nodes * 6);
       sl_k_fill_u32(s->stream, T.cellCount, 0, s->cells);
       (void)hipEventRecord(a, s->stream);
-      sl_k_p2g(launch_of(s, P, sl_p2g_chunk(v)), 0, v);
+      encode_p2g(s, v);
       (void)hipEventRecord(b, s->stream);
       float ms = time_region(a, b);
       if (r) tot += ms;  // the first run warms the caches
@@ -1306,6 +1322,7 @@ extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
 enum {
   BM_NODES, BM_CELLS, BM_HCELLS, BM_COMPACT, BM_WW, BM_ELEMS, BM_TEX, BM_ROWS, BM_COLS,
   BM_P2G0, BM_G2P0 = BM_P2G0 + SL_BUCKETS, BM_SCRATCH = BM_G2P0 + SL_BUCKETS,
+  BM_BINP, BM_BINS, BM_BINT, BM_BINTILE,  // binned P2G: particles, scan blocks, tanks, tiles
   BM_SOR0, BM_COUNT = BM_SOR0 + SL_SOR_BANDS
 };
 
@@ -1488,6 +1505,19 @@ static uint32_t cols_of(const sl_sim *s) {
   return (M + CW - 1) / CW;
 }
 
+// the four maps of the binned P2G over the slots `inc` selects
+template <class Inc>
+static int put_bin_maps(sl_batch *b, Inc inc) {
+  int r;
+  if ((r = put_map(b, BM_BINP, [](const sl_sim *s) { return (uint64_t)s->cap; }, SL_WG, inc)) ||
+      (r = put_map(b, BM_BINS, [](const sl_sim *s) { return (uint64_t)sl_bin_blocks(s->cells); }, 1, inc)) ||
+      (r = put_map(b, BM_BINT, [](const sl_sim *) { return (uint64_t)1; }, 1, inc)) ||
+      (r = put_map(b, BM_BINTILE, [](const sl_sim *s) { return (uint64_t)sl_bin_tiles(s->nx, s->ny, s->nz); }, 1, inc)))
+    return r;
+  return SL_OK;
+}
+static void batch_p2g_binned(const sl_batch *b);
+
 // every work map and the stepped-slot list from the current membership and variant choice
 static int batch_rebuild(sl_batch *b) {
   std::fill(b->maps_host.begin(), b->maps_host.end(), 0u);
@@ -1525,8 +1555,14 @@ static int batch_rebuild(sl_batch *b) {
       for (int u = 0; u < used; u++) seen |= b->map_variant[base + u] == var[k];
       if (!has || seen) continue;
       const int v = var[k];
-      r = put_map(b, base + used, [](const sl_sim *s) { return (uint64_t)s->cap; }, pass ? SL_WG : sl_p2g_chunk(v),
-                  [&](uint32_t t) { return var[b->bucket[t]] == v; });
+      auto inc = [&](uint32_t t) { return var[b->bucket[t]] == v; };
+      if (!pass && v == SL_P2G_BINNED) {
+        r = put_bin_maps(b, inc);
+        b->total[base + used] = 1;  // launched through the BM_BIN* maps
+      } else {
+        r = put_map(b, base + used, [](const sl_sim *s) { return (uint64_t)s->cap; },
+                    pass ? SL_WG : sl_p2g_chunk(v), inc);
+      }
       if (r) return r;
       b->map_variant[base + used++] = v;
     }
@@ -1762,6 +1798,11 @@ extern "C" int sl_batch_emit(sl_batch *b, const sl_emit *recs, const uint32_t *c
   return e == hipSuccess ? SL_OK : batch_fail(b, SL_EHIP, std::string("batch emit: ") + hipGetErrorString(e));
 }
 
+static void batch_p2g_binned(const sl_batch *b) {
+  sl_k_p2g_binned(batch_launch(b, BM_BINP), batch_launch(b, BM_BINS), batch_launch(b, BM_BINT),
+                  batch_launch(b, BM_BINTILE));
+}
+
 // the kernels of one frame of every stepped tank, in stream order (no host synchronisation and no
 // memset, so the sequence can be captured)
 static int batch_encode_frame(sl_batch *b, double dt) {
@@ -1772,8 +1813,11 @@ static int batch_encode_frame(sl_batch *b, double dt) {
   sl_k_count(P, 5, 0);  // vmax = 0 (50-sim.js:306)
   hipLaunchKernelGGL(k_batch_clear_cells, dim3(b->total[BM_CELLS]), dim3(SL_WG), 0, b->stream,
                      batch_launch(b, BM_CELLS).w);
-  for (int u = 0; u < SL_BUCKETS; u++)
-    if (b->map_variant[BM_P2G0 + u] >= 0) sl_k_p2g(batch_launch(b, BM_P2G0 + u), 0, b->map_variant[BM_P2G0 + u]);
+  for (int u = 0; u < SL_BUCKETS; u++) {
+    const int v = b->map_variant[BM_P2G0 + u];
+    if (v == SL_P2G_BINNED) batch_p2g_binned(b);
+    else if (v >= 0) sl_k_p2g(batch_launch(b, BM_P2G0 + u), 0, v);
+  }
   sl_k_normforce(batch_launch(b, BM_NODES));
   sl_k_divsetup(batch_launch(b, BM_CELLS));
   for (uint32_t j = 0; j < b->sor_bands; j++) {  // only the tanks that still iterate
@@ -2023,16 +2067,19 @@ extern "C" int sl_batch_autotune(sl_batch *b, uint32_t reps, uint32_t *choice) {
     if (!has) continue;
     auto in_k = [&](uint32_t t) { return b->bucket[t] == k; };
     float best = 1e30f;
-    for (int v = 0; v < 8; v++) {
-      if ((r = put_map(b, BM_SCRATCH, [](const sl_sim *s) { return (uint64_t)s->cap; }, sl_p2g_chunk(v), in_k)) ||
-          (r = maps_upload(b)))
-        break;
+    for (int v = 0; v <= SL_P2G_BINNED; v++) {
+      if (v == SL_P2G_BINNED) r = put_bin_maps(b, in_k);
+      else r = put_map(b, BM_SCRATCH, [](const sl_sim *s) { return (uint64_t)s->cap; }, sl_p2g_chunk(v), in_k);
+      if (r || (r = maps_upload(b))) break;
       float tot = 0;
       for (uint32_t rep = 0; rep <= reps; rep++) {
         clear_acc();
         hipLaunchKernelGGL(k_batch_clear_cells, dim3(b->total[BM_CELLS]), dim3(SL_WG), 0, b->stream,
                            batch_launch(b, BM_CELLS).w);
-        float ms = timed([&] { sl_k_p2g(batch_launch(b, BM_SCRATCH), 0, v); });
+        float ms = timed([&] {
+          if (v == SL_P2G_BINNED) batch_p2g_binned(b);
+          else sl_k_p2g(batch_launch(b, BM_SCRATCH), 0, v);
+        });
         if (rep) tot += ms;
       }
       if (tot < best) { best = tot; b->p2g_variant[k] = v; }
@@ -2068,7 +2115,7 @@ extern "C" int sl_batch_autotune(sl_batch *b, uint32_t reps, uint32_t *choice) {
 extern "C" int sl_batch_set_variants(sl_batch *b, const uint32_t *choice) {
   if (!b || !choice) return batch_fail(b, SL_EARG, "sl_batch_set_variants: bad arguments");
   for (int k = 0; k < SL_BUCKETS; k++)
-    if (choice[2 * k] > 7 || choice[2 * k + 1] > 3) return batch_fail(b, SL_EARG, "variant out of range");
+    if (choice[2 * k] > SL_P2G_BINNED || choice[2 * k + 1] > 3) return batch_fail(b, SL_EARG, "variant out of range");
   for (int k = 0; k < SL_BUCKETS; k++) { b->p2g_variant[k] = (int)choice[2 * k]; b->g2p_variant[k] = (int)choice[2 * k + 1]; }
   b->maps_dirty = true;
   return SL_OK;

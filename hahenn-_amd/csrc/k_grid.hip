@@ -283,6 +283,251 @@ void sl_k_p2g(const SlLaunch &L, int count_only, int variant) {
 }
 
 // ------------------------------------------------------------------------------------------------
+// Binned P2G (variant SL_P2G_BINNED). The particles of every cell are listed in cell order each
+// substep (count with a per-cell rank, scan, scatter of indices; the particle arrays themselves
+// are not moved), then one thread per cell sums the fixed-point terms of its particles for the
+// face nodes its stencil reaches in registers, adds them into an LDS window of its workgroup's
+// 8x8x4-cell tile, and the window is added to the global accumulators once. Every (particle, node)
+// term is the reference's round(w * v * 2^18) and round(w * 2^18) (30-wgsl-sim.js:159-200), and
+// integer sums do not depend on the order, so the result equals per-particle atomics; the cost no
+// longer depends on how scattered the particle order is.
+// ------------------------------------------------------------------------------------------------
+#define BIN_SCAN 4096  // cells per scan block (the block prefix lives in binTop)
+#define BIN_TX 8
+#define BIN_TY 8
+#define BIN_TZ 4
+#define WIN_X (BIN_TX + 2)
+#define WIN_Y (BIN_TY + 2)
+#define WIN_Z (BIN_TZ + 2)
+#define WIN (WIN_X * WIN_Y * WIN_Z)
+
+// The cell of each live in-grid particle (P2G's own test) and its rank among the cell's particles.
+// Lanes of a wavefront that hold the same cell in a run (the usual case in a spatially ordered
+// particle array) take their ranks with one atomic.
+__global__ __launch_bounds__(SL_WG) void k_bin_count(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t id = lb * SL_WG + threadIdx.x;
+  const bool in = id < min(T.C->count, T.cap);
+  const int cur = T.cur;
+  uint32_t cell = 0xffffffffu;
+  if (in && (T.pf[cur][id] & 1u)) {
+    const f3 g = grid_pos(T, F3(T.px[cur][id], T.py[cur][id], T.pz[cur][id]));
+    if (in_grid(T, g)) {
+      const i3 c = toi3(g);
+      cell = cell_idx(T, c.x, c.y, c.z);
+    }
+  }
+  // runs: a lane starts one unless it holds the same valid cell as the lane before it, so every run
+  // is either one lane without a cell or lanes of one cell; the run's first lane takes the ranks
+  const int lane = __lane_id();
+  const uint32_t prev = __shfl_up(cell, 1, 64);
+  const bool valid = cell != 0xffffffffu;
+  const uint64_t starts = __ballot(!valid || lane == 0 || prev != cell);
+  const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
+  const int head = 63 - __clzll((long long)(starts & upto));
+  const uint64_t above = starts & ~upto;
+  const int next = above ? __ffsll((unsigned long long)above) - 1 : 64;
+  uint32_t base = 0;
+  if (valid && head == lane) base = atomicAdd(&T.cellCount[cell], (uint32_t)(next - lane));
+  base = __shfl(base, head, 64);
+  if (!in) return;
+  T.binCell[id] = cell;
+  T.binRank[id] = base + (uint32_t)(lane - head);
+}
+
+// exclusive prefix of the cell counts inside each block of BIN_SCAN cells; the block total goes to
+// binTop[block] (256 threads x 16 consecutive cells)
+__global__ __launch_bounds__(256) void k_bin_scan(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, blk);
+  const SlTank &T = W.tanks[t];
+  const uint32_t C = T.cells, base = blk * BIN_SCAN + threadIdx.x * 16u;
+  uint32_t v[16], s = 0;
+  if (base + 16u <= C) {
+    const uint4 *p = (const uint4 *)(T.cellCount + base);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      uint4 u = p[q];
+      v[4 * q] = u.x; v[4 * q + 1] = u.y; v[4 * q + 2] = u.z; v[4 * q + 3] = u.w;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; q++) v[q] = base + q < C ? T.cellCount[base + q] : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < 16; q++) s += v[q];
+  // exclusive scan of the per-thread sums over the workgroup
+  __shared__ uint32_t wsum[4];
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t x = s;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t y = __shfl_up(x, o, 64);
+    if (lane >= o) x += y;
+  }
+  if (lane == 63) wsum[wv] = x;
+  __syncthreads();
+  uint32_t off = 0;
+  for (int w = 0; w < wv; w++) off += wsum[w];
+  uint32_t run = off + x - s;
+  if (threadIdx.x == 255) T.binTop[blk] = run + s;
+  if (base + 16u <= C) {
+    uint4 *p = (uint4 *)(T.cellStart + base);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      uint4 u;
+      u.x = run; run += v[4 * q];
+      u.y = run; run += v[4 * q + 1];
+      u.z = run; run += v[4 * q + 2];
+      u.w = run; run += v[4 * q + 3];
+      p[q] = u;
+    }
+  } else {
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      if (base + q < C) T.cellStart[base + q] = run;
+      run += v[q];
+    }
+  }
+}
+
+// exclusive scan of the block totals of one tank, in place (one workgroup per tank)
+__global__ __launch_bounds__(1024) void k_bin_top(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  (void)lb;
+  const SlTank &T = W.tanks[t];
+  const uint32_t nb = (T.cells + BIN_SCAN - 1) / BIN_SCAN;
+  __shared__ uint32_t wsum[16];
+  __shared__ uint32_t carry;
+  if (threadIdx.x == 0) carry = 0;
+  __syncthreads();
+  const int lane = threadIdx.x & 63;
+  for (uint32_t base = 0; base < nb; base += 1024) {
+    const uint32_t i = base + threadIdx.x;
+    const uint32_t v = i < nb ? T.binTop[i] : 0u;
+    uint32_t x = v;
+    for (int o = 1; o < 64; o <<= 1) {
+      uint32_t y = __shfl_up(x, o, 64);
+      if (lane >= o) x += y;
+    }
+    if (lane == 63) wsum[threadIdx.x >> 6] = x;
+    __syncthreads();
+    if (threadIdx.x < 16) {
+      uint32_t ws = wsum[threadIdx.x], acc = ws;
+      for (int o = 1; o < 16; o <<= 1) {
+        uint32_t y = __shfl_up(acc, o, 16);
+        if ((int)threadIdx.x >= o) acc += y;
+      }
+      wsum[threadIdx.x] = acc - ws;
+    }
+    __syncthreads();
+    const uint32_t excl = carry + wsum[threadIdx.x >> 6] + x - v;
+    if (i < nb) T.binTop[i] = excl;
+    __syncthreads();
+    if (threadIdx.x == 1023) carry = excl + v;
+    __syncthreads();
+  }
+}
+
+// particle index -> its slot in cell order
+__global__ __launch_bounds__(SL_WG) void k_bin_scatter(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t id = lb * SL_WG + threadIdx.x;
+  if (id >= min(T.C->count, T.cap)) return;
+  const uint32_t cell = T.binCell[id];
+  if (cell == 0xffffffffu) return;
+  T.perm[T.cellStart[cell] + T.binTop[cell / BIN_SCAN] + T.binRank[id]] = id;
+}
+
+// round(x) to i32 for |x| <= 30 * 2^18 and x not NaN: the splat terms below, whose velocity is clamped
+// to [-30, 30] and whose weight lies in [0, 1], so f2i's NaN and range cases cannot occur
+__device__ __forceinline__ int32_t fix_term(float x) { return (int32_t)rintf(x); }
+
+// One cell's particles, the reference's splat (30-wgsl-sim.js:159-200) into the LDS node window
+// of the workgroup's tile: 8 corners per component with the reference's weight products and
+// rounding. One thread walks one cell, so the lanes of a wave add into different cells' corners.
+__device__ __forceinline__ void p2g_cell(const SlTank &T, int cur, uint32_t start, uint32_t n, int32_t X0, int32_t Y0,
+                                         int32_t Z0, int32_t *wm, int32_t *ww) {
+  for (uint32_t p = 0; p < n; p++) {
+    const uint32_t id = T.perm[start + p];
+    const f3 g = grid_pos(T, F3(T.px[cur][id], T.py[cur][id], T.pz[cur][id]));
+    const float v3[3] = {clampf(T.vx[cur][id], -30.0f, 30.0f), clampf(T.vy[cur][id], -30.0f, 30.0f),
+                         clampf(T.vz[cur][id], -30.0f, 30.0f)};
+#pragma unroll
+    for (int comp = 0; comp < 3; comp++) {
+      const f3 q = sub3(g, comp == 0 ? F3(0.0f, 0.5f, 0.5f) : (comp == 1 ? F3(0.5f, 0.0f, 0.5f) : F3(0.5f, 0.5f, 0.0f)));
+      const float val = v3[comp];
+      const int32_t bx = f2i(floorf(q.x)), by = f2i(floorf(q.y)), bz = f2i(floorf(q.z));
+      const float fx = q.x - (float)bx, fy = q.y - (float)by, fz = q.z - (float)bz;
+      const int e0 = ((bz - Z0) * WIN_Y + (by - Y0)) * WIN_X + (bx - X0) + comp * WIN;
+#pragma unroll
+      for (int dz = 0; dz < 2; dz++) {
+        const float wz = dz == 1 ? fz : 1.0f - fz;
+#pragma unroll
+        for (int dy = 0; dy < 2; dy++) {
+          const float wy = dy == 1 ? fy : 1.0f - fy;
+#pragma unroll
+          for (int dx = 0; dx < 2; dx++) {
+            const float w = (dx == 1 ? fx : 1.0f - fx) * wy * wz;
+            const int e = e0 + (dz * WIN_Y + dy) * WIN_X + dx;
+            atomicAdd(&wm[e], fix_term(w * val * VSCALE));
+            atomicAdd(&ww[e], fix_term(w * VSCALE));
+          }
+        }
+      }
+    }
+  }
+}
+
+__global__ __launch_bounds__(256) void k_p2g_tiles(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  __shared__ int32_t wm[3 * WIN], ww[3 * WIN];
+  for (int e = threadIdx.x; e < 3 * WIN; e += 256) { wm[e] = 0; ww[e] = 0; }
+  const int ntx = (T.nx + BIN_TX - 1) / BIN_TX, nty = (T.ny + BIN_TY - 1) / BIN_TY;
+  const int tx = (int)(lb % (uint32_t)ntx), ty = (int)((lb / (uint32_t)ntx) % (uint32_t)nty);
+  const int tz = (int)(lb / (uint32_t)(ntx * nty));
+  const int lx = threadIdx.x & 7, ly = (threadIdx.x >> 3) & 7, lz = threadIdx.x >> 6;
+  i3 c;
+  c.x = tx * BIN_TX + lx; c.y = ty * BIN_TY + ly; c.z = tz * BIN_TZ + lz;
+  uint32_t start = 0, n = 0;
+  if (c.x < T.nx && c.y < T.ny && c.z < T.nz) {
+    const uint32_t cell = cell_idx(T, c.x, c.y, c.z);
+    n = T.cellCount[cell];
+    if (n) start = T.cellStart[cell] + T.binTop[cell / BIN_SCAN];
+  }
+  __syncthreads();
+  const int32_t X0 = tx * BIN_TX - 1, Y0 = ty * BIN_TY - 1, Z0 = tz * BIN_TZ - 1;  // window origin
+  if (n) p2g_cell(T, T.cur, start, n, X0, Y0, Z0, wm, ww);
+  __syncthreads();
+  unsigned long long *gacc = (unsigned long long *)T.acc;
+  const size_t N = T.nodes;
+  for (int e = threadIdx.x; e < 3 * WIN; e += 256) {
+    const int32_t m = wm[e], w = ww[e];
+    if (!(m | w)) continue;
+    const int comp = e / WIN, l = e - comp * WIN;
+    const int wz = l / (WIN_X * WIN_Y), r = l - wz * (WIN_X * WIN_Y), wy = r / WIN_X, wx = r - wy * WIN_X;
+    const int32_t i = X0 + wx, j = Y0 + wy, k = Z0 + wz;
+    if (i < 0 || i > T.nx || j < 0 || j > T.ny || k < 0 || k > T.nz) continue;  // the reference's node range
+    atomicAdd(&gacc[(size_t)comp * N + node_idx(T, i, j, k)], pack_mw(m, w));
+  }
+}
+
+uint32_t sl_bin_tiles(int nx, int ny, int nz) {
+  return (uint32_t)((nx + BIN_TX - 1) / BIN_TX) * (uint32_t)((ny + BIN_TY - 1) / BIN_TY) *
+         (uint32_t)((nz + BIN_TZ - 1) / BIN_TZ);
+}
+uint32_t sl_bin_blocks(uint32_t cells) { return (cells + BIN_SCAN - 1) / BIN_SCAN; }
+void sl_k_p2g_binned(const SlLaunch &parts, const SlLaunch &blocks, const SlLaunch &tanks, const SlLaunch &tiles) {
+  if (!parts.w.total || !tiles.w.total) return;
+  hipLaunchKernelGGL(k_bin_count, dim3(parts.w.total), dim3(SL_WG), 0, parts.stream, parts.w);
+  hipLaunchKernelGGL(k_bin_scan, dim3(blocks.w.total), dim3(256), 0, parts.stream, blocks.w);
+  hipLaunchKernelGGL(k_bin_top, dim3(tanks.w.total), dim3(1024), 0, parts.stream, tanks.w);
+  hipLaunchKernelGGL(k_bin_scatter, dim3(parts.w.total), dim3(SL_WG), 0, parts.stream, parts.w);
+  hipLaunchKernelGGL(k_p2g_tiles, dim3(tiles.w.total), dim3(256), 0, parts.stream, tiles.w);
+}
+
+// ------------------------------------------------------------------------------------------------
 // NORMALIZE (30-wgsl-sim.js:203-222) fused with FORCES (30-wgsl-sim.js:224-238), per node.
 // The accumulators are cleared after they are read, ready for the next substep's P2G.
 // ------------------------------------------------------------------------------------------------

@@ -63,6 +63,9 @@ struct SlTank {
   SlObj *objs;
   SlEmitDev *emits;
   uint32_t *scanBlocks;   // compaction per-workgroup counts / offsets
+  // binned P2G: per particle its cell and rank in the cell, the particle indices in cell order,
+  // per cell the start inside its scan block, per scan block its start
+  uint32_t *binCell, *binRank, *perm, *cellStart, *binTop;
   // ripple
   int32_t *rsrc;
   float2 *rstate, *ra, *rb, *rhat, *tw;
@@ -296,6 +299,12 @@ void sl_k_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 void sl_k_geom(const SlLaunch &L);
 void sl_k_p2g(const SlLaunch &L, int count_only, int variant);
 int sl_p2g_chunk(int variant);
+// binned P2G: particles (SL_WG per workgroup), scan blocks (one workgroup each), tanks (one workgroup
+// each), tiles (one workgroup each)
+#define SL_P2G_BINNED 8
+uint32_t sl_bin_tiles(int nx, int ny, int nz);
+uint32_t sl_bin_blocks(uint32_t cells);
+void sl_k_p2g_binned(const SlLaunch &parts, const SlLaunch &blocks, const SlLaunch &tanks, const SlLaunch &tiles);
 void sl_k_normforce(const SlLaunch &L);
 void sl_k_divsetup(const SlLaunch &L);
 void sl_k_sor(const SlLaunch &L, uint32_t parity, uint32_t pass);

@@ -57,6 +57,38 @@ def test_one_step_fields():
     assert np.array_equal(gp[:, 3], op[:, 3]) and np.array_equal(gp[:, 7], op[:, 7])
 
 
+def test_p2g_variants_from_identical_state():
+    """Every P2G variant (chunked LDS boxes 0-7, binned 8) reproduces the oracle's fixed-point
+    accumulators and cell counts exactly, also when the particle order is shuffled (integer sums do
+    not depend on the order; scattered chunks exercise the global-atomic fallback)."""
+    o, g = twin.make_twin(twin.small_scene())
+    for _ in range(8):
+        twin.oracle_step(o, 1 / 120)
+    n = int(o.field("counters")[0])
+    dt = 1 / 120
+    o.L.or_step_begin(o.ptr)
+    o.set("frame", o.info()["frame"] + 1)
+    o.write_params(dt)
+    o.field("acc")[:] = 0
+    o.field("cellCount")[:] = 0
+    o.kernel("p2g")
+    want_acc, want_cc = o.field("acc").copy(), o.field("cellCount").copy()
+    parts = o.field("parts").copy()
+    rng = np.random.default_rng(5)
+    shuffled = parts.copy()
+    shuffled[:n] = parts[:n][rng.permutation(n)]
+    for order, p in (("sorted", parts), ("shuffled", shuffled)):
+        g.upload(suiri.F_PARTICLES, p)
+        g.upload(suiri.F_COUNTERS, o.field("counters"))
+        g.set_scalar("count", n)
+        for v in range(9):
+            g.set_scalar("p2g_variant", v)
+            g.upload(suiri.F_ACC, np.zeros_like(want_acc))
+            g.run_stage(suiri.S_P2G, 0, dt)
+            assert np.array_equal(g.readback(suiri.F_ACC), want_acc), (order, v)
+            assert np.array_equal(g.readback(suiri.F_CELLCOUNT), want_cc), (order, v)
+
+
 def test_stage_by_stage_from_identical_state():
     """Each stage started from the oracle's own state reproduces the oracle's output."""
     o, g = twin.make_twin(twin.small_scene())
@@ -145,8 +177,8 @@ def test_autotune_keeps_results_bitwise():
         twin.oracle_step(o, 1 / 120)
         twin.gpu_step(g, 1 / 120)
     ch = g.autotune(reps=1)
-    assert 0 <= ch["p2g_variant"] < 8 and 0 <= ch["g2p_variant"] < 4
-    for v in range(8):   # and every P2G variant on its own
+    assert 0 <= ch["p2g_variant"] <= 8 and 0 <= ch["g2p_variant"] < 4
+    for v in range(9):   # and every P2G variant on its own (8: binned)
         g.set_scalar("p2g_variant", v)
         twin.oracle_step(o, 1 / 120)
         twin.gpu_step(g, 1 / 120)
