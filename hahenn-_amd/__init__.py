@@ -44,7 +44,7 @@ class SlConfig(ctypes.Structure):
         ("r_film", ctypes.c_float), ("r_hp_k", ctypes.c_float), ("r_rho_kick", ctypes.c_float),
         ("wmap", ctypes.c_uint32), ("whitewater", ctypes.c_uint32), ("device", ctypes.c_int32),
         ("reorder_every", ctypes.c_uint32), ("reorder_phase", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 3),
+                ("top_height", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2),
     ]
 
 

@@ -358,6 +358,7 @@ static int tank_buffers(sl_sim *s, SlTank &T, Take take) {
     TAKE(T.rtex, R1 * 16);
     const size_t wm = s->cfg.wmap ? s->cfg.wmap : 256;
     TAKE(T.topH, wm * wm * 4);
+    if (s->cfg.top_height) TAKE(T.topB, wm * wm * 4);
   }
   // brick-Morton keys: 3 bits per level of the largest brick coordinate, + 1 for the outside key
   {
@@ -415,6 +416,8 @@ static int create_impl(sl_sim *s) {
   T.radius = (float)s->radius;
   T.R_f = (float)s->R;
   T.h_d = s->h; T.vp_d = s->vp; T.rim_d = s->rim; T.time_d = 0;
+  T.top_height = c.top_height && s->R ? 1 : 0;
+  T.tankH = (float)c.tank_h;
   T.rsr = c.r_sigma_rho; T.rnu = c.r_nu; T.rfilm = c.r_film; T.rhpk = c.r_hp_k; T.rrho = c.r_rho_kick;
   {
     int r = tank_buffers(s, T, [s](void **p, size_t bytes) { return alloc(s, p, bytes); });
@@ -734,6 +737,12 @@ extern "C" int sl_step_end(sl_sim *s, double frame_dt) {
   md = md < 0.003 ? 0.003 : (md > s->rim ? s->rim : md);
   r = encode_ripple(s, frame_dt, md);
   if (r) return r;
+  if (s->ht.top_height) {  // drawn after the step, read by the next frame's ripple mask
+    StageScope sc(s, SL_T_RIPPLE, 6);
+    const uint32_t wm = s->ht.wmap;
+    sl_k_toph(launch_of(s, s->p_ub).w, launch_of(s, (uint64_t)wm * wm).w, s->stream);
+    if ((r = check_launch(s, "top height"))) return r;
+  }
   s->frames++;
   return encode_stats_copy(s);
 }
@@ -1323,6 +1332,7 @@ enum {
   BM_NODES, BM_CELLS, BM_HCELLS, BM_COMPACT, BM_WW, BM_ELEMS, BM_TEX, BM_ROWS, BM_COLS,
   BM_P2G0, BM_G2P0 = BM_P2G0 + SL_BUCKETS, BM_SCRATCH = BM_G2P0 + SL_BUCKETS,
   BM_BINP, BM_BINS, BM_BINT, BM_BINTILE,  // binned P2G: particles, scan blocks, tanks, tiles
+  BM_TOPP, BM_TOPT,                       // top-height map: particles, texels
   BM_SOR0, BM_COUNT = BM_SOR0 + SL_SOR_BANDS
 };
 
@@ -1391,6 +1401,7 @@ struct sl_batch {
   uint32_t total[BM_COUNT] = {};
   int map_variant[BM_COUNT];
   uint32_t n_stepped = 0, max_iters = 0;
+  bool any_toph = false;
   uint32_t sor_bands = 0, band_start[SL_SOR_BANDS + 1] = {};
   bool any_ripple = false, maps_dirty = true;
   int p2g_variant[SL_BUCKETS] = {0, 0, 0}, g2p_variant[SL_BUCKETS] = {1, 1, 1};
@@ -1543,6 +1554,11 @@ static int batch_rebuild(sl_batch *b) {
       (r = put_map(b, BM_ROWS, [](const sl_sim *s) { return (uint64_t)s->M; }, 1, all)) ||
       (r = put_map(b, BM_COLS, [](const sl_sim *s) { return (uint64_t)cols_of(s); }, 1, all)))
     return r;
+  auto toph = [&](uint32_t t) { return b->sims[t]->ht.top_height != 0; };
+  if ((r = put_map(b, BM_TOPP, [](const sl_sim *s) { return (uint64_t)s->cap; }, SL_WG, toph)) ||
+      (r = put_map(b, BM_TOPT, [](const sl_sim *s) { return (uint64_t)s->ht.wmap * s->ht.wmap; }, SL_WG, toph)))
+    return r;
+  b->any_toph = b->total[BM_TOPT] > 0;
   // P2G / G2P: one map per distinct variant among the buckets that hold stepped tanks
   for (int pass = 0; pass < 2; pass++) {
     const int *var = pass ? b->g2p_variant : b->p2g_variant;
@@ -1839,6 +1855,7 @@ static int batch_encode_frame(sl_batch *b, double dt) {
   if (b->any_ripple)  // the mask stages return for tanks whose mask is off
     sl_k_ripple_frame_ex(batch_launch(b, BM_ROWS).w, batch_launch(b, BM_COLS).w, batch_launch(b, BM_ELEMS).w,
                          batch_launch(b, BM_TEX).w, b->stream, 1);
+  if (b->any_toph) sl_k_toph(batch_launch(b, BM_TOPP).w, batch_launch(b, BM_TOPT).w, b->stream);
   hipError_t e = hipMemcpyAsync(b->h_counters, b->d_counters, sizeof(SlCounters) * b->cap, hipMemcpyDeviceToHost,
                                 b->stream);
   if (e == hipSuccess) e = hipGetLastError();

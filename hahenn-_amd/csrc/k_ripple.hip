@@ -217,3 +217,89 @@ void sl_k_ripple_frame_ex(const SlWork &rows, const SlWork &cols, const SlWork &
   }
   hipLaunchKernelGGL(k_clear_rsrc, dim3(tex.total), dim3(256), 0, s, tex);  // 64-ripple.js:252
 }
+
+// ------------------------------------------------------------------------------------------------
+// Top-surface height map: the renderer's water-seen-from-the-sun pass (63-wgsl-water.js:5-33, drawn by
+// 66-draw.js:122-126 over the grid footprint of 66-draw.js:45-46) and its four separable blur passes
+// (63-wgsl-water.js:35-55, 66-draw.js:127-131). The rasteriser's coverage is stated with explicit
+// texel centres: texel (ix, iy) of the W x W map lies at xz = wmap.xy + ((i + 0.5) / W) * wmap.zw; a
+// live particle at or below tank height + 0.01 covers it when q = r^2 - |xz - p.xz|^2 > 0
+// (r = 1.25 x render radius) and offers p.y + sqrt(q); the depth test keeps the largest (an atomic
+// max on an order-preserving integer image of the float). Texels no particle covers hold -1.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint32_t ord_key(float f) {
+  const uint32_t b = __float_as_uint(f);
+  return (b & 0x80000000u) ? ~b : (b | 0x80000000u);
+}
+__device__ __forceinline__ float ord_val(uint32_t k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+__global__ __launch_bounds__(256) void k_toph_init(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t n = T.wmap * T.wmap, i = lb * 256 + threadIdx.x;
+  if (!T.top_height || i >= n) return;
+  ((uint32_t *)T.topH)[i] = ord_key(-1.0f);
+}
+__global__ __launch_bounds__(256) void k_toph_splat(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t id = lb * 256 + threadIdx.x;
+  if (!T.top_height || id >= min(T.C->count, T.cap)) return;
+  const int cur = T.cur;
+  if (!(T.pf[cur][id] & 1u)) return;
+  const float px = T.px[cur][id], py = T.py[cur][id], pz = T.pz[cur][id];
+  if (py > T.tankH + 0.01f) return;
+  const uint32_t Wm = T.wmap;
+  const float dW = (float)Wm, wx = T.ox, wz = T.oz, ww = T.rLx, wd = T.rLz;
+  const float r = T.radius * 1.25f;
+  int ix0 = (int)floorf((px - r - wx) / ww * dW - 0.5f), ix1 = (int)ceilf((px + r - wx) / ww * dW - 0.5f);
+  int iy0 = (int)floorf((pz - r - wz) / wd * dW - 0.5f), iy1 = (int)ceilf((pz + r - wz) / wd * dW - 0.5f);
+  ix0 = max(ix0, 0); iy0 = max(iy0, 0);
+  ix1 = min(ix1, (int)Wm - 1); iy1 = min(iy1, (int)Wm - 1);
+  uint32_t *key = (uint32_t *)T.topH;
+  for (int iy = iy0; iy <= iy1; iy++)
+    for (int ix = ix0; ix <= ix1; ix++) {
+      const float cx = wx + (((float)ix + 0.5f) / dW) * ww, cz = wz + (((float)iy + 0.5f) / dW) * wd;
+      const float ddx = cx - px, ddz = cz - pz;
+      const float q = r * r - (ddx * ddx + ddz * ddz);
+      if (q <= 0.0f) continue;
+      const uint32_t k = ord_key(py + sqrtf(q));
+      uint32_t *dst = &key[(size_t)iy * Wm + ix];
+      if (k > *dst) atomicMax(dst, k);  // the read only skips offers that cannot win
+    }
+}
+// one blur pass along (dx, dy) (63-wgsl-water.js:40-55); pass 0 reads the splat's integer image
+__global__ __launch_bounds__(256) void k_toph_blur(SlWork W, int pass) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const int Wm = (int)T.wmap;
+  const uint32_t i = lb * 256 + threadIdx.x;
+  if (!T.top_height || i >= (uint32_t)(Wm * Wm)) return;
+  const float *src = (pass & 1) ? T.topB : T.topH;
+  float *dst = (pass & 1) ? T.topH : T.topB;
+  const int dx = (pass & 1) ? 0 : 1, dy = (pass & 1) ? 1 : 0;
+  const int x = (int)(i % (uint32_t)Wm), y = (int)(i / (uint32_t)Wm);
+  auto load = [&](int qx, int qy) {
+    const float v = src[(size_t)qy * Wm + qx];
+    return pass == 0 ? ord_val(__float_as_uint(v)) : v;
+  };
+  const float h0 = load(x, y);
+  if (h0 < 0.0f) { dst[i] = -1.0f; return; }
+  float sum = 0.0f, wsum = 0.0f;
+  for (int k = -6; k <= 6; k++) {
+    const int qx = min(max(x + dx * k, 0), Wm - 1), qy = min(max(y + dy * k, 0), Wm - 1);
+    const float h = load(qx, qy);
+    if (h < 0.0f) continue;
+    const float w = sl_exp(-(float)(k * k) / 18.0f);
+    sum += h * w;
+    wsum += w;
+  }
+  dst[i] = sum / wsum;
+}
+void sl_k_toph(const SlWork &parts, const SlWork &texels, hipStream_t s) {
+  if (!texels.total) return;
+  hipLaunchKernelGGL(k_toph_init, dim3(texels.total), dim3(256), 0, s, texels);
+  if (parts.total) hipLaunchKernelGGL(k_toph_splat, dim3(parts.total), dim3(256), 0, s, parts);
+  for (int pass = 0; pass < 4; pass++) hipLaunchKernelGGL(k_toph_blur, dim3(texels.total), dim3(256), 0, s, texels, pass);
+}

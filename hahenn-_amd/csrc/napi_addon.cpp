@@ -84,7 +84,7 @@ static bool get_args(napi_env env, napi_callback_info info, size_t want, napi_va
 #define CFG_FIELDS(X)                                                                           \
   X(tank_w) X(tank_h) X(tank_d) X(h) X(cap) X(ww_cap) X(iters) X(ripple_R) X(rc_factor) X(gravity) \
   X(flip) X(ppc) X(drift) X(omega) X(friction) X(ww_rate) X(r_sigma_rho) X(r_nu) X(r_film)         \
-  X(r_hp_k) X(r_rho_kick) X(wmap) X(whitewater) X(device) X(reorder_every) X(reorder_phase)
+  X(r_hp_k) X(r_rho_kick) X(wmap) X(whitewater) X(device) X(reorder_every) X(reorder_phase) X(top_height)
 
 static napi_value config_to_js(napi_env env, const sl_config &c) {
   napi_value o;

@@ -38,7 +38,8 @@ struct SlTank {
   int32_t nx, ny, nz, hw;
   uint32_t nodes, cells, hcells;  // hcells = hw*ny*nz
   uint32_t cap, ww_cap, R, M, logM, iters, obj_count, frame, emit_k, emit_total, cur, ww_cur, wmap;
-  int32_t mask_bound, whitewater_on, brick_bits, pad_i;
+  int32_t mask_bound, whitewater_on, brick_bits, top_height;
+  float tankH;  // scene tank height (the renderer's F.tank.y)
   // params as the reference uniform (f32)
   float ox, oy, oz, h, invh, hh, dt, g, gdt, flip, ppc, drift, omega, omega1, rc, rim;
   float ww_rate, ww_cap_f, friction, time_f, radius, R_f;
@@ -69,7 +70,7 @@ struct SlTank {
   // ripple
   int32_t *rsrc;
   float2 *rstate, *ra, *rb, *rhat, *tw;
-  float *rmask, *topH;
+  float *rmask, *topH, *topB;  // topB: blur ping-pong of the top-height map
   float4 *rtex;
 };
 
@@ -324,6 +325,9 @@ size_t sl_sort_temp_bytes(uint32_t n, int end_bit);
 int sl_k_reorder(const SlLaunch &L, uint32_t n_ub, uint32_t *keys, uint32_t *vals, void *temp, size_t temp_bytes,
                  int end_bit);
 // ripple: rows = M workgroups per tank, cols = M/CW, elems = M*M/256, tex = (R+1)^2/256
+// top-surface height map of the tanks with top_height set: particles (256 per workgroup), texels
+// (wmap^2, 256 per workgroup)
+void sl_k_toph(const SlWork &parts, const SlWork &texels, hipStream_t s);
 void sl_k_ripple_frame_ex(const SlWork &rows, const SlWork &cols, const SlWork &elems, const SlWork &tex,
                           hipStream_t s, int mask);
 // workgroups of compaction per tank (each compaction workgroup covers SL_COMPACT_ITEMS particles)

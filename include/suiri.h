@@ -61,7 +61,11 @@ typedef struct sl_config {
   uint32_t reorder_every;        /* spatial particle re-ordering period in frames (0 = never) */
   uint32_t reorder_phase;        /* frames already counted towards the first re-ordering (staggers
                                     the re-orderings of many tanks) */
-  uint32_t reserved[3];
+  uint32_t top_height;           /* 1: each frame's end draws the top-surface height map from the
+                                    particles (63-wgsl-water.js:5-55, what the renderer draws for the
+                                    next frame's ripple mask); 0: the map stays as uploaded (zeros, as
+                                    in the reference's headless runs) */
+  uint32_t reserved[2];
 } sl_config;
 
 /* One emission record, as produced by sourceEmits / burstEmit / blockEmits (src/js/55-emit.js:13-106)

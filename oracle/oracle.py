@@ -24,6 +24,7 @@ class OrQuality(ctypes.Structure):
         ("ww_rate", ctypes.c_float), ("r_sigma_rho", ctypes.c_float), ("r_nu", ctypes.c_float),
         ("r_film", ctypes.c_float), ("r_hp_k", ctypes.c_float), ("r_rho_kick", ctypes.c_float),
         ("wmap", ctypes.c_uint32), ("reorder_every", ctypes.c_uint32), ("reorder_phase", ctypes.c_uint32),
+        ("top_height", ctypes.c_uint32),
     ]
 
 
@@ -58,7 +59,8 @@ def lib():
         L.or_set_objects.restype = ctypes.c_int
         for name in ["or_geom", "or_p2g", "or_normalize", "or_forces", "or_divergence", "or_project",
                      "or_g2p", "or_compact", "or_substep", "or_step_begin", "or_poll_stats",
-                     "or_clear_water", "or_revolve", "or_rmaskbuild", "or_rscale", "or_reorder"]:
+                     "or_clear_water", "or_revolve", "or_rmaskbuild", "or_rscale", "or_reorder", "or_toph",
+                     "or_toph_splat", "or_toph_blur"]:
             getattr(L, name).argtypes = [P]
         L.or_pressure.argtypes = [P, ctypes.c_uint32]
         L.or_count.argtypes = [P, ctypes.c_uint32, ctypes.c_uint32]

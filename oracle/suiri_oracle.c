@@ -948,6 +948,73 @@ void or_revolve(or_sim *s) {
     st[0] = sx; st[1] = sy;
   }
 }
+/* Top-surface height map: the renderer's water-seen-from-the-sun pass (63-wgsl-water.js:5-33, drawn
+ * by 66-draw.js:122-126 over the grid footprint F.wmap of 66-draw.js:45-46) and its four separable
+ * blur passes (63-wgsl-water.js:35-55, 66-draw.js:127-131). The rasteriser's coverage is restated
+ * with explicit texel centres: texel (ix, iy) of the W x W map lies at
+ * xz = wmap.xy + ((i + 0.5) / W) * wmap.zw; a live particle at or below tank height + 0.01 covers it
+ * when q = r^2 - |xz - p.xz|^2 > 0 (r = 1.25 x render radius) and offers p.y + sqrt(q); the depth
+ * test keeps the largest. Texels no particle covers hold -1. */
+static void toph_blur(const float *src, float *dst, uint32_t W, int dx, int dy) {
+  for (int y = 0; y < (int)W; y++)
+    for (int x = 0; x < (int)W; x++) {
+      float h0 = src[(size_t)y * W + x];
+      if (h0 < 0.0f) { dst[(size_t)y * W + x] = -1.0f; continue; }
+      float sum = 0.0f, wsum = 0.0f;
+      for (int i = -6; i <= 6; i++) {
+        int qx = x + dx * i, qy = y + dy * i;
+        qx = qx < 0 ? 0 : (qx > (int)W - 1 ? (int)W - 1 : qx);
+        qy = qy < 0 ? 0 : (qy > (int)W - 1 ? (int)W - 1 : qy);
+        float h = src[(size_t)qy * W + qx];
+        if (h < 0.0f) continue;
+        float k = cr_expf(-(float)(i * i) / 18.0f);
+        sum += h * k;
+        wsum += k;
+      }
+      dst[(size_t)y * W + x] = sum / wsum;
+    }
+}
+void or_toph_splat(or_sim *s) {
+  const uint32_t W = s->q.wmap;
+  float *H = s->topH;
+  for (size_t i = 0; i < (size_t)W * W; i++) H[i] = -1.0f;
+  const float wx = (float)s->origin[0], wz = (float)s->origin[2];
+  const float ww = (float)(s->nx * s->h), wd = (float)(s->nz * s->h);
+  const float r = (float)s->radius * 1.25f, ylim = (float)s->tank_h + 0.01f, dW = (float)W;
+  const or_particle *P = s->parts[s->cur];
+  for (uint32_t n = 0; n < s->C.count && n < s->cap; n++) {
+    const or_particle *p = &P[n];
+    if (p->pos[3] < 0.5f || p->pos[1] > ylim) continue;
+    int ix0 = (int)floorf((p->pos[0] - r - wx) / ww * dW - 0.5f), ix1 = (int)ceilf((p->pos[0] + r - wx) / ww * dW - 0.5f);
+    int iy0 = (int)floorf((p->pos[2] - r - wz) / wd * dW - 0.5f), iy1 = (int)ceilf((p->pos[2] + r - wz) / wd * dW - 0.5f);
+    if (ix0 < 0) ix0 = 0;
+    if (iy0 < 0) iy0 = 0;
+    if (ix1 > (int)W - 1) ix1 = (int)W - 1;
+    if (iy1 > (int)W - 1) iy1 = (int)W - 1;
+    for (int iy = iy0; iy <= iy1; iy++)
+      for (int ix = ix0; ix <= ix1; ix++) {
+        float cx = wx + (((float)ix + 0.5f) / dW) * ww, cz = wz + (((float)iy + 0.5f) / dW) * wd;
+        float ddx = cx - p->pos[0], ddz = cz - p->pos[2];
+        float q = r * r - (ddx * ddx + ddz * ddz);
+        if (q <= 0.0f) continue;
+        float y = p->pos[1] + sqrtf(q);
+        if (y > H[(size_t)iy * W + ix]) H[(size_t)iy * W + ix] = y;
+      }
+  }
+}
+void or_toph_blur(or_sim *s) {
+  const uint32_t W = s->q.wmap;
+  float *H = s->topH;
+  toph_blur(H, s->topB, W, 1, 0);
+  toph_blur(s->topB, H, W, 0, 1);
+  toph_blur(H, s->topB, W, 1, 0);
+  toph_blur(s->topB, H, W, 0, 1);
+}
+void or_toph(or_sim *s) {
+  or_toph_splat(s);
+  or_toph_blur(s);
+}
+
 /* RMASKBUILD, 64-ripple.js:106-126 */
 static float smoothstepw(float e0, float e1, float x) {
   float t = clampw((x - e0) / (e1 - e0), 0.0f, 1.0f);
@@ -1046,6 +1113,7 @@ or_sim *or_create(double tank_w, double tank_h, double tank_d, const or_quality 
   or_sim *s = (or_sim *)xcalloc(1, sizeof(or_sim));
   s->q = *q;
   s->reorder_clock = q->reorder_every ? q->reorder_phase % q->reorder_every : 0;
+  s->tank_h = tank_h;
   s->nthreads = 1;
   /* grid sizing, 50-sim.js:33-45 */
   s->h = q->h;
@@ -1092,6 +1160,7 @@ or_sim *or_create(double tank_w, double tank_h, double tank_d, const or_quality 
     s->mask = (float *)xcalloc((size_t)(s->R + 1) * (s->R + 1), 4);
     s->tex = (float *)xcalloc((size_t)(s->R + 1) * (s->R + 1) * 4, 4);
     s->topH = (float *)xcalloc((size_t)q->wmap * q->wmap, 4);
+    s->topB = (float *)xcalloc((size_t)q->wmap * q->wmap, 4);
     s->mask_bound = 1;
   }
   or_write_params(s, 1.0 / 60.0);
@@ -1105,7 +1174,7 @@ void or_destroy(or_sim *s) {
   free(s->acc); free(s->cellCount); free(s->vel); free(s->velOld); free(s->frac); free(s->svel);
   free(s->cellType); free(s->divg); free(s->pres);
   free(s->rsrc); free(s->rstate); free(s->ra); free(s->rb); free(s->rhat); free(s->tw);
-  free(s->mask); free(s->tex); free(s->topH);
+  free(s->mask); free(s->tex); free(s->topH); free(s->topB);
   free(s);
 }
 
@@ -1235,6 +1304,7 @@ void or_frame_tail(or_sim *s, double frame_dt) {
   double md = liters / 1000 / (s->nx * s->nz * s->h * s->h);
   md = md < 0.003 ? 0.003 : (md > s->rim ? s->rim : md);
   or_ripple_frame(s, frame_dt, md);
+  if (s->R && s->q.top_height) or_toph(s);  /* drawn after the step; the next frame's mask reads it */
 }
 
 /* pollStats, 50-sim.js:346-363 */

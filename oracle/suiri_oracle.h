@@ -64,6 +64,7 @@ typedef struct {
   uint32_t wmap;     /* top-height map size used by the ripple mask (256 in the reference) */
   uint32_t reorder_every; /* spatial re-ordering period in frames (0 = never) */
   uint32_t reorder_phase; /* frames already counted towards the first re-ordering */
+  uint32_t top_height;    /* 1: produce the top-height map from the particles at each frame's end */
 } or_quality;
 
 typedef struct or_sim {
@@ -95,6 +96,8 @@ typedef struct or_sim {
   float *mask;         /* (R+1)^2 */
   float *tex;          /* (R+1)^2 * 4 : height, slope x, slope z, 0 (f32; the reference stores rgba16f) */
   float *topH;         /* wmap*wmap, -1 where dry; zero-initialised like a fresh WebGPU texture */
+  float *topB;         /* blur ping-pong of topH */
+  double tank_h;       /* scene tank height (the renderer's F.tank.y) */
   int mask_bound;
   float rU_l[4];       /* last ripple uniform l = (Lx, Lz, dt, meanDepth) */
   int nthreads;
@@ -123,6 +126,9 @@ void or_compact(or_sim *s);                /* 31-wgsl-sim2.js:317-328 (stable or
 /* stable sort of the live particles by the Morton code of their 4x4x4-cell brick (outside the grid
    last); no counterpart in the reference, whose particle order is arbitrary after compaction */
 void or_reorder(or_sim *s);
+void or_toph(or_sim *s);       /* or_toph_splat + or_toph_blur */
+void or_toph_splat(or_sim *s);
+void or_toph_blur(or_sim *s);
 void or_count(or_sim *s, uint32_t mode, uint32_t y); /* 31-wgsl-sim2.js:332-349 */
 void or_ww(or_sim *s, float dt);           /* 31-wgsl-sim2.js:361-409 */
 

@@ -16,20 +16,22 @@ pytestmark = pytest.mark.gpu
 DT = 1 / 120
 
 # grid sizes 16x10x12 .. 77x32x38 (two size buckets), different SOR iteration counts, ripple on/off,
-# absorbing mask on/off, periodic re-ordering (one with a phase offset)
+# absorbing mask on/off, periodic re-ordering (one with a phase offset), one drawing its top-height map
 SPECS = [
     dict(tank=(0.32, 0.2, 0.24), h=1 / 50, iters=20, R=32, reorder=4, phase=2, obst="sphere", fill=(6, 5, 8),
          mask=True),
     dict(tank=(0.5, 0.25, 0.25), h=1 / 64, iters=30, R=64, reorder=3, obst="box", fill=(12, 10, 16), mask=False),
     dict(tank=(0.3, 0.3, 0.3), h=1 / 40, iters=26, R=0, reorder=0, obst=None, fill=(5, 7, 5), mask=True),
-    dict(tank=(0.6, 0.25, 0.3), h=1 / 128, iters=24, R=64, reorder=0, obst="box", fill=(20, 12, 30), mask=True),
+    dict(tank=(0.6, 0.25, 0.3), h=1 / 128, iters=24, R=64, reorder=0, obst="box", fill=(20, 12, 30), mask=True,
+         top=1),
 ]
 
 
 def cfg_of(sp):
     return suiri.config("standard", tank_w=sp["tank"][0], tank_h=sp["tank"][1], tank_d=sp["tank"][2], h=sp["h"],
                         iters=sp["iters"], ripple_R=sp["R"], cap=40000, ww_cap=30000, flip=0.95, omega=1.7,
-                        whitewater=1, reorder_every=sp["reorder"], reorder_phase=sp.get("phase", 0))
+                        whitewater=1, reorder_every=sp["reorder"], reorder_phase=sp.get("phase", 0),
+                        top_height=sp.get("top", 0))
 
 
 def items_of(sp, shift=0.0):
@@ -116,7 +118,7 @@ def assert_same_tank(got, want, label):
     b = b[np.lexsort(b.T[::-1])]
     assert np.array_equal(bits(a), bits(b)), f"{label}: whitewater multiset differs"
     if got.grid.R:
-        for f in (suiri.F_RSTATE, suiri.F_RTEX, suiri.F_RSRC):
+        for f in (suiri.F_RSTATE, suiri.F_RTEX, suiri.F_RSRC, suiri.F_TOPH, suiri.F_RMASK):
             assert np.array_equal(bits(got.readback(f)), bits(want.readback(f))), f"{label}: ripple field {f} differs"
 
 

@@ -295,3 +295,18 @@ def test_whitewater_multiset():
         ka = np.lexsort(a.T[::-1])
         kb = np.lexsort(b.T[::-1])
         assert_close("whitewater", a[ka], b[kb], 1e-4)
+
+
+def test_top_height_map_feeds_the_mask():
+    """With top_height on, each frame's end draws the top-surface height map from the particles
+    (63-wgsl-water.js:5-55) and the next frame's ripple mask reads it: the map is bit-identical to
+    the oracle's after every frame, and so are the mask and the ripple state it shapes."""
+    o, g = twin.make_twin(twin.small_scene(top_height=1))
+    for f in range(6):
+        twin.oracle_step(o, 1 / 120)
+        twin.gpu_step(g, 1 / 120)
+        got, want = g.readback(suiri.F_TOPH), o.field("topH")
+        assert (want >= 0).sum() > 100, "the water should cover part of the map"
+        assert np.array_equal(got.view(np.uint32), want.view(np.uint32)), f"top height, frame {f}"
+    assert np.array_equal(g.readback(suiri.F_RMASK).view(np.uint32), o.field("mask").view(np.uint32))
+    assert_close("ripple state", g.readback(suiri.F_RSTATE), o.field("rstate"), 1e-5)

@@ -519,3 +519,79 @@ def test_dam_break_steps_conserve_and_are_deterministic():
     p = a.field("parts")[: int(n)]
     assert np.all(p[:, 3] == 1)
     assert np.isfinite(p).all() and p[:, 4:7].std() > 0.01
+
+
+def _toph_oracle(n_parts=300, seed=3):
+    q = O.quality("standard", h=1 / 32, ripple_R=32, cap=4000, wmap=64, top_height=1)
+    tank = (0.5, 0.25, 0.5)
+    o = O.Oracle(tank, q)
+    inf = o.info()
+    rng = np.random.default_rng(seed)
+    nx, nz, h = int(inf["nx"]), int(inf["nz"]), inf["h"]
+    p = o.field("parts")
+    p[:] = 0
+    p[:n_parts, 0] = rng.uniform(inf["ox"], inf["ox"] + nx * h, n_parts)
+    p[:n_parts, 2] = rng.uniform(inf["oz"], inf["oz"] + nz * h, n_parts)
+    p[:n_parts, 1] = rng.uniform(0.0, tank[1] + 0.05, n_parts)   # some above tank height + 0.01
+    p[:n_parts, 3] = (rng.uniform(size=n_parts) > 0.1).astype(np.float32)   # some dead
+    o.field("counters")[0] = n_parts
+    return o, tank, inf
+
+
+def test_top_height_splat_is_the_highest_cap():
+    """Each texel holds the highest spherical cap y + sqrt(r^2 - d^2) over the live particles at or
+    below tank height + 0.01 whose disc (r = 1.25 x 0.525 h) covers its centre, -1 where none does
+    (63-wgsl-water.js:5-33): checked against a float64 evaluation of the geometry, and bitwise
+    independent of the particle order."""
+    o, tank, inf = _toph_oracle()
+    o.kernel("toph_splat")
+    W = 64
+    got = o.field("topH").reshape(W, W).astype(np.float64)
+    n = int(o.field("counters")[0])
+    p = o.field("parts")[:n].astype(np.float64)
+    h = inf["h"]
+    r = 0.5 * h * 1.05 * 1.25
+    live = (p[:, 3] >= 0.5) & (p[:, 1] <= tank[1] + 0.01)
+    cx = inf["ox"] + (np.arange(W) + 0.5) / W * (inf["nx"] * h)
+    cz = inf["oz"] + (np.arange(W) + 0.5) / W * (inf["nz"] * h)
+    d2 = (cx[None, None, :] - p[live, 0][:, None, None]) ** 2 + (cz[None, :, None] - p[live, 2][:, None, None]) ** 2
+    qq = r * r - d2
+    cand = np.where(qq > 0, p[live, 1][:, None, None] + np.sqrt(np.maximum(qq, 0)), -1.0)
+    want = cand.max(axis=0)
+    edge = (np.abs(qq) < 1e-5 * r * r).any(axis=0)   # coverage decided within rounding
+    assert (got[~edge] >= 0).sum() > 50
+    assert np.array_equal(got[~edge] < 0, want[~edge] < 0)
+    both = (~edge) & (want >= 0)
+    assert np.abs(got[both] - want[both]).max() < 1e-5
+    # the maximum does not depend on the particle order
+    first = o.field("topH").copy()
+    parts = o.field("parts")
+    parts[:n] = parts[:n][np.random.default_rng(9).permutation(n)]
+    o.kernel("toph_splat")
+    assert np.array_equal(o.field("topH"), first)
+
+
+def test_top_height_blur_is_the_separable_normalised_gaussian():
+    """The four blur passes (x, y, x, y; 13 taps exp(-i^2/18), normalised over the texels with
+    water, 63-wgsl-water.js:35-55) act on an all-wet map as the 2-D convolution with
+    (g*g) x (g*g), g the normalised kernel (numpy convolution); dry texels stay -1 and a constant
+    wet map stays constant."""
+    o, _, _ = _toph_oracle()
+    W = 64
+    H = o.field("topH")
+    H[:] = 0.0
+    H.reshape(W, W)[32, 30] = 1.0
+    o.kernel("toph_blur")
+    g = np.exp(-np.arange(-6, 7) ** 2 / 18.0)
+    g /= g.sum()
+    g2 = np.convolve(g, g)                       # 25 taps
+    want = np.zeros((W, W))
+    want[32 - 12:32 + 13, 30 - 12:30 + 13] = np.outer(g2, g2)
+    assert np.abs(H.reshape(W, W) - want).max() < 1e-6
+    # dry texels stay dry; a constant wet region keeps its value away from dry texels
+    H[:] = 0.37
+    Hm = H.reshape(W, W)
+    Hm[:, :8] = -1.0
+    o.kernel("toph_blur")
+    assert np.all(Hm[:, :8] == -1.0)
+    assert np.abs(Hm[:, 8:] - 0.37).max() < 1e-6
