@@ -83,7 +83,8 @@ EXPORTS = ["sl_config_default", "sl_create", "sl_destroy", "sl_last_error", "sl_
            "sl_batch_add", "sl_batch_remove", "sl_batch_set_stepped", "sl_batch_emit", "sl_batch_captured",
            "sl_batch_set_dt", "sl_job_bytes", "sl_server_config_default", "sl_server_create", "sl_server_destroy",
            "sl_server_last_error", "sl_server_submit", "sl_server_cancel", "sl_server_tick", "sl_server_poll",
-           "sl_server_info", "sl_server_job_sim", "sl_server_release", "sl_server_stats"]
+           "sl_server_info", "sl_server_job_sim", "sl_server_release", "sl_server_stats", "sl_bake_frame",
+           "sl_write_exr_rgb"]
 
 EMIT_FN = ctypes.CFUNCTYPE(ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                            ctypes.POINTER(SlEmit), ctypes.c_uint32)
@@ -99,7 +100,7 @@ class SlJob(ctypes.Structure):
     _fields_ = [("cfg", SlConfig), ("objects", ctypes.POINTER(ctypes.c_float)), ("n_objects", ctypes.c_uint32),
                 ("fill", ctypes.POINTER(SlEmit)), ("n_fill", ctypes.c_uint32), ("steps", ctypes.c_uint32),
                 ("dt", ctypes.c_double), ("source", EMIT_FN), ("user", ctypes.c_void_p), ("keep", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 3)]
+                ("bake_every", ctypes.c_uint32), ("bake_dir", ctypes.c_char_p), ("reserved", ctypes.c_uint32 * 2)]
 
 
 class SlJobInfo(ctypes.Structure):
@@ -198,6 +199,8 @@ def load():
         "sl_server_job_sim": (P, [P, ctypes.c_uint64]),
         "sl_server_release": (I, [P, ctypes.c_uint64]),
         "sl_server_stats": (I, [P, ctypes.POINTER(SlServerStats)]),
+        "sl_bake_frame": (I, [P, ctypes.c_char_p, U]),
+        "sl_write_exr_rgb": (I, [ctypes.c_char_p, U, U, P, P, P]),
         "sl_batch_timing_read": (I, [P, ctypes.POINTER(D), U, ctypes.POINTER(U)]),
         "sl_version": (I, []),
         "sl_build_info": (ctypes.c_char_p, []),
@@ -302,6 +305,10 @@ class Sim:
 
     def reorder(self):
         self._check(self.L.sl_reorder(self.ptr))
+
+    def bake(self, directory, frame):
+        """frame_%05d.npz (+ surface_%05d.exr with a ripple layer) of the current state into `directory`."""
+        self._check(self.L.sl_bake_frame(self.ptr, os.fsencode(directory), int(frame)))
 
     def clear_water(self):
         self._check(self.L.sl_clear_water(self.ptr))
@@ -513,7 +520,8 @@ class Server:
             msg = self.L.sl_server_last_error(self.ptr)
             raise SuiriError(f"suiri error {r}: {msg.decode() if msg else ''}")
 
-    def submit(self, cfg, rows=None, fill=(), steps=100, dt=1.0 / 120, source=None, keep=False):
+    def submit(self, cfg, rows=None, fill=(), steps=100, dt=1.0 / 120, source=None, keep=False, bake_dir=None,
+               bake_every=0):
         j = SlJob()
         j.cfg = cfg
         rows_a = np.ascontiguousarray(rows if rows is not None else np.zeros((0, 28)), dtype=np.float32).reshape(-1, 28)
@@ -523,6 +531,8 @@ class Server:
         j.fill = ctypes.cast(fill_a, ctypes.POINTER(SlEmit))
         j.n_fill = len(fill)
         j.steps, j.dt, j.keep = int(steps), float(dt), int(bool(keep))
+        j.bake_every = int(bake_every)
+        j.bake_dir = os.fsencode(bake_dir) if bake_dir else None
         cb = None
         if source is not None:
             def _cb(user, sim, d, t, out, mx):
@@ -589,3 +599,12 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def write_exr_rgb(path, r, g, b):
+    """The bake maps' EXR layout: three H x W float32 planes, row 0 at the top of the picture."""
+    r, g, b = (np.ascontiguousarray(x, dtype=np.float32) for x in (r, g, b))
+    H, W = r.shape
+    rc = load().sl_write_exr_rgb(os.fsencode(path), W, H, r.ctypes.data, g.ctypes.data, b.ctypes.data)
+    if rc != SL_OK:
+        raise SuiriError(f"write_exr_rgb({path}) failed")

@@ -25,6 +25,8 @@ struct Job {
   sl_job spec{};
   std::vector<float> objects;
   std::vector<sl_emit> fill;
+  std::string bake_dir;
+  uint32_t baked = 0;   // bake frames written
   int32_t status = SL_JOB_QUEUED;
   uint32_t slot = UINT32_MAX;
   uint32_t frames = 0;  // frames enqueued
@@ -140,6 +142,8 @@ extern "C" int sl_server_submit(sl_server *s, const sl_job *job, uint64_t *id) {
   j.fill.assign(job->fill, job->fill + job->n_fill);
   j.spec.objects = nullptr;
   j.spec.fill = nullptr;
+  if (job->bake_dir) j.bake_dir = job->bake_dir;
+  j.spec.bake_dir = nullptr;
   j.bytes = sl_job_bytes(&job->cfg);
   j.t_submit = s->now();
   if (id) *id = j.id;
@@ -169,6 +173,14 @@ extern "C" int sl_server_cancel(sl_server *s, uint64_t id) {
   if (it == s->jobs.end()) return s->fail(SL_EARG, "sl_server_cancel: unknown job");
   Job &j = it->second;
   if (j.status == SL_JOB_QUEUED || j.status == SL_JOB_RUNNING) j.cancel = true;  // takes effect at the next tick
+  return SL_OK;
+}
+
+// bake output of a job's tank at frame f (after the frame completed)
+static int bake(sl_server *s, Job &j, uint32_t f) {
+  int r = sl_bake_frame(sl_batch_sim(s->batch, j.slot), j.bake_dir.c_str(), f);
+  if (r) return s->fail(r, "bake of job " + std::to_string(j.id) + " frame " + std::to_string(f) + " into " + j.bake_dir);
+  j.baked++;
   return SL_OK;
 }
 
@@ -214,6 +226,10 @@ extern "C" int sl_server_tick(sl_server *s, uint32_t *running) {
     double v = 0;
     if (sl_get_scalar(sim, "count", &v) == SL_OK) j.count = (uint32_t)v;
     if (sl_get_scalar(sim, "ww_count", &v) == SL_OK) j.ww_count = (uint32_t)v;
+    if (!j.cancel && !j.bake_dir.empty() && j.frames &&
+        (j.frames >= j.spec.steps || (j.spec.bake_every && j.frames % j.spec.bake_every == 0)) &&
+        (r = bake(s, j, j.frames)))
+      return r;
     if (j.cancel) {
       evict(s, j);
       retire(s, j, SL_JOB_CANCELLED, t);
@@ -261,6 +277,7 @@ extern "C" int sl_server_tick(sl_server *s, uint32_t *running) {
     s->used += j.bytes;
     s->stable = 0;
     it = s->queue.erase(it);
+    if (!j.bake_dir.empty() && (r = bake(s, j, 0))) return r;  // the start frame, like bake.py:208-214
   }
   // 4. per-frame emission of every running job with a source, in one batched emission
   uint32_t nrun = 0;

@@ -265,6 +265,16 @@ int sl_batch_timing_read(sl_batch *batch, double *ms, uint32_t max, uint32_t *n)
 /* a tank's own frame length, used by sl_batch_step(batch, 0) (tanks of one batch may differ) */
 int sl_batch_set_dt(sl_batch *batch, uint32_t slot, double dt);
 
+/* Bake output of the current frame into `dir`, in the layout of the reference's Blender cache
+   (blender_addon/suiri_lab/bake.py:226-249): frame_%05d.npz with pos, vel (live particles), spray,
+   foam, bubble (whitewater positions by kind), liters, particles, time, in Blender axes
+   (X, Y, Z) = (x, -z, y); and, for a tank with a ripple layer, surface_%05d.exr (exr.py:1-41:
+   uncompressed float R = top height, G = ripple slope d/dX, B = d/dY at the map's texel centres).
+   Synchronous (reads the tank back). */
+int sl_bake_frame(sl_sim *sim, const char *dir, uint32_t frame);
+/* the EXR layout of the bake maps (exr.py:11-41); r, g, b: H x W, row 0 at the top; host only */
+int sl_write_exr_rgb(const char *path, uint32_t W, uint32_t H, const float *r, const float *g, const float *b);
+
 /* device bytes a simulation of this configuration holds (sum of its allocation sizes; host only) */
 size_t sl_job_bytes(const sl_config *cfg);
 
@@ -294,7 +304,10 @@ typedef struct {
   sl_emit_fn source;        /* per-frame emission (sources), or NULL */
   void *user;
   uint32_t keep;            /* keep the final state resident until sl_server_release */
-  uint32_t reserved[3];
+  uint32_t bake_every;      /* with bake_dir: sl_bake_frame at admission, every bake_every frames and at
+                               the last frame (0: only at the last) */
+  const char *bake_dir;     /* NULL: no bake output; copied at submission */
+  uint32_t reserved[2];
 } sl_job;
 enum { SL_JOB_QUEUED = 0, SL_JOB_RUNNING = 1, SL_JOB_DONE = 2, SL_JOB_CANCELLED = 3, SL_JOB_REFUSED = 4 };
 typedef struct {
