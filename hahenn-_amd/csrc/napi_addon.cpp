@@ -7,6 +7,7 @@
 #include <node_api.h>
 #include <stdint.h>
 #include <string.h>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -56,10 +57,11 @@ static void set_num(napi_env env, napi_value obj, const char *key, double v) {
 
 struct Handle {
   sl_sim *sim = nullptr;
+  bool owned = true;  // false: a tank of a served job, destroyed with its server
 };
 static void finalize(napi_env, void *data, void *) {
   Handle *h = (Handle *)data;
-  if (h->sim) sl_destroy(h->sim);
+  if (h->sim && h->owned) sl_destroy(h->sim);
   delete h;
 }
 static sl_sim *get_sim(napi_env env, napi_value v) {
@@ -485,6 +487,325 @@ static napi_value Upload(napi_env env, napi_callback_info info) {
   if (r) return throw_sl(env, s, r);
   return nullptr;
 }
+// bakeFrame(h, dir, frame): frame_%05d.npz (+ surface_%05d.exr) of the current state
+static napi_value BakeFrame(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  char dir[4096];
+  size_t len = 0;
+  NAPI_CALL(env, napi_get_value_string_utf8(env, argv[1], dir, sizeof dir, &len));
+  double f = 0;
+  napi_get_value_double(env, argv[2], &f);
+  int r = sl_bake_frame(s, dir, (uint32_t)f);
+  if (r) return throw_sl(env, s, r);
+  return nullptr;
+}
+
+// jobBytes(config): device bytes of a tank of this configuration
+static napi_value JobBytes(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  sl_config c;
+  sl_config_default(&c, "standard");
+  config_from_js(env, argv[0], c);
+  return num(env, (double)sl_job_bytes(&c));
+}
+
+// ---- serving loop (sl_server_*): jobs with an optional JS source called during tick ----
+struct JobCb {
+  napi_env env;
+  napi_ref fn;
+};
+struct ServerHandle {
+  sl_server *srv = nullptr;
+  napi_env env = nullptr;
+  std::map<uint64_t, JobCb *> cbs;
+};
+static void drop_cb(ServerHandle *h, uint64_t id) {
+  auto it = h->cbs.find(id);
+  if (it == h->cbs.end()) return;
+  napi_delete_reference(h->env, it->second->fn);
+  delete it->second;
+  h->cbs.erase(it);
+}
+static void server_finalize(napi_env, void *data, void *) {
+  ServerHandle *h = (ServerHandle *)data;
+  if (h->srv) sl_server_destroy(h->srv);
+  for (auto &kv : h->cbs) { napi_delete_reference(h->env, kv.second->fn); delete kv.second; }
+  delete h;
+}
+static ServerHandle *get_server(napi_env env, napi_value v) {
+  ServerHandle *h = nullptr;
+  if (napi_get_value_external(env, v, (void **)&h) != napi_ok || !h || !h->srv) {
+    napi_throw_error(env, nullptr, "suiri: invalid or destroyed server handle");
+    return nullptr;
+  }
+  return h;
+}
+static napi_value throw_srv(napi_env env, const sl_server *s, int code) {
+  std::string m = "suiri server error " + std::to_string(code) + ": " + sl_server_last_error(s);
+  napi_throw_error(env, nullptr, m.c_str());
+  return nullptr;
+}
+// the job's source(dt, t) -> emission records, called on the JS thread inside serverTick
+static uint32_t js_source(void *user, sl_sim *, double dt, double t, sl_emit *out, uint32_t max) {
+  JobCb *cb = (JobCb *)user;
+  napi_env env = cb->env;
+  bool pending = false;
+  napi_is_exception_pending(env, &pending);
+  if (pending) return 0;
+  napi_value fn, global, res, argv[2] = {num(env, dt), num(env, t)};
+  if (napi_get_reference_value(env, cb->fn, &fn) != napi_ok || napi_get_global(env, &global) != napi_ok) return 0;
+  if (napi_call_function(env, global, fn, 2, argv, &res) != napi_ok) return 0;
+  std::vector<sl_emit> list;
+  if (!read_emits(env, res, list)) return 0;
+  const uint32_t n = list.size() < max ? (uint32_t)list.size() : max;
+  memcpy(out, list.data(), n * sizeof(sl_emit));
+  return n;
+}
+
+// serverCreate({budget, maxJobs, device, autotuneAfter, graphAfter}) -> server
+static napi_value ServerCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  size_t argc = 1;
+  napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr);
+  sl_server_config c;
+  sl_server_config_default(&c);
+  if (argc >= 1) {
+    c.hbm_budget = (uint64_t)get_num(env, argv[0], "budget", (double)c.hbm_budget);
+    c.max_jobs = (uint32_t)get_num(env, argv[0], "maxJobs", c.max_jobs);
+    c.device = (int32_t)get_num(env, argv[0], "device", c.device);
+    c.autotune_after = (uint32_t)get_num(env, argv[0], "autotuneAfter", c.autotune_after);
+    c.graph_after = (uint32_t)get_num(env, argv[0], "graphAfter", c.graph_after);
+  }
+  sl_server *srv = nullptr;
+  int r = sl_server_create(&c, &srv);
+  if (r) return throw_sl(env, nullptr, r);
+  ServerHandle *h = new ServerHandle();
+  h->srv = srv;
+  h->env = env;
+  napi_value ext;
+  NAPI_CALL(env, napi_create_external(env, h, server_finalize, nullptr, &ext));
+  return ext;
+}
+
+// serverSubmit(srv, {config, objects, fill, steps, dt, source, keep, bakeDir, bakeEvery}) -> id
+static napi_value ServerSubmit(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  ServerHandle *h = get_server(env, argv[0]);
+  if (!h) return nullptr;
+  napi_value o = argv[1], v;
+  sl_job j;
+  memset(&j, 0, sizeof j);
+  sl_config_default(&j.cfg, "standard");
+  bool has = false;
+  napi_has_named_property(env, o, "config", &has);
+  if (has) { napi_get_named_property(env, o, "config", &v); config_from_js(env, v, j.cfg); }
+  std::vector<float> rows;
+  napi_has_named_property(env, o, "objects", &has);
+  if (has) {
+    napi_get_named_property(env, o, "objects", &v);
+    bool ta = false;
+    napi_is_typedarray(env, v, &ta);
+    if (ta) {
+      napi_typedarray_type t;
+      size_t n = 0;
+      void *data = nullptr;
+      napi_get_typedarray_info(env, v, &t, &n, &data, nullptr, nullptr);
+      if (t != napi_float32_array || n % 28) { napi_throw_type_error(env, nullptr, "objects: Float32Array of 28-float rows"); return nullptr; }
+      rows.assign((float *)data, (float *)data + n);
+    }
+  }
+  std::vector<sl_emit> fill;
+  napi_has_named_property(env, o, "fill", &has);
+  if (has) {
+    napi_get_named_property(env, o, "fill", &v);
+    if (!read_emits(env, v, fill)) { napi_throw_type_error(env, nullptr, "fill: list of emission records"); return nullptr; }
+  }
+  j.objects = rows.data();
+  j.n_objects = (uint32_t)(rows.size() / 28);
+  j.fill = fill.data();
+  j.n_fill = (uint32_t)fill.size();
+  j.steps = (uint32_t)get_num(env, o, "steps", 100);
+  j.dt = get_num(env, o, "dt", 1.0 / 120);
+  j.keep = get_num(env, o, "keep", 0) != 0 ? 1u : 0u;
+  j.bake_every = (uint32_t)get_num(env, o, "bakeEvery", 0);
+  char dir[4096] = {0};
+  napi_has_named_property(env, o, "bakeDir", &has);
+  if (has) {
+    napi_get_named_property(env, o, "bakeDir", &v);
+    size_t len = 0;
+    if (napi_get_value_string_utf8(env, v, dir, sizeof dir, &len) == napi_ok && len) j.bake_dir = dir;
+  }
+  JobCb *cb = nullptr;
+  napi_has_named_property(env, o, "source", &has);
+  if (has) {
+    napi_get_named_property(env, o, "source", &v);
+    napi_valuetype t;
+    napi_typeof(env, v, &t);
+    if (t == napi_function) {
+      cb = new JobCb{env, nullptr};
+      napi_create_reference(env, v, 1, &cb->fn);
+      j.source = js_source;
+      j.user = cb;
+    }
+  }
+  uint64_t id = 0;
+  int r = sl_server_submit(h->srv, &j, &id);
+  if (cb) h->cbs[id] = cb;
+  if (r && r != SL_EBUDGET) return throw_srv(env, h->srv, r);
+  return num(env, (double)id);  // a refused job is reported by serverPoll
+}
+
+static napi_value info_to_js(napi_env env, const sl_job_info &i) {
+  static const char *st[] = {"queued", "running", "done", "cancelled", "refused"};
+  napi_value o, s;
+  napi_create_object(env, &o);
+  set_num(env, o, "id", (double)i.id);
+  napi_create_string_utf8(env, i.status >= 0 && i.status < 5 ? st[i.status] : "?", NAPI_AUTO_LENGTH, &s);
+  napi_set_named_property(env, o, "status", s);
+  set_num(env, o, "frames", i.frames);
+  set_num(env, o, "steps", i.steps);
+  set_num(env, o, "tSubmit", i.t_submit);
+  set_num(env, o, "tAdmit", i.t_admit);
+  set_num(env, o, "tDone", i.t_done);
+  set_num(env, o, "particleSteps", i.particle_steps);
+  set_num(env, o, "bytes", (double)i.bytes);
+  set_num(env, o, "count", i.count);
+  set_num(env, o, "wwCount", i.ww_count);
+  return o;
+}
+
+// serverTick(srv) -> number of jobs stepped
+static napi_value ServerTick(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  ServerHandle *h = get_server(env, argv[0]);
+  if (!h) return nullptr;
+  uint32_t running = 0;
+  int r = sl_server_tick(h->srv, &running);
+  bool pending = false;
+  napi_is_exception_pending(env, &pending);
+  if (pending) return nullptr;  // a source threw
+  if (r) return throw_srv(env, h->srv, r);
+  return num(env, running);
+}
+
+// serverPoll(srv) -> [info] of the jobs retired since the last poll
+static napi_value ServerPoll(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  ServerHandle *h = get_server(env, argv[0]);
+  if (!h) return nullptr;
+  napi_value arr;
+  napi_create_array(env, &arr);
+  sl_job_info buf[64];
+  uint32_t k = 0, n = 0;
+  do {
+    sl_server_poll(h->srv, buf, 64, &n);
+    for (uint32_t i = 0; i < n; i++) {
+      napi_set_element(env, arr, k++, info_to_js(env, buf[i]));
+      drop_cb(h, buf[i].id);
+    }
+  } while (n == 64);
+  return arr;
+}
+
+static bool get_id(napi_env env, napi_value v, uint64_t *id) {
+  double d = 0;
+  if (napi_get_value_double(env, v, &d) != napi_ok) return false;
+  *id = (uint64_t)d;
+  return true;
+}
+// serverInfo(srv, id) / serverCancel(srv, id) / serverRelease(srv, id) / serverJobSim(srv, id)
+static napi_value ServerInfo(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  ServerHandle *h = get_server(env, argv[0]);
+  uint64_t id;
+  if (!h || !get_id(env, argv[1], &id)) return nullptr;
+  sl_job_info i;
+  if (sl_server_info(h->srv, id, &i)) { napi_throw_error(env, nullptr, "serverInfo: unknown job"); return nullptr; }
+  return info_to_js(env, i);
+}
+static napi_value ServerCancel(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  ServerHandle *h = get_server(env, argv[0]);
+  uint64_t id;
+  if (!h || !get_id(env, argv[1], &id)) return nullptr;
+  int r = sl_server_cancel(h->srv, id);
+  if (r) return throw_srv(env, h->srv, r);
+  return nullptr;
+}
+static napi_value ServerRelease(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  ServerHandle *h = get_server(env, argv[0]);
+  uint64_t id;
+  if (!h || !get_id(env, argv[1], &id)) return nullptr;
+  int r = sl_server_release(h->srv, id);
+  if (r) return throw_srv(env, h->srv, r);
+  return nullptr;
+}
+static napi_value ServerJobSim(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  ServerHandle *h = get_server(env, argv[0]);
+  uint64_t id;
+  if (!h || !get_id(env, argv[1], &id)) return nullptr;
+  sl_sim *s = sl_server_job_sim(h->srv, id);
+  napi_value r;
+  if (!s) { napi_get_null(env, &r); return r; }
+  Handle *hh = new Handle();
+  hh->sim = s;
+  hh->owned = false;
+  NAPI_CALL(env, napi_create_external(env, hh, finalize, nullptr, &r));
+  return r;
+}
+// serverStats(srv) -> object
+static napi_value ServerStats(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  ServerHandle *h = get_server(env, argv[0]);
+  if (!h) return nullptr;
+  sl_server_stats_t st;
+  sl_server_stats(h->srv, &st);
+  napi_value o;
+  napi_create_object(env, &o);
+  set_num(env, o, "budget", (double)st.budget);
+  set_num(env, o, "used", (double)st.used);
+  set_num(env, o, "queued", st.queued);
+  set_num(env, o, "running", st.running);
+  set_num(env, o, "resident", st.resident);
+  set_num(env, o, "done", st.done);
+  set_num(env, o, "ticks", (double)st.ticks);
+  set_num(env, o, "frames", (double)st.frames);
+  set_num(env, o, "particleSteps", st.particle_steps);
+  set_num(env, o, "uptime", st.uptime);
+  set_num(env, o, "frameMsP50", st.step_ms_p50);
+  set_num(env, o, "frameMsP99", st.step_ms_p99);
+  set_num(env, o, "latencyP50", st.latency_p50);
+  set_num(env, o, "latencyP99", st.latency_p99);
+  return o;
+}
+static napi_value ServerDestroy(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  ServerHandle *h = nullptr;
+  napi_get_value_external(env, argv[0], (void **)&h);
+  if (h && h->srv) {
+    sl_server_destroy(h->srv);
+    h->srv = nullptr;
+    for (auto &kv : h->cbs) { napi_delete_reference(h->env, kv.second->fn); delete kv.second; }
+    h->cbs.clear();
+  }
+  return nullptr;
+}
+
 static napi_value Version(napi_env env, napi_callback_info) { return num(env, sl_version()); }
 static napi_value BuildInfo(napi_env env, napi_callback_info) {
   napi_value v;
@@ -519,6 +840,18 @@ static napi_value Init(napi_env env, napi_value exports) {
       {"upload", 0, Upload, 0, 0, 0, napi_default, 0},
       {"version", 0, Version, 0, 0, 0, napi_default, 0},
       {"buildInfo", 0, BuildInfo, 0, 0, 0, napi_default, 0},
+      {"bakeFrame", 0, BakeFrame, 0, 0, 0, napi_default, 0},
+      {"jobBytes", 0, JobBytes, 0, 0, 0, napi_default, 0},
+      {"serverCreate", 0, ServerCreate, 0, 0, 0, napi_default, 0},
+      {"serverSubmit", 0, ServerSubmit, 0, 0, 0, napi_default, 0},
+      {"serverTick", 0, ServerTick, 0, 0, 0, napi_default, 0},
+      {"serverPoll", 0, ServerPoll, 0, 0, 0, napi_default, 0},
+      {"serverInfo", 0, ServerInfo, 0, 0, 0, napi_default, 0},
+      {"serverCancel", 0, ServerCancel, 0, 0, 0, napi_default, 0},
+      {"serverRelease", 0, ServerRelease, 0, 0, 0, napi_default, 0},
+      {"serverJobSim", 0, ServerJobSim, 0, 0, 0, napi_default, 0},
+      {"serverStats", 0, ServerStats, 0, 0, 0, napi_default, 0},
+      {"serverDestroy", 0, ServerDestroy, 0, 0, 0, napi_default, 0},
   };
   napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
   return exports;

@@ -13,6 +13,55 @@ const { shapeParams, boundRadius, obstacleRot } = require('./scene');
 const addon = require(path.join(__dirname, '..', 'suiri_napi.node'));
 
 const MAX_OBJ = 64;
+
+// obstacles and drains -> packed 28-float object records; velocities from the previous transforms
+// `prevXf` (id -> {pos, R}) (50-sim.js:165-211). Returns {data, n, nextXf, moving}.
+function packObjects(items, prevXf, dtMove) {
+  const data = new Float32Array(MAX_OBJ * 28);
+  let n = 0;
+  const nextXf = new Map();
+  let moving = false;
+  for (const it of items) {
+    if (n >= MAX_OBJ) break;
+    if (it.kind !== 'obstacle' && it.kind !== 'drain') continue;
+    const R = it.kind === 'obstacle' ? obstacleRot(it) : m3.euler(it.rot[0], it.rot[1], it.rot[2]);
+    const RT = m3.transpose(R);
+    let type, d, flag, bound;
+    if (it.kind === 'obstacle') {
+      const sp = shapeParams(it);
+      type = sp[0]; d = sp.slice(1); flag = 1; bound = boundRadius(it) + 0.002;
+    } else {
+      type = 9; d = [it.r, 0.012, 0, 0]; flag = 2; bound = it.r + 0.02;
+    }
+    let lin = [0, 0, 0], ang = [0, 0, 0];
+    const prev = prevXf.get(it.id);
+    if (prev && dtMove > 0) {
+      lin = v3.mul(v3.sub(it.pos, prev.pos), 1 / dtMove);
+      ang = v3.mul(m3.log(m3.mul(R, m3.transpose(prev.R))), 1 / dtMove);
+      if (v3.len(lin) > 1e-5 || v3.len(ang) > 1e-4) moving = true;
+      if (v3.len(lin) > 4) lin = v3.mul(v3.norm(lin), 4);
+      if (v3.len(ang) > 40) ang = v3.mul(v3.norm(ang), 40);
+    }
+    nextXf.set(it.id, { pos: [...it.pos], R });
+    const o = n * 28;
+    data.set([RT[0], RT[1], RT[2], 0, RT[3], RT[4], RT[5], 0, RT[6], RT[7], RT[8], 0], o);
+    data.set([it.pos[0], it.pos[1], it.pos[2], type], o + 12);
+    data.set([d[0], d[1], d[2], d[3]], o + 16);
+    data.set([lin[0], lin[1], lin[2], bound], o + 20);
+    data.set([ang[0], ang[1], ang[2], flag], o + 24);
+    n++;
+  }
+  return { data: data.subarray(0, n * 28), n, nextXf, moving };
+}
+
+// grid of a tank configuration, 50-sim.js:33-45: {nx, ny, nz, h, origin, rim, vp, radius}
+function gridOf(cfg) {
+  const nx = Math.max(8, Math.round(cfg.tank_w / cfg.h));
+  const ny = Math.max(8, Math.round(cfg.tank_h / cfg.h));
+  const nz = Math.max(8, Math.round(cfg.tank_d / cfg.h));
+  const h = Math.min(cfg.tank_w / nx, cfg.tank_h / ny, cfg.tank_d / nz);
+  return { nx, ny, nz, h, origin: [-nx * h / 2, 0, -nz * h / 2], rim: ny * h, vp: Math.pow(h / 2, 3), radius: 0.5 * h * 1.05 };
+}
 const FIELDS = {
   particles: 0, ww: 1, counters: 2, acc: 3, cellCount: 4, vel: 5, velOld: 6, frac: 7, svel: 8, cellType: 9,
   divg: 10, pres: 11, objs: 12, rsrc: 13, rippleState: 14, rippleHat: 15, rippleField: 16, rippleMask: 17,
@@ -33,7 +82,8 @@ class FluidSim {
     const g = addon.gridInfo(this.handle);
     this.nx = g.nx; this.ny = g.ny; this.nz = g.nz; this.h = g.h;
     this.origin = g.origin; this.rim = g.rim; this.nodes = g.nodes; this.cells = g.cells;
-    this.cap = g.cap; this.wwCap = g.wwCap; this.vp = g.vp; this.radius = g.radius;
+    this.cap = g.cap; this.wwCap = g.wwCap; this.radius = g.radius;
+    this.vp = Math.pow(this.h / 2, 3);  // 50-sim.js:44, in JS like the reference (the emitters use it)
     this.iters = g.iters;
     this.ripple = { R: g.R, M: g.M };
     this.emitAcc = new Map();
@@ -57,45 +107,15 @@ class FluidSim {
   // obstacles and drains -> packed 28-float object records; velocities from the previous transforms
   // (50-sim.js:165-211). Returns whether any object moved.
   setObjects(items, dtMove) {
-    const data = new Float32Array(MAX_OBJ * 28);
-    let n = 0;
-    const nextXf = new Map();
-    let moving = false;
-    for (const it of items) {
-      if (n >= MAX_OBJ) break;
-      if (it.kind !== 'obstacle' && it.kind !== 'drain') continue;
-      const R = it.kind === 'obstacle' ? obstacleRot(it) : m3.euler(it.rot[0], it.rot[1], it.rot[2]);
-      const RT = m3.transpose(R);
-      let type, d, flag, bound;
-      if (it.kind === 'obstacle') {
-        const sp = shapeParams(it);
-        type = sp[0]; d = sp.slice(1); flag = 1; bound = boundRadius(it) + 0.002;
-      } else {
-        type = 9; d = [it.r, 0.012, 0, 0]; flag = 2; bound = it.r + 0.02;
-      }
-      let lin = [0, 0, 0], ang = [0, 0, 0];
-      const prev = this.prevXf.get(it.id);
-      if (prev && dtMove > 0) {
-        lin = v3.mul(v3.sub(it.pos, prev.pos), 1 / dtMove);
-        ang = v3.mul(m3.log(m3.mul(R, m3.transpose(prev.R))), 1 / dtMove);
-        if (v3.len(lin) > 1e-5 || v3.len(ang) > 1e-4) moving = true;
-        if (v3.len(lin) > 4) lin = v3.mul(v3.norm(lin), 4);
-        if (v3.len(ang) > 40) ang = v3.mul(v3.norm(ang), 40);
-      }
-      nextXf.set(it.id, { pos: [...it.pos], R });
-      const o = n * 28;
-      data.set([RT[0], RT[1], RT[2], 0, RT[3], RT[4], RT[5], 0, RT[6], RT[7], RT[8], 0], o);
-      data.set([it.pos[0], it.pos[1], it.pos[2], type], o + 12);
-      data.set([d[0], d[1], d[2], d[3]], o + 16);
-      data.set([lin[0], lin[1], lin[2], bound], o + 20);
-      data.set([ang[0], ang[1], ang[2], flag], o + 24);
-      n++;
-    }
+    const { data, n, nextXf, moving } = packObjects(items, this.prevXf, dtMove);
     this.prevXf = nextXf;
     this.objCount = n;
-    addon.setObjects(this.handle, data.subarray(0, n * 28));
+    addon.setObjects(this.handle, data);
     return moving;
   }
+
+  // frame_%05d.npz (+ surface_%05d.exr) of the current state, the reference's bake cache layout
+  bake(dir, frame) { addon.bakeFrame(this.handle, dir, frame); }
 
   // emissions outside the time stepping (water blocks at a reset, or poured in) (50-sim.js:248-265)
   emitNow(list, fill) {
@@ -184,4 +204,4 @@ class FluidSim {
   get waterLiters() { return this.count * this.vp * 1000; }
 }
 
-module.exports = { FluidSim, FIELDS, addon };
+module.exports = { FluidSim, FIELDS, addon, packObjects, gridOf, QUALITY };

@@ -3,7 +3,8 @@
 // the reference's automation API (window.suiriLab, src/js/90-main.js:145-205) without the DOM parts.
 const scene = require('./scene');
 const emit = require('./emit');
-const { FluidSim, FIELDS, addon } = require('./fluid_sim');
+const { FluidSim, FIELDS, addon, packObjects, gridOf } = require('./fluid_sim');
+const { BakeServer } = require('./server');
 
 // one scene + one simulation, driven like the reference's headless runs
 class SuiriLab {
@@ -74,4 +75,4 @@ class SuiriLab {
   destroy() { if (this.sim) this.sim.destroy(); }
 }
 
-module.exports = { FluidSim, FIELDS, addon, SuiriLab, ...scene, ...emit };
+module.exports = { FluidSim, FIELDS, addon, SuiriLab, BakeServer, packObjects, gridOf, ...scene, ...emit };

@@ -61,3 +61,21 @@ def test_js_fluidsim_dam_preset_matches_oracle(tmp_path):
     assert np.array_equal(gp[:, 3], op[:, 3])
     pres = np.fromfile(tmp_path / "pres.bin", np.float32)
     assert np.abs(pres - o.field("pres")).max() <= 1e-5 * max(np.abs(o.field("pres")).max(), 1e-30)
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(NODE is None, reason="node not installed")
+def test_js_bake_server_job_equals_plain_fluidsim(tmp_path):
+    """A faucet preset served by the Node BakeServer beside another job equals the same preset run
+    alone through FluidSim with one substep per frame, particle for particle; its bake frames land
+    in the requested directory."""
+    frames = 12
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "serve_scene.js"), "faucet", str(frames), str(tmp_path)],
+                       capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert out["plainCount"] > 0 and out["frames"] == frames
+    assert out["equal"], out
+    assert sorted(out["done"]) == ["done", "done"]
+    names = sorted(os.listdir(tmp_path))
+    assert "frame_00000.npz" in names and "frame_%05d.npz" % frames in names
