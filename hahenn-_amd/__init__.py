@@ -18,7 +18,7 @@ LIB_PATH = os.path.join(HERE, "libsuiri_amd.so")
 SL_OK = 0
 SL_EBUDGET = -5
 F_PARTICLES, F_WW, F_COUNTERS, F_ACC, F_CELLCOUNT, F_VEL, F_VELOLD, F_FRAC, F_SVEL, F_CELLTYPE, \
-    F_DIVG, F_PRES, F_OBJS, F_RSRC, F_RSTATE, F_RHAT, F_RFIELD, F_RMASK, F_RTEX, F_TOPH, F_RTEX16 = range(21)
+    F_DIVG, F_PRES, F_OBJS, F_RSRC, F_RSTATE, F_RHAT, F_RFIELD, F_RMASK, F_RTEX, F_TOPH, F_RTEX16, F_ARGS = range(22)
 S_GEOM, S_P2G, S_NORMFORCE, S_DIVERGENCE, S_PRESSURE, S_PROJECT, S_G2P, S_COMPACT, S_WW, S_RIPPLE, \
     S_SOR = range(11)
 T_NAMES = ["emit", "clear", "geom", "p2g", "normforce", "divergence", "sor", "project", "g2p", "compact",
@@ -30,7 +30,8 @@ FIELD_DTYPE = {F_PARTICLES: (np.float32, 8), F_WW: (np.float32, 8), F_COUNTERS: 
                F_CELLTYPE: (np.uint32, 1), F_DIVG: (np.float32, 1), F_PRES: (np.float32, 1),
                F_OBJS: (np.float32, 28), F_RSRC: (np.int32, 1), F_RSTATE: (np.float32, 2),
                F_RHAT: (np.float32, 2), F_RFIELD: (np.float32, 2), F_RMASK: (np.float32, 1),
-               F_RTEX: (np.float32, 4), F_TOPH: (np.float32, 1), F_RTEX16: (np.float16, 4)}
+               F_RTEX: (np.float32, 4), F_TOPH: (np.float32, 1), F_RTEX16: (np.float16, 4),
+               F_ARGS: (np.uint32, 1)}
 
 
 class SlConfig(ctypes.Structure):

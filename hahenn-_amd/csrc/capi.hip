@@ -336,6 +336,7 @@ static int tank_buffers(sl_sim *s, SlTank &T, Take take) {
   }
   if (s->ext_counters) T.C = s->ext_counters;
   else TAKE(T.C, sizeof(SlCounters));
+  TAKE(T.args, 64);
   TAKE(T.acc, N * 24);
   TAKE(T.cellCount, C * 4);
   TAKE(T.vel, N * 12); TAKE(T.velOld, N * 12); TAKE(T.vmask, N);
@@ -909,6 +910,7 @@ extern "C" size_t sl_field_bytes(const sl_sim *s, int f) {
     case SL_F_RSTATE: case SL_F_RHAT: case SL_F_RFIELD: return s->R ? M2 * 8 : 0;
     case SL_F_RTEX: return s->R ? R1 * 16 : 0;
     case SL_F_RTEX16: return s->R ? R1 * 8 : 0;
+    case SL_F_ARGS: return 64;
     case SL_F_TOPH: return s->R ? (size_t)s->ht.wmap * s->ht.wmap * 4 : 0;
   }
   return 0;
@@ -1028,6 +1030,7 @@ extern "C" int sl_readback(sl_sim *s, int f, void *dst, size_t bytes) {
     case SL_F_RMASK: return d2h(s, dst, T.rmask, need);
     case SL_F_RTEX: return d2h(s, dst, T.rtex, need);
     case SL_F_TOPH: return d2h(s, dst, T.topH, need);
+    case SL_F_ARGS: return d2h(s, dst, T.args, need);
     case SL_F_RTEX16: {
       size_t R1 = (size_t)(s->R + 1) * (s->R + 1);
       std::vector<float> a(R1 * 4);

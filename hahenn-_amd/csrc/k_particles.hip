@@ -395,6 +395,13 @@ __global__ void k_count(SlTank *tanks, const uint32_t *list, int ntanks, uint32_
     const uint32_t lim = cap > 64u ? cap - 64u : 0u, c = C->count;
     if (c < lim) C->count = c + min(T.emit_total, lim - c);
   }
+  // dispatch / draw-indirect arguments for the particle and whitewater counts (31-wgsl-sim2.js:343-348)
+  const uint32_t n = min(C->count, cap), w = min(C->wwCount, wcap);
+  uint32_t *a = T.args;
+  a[0] = (n + 255u) / 256u; a[1] = 1u; a[2] = 1u;
+  a[3] = (w + 255u) / 256u; a[4] = 1u; a[5] = 1u;
+  a[6] = 6u; a[7] = n; a[8] = 0u; a[9] = 0u;
+  a[10] = 6u; a[11] = w; a[12] = 0u; a[13] = 0u;
 }
 void sl_k_count(const SlLaunch &L, uint32_t mode, uint32_t y) {
   int nt = L.w.ntanks;

@@ -54,6 +54,7 @@ struct SlTank {
   uint8_t *pf[2];
   float4 *wp[2], *wv[2];
   SlCounters *C;
+  uint32_t *args;  // 16 u32 dispatch / draw-indirect arguments (COUNT)
   // grid
   int32_t *acc;
   uint32_t *cellCount;

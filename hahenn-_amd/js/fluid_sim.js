@@ -65,7 +65,7 @@ function gridOf(cfg) {
 const FIELDS = {
   particles: 0, ww: 1, counters: 2, acc: 3, cellCount: 4, vel: 5, velOld: 6, frac: 7, svel: 8, cellType: 9,
   divg: 10, pres: 11, objs: 12, rsrc: 13, rippleState: 14, rippleHat: 15, rippleField: 16, rippleMask: 17,
-  rippleTex: 18, topH: 19, rippleTex16: 20,
+  rippleTex: 18, topH: 19, rippleTex16: 20, args: 21,
 };
 const QUALITY = ['low', 'standard', 'high'];
 
@@ -184,7 +184,7 @@ class FluidSim {
     if (f === undefined) throw new Error(`unknown field ${name}`);
     const ab = addon.readback(this.handle, f);
     if (name === 'acc' || name === 'rsrc') return new Int32Array(ab);
-    if (name === 'counters' || name === 'cellCount' || name === 'cellType') return new Uint32Array(ab);
+    if (name === 'counters' || name === 'cellCount' || name === 'cellType' || name === 'args') return new Uint32Array(ab);
     if (name === 'rippleTex16') return new Uint16Array(ab);
     return new Float32Array(ab);
   }

@@ -116,7 +116,10 @@ enum sl_field {
   SL_F_RTEX = 18,     /* (R+1)^2 x 4 f32: height, slope x, slope z, 0 (readback only) */
   SL_F_TOPH = 19,     /* wmap^2 f32 top-surface height map, -1 dry (input of the ripple mask) */
   SL_F_RTEX16 = 20,   /* (R+1)^2 x 4 f16 rgba16float texture of the reference (64-ripple.js:160) */
-  SL_F_COUNT_ = 21
+  SL_F_ARGS = 21,     /* 16 u32 dispatch / draw-indirect arguments written by every COUNT
+                         (31-wgsl-sim2.js:343-348): [ceil(n/256),1,1, ceil(w/256),1,1, 6,n,0,0, 6,w,0,0]
+                         for n live particles and w whitewater particles (readback only) */
+  SL_F_COUNT_ = 22
 };
 
 /* Single stages, for stage-level parity checks. One call = one dispatch of the reference kernel(s). */

@@ -55,6 +55,8 @@ def test_one_step_fields():
     assert_close("positions", gp[:, :3], op[:, :3], TOL)
     assert_close("velocities", gp[:, 4:7], op[:, 4:7], TOL)
     assert np.array_equal(gp[:, 3], op[:, 3]) and np.array_equal(gp[:, 7], op[:, 7])
+    # renderer-facing dispatch / draw-indirect arguments of the last COUNT (31-wgsl-sim2.js:343-348)
+    assert np.array_equal(g.readback(suiri.F_ARGS)[:14], o.field("args")[:14])
 
 
 def test_p2g_variants_from_identical_state():
