@@ -62,7 +62,8 @@ def test_one_step_fields():
 def test_p2g_variants_from_identical_state():
     """Every P2G variant (chunked LDS boxes 0-7, binned 8) reproduces the oracle's fixed-point
     accumulators and cell counts exactly, also when the particle order is shuffled (integer sums do
-    not depend on the order; scattered chunks exercise the global-atomic fallback)."""
+    not depend on the order; scattered chunks exercise the global-atomic fallback) and when two cells
+    hold 600 and 100 particles (the binned splat's chunk queue)."""
     o, g = twin.make_twin(twin.small_scene())
     for _ in range(8):
         twin.oracle_step(o, 1 / 120)
@@ -79,7 +80,21 @@ def test_p2g_variants_from_identical_state():
     rng = np.random.default_rng(5)
     shuffled = parts.copy()
     shuffled[:n] = parts[:n][rng.permutation(n)]
-    for order, p in (("sorted", parts), ("shuffled", shuffled)):
+    # crowded: 700 particles packed into two cells (as under an emitter's nozzle), the rest unchanged
+    crowded = parts.copy()
+    inf = o.info()
+    cell = np.array([inf["ox"], inf["oy"], inf["oz"]]) + inf["h"] * np.array([10.0, 5.0, 7.0])
+    crowded[:600, :3] = cell + inf["h"] * rng.uniform(0.01, 0.99, (600, 3))
+    crowded[600:700, :3] = cell + inf["h"] * (np.array([1.0, 0.0, 0.0]) + rng.uniform(0.01, 0.99, (100, 3)))
+    crowded[:700, 3] = 1.0
+    o.field("parts")[:] = crowded
+    o.field("acc")[:] = 0
+    o.field("cellCount")[:] = 0
+    o.kernel("p2g")
+    want_crowded = o.field("acc").copy(), o.field("cellCount").copy()
+    cases = (("sorted", parts, (want_acc, want_cc)), ("shuffled", shuffled, (want_acc, want_cc)),
+             ("crowded", crowded, want_crowded))
+    for order, p, (wa, wc) in cases:
         g.upload(suiri.F_PARTICLES, p)
         g.upload(suiri.F_COUNTERS, o.field("counters"))
         g.set_scalar("count", n)
@@ -87,8 +102,8 @@ def test_p2g_variants_from_identical_state():
             g.set_scalar("p2g_variant", v)
             g.upload(suiri.F_ACC, np.zeros_like(want_acc))
             g.run_stage(suiri.S_P2G, 0, dt)
-            assert np.array_equal(g.readback(suiri.F_ACC), want_acc), (order, v)
-            assert np.array_equal(g.readback(suiri.F_CELLCOUNT), want_cc), (order, v)
+            assert np.array_equal(g.readback(suiri.F_ACC), wa), (order, v)
+            assert np.array_equal(g.readback(suiri.F_CELLCOUNT), wc), (order, v)
 
 
 def test_stage_by_stage_from_identical_state():
