@@ -292,10 +292,6 @@ void sl_k_p2g(const SlLaunch &L, int count_only, int variant) {
 // integer sums do not depend on the order, so the result equals per-particle atomics; the cost no
 // longer depends on how scattered the particle order is.
 // ------------------------------------------------------------------------------------------------
-#define BIN_SCAN 4096  // cells per scan block (the block prefix lives in binTop)
-#define BIN_TX 8
-#define BIN_TY 8
-#define BIN_TZ 4
 #define WIN_X (BIN_TX + 2)
 #define WIN_Y (BIN_TY + 2)
 #define WIN_Z (BIN_TZ + 2)

@@ -131,6 +131,16 @@ __device__ __forceinline__ uint32_t hashU(uint32_t x) {
 }
 __device__ __forceinline__ float rndu(uint32_t &s) { s = hashU(s); return (float)(s >> 8) / 16777216.0f; }
 
+// element i of an array of fewer than 2^32 bytes at a wave-uniform base: a 32-bit byte offset, so
+// the load takes the scalar-base + vector-offset form without 64-bit address arithmetic
+template <class E>
+__device__ __forceinline__ E ld32(const E *base, uint32_t i) {
+  return *(const E *)((const char *)base + i * (uint32_t)sizeof(E));
+}
+template <class E>
+__device__ __forceinline__ void st32(E *base, uint32_t i, E v) {
+  *(E *)((char *)base + i * (uint32_t)sizeof(E)) = v;
+}
 __device__ __forceinline__ uint32_t node_idx(const SlTank &T, int32_t i, int32_t j, int32_t k) {
   return (uint32_t)(i + (j + k * (T.ny + 1)) * (T.nx + 1));
 }
@@ -304,6 +314,10 @@ int sl_p2g_chunk(int variant);
 // binned P2G: particles (SL_WG per workgroup), scan blocks (one workgroup each), tanks (one workgroup
 // each), tiles (one workgroup each)
 #define SL_P2G_BINNED 8
+#define BIN_SCAN 4096  // cells per scan block (the block prefix lives in binTop)
+#define BIN_TX 8       // cell tile of the binned P2G and the tile G2P
+#define BIN_TY 8
+#define BIN_TZ 4
 uint32_t sl_bin_tiles(int nx, int ny, int nz);
 uint32_t sl_bin_blocks(uint32_t cells);
 void sl_k_p2g_binned(const SlLaunch &parts, const SlLaunch &blocks, const SlLaunch &tanks, const SlLaunch &tiles);
@@ -313,6 +327,13 @@ void sl_k_sor(const SlLaunch &L, uint32_t parity, uint32_t pass);
 void sl_k_project(const SlLaunch &L);
 // particle stages
 void sl_k_g2p(const SlLaunch &L, int variant);
+// G2P over the cell lists of the binned P2G of the same substep: tiles (one workgroup each) and
+// particles (SL_WG per workgroup; the live particles without a cell). G2P variants 0..3 are the
+// particle-parallel kernel, SL_G2P_TILES..SL_G2P_LAST the tile kernel, each at four occupancy
+// targets (variant & 3).
+#define SL_G2P_TILES 4
+#define SL_G2P_LAST 7
+void sl_k_g2p_tiles(const SlLaunch &tiles, const SlLaunch &parts, int occ);
 void sl_k_emit(const SlLaunch &L, uint32_t reserve = 0);
 void sl_k_count(const SlLaunch &L, uint32_t mode, uint32_t y);
 void sl_k_vmax_reset(const SlLaunch &L);

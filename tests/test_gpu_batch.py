@@ -122,9 +122,11 @@ def assert_same_tank(got, want, label):
             assert np.array_equal(bits(got.readback(f)), bits(want.readback(f))), f"{label}: ripple field {f} differs"
 
 
-def run_batch(frames, capture_at=None, tune_at=None):
+def run_batch(frames, capture_at=None, tune_at=None, variants=None):
     b = suiri.Batch([cfg_of(sp) for sp in SPECS])
     drivers = [Driver(t, sp, 100 * i + 7) for i, (t, sp) in enumerate(zip(b.tanks, SPECS))]
+    if variants:
+        b.set_variants(variants)
     for f in range(frames):
         for d in drivers:
             d.event(f)
@@ -169,6 +171,19 @@ def test_batch_autotune_keeps_results(singles):
     b.step(DT)
     b.sync()
     assert b.tanks[0].stats()["frames"] == 9
+    b.close()
+
+
+def test_batch_tile_g2p_equals_single(singles):
+    """Binned P2G with the tile G2P (variants 4-7) in every size bucket, eager then replayed from a
+    captured graph: each tank equals its single run with the default kernels. A tile G2P without the
+    binned P2G is refused."""
+    print("batch tile G2P, 4 tanks, 8 frames", flush=True)
+    b = run_batch(8, capture_at=3, variants=[(8, 4), (8, 7), (8, 5)])
+    for i, (t, s) in enumerate(zip(b.tanks, singles)):
+        assert_same_tank(t, s, f"tank {i}")
+    with pytest.raises(suiri.SuiriError, match="binned"):
+        b.set_variants([(8, 4), (3, 6), (8, 4)])
     b.close()
 
 

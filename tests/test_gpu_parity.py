@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import twin
-from twin import assert_close, suiri
+from twin import assert_close, scene, suiri
 
 pytestmark = pytest.mark.gpu
 
@@ -194,7 +194,8 @@ def test_autotune_keeps_results_bitwise():
         twin.oracle_step(o, 1 / 120)
         twin.gpu_step(g, 1 / 120)
     ch = g.autotune(reps=1)
-    assert 0 <= ch["p2g_variant"] <= 8 and 0 <= ch["g2p_variant"] < 4
+    assert 0 <= ch["p2g_variant"] <= 8 and 0 <= ch["g2p_variant"] <= 7
+    assert ch["g2p_variant"] < 4 or ch["p2g_variant"] == 8   # tile G2P only with the binned P2G
     for v in range(9):   # and every P2G variant on its own (8: binned)
         g.set_scalar("p2g_variant", v)
         twin.oracle_step(o, 1 / 120)
@@ -202,6 +203,43 @@ def test_autotune_keeps_results_bitwise():
     n = int(o.field("counters")[0])
     assert g.stats()["count"] == n
     assert np.array_equal(g.readback(suiri.F_PARTICLES)[:n], o.field("parts")[:n])
+
+
+def test_g2p_tile_variants_bitwise():
+    """The tile G2P (variants 4-7, with the binned P2G) reads nodes and cell counts from LDS windows
+    and walks the cell lists; it must give the oracle's particles bit for bit, frame after frame, with
+    a stream source pouring in from above the grid (particles outside the grid, which the tiles leave
+    to the remainder pass, and crowded cells under the stream), whitewater spawns and splash crowns
+    (same multiset; their slot order follows atomics), ripple impulses and counters. A tile variant
+    without the binned P2G runs the particle-parallel kernel."""
+    sc = twin.small_scene(whitewater=True)
+    o, g = twin.make_twin(sc)
+    inf = o.info()
+    top = inf["oy"] + 0.25
+    src = scene.make_source(pos=[inf["ox"] + 0.3, top + 0.03, inf["oz"] + 0.12], mode="stream", flow=120.0,
+                            speed=1.2, pitch=-90, yaw=0)
+    em = scene.Emitters([src], float(inf["vp"]), seed=9)
+    plan = [(8, 4), (8, 5), (8, 6), (8, 7), (3, 6), (8, 4)] * 3
+    outside = 0
+    for f, (pv, gv) in enumerate(plan):
+        g.set_scalar("p2g_variant", pv)
+        g.set_scalar("g2p_variant", gv)
+        recs = em(1 / 120, f / 120)
+        twin.oracle_step(o, 1 / 120, recs)
+        twin.gpu_step(g, 1 / 120, recs)
+        n = int(o.field("counters")[0])
+        assert g.stats()["count"] == n, f
+        gp, op = g.readback(suiri.F_PARTICLES)[:n], o.field("parts")[:n]
+        assert np.array_equal(gp.view(np.uint32), op.view(np.uint32)), (f, pv, gv)
+        outside += int((op[:, 1] > top).sum())
+        assert np.array_equal(g.readback(suiri.F_CELLCOUNT), o.field("cellCount")), f
+        gc, oc = g.readback(suiri.F_COUNTERS), o.field("counters")
+        assert [gc[0], gc[4], gc[5]] == [oc[0], oc[4], oc[5]], f   # count, vmax bits, impacts
+    assert outside > 0, "the stream should leave particles above the grid"
+    wo = int(o.info()["ww_count"])
+    assert wo > 0 and g.stats()["ww"] == wo
+    a, b = g.readback(suiri.F_WW)[:wo], o.field("ww")[:wo]
+    assert_close("whitewater", a[np.lexsort(a.T[::-1])], b[np.lexsort(b.T[::-1])], 1e-4)
 
 
 def test_multi_step_dam_break():
