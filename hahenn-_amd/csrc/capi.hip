@@ -630,16 +630,6 @@ static void encode_p2g(sl_sim *s, int variant) {
     sl_k_p2g(launch_of(s, pb, sl_p2g_chunk(variant)), 0, variant);
 }
 
-// G2P of one tank. The tile variants walk the cell lists of this substep's binned P2G (`binned`:
-// they are current); otherwise the particle-parallel kernel runs at the same occupancy target, with
-// the same results.
-static void encode_g2p(sl_sim *s, int variant, bool binned) {
-  if (variant >= SL_G2P_TILES && binned)
-    sl_k_g2p_tiles(launch_of(s, sl_bin_tiles(s->nx, s->ny, s->nz), 1), launch_of(s, s->p_ub), variant & 3);
-  else
-    sl_k_g2p(launch_of(s, s->p_ub), variant & 3);
-}
-
 static int encode_substep(sl_sim *s) {
   const uint32_t pb = s->p_ub;
   if (s->geom_dirty) {
@@ -660,11 +650,7 @@ static int encode_substep(sl_sim *s) {
     for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor(L, 0, 2 * i); sl_k_sor(L, 1, 2 * i + 1); }
   }
   { StageScope sc(s, SL_T_PROJECT, 1); sl_k_project(launch_of(s, s->nodes)); }
-  {
-    const bool tiles = s->g2p_variant >= SL_G2P_TILES && s->p2g_variant == SL_P2G_BINNED;
-    StageScope sc(s, SL_T_G2P, tiles ? 2 : 1);
-    encode_g2p(s, s->g2p_variant, s->p2g_variant == SL_P2G_BINNED);
-  }
+  { StageScope sc(s, SL_T_G2P, 1); sl_k_g2p(launch_of(s, pb), s->g2p_variant); }
   return check_launch(s, "substep");
 }
 
@@ -1149,7 +1135,7 @@ extern "C" int sl_set_scalar(sl_sim *s, const char *name, double v) {
     s->p2g_variant = (int)v;
   }
   else if (!strcmp(name, "g2p_variant")) {
-    if (!(v >= 0 && v <= SL_G2P_LAST)) FAIL(s, SL_EARG, "g2p_variant must be 0..%d", SL_G2P_LAST);
+    if (!(v >= 0 && v <= 3)) FAIL(s, SL_EARG, "g2p_variant must be 0..3");
     s->g2p_variant = (int)v;
   }
   else if (!strcmp(name, "reorder_every")) s->cfg.reorder_every = (uint32_t)v;
@@ -1204,7 +1190,7 @@ extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
       break;
     }
     case SL_S_PROJECT: sl_k_project(launch_of(s, s->nodes)); break;
-    case SL_S_G2P: encode_g2p(s, s->g2p_variant, false); break;  // cell lists need this substep's P2G
+    case SL_S_G2P: sl_k_g2p(launch_of(s, s->p_ub), s->g2p_variant); break;
     case SL_S_COMPACT: return encode_compact(s);
     case SL_S_WW:
       sl_k_ww(launch_of(s, s->ww_cap), (float)dt);
@@ -1286,13 +1272,11 @@ extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
     for (void *p : snap) (void)hipFree(p);
     FAIL(s, SL_ENOMEM, "sl_autotune: no memory for the state snapshot");
   }
-  // `cells`: the cell counts too (the G2P trials keep those of the last P2G trial, which match its
-  // cell lists)
-  auto restore = [&](bool cells) {
+  auto restore = [&]() {
     for (int i = 0; i < 7; i++)
       if (pcpy[i]) (void)hipMemcpyAsync(psrc[i], pcpy[i], i < 6 ? P * 4 : P, hipMemcpyDeviceToDevice, s->stream);
     if (ccpy) (void)hipMemcpyAsync(T.C, ccpy, sizeof(SlCounters), hipMemcpyDeviceToDevice, s->stream);
-    if (cccpy && cells) (void)hipMemcpyAsync(T.cellCount, cccpy, C * 4, hipMemcpyDeviceToDevice, s->stream);
+    if (cccpy) (void)hipMemcpyAsync(T.cellCount, cccpy, C * 4, hipMemcpyDeviceToDevice, s->stream);
     if (rcpy) (void)hipMemcpyAsync(T.rsrc, rcpy, R1 * 4, hipMemcpyDeviceToDevice, s->stream);
   };
   hipEvent_t a = ev_get(s), b = ev_get(s);
@@ -1317,14 +1301,12 @@ extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
   // G2P variants: restore the particles before every run
   best = 1e30f;
   int bestg = s->g2p_variant;
-  // the last P2G trial was the binned one, so its cell lists match the restored particles; the
-  // tile variant is kept only with binned P2G
-  for (int v = 0; v <= (s->p2g_variant == SL_P2G_BINNED ? SL_G2P_LAST : SL_G2P_TILES - 1); v++) {
+  for (int v = 0; v < 4; v++) {
     float tot = 0;
     for (uint32_t r = 0; r <= reps; r++) {
-      restore(false);
+      restore();
       (void)hipEventRecord(a, s->stream);
-      encode_g2p(s, v, true);
+      sl_k_g2p(launch_of(s, P), v);
       (void)hipEventRecord(b, s->stream);
       float ms = time_region(a, b);
       if (r) tot += ms;
@@ -1332,7 +1314,7 @@ extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
     if (tot < best) { best = tot; bestg = v; }
   }
   s->g2p_variant = bestg;
-  restore(true);
+  restore();
   HIPCHK(s, hipStreamSynchronize(s->stream));
   for (void *p : snap) (void)hipFree(p);
   s->ev_pool.push_back(a);
@@ -1359,8 +1341,7 @@ enum {
   BM_P2G0, BM_G2P0 = BM_P2G0 + SL_BUCKETS, BM_SCRATCH = BM_G2P0 + SL_BUCKETS,
   BM_BINP, BM_BINS, BM_BINT, BM_BINTILE,  // binned P2G: particles, scan blocks, tanks, tiles
   BM_TOPP, BM_TOPT,                       // top-height map: particles, texels
-  BM_GTILE0, BM_GREST0 = BM_GTILE0 + SL_BUCKETS,  // tile G2P per variant group: tiles, particles without a cell
-  BM_SOR0 = BM_GREST0 + SL_BUCKETS, BM_COUNT = BM_SOR0 + SL_SOR_BANDS
+  BM_SOR0, BM_COUNT = BM_SOR0 + SL_SOR_BANDS
 };
 
 // frame++ and the per-substep uniform (50-sim.js:151-162, :308-317), and the frame's ripple uniform
@@ -1555,15 +1536,6 @@ static int put_bin_maps(sl_batch *b, Inc inc) {
   return SL_OK;
 }
 static void batch_p2g_binned(const sl_batch *b);
-// the two maps of the tile G2P of variant group `u` over the slots `inc` selects
-template <class Inc>
-static int put_g2p_tile_maps(sl_batch *b, int u, Inc inc) {
-  int r;
-  if ((r = put_map(b, BM_GTILE0 + u, [](const sl_sim *s) { return (uint64_t)sl_bin_tiles(s->nx, s->ny, s->nz); }, 1, inc)) ||
-      (r = put_map(b, BM_GREST0 + u, [](const sl_sim *s) { return (uint64_t)s->cap; }, SL_WG, inc)))
-    return r;
-  return SL_OK;
-}
 
 // every work map and the stepped-slot list from the current membership and variant choice
 static int batch_rebuild(sl_batch *b) {
@@ -1611,9 +1583,6 @@ static int batch_rebuild(sl_batch *b) {
       if (!pass && v == SL_P2G_BINNED) {
         r = put_bin_maps(b, inc);
         b->total[base + used] = 1;  // launched through the BM_BIN* maps
-      } else if (pass && v >= SL_G2P_TILES) {  // set only where the bucket's P2G is binned
-        r = put_g2p_tile_maps(b, used, inc);
-        b->total[base + used] = 1;  // launched through BM_GTILE0 / BM_GREST0 + used
       } else {
         r = put_map(b, base + used, [](const sl_sim *s) { return (uint64_t)s->cap; },
                     pass ? SL_WG : sl_p2g_chunk(v), inc);
@@ -1880,11 +1849,8 @@ static int batch_encode_frame(sl_batch *b, double dt) {
     for (uint32_t i = b->band_start[j]; i < b->band_start[j + 1]; i++) { sl_k_sor(Ls, 0, 2 * i); sl_k_sor(Ls, 1, 2 * i + 1); }
   }
   sl_k_project(batch_launch(b, BM_NODES));
-  for (int u = 0; u < SL_BUCKETS; u++) {
-    const int v = b->map_variant[BM_G2P0 + u];
-    if (v >= SL_G2P_TILES) sl_k_g2p_tiles(batch_launch(b, BM_GTILE0 + u), batch_launch(b, BM_GREST0 + u), v & 3);
-    else if (v >= 0) sl_k_g2p(batch_launch(b, BM_G2P0 + u), v);
-  }
+  for (int u = 0; u < SL_BUCKETS; u++)
+    if (b->map_variant[BM_G2P0 + u] >= 0) sl_k_g2p(batch_launch(b, BM_G2P0 + u), b->map_variant[BM_G2P0 + u]);
   const SlLaunch Lc = batch_launch(b, BM_COMPACT);
   sl_k_compact_count(Lc);
   SlLaunch Lscan = Lc;
@@ -2083,7 +2049,7 @@ extern "C" int sl_batch_autotune(sl_batch *b, uint32_t reps, uint32_t *choice) {
   if (!reps) reps = 2;
   int r = batch_prepare(b);
   if (r || (r = sl_batch_sync(b))) return r;
-  struct Saved { void *live, *copy; size_t bytes; bool cells; };
+  struct Saved { void *live, *copy; size_t bytes; };
   std::vector<Saved> items;
   auto release = [&]() { for (auto &it : items) (void)hipFree(it.copy); };
   for (uint32_t t = 0; t < b->cap; t++) {
@@ -2100,15 +2066,12 @@ extern "C" int sl_batch_autotune(sl_batch *b, uint32_t reps, uint32_t *choice) {
       if (!by[i]) continue;
       void *p = nullptr;
       if (hipMalloc(&p, by[i]) != hipSuccess) { release(); return batch_fail(b, SL_ENOMEM, "batch autotune: snapshot"); }
-      items.push_back({src[i], p, by[i], i == 8});
+      items.push_back({src[i], p, by[i]});
       (void)hipMemcpyAsync(p, src[i], by[i], hipMemcpyDeviceToDevice, b->stream);
     }
   }
-  // `cells`: the cell counts too (the G2P trials keep those of the last P2G trial, which match its
-  // cell lists)
-  auto restore = [&](bool cells) {
-    for (auto &it : items)
-      if (cells || !it.cells) (void)hipMemcpyAsync(it.live, it.copy, it.bytes, hipMemcpyDeviceToDevice, b->stream);
+  auto restore = [&]() {
+    for (auto &it : items) (void)hipMemcpyAsync(it.live, it.copy, it.bytes, hipMemcpyDeviceToDevice, b->stream);
   };
   auto clear_acc = [&]() {
     for (uint32_t t = 0; t < b->cap; t++)
@@ -2150,24 +2113,19 @@ extern "C" int sl_batch_autotune(sl_batch *b, uint32_t reps, uint32_t *choice) {
     if ((r = put_map(b, BM_SCRATCH, [](const sl_sim *s) { return (uint64_t)s->cap; }, SL_WG, in_k)) ||
         (r = maps_upload(b)))
       break;
-    // the last P2G trial was the binned one over this bucket: its cell lists serve the tile trials
-    if (b->p2g_variant[k] == SL_P2G_BINNED && ((r = put_g2p_tile_maps(b, 0, in_k)) || (r = maps_upload(b)))) break;
     best = 1e30f;
-    for (int v = 0; v <= (b->p2g_variant[k] == SL_P2G_BINNED ? SL_G2P_LAST : SL_G2P_TILES - 1); v++) {
+    for (int v = 0; v < 4; v++) {
       float tot = 0;
       for (uint32_t rep = 0; rep <= reps; rep++) {
-        restore(false);
-        float ms = timed([&] {
-          if (v >= SL_G2P_TILES) sl_k_g2p_tiles(batch_launch(b, BM_GTILE0), batch_launch(b, BM_GREST0), v & 3);
-          else sl_k_g2p(batch_launch(b, BM_SCRATCH), v);
-        });
+        restore();
+        float ms = timed([&] { sl_k_g2p(batch_launch(b, BM_SCRATCH), v); });
         if (rep) tot += ms;
       }
       if (tot < best) { best = tot; b->g2p_variant[k] = v; }
     }
   }
   clear_acc();  // the next P2G adds into cleared accumulators
-  restore(true);
+  restore();
   hipError_t e = hipStreamSynchronize(b->stream);
   release();
   if (r) return r;
@@ -2182,9 +2140,7 @@ extern "C" int sl_batch_autotune(sl_batch *b, uint32_t reps, uint32_t *choice) {
 extern "C" int sl_batch_set_variants(sl_batch *b, const uint32_t *choice) {
   if (!b || !choice) return batch_fail(b, SL_EARG, "sl_batch_set_variants: bad arguments");
   for (int k = 0; k < SL_BUCKETS; k++)
-    if (choice[2 * k] > SL_P2G_BINNED || choice[2 * k + 1] > SL_G2P_LAST ||
-        (choice[2 * k + 1] >= SL_G2P_TILES && choice[2 * k] != SL_P2G_BINNED))
-      return batch_fail(b, SL_EARG, "variant out of range (the tile G2P needs the binned P2G)");
+    if (choice[2 * k] > SL_P2G_BINNED || choice[2 * k + 1] > 3) return batch_fail(b, SL_EARG, "variant out of range");
   for (int k = 0; k < SL_BUCKETS; k++) { b->p2g_variant[k] = (int)choice[2 * k]; b->g2p_variant[k] = (int)choice[2 * k + 1]; }
   b->maps_dirty = true;
   return SL_OK;

@@ -8,31 +8,24 @@
   if ((W).ntanks > 1) { t = tank_of((W).start, (W).ntanks, b); lb = (b) - (W).start[t]; }
 
 // ------------------------------------------------------------------------------------------------
-// Node and cell-count lookups of G2P. GlobalLook reads the grid arrays; TileLook (below) reads the
-// LDS windows of a cell tile and falls back to GlobalLook outside them. Both return the same values,
-// so the G2P body computes the same bits through either.
-// ------------------------------------------------------------------------------------------------
-struct GlobalLook {
-  // mask byte and new and old velocity component of in-grid node (i,j,k)
-  __device__ __forceinline__ void fetch(const SlTank &T, int comp, int32_t i, int32_t j, int32_t k, uint32_t &m,
-                                        float &vn, float &vo) const {
-    const uint32_t n = node_idx(T, i, j, k);
-    m = ld32(T.vmask, n);
-    vn = ld32(T.vel + comp * (size_t)T.nodes, n);
-    vo = ld32(T.velOld + comp * (size_t)T.nodes, n);
-  }
-  // particle count of an in-grid cell
-  __device__ __forceinline__ uint32_t count(const SlTank &T, int32_t x, int32_t y, int32_t z) const {
-    return ld32(T.cellCount, cell_idx(T, x, y, z));
-  }
-};
+// mask byte and new and old velocity component of in-grid node (i,j,k)
+__device__ __forceinline__ void node_fetch(const SlTank &T, int comp, int32_t i, int32_t j, int32_t k, uint32_t &m,
+                                           float &vn, float &vo) {
+  const uint32_t n = node_idx(T, i, j, k);
+  m = ld32(T.vmask, n);
+  vn = ld32(T.vel + comp * (size_t)T.nodes, n);
+  vo = ld32(T.velOld + comp * (size_t)T.nodes, n);
+}
+// particle count of an in-grid cell
+__device__ __forceinline__ uint32_t cell_count(const SlTank &T, int32_t x, int32_t y, int32_t z) {
+  return ld32(T.cellCount, cell_idx(T, x, y, z));
+}
 
 // masked PIC/FLIP gather, 31-wgsl-sim2.js:9-36 -> (PIC, FLIP, found). The mask byte and both
 // velocities of all 8 corners are loaded up front (axis indices clamped into the grid, so every load
 // stays inside the arrays); corners outside the grid or masked out are then dropped by select, so
 // the sums take the reference's terms in the reference's order.
-template <class Look>
-__device__ __forceinline__ f3 transfer1(const SlTank &T, const Look &lk, f3 q, int comp, float vp) {
+__device__ __forceinline__ f3 transfer1(const SlTank &T, f3 q, int comp, float vp) {
   const int32_t bx = f2i(floorf(q.x)), by = f2i(floorf(q.y)), bz = f2i(floorf(q.z));
   const float fx = q.x - (float)bx, fy = q.y - (float)by, fz = q.z - (float)bz;
   bool ix[2], iy[2], iz[2];
@@ -47,7 +40,7 @@ __device__ __forceinline__ f3 transfer1(const SlTank &T, const Look &lk, f3 q, i
   uint32_t m[8];
   float a[8], b[8];
 #pragma unroll
-  for (int c = 0; c < 8; c++) lk.fetch(T, comp, cx[c & 1], cy[(c >> 1) & 1], cz[c >> 2], m[c], a[c], b[c]);
+  for (int c = 0; c < 8; c++) node_fetch(T, comp, cx[c & 1], cy[(c >> 1) & 1], cz[c >> 2], m[c], a[c], b[c]);
   const uint32_t bit = 1u << comp;
   float sn = 0.0f, so = 0.0f, sw = 0.0f;
 #pragma unroll
@@ -118,22 +111,19 @@ __device__ __forceinline__ void tank_clamp(const SlTank &T, f3 x0, f3 &x, f3 &v,
   }
 }
 // 31-wgsl-sim2.js:93-109
-template <class Look>
-__device__ __forceinline__ uint32_t count_at(const SlTank &T, const Look &lk, int32_t x, int32_t y, int32_t z) {
+__device__ __forceinline__ uint32_t count_at(const SlTank &T, int32_t x, int32_t y, int32_t z) {
   if (y >= T.ny || x < 0 || y < 0 || z < 0 || x >= T.nx || z >= T.nz) return 0u;
-  return lk.count(T, x, y, z);
+  return cell_count(T, x, y, z);
 }
-template <class Look>
-__device__ __forceinline__ bool air_at(const SlTank &T, const Look &lk, int32_t x, int32_t y, int32_t z) {
+__device__ __forceinline__ bool air_at(const SlTank &T, int32_t x, int32_t y, int32_t z) {
   if (y >= T.ny) return true;
   if (x < 0 || y < 0 || z < 0 || x >= T.nx || z >= T.nz) return false;
-  return lk.count(T, x, y, z) == 0u;
+  return cell_count(T, x, y, z) == 0u;
 }
-template <class Look>
-__device__ __forceinline__ bool near_air(const SlTank &T, const Look &lk, i3 c) {
-  return air_at(T, lk, wadd(c.x, -1), c.y, c.z) || air_at(T, lk, wadd(c.x, 1), c.y, c.z) ||
-         air_at(T, lk, c.x, wadd(c.y, -1), c.z) || air_at(T, lk, c.x, wadd(c.y, 1), c.z) ||
-         air_at(T, lk, c.x, c.y, wadd(c.z, -1)) || air_at(T, lk, c.x, c.y, wadd(c.z, 1));
+__device__ __forceinline__ bool near_air(const SlTank &T, i3 c) {
+  return air_at(T, wadd(c.x, -1), c.y, c.z) || air_at(T, wadd(c.x, 1), c.y, c.z) ||
+         air_at(T, c.x, wadd(c.y, -1), c.z) || air_at(T, c.x, wadd(c.y, 1), c.z) ||
+         air_at(T, c.x, c.y, wadd(c.z, -1)) || air_at(T, c.x, c.y, wadd(c.z, 1));
 }
 // 31-wgsl-sim2.js:110-114
 __device__ __forceinline__ void deposit_ripple(const SlTank &T, float x, float z, float impulse) {
@@ -194,8 +184,7 @@ __device__ __forceinline__ void splash_crown(const SlTank &T, f3 x, float U, flo
 // G2P, 31-wgsl-sim2.js:149-251
 // ------------------------------------------------------------------------------------------------
 // one particle; returns the bit pattern of its speed candidate for vmax (0 when it has none)
-template <class Look>
-__device__ __forceinline__ uint32_t g2p_one(const SlTank &T, const Look &lk, uint32_t id) {
+__device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
   const int cur = T.cur;
   // id < count <= capacity: the state loads go out together with the flags
   const uint32_t pf = ld32(T.pf[cur], id);
@@ -207,9 +196,9 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, const Look &lk, uin
   const f3 g = grid_pos(T, x0);
   const bool ing = in_grid(T, g);
   if (ing) {
-    f3 tx = transfer1(T, lk, sub3(g, F3(0.0f, 0.5f, 0.5f)), 0, v.x);
-    f3 ty = transfer1(T, lk, sub3(g, F3(0.5f, 0.0f, 0.5f)), 1, v.y);
-    f3 tz = transfer1(T, lk, sub3(g, F3(0.5f, 0.5f, 0.0f)), 2, v.z);
+    f3 tx = transfer1(T, sub3(g, F3(0.0f, 0.5f, 0.5f)), 0, v.x);
+    f3 ty = transfer1(T, sub3(g, F3(0.5f, 0.0f, 0.5f)), 1, v.y);
+    f3 tz = transfer1(T, sub3(g, F3(0.5f, 0.5f, 0.0f)), 2, v.z);
     v = F3(mixf(tx.x, tx.y, T.flip), mixf(ty.x, ty.y, T.flip), mixf(tz.x, tz.y, T.flip));
   } else {
     v.y -= T.g * dt;
@@ -233,7 +222,7 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, const Look &lk, uin
   const i3 c0 = toi3(g);
   bool airborne = (pf & 2u) != 0u;
   if (ing) {
-    uint32_t n0 = count_at(T, lk, c0.x, c0.y, c0.z);
+    uint32_t n0 = count_at(T, c0.x, c0.y, c0.z);
     if (n0 >= 4u) airborne = false;
     else if (n0 <= 2u && wadd(c0.y, 2) < T.ny) {
       // 3x3 columns of rows y-1 .. y+2 (rows y+1 and y+2 lie in the grid here, row y-1 when y > 0):
@@ -246,8 +235,8 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, const Look &lk, uin
           const int32_t xx = wadd(c0.x, dx), zz = wadd(c0.z, dz);
           const bool col = xx >= 0 && zz >= 0 && xx < T.nx && zz < T.nz;
           const int32_t xc = min(max(xx, 0), T.nx - 1), zc = min(max(zz, 0), T.nz - 1);
-          const uint32_t lo = lk.count(T, xc, max(c0.y - 1, 0), zc), mid = lk.count(T, xc, c0.y, zc);
-          const uint32_t up1 = lk.count(T, xc, c0.y + 1, zc), up2 = lk.count(T, xc, c0.y + 2, zc);
+          const uint32_t lo = cell_count(T, xc, max(c0.y - 1, 0), zc), mid = cell_count(T, xc, c0.y, zc);
+          const uint32_t up1 = cell_count(T, xc, c0.y + 1, zc), up2 = cell_count(T, xc, c0.y + 2, zc);
           around += col ? (c0.y > 0 ? lo : 0u) + mid + up1 : 0u;
           above += col ? up1 + up2 : 0u;
         }
@@ -257,11 +246,11 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, const Look &lk, uin
   const f3 g1 = grid_pos(T, x);
   const i3 c1 = toi3(g1);
   if (airborne && v0.y < -0.3f && in_grid(T, g1)) {
-    uint32_t others = count_at(T, lk, c1.x, c1.y, c1.z) - ((c1.x == c0.x && c1.y == c0.y && c1.z == c0.z) ? 1u : 0u);
+    uint32_t others = count_at(T, c1.x, c1.y, c1.z) - ((c1.x == c0.x && c1.y == c0.y && c1.z == c0.z) ? 1u : 0u);
     if (others >= 3u) {
-      f3 vl = F3(transfer1(T, lk, sub3(g1, F3(0.0f, 0.5f, 0.5f)), 0, v.x).x,
-                 transfer1(T, lk, sub3(g1, F3(0.5f, 0.0f, 0.5f)), 1, v.y).x,
-                 transfer1(T, lk, sub3(g1, F3(0.5f, 0.5f, 0.0f)), 2, v.z).x);
+      f3 vl = F3(transfer1(T, sub3(g1, F3(0.0f, 0.5f, 0.5f)), 0, v.x).x,
+                 transfer1(T, sub3(g1, F3(0.5f, 0.0f, 0.5f)), 1, v.y).x,
+                 transfer1(T, sub3(g1, F3(0.5f, 0.5f, 0.0f)), 2, v.z).x);
       float dv = vl.y - v0.y;
       if (dv > 0.3f) {
         v = vl;
@@ -269,7 +258,7 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, const Look &lk, uin
         atomicAdd(&T.C->impacts, 1u);
         float m = 998.0f * sl_pow(0.5f * h, 3.0f);
         if (T.R > 0u) deposit_ripple(T, x.x, x.z, 2.0f * m * fminf(dv, 8.0f));
-        float level = T.oy + ((float)c1.y + fminf((float)count_at(T, lk, c1.x, c1.y, c1.z) / T.ppc, 1.0f)) * h;
+        float level = T.oy + ((float)c1.y + fminf((float)count_at(T, c1.x, c1.y, c1.z) / T.ppc, 1.0f)) * h;
         splash_crown(T, F3(x.x, level + 0.25f * h, x.z), -v0.y, m, id);
       }
     }
@@ -277,7 +266,7 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, const Look &lk, uin
   const uint32_t vm = ing ? __float_as_uint(fminf(len3(v), 60.0f)) : 0u;
   if (ing && T.ww_rate > 0.0f) {
     float sp = len3(v);
-    if (sp > 0.75f && near_air(T, lk, c0)) {
+    if (sp > 0.75f && near_air(T, c0)) {
       uint32_t s = hashU(id * 9781u + T.frame * 6271u + 17u);
       float e = clampf((sp - 0.75f) / 1.6f, 0.0f, 1.0f);
       if (rndu(s) < T.ww_rate * e * e * dt) {
@@ -306,7 +295,7 @@ __global__ __launch_bounds__(SL_WG, MINW) void k_g2p(SlWork W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
   uint32_t id = lb * SL_WG + threadIdx.x;
-  uint32_t vm = id < T.C->count ? g2p_one(T, GlobalLook(), id) : 0u;
+  uint32_t vm = id < T.C->count ? g2p_one(T, id) : 0u;
   for (int o = 32; o > 0; o >>= 1) vm = max(vm, (uint32_t)__shfl_xor((int)vm, o, 64));
   __shared__ uint32_t red[SL_WG / 64];
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vm;
@@ -325,154 +314,6 @@ void sl_k_g2p(const SlLaunch &L, int variant) {
     case 2: hipLaunchKernelGGL(k_g2p<6>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w); break;
     default: hipLaunchKernelGGL(k_g2p<8>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w); break;
   }
-}
-
-// ------------------------------------------------------------------------------------------------
-// Cell-tile G2P, with the binned P2G of the same substep (whose cell lists it walks; nothing moves
-// particles between the two). A workgroup takes one BIN_TX x BIN_TY x BIN_TZ tile of cells: it
-// stages the velocity nodes (new, old, mask) that the gathers of its cells reach and the cell
-// counts that the airborne and whitewater tests read into LDS, then walks the tile's particles in
-// cell order, several per thread, so a crowded cell does not idle the other lanes. Lookups that
-// leave the windows (after advection) read global memory. Live particles outside the grid, which
-// have no cell, go through k_g2p_rest. Each particle runs the same body as k_g2p.
-// ------------------------------------------------------------------------------------------------
-#define GW_X (BIN_TX + 2)  // nodes origin .. origin + BIN + 1 per axis, origin = tile's first cell - 1
-#define GW_Y (BIN_TY + 2)
-#define GW_Z (BIN_TZ + 2)
-#define GWIN (GW_X * GW_Y * GW_Z)
-#define GC_X (BIN_TX + 2)  // cells origin .. origin + BIN + 1 per axis, + 1 more above (the 2-cell test)
-#define GC_Y (BIN_TY + 3)
-#define GC_Z (BIN_TZ + 2)
-#define GCWIN (GC_X * GC_Y * GC_Z)
-
-struct TileLook {
-  const float *vn, *vo;  // [3][GWIN]
-  const uint8_t *m;      // [GWIN] velocity mask bits
-  const uint32_t *cnt;   // [GCWIN]
-  int32_t X0, Y0, Z0;    // origin of both windows
-  __device__ __forceinline__ void fetch(const SlTank &T, int comp, int32_t i, int32_t j, int32_t k, uint32_t &mk,
-                                        float &a, float &b) const {
-    const uint32_t x = (uint32_t)(i - X0), y = (uint32_t)(j - Y0), z = (uint32_t)(k - Z0);
-    if (x < GW_X && y < GW_Y && z < GW_Z) {
-      const uint32_t e = (z * GW_Y + y) * GW_X + x;
-      mk = m[e];
-      a = vn[comp * GWIN + e];
-      b = vo[comp * GWIN + e];
-    } else {
-      GlobalLook().fetch(T, comp, i, j, k, mk, a, b);
-    }
-  }
-  __device__ __forceinline__ uint32_t count(const SlTank &T, int32_t cx, int32_t cy, int32_t cz) const {
-    const uint32_t x = (uint32_t)(cx - X0), y = (uint32_t)(cy - Y0), z = (uint32_t)(cz - Z0);
-    if (x < GC_X && y < GC_Y && z < GC_Z) return cnt[(z * GC_Y + y) * GC_X + x];
-    return GlobalLook().count(T, cx, cy, cz);
-  }
-};
-
-template <int MINW>
-__global__ __launch_bounds__(256, MINW) void k_g2p_tiles(SlWork W) {
-  SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
-  __shared__ float svn[3 * GWIN], svo[3 * GWIN];
-  __shared__ uint32_t scnt[GCWIN];
-  __shared__ uint8_t sm[GWIN];
-  __shared__ uint32_t rstart[32], roff[33], red[4];
-  const int ntx = (T.nx + BIN_TX - 1) / BIN_TX, nty = (T.ny + BIN_TY - 1) / BIN_TY;
-  const int tx = (int)(lb % (uint32_t)ntx), ty = (int)((lb / (uint32_t)ntx) % (uint32_t)nty);
-  const int tz = (int)(lb / (uint32_t)(ntx * nty));
-  // rows of BIN_TX cells along x are contiguous in the cell lists: 32 ranges cover the tile
-  const int lx = threadIdx.x & 7, row = threadIdx.x >> 3;
-  const int32_t cx = tx * BIN_TX + lx, cy = ty * BIN_TY + (row & 7), cz = tz * BIN_TZ + (row >> 3);
-  uint32_t n = 0, start = 0;
-  if (cx < T.nx && cy < T.ny && cz < T.nz) {
-    const uint32_t cell = cell_idx(T, cx, cy, cz);
-    n = T.cellCount[cell];
-    if (lx == 0) start = T.cellStart[cell] + T.binTop[cell / BIN_SCAN];
-  }
-  for (int o = 1; o < 8; o <<= 1) n += (uint32_t)__shfl_xor((int)n, o, 64);
-  if (lx == 0) { rstart[row] = start; roff[row] = n; }
-  __syncthreads();
-  if (threadIdx.x < 32) {
-    const uint32_t v = roff[threadIdx.x];
-    uint32_t x = v;
-    for (int o = 1; o < 32; o <<= 1) {
-      const uint32_t y = (uint32_t)__shfl_up((int)x, o, 32);
-      if ((int)threadIdx.x >= o) x += y;
-    }
-    roff[threadIdx.x] = x - v;
-    if (threadIdx.x == 31) roff[32] = x;
-  }
-  __syncthreads();
-  const uint32_t total = roff[32];
-  if (total == 0u) return;  // the same value in every thread
-  const int32_t X0 = tx * BIN_TX - 1, Y0 = ty * BIN_TY - 1, Z0 = tz * BIN_TZ - 1;
-  const size_t N = T.nodes;
-  for (int e = threadIdx.x; e < GWIN; e += 256) {
-    const int z = e / (GW_X * GW_Y), r = e - z * (GW_X * GW_Y), y = r / GW_X, x = r - y * GW_X;
-    const int32_t i = X0 + x, j = Y0 + y, k = Z0 + z;
-    uint8_t m = 0;
-    if (i >= 0 && i <= T.nx && j >= 0 && j <= T.ny && k >= 0 && k <= T.nz) {  // others stay masked out
-      const uint32_t nd = node_idx(T, i, j, k);
-      m = T.vmask[nd];
-#pragma unroll
-      for (int c = 0; c < 3; c++) {
-        svn[c * GWIN + e] = T.vel[c * N + nd];
-        svo[c * GWIN + e] = T.velOld[c * N + nd];
-      }
-    }
-    sm[e] = m;
-  }
-  for (int e = threadIdx.x; e < GCWIN; e += 256) {
-    const int z = e / (GC_X * GC_Y), r = e - z * (GC_X * GC_Y), y = r / GC_X, x = r - y * GC_X;
-    const int32_t i = X0 + x, j = Y0 + y, k = Z0 + z;
-    scnt[e] = (i >= 0 && i < T.nx && j >= 0 && j < T.ny && k >= 0 && k < T.nz) ? T.cellCount[cell_idx(T, i, j, k)] : 0u;
-  }
-  __syncthreads();
-  TileLook lk;
-  lk.vn = svn; lk.vo = svo; lk.m = sm; lk.cnt = scnt;
-  lk.X0 = X0; lk.Y0 = Y0; lk.Z0 = Z0;
-  uint32_t vm = 0;
-  for (uint32_t f = threadIdx.x; f < total; f += 256) {
-    int r = 0;  // the last row whose offset is <= f (roff[32] = total > f)
-#pragma unroll
-    for (int s = 16; s > 0; s >>= 1)
-      if (roff[r + s] <= f) r += s;
-    const uint32_t slot = rstart[r] + (f - roff[r]);
-    if (slot >= T.cap) continue;  // only with cell counts that do not match the lists
-    const uint32_t id = T.perm[slot];
-    if (id < T.cap) vm = max(vm, g2p_one(T, lk, id));
-  }
-  for (int o = 32; o > 0; o >>= 1) vm = max(vm, (uint32_t)__shfl_xor((int)vm, o, 64));
-  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = vm;
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    const uint32_t m = max(max(red[0], red[1]), max(red[2], red[3]));
-    if (m) atomicMax(&T.C->vmax, m);
-  }
-}
-
-// the live particles that the binning gave no cell (outside the grid): no vmax candidate (k_g2p's
-// vmax counts in-grid particles only)
-__global__ __launch_bounds__(SL_WG) void k_g2p_rest(SlWork W) {
-  SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
-  const uint32_t id = lb * SL_WG + threadIdx.x;
-  if (id >= min(T.C->count, T.cap) || T.binCell[id] != 0xffffffffu) return;
-  (void)g2p_one(T, GlobalLook(), id);
-}
-
-// `occ` 0..3: occupancy targets 1, 4, 6 and 8 waves per SIMD
-void sl_k_g2p_tiles(const SlLaunch &tiles, const SlLaunch &parts, int occ) {
-  if (tiles.w.total) {
-    const dim3 g(tiles.w.total), b(256);
-    switch (occ & 3) {
-      case 0: hipLaunchKernelGGL(k_g2p_tiles<1>, g, b, 0, tiles.stream, tiles.w); break;
-      case 1: hipLaunchKernelGGL(k_g2p_tiles<4>, g, b, 0, tiles.stream, tiles.w); break;
-      case 2: hipLaunchKernelGGL(k_g2p_tiles<6>, g, b, 0, tiles.stream, tiles.w); break;
-      default: hipLaunchKernelGGL(k_g2p_tiles<8>, g, b, 0, tiles.stream, tiles.w); break;
-    }
-  }
-  if (parts.w.total) hipLaunchKernelGGL(k_g2p_rest, dim3(parts.w.total), dim3(SL_WG), 0, parts.stream, parts.w);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -677,17 +518,27 @@ __global__ __launch_bounds__(SL_WG) void k_compact_scatter(SlWork W) {
   const uint32_t lim = min(T.C->count, T.cap);
   const uint32_t base = lb * SL_COMPACT_ITEMS;
   if (base >= lim || T.C->newCount == lim) return;  // nothing died: the order is already compact
+  constexpr int Q = SL_COMPACT_ITEMS / SL_WG;
   const int cur = T.cur, oth = 1 - cur;
-  const uint8_t *pf = T.pf[cur];
   __shared__ uint32_t wtot[SL_COMPACT_ITEMS / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  // pass 1: per-wave (64-item chunk) counts
-  bool live[SL_COMPACT_ITEMS / SL_WG];
-  uint64_t masks[SL_COMPACT_ITEMS / SL_WG];
-  for (uint32_t q = 0; q < SL_COMPACT_ITEMS / SL_WG; q++) {
-    uint32_t id = base + q * SL_WG + threadIdx.x;
-    live[q] = id < lim && (pf[id] & 1u);
-    masks[q] = __ballot(live[q]);
+  // every in-range particle's state is loaded before the ranks are known (the few dead ones are
+  // read for nothing), so a workgroup has all its loads in flight at once
+  uint8_t f[Q];
+  float r[Q][6];
+  uint64_t masks[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const uint32_t id = base + q * SL_WG + threadIdx.x;
+    const bool in = id < lim;
+    f[q] = in ? ld32(T.pf[cur], id) : (uint8_t)0;
+    const uint32_t k = in ? id : base;
+    r[q][0] = ld32(T.px[cur], k); r[q][1] = ld32(T.py[cur], k); r[q][2] = ld32(T.pz[cur], k);
+    r[q][3] = ld32(T.vx[cur], k); r[q][4] = ld32(T.vy[cur], k); r[q][5] = ld32(T.vz[cur], k);
+  }
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    masks[q] = __ballot((f[q] & 1u) != 0u);
     if (lane == 0) wtot[q * (SL_WG / 64) + wv] = (uint32_t)__popcll(masks[q]);
   }
   __syncthreads();
@@ -697,14 +548,14 @@ __global__ __launch_bounds__(SL_WG) void k_compact_scatter(SlWork W) {
   }
   __syncthreads();
   const uint32_t off0 = T.scanBlocks[lb];
-  for (uint32_t q = 0; q < SL_COMPACT_ITEMS / SL_WG; q++) {
-    if (!live[q]) continue;
-    uint32_t id = base + q * SL_WG + threadIdx.x;
-    uint64_t below = masks[q] & ((lane == 0) ? 0ull : (~0ull >> (64 - lane)));
-    uint32_t d = off0 + wtot[q * (SL_WG / 64) + wv] + (uint32_t)__popcll(below);
-    T.px[oth][d] = T.px[cur][id]; T.py[oth][d] = T.py[cur][id]; T.pz[oth][d] = T.pz[cur][id];
-    T.vx[oth][d] = T.vx[cur][id]; T.vy[oth][d] = T.vy[cur][id]; T.vz[oth][d] = T.vz[cur][id];
-    T.pf[oth][d] = pf[id];
+  const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    if (!(f[q] & 1u)) continue;
+    const uint32_t d = off0 + wtot[q * (SL_WG / 64) + wv] + (uint32_t)__popcll(masks[q] & below);
+    st32(T.px[oth], d, r[q][0]); st32(T.py[oth], d, r[q][1]); st32(T.pz[oth], d, r[q][2]);
+    st32(T.vx[oth], d, r[q][3]); st32(T.vy[oth], d, r[q][4]); st32(T.vz[oth], d, r[q][5]);
+    st32(T.pf[oth], d, f[q]);
   }
 }
 void sl_k_compact_scatter(const SlLaunch &L) {

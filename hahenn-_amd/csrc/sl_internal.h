@@ -315,7 +315,7 @@ int sl_p2g_chunk(int variant);
 // each), tiles (one workgroup each)
 #define SL_P2G_BINNED 8
 #define BIN_SCAN 4096  // cells per scan block (the block prefix lives in binTop)
-#define BIN_TX 8       // cell tile of the binned P2G and the tile G2P
+#define BIN_TX 8       // cell tile of the binned P2G
 #define BIN_TY 8
 #define BIN_TZ 4
 uint32_t sl_bin_tiles(int nx, int ny, int nz);
@@ -327,13 +327,6 @@ void sl_k_sor(const SlLaunch &L, uint32_t parity, uint32_t pass);
 void sl_k_project(const SlLaunch &L);
 // particle stages
 void sl_k_g2p(const SlLaunch &L, int variant);
-// G2P over the cell lists of the binned P2G of the same substep: tiles (one workgroup each) and
-// particles (SL_WG per workgroup; the live particles without a cell). G2P variants 0..3 are the
-// particle-parallel kernel, SL_G2P_TILES..SL_G2P_LAST the tile kernel, each at four occupancy
-// targets (variant & 3).
-#define SL_G2P_TILES 4
-#define SL_G2P_LAST 7
-void sl_k_g2p_tiles(const SlLaunch &tiles, const SlLaunch &parts, int occ);
 void sl_k_emit(const SlLaunch &L, uint32_t reserve = 0);
 void sl_k_count(const SlLaunch &L, uint32_t mode, uint32_t y);
 void sl_k_vmax_reset(const SlLaunch &L);

@@ -174,16 +174,16 @@ def test_batch_autotune_keeps_results(singles):
     b.close()
 
 
-def test_batch_tile_g2p_equals_single(singles):
-    """Binned P2G with the tile G2P (variants 4-7) in every size bucket, eager then replayed from a
-    captured graph: each tank equals its single run with the default kernels. A tile G2P without the
-    binned P2G is refused."""
-    print("batch tile G2P, 4 tanks, 8 frames", flush=True)
-    b = run_batch(8, capture_at=3, variants=[(8, 4), (8, 7), (8, 5)])
+def test_batch_binned_p2g_equals_single(singles):
+    """The binned P2G in every size bucket (its cell lists and tiles over the batch's work maps),
+    eager then replayed from a captured graph: each tank equals its single run with the default
+    kernels."""
+    print("batch binned P2G, 4 tanks, 8 frames", flush=True)
+    b = run_batch(8, capture_at=3, variants=[(8, 0), (8, 3), (8, 1)])
     for i, (t, s) in enumerate(zip(b.tanks, singles)):
         assert_same_tank(t, s, f"tank {i}")
-    with pytest.raises(suiri.SuiriError, match="binned"):
-        b.set_variants([(8, 4), (3, 6), (8, 4)])
+    with pytest.raises(suiri.SuiriError, match="out of range"):
+        b.set_variants([(8, 4), (3, 2), (8, 0)])
     b.close()
 
 

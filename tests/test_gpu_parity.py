@@ -194,8 +194,7 @@ def test_autotune_keeps_results_bitwise():
         twin.oracle_step(o, 1 / 120)
         twin.gpu_step(g, 1 / 120)
     ch = g.autotune(reps=1)
-    assert 0 <= ch["p2g_variant"] <= 8 and 0 <= ch["g2p_variant"] <= 7
-    assert ch["g2p_variant"] < 4 or ch["p2g_variant"] == 8   # tile G2P only with the binned P2G
+    assert 0 <= ch["p2g_variant"] <= 8 and 0 <= ch["g2p_variant"] < 4
     for v in range(9):   # and every P2G variant on its own (8: binned)
         g.set_scalar("p2g_variant", v)
         twin.oracle_step(o, 1 / 120)
@@ -205,13 +204,11 @@ def test_autotune_keeps_results_bitwise():
     assert np.array_equal(g.readback(suiri.F_PARTICLES)[:n], o.field("parts")[:n])
 
 
-def test_g2p_tile_variants_bitwise():
-    """The tile G2P (variants 4-7, with the binned P2G) reads nodes and cell counts from LDS windows
-    and walks the cell lists; it must give the oracle's particles bit for bit, frame after frame, with
-    a stream source pouring in from above the grid (particles outside the grid, which the tiles leave
-    to the remainder pass, and crowded cells under the stream), whitewater spawns and splash crowns
-    (same multiset; their slot order follows atomics), ripple impulses and counters. A tile variant
-    without the binned P2G runs the particle-parallel kernel."""
+def test_g2p_variants_with_stream_source_bitwise():
+    """Every G2P variant (occupancy targets) after the binned and a chunked P2G gives the oracle's
+    particles bit for bit, frame after frame, with a stream source pouring in from above the grid
+    (particles outside the grid and crowded cells under the stream), whitewater spawns and splash
+    crowns (same multiset: their slot order follows atomics), ripple impulses and counters."""
     sc = twin.small_scene(whitewater=True)
     o, g = twin.make_twin(sc)
     inf = o.info()
@@ -219,7 +216,7 @@ def test_g2p_tile_variants_bitwise():
     src = scene.make_source(pos=[inf["ox"] + 0.3, top + 0.03, inf["oz"] + 0.12], mode="stream", flow=120.0,
                             speed=1.2, pitch=-90, yaw=0)
     em = scene.Emitters([src], float(inf["vp"]), seed=9)
-    plan = [(8, 4), (8, 5), (8, 6), (8, 7), (3, 6), (8, 4)] * 3
+    plan = [(8, 0), (8, 1), (8, 2), (8, 3), (3, 2), (5, 3)] * 3
     outside = 0
     for f, (pv, gv) in enumerate(plan):
         g.set_scalar("p2g_variant", pv)
