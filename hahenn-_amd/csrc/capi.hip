@@ -210,6 +210,16 @@ static SlLaunch launch_of(sl_sim *s, uint64_t items, uint32_t per_wg = SL_WG) {
   return L;
 }
 
+// multiplier and shifts for udiv_mag by d (1 <= d < 2^31): l = ceil(log2 d),
+// m = floor(2^32 (2^l - d) / d) + 1, s1 = min(l, 1), s2 = max(l - 1, 0)
+static void sl_magic_u32(uint32_t d, uint32_t g[3]) {
+  uint32_t l = 0;
+  while ((1ull << l) < d) l++;
+  g[0] = (uint32_t)(((1ull << 32) * ((1ull << l) - d)) / d + 1);
+  g[1] = l < 1 ? l : 1;
+  g[2] = l > 0 ? l - 1 : 0;
+}
+
 static int push_dyn(sl_sim *s, uint32_t emit_k = 0, uint32_t emit_total = 0) {
   SlTankDyn d;
   d.dt = s->ht.dt; d.gdt = s->ht.gdt; d.time_f = s->ht.time_f;
@@ -344,7 +354,14 @@ static int tank_buffers(sl_sim *s, SlTank &T, Take take) {
   TAKE(T.vel, N * 12); TAKE(T.velOld, N * 12); TAKE(T.vmask, N);
   TAKE(T.frac, N * 12); TAKE(T.svel, N * 12);
   TAKE(T.divg, C * 4);
-  TAKE(T.pres, H * 8); TAKE(T.rhs, H * 8); TAKE(T.sflag, H * 4);
+  {
+    // both colour arrays with hw*ny + 1 guard slots before and after: every SOR neighbour slot of
+    // an in-range slot lies in the allocation, so the kernel needs no clamps
+    const size_t G = (size_t)s->hw * (size_t)s->ny + 1;
+    TAKE(T.pres, (H * 2 + 2 * G) * 4);
+    if (T.pres) T.pres += G;
+  }
+  TAKE(T.rhs, H * 8); TAKE(T.sflag, H * 4);
   TAKE(T.objs, sizeof(SlObj) * SL_MAX_OBJ);
   if (s->ext_emits) T.emits = s->ext_emits;
   else TAKE(T.emits, sizeof(SlEmitDev) * SL_MAX_EMIT);
@@ -398,6 +415,12 @@ static int create_impl(sl_sim *s) {
   SlTank &T = s->ht;
   memset(&T, 0, sizeof T);
   T.nx = s->nx; T.ny = s->ny; T.nz = s->nz; T.hw = (int32_t)s->hw;
+  sl_magic_u32(s->hw, T.hw_mag);
+  sl_magic_u32(s->hw * (uint32_t)s->ny, T.rowl_mag);
+  sl_magic_u32((uint32_t)s->nx, T.nx_mag);
+  sl_magic_u32((uint32_t)(s->nx * s->ny), T.nxy_mag);
+  sl_magic_u32((uint32_t)(s->nx + 1), T.nx1_mag);
+  sl_magic_u32((uint32_t)((s->nx + 1) * (s->ny + 1)), T.nxy1_mag);
   T.nodes = s->nodes; T.cells = s->cells; T.hcells = s->hcells;
   T.cap = s->cap; T.ww_cap = s->ww_cap; T.R = s->R; T.M = s->M; T.iters = s->iters;
   uint32_t lm = 0;

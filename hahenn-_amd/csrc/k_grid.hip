@@ -651,8 +651,8 @@ __global__ __launch_bounds__(SL_WG) void k_divsetup(SlWork W) {
   uint32_t c = lb * SL_WG + threadIdx.x;
   if (c >= T.cells) return;
   const int32_t nx = T.nx, ny = T.ny, nz = T.nz;
-  int32_t i = (int32_t)(c % (uint32_t)nx), j = (int32_t)((c / (uint32_t)nx) % (uint32_t)ny);
-  int32_t k = (int32_t)(c / (uint32_t)(nx * ny));
+  const uint32_t kq = udiv_mag(c, T.nxy_mag), rq = c - kq * (uint32_t)(nx * ny), jq = udiv_mag(rq, T.nx_mag);
+  const int32_t i = (int32_t)(rq - jq * (uint32_t)nx), j = (int32_t)jq, k = (int32_t)kq;
   const uint32_t slot = colour_base(T, i, j, k) + packed_idx(T, i, j, k);
   const uint32_t cnt = T.cellCount[c];
   if (cnt == 0u) {
@@ -707,25 +707,26 @@ __global__ __launch_bounds__(SL_WG) void k_sor(SlWork W, uint32_t parity, uint32
   SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
   const SlTank &T = W.tanks[t];
   if (pass >= 2u * T.iters) return;
-  uint32_t s = lb * SL_WG + threadIdx.x;
-  if (s >= T.hcells) return;
-  const uint32_t hw = (uint32_t)T.hw, rowl = hw * (uint32_t)T.ny;
-  uint32_t k = s / rowl, rem = s - k * rowl, j = rem / hw, m = rem - j * hw;
-  uint32_t r = (parity + j + k) & 1u;
-  int32_t i = (int32_t)(2u * m + r);
-  if (i >= T.nx) return;
-  const uint32_t me = parity * T.hcells + s;
-  const uint32_t fl = T.sflag[me];
-  if (!(fl & SF_FLUID)) return;
-  const float *po = T.pres + (1u - parity) * T.hcells;
+  const uint32_t s = lb * SL_WG + threadIdx.x;
   const uint32_t H = T.hcells;
-  // neighbour slots (clamped into the array; unused ones are masked by the flags)
-  uint32_t sxm = s + r - 1u, sxp = s + r;
-  uint32_t sym = s - hw, syp = s + hw, szm = s - rowl, szp = s + rowl;
-  sxm = sxm < H ? sxm : s; sxp = sxp < H ? sxp : s;
-  sym = sym < H ? sym : s; syp = syp < H ? syp : s;
-  szm = szm < H ? szm : s; szp = szp < H ? szp : s;
-  const float p_n[6] = {po[sxm], po[sxp], po[sym], po[syp], po[szm], po[szp]};
+  if (s >= H) return;
+  const uint32_t hw = (uint32_t)T.hw, rowl = hw * (uint32_t)T.ny;
+  const uint32_t k = udiv_mag(s, T.rowl_mag), rem = s - k * rowl, j = udiv_mag(rem, T.hw_mag), m = rem - j * hw;
+  const uint32_t r = (parity + j + k) & 1u;
+  const int32_t i = (int32_t)(2u * m + r);
+  if (i >= T.nx) return;
+  const uint16_t *sf = T.sflag + parity * H;
+  const uint32_t fl = ld32(sf, s);
+  if (!(fl & SF_FLUID)) return;
+  float *pm = T.pres + parity * H;
+  // neighbours in the other colour's array, read from a base rowl + 1 slots lower: the pressure
+  // allocation has that many guard slots on both sides, so every read is inside it (slots that do
+  // not contribute are masked by the flags)
+  const uint32_t G = rowl + 1u, g = s + G;
+  const float *pb = T.pres + (1u - parity) * H - G;
+  const float p_n[6] = {ld32(pb, g + r - 1u), ld32(pb, g + r), ld32(pb, g - hw),
+                        ld32(pb, g + hw), ld32(pb, g - rowl), ld32(pb, g + rowl)};
+  const float b = ld32(T.rhs + parity * H, s), p = ld32((const float *)pm, s);
   float num = 0.0f, den = 0.0f;
   if (!(fl & SF_SLOW)) {
 #pragma unroll
@@ -734,8 +735,7 @@ __global__ __launch_bounds__(SL_WG) void k_sor(SlWork W, uint32_t parity, uint32
     den = (float)__builtin_popcount(fl & 0x3fu);
   } else {
     const size_t N = T.nodes;
-    const int32_t jj = (int32_t)j, kk = (int32_t)k;
-    uint32_t n0 = node_idx(T, i, jj, kk);
+    const uint32_t n0 = node_idx(T, i, (int32_t)j, (int32_t)k);
     const float f[6] = {T.frac[n0], T.frac[n0 + 1], T.frac[N + n0], T.frac[N + n0 + (uint32_t)(T.nx + 1)],
                         T.frac[2 * N + n0], T.frac[2 * N + n0 + (uint32_t)((T.nx + 1) * (T.ny + 1))]};
 #pragma unroll
@@ -746,8 +746,8 @@ __global__ __launch_bounds__(SL_WG) void k_sor(SlWork W, uint32_t parity, uint32
     }
   }
   if (den > 0.0f) {
-    float pn = (num - T.rhs[me]) / den;
-    T.pres[me] = T.pres[me] * T.omega1 + pn * T.omega;
+    const float pn = (num - b) / den;
+    st32(pm, s, p * T.omega1 + pn * T.omega);
   }
 }
 void sl_k_sor(const SlLaunch &L, uint32_t parity, uint32_t pass) {
@@ -772,8 +772,9 @@ __global__ __launch_bounds__(SL_WG) void k_project(SlWork W) {
   const SlTank &T = W.tanks[t];
   uint32_t n = lb * SL_WG + threadIdx.x;
   if (n >= T.nodes) return;
-  uint32_t sxn = T.nx + 1, syn = T.ny + 1;
-  int32_t cx = (int32_t)(n % sxn), cy = (int32_t)((n / sxn) % syn), cz = (int32_t)(n / (sxn * syn));
+  const uint32_t sxn = T.nx + 1, syn = T.ny + 1;
+  const uint32_t kq = udiv_mag(n, T.nxy1_mag), rq = n - kq * (sxn * syn), jq = udiv_mag(rq, T.nx1_mag);
+  const int32_t cx = (int32_t)(rq - jq * sxn), cy = (int32_t)jq, cz = (int32_t)kq;
   const size_t N = T.nodes;
   const float ih = T.invh;
   const uint32_t t0 = ctype(T, cx, cy, cz);

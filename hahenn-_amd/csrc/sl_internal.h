@@ -39,6 +39,8 @@ struct SlTank {
   uint32_t nodes, cells, hcells;  // hcells = hw*ny*nz
   uint32_t cap, ww_cap, R, M, logM, iters, obj_count, frame, emit_k, emit_total, cur, ww_cur, wmap;
   int32_t mask_bound, whitewater_on, brick_bits, top_height;
+  // division (udiv_mag) by hw, hw*ny, nx, nx*ny, nx+1, (nx+1)*(ny+1)
+  uint32_t hw_mag[3], rowl_mag[3], nx_mag[3], nxy_mag[3], nx1_mag[3], nxy1_mag[3];
   float tankH;  // scene tank height (the renderer's F.tank.y)
   // params as the reference uniform (f32)
   float ox, oy, oz, h, invh, hh, dt, g, gdt, flip, ppc, drift, omega, omega1, rc, rim;
@@ -140,6 +142,13 @@ __device__ __forceinline__ E ld32(const E *base, uint32_t i) {
 template <class E>
 __device__ __forceinline__ void st32(E *base, uint32_t i, E v) {
   *(E *)((char *)base + i * (uint32_t)sizeof(E)) = v;
+}
+// n / d from the triple (m, s1, s2) that sl_magic_u32 computes for d: Granlund & Montgomery,
+// "Division by Invariant Integers using Multiplication" (PLDI 1994), fig. 4.1, exact for every
+// 32-bit n and 1 <= d < 2^31
+__device__ __forceinline__ uint32_t udiv_mag(uint32_t n, const uint32_t g[3]) {
+  const uint32_t t = __umulhi(g[0], n);
+  return (t + ((n - t) >> g[1])) >> g[2];
 }
 __device__ __forceinline__ uint32_t node_idx(const SlTank &T, int32_t i, int32_t j, int32_t k) {
   return (uint32_t)(i + (j + k * (T.ny + 1)) * (T.nx + 1));
