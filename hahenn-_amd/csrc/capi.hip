@@ -380,13 +380,11 @@ static int tank_buffers(sl_sim *s, SlTank &T, Take take) {
     TAKE(T.topH, wm * wm * 4);
     if (s->cfg.top_height) TAKE(T.topB, wm * wm * 4);
   }
-  // brick-Morton keys: 3 bits per level of the largest brick coordinate, + 1 for the outside key
+  // re-ordering keys: cell indices 0 .. C-1 and C for particles outside the grid
   {
-    uint32_t mb = (uint32_t)std::max(std::max((s->nx + 3) / 4, (s->ny + 3) / 4), (s->nz + 3) / 4);
     int bits = 0;
-    while ((1u << bits) < mb) bits++;
-    T.brick_bits = bits;
-    s->sort_end_bit = 3 * bits + 1;
+    while ((1ull << bits) <= (uint64_t)s->cells) bits++;
+    s->sort_end_bit = bits;
     TAKE(s->sort_keys, (size_t)s->cap * 8);
     TAKE(s->sort_vals, (size_t)s->cap * 8);
     s->sort_temp_bytes = sl_sort_temp_bytes(s->cap, s->sort_end_bit);

@@ -1,9 +1,11 @@
-// Spatial re-ordering of the particle arrays: a stable sort of the live particles by the Morton
-// code of the 4x4x4-cell brick that holds them (particles outside the grid last, in their order).
+// Spatial re-ordering of the particle arrays: a stable sort of the live particles by the index of
+// the cell that holds them, x fastest (particles outside the grid last, in their order). The
+// particles of one cell end up next to each other and a row of cells is one contiguous range: the
+// binning of P2G then takes a cell's ranks with one atomic per run of lanes, a P2G tile reads its
+// cell rows as contiguous ranges, and the lanes of a G2P wavefront gather from nodes along x.
 //
 // The reference appends survivors with an atomic counter (31-wgsl-sim2.js:317-328), so its particle
-// order is arbitrary; any permutation is a valid state. A brick-Morton order keeps the particles of
-// one workgroup in a compact block of cells, which P2G's LDS privatisation and G2P's gathers need.
+// order is arbitrary; any permutation is a valid state.
 #include <string.h>
 #include <hip/hip_runtime.h>
 #include <rocprim/rocprim.hpp>
@@ -19,13 +21,13 @@ __global__ __launch_bounds__(SL_WG) void k_sort_keys(SlWork W, uint32_t *keys, u
   const SlTank &T = W.tanks[t];
   uint32_t id = lb * SL_WG + threadIdx.x;
   if (id >= n_ub) return;
-  uint32_t key = sl_brick_sentinel(T);
+  uint32_t key = sl_key_sentinel(T);
   if (id < T.C->count) {
     const int cur = T.cur;
     f3 g = grid_pos(T, F3(T.px[cur][id], T.py[cur][id], T.pz[cur][id]));
     if (in_grid(T, g)) {
       i3 c = toi3(g);
-      key = sl_morton3((uint32_t)c.x >> 2, (uint32_t)c.y >> 2, (uint32_t)c.z >> 2);
+      key = cell_idx(T, c.x, c.y, c.z);
     }
   }
   keys[id] = key;

@@ -38,7 +38,7 @@ struct SlTank {
   int32_t nx, ny, nz, hw;
   uint32_t nodes, cells, hcells;  // hcells = hw*ny*nz
   uint32_t cap, ww_cap, R, M, logM, iters, obj_count, frame, emit_k, emit_total, cur, ww_cur, wmap;
-  int32_t mask_bound, whitewater_on, brick_bits, top_height;
+  int32_t mask_bound, whitewater_on, top_height;
   // division (udiv_mag) by hw, hw*ny, nx, nx*ny, nx+1, (nx+1)*(ny+1)
   uint32_t hw_mag[3], rowl_mag[3], nx_mag[3], nxy_mag[3], nx1_mag[3], nxy1_mag[3];
   float tankH;  // scene tank height (the renderer's F.tank.y)
@@ -171,20 +171,8 @@ __device__ __forceinline__ bool in_grid(const SlTank &T, f3 g) {
          g.z < (float)T.nz;
 }
 
-// Morton code (x in bit 0, y in bit 1, z in bit 2) of brick coordinates (< 1024 each)
-__device__ __forceinline__ uint32_t sl_spread3(uint32_t v) {
-  v &= 0x3ffu;
-  v = (v | (v << 16)) & 0x030000ffu;
-  v = (v | (v << 8)) & 0x0300f00fu;
-  v = (v | (v << 4)) & 0x030c30c3u;
-  v = (v | (v << 2)) & 0x09249249u;
-  return v;
-}
-__device__ __forceinline__ uint32_t sl_morton3(uint32_t x, uint32_t y, uint32_t z) {
-  return sl_spread3(x) | (sl_spread3(y) << 1) | (sl_spread3(z) << 2);
-}
-// key of particles outside the grid: above every brick key
-__device__ __forceinline__ uint32_t sl_brick_sentinel(const SlTank &T) { return 1u << (3 * T.brick_bits); }
+// key of particles outside the grid: above every cell index
+__device__ __forceinline__ uint32_t sl_key_sentinel(const SlTank &T) { return T.cells; }
 
 // Append with one atomic per wavefront: the lanes with `keep` get consecutive slots in lane order
 // after the counter's old value (the reference appends with one atomicAdd per item; the order of
@@ -344,7 +332,7 @@ void sl_k_compact_scan(const SlLaunch &L, int ntanks);
 void sl_k_compact_scatter(const SlLaunch &L);
 void sl_k_ww(const SlLaunch &L, float dt);
 void sl_k_clear_i32(const SlLaunch &L, int which);
-// spatial re-ordering (rocPRIM radix sort of brick-Morton keys)
+// spatial re-ordering (rocPRIM radix sort of cell-index keys)
 size_t sl_sort_temp_bytes(uint32_t n, int end_bit);
 int sl_k_reorder(const SlLaunch &L, uint32_t n_ub, uint32_t *keys, uint32_t *vals, void *temp, size_t temp_bytes,
                  int end_bit);

@@ -755,22 +755,12 @@ void or_compact(or_sim *s) {
   s->C.newCount = j;
 }
 
-/* spatial re-ordering: stable counting sort by brick-Morton key */
-static uint32_t spread3(uint32_t v) {
-  uint32_t r = 0;
-  for (int b = 0; b < 10; b++) r |= ((v >> b) & 1u) << (3 * b);
-  return r;
-}
+/* spatial re-ordering: stable counting sort by cell index (x fastest), outside the grid last */
 void or_reorder(or_sim *s) {
   const uint32_t n = s->C.count;
   const or_particle *src = s->parts[s->cur];
   or_particle *dst = s->parts[1 - s->cur];
-  int a = (s->nx + 3) / 4, b = (s->ny + 3) / 4, c = (s->nz + 3) / 4;
-  int mb = a > b ? a : b;
-  mb = mb > c ? mb : c;
-  int bits = 0;
-  while ((1 << bits) < mb) bits++;
-  const uint32_t sentinel = 1u << (3 * bits);
+  const uint32_t sentinel = (uint32_t)s->nx * (uint32_t)s->ny * (uint32_t)s->nz;
   uint32_t *key = (uint32_t *)malloc((size_t)(n ? n : 1) * 4);
   uint32_t *hist = (uint32_t *)calloc((size_t)sentinel + 2, 4);
   for (uint32_t i = 0; i < n; i++) {
@@ -778,7 +768,7 @@ void or_reorder(or_sim *s) {
     uint32_t k = sentinel;
     if (inGrid(s, g)) {
       v3i q = v2i(g);
-      k = spread3((uint32_t)q.x >> 2) | (spread3((uint32_t)q.y >> 2) << 1) | (spread3((uint32_t)q.z >> 2) << 2);
+      k = (uint32_t)q.x + (uint32_t)s->nx * ((uint32_t)q.y + (uint32_t)s->ny * (uint32_t)q.z);
     }
     key[i] = k;
     hist[k + 1]++;

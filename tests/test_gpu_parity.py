@@ -162,7 +162,7 @@ def test_stage_by_stage_from_identical_state():
 
 
 def test_reorder_is_bitwise():
-    """The spatial re-ordering (stable sort by brick-Morton key) produces the oracle's permutation."""
+    """The spatial re-ordering (stable sort by cell index) produces the oracle's permutation."""
     o, g = twin.make_twin(twin.small_scene())
     for _ in range(3):
         twin.oracle_step(o, 1 / 120)
