@@ -439,39 +439,59 @@ __global__ __launch_bounds__(SL_WG) void k_bin_scatter(SlWork W) {
 // to [-30, 30] and whose weight lies in [0, 1], so f2i's NaN and range cases cannot occur
 __device__ __forceinline__ int32_t fix_term(float x) { return (int32_t)rintf(x); }
 
-// One cell's particles, the reference's splat (30-wgsl-sim.js:159-200) into the LDS node window
-// of the workgroup's tile: 8 corners per component with the reference's weight products and
-// rounding. One thread walks one cell, so the lanes of a wave add into different cells' corners.
-__device__ __forceinline__ void p2g_cell(const SlTank &T, int cur, uint32_t start, uint32_t n, int32_t X0, int32_t Y0,
-                                         int32_t Z0, int32_t *wm, int32_t *ww) {
-  for (uint32_t p = 0; p < n; p++) {
-    const uint32_t id = T.perm[start + p];
-    const f3 g = grid_pos(T, F3(T.px[cur][id], T.py[cur][id], T.pz[cur][id]));
-    const float v3[3] = {clampf(T.vx[cur][id], -30.0f, 30.0f), clampf(T.vy[cur][id], -30.0f, 30.0f),
-                         clampf(T.vz[cur][id], -30.0f, 30.0f)};
+// one particle's splat into the LDS window
+__device__ __forceinline__ void p2g_splat(f3 g, const float v3[3], int32_t X0, int32_t Y0, int32_t Z0, int32_t *wm,
+                                          int32_t *ww) {
 #pragma unroll
-    for (int comp = 0; comp < 3; comp++) {
-      const f3 q = sub3(g, comp == 0 ? F3(0.0f, 0.5f, 0.5f) : (comp == 1 ? F3(0.5f, 0.0f, 0.5f) : F3(0.5f, 0.5f, 0.0f)));
-      const float val = v3[comp];
-      const int32_t bx = f2i(floorf(q.x)), by = f2i(floorf(q.y)), bz = f2i(floorf(q.z));
-      const float fx = q.x - (float)bx, fy = q.y - (float)by, fz = q.z - (float)bz;
-      const int e0 = ((bz - Z0) * WIN_Y + (by - Y0)) * WIN_X + (bx - X0) + comp * WIN;
+  for (int comp = 0; comp < 3; comp++) {
+    const f3 q = sub3(g, comp == 0 ? F3(0.0f, 0.5f, 0.5f) : (comp == 1 ? F3(0.5f, 0.0f, 0.5f) : F3(0.5f, 0.5f, 0.0f)));
+    const float val = v3[comp];
+    const int32_t bx = f2i(floorf(q.x)), by = f2i(floorf(q.y)), bz = f2i(floorf(q.z));
+    const float fx = q.x - (float)bx, fy = q.y - (float)by, fz = q.z - (float)bz;
+    const int e0 = ((bz - Z0) * WIN_Y + (by - Y0)) * WIN_X + (bx - X0) + comp * WIN;
 #pragma unroll
-      for (int dz = 0; dz < 2; dz++) {
-        const float wz = dz == 1 ? fz : 1.0f - fz;
+    for (int dz = 0; dz < 2; dz++) {
+      const float wz = dz == 1 ? fz : 1.0f - fz;
 #pragma unroll
-        for (int dy = 0; dy < 2; dy++) {
-          const float wy = dy == 1 ? fy : 1.0f - fy;
+      for (int dy = 0; dy < 2; dy++) {
+        const float wy = dy == 1 ? fy : 1.0f - fy;
 #pragma unroll
-          for (int dx = 0; dx < 2; dx++) {
-            const float w = (dx == 1 ? fx : 1.0f - fx) * wy * wz;
-            const int e = e0 + (dz * WIN_Y + dy) * WIN_X + dx;
-            atomicAdd(&wm[e], fix_term(w * val * VSCALE));
-            atomicAdd(&ww[e], fix_term(w * VSCALE));
-          }
+        for (int dx = 0; dx < 2; dx++) {
+          const float w = (dx == 1 ? fx : 1.0f - fx) * wy * wz;
+          const int e = e0 + (dz * WIN_Y + dy) * WIN_X + dx;
+          atomicAdd(&wm[e], fix_term(w * val * VSCALE));
+          atomicAdd(&ww[e], fix_term(w * VSCALE));
         }
       }
     }
+  }
+}
+
+// One cell's particles, the reference's splat (30-wgsl-sim.js:159-200) into the LDS node window
+// of the workgroup's tile: 8 corners per component with the reference's weight products and
+// rounding. One thread walks one cell, so the lanes of a wave add into different cells' corners.
+// Particles are read P2G_BATCH at a time (indices clamped to the cell's last particle), so their
+// loads are in flight together.
+#define P2G_BATCH 4
+__device__ __forceinline__ void p2g_cell(const SlTank &T, int cur, uint32_t start, uint32_t n, int32_t X0, int32_t Y0,
+                                         int32_t Z0, int32_t *wm, int32_t *ww) {
+  for (uint32_t p0 = 0; p0 < n; p0 += P2G_BATCH) {
+    uint32_t ids[P2G_BATCH];
+#pragma unroll
+    for (int u = 0; u < P2G_BATCH; u++) ids[u] = ld32(T.perm, start + min(p0 + (uint32_t)u, n - 1u));
+    f3 g[P2G_BATCH];
+    float v3[P2G_BATCH][3];
+#pragma unroll
+    for (int u = 0; u < P2G_BATCH; u++) {
+      const uint32_t id = ids[u];
+      g[u] = grid_pos(T, F3(ld32(T.px[cur], id), ld32(T.py[cur], id), ld32(T.pz[cur], id)));
+      v3[u][0] = clampf(ld32(T.vx[cur], id), -30.0f, 30.0f);
+      v3[u][1] = clampf(ld32(T.vy[cur], id), -30.0f, 30.0f);
+      v3[u][2] = clampf(ld32(T.vz[cur], id), -30.0f, 30.0f);
+    }
+#pragma unroll
+    for (int u = 0; u < P2G_BATCH; u++)
+      if (p0 + (uint32_t)u < n) p2g_splat(g[u], v3[u], X0, Y0, Z0, wm, ww);
   }
 }
 
