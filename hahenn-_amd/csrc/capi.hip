@@ -703,7 +703,7 @@ static int encode_ripple(sl_sim *s, double frame_dt, double mean_depth) {
   const uint32_t M = s->M, R1 = s->R + 1;
   uint32_t CW = 4096 / M; if (CW > M) CW = M; if (CW < 1) CW = 1;
   sl_k_ripple_frame_ex(launch_of(s, M, 1).w, launch_of(s, M, CW).w, launch_of(s, (uint64_t)M * M).w,
-                       launch_of(s, (uint64_t)R1 * R1).w, s->stream, s->mask_bound);
+                       launch_of(s, (uint64_t)R1 * R1).w, s->stream, s->mask_bound, M);
   return check_launch(s, "ripple");
 }
 
@@ -1433,6 +1433,7 @@ struct sl_batch {
   bool any_toph = false;
   uint32_t sor_bands = 0, band_start[SL_SOR_BANDS + 1] = {};
   bool any_ripple = false, maps_dirty = true;
+  uint32_t max_M = 0;  // largest ripple M of the stepped tanks
   int p2g_variant[SL_BUCKETS] = {0, 0, 0}, g2p_variant[SL_BUCKETS] = {1, 1, 1};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
@@ -1565,11 +1566,13 @@ static int batch_rebuild(sl_batch *b) {
   b->n_stepped = 0;
   b->max_iters = 0;
   b->any_ripple = false;
+  b->max_M = 0;
   for (uint32_t t = 0; t < b->cap; t++) {
     if (!live(b, t)) continue;
     list[b->n_stepped++] = t;
     b->max_iters = std::max(b->max_iters, b->sims[t]->iters);
     if (b->sims[t]->R) b->any_ripple = true;
+    b->max_M = std::max(b->max_M, b->sims[t]->M);
   }
   auto all = [](uint32_t) { return true; };
   int r;
@@ -1883,7 +1886,7 @@ static int batch_encode_frame(sl_batch *b, double dt) {
   sl_k_count(P, 3, 0);
   if (b->any_ripple)  // the mask stages return for tanks whose mask is off
     sl_k_ripple_frame_ex(batch_launch(b, BM_ROWS).w, batch_launch(b, BM_COLS).w, batch_launch(b, BM_ELEMS).w,
-                         batch_launch(b, BM_TEX).w, b->stream, 1);
+                         batch_launch(b, BM_TEX).w, b->stream, 1, b->max_M);
   if (b->any_toph) sl_k_toph(batch_launch(b, BM_TOPP).w, batch_launch(b, BM_TOPT).w, b->stream);
   hipError_t e = hipMemcpyAsync(b->h_counters, b->d_counters, sizeof(SlCounters) * b->cap, hipMemcpyDeviceToHost,
                                 b->stream);

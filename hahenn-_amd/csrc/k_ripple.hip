@@ -18,14 +18,14 @@
 enum { IN_BUF = 0, IN_SRC = 1, IN_MASK = 2 };
 enum { OUT_BUF = 0, OUT_SCALE = 1 };
 
-// butterfly of one output of a Stockham pass (64-ripple.js:44-55)
-__device__ __forceinline__ float2 bfly(const float2 *A, uint32_t idx, uint32_t ns, uint32_t M, uint32_t stride,
+// butterfly of one output of the Stockham pass of span ns = 2^lns over M = 2^lM points
+// (64-ripple.js:44-55): e = (idx / ns) * ns/2 + idx % (ns/2), o = e + M/2, twiddle (idx % ns) * (M / ns)
+__device__ __forceinline__ float2 bfly(const float2 *A, uint32_t idx, uint32_t lns, uint32_t lM, uint32_t stride,
                                        const float2 *tw, float sign) {
-  uint32_t hs = ns >> 1;
-  uint32_t e = (idx / ns) * hs + (idx % hs);
-  uint32_t o = e + (M >> 1);
+  const uint32_t e = ((idx >> lns) << (lns - 1u)) + (idx & ((1u << (lns - 1u)) - 1u));
+  const uint32_t o = e + (1u << (lM - 1u));
   float2 a = A[e * stride], b = A[o * stride];
-  float2 t = tw[(idx % ns) * (M / ns)];
+  float2 t = tw[(idx & ((1u << lns) - 1u)) << (lM - lns)];
   float wx = t.x, wy = sign * t.y;
   return make_float2(a.x + (wx * b.x - wy * b.y), a.y + (wx * b.y + wy * b.x));
 }
@@ -44,36 +44,38 @@ __device__ __forceinline__ float2 fft_input(const SlTank &T, int mode, const flo
   return make_float2(f.x * m, f.y * m);
 }
 
-// all horizontal passes of one row
-__global__ __launch_bounds__(256) void k_fft_rows(SlWork W, int which_in, int which_out, int mode, float sign) {
+// all horizontal passes of one row; dynamic LDS of two buffers of the largest M in the launch
+__global__ __launch_bounds__(256) void k_fft_rows(SlWork W, int which_in, int which_out, int mode, float sign,
+                                                  uint32_t ldsM) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
-  const uint32_t M = T.M, y = lb;
+  const uint32_t M = T.M, lM = T.logM, y = lb;
   if (y >= M) return;
   if (mode == IN_MASK && !T.mask_bound) return;  // tanks of a batch without the absorbing mask
-  __shared__ float2 lds[2][FFT_LDS_COMPLEX];
+  extern __shared__ float2 lds[];  // buffer b at lds + b * ldsM
   const float2 *bufs[5] = {T.rstate, T.ra, T.rb, T.rhat, nullptr};
   float2 *obufs[5] = {T.rstate, T.ra, T.rb, T.rhat, nullptr};
   const float2 *in = bufs[which_in];
   float2 *out = obufs[which_out];
-  for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) lds[0][x] = fft_input(T, mode, in, x, y);
+  for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) lds[x] = fft_input(T, mode, in, x, y);
   __syncthreads();
-  int cur = 0;
-  for (uint32_t ns = 2; ns <= M; ns <<= 1) {
-    for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) lds[cur ^ 1][x] = bfly(lds[cur], x, ns, M, 1, T.tw, sign);
+  uint32_t cur = 0;
+  for (uint32_t lns = 1; lns <= lM; lns++) {
+    const float2 *src = lds + cur * ldsM;
+    float2 *dst = lds + (cur ^ 1u) * ldsM;
+    for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) dst[x] = bfly(src, x, lns, lM, 1, T.tw, sign);
     __syncthreads();
-    cur ^= 1;
+    cur ^= 1u;
   }
-  for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) out[y * M + x] = lds[cur][x];
+  for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) out[y * M + x] = lds[cur * ldsM + x];
 }
 
 // all vertical passes of a tile of CW = 4096/M adjacent columns
 __global__ __launch_bounds__(256) void k_fft_cols(SlWork W, int which_in, int which_out, int omode, float sign) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
-  const uint32_t M = T.M;
-  uint32_t CW = FFT_LDS_COMPLEX / M;
-  if (CW > M) CW = M;
+  const uint32_t M = T.M, lM = T.logM;
+  const uint32_t lCW = lM <= 6u ? lM : 12u - lM, CW = 1u << lCW;  // FFT_LDS_COMPLEX / M, at most M
   const uint32_t x0 = lb * CW;
   if (x0 >= M) return;
   if (omode == OUT_SCALE && !T.mask_bound) return;  // tanks of a batch without the absorbing mask
@@ -85,22 +87,22 @@ __global__ __launch_bounds__(256) void k_fft_cols(SlWork W, int which_in, int wh
   const uint32_t n = CW * M;
   // LDS layout: column c at offset c, row y at y*CW (stride CW)
   for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
-    uint32_t c = q % CW, y = q / CW;
+    const uint32_t c = q & (CW - 1u), y = q >> lCW;
     lds[0][q] = in[y * M + x0 + c];
   }
   __syncthreads();
   int cur = 0;
-  for (uint32_t ns = 2; ns <= M; ns <<= 1) {
+  for (uint32_t lns = 1; lns <= lM; lns++) {
     for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
-      uint32_t c = q % CW, y = q / CW;
-      lds[cur ^ 1][q] = bfly(lds[cur] + c, y, ns, M, CW, T.tw, sign);
+      const uint32_t c = q & (CW - 1u), y = q >> lCW;
+      lds[cur ^ 1][q] = bfly(lds[cur] + c, y, lns, lM, CW, T.tw, sign);
     }
     __syncthreads();
     cur ^= 1;
   }
   const float sc = (float)(M * M);
   for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
-    uint32_t c = q % CW, y = q / CW;
+    const uint32_t c = q & (CW - 1u), y = q >> lCW;
     float2 v = lds[cur][q];
     if (omode == OUT_SCALE) v = make_float2(v.x / sc, v.y / sc);  // RSCALE, 64-ripple.js:127-133
     out[y * M + x0 + c] = v;
@@ -197,22 +199,23 @@ __global__ void k_clear_rsrc(SlWork W) {
 
 // buffers: 0 state, 1 a, 2 b, 3 hat
 void sl_k_ripple_frame_ex(const SlWork &rows, const SlWork &cols, const SlWork &elems, const SlWork &tex,
-                          hipStream_t s, int mask) {
+                          hipStream_t s, int mask, uint32_t maxM) {
   if (!rows.total) return;
+  const size_t rl = (size_t)2 * maxM * sizeof(float2);  // row LDS: two buffers of M points
   // forward transform of the mirrored source -> hat (64-ripple.js:229-232)
-  hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), 0, s, rows, 0, 1, (int)IN_SRC, -1.0f);
+  hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), rl, s, rows, 0, 1, (int)IN_SRC, -1.0f, maxM);
   hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), 0, s, cols, 1, 3, (int)OUT_BUF, -1.0f);
   // evolve the modes and add the kick (64-ripple.js:233)
   hipLaunchKernelGGL(k_revolve, dim3(elems.total), dim3(256), 0, s, elems);
   // inverse transform of the state -> b (64-ripple.js:234-235)
-  hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), 0, s, rows, 0, 1, (int)IN_BUF, 1.0f);
+  hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), rl, s, rows, 0, 1, (int)IN_BUF, 1.0f, maxM);
   hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), 0, s, cols, 1, 2, (int)OUT_BUF, 1.0f);
   // height and slopes (64-ripple.js:236-238)
   hipLaunchKernelGGL(k_rextract, dim3(tex.total), dim3(256), 0, s, tex);
   if (mask) {
     // absorb: mask the spatial field, transform back into the state, rescale (64-ripple.js:239-250)
     hipLaunchKernelGGL(k_rmaskbuild, dim3(tex.total), dim3(256), 0, s, tex);
-    hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), 0, s, rows, 2, 1, (int)IN_MASK, -1.0f);
+    hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), rl, s, rows, 2, 1, (int)IN_MASK, -1.0f, maxM);
     hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), 0, s, cols, 1, 0, (int)OUT_SCALE, -1.0f);
   }
   hipLaunchKernelGGL(k_clear_rsrc, dim3(tex.total), dim3(256), 0, s, tex);  // 64-ripple.js:252

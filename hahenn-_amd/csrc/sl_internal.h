@@ -340,7 +340,8 @@ int sl_k_reorder(const SlLaunch &L, uint32_t n_ub, uint32_t *keys, uint32_t *val
 // top-surface height map of the tanks with top_height set: particles (256 per workgroup), texels
 // (wmap^2, 256 per workgroup)
 void sl_k_toph(const SlWork &parts, const SlWork &texels, hipStream_t s);
+// maxM: the largest ripple M among the tanks of the launch (sizes the row transforms' LDS)
 void sl_k_ripple_frame_ex(const SlWork &rows, const SlWork &cols, const SlWork &elems, const SlWork &tex,
-                          hipStream_t s, int mask);
+                          hipStream_t s, int mask, uint32_t maxM);
 // workgroups of compaction per tank (each compaction workgroup covers SL_COMPACT_ITEMS particles)
 #define SL_COMPACT_ITEMS 1024
