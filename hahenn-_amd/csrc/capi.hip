@@ -224,7 +224,9 @@ static int push_dyn(sl_sim *s, uint32_t emit_k = 0, uint32_t emit_total = 0) {
   SlTankDyn d;
   d.dt = s->ht.dt; d.gdt = s->ht.gdt; d.time_f = s->ht.time_f;
   d.rLx = s->ht.rLx; d.rLz = s->ht.rLz; d.rdt = s->ht.rdt; d.rmd = s->ht.rmd;
-  d.frame = s->frame; d.obj_count = s->obj_count; d.emit_k = emit_k; d.emit_total = emit_total;
+  // the object count of the parameter block (50-sim.js:155), as of the last writeParams: setObjects
+  // alone does not change it (50-sim.js:206), so emitNow before a step sees the count of the last step
+  d.frame = s->frame; d.obj_count = s->ht.obj_count; d.emit_k = emit_k; d.emit_total = emit_total;
   d.iters = s->iters; d.mask_bound = s->mask_bound;
   d.time_d = s->time;
   d.dt_frame = s->batch_dt;
@@ -535,6 +537,40 @@ extern "C" int sl_grid_info(const sl_sim *s, sl_grid_t *o) {
 // objects, 50-sim.js:194-210
 // ------------------------------------------------------------------------------------------------
 static bool same_f(float a, float b) { return a == b || (a != a && b != b); }
+// Radius of a sphere about an object's position that contains its solid, from the shape table of
+// shape_sdf (sl_internal.h), stored in the unused row pad r0.w of the device copy. For the shapes
+// below the signed distance is at least the distance to that sphere (the exact distance of a solid
+// inside it; for the cup and container the max() of the outer solid's exact distance and more), so
+// G2P may skip an obstacle whose sphere is farther than the contact radius, and a drain whose
+// cylinder sphere does not hold the point. +inf (never skipped) where that is not guaranteed: the
+// gutter (an intersection of three distances), negative or degenerate dimensions, and rotation rows
+// that are not orthonormal; -inf for types whose distance is the constant 1e9.
+static float cull_radius(const float *row) {
+  const float *a = row, *b = row + 4, *c = row + 8, *d = row + 16;
+  auto dot = [](const float *x, const float *y) { return (double)x[0] * y[0] + (double)x[1] * y[1] + (double)x[2] * y[2]; };
+  const double err = fabs(dot(a, a) - 1) + fabs(dot(b, b) - 1) + fabs(dot(c, c) - 1) + fabs(dot(a, b)) +
+                     fabs(dot(a, c)) + fabs(dot(b, c));
+  if (!(err < 1e-4)) return INFINITY;
+  const float flag = row[27];
+  const double d0 = d[0], d1 = d[1], d2 = d[2], d3 = d[3];
+  if (flag == 2.0f) return d0 >= 0 && d1 >= 0 ? (float)hypot(d0, d1) : INFINITY;  // in_drain: sd_cyl < 0
+  if (flag != 1.0f) return INFINITY;
+  const double t = row[15] + 0.5;
+  if (!(t >= 0 && t < 4294967296.0)) return -INFINITY;
+  switch ((uint32_t)t) {
+    case 0:
+      return d0 >= 0 && d1 >= 0 && d2 >= 0 && d3 >= 0 && d3 <= std::min(std::min(d0, d1), d2)
+                 ? (float)sqrt(d0 * d0 + d1 * d1 + d2 * d2) : INFINITY;
+    case 1: return (float)d0;  // |q| - r exactly
+    case 2: case 3: case 5: return d0 >= 0 && d1 >= 0 ? (float)hypot(d0, d1) : INFINITY;
+    case 4: return d1 >= 0 ? (float)(fabs(d0) + d1) : INFINITY;
+    case 6:
+      return d0 >= 0.002 && d1 >= 0.002 && d2 >= 0.002 ? (float)sqrt(d0 * d0 + d1 * d1 + d2 * d2) : INFINITY;
+    case 7: return INFINITY;
+    default: return -INFINITY;
+  }
+}
+
 extern "C" int sl_set_objects(sl_sim *s, const float *rows, uint32_t n, int *changed) {
   if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
   if (n && !rows) FAIL(s, SL_EARG, "sl_set_objects: null rows");
@@ -547,7 +583,10 @@ extern "C" int sl_set_objects(sl_sim *s, const float *rows, uint32_t n, int *cha
   if (n) memcpy(s->objs, rows, (size_t)n * 28 * 4);
   s->obj_count = n;
   s->geom_dirty = true;
-  return stage_upload(s, s->ht.objs, s->objs, sizeof(SlObj) * (n ? n : 1));
+  float dev[SL_MAX_OBJ * 28];
+  memcpy(dev, s->objs, sizeof dev);
+  for (uint32_t i = 0; i < n; i++) dev[i * 28 + 3] = cull_radius(dev + i * 28);
+  return stage_upload(s, s->ht.objs, dev, sizeof(SlObj) * (n ? n : 1));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -1801,6 +1840,11 @@ extern "C" int sl_batch_emit(sl_batch *b, const sl_emit *recs, const uint32_t *c
     off += m;
     if (!m) continue;
     sl_sim *s = b->sims[t];
+    if (s->ht.obj_count != s->obj_count) {  // the single path's emission follows writeParams
+      s->ht.obj_count = s->obj_count;
+      const int r = push_dyn(s);
+      if (r) return batch_fail(b, r, s->err);
+    }
     uint32_t k = 0, tot = 0;
     for (uint32_t i = 0; i < m && k < SL_MAX_EMIT && tot < s->cap; i++) {
       if (!L[i].count) continue;
@@ -1900,6 +1944,7 @@ static int batch_prepare(sl_batch *b) {
   for (uint32_t t = 0; t < b->cap; t++) {
     sl_sim *s = b->sims[t];
     if (!s || !b->stepped[t] || !s->geom_dirty) continue;
+    s->ht.obj_count = s->obj_count;  // this frame's parameters, as the single path writes before GEOM
     int r = push_dyn(s);
     if (r) return batch_fail(b, r, s->err);
     sl_k_geom(launch_of(s, s->nodes));

@@ -205,7 +205,8 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
   }
   f3 x = add3(x0, mul3(v, dt));
   const float rc = T.rc;
-  float d = scene_sdf(T, x);
+  // only d < rc matters; no obstacle within reach means scene_sdf(x) > rc
+  const float d = obstacle_within(T, x, rc) ? scene_sdf(T, x) : rc;
   if (d < rc) {
     f3 nrm = scene_normal(T, x);
     x = add3(x, mul3(nrm, rc - d));

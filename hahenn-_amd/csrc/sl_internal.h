@@ -286,10 +286,24 @@ __device__ __forceinline__ f3 solid_vel_at(const SlTank &T, f3 p) {
   }
   return v;
 }
+// The host's cull radius of object o (row pad r0.w, see cull_radius in capi.hip) is farther than
+// `reach` from p, with slack for the rotation rows' rounding: then o's signed distance at p exceeds
+// `reach` (for a drain, p is outside its cylinder).
+__device__ __forceinline__ bool beyond_cull(const SlObj &o, f3 p, float reach) {
+  return len3(sub3(p, obj_pos(o))) > (o.r0[3] + reach) * 1.0002f + 1e-4f;
+}
+// some obstacle could be within `reach` of p (otherwise scene_sdf(p) > reach)
+__device__ __forceinline__ bool obstacle_within(const SlTank &T, f3 p, float reach) {
+  for (uint32_t i = 0; i < T.obj_count; i++) {
+    const SlObj &o = T.objs[i];
+    if (o.angv[3] == 1.0f && !beyond_cull(o, p, reach)) return true;
+  }
+  return false;
+}
 __device__ __forceinline__ bool in_drain(const SlTank &T, f3 p) {
   for (uint32_t i = 0; i < T.obj_count; i++) {
     const SlObj &o = T.objs[i];
-    if (o.angv[3] != 2.0f) continue;
+    if (o.angv[3] != 2.0f || beyond_cull(o, p, 0.0f)) continue;
     if (sd_cyl(to_local(o, p), o.dims[0], o.dims[1]) < 0.0f) return true;
   }
   return false;
