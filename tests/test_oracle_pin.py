@@ -473,6 +473,46 @@ def test_ripple_frame_matches_the_numpy_port():
             assert np.abs(got[:, part] - want).max() <= 1e-5 * np.abs(want).max(), (c, part)
 
 
+def test_whitewater_pass_matches_the_numpy_port():
+    """One WW pass (classification from the cell counts, landing deposits into the ripple source,
+    spray / foam / bubble motion, tank clamp, survival) against the golden vectors of the
+    reference's own NumPy solver (Solver._whitewater, blender_addon/suiri_lab/solver.py:675-713,
+    written by tests/golden/make_ww_fixtures.py): 400 particles in and above a ragged water body and
+    beside the tank. Both keep survivors in order, so they compare one by one: kinds exactly,
+    positions, velocities and lives to f32 rounding, deposits to the truncation of each impulse x 1e9."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "ww_pass.npz"))
+    o = O.Oracle(tuple(fx["tank"]), O.quality("standard", cap=4000, ww_cap=4000))
+    inf = o.info()
+    nx, ny, nz = (int(v) for v in fx["grid"])
+    assert (int(inf["nx"]), int(inf["ny"]), int(inf["nz"]), int(inf["R"])) == (nx, ny, nz, int(fx["ripple_R"]))
+    assert abs(inf["h"] - float(fx["h"])) < 1e-9
+    dt = float(fx["dt"])
+    o.write_params(dt)
+    o.field("vel")[:, :3] = fx["velg"].transpose(3, 2, 1, 0).reshape(-1, 3)
+    o.field("cellCount")[:] = fx["count"].astype(np.uint32).transpose(2, 1, 0).ravel()
+    n = len(fx["pos"])
+    ww = o.field("ww")
+    ww[:n, 0:3], ww[:n, 3], ww[:n, 4:7], ww[:n, 7] = fx["pos"], fx["life"], fx["vel"], fx["kind"]
+    ctr = o.field("counters")
+    ctr[2], ctr[3] = n, 0
+    o.field("rsrc")[:] = 0
+    o.L.or_ww(o.ptr, ctypes.c_float(dt))
+    m = int(o.field("counters")[3])
+    assert m == len(fx["out_pos"]) and 0 < m < n
+    got = o.field("ww_other")[:m].astype(np.float64)
+    assert np.array_equal(got[:, 7], fx["out_kind"])
+    assert {0.0, 1.0, 2.0} <= set(fx["out_kind"].tolist())
+    assert np.abs(got[:, 0:3] - fx["out_pos"]).max() <= 1e-6
+    assert np.abs(got[:, 3] - fx["out_life"]).max() <= 1e-6
+    assert np.abs(got[:, 4:7] - fx["out_vel"]).max() <= 1e-5 * np.abs(fx["out_vel"]).max()
+    R = int(fx["ripple_R"])
+    rs = o.field("rsrc").reshape(R + 1, R + 1).astype(np.int64)
+    want = np.zeros((R + 1, R + 1))
+    want[fx["rsrc_j"], fx["rsrc_i"]] = fx["rsrc_v"] * 1e9
+    assert len(fx["rsrc_v"]) > 5 and np.abs(rs - want).max() <= 2.0
+
+
 def test_lattice_emit_and_stable_compaction():
     o = empty_sim(cap=50000)
     o.write_params(1 / 60)
