@@ -80,6 +80,7 @@ def lib():
         L.or_rsrc.argtypes = [P, P]
         L.or_rmask.argtypes = [P, P, P]
         L.or_rextract.argtypes = [P, P]
+        L.or_crown.argtypes = [P, ctypes.POINTER(ctypes.c_float), ctypes.c_float, ctypes.c_float, ctypes.c_uint32]
         L.or_hashU.restype = ctypes.c_uint32
         L.or_hashU.argtypes = [ctypes.c_uint32]
         L.or_scene_sdf.restype = ctypes.c_float

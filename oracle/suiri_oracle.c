@@ -558,6 +558,10 @@ static void splashCrown(or_sim *s, v3 x, float U, float m, uint32_t id, or_parti
     ww[k] = q;
   }
 }
+/* test hook: one splashCrown at x into the current whitewater buffer (appends at wwCount) */
+void or_crown(or_sim *s, const float *x, float U, float m, uint32_t id) {
+  splashCrown(s, V(x[0], x[1], x[2]), U, m, id, s->ww[s->ww_cur]);
+}
 static inline void atomic_max_u32(uint32_t *p, uint32_t v) {
   uint32_t cur = __atomic_load_n(p, __ATOMIC_RELAXED);
   while (v > cur && !__atomic_compare_exchange_n(p, &cur, v, 1, __ATOMIC_RELAXED, __ATOMIC_RELAXED)) {}

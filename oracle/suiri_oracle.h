@@ -131,6 +131,7 @@ void or_toph_splat(or_sim *s);
 void or_toph_blur(or_sim *s);
 void or_count(or_sim *s, uint32_t mode, uint32_t y); /* 31-wgsl-sim2.js:332-349 */
 void or_ww(or_sim *s, float dt);           /* 31-wgsl-sim2.js:361-409 */
+void or_crown(or_sim *s, const float *x, float U, float m, uint32_t id); /* splashCrown, 31-wgsl-sim2.js:119-148 */
 
 /* ripple kernels, src/js/64-ripple.js */
 void or_rsrc(or_sim *s, float *out);                                   /* :18-31 */

@@ -513,6 +513,55 @@ def test_whitewater_pass_matches_the_numpy_port():
     assert len(fx["rsrc_v"]) > 5 and np.abs(rs - want).max() <= 2.0
 
 
+def test_crown_splash_counts_match_the_numpy_port():
+    """splashCrown (31-wgsl-sim2.js:119-148) against the counts of the reference's own NumPy solver
+    (Solver._crown, blender_addon/suiri_lab/solver.py:632-657, written by
+    tests/golden/make_crown_fixtures.py) for 420 (speed, particle mass) pairs: the same number of
+    ring droplets and jets (the K = We Oh^-0.4 thresholds), ring droplets on the circle of radius
+    0.6 D at the drop's height, jets stacked 1 mm apart with life 0.7, and speeds and lives inside the
+    ranges the random draws allow (the port draws from another generator). Pairs within f32 rounding
+    of a count threshold are skipped."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "crown.npz"))
+    o = empty_sim(tank=(0.2, 0.12, 0.15), h=1 / 64, R=0)
+    o.write_params(1 / 120)
+    x = np.array([0.01, 0.05, -0.02], np.float32)
+    checked = 0
+    seen = set()
+    for k in range(len(fx["U"])):
+        U, m, n, nj, D = (float(fx[key][k]) for key in ("U", "m", "n", "nj", "D"))
+        n, nj = int(n), int(nj)
+        K = 998.0 * U * U * D / 0.0728 * (0.001 / math.sqrt(998.0 * 0.0728 * D)) ** -0.4
+        r = 4 * (K / 2100 - 1)
+        if abs(r - math.floor(r) - 0.5) < 1e-3 or min(abs(K / 700 - 1), abs(K / 2100 - 1)) < 1e-4:
+            continue
+        ctr = o.field("counters")
+        ctr[2] = 0
+        o.L.or_crown(o.ptr, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), ctypes.c_float(U), ctypes.c_float(m), k)
+        tot = int(o.field("counters")[2])
+        assert tot == n + nj, (U, m, tot, n, nj)
+        w = o.field("ww")[:tot].astype(np.float64)
+        ring, jet = w[:n], w[n:]
+        if n:
+            rad = np.hypot(ring[:, 0] - x[0], ring[:, 2] - x[2])
+            assert np.abs(rad - 0.6 * D).max() <= 1e-5 * D + 1e-7
+            assert np.abs(ring[:, 1] - x[1]).max() <= 1e-7
+            assert ring[:, 3].min() >= 0.5 - 1e-6 and ring[:, 3].max() <= 0.9 + 1e-6
+            sp = np.linalg.norm(ring[:, 4:7], axis=1)
+            assert sp.min() >= 0.15 * U * (1 - 1e-5) and sp.max() <= 0.35 * U * (1 + 1e-5)
+            el = np.arctan2(ring[:, 5], np.hypot(ring[:, 4], ring[:, 6]))
+            assert el.min() >= 0.9 - 1e-5 and el.max() <= 1.35 + 1e-5
+        for j in range(nj):
+            assert np.allclose(jet[j, :3], x + [0, 0.001 * j, 0], atol=1e-7) and jet[j, 3] == np.float32(0.7)
+            lo, hi = U * (0.22 + 0.04 * j) * 0.9, U * (0.22 + 0.04 * j) * 1.1
+            assert lo * (1 - 1e-5) <= jet[j, 5] <= hi * (1 + 1e-5) and abs(jet[j, 4]) <= 0.025 and abs(jet[j, 6]) <= 0.025
+        seen.add((min(n, 1) + (n == 24), nj))
+        checked += 1
+    assert checked > 400
+    # none; one jet (700 < K <= 2100); two jets, no ring yet; a ring and two jets; the full ring of 24
+    assert {(0, 0), (0, 1), (0, 2), (1, 2), (2, 2)} <= seen, seen
+
+
 def test_lattice_emit_and_stable_compaction():
     o = empty_sim(cap=50000)
     o.write_params(1 / 60)
