@@ -562,6 +562,29 @@ def test_crown_splash_counts_match_the_numpy_port():
     assert {(0, 0), (0, 1), (0, 2), (1, 2), (2, 2)} <= seen, seen
 
 
+def test_ripple_mask_matches_the_numpy_port():
+    """RMASKBUILD (64-ripple.js:106-126) against the golden mask of the reference's own NumPy solver
+    (Solver._ripple_mask, blender_addon/suiri_lab/solver.py:781-794, written by
+    tests/golden/make_mask_fixtures.py) on the same 199 x 199 top-height map over a box whose top
+    face both solvers measure exactly: the texel under each of the 129 x 129 ripple nodes, 0.75 where
+    dry, smoothstep(-0.003, 0.006) of the distance 2 mm below the surface where wet (a fifth of the
+    nodes inside the band)."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "ripple_mask.npz"))
+    R, W, face = int(fx["R"]), int(fx["W"]), float(fx["face"])
+    q = O.quality("standard", ripple_R=R, cap=4000, ww_cap=4000)
+    q.wmap = W
+    o = O.Oracle(tuple(fx["tank"]), q)
+    o.set_objects(obj_row(0, (0.0, 0.0, 0.0), (1.0, face, 1.0, 0.002), bound=1.5))
+    o.write_params(1 / 120)  # the object count of the parameter block
+    o.field("topH")[:] = fx["top"].ravel()
+    o.kernel("rmaskbuild")
+    got = o.field("mask").astype(np.float64)
+    want = fx["mask"].ravel()
+    assert np.array_equal(got == np.float32(0.75), want == 0.75)
+    assert np.abs(got - want).max() <= 2e-5
+
+
 def test_lattice_emit_and_stable_compaction():
     o = empty_sim(cap=50000)
     o.write_params(1 / 60)
