@@ -585,6 +585,55 @@ def test_ripple_mask_matches_the_numpy_port():
     assert np.abs(got - want).max() <= 2e-5
 
 
+def test_g2p_pass_matches_the_numpy_port():
+    """One G2P pass (masked PIC/FLIP transfer, advection, tank clamp and kill rules, the free-flight
+    mark from the cell counts, drop merging with its ripple deposit and impact count, vmax) against
+    the golden vectors of the reference's own NumPy solver (Solver._g2p,
+    blender_addon/suiri_lab/solver.py:527-631, written by tests/golden/make_g2p_fixtures.py):
+    1500 particles in and above a ragged water body with sparse drops, at the walls and beside the
+    tank, whitewater off, no obstacles. Survivors compare one by one: marks exactly, positions and
+    velocities to f32 rounding; impacts exactly; deposits to the truncation of each impulse x 1e9."""
+    import os
+    fx = np.load(os.path.join(os.path.dirname(__file__), "golden", "g2p_pass.npz"))
+    q = O.quality("standard", cap=4000, ww_cap=4000, flip=0.96)
+    q.ww_rate = 0.0
+    o = O.Oracle(tuple(fx["tank"]), q)
+    inf = o.info()
+    nx, ny, nz = (int(v) for v in fx["grid"])
+    assert (int(inf["nx"]), int(inf["ny"]), int(inf["nz"])) == (nx, ny, nz) and abs(inf["h"] - float(fx["h"])) < 1e-9
+    dt = float(fx["dt"])
+    o.write_params(dt)
+    o.field("vel")[:, :3] = fx["velg"].transpose(3, 2, 1, 0).reshape(-1, 3)
+    vo = o.field("velOld")
+    vo[:, :3] = fx["velold"].transpose(3, 2, 1, 0).reshape(-1, 3)
+    bits = (fx["mask"][0] * 1 + fx["mask"][1] * 2 + fx["mask"][2] * 4).astype(np.float32)
+    vo[:, 3] = bits.transpose(2, 1, 0).ravel()
+    o.field("cellCount")[:] = fx["count"].astype(np.uint32).transpose(2, 1, 0).ravel()
+    n = len(fx["pos"])
+    parts = o.field("parts")
+    parts[:n, 0:3], parts[:n, 3], parts[:n, 4:7], parts[:n, 7] = fx["pos"], 1.0, fx["vel"], fx["air"]
+    ctr = o.field("counters")
+    ctr[0], ctr[4], ctr[5] = n, 0, 0
+    o.field("rsrc")[:] = 0
+    o.kernel("g2p")
+    out = o.field("parts")[:n].astype(np.float64)
+    alive = out[:, 3] > 0.5
+    assert alive.sum() == len(fx["out_pos"]) < n
+    got = out[alive]
+    assert np.array_equal(got[:, 7], fx["out_air"]) and 0 < fx["out_air"].sum() < len(got)
+    assert np.abs(got[:, 0:3] - fx["out_pos"]).max() <= 1e-6
+    assert np.abs(got[:, 4:7] - fx["out_vel"]).max() <= 1e-5 * np.abs(fx["out_vel"]).max()
+    ctr = o.field("counters")
+    assert int(ctr[5]) == int(fx["impacts"]) > 10
+    vmax = float(np.uint32(ctr[4]).view(np.float32))
+    assert abs(vmax - float(fx["vmax"])) <= 1e-5 * float(fx["vmax"])
+    R = int(o.info()["R"])
+    rs = o.field("rsrc").reshape(R + 1, R + 1).astype(np.int64)
+    want = np.zeros((R + 1, R + 1))
+    want[fx["rsrc_j"], fx["rsrc_i"]] = fx["rsrc_v"] * 1e9
+    assert np.abs(rs - want).max() <= 3.0
+
+
 def test_lattice_emit_and_stable_compaction():
     o = empty_sim(cap=50000)
     o.write_params(1 / 60)
