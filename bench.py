@@ -11,6 +11,9 @@ A step = one FluidSim.step (emission, P2G, normalise+forces, divergence, 60 SOR 
 projection, G2P, compaction, whitewater, ripple frame). Inputs are resident in HBM before timing.
 Prints one JSON line: particle-steps/s, the SOR kernel's HBM roofline and a CPU baseline (the fp32
 C restatement of the reference, OpenMP on the host cores, on a bounded sample of the same workload).
+Beside the headline: the projection residual the reference's 60 sweeps leave, the same workload with
+the converged pressure solve (multigrid-preconditioned CG to a residual tolerance), the batched tanks
+and the serving loop.
 """
 import argparse
 import importlib
@@ -113,6 +116,57 @@ def pmc_traffic(out_dir, timeout=150):
     write = vals["write"] * 1024 * kib / vals["cwrite"]
     return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
             "fetch_factor": kib / vals["cfetch"], "write_factor": kib / vals["cwrite"]}
+
+
+def projection_residual(g, dt):
+    """The residual the pressure solve leaves after a step: the DIVERGENCE stage run again on the
+    projected field gives, per liquid cell, its divergence minus the drift source the solve aimed at
+    (src/js/30-wgsl-sim.js:240-264), in 1/s. Taken over liquid cells with an open face; a liquid cell
+    whose six faces are all closed (inside an obstacle) has no pressure equation, so it is counted
+    apart."""
+    g.run_stage(suiri.S_DIVERGENCE, 0, dt)
+    nx, ny, nz = g.nx, g.ny, g.nz
+    liquid = g.readback(suiri.F_CELLTYPE) == 1
+    fr = g.readback(suiri.F_FRAC).reshape(nz + 1, ny + 1, nx + 1, 4)
+    fx, fy, fz = fr[:nz, :ny, :, 0] > 0, fr[:nz, :, :nx, 1] > 0, fr[:, :ny, :nx, 2] > 0
+    has_open = (fx[:, :, :-1] | fx[:, :, 1:] | fy[:, :-1] | fy[:, 1:] | fz[:-1] | fz[1:]).ravel()
+    r = np.abs(g.readback(suiri.F_DIVG)[liquid & has_open].astype(np.float64))
+    return {"unit": "1/s", "cells": int(r.size), "max": float(r.max()) if r.size else 0.0,
+            "p99": float(np.percentile(r, 99)) if r.size else 0.0,
+            "rms": float(np.sqrt((r ** 2).mean())) if r.size else 0.0,
+            "closed_liquid_cells": int((liquid & ~has_open).sum())}
+
+
+def converged_leg(wl, steps, warmup, tol, max_iters):
+    """The workload with the converged pressure solve (multigrid-preconditioned CG to `tol` 1/s)."""
+    g, em = build_sim(wl)
+    for _ in range(warmup):
+        g.step(wl["frame_dt"], em)
+    g.autotune(reps=2)
+    g.set_pressure_solver(suiri.SOLVER_MGPCG, tol, max_iters)
+    for _ in range(5):
+        g.step(wl["frame_dt"], em)
+    c0 = g.stats()["count"]
+    g.sync()
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        g.step(wl["frame_dt"], em)
+    g.sync()
+    el = time.perf_counter() - t0
+    c1 = g.stats()["count"]
+    # after the timed region: iterations of a few more solves, then the residual of the last one
+    its = []
+    for _ in range(5):
+        g.step(wl["frame_dt"], em)
+        its.append(g.pressure_solver_stats()["iters"])
+    st = g.pressure_solver_stats()
+    res = projection_residual(g, wl["frame_dt"])
+    g.close()
+    mean_p = (c0 + c1) / 2
+    return {"solver": "mgpcg", "tol_per_s": tol, "max_iters": max_iters, "steps": steps,
+            "ms_per_step": el / steps * 1e3, "steps_per_s": steps / el,
+            "particle_steps_per_s": mean_p * steps / el, "iters_sampled": its,
+            "last_solve": st, "divergence_after_projection": res}
 
 
 def batched_leg(n_tanks, steps, warmup, seq_steps, graph_steps=50, seed=7):
@@ -306,8 +360,11 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ripple", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter runs for `traffic`")
-    ap.add_argument("--only", choices=["all", "headline", "batched", "serve"], default="all",
+    ap.add_argument("--only", choices=["all", "headline", "batched", "serve", "converged"], default="all",
                     help="run one leg (its JSON object alone) instead of the full line")
+    ap.add_argument("--conv-tol", type=float, default=1e-4, help="converged leg: residual tolerance (1/s)")
+    ap.add_argument("--conv-iters", type=int, default=24, help="converged leg: CG iterations enqueued per solve")
+    ap.add_argument("--conv-steps", type=int, default=60)
     ap.add_argument("--batch-tanks", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--seq-steps", type=int, default=10)
@@ -342,6 +399,9 @@ def main():
         return
     grid = tuple(int(x) for x in args.grid.split("x"))
     wl = workloads.faucet(grid=grid, R=0 if args.no_ripple else 512)
+    if args.only == "converged":
+        print(json.dumps(converged_leg(wl, args.conv_steps, args.warmup, args.conv_tol, args.conv_iters)))
+        return
     g, em = build_sim(wl)
     for _ in range(args.warmup):
         g.step(wl["frame_dt"], em)
@@ -369,6 +429,9 @@ def main():
     g.sync()
     elapsed_plain = time.perf_counter() - t1
     st2 = g.stats()
+    # after the timed region: the residual the pressure solve left in the last step, i.e. the divergence
+    # of the projected field minus the drift source it was solved for, over the liquid cells (1/s)
+    resid = projection_residual(g, wl["frame_dt"])
     mean_p = (st1["count"] + st2["count"]) / 2
     ms_per_step = elapsed_plain / args.steps * 1e3
     value = mean_p * args.steps / elapsed_plain
@@ -401,11 +464,13 @@ def main():
         "sor_GBps": iters * 2 * bpl / (sor_ms / 1e3 / args.steps) / 1e9,
         "stage_ms_per_step": {k: v[0] / args.steps for k, v in tm.items()},
         "step_algorithmic_GB": workloads.step_bytes(nx, ny, nz, mean_p, iters) / 1e9,
+        "divergence_after_projection": resid,
         "gpu_errors": st2["gpuErrors"],
         "tuned_variants": tuned,
     }
     g.close()
     if args.only == "all":
+        out["converged"] = converged_leg(wl, args.conv_steps, args.warmup, args.conv_tol, args.conv_iters)
         out["batched"] = batched_leg(args.batch_tanks, args.batch_steps, 5, args.seq_steps, args.graph_steps)
         out["batched_p99_step_ms"] = out["batched"]["step_latency_ms"]["p99"]
         out["served"] = serve_leg(args.serve_rate, args.serve_window, args.serve_limit,

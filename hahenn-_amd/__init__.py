@@ -21,6 +21,7 @@ F_PARTICLES, F_WW, F_COUNTERS, F_ACC, F_CELLCOUNT, F_VEL, F_VELOLD, F_FRAC, F_SV
     F_DIVG, F_PRES, F_OBJS, F_RSRC, F_RSTATE, F_RHAT, F_RFIELD, F_RMASK, F_RTEX, F_TOPH, F_RTEX16, F_ARGS = range(22)
 S_GEOM, S_P2G, S_NORMFORCE, S_DIVERGENCE, S_PRESSURE, S_PROJECT, S_G2P, S_COMPACT, S_WW, S_RIPPLE, \
     S_SOR = range(11)
+SOLVER_SOR, SOLVER_MGPCG = 0, 1
 T_NAMES = ["emit", "clear", "geom", "p2g", "normforce", "divergence", "sor", "project", "g2p", "compact",
            "ww", "ripple"]
 
@@ -77,7 +78,8 @@ EXPORTS = ["sl_config_default", "sl_create", "sl_destroy", "sl_last_error", "sl_
            "sl_clear_water", "sl_save_water", "sl_has_water_copy", "sl_restore_water", "sl_poll_stats",
            "sl_stats", "sl_sync", "sl_set_ripple_mask", "sl_field_bytes", "sl_upload", "sl_readback",
            "sl_set_scalar", "sl_get_scalar", "sl_run_stage", "sl_timing_enable", "sl_timing_read",
-           "sl_version", "sl_build_info", "sl_reorder", "sl_autotune",
+           "sl_version", "sl_build_info", "sl_reorder", "sl_autotune", "sl_set_pressure_solver",
+           "sl_pressure_solver_stats",
            "sl_batch_create", "sl_batch_destroy", "sl_batch_last_error", "sl_batch_size", "sl_batch_sim",
            "sl_batch_step", "sl_batch_capture", "sl_batch_sync", "sl_batch_reorder", "sl_batch_autotune",
            "sl_batch_set_variants", "sl_batch_timing_enable", "sl_batch_timing_read", "sl_batch_open",
@@ -150,6 +152,9 @@ def load():
         "sl_step_end": (I, [P, D]),
         "sl_step": (I, [P, D, P, P]),
         "sl_set_substeps": (I, [P, U]),
+        "sl_set_pressure_solver": (I, [P, I, D, U]),
+        "sl_pressure_solver_stats": (I, [P, ctypes.POINTER(U), ctypes.POINTER(I), ctypes.POINTER(D),
+                                         ctypes.POINTER(D)]),
         "sl_clear_water": (I, [P]),
         "sl_reorder": (I, [P]),
         "sl_autotune": (I, [P, U, ctypes.POINTER(U)]),
@@ -287,6 +292,18 @@ class Sim:
 
     def set_substeps(self, n):
         self._check(self.L.sl_set_substeps(self.ptr, int(n)))
+
+    def set_pressure_solver(self, solver, tol_per_s=1e-5, max_iters=40):
+        """SOLVER_SOR: the reference's red-black SOR sweeps (default). SOLVER_MGPCG: multigrid-
+        preconditioned CG on the same system until the largest residual is at most tol_per_s (1/s)."""
+        self._check(self.L.sl_set_pressure_solver(self.ptr, int(solver), float(tol_per_s), int(max_iters)))
+
+    def pressure_solver_stats(self):
+        it, br = ctypes.c_uint32(0), ctypes.c_int(0)
+        r, r0 = ctypes.c_double(0), ctypes.c_double(0)
+        self._check(self.L.sl_pressure_solver_stats(self.ptr, ctypes.byref(it), ctypes.byref(br), ctypes.byref(r),
+                                                    ctypes.byref(r0)))
+        return {"iters": it.value, "breakdown": bool(br.value), "resid_per_s": r.value, "resid0_per_s": r0.value}
 
     def step(self, frame_dt, emit_fn=None):
         n = ctypes.c_uint32(0)

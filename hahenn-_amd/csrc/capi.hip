@@ -105,6 +105,10 @@ struct sl_sim {
   std::vector<hipEvent_t> ev_pool;
   double t_ms[SL_T_COUNT_] = {0};
   uint32_t t_launch[SL_T_COUNT_] = {0};
+  // pressure solver (SL_SOLVER_SOR: the reference's sweeps; SL_SOLVER_MGPCG: converged solve)
+  int solver = SL_SOLVER_SOR;
+  SlMg mg;              // hierarchy and vectors, allocated when the converged solve is first chosen
+  bool mg_seed = false; // the next solve starts from the SOR pressure
 };
 
 static uint32_t g_copies = 0;
@@ -663,6 +667,92 @@ extern "C" int sl_set_substeps(sl_sim *s, uint32_t n) {
   return SL_OK;
 }
 
+// levels of the converged solve: the cell grid, then every dimension halved (rounding up) until the
+// level has at most SL_MG_COARSE cells
+static int mg_alloc(sl_sim *s) {
+  SlMg &m = s->mg;
+  if (m.nlev) return SL_OK;
+  int32_t X = s->nx, Y = s->ny, Z = s->nz;
+  int l = 0;
+  for (;;) {
+    if (l == SL_MG_MAXLEV) FAIL(s, SL_EARG, "pressure solver: grid %dx%dx%d needs more than %d levels", s->nx, s->ny, s->nz, SL_MG_MAXLEV);
+    SlMgLevel &L = m.lev[l];
+    L.X = X; L.Y = Y; L.Z = Z;
+    L.cells = (uint32_t)X * (uint32_t)Y * (uint32_t)Z;
+    sl_magic_u32((uint32_t)X, L.magX);
+    sl_magic_u32((uint32_t)X * (uint32_t)Y, L.magXY);
+    ALLOC(L.A, (size_t)L.cells * sizeof(float4));
+    ALLOC(L.F, (size_t)L.cells * 2);
+    ALLOC(L.b, (size_t)L.cells * 4);
+    ALLOC(L.x, (size_t)L.cells * 4);
+    l++;
+    if (L.cells <= SL_MG_COARSE) break;
+    X = (X + 1) / 2; Y = (Y + 1) / 2; Z = (Z + 1) / 2;
+  }
+  const size_t C = s->cells;
+  ALLOC(m.xs, C * 8); ALLOC(m.bs, C * 8); ALLOC(m.r, C * 8); ALLOC(m.p[0], C * 8); ALLOC(m.p[1], C * 8);
+  ALLOC(m.q, C * 8);
+  ALLOC(m.part, (size_t)sl_mg_parts(s->cells) * 8);
+  ALLOC(m.scal, 8 * 8);
+  ALLOC(m.ctl, 8 * 4);
+  m.nlev = l;
+  return SL_OK;
+}
+
+extern "C" int sl_set_pressure_solver(sl_sim *s, int solver, double tol_per_s, uint32_t max_iters) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  if (solver == SL_SOLVER_SOR) { s->solver = solver; return SL_OK; }
+  if (solver != SL_SOLVER_MGPCG) FAIL(s, SL_EARG, "sl_set_pressure_solver: unknown solver %d", solver);
+  if (s->ext_stream) FAIL(s, SL_EARG, "sl_set_pressure_solver: the tanks of a batch use the SOR sweeps");
+  if (!(tol_per_s >= 0) || max_iters < 1 || max_iters > 1000)
+    FAIL(s, SL_EARG, "sl_set_pressure_solver: tolerance %g must be >= 0 and max_iters %u in 1..1000", tol_per_s, max_iters);
+  int r = mg_alloc(s);
+  if (r) return r;
+  if (s->solver != SL_SOLVER_MGPCG) s->mg_seed = true;
+  s->solver = solver;
+  s->mg.rtol = tol_per_s * s->h * s->h;
+  s->mg.max_iters = max_iters;
+  return SL_OK;
+}
+
+extern "C" int sl_pressure_solver_stats(sl_sim *s, uint32_t *iters, int *breakdown, double *resid_per_s,
+                                        double *resid0_per_s) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  if (!s->mg.nlev) FAIL(s, SL_EARG, "sl_pressure_solver_stats: the converged solve has not been chosen");
+  uint32_t ctl[3];
+  double scal[5];
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  HIPCHK(s, hipMemcpy(ctl, s->mg.ctl, sizeof ctl, hipMemcpyDeviceToHost));
+  HIPCHK(s, hipMemcpy(scal, s->mg.scal, sizeof scal, hipMemcpyDeviceToHost));
+  const double h2 = s->h * s->h;
+  if (iters) *iters = ctl[1];
+  if (breakdown) *breakdown = (int)ctl[2];
+  if (resid_per_s) *resid_per_s = scal[3] / h2;
+  if (resid0_per_s) *resid0_per_s = scal[4] / h2;
+  return SL_OK;
+}
+
+// the pressure solve and PROJECT of a substep, by the chosen solver
+static void encode_pressure(sl_sim *s) {
+  if (s->solver == SL_SOLVER_MGPCG) {
+    {
+      StageScope sc(s, SL_T_SOR, 1);
+      sl_mg_solve(s->stream, s->d_tank, s->mg, s->mg_seed ? 1 : 0);
+      s->mg_seed = false;
+    }
+    StageScope sc(s, SL_T_PROJECT, 1);
+    sl_k_project_mg(s->stream, s->d_tank, s->nodes, s->mg.xs);
+    return;
+  }
+  {
+    StageScope sc(s, SL_T_SOR, 2 * s->iters);
+    SlLaunch L = launch_of(s, s->hcells);
+    for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor(L, 0, 2 * i); sl_k_sor(L, 1, 2 * i + 1); }
+  }
+  StageScope sc(s, SL_T_PROJECT, 1);
+  sl_k_project(launch_of(s, s->nodes));
+}
+
 extern "C" int sl_step_begin(sl_sim *s, double frame_dt, uint32_t *nsub, double *dt) {
   if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
   if (!(frame_dt > 0)) { if (nsub) *nsub = 0; if (dt) *dt = 0; return SL_OK; }
@@ -704,12 +794,7 @@ static int encode_substep(sl_sim *s) {
   }
   { StageScope sc(s, SL_T_NORMFORCE, 1); sl_k_normforce(launch_of(s, s->nodes)); }
   { StageScope sc(s, SL_T_DIVERGENCE, 1); sl_k_divsetup(launch_of(s, s->cells)); }
-  {
-    StageScope sc(s, SL_T_SOR, 2 * s->iters);
-    SlLaunch L = launch_of(s, s->hcells);
-    for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor(L, 0, 2 * i); sl_k_sor(L, 1, 2 * i + 1); }
-  }
-  { StageScope sc(s, SL_T_PROJECT, 1); sl_k_project(launch_of(s, s->nodes)); }
+  encode_pressure(s);
   { StageScope sc(s, SL_T_G2P, 1); sl_k_g2p(launch_of(s, pb), s->g2p_variant); }
   return check_launch(s, "substep");
 }
@@ -1244,12 +1329,21 @@ extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
     case SL_S_NORMFORCE: sl_k_normforce(launch_of(s, s->nodes)); break;
     case SL_S_DIVERGENCE: sl_k_divsetup(launch_of(s, s->cells)); break;
     case SL_S_PRESSURE: sl_k_sor(launch_of(s, s->hcells), arg & 1u, 0); break;
-    case SL_S_SOR: {
-      SlLaunch L = launch_of(s, s->hcells);
-      for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor(L, 0, 2 * i); sl_k_sor(L, 1, 2 * i + 1); }
+    case SL_S_SOR:
+      if (s->solver == SL_SOLVER_MGPCG) {
+        sl_mg_solve(s->stream, s->d_tank, s->mg, s->mg_seed ? 1 : 0);
+        s->mg_seed = false;
+        break;
+      }
+      {
+        SlLaunch L = launch_of(s, s->hcells);
+        for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor(L, 0, 2 * i); sl_k_sor(L, 1, 2 * i + 1); }
+      }
       break;
-    }
-    case SL_S_PROJECT: sl_k_project(launch_of(s, s->nodes)); break;
+    case SL_S_PROJECT:
+      if (s->solver == SL_SOLVER_MGPCG) sl_k_project_mg(s->stream, s->d_tank, s->nodes, s->mg.xs);
+      else sl_k_project(launch_of(s, s->nodes));
+      break;
     case SL_S_G2P: sl_k_g2p(launch_of(s, s->p_ub), s->g2p_variant); break;
     case SL_S_COMPACT: return encode_compact(s);
     case SL_S_WW:

@@ -336,6 +336,49 @@ void sl_k_normforce(const SlLaunch &L);
 void sl_k_divsetup(const SlLaunch &L);
 void sl_k_sor(const SlLaunch &L, uint32_t parity, uint32_t pass);
 void sl_k_project(const SlLaunch &L);
+
+// Converged pressure solve (k_mgpcg.hip): multigrid-preconditioned conjugate gradients on the system
+// the SOR sweeps relax. One level of the hierarchy: X x Y x Z cells (x fastest), per cell the couplings
+// on its lower x, y, z faces and its diagonal (0: not an unknown), the level's right-hand side and
+// solution (single precision).
+#define SL_MG_MAXLEV 16
+#define SL_MG_COARSE 1024  // cells of the coarsest level at most (solved in one workgroup)
+struct SlMgLevel {
+  int32_t X, Y, Z;
+  uint32_t cells;
+  uint32_t magX[3], magXY[3];  // udiv_mag by X and by X*Y
+  float4 *A;
+  uint16_t *F;  // per cell: MGF_* bits
+  float *b, *x;
+};
+// per-cell flags of a level: coupling to the neighbour across face q (order -x, +x, -y, +y, -z, +z),
+// open face q (level 0: diag = number of open faces when no open face is fractional), fractional
+// faces present (use the records), unknown, red ((i + j + k) even)
+#define MGF_OPEN0 0
+#define MGF_NUM0 6
+#define MGF_SLOW (1u << 12)
+#define MGF_ACTIVE (1u << 13)
+#define MGF_RED (1u << 14)
+struct SlMg {
+  int nlev = 0;
+  SlMgLevel lev[SL_MG_MAXLEV];
+  // level-0 conjugate-gradient vectors (double): solution (kept between substeps as the warm start),
+  // right-hand side, residual, direction, matrix times direction
+  double *xs = nullptr, *bs = nullptr, *r = nullptr, *p[2] = {nullptr, nullptr}, *q = nullptr;
+  double *part = nullptr;   // per-workgroup partial sums / maxima
+  double *scal = nullptr;   // rho, alpha, beta, max|r| of the last check, max|r| at the start
+  uint32_t *ctl = nullptr;  // done, iterations, breakdown
+  double rtol = 0;          // tolerance on max |b - A x| (h^2 times the divergence tolerance in 1/s)
+  uint32_t max_iters = 0;
+  int coarse_sweeps = 8;
+  float over = 1.6f;  // coarse-correction factor of the V-cycle
+};
+uint32_t sl_mg_parts(uint32_t cells);
+// enqueues the whole solve on `st`: set-up, coarsening, max_iters CG iterations (early returns once
+// converged), and the solution into the SOR pressure layout; `seed`: start from that pressure
+void sl_mg_solve(hipStream_t st, const SlTank *d_tank, const SlMg &mg, int seed);
+// PROJECT with the double-precision solution as the pressure
+void sl_k_project_mg(hipStream_t st, const SlTank *d_tank, uint32_t nodes, const double *xs);
 // particle stages
 void sl_k_g2p(const SlLaunch &L, int variant);
 void sl_k_emit(const SlLaunch &L, uint32_t reserve = 0);

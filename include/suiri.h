@@ -206,6 +206,20 @@ int sl_get_scalar(const sl_sim *sim, const char *name, double *value);
 /* One stage with the current params (stage-level parity). `dt` sets params.dt for the call. */
 int sl_run_stage(sl_sim *sim, int stage, uint32_t arg, double dt);
 
+/* Pressure solver. SL_SOLVER_SOR (the default) is the reference's PRESSURE pass: cfg.iters red-black
+   SOR sweeps (src/js/30-wgsl-sim.js:267-297). SL_SOLVER_MGPCG is a converged solve with no
+   counterpart in the reference, for callers that need the projection's residual small rather than the
+   reference's values: conjugate gradients preconditioned with a multigrid V-cycle on the same system,
+   in double precision, iterated until the largest residual, as a divergence, is at most tol_per_s
+   (1/s) or max_iters (1..1000) iterations ran; PROJECT then reads the double-precision solution. The
+   SL_S_SOR and SL_S_PROJECT stages follow the choice. Not for the tanks of a batch. */
+enum sl_solver { SL_SOLVER_SOR = 0, SL_SOLVER_MGPCG = 1 };
+int sl_set_pressure_solver(sl_sim *sim, int solver, double tol_per_s, uint32_t max_iters);
+/* The last converged solve (waits for the stream): iterations run, 1 if it stopped on a breakdown,
+   the largest residual at the end and at the start, in 1/s. */
+int sl_pressure_solver_stats(sl_sim *sim, uint32_t *iters, int *breakdown, double *resid_per_s,
+                             double *resid0_per_s);
+
 /* Per-stage device timing with HIP events on the sim's stream. */
 enum sl_tstage {
   SL_T_EMIT = 0, SL_T_CLEAR, SL_T_GEOM, SL_T_P2G, SL_T_NORMFORCE, SL_T_DIVERGENCE, SL_T_SOR,
