@@ -351,6 +351,41 @@ static napi_value SetSubsteps(napi_env env, napi_callback_info info) {
   if (r) return throw_sl(env, s, r);
   return nullptr;
 }
+// setPressureSolver(sim, solver, tolPerS, maxIters): 0 = the reference's SOR sweeps, 1 = converged solve
+static napi_value SetPressureSolver(napi_env env, napi_callback_info info) {
+  napi_value argv[4];
+  if (!get_args(env, info, 4, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  int32_t solver = 0;
+  double tol = 0;
+  uint32_t iters = 0;
+  napi_get_value_int32(env, argv[1], &solver);
+  napi_get_value_double(env, argv[2], &tol);
+  napi_get_value_uint32(env, argv[3], &iters);
+  int r = sl_set_pressure_solver(s, solver, tol, iters);
+  if (r) return throw_sl(env, s, r);
+  return nullptr;
+}
+// pressureSolverStats(sim) -> {iters, breakdown, residPerS, resid0PerS}
+static napi_value PressureSolverStats(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  uint32_t it = 0;
+  int br = 0;
+  double r1 = 0, r0 = 0;
+  int r = sl_pressure_solver_stats(s, &it, &br, &r1, &r0);
+  if (r) return throw_sl(env, s, r);
+  napi_value o;
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "iters", num(env, it));
+  napi_set_named_property(env, o, "breakdown", boolean(env, br != 0));
+  napi_set_named_property(env, o, "residPerS", num(env, r1));
+  napi_set_named_property(env, o, "resid0PerS", num(env, r0));
+  return o;
+}
 static napi_value SetRippleMask(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return nullptr;
@@ -825,6 +860,8 @@ static napi_value Init(napi_env env, napi_value exports) {
       {"substep", 0, Substep, 0, 0, 0, napi_default, 0},
       {"stepEnd", 0, StepEnd, 0, 0, 0, napi_default, 0},
       {"setSubsteps", 0, SetSubsteps, 0, 0, 0, napi_default, 0},
+      {"setPressureSolver", 0, SetPressureSolver, 0, 0, 0, napi_default, 0},
+      {"pressureSolverStats", 0, PressureSolverStats, 0, 0, 0, napi_default, 0},
       {"clearWater", 0, ClearWater, 0, 0, 0, napi_default, 0},
       {"saveWater", 0, SaveWater, 0, 0, 0, napi_default, 0},
       {"hasWaterCopy", 0, HasWaterCopy, 0, 0, 0, napi_default, 0},

@@ -141,6 +141,15 @@ class FluidSim {
   // pins the substep count (fixed-dt workloads); 0 restores the adaptive count
   setSubsteps(n) { addon.setSubsteps(this.handle, n); }
 
+  // pressure solver: 'sor' (the reference's sweeps, the default) or 'converged' (multigrid-
+  // preconditioned CG until the largest residual is at most tolPerS 1/s, or maxIters iterations)
+  setPressureSolver(kind, tolPerS = 1e-4, maxIters = 24) {
+    const id = { sor: 0, converged: 1 }[kind];
+    if (id === undefined) throw new Error(`unknown pressure solver '${kind}'`);
+    addon.setPressureSolver(this.handle, id, tolPerS, maxIters);
+  }
+  pressureSolverStats() { return addon.pressureSolverStats(this.handle); }
+
   pollStats() {
     const updated = addon.pollStats(this.handle);
     this._pull();

@@ -79,3 +79,18 @@ def test_js_bake_server_job_equals_plain_fluidsim(tmp_path):
     assert sorted(out["done"]) == ["done", "done"]
     names = sorted(os.listdir(tmp_path))
     assert "frame_00000.npz" in names and "frame_%05d.npz" % frames in names
+
+
+@pytest.mark.gpu
+@pytest.mark.skipif(NODE is None, reason="node not installed")
+def test_js_converged_pressure_solver():
+    """FluidSim.setPressureSolver('converged') through the addon: every frame's solve ends at the
+    tolerance without breakdown, and an unknown solver name is an error."""
+    r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "converged_scene.js"), "6"], capture_output=True,
+                       text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    out = json.loads(r.stdout.strip().splitlines()[-1])
+    assert "unknown pressure solver" in out["unknown"]
+    for f in out["frames"]:
+        assert not f["breakdown"] and 1 <= f["iters"] <= 40 and f["residPerS"] <= 1e-5 < f["resid0PerS"], f
+    assert out["frames"][-1]["count"] == out["frames"][0]["count"] > 0
