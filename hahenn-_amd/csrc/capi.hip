@@ -836,7 +836,7 @@ static int encode_compact(sl_sim *s) {
   SlLaunch L = launch_of(s, s->p_ub, SL_COMPACT_ITEMS);
   sl_k_compact_count(L);
   sl_k_compact_scan(L, 1);
-  sl_k_compact_scatter(L);
+  sl_k_compact_move(L);
   sl_k_count(launch_of(s, 1), 1, 0);
   return check_launch(s, "compact");
 }
@@ -2018,7 +2018,7 @@ static int batch_encode_frame(sl_batch *b, double dt) {
   SlLaunch Lscan = Lc;
   Lscan.w.list = list_dev(b);
   sl_k_compact_scan(Lscan, n);
-  sl_k_compact_scatter(Lc);
+  sl_k_compact_move(Lc);
   sl_k_count(P, 1, 0);
   sl_k_ww(batch_launch(b, BM_WW), (float)dt);  // dt = 0: each tank's own (its substep dt)
   sl_k_count(P, 3, 0);

@@ -411,8 +411,7 @@ __global__ void k_count(SlTank *tanks, const uint32_t *list, int ntanks, uint32_
   SlTank &T = tanks[list ? list[i] : i];
   SlCounters *C = T.C;
   uint32_t cap = T.cap, wcap = f2u(T.ww_cap_f);
-  if (mode == 1) {  // the scatter ran only when a particle died; otherwise the buffer stays current
-    if (C->newCount != min(C->count, cap)) T.cur = 1 - T.cur;
+  if (mode == 1) {  // compaction is in place: the buffer stays current
     C->count = C->newCount;
     C->newCount = 0;
   }
@@ -440,10 +439,14 @@ void sl_k_count(const SlLaunch &L, uint32_t mode, uint32_t y) {
 }
 
 // ------------------------------------------------------------------------------------------------
-// COMPACT, 31-wgsl-sim2.js:317-328, order-preserving: (1) live count per workgroup of
-// SL_COMPACT_ITEMS particles, (2) exclusive scan of the workgroup counts per tank (+ newCount),
-// (3) scatter of the live particles to the other buffer at offset + rank within the workgroup,
-// skipped (and the buffers not flipped, COUNT mode 1) when every particle is alive.
+// COMPACT, 31-wgsl-sim2.js:317-328, in place and deterministic. The reference appends survivors with
+// an atomic counter, so any order of the survivors is one of its outcomes; here the survivors below
+// the new count n' keep their slots and the dead slots below n' are filled, in ascending order, by
+// the survivors at or above n', in ascending order -- only as many particles move as died there.
+// (1) live count per workgroup of SL_COMPACT_ITEMS particles, (2) exclusive scan of the workgroup
+// counts per tank (+ newCount = n'), (3) the dead slots below n' into a list (the P2G binning's
+// particle list, unused at this point), (4) the survivors at or above n' into those slots. (3) and
+// (4) return at once when nothing died.
 // ------------------------------------------------------------------------------------------------
 __global__ __launch_bounds__(SL_WG) void k_compact_count(SlWork W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
@@ -513,30 +516,13 @@ void sl_k_compact_scan(const SlLaunch &L, int ntanks) {
   if (ntanks <= 0) return;
   hipLaunchKernelGGL(k_compact_scan, dim3(ntanks), dim3(1024), 0, L.stream, L.w);
 }
-__global__ __launch_bounds__(SL_WG) void k_compact_scatter(SlWork W) {
-  SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
-  const uint32_t lim = min(T.C->count, T.cap);
-  const uint32_t base = lb * SL_COMPACT_ITEMS;
-  if (base >= lim || T.C->newCount == lim) return;  // nothing died: the order is already compact
-  constexpr int Q = SL_COMPACT_ITEMS / SL_WG;
-  const int cur = T.cur, oth = 1 - cur;
+// live ranks inside a compaction workgroup: for each of its Q items per thread, the number of live
+// particles before it in the workgroup (positions base + q * SL_WG + thread, in position order)
+template <int Q>
+__device__ __forceinline__ void compact_ranks(const uint8_t (&f)[Q], uint32_t (&rank)[Q]) {
   __shared__ uint32_t wtot[SL_COMPACT_ITEMS / 64];
   const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
-  // every in-range particle's state is loaded before the ranks are known (the few dead ones are
-  // read for nothing), so a workgroup has all its loads in flight at once
-  uint8_t f[Q];
-  float r[Q][6];
   uint64_t masks[Q];
-#pragma unroll
-  for (int q = 0; q < Q; q++) {
-    const uint32_t id = base + q * SL_WG + threadIdx.x;
-    const bool in = id < lim;
-    f[q] = in ? ld32(T.pf[cur], id) : (uint8_t)0;
-    const uint32_t k = in ? id : base;
-    r[q][0] = ld32(T.px[cur], k); r[q][1] = ld32(T.py[cur], k); r[q][2] = ld32(T.pz[cur], k);
-    r[q][3] = ld32(T.vx[cur], k); r[q][4] = ld32(T.vy[cur], k); r[q][5] = ld32(T.vz[cur], k);
-  }
 #pragma unroll
   for (int q = 0; q < Q; q++) {
     masks[q] = __ballot((f[q] & 1u) != 0u);
@@ -548,20 +534,71 @@ __global__ __launch_bounds__(SL_WG) void k_compact_scatter(SlWork W) {
     for (int c = 0; c < SL_COMPACT_ITEMS / 64; c++) { uint32_t v = wtot[c]; wtot[c] = s; s += v; }
   }
   __syncthreads();
-  const uint32_t off0 = T.scanBlocks[lb];
   const uint64_t below = lane == 0 ? 0ull : (~0ull >> (64 - lane));
 #pragma unroll
+  for (int q = 0; q < Q; q++) rank[q] = wtot[q * (SL_WG / 64) + wv] + (uint32_t)__popcll(masks[q] & below);
+}
+// the slot after the scan entries that holds the number of survivors below n'
+__device__ __forceinline__ uint32_t compact_slot_L(const SlTank &T) {
+  return (T.cap + SL_COMPACT_ITEMS - 1u) / SL_COMPACT_ITEMS;
+}
+// (3): dead slot p < n' is the (p - live before p)-th hole
+__global__ __launch_bounds__(SL_WG) void k_compact_holes(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t lim = min(T.C->count, T.cap), n1 = T.C->newCount;
+  const uint32_t base = lb * SL_COMPACT_ITEMS;
+  if (n1 >= lim || base > n1) return;  // nothing died, or no slot below n' here
+  constexpr int Q = SL_COMPACT_ITEMS / SL_WG;
+  uint8_t f[Q];
+  uint32_t rank[Q];
+#pragma unroll
   for (int q = 0; q < Q; q++) {
-    if (!(f[q] & 1u)) continue;
-    const uint32_t d = off0 + wtot[q * (SL_WG / 64) + wv] + (uint32_t)__popcll(masks[q] & below);
-    st32(T.px[oth], d, r[q][0]); st32(T.py[oth], d, r[q][1]); st32(T.pz[oth], d, r[q][2]);
-    st32(T.vx[oth], d, r[q][3]); st32(T.vy[oth], d, r[q][4]); st32(T.vz[oth], d, r[q][5]);
-    st32(T.pf[oth], d, f[q]);
+    const uint32_t id = base + q * SL_WG + threadIdx.x;
+    f[q] = id < lim ? ld32(T.pf[T.cur], id) : (uint8_t)0;
+  }
+  compact_ranks<Q>(f, rank);
+  const uint32_t off0 = T.scanBlocks[lb];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const uint32_t id = base + q * SL_WG + threadIdx.x, live_before = off0 + rank[q];
+    if (id < n1 && !(f[q] & 1u)) T.perm[id - live_before] = id;
+    if (id == n1) T.scanBlocks[compact_slot_L(T)] = live_before;
   }
 }
-void sl_k_compact_scatter(const SlLaunch &L) {
+// (4): survivor p >= n' with r survivors in [n', p) moves into the r-th hole
+__global__ __launch_bounds__(SL_WG) void k_compact_fill(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t lim = min(T.C->count, T.cap), n1 = T.C->newCount;
+  const uint32_t base = lb * SL_COMPACT_ITEMS;
+  if (n1 >= lim || base >= lim || base + SL_COMPACT_ITEMS <= n1) return;
+  constexpr int Q = SL_COMPACT_ITEMS / SL_WG;
+  const int cur = T.cur;
+  uint8_t f[Q];
+  uint32_t rank[Q];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const uint32_t id = base + q * SL_WG + threadIdx.x;
+    f[q] = id < lim ? ld32(T.pf[cur], id) : (uint8_t)0;
+  }
+  compact_ranks<Q>(f, rank);
+  const uint32_t off0 = T.scanBlocks[lb], L = T.scanBlocks[compact_slot_L(T)];
+#pragma unroll
+  for (int q = 0; q < Q; q++) {
+    const uint32_t id = base + q * SL_WG + threadIdx.x;
+    if (id < n1 || !(f[q] & 1u)) continue;  // f = 0 beyond lim
+    const uint32_t d = T.perm[off0 + rank[q] - L];
+    st32(T.px[cur], d, ld32(T.px[cur], id)); st32(T.py[cur], d, ld32(T.py[cur], id));
+    st32(T.pz[cur], d, ld32(T.pz[cur], id)); st32(T.vx[cur], d, ld32(T.vx[cur], id));
+    st32(T.vy[cur], d, ld32(T.vy[cur], id)); st32(T.vz[cur], d, ld32(T.vz[cur], id));
+    st32(T.pf[cur], d, f[q]);
+  }
+}
+void sl_k_compact_move(const SlLaunch &L) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_compact_scatter, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  hipLaunchKernelGGL(k_compact_holes, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  hipLaunchKernelGGL(k_compact_fill, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -386,7 +386,7 @@ void sl_k_count(const SlLaunch &L, uint32_t mode, uint32_t y);
 void sl_k_vmax_reset(const SlLaunch &L);
 void sl_k_compact_count(const SlLaunch &L);
 void sl_k_compact_scan(const SlLaunch &L, int ntanks);
-void sl_k_compact_scatter(const SlLaunch &L);
+void sl_k_compact_move(const SlLaunch &L);
 void sl_k_ww(const SlLaunch &L, float dt);
 void sl_k_clear_i32(const SlLaunch &L, int which);
 // spatial re-ordering (rocPRIM radix sort of cell-index keys)

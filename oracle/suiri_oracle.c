@@ -746,17 +746,24 @@ void or_emit_kernel(or_sim *s, uint32_t k, uint32_t total) {
   }
 }
 
-/* COMPACT, src/js/31-wgsl-sim2.js:317-328, restated order-preserving (see header) */
+/* COMPACT, src/js/31-wgsl-sim2.js:317-328, restated in place (see header) */
 void or_compact(or_sim *s) {
-  const or_particle *src = s->parts[s->cur];
-  or_particle *dst = s->parts[1 - s->cur];
+  /* in place: survivors below the new count n' keep their slots; the dead slots below n' take, in
+     ascending order, the survivors at or above n', in ascending order (a deterministic one of the
+     orders the reference's atomic append can produce) */
+  or_particle *p = s->parts[s->cur];
   uint32_t lim = s->C.count < s->P.ext[1] ? s->C.count : s->P.ext[1];
-  uint32_t j = s->C.newCount;
-  for (uint32_t id = 0; id < lim; id++) {
-    if (src[id].pos[3] < 0.5f) continue;
-    dst[j++] = src[id];
+  uint32_t n1 = 0;
+  for (uint32_t id = 0; id < lim; id++)
+    if (p[id].pos[3] >= 0.5f) n1++;
+  uint32_t hole = 0, src = n1;
+  for (;;) {
+    while (hole < n1 && p[hole].pos[3] >= 0.5f) hole++;
+    while (src < lim && p[src].pos[3] < 0.5f) src++;
+    if (hole >= n1 || src >= lim) break;
+    p[hole++] = p[src++];
   }
-  s->C.newCount = j;
+  s->C.newCount = n1;
 }
 
 /* spatial re-ordering: stable counting sort by cell index (x fastest), outside the grid last */
@@ -1285,7 +1292,6 @@ void or_step_substep(or_sim *s, double dt, const or_emit *list, const uint32_t *
 /* frame tail of step(), 50-sim.js:319-335 */
 void or_frame_tail(or_sim *s, double frame_dt) {
   or_compact(s);
-  s->cur = 1 - s->cur;
   or_count(s, 1, 0);
   if (s->q.reorder_every && ++s->reorder_clock >= s->q.reorder_every) {
     or_reorder(s);

@@ -18,8 +18,10 @@
  *
  * One deliberate, documented choice where the reference is nondeterministic: particle compaction
  * (src/js/31-wgsl-sim2.js:317-328 appends survivors with an atomic counter, so the order changes from
- * run to run) is restated as an order-preserving (stable) compaction. The stable order is one of the
- * orders the reference can produce, and it makes every later step reproducible.
+ * run to run) is restated as a deterministic in-place compaction: survivors below the new count n'
+ * keep their slots, and the dead slots below n' take, in ascending order, the survivors at or above
+ * n', in ascending order. That order is one of the orders the reference can produce, and it makes
+ * every later step reproducible.
  */
 #ifndef SUIRI_ORACLE_H
 #define SUIRI_ORACLE_H
@@ -122,7 +124,7 @@ void or_pressure(or_sim *s, uint32_t parity); /* 30-wgsl-sim.js:267-297 */
 void or_project(or_sim *s);                /* 30-wgsl-sim.js:299-341 */
 void or_g2p(or_sim *s);                    /* 31-wgsl-sim2.js:149-251 */
 void or_emit_kernel(or_sim *s, uint32_t k, uint32_t total); /* 31-wgsl-sim2.js:262-315 */
-void or_compact(or_sim *s);                /* 31-wgsl-sim2.js:317-328 (stable order) */
+void or_compact(or_sim *s);                /* 31-wgsl-sim2.js:317-328 (in place, holes filled from the tail) */
 /* stable sort of the live particles by the Morton code of their 4x4x4-cell brick (outside the grid
    last); no counterpart in the reference, whose particle order is arbitrary after compaction */
 void or_reorder(or_sim *s);

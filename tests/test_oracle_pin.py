@@ -683,7 +683,7 @@ def test_substep_matches_the_numpy_port():
     assert abs(vmax - float(fx["vmax"])) <= 1e-4 * float(fx["vmax"])
 
 
-def test_lattice_emit_and_stable_compaction():
+def test_lattice_emit_and_in_place_compaction():
     o = empty_sim(cap=50000)
     o.write_params(1 / 60)
     inf = o.info()
@@ -702,15 +702,31 @@ def test_lattice_emit_and_stable_compaction():
     rel = (parts[:, :3] - np.array(start, np.float32)) / sp - (lat + 0.5)
     assert np.all(np.abs(rel) <= 0.15 + 1e-5)          # jitter of +-0.15 lattice spacings
     assert np.all(parts[:, 3] == 1) and np.allclose(parts[:, 4:7], [0.1, -0.2, 0.3])
-    # kill every third particle, compact: survivors keep their order
-    parts_all = o.field("parts")
-    parts_all[:n:3, 3] = 0
-    keep = parts_all[:n][parts_all[:n, 3] >= 0.5].copy()
-    o.kernel("compact")
-    o.set("cur", 1 - o.info()["cur"])
-    o.kernel("count", 1, 0)
-    assert o.field("counters")[0] == len(keep)
-    assert np.array_equal(o.field("parts")[: len(keep)], keep)
+    # compaction in place: survivors below the new count n' stay; the dead slots below n' take the
+    # survivors at or above n', both in ascending order (any survivor order is an outcome of the
+    # reference's atomic append, 31-wgsl-sim2.js:317-328)
+    before = o.field("parts")[:n].copy()
+    rng = np.random.default_rng(5)
+    for kill in (np.arange(n) % 3 == 0, np.zeros(n, bool), np.arange(n) >= n - 10, rng.random(n) < 0.5,
+                 np.arange(n) < 7):
+        o.field("parts")[:n] = before
+        o.field("parts")[:n, 3] = np.where(kill, 0.0, 1.0)
+        o.field("counters")[0] = n
+        state = o.field("parts")[:n].copy()
+        alive = ~kill
+        n1 = int(alive.sum())
+        holes = np.nonzero(~alive[:n1])[0]
+        tail = n1 + np.nonzero(alive[n1:])[0]
+        want = state.copy()
+        want[holes] = state[tail]
+        o.kernel("compact")
+        o.kernel("count", 1, 0)
+        assert o.field("counters")[0] == n1 and len(holes) == len(tail)
+        got = o.field("parts")[:n1]
+        assert np.array_equal(got, want[:n1])
+        # the same survivors, each once
+        assert np.array_equal(np.sort(got[:, 0] * 1e4 + got[:, 1], kind="stable"),
+                              np.sort(state[alive][:, 0] * 1e4 + state[alive][:, 1], kind="stable"))
 
 
 def test_g2p_uniform_field_and_ballistic():
