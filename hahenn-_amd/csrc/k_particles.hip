@@ -8,39 +8,49 @@
   if ((W).ntanks > 1) { t = tank_of((W).start, (W).ntanks, b); lb = (b) - (W).start[t]; }
 
 // ------------------------------------------------------------------------------------------------
-// mask byte and new and old velocity component of in-grid node (i,j,k)
-__device__ __forceinline__ void node_fetch(const SlTank &T, int comp, int32_t i, int32_t j, int32_t k, uint32_t &m,
-                                           float &vn, float &vo) {
-  const uint32_t n = node_idx(T, i, j, k);
-  m = ld32(T.vmask, n);
-  vn = ld32(T.vel + comp * (size_t)T.nodes, n);
-  vo = ld32(T.velOld + comp * (size_t)T.nodes, n);
-}
 // particle count of an in-grid cell
 __device__ __forceinline__ uint32_t cell_count(const SlTank &T, int32_t x, int32_t y, int32_t z) {
   return ld32(T.cellCount, cell_idx(T, x, y, z));
 }
 
-// masked PIC/FLIP gather, 31-wgsl-sim2.js:9-36 -> (PIC, FLIP, found). The mask byte and both
-// velocities of all 8 corners are loaded up front (axis indices clamped into the grid, so every load
-// stays inside the arrays); corners outside the grid or masked out are then dropped by select, so
-// the sums take the reference's terms in the reference's order.
+// the nodes (n, n + 1) of one array as a pair (one 8-byte load; the arrays are 4-byte aligned)
+__device__ __forceinline__ float2 ld_pair(const float *base, uint32_t n) {
+  float2 r;
+  __builtin_memcpy(&r, __builtin_assume_aligned((const char *)base + n * 4u, 4), 8);
+  return r;
+}
+// masked PIC/FLIP gather, 31-wgsl-sim2.js:9-36 -> (PIC, FLIP, found). The mask bytes and both
+// velocities of all 8 corners are loaded up front: per row (y, z clamped into the grid) the two
+// x-corners are the pair of nodes at b0 = clamp(bx, 0, nx - 1), so the corner at i = nx is the
+// pair's second node and the corner at i = 0 reached from bx = -1 its first; corners outside the
+// grid or masked out are then dropped by select, so the sums take the reference's terms in the
+// reference's order.
 __device__ __forceinline__ f3 transfer1(const SlTank &T, f3 q, int comp, float vp) {
   const int32_t bx = f2i(floorf(q.x)), by = f2i(floorf(q.y)), bz = f2i(floorf(q.z));
   const float fx = q.x - (float)bx, fy = q.y - (float)by, fz = q.z - (float)bz;
   bool ix[2], iy[2], iz[2];
-  int32_t cx[2], cy[2], cz[2];
+  int32_t cy[2], cz[2];
 #pragma unroll
   for (int d = 0; d < 2; d++) {
     const int32_t i = bx + d, j = by + d, k = bz + d;
-    ix[d] = i >= 0 && i <= T.nx; cx[d] = min(max(i, 0), T.nx);
+    ix[d] = i >= 0 && i <= T.nx;
     iy[d] = j >= 0 && j <= T.ny; cy[d] = min(max(j, 0), T.ny);
     iz[d] = k >= 0 && k <= T.nz; cz[d] = min(max(k, 0), T.nz);
   }
+  const int32_t b0 = min(max(bx, 0), T.nx - 1);
+  const bool hi0 = bx == T.nx, lo1 = bx == -1;
+  const float *vn = T.vel + comp * (size_t)T.nodes, *vo = T.velOld + comp * (size_t)T.nodes;
   uint32_t m[8];
   float a[8], b[8];
 #pragma unroll
-  for (int c = 0; c < 8; c++) node_fetch(T, comp, cx[c & 1], cy[(c >> 1) & 1], cz[c >> 2], m[c], a[c], b[c]);
+  for (int r = 0; r < 4; r++) {
+    const uint32_t n = node_idx(T, b0, cy[r & 1], cz[r >> 1]);
+    const uint32_t m0 = ld32(T.vmask, n), m1 = ld32(T.vmask, n + 1u);
+    const float2 pa = ld_pair(vn, n), pb = ld_pair(vo, n);
+    m[2 * r] = hi0 ? m1 : m0; m[2 * r + 1] = lo1 ? m0 : m1;
+    a[2 * r] = hi0 ? pa.y : pa.x; a[2 * r + 1] = lo1 ? pa.x : pa.y;
+    b[2 * r] = hi0 ? pb.y : pb.x; b[2 * r + 1] = lo1 ? pb.x : pb.y;
+  }
   const uint32_t bit = 1u << comp;
   float sn = 0.0f, so = 0.0f, sw = 0.0f;
 #pragma unroll
