@@ -357,7 +357,8 @@ static int tank_buffers(sl_sim *s, SlTank &T, Take take) {
   TAKE(T.args, 64);
   TAKE(T.acc, N * 24);
   TAKE(T.cellCount, C * 4);
-  TAKE(T.vel, N * 12); TAKE(T.velOld, N * 12); TAKE(T.vmask, N);
+  TAKE(T.vel, N * 12); TAKE(T.velOld, N * 12);
+  TAKE(T.vmask, N + 8);  // + 8: G2P reads mask bytes as 8-byte words from their 4-byte-aligned base
   TAKE(T.frac, N * 12); TAKE(T.svel, N * 12);
   TAKE(T.divg, C * 4);
   {

@@ -45,7 +45,12 @@ __device__ __forceinline__ f3 transfer1(const SlTank &T, f3 q, int comp, float v
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const uint32_t n = node_idx(T, b0, cy[r & 1], cz[r >> 1]);
-    const uint32_t m0 = ld32(T.vmask, n), m1 = ld32(T.vmask, n + 1u);
+    // mask bytes n and n + 1 from the 8 bytes at the 4-byte-aligned base below n (the mask array
+    // has 8 bytes of padding, so the word stays inside it)
+    uint2 mw;
+    __builtin_memcpy(&mw, __builtin_assume_aligned(T.vmask + (n & ~3u), 4), 8);
+    const uint64_t mq = ((uint64_t)mw.y << 32 | mw.x) >> (8u * (n & 3u));
+    const uint32_t m0 = (uint32_t)mq & 0xffu, m1 = (uint32_t)(mq >> 8) & 0xffu;
     const float2 pa = ld_pair(vn, n), pb = ld_pair(vo, n);
     m[2 * r] = hi0 ? m1 : m0; m[2 * r + 1] = lo1 ? m0 : m1;
     a[2 * r] = hi0 ? pa.y : pa.x; a[2 * r + 1] = lo1 ? pa.x : pa.y;
