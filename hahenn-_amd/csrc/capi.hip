@@ -747,7 +747,7 @@ static void encode_pressure(sl_sim *s) {
   }
   {
     StageScope sc(s, SL_T_SOR, 2 * s->iters);
-    SlLaunch L = launch_of(s, s->hcells);
+    SlLaunch L = launch_of(s, s->hcells, SL_SOR_WG);
     for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor(L, 0, 2 * i); sl_k_sor(L, 1, 2 * i + 1); }
   }
   StageScope sc(s, SL_T_PROJECT, 1);
@@ -1329,7 +1329,7 @@ extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
       break;
     case SL_S_NORMFORCE: sl_k_normforce(launch_of(s, s->nodes)); break;
     case SL_S_DIVERGENCE: sl_k_divsetup(launch_of(s, s->cells)); break;
-    case SL_S_PRESSURE: sl_k_sor(launch_of(s, s->hcells), arg & 1u, 0); break;
+    case SL_S_PRESSURE: sl_k_sor(launch_of(s, s->hcells, SL_SOR_WG), arg & 1u, 0); break;
     case SL_S_SOR:
       if (s->solver == SL_SOLVER_MGPCG) {
         sl_mg_solve(s->stream, s->d_tank, s->mg, s->mg_seed ? 1 : 0);
@@ -1337,7 +1337,7 @@ extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
         break;
       }
       {
-        SlLaunch L = launch_of(s, s->hcells);
+        SlLaunch L = launch_of(s, s->hcells, SL_SOR_WG);
         for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor(L, 0, 2 * i); sl_k_sor(L, 1, 2 * i + 1); }
       }
       break;
@@ -1770,7 +1770,7 @@ static int batch_rebuild(sl_batch *b) {
     for (uint32_t j = 0; j < b->sor_bands; j++) {
       const uint32_t a = starts[j];
       b->band_start[j] = a;
-      r = put_map(b, BM_SOR0 + (int)j, [](const sl_sim *s) { return (uint64_t)s->hcells; }, SL_WG,
+      r = put_map(b, BM_SOR0 + (int)j, [](const sl_sim *s) { return (uint64_t)s->hcells; }, SL_SOR_WG,
                   [&](uint32_t t) { return b->sims[t]->iters > a; });
       if (r) return r;
     }

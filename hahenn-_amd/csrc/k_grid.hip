@@ -439,9 +439,11 @@ __global__ __launch_bounds__(SL_WG) void k_bin_scatter(SlWork W) {
 // to [-30, 30] and whose weight lies in [0, 1], so f2i's NaN and range cases cannot occur
 __device__ __forceinline__ int32_t fix_term(float x) { return (int32_t)rintf(x); }
 
-// one particle's splat into the LDS window
-__device__ __forceinline__ void p2g_splat(f3 g, const float v3[3], int32_t X0, int32_t Y0, int32_t Z0, int32_t *wm,
-                                          int32_t *ww) {
+// one particle's splat into the LDS window: one 64-bit add per corner of the packed (momentum << 32) +
+// weight term, the global accumulators' format (the weight sums stay below 2^32, so no carry reaches
+// the momentum half)
+__device__ __forceinline__ void p2g_splat(f3 g, const float v3[3], int32_t X0, int32_t Y0, int32_t Z0,
+                                          unsigned long long *wmw) {
 #pragma unroll
   for (int comp = 0; comp < 3; comp++) {
     const f3 q = sub3(g, comp == 0 ? F3(0.0f, 0.5f, 0.5f) : (comp == 1 ? F3(0.5f, 0.0f, 0.5f) : F3(0.5f, 0.5f, 0.0f)));
@@ -459,8 +461,7 @@ __device__ __forceinline__ void p2g_splat(f3 g, const float v3[3], int32_t X0, i
         for (int dx = 0; dx < 2; dx++) {
           const float w = (dx == 1 ? fx : 1.0f - fx) * wy * wz;
           const int e = e0 + (dz * WIN_Y + dy) * WIN_X + dx;
-          atomicAdd(&wm[e], fix_term(w * val * VSCALE));
-          atomicAdd(&ww[e], fix_term(w * VSCALE));
+          atomicAdd(&wmw[e], pack_mw(fix_term(w * val * VSCALE), fix_term(w * VSCALE)));
         }
       }
     }
@@ -474,7 +475,7 @@ __device__ __forceinline__ void p2g_splat(f3 g, const float v3[3], int32_t X0, i
 // loads are in flight together.
 #define P2G_BATCH 4
 __device__ __forceinline__ void p2g_cell(const SlTank &T, int cur, uint32_t start, uint32_t n, int32_t X0, int32_t Y0,
-                                         int32_t Z0, int32_t *wm, int32_t *ww) {
+                                         int32_t Z0, unsigned long long *wmw) {
   for (uint32_t p0 = 0; p0 < n; p0 += P2G_BATCH) {
     uint32_t ids[P2G_BATCH];
 #pragma unroll
@@ -491,7 +492,7 @@ __device__ __forceinline__ void p2g_cell(const SlTank &T, int cur, uint32_t star
     }
 #pragma unroll
     for (int u = 0; u < P2G_BATCH; u++)
-      if (p0 + (uint32_t)u < n) p2g_splat(g[u], v3[u], X0, Y0, Z0, wm, ww);
+      if (p0 + (uint32_t)u < n) p2g_splat(g[u], v3[u], X0, Y0, Z0, wmw);
   }
 }
 
@@ -501,7 +502,7 @@ __device__ __forceinline__ void p2g_cell(const SlTank &T, int cur, uint32_t star
 // compile-time indices and added to the LDS window once per chunk.
 template <int COMP>
 __device__ __forceinline__ void p2g_chunk(const SlTank &T, int cur, uint32_t start, uint32_t n, i3 c, int32_t X0,
-                                          int32_t Y0, int32_t Z0, int32_t *wm, int32_t *ww) {
+                                          int32_t Y0, int32_t Z0, unsigned long long *wmw) {
   constexpr int SX = COMP == 0 ? 2 : 3, SY = COMP == 1 ? 2 : 3, SZ = COMP == 2 ? 2 : 3;
   constexpr int OX = COMP == 0 ? 0 : -1, OY = COMP == 1 ? 0 : -1, OZ = COMP == 2 ? 0 : -1;
   int32_t am[SX * SY * SZ], aw[SX * SY * SZ];
@@ -545,8 +546,7 @@ __device__ __forceinline__ void p2g_chunk(const SlTank &T, int cur, uint32_t sta
         const int e = (rz * SY + ry) * SX + rx;
         if (!(am[e] | aw[e])) continue;
         const int l = ((c.z + OZ + rz - Z0) * WIN_Y + (c.y + OY + ry - Y0)) * WIN_X + (c.x + OX + rx - X0);
-        atomicAdd(&wm[COMP * WIN + l], am[e]);
-        atomicAdd(&ww[COMP * WIN + l], aw[e]);
+        atomicAdd(&wmw[COMP * WIN + l], pack_mw(am[e], aw[e]));
       }
 }
 
@@ -556,9 +556,9 @@ __device__ __forceinline__ void p2g_chunk(const SlTank &T, int cur, uint32_t sta
 __global__ __launch_bounds__(256) void k_p2g_tiles(SlWork W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
-  __shared__ int32_t wm[3 * WIN], ww[3 * WIN];
+  __shared__ unsigned long long wmw[3 * WIN];
   __shared__ uint32_t qstart[P2G_QUEUE], qinfo[P2G_QUEUE], qn;
-  for (int e = threadIdx.x; e < 3 * WIN; e += 256) { wm[e] = 0; ww[e] = 0; }
+  for (int e = threadIdx.x; e < 3 * WIN; e += 256) wmw[e] = 0ull;
   if (threadIdx.x == 0) qn = 0;
   const int ntx = (T.nx + BIN_TX - 1) / BIN_TX, nty = (T.ny + BIN_TY - 1) / BIN_TY;
   const int tx = (int)(lb % (uint32_t)ntx), ty = (int)((lb / (uint32_t)ntx) % (uint32_t)nty);
@@ -578,11 +578,11 @@ __global__ __launch_bounds__(256) void k_p2g_tiles(SlWork W) {
   // one thread per cell for the first P2G_CELL_MAX particles; a crowded cell queues the rest in
   // chunks, so a cell with hundreds of particles (under an emitter) does not hold up the launch
   const uint32_t nmain = min(n, (uint32_t)P2G_CELL_MAX);
-  if (nmain) p2g_cell(T, cur, start, nmain, X0, Y0, Z0, wm, ww);
+  if (nmain) p2g_cell(T, cur, start, nmain, X0, Y0, Z0, wmw);
   for (uint32_t o = nmain; o < n; o += P2G_CELL_MAX) {
     const uint32_t m = min(n - o, (uint32_t)P2G_CELL_MAX), k = atomicAdd(&qn, 1u);
     if (k < P2G_QUEUE) { qstart[k] = start + o; qinfo[k] = (threadIdx.x << 8) | m; }
-    else p2g_cell(T, cur, start + o, m, X0, Y0, Z0, wm, ww);  // queue full: the cell's own thread
+    else p2g_cell(T, cur, start + o, m, X0, Y0, Z0, wmw);  // queue full: the cell's own thread
   }
   __syncthreads();
   const uint32_t nq = min(qn, (uint32_t)P2G_QUEUE);
@@ -590,21 +590,21 @@ __global__ __launch_bounds__(256) void k_p2g_tiles(SlWork W) {
     const uint32_t l = qinfo[k] >> 8, m = qinfo[k] & 255u;
     i3 cc;
     cc.x = tx * BIN_TX + (int)(l & 7u); cc.y = ty * BIN_TY + (int)((l >> 3) & 7u); cc.z = tz * BIN_TZ + (int)(l >> 6);
-    p2g_chunk<0>(T, cur, qstart[k], m, cc, X0, Y0, Z0, wm, ww);
-    p2g_chunk<1>(T, cur, qstart[k], m, cc, X0, Y0, Z0, wm, ww);
-    p2g_chunk<2>(T, cur, qstart[k], m, cc, X0, Y0, Z0, wm, ww);
+    p2g_chunk<0>(T, cur, qstart[k], m, cc, X0, Y0, Z0, wmw);
+    p2g_chunk<1>(T, cur, qstart[k], m, cc, X0, Y0, Z0, wmw);
+    p2g_chunk<2>(T, cur, qstart[k], m, cc, X0, Y0, Z0, wmw);
   }
   __syncthreads();
   unsigned long long *gacc = (unsigned long long *)T.acc;
   const size_t N = T.nodes;
   for (int e = threadIdx.x; e < 3 * WIN; e += 256) {
-    const int32_t m = wm[e], w = ww[e];
-    if (!(m | w)) continue;
+    const unsigned long long v = wmw[e];
+    if (!v) continue;
     const int comp = e / WIN, l = e - comp * WIN;
     const int wz = l / (WIN_X * WIN_Y), r = l - wz * (WIN_X * WIN_Y), wy = r / WIN_X, wx = r - wy * WIN_X;
     const int32_t i = X0 + wx, j = Y0 + wy, k = Z0 + wz;
     if (i < 0 || i > T.nx || j < 0 || j > T.ny || k < 0 || k > T.nz) continue;  // the reference's node range
-    atomicAdd(&gacc[(size_t)comp * N + node_idx(T, i, j, k)], pack_mw(m, w));
+    atomicAdd(&gacc[(size_t)comp * N + node_idx(T, i, j, k)], v);
   }
 }
 
@@ -723,11 +723,11 @@ void sl_k_divsetup(const SlLaunch &L) {
 // Fast path (all open faces have f = 1): num = sum of contributing neighbour pressures in face
 // order, den = number of open faces -- bit-identical to the reference's f*p and f sums.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(SL_WG) void k_sor(SlWork W, uint32_t parity, uint32_t pass) {
+__global__ __launch_bounds__(SL_SOR_WG) void k_sor(SlWork W, uint32_t parity, uint32_t pass) {
   SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
   const SlTank &T = W.tanks[t];
   if (pass >= 2u * T.iters) return;
-  const uint32_t s = lb * SL_WG + threadIdx.x;
+  const uint32_t s = lb * SL_SOR_WG + threadIdx.x;
   const uint32_t H = T.hcells;
   if (s >= H) return;
   const uint32_t hw = (uint32_t)T.hw, rowl = hw * (uint32_t)T.ny;
@@ -772,7 +772,7 @@ __global__ __launch_bounds__(SL_WG) void k_sor(SlWork W, uint32_t parity, uint32
 }
 void sl_k_sor(const SlLaunch &L, uint32_t parity, uint32_t pass) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_sor, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w, parity, pass);
+  hipLaunchKernelGGL(k_sor, dim3(L.w.total), dim3(SL_SOR_WG), 0, L.stream, L.w, parity, pass);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -22,6 +22,7 @@
 #define SL_MAX_EMIT 48
 #endif
 #define SL_WG 256
+#define SL_SOR_WG 256  // slots per SOR workgroup (one per thread; 128 and 1024 measured no better)
 
 // per-cell SOR stencil flags (sflag)
 #define SF_OPEN0 0       // bits 0..5: face open (f > 0), faces -x,+x,-y,+y,-z,+z
