@@ -135,3 +135,32 @@ def test_solver_choice_errors():
         g.set_pressure_solver(suiri.SOLVER_MGPCG, 1e-5, 0)
     with pytest.raises(suiri.SuiriError):
         g.pressure_solver_stats()   # not chosen yet
+
+
+def test_single_level_grid_solves_directly():
+    """A grid of at most 1024 cells is its own coarsest level: the preconditioner is the one-workgroup
+    symmetric sweeps alone. Its pressure still agrees with a direct solve."""
+    sc = twin.small_scene(tank=(0.125, 0.125, 0.125), h=1 / 64, iters=20, whitewater=False, fill_cells=(8, 4, 8),
+                          cap=20000)
+    g = run_to_state(sc, 20)
+    assert g.nx * g.ny * g.nz <= 1024
+    g.set_pressure_solver(suiri.SOLVER_MGPCG, 1e-7, 200)
+    for st in (suiri.S_P2G, suiri.S_NORMFORCE, suiri.S_DIVERGENCE, suiri.S_SOR):
+        g.run_stage(st, 0, 1 / 120)
+    stats = g.pressure_solver_stats()
+    A, b, act = sor_system(g)
+    want = spl.spsolve(A.tocsc(), b)
+    got = g.readback(suiri.F_PRES).reshape(g.nz, g.ny, g.nx).transpose(2, 1, 0)[act].astype(np.float64)
+    print(stats, "unknowns", act.sum(), "max err", np.abs(got - want).max())
+    assert act.sum() > 50 and not stats["breakdown"] and stats["resid_per_s"] <= 1e-7
+    assert np.abs(got - want).max() <= 2e-6 * np.abs(want).max()
+
+
+def test_batch_tanks_refuse_the_converged_solve():
+    cfg = suiri.config("standard", tank_w=0.25, tank_h=0.125, tank_d=0.125, h=1 / 32, cap=20000, ww_cap=2000,
+                       ripple_R=0)
+    b = suiri.Batch([cfg])
+    with pytest.raises(suiri.SuiriError):
+        b.tanks[0].set_pressure_solver(suiri.SOLVER_MGPCG, 1e-5, 20)
+    b.tanks[0].set_pressure_solver(suiri.SOLVER_SOR)
+    b.close()
