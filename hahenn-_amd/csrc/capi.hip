@@ -72,7 +72,7 @@ struct sl_sim {
   uint32_t reorder_clock = 0;
   double batch_dt = 0;    // frame length of a batched tank stepped with dt = 0
   size_t bytes = 0;       // device memory held (sum of the allocation sizes)
-  int p2g_variant = 0;  // kernel variant (chunk size, LDS word size, lane mapping); see k_grid.hip
+  int p2g_variant = SL_P2G_BINNED;  // kernel variant (binned, or chunk size / LDS word / lane mapping); see k_grid.hip
   int g2p_variant = 1;  // occupancy target of G2P's register allocation; see k_particles.hip
   double cur_dt = 1.0 / 60.0;
   int mask_bound = 1;
@@ -1396,6 +1396,11 @@ static float time_region(hipEvent_t a, hipEvent_t b) {
   (void)hipEventElapsedTime(&ms, a, b);
   return ms;
 }
+// First P2G variant the tuners time: above 256 Ki cells only the binned variant, whose cost does not
+// depend on the particle order; the chunked variants measured 10-30x slower there (a tuning trial of
+// one of them on a 4 M-cell tank costs ~10-30 ms of device time).
+static int p2g_first_trial(uint32_t cells) { return cells > 262144u ? SL_P2G_BINNED : 0; }
+
 extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
   if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
   if (!reps) reps = 3;
@@ -1438,7 +1443,7 @@ extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
   // P2G variants: clear, accumulate, time; the accumulators are cleared again afterwards
   float best = 1e30f;
   int bestv = s->p2g_variant;
-  for (int v = 0; v <= SL_P2G_BINNED; v++) {
+  for (int v = p2g_first_trial(s->cells); v <= SL_P2G_BINNED; v++) {
     float tot = 0;
     for (uint32_t r = 0; r <= reps; r++) {
       sl_k_fill_u32(s->stream, (uint32_t *)T.acc, 0, (size_t)s->nodes * 6);
@@ -1568,7 +1573,7 @@ struct sl_batch {
   uint32_t sor_bands = 0, band_start[SL_SOR_BANDS + 1] = {};
   bool any_ripple = false, maps_dirty = true;
   uint32_t max_M = 0;  // largest ripple M of the stepped tanks
-  int p2g_variant[SL_BUCKETS] = {0, 0, 0}, g2p_variant[SL_BUCKETS] = {1, 1, 1};
+  int p2g_variant[SL_BUCKETS] = {SL_P2G_BINNED, SL_P2G_BINNED, SL_P2G_BINNED}, g2p_variant[SL_BUCKETS] = {1, 1, 1};
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
   double graph_dt = 0;
@@ -2256,7 +2261,8 @@ extern "C" int sl_batch_autotune(sl_batch *b, uint32_t reps, uint32_t *choice) {
     if (!has) continue;
     auto in_k = [&](uint32_t t) { return b->bucket[t] == k; };
     float best = 1e30f;
-    for (int v = 0; v <= SL_P2G_BINNED; v++) {
+    // the largest bucket holds the tanks above 256 Ki cells: binned only (p2g_first_trial)
+    for (int v = k == SL_BUCKETS - 1 ? SL_P2G_BINNED : 0; v <= SL_P2G_BINNED; v++) {
       if (v == SL_P2G_BINNED) r = put_bin_maps(b, in_k);
       else r = put_map(b, BM_SCRATCH, [](const sl_sim *s) { return (uint64_t)s->cap; }, sl_p2g_chunk(v), in_k);
       if (r || (r = maps_upload(b))) break;
