@@ -326,11 +326,13 @@ def per_component(name, got, want, tol):
             assert_close(f"{name}[{c}]", got[:, c], want[:, c], tol)
 
 
-def test_ripple_transforms_and_revolve():
-    """One unmasked frame from an arbitrary (asymmetric) state: the forward transform of the source
-    and REVOLVE match the oracle; the inverse transform matches to 1e-5 of each component except where
-    the real part holds only the rounding noise of a 1e5x larger imaginary part."""
-    o, g = twin.make_twin(twin.small_scene())
+@pytest.mark.parametrize("R", [64, 256, 512])
+def test_ripple_transforms_and_revolve(R):
+    """One unmasked frame from an arbitrary (asymmetric) state, at M = 128, 512 and 1024 (each M has
+    its own column tiling): the forward transform of the source and REVOLVE match the oracle; the
+    inverse transform matches to 1e-5 of each component except where the real part holds only the
+    rounding noise of a 1e5x larger imaginary part."""
+    o, g = twin.make_twin(twin.small_scene(R=R))
     md = ripple_inputs(o, g)
     o.set("mask_bound", 0)
     g.set_ripple_mask(0)
@@ -348,10 +350,12 @@ def test_ripple_transforms_and_revolve():
     assert np.all(g.readback(suiri.F_RSRC) == 0)
 
 
-def test_ripple_frames_from_rest():
+@pytest.mark.parametrize("R", [64, 512])
+def test_ripple_frames_from_rest(R):
     """The reference's use: the state starts at rest and is driven by mirrored impulses, frame after
-    frame, with the absorption mask. Height field (texture), mask and state agree to 1e-5."""
-    o, g = twin.make_twin(twin.small_scene())
+    frame, with the absorption mask, at M = 128 and 1024. Height field (texture), mask and state agree
+    to 1e-5."""
+    o, g = twin.make_twin(twin.small_scene(R=R))
     inf = o.info()
     R = int(inf["R"])
     vp, h = inf["vp"], inf["h"]
