@@ -36,7 +36,6 @@ namespace {
 
 constexpr uint32_t MG_WG = 256;
 constexpr uint32_t MG_CWG = 1024;
-constexpr uint32_t NUM_ALL = 0x3fu << MGF_NUM0;
 
 __device__ __forceinline__ bool mg_done(const uint32_t *ctl) { return ctl[0] != 0u; }
 
