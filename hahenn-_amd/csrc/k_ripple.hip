@@ -30,6 +30,36 @@ __device__ __forceinline__ float2 bfly(const float2 *A, uint32_t idx, uint32_t l
   return make_float2(a.x + (wx * b.x - wy * b.y), a.y + (wx * b.y + wy * b.x));
 }
 
+// the butterfly a + w*b of 64-ripple.js:52-54 with twiddle t (its imaginary part times sign)
+__device__ __forceinline__ float2 bf(float2 a, float2 b, float2 t, float sign) {
+  const float wx = t.x, wy = sign * t.y;
+  return make_float2(a.x + (wx * b.x - wy * b.y), a.y + (wx * b.y + wy * b.x));
+}
+// Two consecutive Stockham passes (spans ns = 2^lns and 2 ns, lns + 1 <= lM) in registers. Group g
+// of [0, M/4) takes the first pass's inputs g + {0, 1, 2, 3} M/4: they give its outputs E, E + ns/2
+// (from g, g + M/2) and E + M/2, E + M/2 + ns/2 (from g + M/4, g + 3M/4), E = (g / (ns/2)) ns +
+// g % (ns/2), which are all the inputs of the second pass's outputs j, j + ns/2, j + ns, j + 3ns/2,
+// j = (g / (ns/2)) 2ns + g % (ns/2). Each output is the same butterfly with the same table twiddle
+// as in the pass-by-pass transform, so the result is unchanged; LDS passes and barriers halve.
+__device__ __forceinline__ void bfly4(const float2 *src, float2 *dst, uint32_t g, uint32_t lns, uint32_t lM,
+                                      uint32_t stride, const float2 *tw, float sign) {
+  const uint32_t hs = 1u << (lns - 1u), ns = hs << 1, Q = 1u << (lM - 2u);
+  const uint32_t blk = g >> (lns - 1u), r = g & (hs - 1u);
+  const uint32_t E = blk * ns + r, j = blk * 2u * ns + r;
+  const float2 i0 = src[g * stride], i1 = src[(g + Q) * stride], i2 = src[(g + 2u * Q) * stride];
+  const float2 i3 = src[(g + 3u * Q) * stride];
+  const uint32_t s1 = lM - lns, s2 = lM - lns - 1u;
+  const float2 fE = bf(i0, i2, tw[(E & (ns - 1u)) << s1], sign);
+  const float2 fEh = bf(i0, i2, tw[((E + hs) & (ns - 1u)) << s1], sign);
+  const float2 fM = bf(i1, i3, tw[((E + 2u * Q) & (ns - 1u)) << s1], sign);
+  const float2 fMh = bf(i1, i3, tw[((E + 2u * Q + hs) & (ns - 1u)) << s1], sign);
+  const uint32_t m2 = 2u * ns - 1u;
+  dst[j * stride] = bf(fE, fM, tw[(j & m2) << s2], sign);
+  dst[(j + hs) * stride] = bf(fEh, fMh, tw[((j + hs) & m2) << s2], sign);
+  dst[(j + ns) * stride] = bf(fE, fM, tw[((j + ns) & m2) << s2], sign);
+  dst[(j + ns + hs) * stride] = bf(fEh, fMh, tw[((j + ns + hs) & m2) << s2], sign);
+}
+
 __device__ __forceinline__ float2 fft_input(const SlTank &T, int mode, const float2 *in, uint32_t x, uint32_t y) {
   const uint32_t M = T.M, R = T.R;
   if (mode == IN_BUF) return in[y * M + x];
@@ -59,8 +89,15 @@ __global__ __launch_bounds__(256) void k_fft_rows(SlWork W, int which_in, int wh
   float2 *out = obufs[which_out];
   for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) lds[x] = fft_input(T, mode, in, x, y);
   __syncthreads();
-  uint32_t cur = 0;
-  for (uint32_t lns = 1; lns <= lM; lns++) {
+  uint32_t cur = 0, lns = 1;
+  for (; lns + 1u <= lM; lns += 2u) {
+    const float2 *src = lds + cur * ldsM;
+    float2 *dst = lds + (cur ^ 1u) * ldsM;
+    for (uint32_t g = threadIdx.x; g < (M >> 2); g += blockDim.x) bfly4(src, dst, g, lns, lM, 1u, T.tw, sign);
+    __syncthreads();
+    cur ^= 1u;
+  }
+  if (lns == lM) {  // odd log2 M: the last pass alone
     const float2 *src = lds + cur * ldsM;
     float2 *dst = lds + (cur ^ 1u) * ldsM;
     for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) dst[x] = bfly(src, x, lns, lM, 1, T.tw, sign);
@@ -92,7 +129,16 @@ __global__ __launch_bounds__(256) void k_fft_cols(SlWork W, int which_in, int wh
   }
   __syncthreads();
   int cur = 0;
-  for (uint32_t lns = 1; lns <= lM; lns++) {
+  uint32_t lns = 1;
+  for (; lns + 1u <= lM; lns += 2u) {
+    for (uint32_t q = threadIdx.x; q < (n >> 2); q += blockDim.x) {
+      const uint32_t c = q & (CW - 1u), g = q >> lCW;
+      bfly4(lds[cur] + c, lds[cur ^ 1] + c, g, lns, lM, CW, T.tw, sign);
+    }
+    __syncthreads();
+    cur ^= 1;
+  }
+  if (lns == lM) {  // odd log2 M: the last pass alone
     for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
       const uint32_t c = q & (CW - 1u), y = q >> lCW;
       lds[cur ^ 1][q] = bfly(lds[cur] + c, y, lns, lM, CW, T.tw, sign);
