@@ -30,6 +30,7 @@ sys.path.insert(0, ROOT)
 suiri = importlib.import_module("hahenn-_amd")
 scene = importlib.import_module("hahenn-_amd.scene")
 workloads = importlib.import_module("hahenn-_amd.workloads")
+residual = importlib.import_module("hahenn-_amd.residual")
 
 HBM_PEAK_GBS = 8000.0
 
@@ -123,18 +124,26 @@ def projection_residual(g, dt):
     projected field gives, per liquid cell, its divergence minus the drift source the solve aimed at
     (src/js/30-wgsl-sim.js:240-264), in 1/s. Taken over liquid cells with an open face; a liquid cell
     whose six faces are all closed (inside an obstacle) has no pressure equation, so it is counted
-    apart."""
+    apart. `f64` evaluates the same formula in double precision from the stored fields (the
+    divergence of the field itself, without the rounding of the f32 formula's subtractions);
+    `above_f32_floor` is what remains of it above the per-cell bound of storing each face velocity
+    in f32 (f * ulp(v) / 2 per face)."""
     g.run_stage(suiri.S_DIVERGENCE, 0, dt)
-    nx, ny, nz = g.nx, g.ny, g.nz
-    liquid = g.readback(suiri.F_CELLTYPE) == 1
-    fr = g.readback(suiri.F_FRAC).reshape(nz + 1, ny + 1, nx + 1, 4)
-    fx, fy, fz = fr[:nz, :ny, :, 0] > 0, fr[:nz, :, :nx, 1] > 0, fr[:, :ny, :nx, 2] > 0
-    has_open = (fx[:, :, :-1] | fx[:, :, 1:] | fy[:, :-1] | fy[:, 1:] | fz[:-1] | fz[1:]).ravel()
-    r = np.abs(g.readback(suiri.F_DIVG)[liquid & has_open].astype(np.float64))
-    return {"unit": "1/s", "cells": int(r.size), "max": float(r.max()) if r.size else 0.0,
-            "p99": float(np.percentile(r, 99)) if r.size else 0.0,
-            "rms": float(np.sqrt((r ** 2).mean())) if r.size else 0.0,
-            "closed_liquid_cells": int((liquid & ~has_open).sum())}
+    div, liquid, has_open, floor = residual.divergence_f64(g)
+    sel = (liquid & has_open).ravel()
+
+    def stats(r):
+        return {"max": float(r.max()) if r.size else 0.0, "p99": float(np.percentile(r, 99)) if r.size else 0.0,
+                "rms": float(np.sqrt((r ** 2).mean())) if r.size else 0.0}
+
+    out = {"unit": "1/s", "cells": int(sel.sum()), "closed_liquid_cells": int((liquid & ~has_open).sum())}
+    out.update(stats(np.abs(g.readback(suiri.F_DIVG)[sel].astype(np.float64))))
+    d = np.abs(div.ravel()[sel])
+    fl = floor.ravel()[sel]
+    out["f64"] = stats(d)
+    out["f64"]["above_f32_floor"] = stats(np.maximum(d - fl, 0.0))
+    out["f64"]["f32_floor_max"] = float(fl.max()) if fl.size else 0.0
+    return out
 
 
 def converged_leg(wl, steps, warmup, tol, max_iters):

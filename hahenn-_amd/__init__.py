@@ -249,6 +249,7 @@ class Sim:
     def __init__(self, cfg, _handle=None, _owner=None):
         self.L = load()
         self.ptr = ctypes.c_void_p()
+        self.cfg = cfg
         self._owner = _owner  # a Batch owns its tanks: they are destroyed with it
         if _handle is not None:
             self.ptr = ctypes.c_void_p(_handle)
@@ -419,7 +420,7 @@ class Batch:
         for i in range(self.capacity):
             h = self.L.sl_batch_sim(self.ptr, i)
             if h:
-                self.tanks[i] = Sim(None, _handle=h, _owner=self)
+                self.tanks[i] = Sim(cfgs[i] if cfgs is not None else None, _handle=h, _owner=self)
 
     def _check(self, r):
         if r != SL_OK:
@@ -432,7 +433,7 @@ class Batch:
     def add(self, cfg):
         slot = ctypes.c_uint32(0)
         self._check(self.L.sl_batch_add(self.ptr, ctypes.byref(cfg), ctypes.byref(slot)))
-        self.tanks[slot.value] = Sim(None, _handle=self.L.sl_batch_sim(self.ptr, slot.value), _owner=self)
+        self.tanks[slot.value] = Sim(cfg, _handle=self.L.sl_batch_sim(self.ptr, slot.value), _owner=self)
         return slot.value
 
     def remove(self, slot):

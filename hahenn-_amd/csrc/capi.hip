@@ -693,6 +693,7 @@ static int mg_alloc(sl_sim *s) {
   const size_t C = s->cells;
   ALLOC(m.xs, C * 8); ALLOC(m.bs, C * 8); ALLOC(m.r, C * 8); ALLOC(m.p[0], C * 8); ALLOC(m.p[1], C * 8);
   ALLOC(m.q, C * 8);
+  ALLOC(m.dlo, C * 4);
   ALLOC(m.part, (size_t)sl_mg_parts(s->cells) * 8);
   ALLOC(m.scal, 8 * 8);
   ALLOC(m.ctl, 8 * 4);

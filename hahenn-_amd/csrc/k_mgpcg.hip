@@ -50,6 +50,11 @@ __device__ __forceinline__ void cell_ijk(const SlMgLevel &L, uint32_t c, uint32_
 __device__ __forceinline__ float diag_of(const SlMgLevel &L, uint32_t c, uint32_t fl) {
   return (fl & MGF_SLOW) ? L.A[c].w : (float)__builtin_popcount(fl & 0x3fu);
 }
+// the level-0 diagonal of the double-precision operator: the single-precision record plus the
+// residue that makes it the exact sum of the open faces' fractions
+__device__ __forceinline__ double diag0(const SlMgLevel &L, const float *dlo, uint32_t c, uint32_t fl) {
+  return (fl & MGF_SLOW) ? (double)L.A[c].w + (double)dlo[c] : (double)__builtin_popcount(fl & 0x3fu);
+}
 
 // sum over the coupled neighbours n of c: w_cn * v(n), in the type R
 template <typename R, typename Get>
@@ -92,7 +97,7 @@ __device__ __forceinline__ T wg_reduce(T v, T *sh, uint32_t n, bool is_max) {
 // Level 0 records and flags from the face fractions and the cell counts; right-hand side -h^2 div; the
 // warm start (the previous solution, or the SOR pressure when `seed`), 0 on cells that are not unknowns.
 __global__ __launch_bounds__(MG_WG) void k_mg_setup0(const SlTank *Tp, SlMgLevel L, double *xs, double *bs,
-                                                     uint32_t *ctl, int seed) {
+                                                     float *dlo, uint32_t *ctl, int seed) {
   const SlTank &T = *Tp;
   const uint32_t c = blockIdx.x * MG_WG + threadIdx.x;
   if (c == 0u) { ctl[0] = 0u; ctl[1] = 0u; ctl[2] = 0u; }
@@ -109,18 +114,20 @@ __global__ __launch_bounds__(MG_WG) void k_mg_setup0(const SlTank *Tp, SlMgLevel
     const uint32_t X = (uint32_t)L.X, XY = X * (uint32_t)L.Y;
     const bool in[6] = {i > 0u, i + 1u < X, j > 0u, j + 1u < (uint32_t)L.Y, k > 0u, k + 1u < (uint32_t)L.Z};
     const uint32_t off[6] = {c - 1u, c + 1u, c - X, c + X, c - XY, c + XY};
-    float den = 0.0f;
+    double den_d = 0.0;  // exact: six single-precision fractions in [0, 1]
 #pragma unroll
     for (int q = 0; q < 6; q++) {
       if (!(f[q] > 0.0f)) continue;
-      den += f[q];
+      den_d += (double)f[q];
       fl |= 1u << (MGF_OPEN0 + q);
       if (f[q] != 1.0f) fl |= MGF_SLOW;
       // an open face to a liquid neighbour makes that neighbour an unknown too (its den >= f > 0)
       if (in[q] && T.cellCount[off[q]] > 0u) fl |= 1u << (MGF_NUM0 + q);
     }
+    const float den = (float)den_d;
     if (den > 0.0f) {
       fl |= MGF_ACTIVE;
+      dlo[c] = (float)(den_d - (double)den);
       a = make_float4((fl & (1u << (MGF_NUM0 + 0))) ? f[0] : 0.0f, (fl & (1u << (MGF_NUM0 + 2))) ? f[2] : 0.0f,
                       (fl & (1u << (MGF_NUM0 + 4))) ? f[4] : 0.0f, den);
     } else {
@@ -130,7 +137,24 @@ __global__ __launch_bounds__(MG_WG) void k_mg_setup0(const SlTank *Tp, SlMgLevel
   L.A[c] = a;
   L.F[c] = (uint16_t)fl;
   if (fl & MGF_ACTIVE) {
-    bs[c] = -(T.h_d * T.h_d) * (double)T.divg[c];
+    // the divergence of DIVERGENCE (30-wgsl-sim.js:240-264) evaluated in double from the stored
+    // velocities, fractions and solid velocities: the single-precision one carries rounding of the
+    // order of ulp(flux) / h from its subtractions, above the tolerances this solve is asked for
+    const size_t N = T.nodes;
+    const uint32_t n0 = node_idx(T, (int32_t)i, (int32_t)j, (int32_t)k);
+    const uint32_t nn[6] = {n0, n0 + 1u, n0, n0 + (uint32_t)(T.nx + 1), n0,
+                            n0 + (uint32_t)((T.nx + 1) * (T.ny + 1))};
+    double d = 0.0;
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const size_t e = (size_t)(q >> 1) * N + nn[q];
+      const double fq = (double)T.frac[e];
+      const double fl_q = fq * (double)T.vel[e] + (1.0 - fq) * (double)T.svel[e];
+      d += (q & 1) ? fl_q : -fl_q;
+    }
+    const double crowd = (double)T.cellCount[c] / (double)T.ppc - 1.0;
+    const double dv = d / T.h_d - (double)T.drift * fmax(crowd, 0.0);
+    bs[c] = -(T.h_d * T.h_d) * dv;
     if (seed) xs[c] = (double)T.pres[colour_base(T, (int32_t)i, (int32_t)j, (int32_t)k) + packed_idx(T, (int32_t)i, (int32_t)j, (int32_t)k)];
   } else {
     bs[c] = 0.0;
@@ -326,7 +350,7 @@ __device__ __forceinline__ void store_residual(const SlMgLevel &L, uint32_t c, u
 
 // r = b - A x
 __global__ __launch_bounds__(MG_WG) void k_cg_resid(SlMgLevel L, const double *xs, const double *bs, double *r,
-                                                    double *part) {
+                                                    double *part, const float *dlo) {
   __shared__ double sh[MG_WG];
   const uint32_t c = blockIdx.x * MG_WG + threadIdx.x;
   double m = 0.0;
@@ -334,7 +358,7 @@ __global__ __launch_bounds__(MG_WG) void k_cg_resid(SlMgLevel L, const double *x
     const uint32_t fl = L.F[c];
     double rc = 0.0;
     if (fl & MGF_ACTIVE)
-      rc = bs[c] + nbr_sum<double>(L, c, fl, [&](uint32_t n) { return xs[n]; }) - (double)diag_of(L, c, fl) * xs[c];
+      rc = bs[c] + nbr_sum<double>(L, c, fl, [&](uint32_t n) { return xs[n]; }) - diag0(L, dlo, c, fl) * xs[c];
     // every cell once per solve: cells that stopped being unknowns drop what they held
     store_residual(L, c, fl, r, rc);
     m = fabs(rc);
@@ -411,7 +435,7 @@ __global__ __launch_bounds__(MG_CWG) void k_cg_beta(const double *part, uint32_t
 // p' = z + beta p (in a second buffer, so that neighbours read the old p), q = A p', and the
 // per-workgroup partial of sum p' q
 __global__ __launch_bounds__(MG_WG) void k_cg_spmv(SlMgLevel L, const double *pold, double *pnew, double *q, double *part,
-                                                   const double *scal, const uint32_t *ctl) {
+                                                   const double *scal, const uint32_t *ctl, const float *dlo) {
   __shared__ double sh[MG_WG];
   if (mg_done(ctl)) return;
   const uint32_t c = blockIdx.x * MG_WG + threadIdx.x;
@@ -423,7 +447,7 @@ __global__ __launch_bounds__(MG_WG) void k_cg_spmv(SlMgLevel L, const double *po
       const float *z = L.x;
       auto pv = [&](uint32_t n) { return (double)z[n] + beta * pold[n]; };
       const double pc = pv(c);
-      const double qc = (double)diag_of(L, c, fl) * pc - nbr_sum<double>(L, c, fl, pv);
+      const double qc = diag0(L, dlo, c, fl) * pc - nbr_sum<double>(L, c, fl, pv);
       pnew[c] = pc;
       q[c] = qc;
       v = pc * qc;
@@ -568,19 +592,19 @@ uint32_t sl_mg_parts(uint32_t cells) { return blocks(cells, MG_WG); }
 void sl_mg_solve(hipStream_t st, const SlTank *d_tank, const SlMg &mg, int seed) {
   const SlMgLevel &L0 = mg.lev[0];
   const uint32_t nb = blocks(L0.cells, MG_WG);
-  hipLaunchKernelGGL(k_mg_setup0, dim3(nb), dim3(MG_WG), 0, st, d_tank, L0, mg.xs, mg.bs, mg.ctl, seed);
+  hipLaunchKernelGGL(k_mg_setup0, dim3(nb), dim3(MG_WG), 0, st, d_tank, L0, mg.xs, mg.bs, mg.dlo, mg.ctl, seed);
   for (int l = 0; l + 1 < mg.nlev; l++) {
     const uint32_t nc = blocks(mg.lev[l + 1].cells, MG_WG);
     hipLaunchKernelGGL(k_mg_coarsen, dim3(nc), dim3(MG_WG), 0, st, mg.lev[l], mg.lev[l + 1]);
     hipLaunchKernelGGL(k_mg_flags, dim3(nc), dim3(MG_WG), 0, st, mg.lev[l + 1]);
   }
-  hipLaunchKernelGGL(k_cg_resid, dim3(nb), dim3(MG_WG), 0, st, L0, mg.xs, mg.bs, mg.r, mg.part);
+  hipLaunchKernelGGL(k_cg_resid, dim3(nb), dim3(MG_WG), 0, st, L0, mg.xs, mg.bs, mg.r, mg.part, mg.dlo);
   hipLaunchKernelGGL(k_cg_check, dim3(1), dim3(MG_CWG), 0, st, mg.part, nb, mg.scal, mg.ctl, mg.rtol, 0u);
   precondition(st, mg);
   hipLaunchKernelGGL(k_cg_beta, dim3(1), dim3(MG_CWG), 0, st, mg.part, nb, mg.scal, mg.ctl, 1);
   for (uint32_t it = 1; it <= mg.max_iters; it++) {
     double *pold = mg.p[(it + 1) & 1u], *pnew = mg.p[it & 1u];
-    hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(MG_WG), 0, st, L0, pold, pnew, mg.q, mg.part, mg.scal, mg.ctl);
+    hipLaunchKernelGGL(k_cg_spmv, dim3(nb), dim3(MG_WG), 0, st, L0, pold, pnew, mg.q, mg.part, mg.scal, mg.ctl, mg.dlo);
     hipLaunchKernelGGL(k_cg_alpha, dim3(1), dim3(MG_CWG), 0, st, mg.part, nb, mg.scal, mg.ctl);
     hipLaunchKernelGGL(k_cg_update, dim3(nb), dim3(MG_WG), 0, st, L0, mg.xs, mg.r, pnew, mg.q, mg.part, mg.scal,
                        mg.ctl);

@@ -367,6 +367,7 @@ struct SlMg {
   // right-hand side, residual, direction, matrix times direction
   double *xs = nullptr, *bs = nullptr, *r = nullptr, *p[2] = {nullptr, nullptr}, *q = nullptr;
   double *part = nullptr;   // per-workgroup partial sums / maxima
+  float *dlo = nullptr;     // per cut cell: exact sum of its open fractions minus the level-0 record's diagonal
   double *scal = nullptr;   // rho, alpha, beta, max|r| of the last check, max|r| at the start
   uint32_t *ctl = nullptr;  // done, iterations, breakdown
   double rtol = 0;          // tolerance on max |b - A x| (h^2 times the divergence tolerance in 1/s)
