@@ -259,8 +259,11 @@ def batched_leg(n_tanks, steps, warmup, seq_steps, graph_steps=50, seed=7):
     }
 
 
+KIND_NAMES = {0: "dam-break 64x32x32", 1: "faucet 256x128x128", 3: "batched-mix tanks"}
+
+
 def serve_leg(rate=20.0, window=3.0, limit=45.0, steps=(100, 2000), kinds=(0, 1, 3), seed=11, budget_gb=200,
-              max_jobs=128):
+              max_jobs=128, frames_per_tick=4):
     """Serving stream: scene-bake requests arrive as a Poisson process (`rate` per second for `window`
     seconds), each scene drawn from the 64x32x32 dam break (0), the 256x128x128 faucet (1) and the
     batched-tank mix (3), each with 100..2000 frames; admitted against a `budget_gb` device-memory
@@ -281,7 +284,8 @@ def serve_leg(rate=20.0, window=3.0, limit=45.0, steps=(100, 2000), kinds=(0, 1,
         wl = workloads.dam_break() if kind == 0 else (workloads.faucet() if kind == 1 else
                                                         workloads.mix_tank(seed * 7919 + i))
         reqs.append(dict(t=ta, kind=kind, steps=n, wl=wl))
-    srv = suiri.Server(budget=int(budget_gb * (1 << 30)), max_jobs=max_jobs, autotune_after=8)
+    srv = suiri.Server(budget=int(budget_gb * (1 << 30)), max_jobs=max_jobs, autotune_after=8,
+                       frames_per_tick=frames_per_tick)
     ids = {}
     t0 = time.perf_counter()
     i = 0
@@ -339,9 +343,10 @@ def serve_leg(rate=20.0, window=3.0, limit=45.0, steps=(100, 2000), kinds=(0, 1,
     seq_ps = sum(r["steps"] * per_kind[r["kind"]][1] for r in reqs)
     served = st["particle_steps"] / elapsed
     return {
-        "workload": f"Poisson {rate}/s for {window} s, scenes from dam-break 64x32x32 / faucet 256x128x128 / "
-                    f"batched-mix tanks, {steps[0]}-{steps[1]} frames each, {budget_gb} GB budget, "
-                    f"stopped at {limit} s or when drained",
+        "workload": f"Poisson {rate}/s for {window} s, scenes from "
+                    + " / ".join(KIND_NAMES[k] for k in kinds)
+                    + f", {steps[0]}-{steps[1]} frames each, {budget_gb} GB budget, up to {frames_per_tick} "
+                    f"frames per tick, stopped at {limit} s or when drained",
         "requests": len(reqs), "completed": len(done),
         "unfinished": len(reqs) - len(done) - sum(x["status"] in ("cancelled", "refused") for x in infos),
         "refused": sum(x["status"] == "refused" for x in infos),
@@ -349,6 +354,7 @@ def serve_leg(rate=20.0, window=3.0, limit=45.0, steps=(100, 2000), kinds=(0, 1,
         "value": served, "unit": "particle-steps/s",
         "frames": st["frames"],
         "frame_latency_ms": {"p50": st["step_ms_p50"], "p99": st["step_ms_p99"]},
+        "device_busy": st["step_ms_sum"] / 1e3 / elapsed,
         "request_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
                               "of": "completed requests"},
         "sequential_estimate": {"value": seq_ps / seq_t, "unit": "particle-steps/s",
@@ -382,6 +388,7 @@ def main():
     ap.add_argument("--serve-window", type=float, default=3.0)
     ap.add_argument("--serve-limit", type=float, default=45.0)
     ap.add_argument("--serve-kinds", default="0,1,3")
+    ap.add_argument("--serve-fpt", type=int, default=4, help="serving: frames per tick")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -401,7 +408,8 @@ def main():
         sys.exit(2)
     if args.only == "serve":
         print(json.dumps(serve_leg(args.serve_rate, args.serve_window, args.serve_limit,
-                                   kinds=tuple(int(k) for k in args.serve_kinds.split(",")))))
+                                   kinds=tuple(int(k) for k in args.serve_kinds.split(",")),
+                                  frames_per_tick=args.serve_fpt)))
         return
     if args.only == "batched":
         print(json.dumps(batched_leg(args.batch_tanks, args.batch_steps, 5, args.seq_steps, args.graph_steps)))
@@ -483,7 +491,12 @@ def main():
         out["batched"] = batched_leg(args.batch_tanks, args.batch_steps, 5, args.seq_steps, args.graph_steps)
         out["batched_p99_step_ms"] = out["batched"]["step_latency_ms"]["p99"]
         out["served"] = serve_leg(args.serve_rate, args.serve_window, args.serve_limit,
-                                  kinds=tuple(int(k) for k in args.serve_kinds.split(",")))
+                                  kinds=tuple(int(k) for k in args.serve_kinds.split(",")),
+                                  frames_per_tick=args.serve_fpt)
+        # the same stream without the 256x128x128 faucet jobs, each of which fills the GPU on its own:
+        # the scenes that a continuous batch can run side by side
+        out["served_small_scenes"] = serve_leg(args.serve_rate, args.serve_window, args.serve_limit, kinds=(0, 3),
+                                               frames_per_tick=args.serve_fpt)
     if not args.no_pmc:
         t = pmc_traffic(os.path.join(ROOT, "gpurun_out", "bench_pmc"))
         if t:

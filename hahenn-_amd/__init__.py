@@ -96,7 +96,7 @@ EMIT_FN = ctypes.CFUNCTYPE(ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ct
 class SlServerConfig(ctypes.Structure):
     _fields_ = [("hbm_budget", ctypes.c_uint64), ("max_jobs", ctypes.c_uint32), ("device", ctypes.c_int32),
                 ("autotune_after", ctypes.c_uint32), ("graph_after", ctypes.c_uint32),
-                ("reserved", ctypes.c_uint32 * 4)]
+                ("frames_per_tick", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 3)]
 
 
 class SlJob(ctypes.Structure):
@@ -519,15 +519,18 @@ def _info_dict(i):
 
 class Server:
     """The serving loop (sl_server_*): jobs admitted against a device-memory budget, stepped together
-    as a continuous batch, one frame per tick. `source(dt, t)` returns a job's per-frame emission
-    records (like the emitter callback of Sim.step)."""
+    as a continuous batch, one frame per tick (up to `frames_per_tick` back to back while no job
+    finishes or bakes). `source(dt, t)` returns a job's per-frame emission records (like the emitter
+    callback of Sim.step)."""
 
-    def __init__(self, budget=200 << 30, max_jobs=128, device=0, autotune_after=8, graph_after=0):
+    def __init__(self, budget=200 << 30, max_jobs=128, device=0, autotune_after=8, graph_after=0,
+                 frames_per_tick=4):
         self.L = load()
         c = SlServerConfig()
         self.L.sl_server_config_default(ctypes.byref(c))
         c.hbm_budget, c.max_jobs, c.device = int(budget), int(max_jobs), int(device)
         c.autotune_after, c.graph_after = int(autotune_after), int(graph_after)
+        c.frames_per_tick = int(frames_per_tick)
         self.ptr = ctypes.c_void_p()
         r = self.L.sl_server_create(ctypes.byref(c), ctypes.byref(self.ptr))
         if r != SL_OK:

@@ -614,6 +614,7 @@ static napi_value ServerCreate(napi_env env, napi_callback_info info) {
     c.device = (int32_t)get_num(env, argv[0], "device", c.device);
     c.autotune_after = (uint32_t)get_num(env, argv[0], "autotuneAfter", c.autotune_after);
     c.graph_after = (uint32_t)get_num(env, argv[0], "graphAfter", c.graph_after);
+    c.frames_per_tick = (uint32_t)get_num(env, argv[0], "framesPerTick", c.frames_per_tick);
   }
   sl_server *srv = nullptr;
   int r = sl_server_create(&c, &srv);

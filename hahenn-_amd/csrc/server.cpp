@@ -101,11 +101,12 @@ extern "C" int sl_server_config_default(sl_server_config *c) {
   c->device = 0;
   c->autotune_after = 8;
   c->graph_after = 0;
+  c->frames_per_tick = 4;
   return SL_OK;
 }
 
 extern "C" int sl_server_create(const sl_server_config *cfg, sl_server **out) {
-  if (!cfg || !out || !cfg->max_jobs) return SL_EARG;
+  if (!cfg || !out || !cfg->max_jobs || cfg->frames_per_tick > SL_SERVER_MAX_FRAMES_PER_TICK) return SL_EARG;
   *out = nullptr;
   sl_server *s = new sl_server();
   s->cfg = *cfg;
@@ -210,9 +211,10 @@ extern "C" int sl_server_tick(sl_server *s, uint32_t *running) {
   // 1. the frame enqueued by the previous tick has completed
   if (s->pending) {
     if ((r = sl_batch_sync(s->batch))) return s->fail(r, sl_batch_last_error(s->batch));
-    double ms[4];
+    // the device time of every frame the previous tick enqueued (at most frames_per_tick)
+    double ms[SL_SERVER_MAX_FRAMES_PER_TICK];
     uint32_t n = 0;
-    sl_batch_timing_read(s->batch, ms, 4, &n);
+    sl_batch_timing_read(s->batch, ms, SL_SERVER_MAX_FRAMES_PER_TICK, &n);
     for (uint32_t i = 0; i < n; i++) s->step_ms.push_back(ms[i]);
     s->pending = false;
   }
@@ -279,38 +281,49 @@ extern "C" int sl_server_tick(sl_server *s, uint32_t *running) {
     it = s->queue.erase(it);
     if (!j.bake_dir.empty() && (r = bake(s, j, 0))) return r;  // the start frame, like bake.py:208-214
   }
-  // 4. per-frame emission of every running job with a source, in one batched emission
-  uint32_t nrun = 0;
-  s->emit_buf.clear();
-  bool any_emit = false;
+  // 4. the frames of this tick: one, or up to frames_per_tick back to back while no running job
+  // reaches its last frame or a bake frame inside them (the host work of a tick then overlaps the
+  // frames already enqueued instead of leaving the device idle after every frame)
+  uint32_t burst = std::max<uint32_t>(1u, s->cfg.frames_per_tick), nrun = 0;
   for (uint32_t slot = 0; slot < s->cfg.max_jobs; slot++) {
-    s->emit_counts[slot] = 0;
     if (!s->slot_job[slot]) continue;
-    Job &j = s->jobs[s->slot_job[slot]];
+    const Job &j = s->jobs[s->slot_job[slot]];
     if (j.status != SL_JOB_RUNNING) continue;
     nrun++;
-    if (!j.spec.source) continue;
-    sl_sim *sim = sl_batch_sim(s->batch, slot);
-    double time = 0;
-    sl_get_scalar(sim, "time", &time);
-    sl_emit list[SL_MAX_EMIT];
-    uint32_t n = j.spec.source(j.spec.user, sim, j.spec.dt, time, list, SL_MAX_EMIT);
-    n = std::min<uint32_t>(n, SL_MAX_EMIT);
-    s->emit_buf.insert(s->emit_buf.end(), list, list + n);
-    s->emit_counts[slot] = n;
-    any_emit |= n > 0;
+    burst = std::min(burst, j.spec.steps > j.frames ? j.spec.steps - j.frames : 1u);
+    if (!j.bake_dir.empty() && j.spec.bake_every) burst = std::min(burst, j.spec.bake_every - j.frames % j.spec.bake_every);
   }
-  if (any_emit && (r = sl_batch_emit(s->batch, s->emit_buf.data(), s->emit_counts.data())))
-    return s->fail(r, sl_batch_last_error(s->batch));
-  // 5. one frame of every running job
-  if (nrun) {
-    if (s->cfg.autotune_after && !s->tuned && s->stable >= s->cfg.autotune_after) {
-      uint32_t ch[6];
-      if ((r = sl_batch_autotune(s->batch, 1, ch))) return s->fail(r, sl_batch_last_error(s->batch));
-      s->tuned = true;
+  for (uint32_t f = 0; nrun && f < burst; f++) {
+    // per-frame emission of every running job with a source, in one batched emission
+    s->emit_buf.clear();
+    bool any_emit = false;
+    for (uint32_t slot = 0; slot < s->cfg.max_jobs; slot++) {
+      s->emit_counts[slot] = 0;
+      if (!s->slot_job[slot]) continue;
+      Job &j = s->jobs[s->slot_job[slot]];
+      if (j.status != SL_JOB_RUNNING || !j.spec.source) continue;
+      sl_sim *sim = sl_batch_sim(s->batch, slot);
+      double time = 0;
+      sl_get_scalar(sim, "time", &time);
+      sl_emit list[SL_MAX_EMIT];
+      uint32_t n = j.spec.source(j.spec.user, sim, j.spec.dt, time, list, SL_MAX_EMIT);
+      n = std::min<uint32_t>(n, SL_MAX_EMIT);
+      s->emit_buf.insert(s->emit_buf.end(), list, list + n);
+      s->emit_counts[slot] = n;
+      any_emit |= n > 0;
     }
-    if (s->cfg.graph_after && s->stable == s->cfg.graph_after && (r = sl_batch_capture(s->batch, 0)))
+    if (any_emit && (r = sl_batch_emit(s->batch, s->emit_buf.data(), s->emit_counts.data())))
       return s->fail(r, sl_batch_last_error(s->batch));
+    // one frame of every running job
+    if (f == 0) {
+      if (s->cfg.autotune_after && !s->tuned && s->stable >= s->cfg.autotune_after) {
+        uint32_t ch[6];
+        if ((r = sl_batch_autotune(s->batch, 1, ch))) return s->fail(r, sl_batch_last_error(s->batch));
+        s->tuned = true;
+      }
+      if (s->cfg.graph_after && s->stable == s->cfg.graph_after && (r = sl_batch_capture(s->batch, 0)))
+        return s->fail(r, sl_batch_last_error(s->batch));
+    }
     if ((r = sl_batch_step(s->batch, 0))) return s->fail(r, sl_batch_last_error(s->batch));
     s->pending = true;
     s->frames++;

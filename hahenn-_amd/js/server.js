@@ -9,7 +9,7 @@ const { seedStream } = require('./util');
 const emit = require('./emit');
 
 class BakeServer {
-  // opts: {budget (bytes), maxJobs, device, autotuneAfter, graphAfter}
+  // opts: {budget (bytes), maxJobs, device, autotuneAfter, graphAfter, framesPerTick}
   constructor(opts = {}) {
     this.handle = addon.serverCreate(opts);
     this.jobs = new Map();

@@ -302,13 +302,18 @@ size_t sl_job_bytes(const sl_config *cfg);
    State stays resident on the device from admission to retirement (or to sl_server_release for
    jobs submitted with keep = 1). A job's result does not depend on the other jobs. */
 typedef struct sl_server sl_server;
+#define SL_SERVER_MAX_FRAMES_PER_TICK 64
 typedef struct {
   uint64_t hbm_budget;      /* device bytes the resident jobs may hold (sl_job_bytes each) */
   uint32_t max_jobs;        /* batch slots */
   int32_t device;
   uint32_t autotune_after;  /* ticks of unchanged membership before the one variant tuning (0 = never) */
   uint32_t graph_after;     /* capture the frame after this many unchanged ticks (0 = eager launches) */
-  uint32_t reserved[4];
+  uint32_t frames_per_tick; /* frames one tick enqueues back to back (0 = 1, default 4, at most
+                               SL_SERVER_MAX_FRAMES_PER_TICK); a tick stops early at any running
+                               job's last frame or bake frame, so retirement and bakes keep their
+                               frames, and jobs submitted meanwhile wait at most this many */
+  uint32_t reserved[3];
 } sl_server_config;
 typedef struct {
   sl_config cfg;            /* the tank */
@@ -352,7 +357,8 @@ const char *sl_server_last_error(const sl_server *srv);
 int sl_server_submit(sl_server *srv, const sl_job *job, uint64_t *id);
 /* cancels a queued or running job at the next tick (its memory is freed then) */
 int sl_server_cancel(sl_server *srv, uint64_t id);
-/* one scheduling round and one frame of every running job; *running = jobs stepped */
+/* one scheduling round and one frame (up to frames_per_tick frames) of every running job;
+   *running = jobs stepped */
 int sl_server_tick(sl_server *srv, uint32_t *running);
 /* jobs retired (done, cancelled, refused) since the last poll */
 int sl_server_poll(sl_server *srv, sl_job_info *out, uint32_t max, uint32_t *n);

@@ -81,8 +81,11 @@ def test_job_bytes_is_what_creation_allocates():
         sim.close()
 
 
-def test_job_alone_equals_job_in_mix():
-    print("server: job alone", flush=True)
+@pytest.mark.parametrize("frames_per_tick", [1, 4])
+def test_job_alone_equals_job_in_mix(frames_per_tick):
+    """Bit-identical results alone and in a mix, also when ticks enqueue several frames back to back
+    (the mix's jobs end at 9, 7, 12 and 5 frames, so bursts stop at every job's last frame)."""
+    print(f"server: job alone (mix at {frames_per_tick} frames per tick)", flush=True)
     alone = suiri.Server(budget=4 << 30, max_jobs=4, autotune_after=0)
     ja = submit_j(alone)
     run_until_idle(alone)
@@ -96,7 +99,7 @@ def test_job_alone_equals_job_in_mix():
     cfg_b = tank_cfg(0.5, 0.3, 0.3, 1 / 64, iters=40, R=0, reorder=0)  # 32x19x19
     cfg_c = tank_cfg(0.25, 0.25, 0.25, 1 / 64, iters=24)
     budget = suiri.job_bytes(J["cfg"]()) + suiri.job_bytes(cfg_a) + suiri.job_bytes(cfg_c) + (1 << 20)
-    mix = suiri.Server(budget=budget, max_jobs=3, autotune_after=3)
+    mix = suiri.Server(budget=budget, max_jobs=3, autotune_after=3, frames_per_tick=frames_per_tick)
     ids = {}
     arrivals = [
         (0, lambda: ids.__setitem__("a", mix.submit(cfg_a, fill=fill_block(cfg_a, 20, 8, 16, 5), steps=9,
