@@ -35,6 +35,11 @@ residual = importlib.import_module("hahenn-_amd.residual")
 HBM_PEAK_GBS = 8000.0
 
 
+def progress(msg):
+    """One line on stderr as each leg ends (stdout carries only the result line)."""
+    print(f"bench: {msg}", file=sys.stderr, flush=True)
+
+
 def build_sim(wl):
     g = suiri.Sim(suiri.config("high", **wl["cfg"]))
     workloads.setup(g, wl)
@@ -284,8 +289,7 @@ def serve_leg(rate=20.0, window=3.0, limit=45.0, steps=(100, 2000), kinds=(0, 1,
         wl = workloads.dam_break() if kind == 0 else (workloads.faucet() if kind == 1 else
                                                         workloads.mix_tank(seed * 7919 + i))
         reqs.append(dict(t=ta, kind=kind, steps=n, wl=wl))
-    srv = suiri.Server(budget=int(budget_gb * (1 << 30)), max_jobs=max_jobs, autotune_after=8,
-                       frames_per_tick=frames_per_tick)
+    srv = suiri.Server(budget=int(budget_gb * (1 << 30)), max_jobs=max_jobs, frames_per_tick=frames_per_tick)
     ids = {}
     t0 = time.perf_counter()
     i = 0
@@ -486,17 +490,22 @@ def main():
         "tuned_variants": tuned,
     }
     g.close()
+    progress(f"headline done: {out['steps_per_s']:.1f} steps/s")
     if args.only == "all":
         out["converged"] = converged_leg(wl, args.conv_steps, args.warmup, args.conv_tol, args.conv_iters)
+        progress("converged solve done")
         out["batched"] = batched_leg(args.batch_tanks, args.batch_steps, 5, args.seq_steps, args.graph_steps)
         out["batched_p99_step_ms"] = out["batched"]["step_latency_ms"]["p99"]
+        progress("batched tanks done")
         out["served"] = serve_leg(args.serve_rate, args.serve_window, args.serve_limit,
                                   kinds=tuple(int(k) for k in args.serve_kinds.split(",")),
                                   frames_per_tick=args.serve_fpt)
+        progress("served stream done")
         # the same stream without the 256x128x128 faucet jobs, each of which fills the GPU on its own:
         # the scenes that a continuous batch can run side by side
         out["served_small_scenes"] = serve_leg(args.serve_rate, args.serve_window, args.serve_limit, kinds=(0, 3),
                                                frames_per_tick=args.serve_fpt)
+        progress("served small-scene stream done")
     if not args.no_pmc:
         t = pmc_traffic(os.path.join(ROOT, "gpurun_out", "bench_pmc"))
         if t:
@@ -508,6 +517,7 @@ def main():
             rl = out["roofline"]
             rl["traffic_GBps"] = t["bytes"] / (rl["avg_launch_us"] * 1e-6) / 1e9
             rl["traffic_frac"] = rl["traffic_GBps"] / HBM_PEAK_GBS
+        progress("counter runs done")
     if not args.no_cpu:
         threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
         v, secs = cpu_baseline(wl, args.cpu_steps, threads)

@@ -523,7 +523,7 @@ class Server:
     finishes or bakes). `source(dt, t)` returns a job's per-frame emission records (like the emitter
     callback of Sim.step)."""
 
-    def __init__(self, budget=200 << 30, max_jobs=128, device=0, autotune_after=8, graph_after=0,
+    def __init__(self, budget=200 << 30, max_jobs=128, device=0, autotune_after=0, graph_after=0,
                  frames_per_tick=4):
         self.L = load()
         c = SlServerConfig()

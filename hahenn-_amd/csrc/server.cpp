@@ -99,7 +99,11 @@ extern "C" int sl_server_config_default(sl_server_config *c) {
   c->hbm_budget = 200ull << 30;  // 200 GiB of the device's 288 GB
   c->max_jobs = 128;
   c->device = 0;
-  c->autotune_after = 8;
+  // no variant tuning by default: the one tuning runs on the jobs resident at that moment, and on a
+  // stream of short jobs those are freshly filled tanks whose particles are still lattice-ordered, on
+  // which chunked P2G variants win and which they then lose to the binned default as the particles
+  // scatter (a stream of small scenes ran 45% slower per frame after such a choice)
+  c->autotune_after = 0;
   c->graph_after = 0;
   c->frames_per_tick = 4;
   return SL_OK;

@@ -307,7 +307,8 @@ typedef struct {
   uint64_t hbm_budget;      /* device bytes the resident jobs may hold (sl_job_bytes each) */
   uint32_t max_jobs;        /* batch slots */
   int32_t device;
-  uint32_t autotune_after;  /* ticks of unchanged membership before the one variant tuning (0 = never) */
+  uint32_t autotune_after;  /* ticks of unchanged membership before the one variant tuning (0 = never,
+                               the default) */
   uint32_t graph_after;     /* capture the frame after this many unchanged ticks (0 = eager launches) */
   uint32_t frames_per_tick; /* frames one tick enqueues back to back (0 = 1, default 4, at most
                                SL_SERVER_MAX_FRAMES_PER_TICK); a tick stops early at any running
