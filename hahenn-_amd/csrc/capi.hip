@@ -776,7 +776,7 @@ extern "C" int sl_step_begin(sl_sim *s, double frame_dt, uint32_t *nsub, double 
 static void encode_p2g(sl_sim *s, int variant) {
   const uint32_t pb = s->p_ub;
   if (variant == SL_P2G_BINNED)
-    sl_k_p2g_binned(launch_of(s, pb), launch_of(s, sl_bin_blocks(s->cells), 1), launch_of(s, 1, 1),
+    sl_k_p2g_binned(launch_of(s, pb, SL_BIN_PER_WG), launch_of(s, sl_bin_blocks(s->cells), 1), launch_of(s, 1, 1),
                     launch_of(s, sl_bin_tiles(s->nx, s->ny, s->nz), 1));
   else
     sl_k_p2g(launch_of(s, pb, sl_p2g_chunk(variant)), 0, variant);
@@ -1690,7 +1690,7 @@ static uint32_t cols_of(const sl_sim *s) {
 template <class Inc>
 static int put_bin_maps(sl_batch *b, Inc inc) {
   int r;
-  if ((r = put_map(b, BM_BINP, [](const sl_sim *s) { return (uint64_t)s->cap; }, SL_WG, inc)) ||
+  if ((r = put_map(b, BM_BINP, [](const sl_sim *s) { return (uint64_t)s->cap; }, SL_BIN_PER_WG, inc)) ||
       (r = put_map(b, BM_BINS, [](const sl_sim *s) { return (uint64_t)sl_bin_blocks(s->cells); }, 1, inc)) ||
       (r = put_map(b, BM_BINT, [](const sl_sim *) { return (uint64_t)1; }, 1, inc)) ||
       (r = put_map(b, BM_BINTILE, [](const sl_sim *s) { return (uint64_t)sl_bin_tiles(s->nx, s->ny, s->nz); }, 1, inc)))

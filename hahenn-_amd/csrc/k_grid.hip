@@ -299,37 +299,58 @@ void sl_k_p2g(const SlLaunch &L, int count_only, int variant) {
 
 // The cell of each live in-grid particle (P2G's own test) and its rank among the cell's particles.
 // Lanes of a wavefront that hold the same cell in a run (the usual case in a spatially ordered
-// particle array) take their ranks with one atomic.
+// particle array) take their ranks with one atomic. A wavefront takes SL_BIN_CHUNKS consecutive runs
+// of 64 particles: their flag and position loads are issued together, and so are their atomics, so
+// the kernel waits on memory a few times per wavefront rather than a few times per 64 particles.
 __global__ __launch_bounds__(SL_WG) void k_bin_count(SlWork W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
-  const uint32_t id = lb * SL_WG + threadIdx.x;
-  const bool in = id < min(T.C->count, T.cap);
+  const uint32_t n = min(T.C->count, T.cap);
   const int cur = T.cur;
-  uint32_t cell = 0xffffffffu;
-  if (in && (T.pf[cur][id] & 1u)) {
-    const f3 g = grid_pos(T, F3(T.px[cur][id], T.py[cur][id], T.pz[cur][id]));
-    if (in_grid(T, g)) {
-      const i3 c = toi3(g);
-      cell = cell_idx(T, c.x, c.y, c.z);
+  const int lane = __lane_id();
+  const uint32_t base_id = lb * SL_BIN_PER_WG + (threadIdx.x >> 6) * (64u * SL_BIN_CHUNKS) + (uint32_t)lane;
+  uint32_t cell[SL_BIN_CHUNKS];
+#pragma unroll
+  for (int q = 0; q < SL_BIN_CHUNKS; q++) {
+    const uint32_t id = base_id + 64u * q;
+    const bool in = id < n;
+    // loaded for every particle below n (the flag decides afterwards), so no load waits on another
+    const uint8_t pf = in ? T.pf[cur][id] : (uint8_t)0;
+    const f3 p = in ? F3(T.px[cur][id], T.py[cur][id], T.pz[cur][id]) : F3(0.0f, 0.0f, 0.0f);
+    cell[q] = 0xffffffffu;
+    if (pf & 1u) {
+      const f3 g = grid_pos(T, p);
+      if (in_grid(T, g)) {
+        const i3 c = toi3(g);
+        cell[q] = cell_idx(T, c.x, c.y, c.z);
+      }
     }
   }
   // runs: a lane starts one unless it holds the same valid cell as the lane before it, so every run
   // is either one lane without a cell or lanes of one cell; the run's first lane takes the ranks
-  const int lane = __lane_id();
-  const uint32_t prev = __shfl_up(cell, 1, 64);
-  const bool valid = cell != 0xffffffffu;
-  const uint64_t starts = __ballot(!valid || lane == 0 || prev != cell);
   const uint64_t upto = lane == 63 ? ~0ull : ((2ull << lane) - 1ull);
-  const int head = 63 - __clzll((long long)(starts & upto));
-  const uint64_t above = starts & ~upto;
-  const int next = above ? __ffsll((unsigned long long)above) - 1 : 64;
-  uint32_t base = 0;
-  if (valid && head == lane) base = atomicAdd(&T.cellCount[cell], (uint32_t)(next - lane));
-  base = __shfl(base, head, 64);
-  if (!in) return;
-  T.binCell[id] = cell;
-  T.binRank[id] = base + (uint32_t)(lane - head);
+  int head[SL_BIN_CHUNKS];
+  uint32_t base[SL_BIN_CHUNKS];
+#pragma unroll
+  for (int q = 0; q < SL_BIN_CHUNKS; q++) {
+    const uint32_t prev = __shfl_up(cell[q], 1, 64);
+    const bool valid = cell[q] != 0xffffffffu;
+    const uint64_t starts = __ballot(!valid || lane == 0 || prev != cell[q]);
+    head[q] = 63 - __clzll((long long)(starts & upto));
+    const uint64_t above = starts & ~upto;
+    const int next = above ? __ffsll((unsigned long long)above) - 1 : 64;
+    base[q] = 0;
+    if (valid && head[q] == lane) base[q] = atomicAdd(&T.cellCount[cell[q]], (uint32_t)(next - lane));
+  }
+#pragma unroll
+  for (int q = 0; q < SL_BIN_CHUNKS; q++) {
+    const uint32_t b = __shfl(base[q], head[q], 64);
+    const uint32_t id = base_id + 64u * q;
+    if (id < n) {
+      T.binCell[id] = cell[q];
+      T.binRank[id] = b + (uint32_t)(lane - head[q]);
+    }
+  }
 }
 
 // exclusive prefix of the cell counts inside each block of BIN_SCAN cells; the block total goes to
@@ -425,14 +446,24 @@ __global__ __launch_bounds__(1024) void k_bin_top(SlWork W) {
 }
 
 // particle index -> its slot in cell order
+// SL_BIN_CHUNKS runs of 64 particles per wavefront, as k_bin_count
 __global__ __launch_bounds__(SL_WG) void k_bin_scatter(SlWork W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
-  const uint32_t id = lb * SL_WG + threadIdx.x;
-  if (id >= min(T.C->count, T.cap)) return;
-  const uint32_t cell = T.binCell[id];
-  if (cell == 0xffffffffu) return;
-  T.perm[T.cellStart[cell] + T.binTop[cell / BIN_SCAN] + T.binRank[id]] = id;
+  const uint32_t n = min(T.C->count, T.cap);
+  const uint32_t base_id = lb * SL_BIN_PER_WG + (threadIdx.x >> 6) * (64u * SL_BIN_CHUNKS) + (uint32_t)__lane_id();
+  uint32_t cell[SL_BIN_CHUNKS], rank[SL_BIN_CHUNKS];
+#pragma unroll
+  for (int q = 0; q < SL_BIN_CHUNKS; q++) {
+    const uint32_t id = base_id + 64u * q;
+    cell[q] = id < n ? T.binCell[id] : 0xffffffffu;
+    rank[q] = id < n ? T.binRank[id] : 0u;
+  }
+#pragma unroll
+  for (int q = 0; q < SL_BIN_CHUNKS; q++) {
+    if (cell[q] == 0xffffffffu) continue;
+    T.perm[T.cellStart[cell[q]] + T.binTop[cell[q] / BIN_SCAN] + rank[q]] = base_id + 64u * q;
+  }
 }
 
 // round(x) to i32 for |x| <= 30 * 2^18 and x not NaN: the splat terms below, whose velocity is clamped

@@ -22,6 +22,10 @@
 #define SL_MAX_EMIT 48
 #endif
 #define SL_WG 256
+// particles per workgroup of the binned P2G's per-particle kernels (k_bin_count, k_bin_scatter): each
+// wavefront takes SL_BIN_CHUNKS consecutive runs of 64 particles and issues all their loads at once
+#define SL_BIN_CHUNKS 4
+#define SL_BIN_PER_WG (SL_WG * SL_BIN_CHUNKS)
 #define SL_SOR_WG 256  // slots per SOR workgroup (one per thread; 128 and 1024 measured no better)
 
 // per-cell SOR stencil flags (sflag)
