@@ -303,7 +303,7 @@ void sl_k_p2g(const SlLaunch &L, int count_only, int variant) {
 // of 64 particles: their flag and position loads are issued together, and so are their atomics, so
 // the kernel waits on memory a few times per wavefront rather than a few times per 64 particles.
 __global__ __launch_bounds__(SL_WG) void k_bin_count(SlWork W) {
-  SL_TANK_OF(W, blockIdx.x, t, lb);
+  SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
   const SlTank &T = W.tanks[t];
   const uint32_t n = min(T.C->count, T.cap);
   const int cur = T.cur;
@@ -448,7 +448,7 @@ __global__ __launch_bounds__(1024) void k_bin_top(SlWork W) {
 // particle index -> its slot in cell order
 // SL_BIN_CHUNKS runs of 64 particles per wavefront, as k_bin_count
 __global__ __launch_bounds__(SL_WG) void k_bin_scatter(SlWork W) {
-  SL_TANK_OF(W, blockIdx.x, t, lb);
+  SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
   const SlTank &T = W.tanks[t];
   const uint32_t n = min(T.C->count, T.cap);
   const uint32_t base_id = lb * SL_BIN_PER_WG + (threadIdx.x >> 6) * (64u * SL_BIN_CHUNKS) + (uint32_t)__lane_id();
@@ -585,7 +585,7 @@ __device__ __forceinline__ void p2g_chunk(const SlTank &T, int cur, uint32_t sta
 #define P2G_QUEUE 512     // chunk queue of a workgroup
 
 __global__ __launch_bounds__(256) void k_p2g_tiles(SlWork W) {
-  SL_TANK_OF(W, blockIdx.x, t, lb);
+  SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
   const SlTank &T = W.tanks[t];
   __shared__ unsigned long long wmw[3 * WIN];
   __shared__ uint32_t qstart[P2G_QUEUE], qinfo[P2G_QUEUE], qn;
