@@ -14,6 +14,7 @@
   if ((W).ntanks > 1) { t = tank_of((W).start, (W).ntanks, b); lb = (b) - (W).start[t]; }
 
 #define FFT_LDS_COMPLEX 4096  // complex values per LDS buffer (two buffers = 64 KiB)
+#define FFT_LOADS (FFT_LDS_COMPLEX / 256)  // loads per thread of a 256-thread FFT workgroup, at most
 
 enum { IN_BUF = 0, IN_SRC = 1, IN_MASK = 2 };
 enum { OUT_BUF = 0, OUT_SCALE = 1 };
@@ -87,7 +88,20 @@ __global__ __launch_bounds__(256) void k_fft_rows(SlWork W, int which_in, int wh
   float2 *obufs[5] = {T.rstate, T.ra, T.rb, T.rhat, nullptr};
   const float2 *in = bufs[which_in];
   float2 *out = obufs[which_out];
-  for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) lds[x] = fft_input(T, mode, in, x, y);
+  // every load of the thread is issued before the first LDS write (M <= FFT_LDS_COMPLEX = 16 * 256)
+  {
+    float2 v[FFT_LOADS];
+#pragma unroll
+    for (int k = 0; k < FFT_LOADS; k++) {
+      const uint32_t x = threadIdx.x + 256u * k;
+      if (x < M) v[k] = fft_input(T, mode, in, x, y);
+    }
+#pragma unroll
+    for (int k = 0; k < FFT_LOADS; k++) {
+      const uint32_t x = threadIdx.x + 256u * k;
+      if (x < M) lds[x] = v[k];
+    }
+  }
   __syncthreads();
   uint32_t cur = 0, lns = 1;
   for (; lns + 1u <= lM; lns += 2u) {
@@ -109,7 +123,8 @@ __global__ __launch_bounds__(256) void k_fft_rows(SlWork W, int which_in, int wh
 
 // all vertical passes of a tile of CW = 4096/M adjacent columns
 __global__ __launch_bounds__(256) void k_fft_cols(SlWork W, int which_in, int which_out, int omode, float sign) {
-  SL_TANK_OF(W, blockIdx.x, t, lb);
+  // adjacent column tiles share the 128-byte lines of every row: XCD-aware order keeps them in one L2
+  SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
   const SlTank &T = W.tanks[t];
   const uint32_t M = T.M, lM = T.logM;
   const uint32_t lCW = lM <= 6u ? lM : 12u - lM, CW = 1u << lCW;  // FFT_LDS_COMPLEX / M, at most M
@@ -122,10 +137,20 @@ __global__ __launch_bounds__(256) void k_fft_cols(SlWork W, int which_in, int wh
   const float2 *in = bufs[which_in];
   float2 *out = obufs[which_out];
   const uint32_t n = CW * M;
-  // LDS layout: column c at offset c, row y at y*CW (stride CW)
-  for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
-    const uint32_t c = q & (CW - 1u), y = q >> lCW;
-    lds[0][q] = in[y * M + x0 + c];
+  // LDS layout: column c at offset c, row y at y*CW (stride CW); every load of the thread is issued
+  // before the first LDS write (n <= FFT_LDS_COMPLEX = 16 * 256)
+  {
+    float2 v[FFT_LOADS];
+#pragma unroll
+    for (int k = 0; k < FFT_LOADS; k++) {
+      const uint32_t q = threadIdx.x + 256u * k;
+      if (q < n) v[k] = in[(q >> lCW) * M + x0 + (q & (CW - 1u))];
+    }
+#pragma unroll
+    for (int k = 0; k < FFT_LOADS; k++) {
+      const uint32_t q = threadIdx.x + 256u * k;
+      if (q < n) lds[0][q] = v[k];
+    }
   }
   __syncthreads();
   int cur = 0;
