@@ -583,13 +583,15 @@ __device__ __forceinline__ void p2g_chunk(const SlTank &T, int cur, uint32_t sta
 
 #define P2G_CELL_MAX 16   // particles one thread walks directly; the rest of a crowded cell is chunked
 #define P2G_QUEUE 512     // chunk queue of a workgroup
+#define P2G_TILE_THREADS (BIN_TX * BIN_TY * BIN_TZ)  // one thread per cell of the tile
+static_assert(BIN_TX == 8 && BIN_TY == 8, "the tile kernel decodes a thread as 8 x 8 cells per z layer");
 
-__global__ __launch_bounds__(256) void k_p2g_tiles(SlWork W) {
+__global__ __launch_bounds__(P2G_TILE_THREADS) void k_p2g_tiles(SlWork W) {
   SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
   const SlTank &T = W.tanks[t];
   __shared__ unsigned long long wmw[3 * WIN];
   __shared__ uint32_t qstart[P2G_QUEUE], qinfo[P2G_QUEUE], qn;
-  for (int e = threadIdx.x; e < 3 * WIN; e += 256) wmw[e] = 0ull;
+  for (int e = threadIdx.x; e < 3 * WIN; e += P2G_TILE_THREADS) wmw[e] = 0ull;
   if (threadIdx.x == 0) qn = 0;
   const int ntx = (T.nx + BIN_TX - 1) / BIN_TX, nty = (T.ny + BIN_TY - 1) / BIN_TY;
   const int tx = (int)(lb % (uint32_t)ntx), ty = (int)((lb / (uint32_t)ntx) % (uint32_t)nty);
@@ -617,7 +619,7 @@ __global__ __launch_bounds__(256) void k_p2g_tiles(SlWork W) {
   }
   __syncthreads();
   const uint32_t nq = min(qn, (uint32_t)P2G_QUEUE);
-  for (uint32_t k = threadIdx.x; k < nq; k += 256) {
+  for (uint32_t k = threadIdx.x; k < nq; k += P2G_TILE_THREADS) {
     const uint32_t l = qinfo[k] >> 8, m = qinfo[k] & 255u;
     i3 cc;
     cc.x = tx * BIN_TX + (int)(l & 7u); cc.y = ty * BIN_TY + (int)((l >> 3) & 7u); cc.z = tz * BIN_TZ + (int)(l >> 6);
@@ -628,7 +630,7 @@ __global__ __launch_bounds__(256) void k_p2g_tiles(SlWork W) {
   __syncthreads();
   unsigned long long *gacc = (unsigned long long *)T.acc;
   const size_t N = T.nodes;
-  for (int e = threadIdx.x; e < 3 * WIN; e += 256) {
+  for (int e = threadIdx.x; e < 3 * WIN; e += P2G_TILE_THREADS) {
     const unsigned long long v = wmw[e];
     if (!v) continue;
     const int comp = e / WIN, l = e - comp * WIN;
@@ -650,7 +652,7 @@ void sl_k_p2g_binned(const SlLaunch &parts, const SlLaunch &blocks, const SlLaun
   hipLaunchKernelGGL(k_bin_scan, dim3(blocks.w.total), dim3(256), 0, parts.stream, blocks.w);
   hipLaunchKernelGGL(k_bin_top, dim3(tanks.w.total), dim3(1024), 0, parts.stream, tanks.w);
   hipLaunchKernelGGL(k_bin_scatter, dim3(parts.w.total), dim3(SL_WG), 0, parts.stream, parts.w);
-  hipLaunchKernelGGL(k_p2g_tiles, dim3(tiles.w.total), dim3(256), 0, parts.stream, tiles.w);
+  hipLaunchKernelGGL(k_p2g_tiles, dim3(tiles.w.total), dim3(P2G_TILE_THREADS), 0, parts.stream, tiles.w);
 }
 
 // ------------------------------------------------------------------------------------------------
