@@ -83,7 +83,8 @@ __global__ __launch_bounds__(256) void k_fft_rows(SlWork W, int which_in, int wh
   const uint32_t M = T.M, lM = T.logM, y = lb;
   if (y >= M) return;
   if (mode == IN_MASK && !T.mask_bound) return;  // tanks of a batch without the absorbing mask
-  extern __shared__ float2 lds[];  // buffer b at lds + b * ldsM
+  extern __shared__ float2 lds[];  // buffer b at lds + b * ldsM, then the M twiddles at lds + 2 * ldsM
+  float2 *stw = lds + 2u * ldsM;
   const float2 *bufs[5] = {T.rstate, T.ra, T.rb, T.rhat, nullptr};
   float2 *obufs[5] = {T.rstate, T.ra, T.rb, T.rhat, nullptr};
   const float2 *in = bufs[which_in];
@@ -91,15 +92,16 @@ __global__ __launch_bounds__(256) void k_fft_rows(SlWork W, int which_in, int wh
   // every load of the thread is issued before the first LDS write (M <= FFT_LDS_COMPLEX = 16 * 256)
   {
     float2 v[FFT_LOADS];
+    float2 w[FFT_LOADS];
 #pragma unroll
     for (int k = 0; k < FFT_LOADS; k++) {
       const uint32_t x = threadIdx.x + 256u * k;
-      if (x < M) v[k] = fft_input(T, mode, in, x, y);
+      if (x < M) { v[k] = fft_input(T, mode, in, x, y); w[k] = T.tw[x]; }
     }
 #pragma unroll
     for (int k = 0; k < FFT_LOADS; k++) {
       const uint32_t x = threadIdx.x + 256u * k;
-      if (x < M) lds[x] = v[k];
+      if (x < M) { lds[x] = v[k]; stw[x] = w[k]; }
     }
   }
   __syncthreads();
@@ -107,14 +109,14 @@ __global__ __launch_bounds__(256) void k_fft_rows(SlWork W, int which_in, int wh
   for (; lns + 1u <= lM; lns += 2u) {
     const float2 *src = lds + cur * ldsM;
     float2 *dst = lds + (cur ^ 1u) * ldsM;
-    for (uint32_t g = threadIdx.x; g < (M >> 2); g += blockDim.x) bfly4(src, dst, g, lns, lM, 1u, T.tw, sign);
+    for (uint32_t g = threadIdx.x; g < (M >> 2); g += blockDim.x) bfly4(src, dst, g, lns, lM, 1u, stw, sign);
     __syncthreads();
     cur ^= 1u;
   }
   if (lns == lM) {  // odd log2 M: the last pass alone
     const float2 *src = lds + cur * ldsM;
     float2 *dst = lds + (cur ^ 1u) * ldsM;
-    for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) dst[x] = bfly(src, x, lns, lM, 1, T.tw, sign);
+    for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) dst[x] = bfly(src, x, lns, lM, 1, stw, sign);
     __syncthreads();
     cur ^= 1u;
   }
@@ -132,6 +134,7 @@ __global__ __launch_bounds__(256) void k_fft_cols(SlWork W, int which_in, int wh
   if (x0 >= M) return;
   if (omode == OUT_SCALE && !T.mask_bound) return;  // tanks of a batch without the absorbing mask
   __shared__ float2 lds[2][FFT_LDS_COMPLEX];
+  extern __shared__ float2 stw[];  // the M twiddles (dynamic LDS sized to the largest M of the launch)
   const float2 *bufs[4] = {T.rstate, T.ra, T.rb, T.rhat};
   float2 *obufs[4] = {T.rstate, T.ra, T.rb, T.rhat};
   const float2 *in = bufs[which_in];
@@ -146,10 +149,17 @@ __global__ __launch_bounds__(256) void k_fft_cols(SlWork W, int which_in, int wh
       const uint32_t q = threadIdx.x + 256u * k;
       if (q < n) v[k] = in[(q >> lCW) * M + x0 + (q & (CW - 1u))];
     }
+    float2 w[FFT_LOADS];
+#pragma unroll
+    for (int k = 0; k < FFT_LOADS; k++) {
+      const uint32_t x = threadIdx.x + 256u * k;
+      if (x < M) w[k] = T.tw[x];
+    }
 #pragma unroll
     for (int k = 0; k < FFT_LOADS; k++) {
       const uint32_t q = threadIdx.x + 256u * k;
       if (q < n) lds[0][q] = v[k];
+      if (q < M) stw[q] = w[k];
     }
   }
   __syncthreads();
@@ -158,7 +168,7 @@ __global__ __launch_bounds__(256) void k_fft_cols(SlWork W, int which_in, int wh
   for (; lns + 1u <= lM; lns += 2u) {
     for (uint32_t q = threadIdx.x; q < (n >> 2); q += blockDim.x) {
       const uint32_t c = q & (CW - 1u), g = q >> lCW;
-      bfly4(lds[cur] + c, lds[cur ^ 1] + c, g, lns, lM, CW, T.tw, sign);
+      bfly4(lds[cur] + c, lds[cur ^ 1] + c, g, lns, lM, CW, stw, sign);
     }
     __syncthreads();
     cur ^= 1;
@@ -166,7 +176,7 @@ __global__ __launch_bounds__(256) void k_fft_cols(SlWork W, int which_in, int wh
   if (lns == lM) {  // odd log2 M: the last pass alone
     for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
       const uint32_t c = q & (CW - 1u), y = q >> lCW;
-      lds[cur ^ 1][q] = bfly(lds[cur] + c, y, lns, lM, CW, T.tw, sign);
+      lds[cur ^ 1][q] = bfly(lds[cur] + c, y, lns, lM, CW, stw, sign);
     }
     __syncthreads();
     cur ^= 1;
@@ -272,22 +282,23 @@ __global__ void k_clear_rsrc(SlWork W) {
 void sl_k_ripple_frame_ex(const SlWork &rows, const SlWork &cols, const SlWork &elems, const SlWork &tex,
                           hipStream_t s, int mask, uint32_t maxM) {
   if (!rows.total) return;
-  const size_t rl = (size_t)2 * maxM * sizeof(float2);  // row LDS: two buffers of M points
+  const size_t rl = (size_t)3 * maxM * sizeof(float2);  // row LDS: two buffers of M points and the twiddles
+  const size_t cl = (size_t)maxM * sizeof(float2);      // column LDS beside the static buffers: the twiddles
   // forward transform of the mirrored source -> hat (64-ripple.js:229-232)
   hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), rl, s, rows, 0, 1, (int)IN_SRC, -1.0f, maxM);
-  hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), 0, s, cols, 1, 3, (int)OUT_BUF, -1.0f);
+  hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), cl, s, cols, 1, 3, (int)OUT_BUF, -1.0f);
   // evolve the modes and add the kick (64-ripple.js:233)
   hipLaunchKernelGGL(k_revolve, dim3(elems.total), dim3(256), 0, s, elems);
   // inverse transform of the state -> b (64-ripple.js:234-235)
   hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), rl, s, rows, 0, 1, (int)IN_BUF, 1.0f, maxM);
-  hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), 0, s, cols, 1, 2, (int)OUT_BUF, 1.0f);
+  hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), cl, s, cols, 1, 2, (int)OUT_BUF, 1.0f);
   // height and slopes (64-ripple.js:236-238)
   hipLaunchKernelGGL(k_rextract, dim3(tex.total), dim3(256), 0, s, tex);
   if (mask) {
     // absorb: mask the spatial field, transform back into the state, rescale (64-ripple.js:239-250)
     hipLaunchKernelGGL(k_rmaskbuild, dim3(tex.total), dim3(256), 0, s, tex);
     hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), rl, s, rows, 2, 1, (int)IN_MASK, -1.0f, maxM);
-    hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), 0, s, cols, 1, 0, (int)OUT_SCALE, -1.0f);
+    hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), cl, s, cols, 1, 0, (int)OUT_SCALE, -1.0f);
   }
   hipLaunchKernelGGL(k_clear_rsrc, dim3(tex.total), dim3(256), 0, s, tex);  // 64-ripple.js:252
 }
