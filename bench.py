@@ -40,13 +40,17 @@ def progress(msg):
     print(f"bench: {msg}", file=sys.stderr, flush=True)
 
 
-def build_sim(wl):
-    g = suiri.Sim(suiri.config("high", **wl["cfg"]))
+def build_sim(wl, quality="high"):
+    g = suiri.Sim(suiri.config(quality, **wl["cfg"]))
     workloads.setup(g, wl)
     g.reorder()
     g.set_substeps(1)
-    em = scene.Emitters(wl["items"], g.vp, seed=99)
-    return g, em
+    return g
+
+
+def step(g, wl):
+    """One frame; the workload's sources emit through the library's emitters (sl_step_items)."""
+    g.step_items(wl["frame_dt"], wl["items"])
 
 
 def sor_bytes_per_launch(nx, ny, nz):
@@ -65,12 +69,13 @@ def cpu_baseline(wl, steps, threads):
     q = O.quality("high", h=c["h"], iters=c["iters"], ripple_R=c["ripple_R"], cap=c["cap"], ww_cap=c["ww_cap"],
                   flip=c["flip"], omega=c["omega"])
     o = O.Oracle((c["tank_w"], c["tank_h"], c["tank_d"]), q, threads=threads)
-    rows, _ = scene.ObjectPacker().pack(wl["items"], 0)
+    sc = scene.Scene(suiri.config("high", **c))   # host-side scene state of the same tank
+    rows, _ = sc.pack(wl["items"], 0)
     o.set_objects(rows)
     inf = o.info()
     o.emit_now([scene.cell_block_emit(inf["h"], [inf["ox"], inf["oy"], inf["oz"]], *blk, seed=wl["seed"] + i)
                 for i, blk in enumerate(wl["blocks"])])
-    em = scene.Emitters(wl["items"], inf["vp"], seed=99)
+    em = scene.Emitters(wl["items"], sc)
     o.step(wl["frame_dt"], nsub=1, emit_fn=em)   # first step includes GEOM: untimed
     t0 = time.perf_counter()
     psteps = 0
@@ -153,25 +158,25 @@ def projection_residual(g, dt):
 
 def converged_leg(wl, steps, warmup, tol, max_iters):
     """The workload with the converged pressure solve (multigrid-preconditioned CG to `tol` 1/s)."""
-    g, em = build_sim(wl)
+    g = build_sim(wl)
     for _ in range(warmup):
-        g.step(wl["frame_dt"], em)
+        step(g, wl)
     g.autotune(reps=2)
     g.set_pressure_solver(suiri.SOLVER_MGPCG, tol, max_iters)
     for _ in range(5):
-        g.step(wl["frame_dt"], em)
+        step(g, wl)
     c0 = g.stats()["count"]
     g.sync()
     t0 = time.perf_counter()
     for _ in range(steps):
-        g.step(wl["frame_dt"], em)
+        step(g, wl)
     g.sync()
     el = time.perf_counter() - t0
     c1 = g.stats()["count"]
     # after the timed region: iterations of a few more solves, then the residual of the last one
     its = []
     for _ in range(5):
-        g.step(wl["frame_dt"], em)
+        step(g, wl)
         its.append(g.pressure_solver_stats()["iters"])
     st = g.pressure_solver_stats()
     res = projection_residual(g, wl["frame_dt"])
@@ -297,10 +302,10 @@ def serve_leg(rate=20.0, window=3.0, limit=45.0, steps=(100, 2000), kinds=(0, 1,
         now = time.perf_counter() - t0
         while i < len(reqs) and reqs[i]["t"] <= now:
             r = reqs[i]
-            rows, fill, src = workloads.job_inputs(r["wl"])
+            items, fill = workloads.job_inputs(r["wl"])
             q = "high" if r["kind"] == 1 else "standard"
-            ids[srv.submit(suiri.config(q, **r["wl"]["cfg"]), rows=rows, fill=fill, steps=r["steps"],
-                           dt=r["wl"]["frame_dt"], source=src)] = i
+            ids[srv.submit(suiri.config(q, **r["wl"]["cfg"]), items=items, fill=fill, steps=r["steps"],
+                           dt=r["wl"]["frame_dt"])] = i
             i += 1
         running = srv.tick()
         now = time.perf_counter() - t0
@@ -318,18 +323,14 @@ def serve_leg(rate=20.0, window=3.0, limit=45.0, steps=(100, 2000), kinds=(0, 1,
     # sequential plain solves of the same requests: per-kind time per frame measured alone on the plain
     # path (faucet and dam break at their sizes, batched-mix tanks on a sample), applied to each request
     def plain_rate(wl, q, k=8):
-        g = suiri.Sim(suiri.config(q, **wl["cfg"]))
-        workloads.setup(g, wl)
-        g.reorder()
-        g.set_substeps(1)
-        em = scene.Emitters(wl["items"], g.vp, seed=99) if wl["sources"] else None
+        g = build_sim(wl, q)
         for _ in range(2):
-            g.step(wl["frame_dt"], em)
+            step(g, wl)
         g.sync()
         c0 = g.stats()["count"]
         t1 = time.perf_counter()
         for _ in range(k):
-            g.step(wl["frame_dt"], em)
+            step(g, wl)
         g.sync()
         dtm = (time.perf_counter() - t1) / k
         c1 = g.stats()["count"]
@@ -423,9 +424,9 @@ def main():
     if args.only == "converged":
         print(json.dumps(converged_leg(wl, args.conv_steps, args.warmup, args.conv_tol, args.conv_iters)))
         return
-    g, em = build_sim(wl)
+    g = build_sim(wl)
     for _ in range(args.warmup):
-        g.step(wl["frame_dt"], em)
+        step(g, wl)
     tuned = g.autotune(reps=2)
     st0 = g.stats()
     g.timing(True)
@@ -434,7 +435,7 @@ def main():
     t0 = time.perf_counter()
     counts = []
     for i in range(args.steps):
-        g.step(wl["frame_dt"], em)
+        step(g, wl)
         if i % 25 == 24:
             counts.append(g.stats()["count"])
     g.sync()
@@ -446,7 +447,7 @@ def main():
     g.sync()
     t1 = time.perf_counter()
     for _ in range(args.steps):
-        g.step(wl["frame_dt"], em)
+        step(g, wl)
     g.sync()
     elapsed_plain = time.perf_counter() - t1
     st2 = g.stats()

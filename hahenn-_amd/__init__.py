@@ -20,10 +20,10 @@ SL_EBUDGET = -5
 F_PARTICLES, F_WW, F_COUNTERS, F_ACC, F_CELLCOUNT, F_VEL, F_VELOLD, F_FRAC, F_SVEL, F_CELLTYPE, \
     F_DIVG, F_PRES, F_OBJS, F_RSRC, F_RSTATE, F_RHAT, F_RFIELD, F_RMASK, F_RTEX, F_TOPH, F_RTEX16, F_ARGS = range(22)
 S_GEOM, S_P2G, S_NORMFORCE, S_DIVERGENCE, S_PRESSURE, S_PROJECT, S_G2P, S_COMPACT, S_WW, S_RIPPLE, \
-    S_SOR = range(11)
+    S_SOR, S_DROPLETS = range(12)
 SOLVER_SOR, SOLVER_MGPCG = 0, 1
 T_NAMES = ["emit", "clear", "geom", "p2g", "normforce", "divergence", "sor", "project", "g2p", "compact",
-           "ww", "ripple"]
+           "ww", "ripple", "droplets"]
 
 FIELD_DTYPE = {F_PARTICLES: (np.float32, 8), F_WW: (np.float32, 8), F_COUNTERS: (np.uint32, 1),
                F_ACC: (np.int32, 6), F_CELLCOUNT: (np.uint32, 1), F_VEL: (np.float32, 4),
@@ -46,7 +46,12 @@ class SlConfig(ctypes.Structure):
         ("r_film", ctypes.c_float), ("r_hp_k", ctypes.c_float), ("r_rho_kick", ctypes.c_float),
         ("wmap", ctypes.c_uint32), ("whitewater", ctypes.c_uint32), ("device", ctypes.c_int32),
         ("reorder_every", ctypes.c_uint32), ("reorder_phase", ctypes.c_uint32),
-                ("top_height", ctypes.c_uint32), ("reserved", ctypes.c_uint32 * 2),
+        ("top_height", ctypes.c_uint32), ("seed", ctypes.c_uint32),
+        ("splash_rho", ctypes.c_double), ("splash_sigma", ctypes.c_double), ("splash_mu", ctypes.c_double),
+        ("splash_k_jet", ctypes.c_float), ("splash_k_crown", ctypes.c_float), ("ripple_depth", ctypes.c_double),
+        ("droplets", ctypes.c_uint32), ("drop_seed", ctypes.c_uint32), ("drop_d50", ctypes.c_float),
+        ("drop_sigma_ln", ctypes.c_float), ("drop_vmin", ctypes.c_float), ("drop_vmax", ctypes.c_float),
+        ("reserved", ctypes.c_uint32 * 2),
     ]
 
 
@@ -87,7 +92,9 @@ EXPORTS = ["sl_config_default", "sl_create", "sl_destroy", "sl_last_error", "sl_
            "sl_batch_set_dt", "sl_job_bytes", "sl_server_config_default", "sl_server_create", "sl_server_destroy",
            "sl_server_last_error", "sl_server_submit", "sl_server_cancel", "sl_server_tick", "sl_server_poll",
            "sl_server_info", "sl_server_job_sim", "sl_server_release", "sl_server_stats", "sl_bake_frame",
-           "sl_write_exr_rgb"]
+           "sl_write_exr_rgb", "sl_scene_create", "sl_scene_destroy", "sl_scene_last_error", "sl_sim_scene",
+           "sl_scene_pack_objects", "sl_set_items", "sl_scene_source_emits", "sl_scene_block_emits",
+           "sl_block_liters", "sl_step_items"]
 
 EMIT_FN = ctypes.CFUNCTYPE(ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                            ctypes.POINTER(SlEmit), ctypes.c_uint32)
@@ -103,7 +110,8 @@ class SlJob(ctypes.Structure):
     _fields_ = [("cfg", SlConfig), ("objects", ctypes.POINTER(ctypes.c_float)), ("n_objects", ctypes.c_uint32),
                 ("fill", ctypes.POINTER(SlEmit)), ("n_fill", ctypes.c_uint32), ("steps", ctypes.c_uint32),
                 ("dt", ctypes.c_double), ("source", EMIT_FN), ("user", ctypes.c_void_p), ("keep", ctypes.c_uint32),
-                ("bake_every", ctypes.c_uint32), ("bake_dir", ctypes.c_char_p), ("reserved", ctypes.c_uint32 * 2)]
+                ("bake_every", ctypes.c_uint32), ("bake_dir", ctypes.c_char_p), ("items", ctypes.c_void_p),
+                ("n_items", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
 
 
 class SlJobInfo(ctypes.Structure):
@@ -208,6 +216,16 @@ def load():
         "sl_bake_frame": (I, [P, ctypes.c_char_p, U]),
         "sl_write_exr_rgb": (I, [ctypes.c_char_p, U, U, P, P, P]),
         "sl_batch_timing_read": (I, [P, ctypes.POINTER(D), U, ctypes.POINTER(U)]),
+        "sl_scene_create": (I, [ctypes.POINTER(SlConfig), ctypes.POINTER(P)]),
+        "sl_scene_destroy": (None, [P]),
+        "sl_scene_last_error": (ctypes.c_char_p, [P]),
+        "sl_sim_scene": (P, [P]),
+        "sl_scene_pack_objects": (I, [P, P, U, D, P, ctypes.POINTER(U), ctypes.POINTER(I)]),
+        "sl_set_items": (I, [P, P, U, D, ctypes.POINTER(I)]),
+        "sl_scene_source_emits": (I, [P, P, U, D, P, U, ctypes.POINTER(U)]),
+        "sl_scene_block_emits": (I, [P, P, U, I, P, U, ctypes.POINTER(U)]),
+        "sl_block_liters": (D, [P, D, D, D]),
+        "sl_step_items": (I, [P, D, P, U]),
         "sl_version": (I, []),
         "sl_build_info": (ctypes.c_char_p, []),
     }
@@ -279,6 +297,31 @@ class Sim:
             pass
 
     # FluidSim interface
+    def set_items(self, items, dt_move=0.0):
+        """FluidSim.setObjects(items, dtMove): packs the obstacles and drains among the scene items
+        (velocities from the previous call's poses) and uploads them; returns whether one moved."""
+        from . import scene as _scene
+        arr, n = _scene.items_array(items)
+        mv = ctypes.c_int(0)
+        self._check(self.L.sl_set_items(self.ptr, arr, n, float(dt_move), ctypes.byref(mv)))
+        return bool(mv.value)
+
+    def scene(self):
+        """The simulation's own scene state (sources' progress, object poses, seed stream)."""
+        from . import scene as _scene
+        return _scene.Scene.of(self)
+
+    def fill_blocks(self, items, fill=False):
+        """emitNow(blockEmits(items, fill), fill): the water blocks among the scene items."""
+        recs = self.scene().block_emits(items, fill)
+        return self.emit_now(recs, fill) if recs else 0
+
+    def step_items(self, frame_dt, items):
+        """FluidSim.step with the sources among the scene items as the emitter (sl_step_items)."""
+        from . import scene as _scene
+        arr, n = _scene.items_array(items)
+        self._check(self.L.sl_step_items(self.ptr, float(frame_dt), arr, n))
+
     def set_objects_rows(self, rows):
         rows = np.ascontiguousarray(rows, dtype=np.float32).reshape(-1, 28)
         ch = ctypes.c_int(0)
@@ -543,9 +586,17 @@ class Server:
             raise SuiriError(f"suiri error {r}: {msg.decode() if msg else ''}")
 
     def submit(self, cfg, rows=None, fill=(), steps=100, dt=1.0 / 120, source=None, keep=False, bake_dir=None,
-               bake_every=0):
+               bake_every=0, items=None):
+        """`items`: scene items whose obstacles and drains are packed and whose blocks fill the tank at
+        admission, and whose sources emit every frame (instead of `rows`)."""
         j = SlJob()
         j.cfg = cfg
+        items_keep = None
+        if items:
+            from . import scene as _scene
+            items_keep, n_items = _scene.items_array(items)
+            j.items = ctypes.cast(items_keep, ctypes.c_void_p)
+            j.n_items = n_items
         rows_a = np.ascontiguousarray(rows if rows is not None else np.zeros((0, 28)), dtype=np.float32).reshape(-1, 28)
         j.objects = rows_a.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
         j.n_objects = rows_a.shape[0]

@@ -13,6 +13,7 @@
 #include <algorithm>
 #include "../../include/suiri.h"
 #include "sl_internal.h"
+#include "sl_scene.h"
 
 static thread_local std::string g_err;
 
@@ -88,7 +89,10 @@ struct sl_sim {
   hipEvent_t stats_ev = nullptr;
   bool stats_pending = false;
   Staging stg;
-  // saved water (50-sim.js:111-141)
+  // host-side scene state: source progress, previous object poses, seed stream (scene.cpp)
+  sl_scene scene;
+  // saved water (50-sim.js:111-141): particles, whitewater, counters, ripple state and the sources'
+  // progress
   struct Saved {
     bool valid = false;
     uint32_t id = 0;
@@ -97,6 +101,8 @@ struct sl_sim {
     uint32_t count = 0, ww_count = 0, impacts = 0;
     bool full = false;
     int cur = 0, ww_cur = 0;
+    std::map<uint32_t, SlSourceState> sources;
+    uint32_t seed_state = 0;
   } saved;
   // timing
   bool timing = false;
@@ -281,6 +287,14 @@ extern "C" int sl_config_default(sl_config *c, const char *quality) {
   c->whitewater = 1;
   c->device = 0;
   c->reorder_every = 0;
+  c->seed = 1;
+  // splashCrown, 31-wgsl-sim2.js:121-126
+  c->splash_rho = 998.0; c->splash_sigma = 0.0728; c->splash_mu = 0.001;
+  c->splash_k_jet = 700.0f; c->splash_k_crown = 2100.0f;
+  c->ripple_depth = 0.0;
+  // droplet layer off; when switched on: median 1.5 mm, sigma_ln 0.4, 1-4 m/s
+  c->droplets = 0; c->drop_seed = 1;
+  c->drop_d50 = 0.0015f; c->drop_sigma_ln = 0.4f; c->drop_vmin = 1.0f; c->drop_vmax = 4.0f;
   return SL_OK;
 }
 
@@ -302,6 +316,18 @@ static int alloc(sl_sim *s, void **p, size_t bytes) {
     (ptr) = (decltype(ptr))_p;                                      \
   } while (0)
 
+// grid sizing, 50-sim.js:33-45: at least 8 cells per axis, the cell size that fits all three, the
+// origin at the min corner with x and z centred and the floor at y = 0
+int sl_grid_of(const sl_config &c, int n[3], double &h, double origin[3]) {
+  if (!(c.h > 0) || !(c.tank_w > 0) || !(c.tank_h > 0) || !(c.tank_d > 0)) return SL_EARG;
+  n[0] = (int)fmax(8.0, round(c.tank_w / c.h));
+  n[1] = (int)fmax(8.0, round(c.tank_h / c.h));
+  n[2] = (int)fmax(8.0, round(c.tank_d / c.h));
+  h = fmin(fmin(c.tank_w / n[0], c.tank_h / n[1]), c.tank_d / n[2]);
+  origin[0] = -n[0] * h / 2; origin[1] = 0; origin[2] = -n[2] * h / 2;
+  return SL_OK;
+}
+
 // configuration checks and grid sizing (50-sim.js:33-45); host only
 static int size_grid(sl_sim *s) {
   const sl_config &c = s->cfg;
@@ -310,14 +336,22 @@ static int size_grid(sl_sim *s) {
   if (c.ripple_R && (c.ripple_R & (c.ripple_R - 1))) FAIL(s, SL_EARG, "ripple_R must be a power of two");
   if (c.ripple_R > 2048) FAIL(s, SL_ECAP, "ripple_R above 2048");
   if (c.cap == 0) FAIL(s, SL_EARG, "capacity must be positive");
+  if (!(c.splash_rho > 0) || !(c.splash_sigma > 0) || !(c.splash_mu > 0))
+    FAIL(s, SL_EARG, "splash density, surface tension and viscosity must be positive");
+  if (!(c.ripple_depth >= 0)) FAIL(s, SL_EARG, "ripple_depth must be >= 0 (0: mean depth of the count)");
+  if (c.droplets) {
+    if (c.droplets > (1u << 22)) FAIL(s, SL_ECAP, "droplets above 2^22 per frame");
+    if (!c.ripple_R) FAIL(s, SL_EARG, "droplets need a ripple layer (ripple_R > 0)");
+    if (!(c.drop_d50 > 0) || !(c.drop_sigma_ln >= 0) || !(c.drop_vmin >= 0) || !(c.drop_vmax >= c.drop_vmin))
+      FAIL(s, SL_EARG, "droplets: need drop_d50 > 0, drop_sigma_ln >= 0 and 0 <= drop_vmin <= drop_vmax");
+  }
   if (c.cap >= (1u << 30) || c.ww_cap >= (1u << 28))  // 32-bit byte offsets into the particle arrays
     FAIL(s, SL_ECAP, "capacity too large (2^30 particles, 2^28 whitewater at most)");
-  // grid sizing, 50-sim.js:33-45
-  s->nx = (int)fmax(8.0, round(c.tank_w / c.h));
-  s->ny = (int)fmax(8.0, round(c.tank_h / c.h));
-  s->nz = (int)fmax(8.0, round(c.tank_d / c.h));
-  s->h = fmin(fmin(c.tank_w / s->nx, c.tank_h / s->ny), c.tank_d / s->nz);
-  s->origin[0] = -s->nx * s->h / 2; s->origin[1] = 0; s->origin[2] = -s->nz * s->h / 2;
+  {
+    int n[3];
+    sl_grid_of(c, n, s->h, s->origin);
+    s->nx = n[0]; s->ny = n[1]; s->nz = n[2];
+  }
   s->rim = s->ny * s->h;
   uint64_t nodes = (uint64_t)(s->nx + 1) * (s->ny + 1) * (s->nz + 1);
   if (nodes >= (1ull << 30)) FAIL(s, SL_ECAP, "grid too large (2^30 nodes at most)");  // 32-bit byte offsets
@@ -415,6 +449,7 @@ static int create_impl(sl_sim *s) {
   else HIPCHK(s, hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
   s->time = 0; s->vmax = 1.0; s->frame = 0;
   s->reorder_clock = c.reorder_every ? c.reorder_phase % c.reorder_every : 0;
+  if (sl_scene_init(s->scene, c)) FAIL(s, SL_EARG, "scene state of the configuration");
   memset(s->objs, 0, sizeof s->objs);
 
   SlTank &T = s->ht;
@@ -450,6 +485,16 @@ static int create_impl(sl_sim *s) {
   T.top_height = c.top_height && s->R ? 1 : 0;
   T.tankH = (float)c.tank_h;
   T.rsr = c.r_sigma_rho; T.rnu = c.r_nu; T.rfilm = c.r_film; T.rhpk = c.r_hp_k; T.rrho = c.r_rho_kick;
+  T.rdepth = (float)c.ripple_depth;
+  // splashCrown's constant expressions, folded in double and rounded once as the WGSL folds its
+  // abstract-float constants (31-wgsl-sim2.js:121-123: 3.14159265 * 998.0, 998.0 * 0.0728)
+  T.sp_pirho = (float)(3.14159265 * c.splash_rho);
+  T.sp_rs = (float)(c.splash_rho * c.splash_sigma);
+  T.sp_rho = (float)c.splash_rho; T.sp_sigma = (float)c.splash_sigma; T.sp_mu = (float)c.splash_mu;
+  T.sp_kjet = c.splash_k_jet; T.sp_kcrown = c.splash_k_crown;
+  T.drops = c.droplets; T.drop_seed = c.drop_seed;
+  T.drop_d50 = c.drop_d50; T.drop_sig = c.drop_sigma_ln; T.drop_vmin = c.drop_vmin; T.drop_vmax = c.drop_vmax;
+  T.drop_mc = (float)(M_PI / 6.0 * c.splash_rho);
   {
     int r = tank_buffers(s, T, [s](void **p, size_t bytes) { return alloc(s, p, bytes); });
     if (r) return r;
@@ -592,6 +637,18 @@ extern "C" int sl_set_objects(sl_sim *s, const float *rows, uint32_t n, int *cha
   memcpy(dev, s->objs, sizeof dev);
   for (uint32_t i = 0; i < n; i++) dev[i * 28 + 3] = cull_radius(dev + i * 28);
   return stage_upload(s, s->ht.objs, dev, sizeof(SlObj) * (n ? n : 1));
+}
+
+extern "C" sl_scene *sl_sim_scene(sl_sim *s) { return s ? &s->scene : nullptr; }
+
+extern "C" int sl_set_items(sl_sim *s, const sl_item *items, uint32_t n, double dt_move, int *moving) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  if (n && !items) FAIL(s, SL_EARG, "sl_set_items: null items");
+  float rows[SL_MAX_OBJ * 28];
+  uint32_t k = 0;
+  int r = sl_scene_pack_objects(&s->scene, items, n, dt_move, rows, &k, moving);
+  if (r) FAIL(s, r, "sl_set_items: packing failed");
+  return sl_set_objects(s, rows, k, nullptr);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -819,12 +876,29 @@ extern "C" int sl_substep(sl_sim *s, double dt, const sl_emit *list, uint32_t n,
   return SL_OK;
 }
 
-static int encode_ripple(sl_sim *s, double frame_dt, double mean_depth) {
+// the ripple depth of a frame: the mean depth of the host's count clamped to [0.003, rim]
+// (50-sim.js:332), or the configured fixed depth
+static double frame_mean_depth(const sl_sim *s) {
+  if (s->cfg.ripple_depth > 0) return s->cfg.ripple_depth;
+  double liters = s->count * s->vp * 1000;
+  double md = liters / 1000 / (s->nx * s->nz * s->h * s->h);
+  return md < 0.003 ? 0.003 : (md > s->rim ? s->rim : md);
+}
+// the frame's ripple uniform (64-ripple.js:223-226), read by the droplets and the ripple frame
+static int set_ripple_uniform(sl_sim *s, double frame_dt, double mean_depth) {
   if (!s->R) return SL_OK;
   s->ht.rLx = (float)(s->nx * s->h); s->ht.rLz = (float)(s->nz * s->h);
   s->ht.rdt = (float)frame_dt; s->ht.rmd = (float)mean_depth;
-  int r = push_dyn(s);
-  if (r) return r;
+  return push_dyn(s);
+}
+static int encode_droplets(sl_sim *s) {
+  if (!s->ht.drops) return SL_OK;
+  StageScope sc(s, SL_T_DROPLETS, 1);
+  sl_k_droplets(launch_of(s, s->ht.drops));
+  return check_launch(s, "droplets");
+}
+static int encode_ripple(sl_sim *s) {
+  if (!s->R) return SL_OK;
   StageScope sc(s, SL_T_RIPPLE, s->mask_bound ? 10 : 6);
   const uint32_t M = s->M, R1 = s->R + 1;
   uint32_t CW = 4096 / M; if (CW > M) CW = M; if (CW < 1) CW = 1;
@@ -876,16 +950,16 @@ extern "C" int sl_step_end(sl_sim *s, double frame_dt) {
     r = encode_reorder(s);
     if (r) return r;
   }
+  // the ripple uniform first: the droplets splash at the frame's water level
+  if ((r = set_ripple_uniform(s, frame_dt, frame_mean_depth(s)))) return r;
+  if ((r = encode_droplets(s))) return r;
   {
     StageScope sc(s, SL_T_WW, 2);
     sl_k_ww(launch_of(s, s->ww_cap), (float)frame_dt);
     sl_k_count(launch_of(s, 1), 3, 0);
     s->ww_cur ^= 1;
   }
-  double liters = s->count * s->vp * 1000;
-  double md = liters / 1000 / (s->nx * s->nz * s->h * s->h);
-  md = md < 0.003 ? 0.003 : (md > s->rim ? s->rim : md);
-  r = encode_ripple(s, frame_dt, md);
+  r = encode_ripple(s);
   if (r) return r;
   if (s->ht.top_height) {  // drawn after the step, read by the next frame's ripple mask
     StageScope sc(s, SL_T_RIPPLE, 6);
@@ -908,6 +982,23 @@ extern "C" int sl_step(sl_sim *s, double frame_dt, sl_emit_fn fn, void *user) {
     if (m > SL_MAX_EMIT) m = SL_MAX_EMIT;
     r = sl_substep(s, dt, list, m, nullptr);
     if (r) return r;
+  }
+  return sl_step_end(s, frame_dt);
+}
+
+// FluidSim.step with the sources of `items` as the emitter (55-emit.js:13-69 per substep)
+extern "C" int sl_step_items(sl_sim *s, double frame_dt, const sl_item *items, uint32_t n) {
+  if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  if (n && !items) FAIL(s, SL_EARG, "sl_step_items: null items");
+  uint32_t ns = 0;
+  double dt = 0;
+  int r = sl_step_begin(s, frame_dt, &ns, &dt);
+  if (r || ns == 0) return r;
+  sl_emit list[SL_MAX_EMIT];
+  for (uint32_t i = 0; i < ns; i++) {
+    uint32_t m = 0;
+    if ((r = sl_scene_source_emits(&s->scene, items, n, dt, list, SL_MAX_EMIT, &m))) FAIL(s, r, "sl_step_items: emitters");
+    if ((r = sl_substep(s, dt, list, m, nullptr))) return r;
   }
   return sl_step_end(s, frame_dt);
 }
@@ -964,8 +1055,22 @@ extern "C" int sl_stats(sl_sim *s, sl_stats_t *o) {
 // ------------------------------------------------------------------------------------------------
 // clear / save / restore, 50-sim.js:111-141, :365-373
 // ------------------------------------------------------------------------------------------------
+// A counters readback enqueued before a clear or restore carries the old count: wait for it and drop
+// it, so that the next sl_poll_stats cannot replace the host count (and the launch bound p_ub) with it.
+static int drop_pending_stats(sl_sim *s) {
+  if (!s->stats_pending) return SL_OK;
+  HIPCHK(s, hipEventSynchronize(s->stats_ev));
+  s->stats_pending = false;
+  s->emitted_at_copy = s->emitted_total;
+  return SL_OK;
+}
+
 extern "C" int sl_clear_water(sl_sim *s) {
   if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
+  {
+    int r = drop_pending_stats(s);
+    if (r) return r;
+  }
   sl_k_count(launch_of(s, 1), 4, 0);
   sl_k_fill_u32(s->stream, (uint32_t *)s->ht.pres, 0, (size_t)s->hcells * 2);
   if (s->R) {
@@ -974,6 +1079,7 @@ extern "C" int sl_clear_water(sl_sim *s) {
   }
   s->count = 0; s->ww_count = 0; s->impacts = 0; s->full = false; s->time = 0; s->vmax = 1;
   s->p_ub = 0;
+  s->scene.sources.clear();
   return check_launch(s, "clear_water");
 }
 
@@ -1005,6 +1111,8 @@ extern "C" int sl_save_water(sl_sim *s, uint32_t *id) {
   if (rbytes) HIPCHK(s, hipMemcpyAsync(v.ripple, T.rstate, rbytes, hipMemcpyDeviceToDevice, s->stream));
   v.time = s->time; v.count = s->count; v.ww_count = s->ww_count; v.impacts = s->impacts; v.full = s->full;
   v.cur = c; v.ww_cur = w;
+  v.sources = s->scene.sources;
+  v.seed_state = s->scene.seed_state;
   v.valid = true;
   v.id = ++g_copies;
   if (id) *id = v.id;
@@ -1030,7 +1138,18 @@ extern "C" int sl_restore_water(sl_sim *s, uint32_t id) {
   HIPCHK(s, hipMemcpyAsync(T.C, v.counters, 32, hipMemcpyDeviceToDevice, s->stream));
   if (rbytes) HIPCHK(s, hipMemcpyAsync(T.rstate, v.ripple, rbytes, hipMemcpyDeviceToDevice, s->stream));
   s->time = v.time; s->count = v.count; s->ww_count = v.ww_count; s->impacts = v.impacts; s->full = v.full;
-  s->p_ub = s->cap;  // the device count is restored; bound it by the capacity until the next readback
+  s->scene.sources = v.sources;
+  s->scene.seed_state = v.seed_state;
+  // dev_cur waited for the stream, so a readback enqueued before the restore has landed: drop it, and
+  // take the launch bound from the restored device count
+  s->stats_pending = false;
+  {
+    SlCounters c;
+    HIPCHK(s, hipStreamSynchronize(s->stream));
+    HIPCHK(s, hipMemcpy(&c, T.C, sizeof c, hipMemcpyDeviceToHost));
+    s->emitted_at_copy = s->emitted_total;
+    s->p_ub = c.count < s->cap ? c.count : s->cap;
+  }
   v.valid = false;
   return 1;
 }
@@ -1353,12 +1472,12 @@ extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
       sl_k_count(launch_of(s, 1), 3, 0);
       s->ww_cur ^= 1;
       break;
-    case SL_S_RIPPLE: {
-      double liters = s->count * s->vp * 1000;
-      double md = liters / 1000 / (s->nx * s->nz * s->h * s->h);
-      md = md < 0.003 ? 0.003 : (md > s->rim ? s->rim : md);
-      return encode_ripple(s, dt, md);
-    }
+    case SL_S_RIPPLE:
+      if ((r = set_ripple_uniform(s, dt, frame_mean_depth(s)))) return r;
+      return encode_ripple(s);
+    case SL_S_DROPLETS:
+      if ((r = set_ripple_uniform(s, dt, frame_mean_depth(s)))) return r;
+      return encode_droplets(s);
     default: FAIL(s, SL_EARG, "unknown stage %d", stage);
   }
   return check_launch(s, "stage");
@@ -1524,6 +1643,7 @@ __global__ void k_batch_advance(SlTank *tanks, const uint32_t *list, int n, doub
     double liters = (double)T.C->count * T.vp_d * 1000;
     double md = liters / 1000 / ((double)(T.nx * T.nz) * T.h_d * T.h_d);
     md = md < 0.003 ? 0.003 : (md > T.rim_d ? T.rim_d : md);
+    if (T.rdepth > 0.0f) md = T.rdepth;
     T.rLx = (float)(T.nx * T.h_d);
     T.rLz = (float)(T.nz * T.h_d);
     T.rdt = dt;
@@ -1844,6 +1964,7 @@ extern "C" int sl_batch_open(uint32_t capacity, int device, sl_batch **out) {
 extern "C" int sl_batch_add(sl_batch *b, const sl_config *cfg, uint32_t *slot) {
   if (!b || !cfg) return batch_fail(b, SL_EARG, "sl_batch_add: bad arguments");
   if (cfg->device != b->device) return batch_fail(b, SL_EARG, "sl_batch_add: tank on another device");
+  if (cfg->droplets) return batch_fail(b, SL_EARG, "sl_batch_add: the droplet layer runs on single simulations only");
   uint32_t t = 0;
   while (t < b->cap && b->sims[t]) t++;
   if (t == b->cap) return batch_fail(b, SL_ECAP, "sl_batch_add: every slot is taken");

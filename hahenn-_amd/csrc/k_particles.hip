@@ -155,17 +155,16 @@ __device__ __forceinline__ void ww_store(const SlTank &T, uint32_t k, f3 p, floa
   T.wp[wc][k] = make_float4(p.x, p.y, p.z, life);
   T.wv[wc][k] = make_float4(v.x, v.y, v.z, kind);
 }
-// splash crown, 31-wgsl-sim2.js:119-148
+// splash crown, 31-wgsl-sim2.js:119-148, with the liquid's density, surface tension, viscosity and
+// the two K thresholds taken from the tank (the reference's constants by default)
 __device__ __forceinline__ void splash_crown(const SlTank &T, f3 x, float U, float m, uint32_t id) {
   if (T.ww_rate <= 0.0f) return;
-  const float PI998 = (float)(3.14159265 * 998.0);
-  const float RS = (float)(998.0 * 0.0728);
-  float D = sl_pow(6.0f * m / PI998, (float)(1.0 / 3.0));
-  float We = 998.0f * U * U * D / 0.0728f;
-  float Oh = 0.001f / sqrtf(RS * D);
+  float D = sl_pow(6.0f * m / T.sp_pirho, (float)(1.0 / 3.0));
+  float We = T.sp_rho * U * U * D / T.sp_sigma;
+  float Oh = T.sp_mu / sqrtf(T.sp_rs * D);
   float K = We * sl_pow(Oh, -0.4f);
-  uint32_t n = f2u(clampf(rintf(4.0f * (K / 2100.0f - 1.0f)), 0.0f, 24.0f));
-  uint32_t nj = K > 700.0f ? (K > 2100.0f ? 2u : 1u) : 0u;
+  uint32_t n = f2u(clampf(rintf(4.0f * (K / T.sp_kcrown - 1.0f)), 0.0f, 24.0f));
+  uint32_t nj = K > T.sp_kjet ? (K > T.sp_kcrown ? 2u : 1u) : 0u;
   if (n + nj == 0u) return;
   uint32_t s = hashU(id * 2654435761u + T.frame * 97u + 3u);
   uint32_t base = atomicAdd(&T.C->wwCount, n + nj);
@@ -193,6 +192,42 @@ __device__ __forceinline__ void splash_crown(const SlTank &T, f3 x, float U, flo
     float c = (rndu(s) - 0.5f) * 0.05f;
     ww_store(T, k, p, 0.7f, F3(a, sp, c), 0.0f);
   }
+}
+
+// ------------------------------------------------------------------------------------------------
+// Sub-grid droplets (no counterpart in the reference, whose drops are particles of its sources):
+// T.drops per frame. Droplet d of frame f draws from its own hash stream: a standard normal z by the
+// Box-Muller transform of two uniforms, diameter D = d50 exp(sigma_ln z) (lognormal with median d50),
+// speed v uniform in [vmin, vmax], impact point uniform over the tank footprint at the water level
+// (the ripple depth of the frame). It deposits the impulse 2 m v, the merge impulse of G2P
+// (31-wgsl-sim2.js:226), through depositRipple (:110-114), and splashes as splashCrown (:119-148).
+// One thread per droplet; the whitewater it spawns is advanced by the same frame's WW pass.
+// ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void droplet_draw(const SlTank &T, uint32_t d, float &x, float &z, float &D, float &v) {
+  uint32_t s = hashU(d * 0x9e3779b1u + T.frame * 0x85ebca77u + T.drop_seed * 0xc2b2ae3du + 7u);
+  const float u1 = rndu(s), u2 = rndu(s), u3 = rndu(s), u4 = rndu(s), u5 = rndu(s);
+  const float g = sqrtf(-2.0f * sl_log(1.0f - u1)) * sl_cos(6.2831853f * u2);  // 1 - u1 in (0, 1]
+  D = T.drop_d50 * sl_exp(T.drop_sig * g);
+  v = T.drop_vmin + (T.drop_vmax - T.drop_vmin) * u5;
+  float hx, hz;
+  fp_max(T, hx, hz);
+  x = T.ox + (hx - T.ox) * u3;
+  z = T.oz + (hz - T.oz) * u4;
+}
+__global__ __launch_bounds__(SL_WG) void k_droplets(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t d = lb * SL_WG + threadIdx.x;
+  if (d >= T.drops) return;
+  float x, z, D, v;
+  droplet_draw(T, d, x, z, D, v);
+  const float m = T.drop_mc * D * D * D;
+  if (T.R > 0u) deposit_ripple(T, x, z, 2.0f * m * v);
+  splash_crown(T, F3(x, T.oy + T.rmd, z), v, m, 0x80000000u + d);
+}
+void sl_k_droplets(const SlLaunch &L) {
+  if (!L.w.total) return;
+  hipLaunchKernelGGL(k_droplets, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
 }
 
 // ------------------------------------------------------------------------------------------------

@@ -86,7 +86,9 @@ static bool get_args(napi_env env, napi_callback_info info, size_t want, napi_va
 #define CFG_FIELDS(X)                                                                           \
   X(tank_w) X(tank_h) X(tank_d) X(h) X(cap) X(ww_cap) X(iters) X(ripple_R) X(rc_factor) X(gravity) \
   X(flip) X(ppc) X(drift) X(omega) X(friction) X(ww_rate) X(r_sigma_rho) X(r_nu) X(r_film)         \
-  X(r_hp_k) X(r_rho_kick) X(wmap) X(whitewater) X(device) X(reorder_every) X(reorder_phase) X(top_height)
+  X(r_hp_k) X(r_rho_kick) X(wmap) X(whitewater) X(device) X(reorder_every) X(reorder_phase) X(top_height)  \
+  X(seed) X(splash_rho) X(splash_sigma) X(splash_mu) X(splash_k_jet) X(splash_k_crown) X(ripple_depth)     \
+  X(droplets) X(drop_seed) X(drop_d50) X(drop_sigma_ln) X(drop_vmin) X(drop_vmax)
 
 static napi_value config_to_js(napi_env env, const sl_config &c) {
   napi_value o;
@@ -627,7 +629,9 @@ static napi_value ServerCreate(napi_env env, napi_callback_info info) {
   return ext;
 }
 
-// serverSubmit(srv, {config, objects, fill, steps, dt, source, keep, bakeDir, bakeEvery}) -> id
+static bool read_items(napi_env env, napi_value list, std::vector<sl_item> &out);
+
+// serverSubmit(srv, {config, objects | items, fill, steps, dt, source, keep, bakeDir, bakeEvery}) -> id
 static napi_value ServerSubmit(napi_env env, napi_callback_info info) {
   napi_value argv[2];
   if (!get_args(env, info, 2, argv)) return nullptr;
@@ -655,6 +659,12 @@ static napi_value ServerSubmit(napi_env env, napi_callback_info info) {
       rows.assign((float *)data, (float *)data + n);
     }
   }
+  std::vector<sl_item> items;
+  napi_has_named_property(env, o, "items", &has);
+  if (has) {
+    napi_get_named_property(env, o, "items", &v);
+    if (!read_items(env, v, items)) return nullptr;
+  }
   std::vector<sl_emit> fill;
   napi_has_named_property(env, o, "fill", &has);
   if (has) {
@@ -665,6 +675,8 @@ static napi_value ServerSubmit(napi_env env, napi_callback_info info) {
   j.n_objects = (uint32_t)(rows.size() / 28);
   j.fill = fill.data();
   j.n_fill = (uint32_t)fill.size();
+  j.items = items.data();
+  j.n_items = (uint32_t)items.size();
   j.steps = (uint32_t)get_num(env, o, "steps", 100);
   j.dt = get_num(env, o, "dt", 1.0 / 120);
   j.keep = get_num(env, o, "keep", 0) != 0 ? 1u : 0u;
@@ -842,6 +854,290 @@ static napi_value ServerDestroy(napi_env env, napi_callback_info info) {
   return nullptr;
 }
 
+// ---- scene items (sl_item) ----
+// JS items of the reference's scene model: {id, kind, shape | mode, pos, rot, dims | r, enabled, and
+// the source fields}. Names map to the ids of include/suiri.h.
+static const char *const kKinds[] = {"obstacle", "drain", "source", "block"};
+static const char *const kShapes[] = {"box", "plate", "sphere", "cylinder", "cone", "torus", "cup", "container", "gutter"};
+static const char *const kModes[] = {"stream", "drops", "shower", "burst", "rain"};
+// dims keys of each obstacle shape, in sl_item.dims order
+static const char *const kShapeDims[][4] = {
+    {"w", "h", "d", nullptr}, {"w", "d", "t", nullptr}, {"r", nullptr, nullptr, nullptr}, {"r", "h", nullptr, nullptr},
+    {"r", "h", nullptr, nullptr}, {"R", "r", nullptr, nullptr}, {"r", "h", "t", nullptr}, {"w", "h", "d", "t"},
+    {"len", "r", "t", nullptr}};
+
+static std::string get_str(napi_env env, napi_value obj, const char *key) {
+  bool has = false;
+  napi_has_named_property(env, obj, key, &has);
+  if (!has) return "";
+  napi_value v;
+  napi_get_named_property(env, obj, key, &v);
+  char buf[64] = {0};
+  size_t len = 0;
+  if (napi_get_value_string_utf8(env, v, buf, sizeof buf, &len) != napi_ok) return "";
+  return std::string(buf, len);
+}
+static int name_index(const std::string &s, const char *const *names, int n) {
+  for (int i = 0; i < n; i++) if (s == names[i]) return i;
+  return -1;
+}
+static void get_vec3(napi_env env, napi_value obj, const char *key, double out[3]) {
+  bool has = false;
+  napi_has_named_property(env, obj, key, &has);
+  if (!has) return;
+  napi_value arr;
+  napi_get_named_property(env, obj, key, &arr);
+  bool isarr = false;
+  napi_is_array(env, arr, &isarr);
+  if (!isarr) return;
+  for (uint32_t i = 0; i < 3; i++) {
+    napi_value e;
+    if (napi_get_element(env, arr, i, &e) == napi_ok) napi_get_value_double(env, e, &out[i]);
+  }
+}
+static bool item_from_js(napi_env env, napi_value it, sl_item &o, std::string &err) {
+  memset(&o, 0, sizeof o);
+  const std::string kind = get_str(env, it, "kind");
+  const int k = name_index(kind, kKinds, 4);
+  if (k < 0) { err = "unknown item kind '" + kind + "'"; return false; }
+  o.kind = (uint32_t)k;
+  o.id = (uint32_t)(int64_t)get_num(env, it, "id", 0);
+  get_vec3(env, it, "pos", o.pos);
+  get_vec3(env, it, "rot", o.rot);
+  napi_value dims = nullptr;
+  bool has = false;
+  napi_has_named_property(env, it, "dims", &has);
+  if (has) napi_get_named_property(env, it, "dims", &dims);
+  if (o.kind == SL_ITEM_OBSTACLE) {
+    const std::string shape = get_str(env, it, "shape");
+    const int sh = name_index(shape, kShapes, SL_SHAPE_COUNT_);
+    if (sh < 0) { err = "unknown obstacle shape '" + shape + "'"; return false; }
+    o.shape = (uint32_t)sh;
+    for (int q = 0; q < 4 && kShapeDims[sh][q]; q++) o.dims[q] = dims ? get_num(env, dims, kShapeDims[sh][q], 0) : 0;
+  } else if (o.kind == SL_ITEM_BLOCK) {
+    const char *keys[3] = {"w", "h", "d"};
+    for (int q = 0; q < 3; q++) o.dims[q] = dims ? get_num(env, dims, keys[q], 0) : 0;
+  } else if (o.kind == SL_ITEM_DRAIN) {
+    o.dims[0] = get_num(env, it, "r", 0.025);
+  } else {
+    const std::string mode = get_str(env, it, "mode");
+    const int m = name_index(mode, kModes, SL_MODE_COUNT_);
+    if (m < 0) { err = "unknown source mode '" + mode + "'"; return false; }
+    o.shape = (uint32_t)m;
+    bool en = true;
+    napi_has_named_property(env, it, "enabled", &has);
+    if (has) { napi_value v; napi_get_named_property(env, it, "enabled", &v); napi_get_value_bool(env, v, &en); }
+    o.enabled = en ? 1u : 0u;
+    o.pitch = get_num(env, it, "pitch", 0); o.yaw = get_num(env, it, "yaw", 0);
+    o.flow = get_num(env, it, "flow", 0); o.speed = get_num(env, it, "speed", 0);
+    o.drop_d = get_num(env, it, "dropD", 0); o.drop_rate = get_num(env, it, "dropRate", 0);
+    o.spread = get_num(env, it, "spread", 0); o.burst = get_num(env, it, "burst", 0);
+    o.burst_every = get_num(env, it, "burstEvery", 0); o.rain_rate = get_num(env, it, "rainRate", 0);
+    o.rain_size = get_num(env, it, "rainSize", 0); o.total = get_num(env, it, "total", 0);
+  }
+  return true;
+}
+static bool read_items(napi_env env, napi_value list, std::vector<sl_item> &out) {
+  bool isarr = false;
+  napi_is_array(env, list, &isarr);
+  if (!isarr) { napi_throw_type_error(env, nullptr, "items: an array of scene items"); return false; }
+  uint32_t n = 0;
+  napi_get_array_length(env, list, &n);
+  out.resize(n);
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value it;
+    napi_get_element(env, list, i, &it);
+    std::string err;
+    if (!item_from_js(env, it, out[i], err)) { napi_throw_type_error(env, nullptr, err.c_str()); return false; }
+  }
+  return true;
+}
+static napi_value vec_js(napi_env env, const float *v) {
+  napi_value a;
+  napi_create_array_with_length(env, 4, &a);
+  for (uint32_t i = 0; i < 4; i++) napi_set_element(env, a, i, num(env, v[i]));
+  return a;
+}
+static napi_value records_js(napi_env env, const sl_emit *e, uint32_t n) {
+  napi_value arr;
+  napi_create_array_with_length(env, n, &arr);
+  for (uint32_t i = 0; i < n; i++) {
+    napi_value o;
+    napi_create_object(env, &o);
+    set_num(env, o, "shape", e[i].shape);
+    napi_set_named_property(env, o, "a", vec_js(env, e[i].a));
+    napi_set_named_property(env, o, "b", vec_js(env, e[i].b));
+    napi_set_named_property(env, o, "c", vec_js(env, e[i].c));
+    napi_set_named_property(env, o, "d", vec_js(env, e[i].d));
+    set_num(env, o, "count", e[i].count);
+    set_num(env, o, "seed", e[i].seed);
+    napi_set_element(env, arr, i, o);
+  }
+  return arr;
+}
+
+// a scene handle: a standalone sl_scene (owned) or the one a simulation keeps (borrowed)
+struct SceneHandle {
+  sl_scene *sc = nullptr;
+  bool owned = true;
+  Handle *sim = nullptr;       // the simulation of a borrowed scene (its scene dies with it)
+  napi_ref sim_ref = nullptr;  // keeps that simulation's handle alive
+  napi_env env = nullptr;
+};
+static void scene_finalize(napi_env, void *data, void *) {
+  SceneHandle *h = (SceneHandle *)data;
+  if (h->sc && h->owned) sl_scene_destroy(h->sc);
+  if (h->sim_ref) napi_delete_reference(h->env, h->sim_ref);
+  delete h;
+}
+static sl_scene *get_scene(napi_env env, napi_value v) {
+  SceneHandle *h = nullptr;
+  if (napi_get_value_external(env, v, (void **)&h) != napi_ok || !h || !h->sc || (!h->owned && (!h->sim || !h->sim->sim))) {
+    napi_throw_error(env, nullptr, "suiri: invalid scene handle (or its simulation was destroyed)");
+    return nullptr;
+  }
+  return h->sc;
+}
+static napi_value throw_scene(napi_env env, int code, const char *what) {
+  std::string m = std::string("suiri scene error ") + std::to_string(code) + ": " + what;
+  napi_throw_error(env, nullptr, m.c_str());
+  return nullptr;
+}
+
+// sceneCreate(config) -> scene (host only: no device is touched)
+static napi_value SceneCreate(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  sl_config c;
+  sl_config_default(&c, "standard");
+  config_from_js(env, argv[0], c);
+  sl_scene *sc = nullptr;
+  int r = sl_scene_create(&c, &sc);
+  if (r) return throw_scene(env, r, "sceneCreate: bad configuration");
+  SceneHandle *h = new SceneHandle();
+  h->sc = sc;
+  h->env = env;
+  napi_value ext;
+  NAPI_CALL(env, napi_create_external(env, h, scene_finalize, nullptr, &ext));
+  return ext;
+}
+// simScene(h) -> the scene state the simulation keeps
+static napi_value SimScene(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  SceneHandle *h = new SceneHandle();
+  h->sc = sl_sim_scene(s);
+  h->owned = false;
+  napi_get_value_external(env, argv[0], (void **)&h->sim);
+  h->env = env;
+  napi_create_reference(env, argv[0], 1, &h->sim_ref);
+  napi_value ext;
+  NAPI_CALL(env, napi_create_external(env, h, scene_finalize, nullptr, &ext));
+  return ext;
+}
+// packObjects(scene, items, dtMove) -> {rows: Float32Array, n, moving}
+static napi_value PackObjects(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  sl_scene *sc = get_scene(env, argv[0]);
+  if (!sc) return nullptr;
+  std::vector<sl_item> items;
+  if (!read_items(env, argv[1], items)) return nullptr;
+  double dtm = 0;
+  napi_get_value_double(env, argv[2], &dtm);
+  float rows[SL_MAX_OBJ * 28];
+  uint32_t n = 0;
+  int moving = 0;
+  int r = sl_scene_pack_objects(sc, items.data(), (uint32_t)items.size(), dtm, rows, &n, &moving);
+  if (r) return throw_scene(env, r, "packObjects");
+  napi_value ab, ta, o;
+  void *data = nullptr;
+  NAPI_CALL(env, napi_create_arraybuffer(env, (size_t)n * 28 * 4, &data, &ab));
+  if (n) memcpy(data, rows, (size_t)n * 28 * 4);
+  NAPI_CALL(env, napi_create_typedarray(env, napi_float32_array, (size_t)n * 28, ab, 0, &ta));
+  napi_create_object(env, &o);
+  napi_set_named_property(env, o, "rows", ta);
+  set_num(env, o, "n", n);
+  napi_set_named_property(env, o, "moving", boolean(env, moving != 0));
+  return o;
+}
+// setItems(h, items, dtMove) -> moving (FluidSim.setObjects)
+static napi_value SetItems(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  std::vector<sl_item> items;
+  if (!read_items(env, argv[1], items)) return nullptr;
+  double dtm = 0;
+  napi_get_value_double(env, argv[2], &dtm);
+  int moving = 0;
+  int r = sl_set_items(s, items.data(), (uint32_t)items.size(), dtm, &moving);
+  if (r) return throw_sl(env, s, r);
+  return boolean(env, moving != 0);
+}
+// sourceEmits(scene, items, dt) -> records (sourceEmits, 55-emit.js:13-69)
+static napi_value SourceEmits(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  sl_scene *sc = get_scene(env, argv[0]);
+  if (!sc) return nullptr;
+  std::vector<sl_item> items;
+  if (!read_items(env, argv[1], items)) return nullptr;
+  double dt = 0;
+  napi_get_value_double(env, argv[2], &dt);
+  std::vector<sl_emit> out(items.size() + 1);
+  uint32_t n = 0;
+  int r = sl_scene_source_emits(sc, items.data(), (uint32_t)items.size(), dt, out.data(), (uint32_t)out.size(), &n);
+  if (r) return throw_scene(env, r, "sourceEmits");
+  return records_js(env, out.data(), n);
+}
+// blockEmits(scene, items, fill) -> records (blockEmits, 55-emit.js:83-106)
+static napi_value BlockEmits(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  size_t argc = 3;
+  napi_get_cb_info(env, info, &argc, argv, nullptr, nullptr);
+  if (argc < 2) { napi_throw_type_error(env, nullptr, "blockEmits(scene, items, fill)"); return nullptr; }
+  sl_scene *sc = get_scene(env, argv[0]);
+  if (!sc) return nullptr;
+  std::vector<sl_item> items;
+  if (!read_items(env, argv[1], items)) return nullptr;
+  bool fill = false;
+  if (argc >= 3) napi_get_value_bool(env, argv[2], &fill);
+  std::vector<sl_emit> out(items.size() + 1);
+  uint32_t n = 0;
+  int r = sl_scene_block_emits(sc, items.data(), (uint32_t)items.size(), fill ? 1 : 0, out.data(),
+                               (uint32_t)out.size(), &n);
+  if (r) return throw_scene(env, r, "blockEmits");
+  return records_js(env, out.data(), n);
+}
+// blockLiters(block, tank{w, h, d}) -> litres inside the tank (55-emit.js:108-113)
+static napi_value BlockLiters(napi_env env, napi_callback_info info) {
+  napi_value argv[2];
+  if (!get_args(env, info, 2, argv)) return nullptr;
+  sl_item it;
+  std::string err;
+  if (!item_from_js(env, argv[0], it, err)) { napi_throw_type_error(env, nullptr, err.c_str()); return nullptr; }
+  return num(env, sl_block_liters(&it, get_num(env, argv[1], "w", 0), get_num(env, argv[1], "h", 0),
+                                  get_num(env, argv[1], "d", 0)));
+}
+// stepItems(h, frameDt, items): FluidSim.step with the sources of `items` as the emitter
+static napi_value StepItems(napi_env env, napi_callback_info info) {
+  napi_value argv[3];
+  if (!get_args(env, info, 3, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  double fdt = 0;
+  napi_get_value_double(env, argv[1], &fdt);
+  std::vector<sl_item> items;
+  if (!read_items(env, argv[2], items)) return nullptr;
+  int r = sl_step_items(s, fdt, items.data(), (uint32_t)items.size());
+  if (r) return throw_sl(env, s, r);
+  return nullptr;
+}
+
 static napi_value Version(napi_env env, napi_callback_info) { return num(env, sl_version()); }
 static napi_value BuildInfo(napi_env env, napi_callback_info) {
   napi_value v;
@@ -890,6 +1186,14 @@ static napi_value Init(napi_env env, napi_value exports) {
       {"serverJobSim", 0, ServerJobSim, 0, 0, 0, napi_default, 0},
       {"serverStats", 0, ServerStats, 0, 0, 0, napi_default, 0},
       {"serverDestroy", 0, ServerDestroy, 0, 0, 0, napi_default, 0},
+      {"sceneCreate", 0, SceneCreate, 0, 0, 0, napi_default, 0},
+      {"simScene", 0, SimScene, 0, 0, 0, napi_default, 0},
+      {"packObjects", 0, PackObjects, 0, 0, 0, napi_default, 0},
+      {"setItems", 0, SetItems, 0, 0, 0, napi_default, 0},
+      {"sourceEmits", 0, SourceEmits, 0, 0, 0, napi_default, 0},
+      {"blockEmits", 0, BlockEmits, 0, 0, 0, napi_default, 0},
+      {"blockLiters", 0, BlockLiters, 0, 0, 0, napi_default, 0},
+      {"stepItems", 0, StepItems, 0, 0, 0, napi_default, 0},
   };
   napi_define_properties(env, exports, sizeof d / sizeof d[0], d);
   return exports;

@@ -25,6 +25,7 @@ struct Job {
   sl_job spec{};
   std::vector<float> objects;
   std::vector<sl_emit> fill;
+  std::vector<sl_item> items;
   std::string bake_dir;
   uint32_t baked = 0;   // bake frames written
   int32_t status = SL_JOB_QUEUED;
@@ -137,16 +138,20 @@ extern "C" const char *sl_server_last_error(const sl_server *s) { return s ? s->
 extern "C" int sl_server_submit(sl_server *s, const sl_job *job, uint64_t *id) {
   if (!s || !job) return SL_EARG;
   if (!(job->dt > 0) || !job->steps) return s->fail(SL_EARG, "sl_server_submit: steps and dt must be positive");
-  if ((job->n_objects && !job->objects) || (job->n_fill && !job->fill))
-    return s->fail(SL_EARG, "sl_server_submit: null object or fill list");
+  if ((job->n_objects && !job->objects) || (job->n_fill && !job->fill) || (job->n_items && !job->items))
+    return s->fail(SL_EARG, "sl_server_submit: null object, fill or item list");
+  if (job->n_objects && job->n_items)
+    return s->fail(SL_EARG, "sl_server_submit: give the obstacles as packed objects or as items, not both");
   if (job->cfg.device != s->cfg.device) return s->fail(SL_EARG, "sl_server_submit: job for another device");
   Job j;
   j.id = s->next_id++;
   j.spec = *job;
   j.objects.assign(job->objects, job->objects + (size_t)job->n_objects * 28);
   j.fill.assign(job->fill, job->fill + job->n_fill);
+  if (job->n_items) j.items.assign(job->items, job->items + job->n_items);
   j.spec.objects = nullptr;
   j.spec.fill = nullptr;
+  j.spec.items = nullptr;
   if (job->bake_dir) j.bake_dir = job->bake_dir;
   j.spec.bake_dir = nullptr;
   j.bytes = sl_job_bytes(&job->cfg);
@@ -267,8 +272,14 @@ extern "C" int sl_server_tick(sl_server *s, uint32_t *running) {
       continue;
     }
     sl_sim *sim = sl_batch_sim(s->batch, slot);
-    if ((r = sl_set_objects(sim, j.objects.data(), (uint32_t)(j.objects.size() / 28), nullptr)) ||
+    const uint32_t ni = (uint32_t)j.items.size();
+    std::vector<sl_emit> blocks(ni);
+    uint32_t nb = 0;
+    if ((r = ni ? sl_set_items(sim, j.items.data(), ni, 0, nullptr)
+                : sl_set_objects(sim, j.objects.data(), (uint32_t)(j.objects.size() / 28), nullptr)) ||
         (j.fill.size() && (r = sl_emit_now(sim, j.fill.data(), (uint32_t)j.fill.size(), 0, nullptr))) ||
+        (ni && (r = sl_scene_block_emits(sl_sim_scene(sim), j.items.data(), ni, 0, blocks.data(), ni, &nb))) ||
+        (nb && (r = sl_emit_now(sim, blocks.data(), nb, 0, nullptr))) ||
         (r = sl_batch_set_dt(s->batch, slot, j.spec.dt))) {
       std::string m = sl_last_error(sim);
       sl_batch_remove(s->batch, slot);
@@ -305,13 +316,22 @@ extern "C" int sl_server_tick(sl_server *s, uint32_t *running) {
       s->emit_counts[slot] = 0;
       if (!s->slot_job[slot]) continue;
       Job &j = s->jobs[s->slot_job[slot]];
-      if (j.status != SL_JOB_RUNNING || !j.spec.source) continue;
+      if (j.status != SL_JOB_RUNNING || (!j.spec.source && j.items.empty())) continue;
       sl_sim *sim = sl_batch_sim(s->batch, slot);
-      double time = 0;
-      sl_get_scalar(sim, "time", &time);
       sl_emit list[SL_MAX_EMIT];
-      uint32_t n = j.spec.source(j.spec.user, sim, j.spec.dt, time, list, SL_MAX_EMIT);
-      n = std::min<uint32_t>(n, SL_MAX_EMIT);
+      uint32_t n = 0;
+      if (j.spec.source) {
+        double time = 0;
+        sl_get_scalar(sim, "time", &time);
+        n = std::min<uint32_t>(j.spec.source(j.spec.user, sim, j.spec.dt, time, list, SL_MAX_EMIT), SL_MAX_EMIT);
+      }
+      if (!j.items.empty()) {
+        uint32_t m = 0;
+        if ((r = sl_scene_source_emits(sl_sim_scene(sim), j.items.data(), (uint32_t)j.items.size(), j.spec.dt,
+                                       list + n, SL_MAX_EMIT - n, &m)))
+          return s->fail(r, "sources of job " + std::to_string(j.id));
+        n += m;
+      }
       s->emit_buf.insert(s->emit_buf.end(), list, list + n);
       s->emit_counts[slot] = n;
       any_emit |= n > 0;

@@ -52,7 +52,14 @@ struct SlTank {
   float ww_rate, ww_cap_f, friction, time_f, radius, R_f;
   // ripple uniform (64-ripple.js:223-226)
   float rLx, rLz, rdt, rmd, rsr, rnu, rfilm, rhpk, rrho;
-  float pad_f[3];
+  float rdepth;  // fixed ripple depth (> 0), else the mean depth of the count
+  // splashCrown constants (31-wgsl-sim2.js:121-126), each the f32 rounding of the reference's
+  // constant expression: pi * rho, rho * sigma (folded in double), rho, sigma, mu, K thresholds
+  float sp_pirho, sp_rs, sp_rho, sp_sigma, sp_mu, sp_kjet, sp_kcrown;
+  // sub-grid droplets per frame: count, seed, lognormal median and sigma, speed range, mass factor
+  // f32(pi / 6 * rho)
+  uint32_t drops, drop_seed;
+  float drop_d50, drop_sig, drop_vmin, drop_vmax, drop_mc;
   // host-precision values for the device-side frame advance of batched steps
   double time_d, h_d, vp_d, rim_d;
   double dt_frame;  // a batched tank's own frame length (used when the batch step passes dt = 0)
@@ -119,6 +126,7 @@ __device__ __forceinline__ float sl_pow(float x, float y) { return (float)pow((d
 __device__ __forceinline__ float sl_sin(float x) { return (float)sin((double)x); }
 __device__ __forceinline__ float sl_cos(float x) { return (float)cos((double)x); }
 __device__ __forceinline__ float sl_sinh(float x) { return (float)sinh((double)x); }
+__device__ __forceinline__ float sl_log(float x) { return (float)log((double)x); }
 __device__ __forceinline__ int32_t f2i(float f) {
   if (f != f) return 0;
   if (f >= 2147483648.0f) return INT32_MAX;
@@ -394,6 +402,7 @@ void sl_k_compact_count(const SlLaunch &L);
 void sl_k_compact_scan(const SlLaunch &L, int ntanks);
 void sl_k_compact_move(const SlLaunch &L);
 void sl_k_ww(const SlLaunch &L, float dt);
+void sl_k_droplets(const SlLaunch &L);  // one thread per droplet (SL_WG per workgroup)
 void sl_k_clear_i32(const SlLaunch &L, int which);
 // spatial re-ordering (rocPRIM radix sort of cell-index keys)
 size_t sl_sort_temp_bytes(uint32_t n, int end_bit);

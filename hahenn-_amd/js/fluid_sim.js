@@ -7,61 +7,10 @@
 //   destroy(), waterLiters, and the host fields (nx, ny, nz, h, origin, rim, cap, vp, count, ...).
 // The GPU buffers stay on the device; readback()/upload() move them in the reference layouts.
 const path = require('path');
-const { m3, v3, seedStream } = require('./util');
-const { shapeParams, boundRadius, obstacleRot } = require('./scene');
 
 const addon = require(path.join(__dirname, '..', 'suiri_napi.node'));
 
 const MAX_OBJ = 64;
-
-// obstacles and drains -> packed 28-float object records; velocities from the previous transforms
-// `prevXf` (id -> {pos, R}) (50-sim.js:165-211). Returns {data, n, nextXf, moving}.
-function packObjects(items, prevXf, dtMove) {
-  const data = new Float32Array(MAX_OBJ * 28);
-  let n = 0;
-  const nextXf = new Map();
-  let moving = false;
-  for (const it of items) {
-    if (n >= MAX_OBJ) break;
-    if (it.kind !== 'obstacle' && it.kind !== 'drain') continue;
-    const R = it.kind === 'obstacle' ? obstacleRot(it) : m3.euler(it.rot[0], it.rot[1], it.rot[2]);
-    const RT = m3.transpose(R);
-    let type, d, flag, bound;
-    if (it.kind === 'obstacle') {
-      const sp = shapeParams(it);
-      type = sp[0]; d = sp.slice(1); flag = 1; bound = boundRadius(it) + 0.002;
-    } else {
-      type = 9; d = [it.r, 0.012, 0, 0]; flag = 2; bound = it.r + 0.02;
-    }
-    let lin = [0, 0, 0], ang = [0, 0, 0];
-    const prev = prevXf.get(it.id);
-    if (prev && dtMove > 0) {
-      lin = v3.mul(v3.sub(it.pos, prev.pos), 1 / dtMove);
-      ang = v3.mul(m3.log(m3.mul(R, m3.transpose(prev.R))), 1 / dtMove);
-      if (v3.len(lin) > 1e-5 || v3.len(ang) > 1e-4) moving = true;
-      if (v3.len(lin) > 4) lin = v3.mul(v3.norm(lin), 4);
-      if (v3.len(ang) > 40) ang = v3.mul(v3.norm(ang), 40);
-    }
-    nextXf.set(it.id, { pos: [...it.pos], R });
-    const o = n * 28;
-    data.set([RT[0], RT[1], RT[2], 0, RT[3], RT[4], RT[5], 0, RT[6], RT[7], RT[8], 0], o);
-    data.set([it.pos[0], it.pos[1], it.pos[2], type], o + 12);
-    data.set([d[0], d[1], d[2], d[3]], o + 16);
-    data.set([lin[0], lin[1], lin[2], bound], o + 20);
-    data.set([ang[0], ang[1], ang[2], flag], o + 24);
-    n++;
-  }
-  return { data: data.subarray(0, n * 28), n, nextXf, moving };
-}
-
-// grid of a tank configuration, 50-sim.js:33-45: {nx, ny, nz, h, origin, rim, vp, radius}
-function gridOf(cfg) {
-  const nx = Math.max(8, Math.round(cfg.tank_w / cfg.h));
-  const ny = Math.max(8, Math.round(cfg.tank_h / cfg.h));
-  const nz = Math.max(8, Math.round(cfg.tank_d / cfg.h));
-  const h = Math.min(cfg.tank_w / nx, cfg.tank_h / ny, cfg.tank_d / nz);
-  return { nx, ny, nz, h, origin: [-nx * h / 2, 0, -nz * h / 2], rim: ny * h, vp: Math.pow(h / 2, 3), radius: 0.5 * h * 1.05 };
-}
 const FIELDS = {
   particles: 0, ww: 1, counters: 2, acc: 3, cellCount: 4, vel: 5, velOld: 6, frac: 7, svel: 8, cellType: 9,
   divg: 10, pres: 11, objs: 12, rsrc: 13, rippleState: 14, rippleHat: 15, rippleField: 16, rippleMask: 17,
@@ -83,12 +32,14 @@ class FluidSim {
     this.nx = g.nx; this.ny = g.ny; this.nz = g.nz; this.h = g.h;
     this.origin = g.origin; this.rim = g.rim; this.nodes = g.nodes; this.cells = g.cells;
     this.cap = g.cap; this.wwCap = g.wwCap; this.radius = g.radius;
-    this.vp = Math.pow(this.h / 2, 3);  // 50-sim.js:44, in JS like the reference (the emitters use it)
+    this.vp = g.vp;
     this.iters = g.iters;
     this.ripple = { R: g.R, M: g.M };
-    this.emitAcc = new Map();
-    this.prevXf = new Map();
-    this.seed = seedStream(opts.seed === undefined ? ((Math.random() * 4294967295) >>> 0) : opts.seed);
+    // the library keeps the sources' progress, the objects' previous poses and the seed stream of the
+    // emission records (cfg.seed); records handed in without a seed take one from a stream here
+    this.scene = addon.simScene(this.handle);
+    this._seedState = (opts.seed === undefined ? ((Math.random() * 4294967295) >>> 0) : opts.seed) >>> 0;
+    this.objCount = 0;
     this.sceneItems = null;
     this.dead = false;
     this._pull();
@@ -100,18 +51,32 @@ class FluidSim {
     this.full = !!st.full; this.impacts = st.impacts; this.wwCount = st.wwCount;
   }
 
+  _nextSeed() {
+    this._seedState = (Math.imul(this._seedState, 747796405) + 2891336453) >>> 0;
+    return this._seedState;
+  }
   _seeded(list) {
-    return list.map((e) => ({ ...e, seed: e.seed === undefined ? this.seed() : e.seed }));
+    return list.map((e) => (e.seed === undefined ? { ...e, seed: this._nextSeed() } : e));
   }
 
-  // obstacles and drains -> packed 28-float object records; velocities from the previous transforms
-  // (50-sim.js:165-211). Returns whether any object moved.
+  // obstacles and drains -> object records with velocities from the previous poses, packed and
+  // uploaded by the library (50-sim.js:165-211). Returns whether any object moved.
   setObjects(items, dtMove) {
-    const { data, n, nextXf, moving } = packObjects(items, this.prevXf, dtMove);
-    this.prevXf = nextXf;
-    this.objCount = n;
-    addon.setObjects(this.handle, data);
+    const moving = addon.setItems(this.handle, items, dtMove || 0);
+    this.objCount = items.filter((it) => it.kind === 'obstacle' || it.kind === 'drain').length;
+    if (this.objCount > MAX_OBJ) this.objCount = MAX_OBJ;
     return moving;
+  }
+
+  // emission records of the sources among `items` for one substep of dt (sourceEmits, 55-emit.js:13-69)
+  sourceEmits(items, dt) { return addon.sourceEmits(this.scene, items, dt); }
+  // lattice records of the blocks among `items` (blockEmits, 55-emit.js:83-106)
+  blockEmits(items, fill) { return addon.blockEmits(this.scene, items, !!fill); }
+  // a frame whose substeps emit the sources among `items`, all in the library
+  stepItems(frameDt, items) {
+    if (this.dead || !(frameDt > 0)) return;
+    addon.stepItems(this.handle, frameDt, items);
+    this._pull();
   }
 
   // frame_%05d.npz (+ surface_%05d.exr) of the current state, the reference's bake cache layout
@@ -165,21 +130,16 @@ class FluidSim {
 
   idle() { addon.sync(this.handle); }
 
+  // the library resets the sources' progress with the water, and saves / restores it with the copy
   clearWater() {
     addon.clearWater(this.handle);
-    this.emitAcc.clear();
     this._pull();
   }
 
-  saveWater() {
-    const id = addon.saveWater(this.handle);
-    this.saved = { id, acc: new Map([...this.emitAcc].map(([k, v]) => [k, { ...v }])) };
-    return id;
-  }
+  saveWater() { return addon.saveWater(this.handle); }
   hasWaterCopy(id) { return addon.hasWaterCopy(this.handle, id); }
   restoreWater(id) {
     if (!addon.restoreWater(this.handle, id)) return false;
-    if (this.saved && this.saved.id === id) this.emitAcc = new Map([...this.saved.acc].map(([k, v]) => [k, { ...v }]));
     this._pull();
     return true;
   }
@@ -213,4 +173,4 @@ class FluidSim {
   get waterLiters() { return this.count * this.vp * 1000; }
 }
 
-module.exports = { FluidSim, FIELDS, addon, packObjects, gridOf, QUALITY };
+module.exports = { FluidSim, FIELDS, addon, QUALITY, MAX_OBJ };

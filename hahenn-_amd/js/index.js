@@ -1,9 +1,9 @@
 'use strict';
-// Node entry point: the FluidSim driver, the scene model and emitters, and a headless counterpart of
-// the reference's automation API (window.suiriLab, src/js/90-main.js:145-205) without the DOM parts.
+// Node entry point: the FluidSim driver, the scene model, and a headless counterpart of the
+// reference's automation API (window.suiriLab, src/js/90-main.js:145-205) without the DOM parts.
+// Object packing and the emitters run in the library (FluidSim.setObjects / sourceEmits / blockEmits).
 const scene = require('./scene');
-const emit = require('./emit');
-const { FluidSim, FIELDS, addon, packObjects, gridOf } = require('./fluid_sim');
+const { FluidSim, FIELDS, addon } = require('./fluid_sim');
 const { BakeServer } = require('./server');
 
 // one scene + one simulation, driven like the reference's headless runs
@@ -21,12 +21,12 @@ class SuiriLab {
     this.sim = new FluidSim(this.scene.tank, this.quality, { ...this.opts, ...(this.scene.solver || {}) });
     this.sim.sceneItems = this.scene.items;
     this.sim.setObjects(this.scene.items, 0);
-    if (refill) this.sim.emitNow(emit.blockEmits(this.sim, this.scene.items));
+    if (refill) this.sim.emitNow(this.sim.blockEmits(this.scene.items));
   }
   restartWater() {
     this.sim.clearWater();
     this.sim.setObjects(this.scene.items, 0);
-    this.sim.emitNow(emit.blockEmits(this.sim, this.scene.items));
+    this.sim.emitNow(this.sim.blockEmits(this.scene.items));
   }
   resetSim() {
     const copy = this.sim.saveWater();
@@ -42,7 +42,7 @@ class SuiriLab {
 
   // run(seconds, dt): setObjects + step + wait + pollStats per frame (90-main.js:151-165)
   run(seconds, dt = 1 / 120) {
-    const emitFn = (sim, d, t) => emit.sourceEmits(sim, d, t, this.scene.items);
+    const emitFn = (sim, d) => sim.sourceEmits(this.scene.items, d);
     for (let t = 0; t < seconds - 1e-9; t += dt) {
       this.sim.setObjects(this.scene.items, dt);
       this.sim.step(dt, emitFn);
@@ -75,4 +75,7 @@ class SuiriLab {
   destroy() { if (this.sim) this.sim.destroy(); }
 }
 
-module.exports = { FluidSim, FIELDS, addon, SuiriLab, BakeServer, packObjects, gridOf, ...scene, ...emit };
+// litres of a block inside a tank {w, h, d} (blockLiters, 55-emit.js:108-113)
+const blockLiters = (tank, block) => addon.blockLiters(block, tank);
+
+module.exports = { FluidSim, FIELDS, addon, SuiriLab, BakeServer, blockLiters, ...scene };

@@ -1,8 +1,12 @@
 'use strict';
-// Scene model and scene-JSON schema of the reference (src/js/10-scene.js, src/js/80-ui.js:119-121,
-// :430-460): item factories with their defaults, obstacle shape parameters for the GPU, the six
-// presets, and the file validation that fills anything missing from a fresh item of the same kind.
-const { clamp, m3 } = require('./util');
+// Scene model and scene-file schema of the reference (src/js/10-scene.js:5-87, :254-336; the file
+// check of src/js/80-ui.js:430-460): item factories with their defaults, the six presets, and the
+// table-driven validation that takes every field a file may set from the file when it has the right
+// type and from a fresh item of the same kind otherwise. Object packing and the emitters run in the
+// library (sl_scene_* of include/suiri.h), reached through FluidSim.
+
+const clamp = (x, a, b) => Math.min(b, Math.max(a, x));
+const has = (table, key) => typeof key === 'string' && Object.prototype.hasOwnProperty.call(table, key);
 
 // obstacle shapes: `type` is the shape id of the signed-distance code (10-scene.js:5-15)
 const SHAPES = {
@@ -46,89 +50,65 @@ function makeDrain(pos) {
   return { id: uid++, kind: 'drain', name: 'drain', pos: pos ? [...pos] : [0, 0, 0], rot: [0, 0, 0], r: 0.025 };
 }
 
-// outlet geometry from flow and speed (10-scene.js:91-99)
-function sourceNozzle(src) {
-  const Q = src.flow / 60000;
-  const v = Math.max(src.speed, 0.05);
-  return Math.sqrt(4 * Q / (Math.PI * v));
-}
-function sourceDir(src) {
-  const p = src.pitch * Math.PI / 180, y = src.yaw * Math.PI / 180;
-  return [Math.cos(p) * Math.sin(y), Math.sin(p), Math.cos(p) * Math.cos(y)];
-}
-
-// GPU parameters of an obstacle: [type, d0, d1, d2, d3] (10-scene.js:143-157)
-function shapeParams(o) {
-  const d = o.dims;
-  switch (o.shape) {
-    case 'box': return [0, d.w / 2, d.h / 2, d.d / 2, Math.min(0.003, d.w / 4, d.h / 4, d.d / 4)];
-    case 'plate': return [0, d.w / 2, d.t / 2, d.d / 2, Math.min(0.002, d.t / 4)];
-    case 'sphere': return [1, d.r, 0, 0, 0];
-    case 'cylinder': return [2, d.r, d.h / 2, 0, 0];
-    case 'cone': return [3, d.r, d.h / 2, 0, 0];
-    case 'torus': return [4, d.R, d.r, 0, 0];
-    case 'cup': return [5, d.r, d.h / 2, Math.min(d.t, d.r * 0.8), 0];
-    case 'container': return [6, d.w / 2, d.h / 2, d.d / 2, Math.min(d.t, d.w / 3, d.d / 3, d.h * 0.8)];
-    case 'gutter': return [7, d.len / 2, d.r, d.t, 0];
-    default: return [0, 0.05, 0.05, 0.05, 0];
-  }
-}
-// bounding-sphere radius of an obstacle (10-scene.js:177-188)
-function boundRadius(o) {
-  const [t, a, b, c] = shapeParams(o);
-  switch (t) {
-    case 0: case 6: return Math.hypot(a, b, c);
-    case 1: return a;
-    case 2: case 3: case 5: return Math.hypot(a, b);
-    case 4: return a + b;
-    case 7: return Math.hypot(a, b + c);
-    default: return 0.1;
-  }
-}
-function obstacleRot(o) { return m3.euler(o.rot[0], o.rot[1], o.rot[2]); }
-
 // the file format {v:1, app:'suiri-lab', scene, cam} (80-ui.js:121)
 function sceneFile(scene, cam) { return { v: 1, app: 'suiri-lab', scene, cam: cam || null }; }
 
-// a scene read from a file: every field is checked against a fresh item of the same kind, and
-// anything missing or unusable takes that item's value; tank sizes are clamped (80-ui.js:430-460)
+// How a field read from a file is checked: (file value, fresh item's value) -> value.
+const RULES = {
+  num: (v, d) => (Number.isFinite(v) ? v : d),
+  bool: (v, d) => (typeof v === 'boolean' ? v : d),
+  text: (v, d) => (typeof v === 'string' && v ? v : d),
+  vec3: (v, d) => (Array.isArray(v) && v.length === 3 && v.every(Number.isFinite) ? [...v] : d),
+  // every dimension of the fresh item's shape, each a finite number or the fresh value
+  dims: (v, d) => Object.fromEntries(Object.keys(d).map((k) => [k, Number.isFinite(v && v[k]) ? v[k] : d[k]])),
+  material: (v, d) => (has(MATERIALS, v) ? v : d),
+  mode: (v, d) => (has(SOURCE_MODES, v) ? v : d),
+};
+// Per item kind: the fresh item a file's item is checked against, and the rule of every field of it
+// besides id, kind and shape (the obstacle's shape picks the fresh item: an unknown shape is a box).
+const SCHEMA = {
+  obstacle: {
+    fresh: (it) => makeObstacle(has(SHAPES, it.shape) ? it.shape : 'box'),
+    fields: { name: 'text', pos: 'vec3', rot: 'vec3', dims: 'dims', color: 'text', material: 'material' },
+  },
+  source: {
+    fresh: () => makeSource(),
+    fields: {
+      name: 'text', pos: 'vec3', pitch: 'num', yaw: 'num', mode: 'mode', enabled: 'bool', flow: 'num', speed: 'num',
+      dropD: 'num', dropRate: 'num', spread: 'num', burst: 'num', burstEvery: 'num', rainRate: 'num', rainSize: 'num',
+      total: 'num',
+    },
+  },
+  block: { fresh: () => makeBlock(), fields: { name: 'text', pos: 'vec3', dims: 'dims' } },
+  drain: { fresh: () => makeDrain(), fields: { name: 'text', pos: 'vec3', rot: 'vec3', r: 'num' } },
+};
+// tank sizes a file may set (80-ui.js:459)
+const TANK_LIMITS = { w: [0.2, 1.2], h: [0.1, 0.6], d: [0.15, 1.0] };
+
+// A scene read from a file {v, app, scene: {tank, items}, solver?}: null unless the tank has three
+// positive sizes and the items are a list; items of unknown kinds are dropped; ids stay when they are
+// positive integers not seen before, the others take fresh ids above the largest kept one.
 function validScene(o) {
   if (!o || !o.scene || !o.scene.tank || !Array.isArray(o.scene.items)) return null;
   const t = o.scene.tank;
-  if (![t.w, t.h, t.d].every((x) => Number.isFinite(x) && x > 0)) return null;
-  const fresh = {
-    obstacle: (it) => makeObstacle(SHAPES[it.shape] ? it.shape : 'box'),
-    source: () => makeSource(), block: () => makeBlock(), drain: () => makeDrain(),
-  };
-  const num = (v, d) => (Number.isFinite(v) ? v : d);
-  const vec = (v, d) => (Array.isArray(v) && v.length === 3 && v.every(Number.isFinite) ? [...v] : d);
+  if (!Object.keys(TANK_LIMITS).every((k) => Number.isFinite(t[k]) && t[k] > 0)) return null;
   const items = [];
   const ids = new Set();
   for (const it of o.scene.items) {
-    if (!it || !fresh[it.kind]) continue;
-    const base = fresh[it.kind](it);
-    const out = { ...base };
-    for (const [k, v] of Object.entries(base)) {
-      if (k === 'id') continue;
-      if (typeof v === 'number') out[k] = num(it[k], v);
-      else if (typeof v === 'boolean') out[k] = typeof it[k] === 'boolean' ? it[k] : v;
-      else if (typeof v === 'string') out[k] = typeof it[k] === 'string' && it[k] ? it[k] : v;
-      else if (Array.isArray(v)) out[k] = vec(it[k], v);
-      else if (v && typeof v === 'object') {
-        out[k] = Object.fromEntries(Object.entries(v).map(([dk, dv]) => [dk, num(it[k] && it[k][dk], dv)]));
-      }
-    }
-    if (it.kind === 'obstacle') { out.shape = base.shape; if (!MATERIALS[out.material]) out.material = base.material; }
-    if (it.kind === 'source' && !SOURCE_MODES[out.mode]) out.mode = base.mode;
+    if (!it || !has(SCHEMA, it.kind)) continue;
+    const spec = SCHEMA[it.kind];
+    const out = spec.fresh(it);
+    for (const [field, rule] of Object.entries(spec.fields)) out[field] = RULES[rule](it[field], out[field]);
     out.id = Number.isInteger(it.id) && it.id > 0 && !ids.has(it.id) ? it.id : null;
     if (out.id !== null) ids.add(out.id);
     items.push(out);
   }
   let next = Math.max(0, ...ids) + 1;
   for (const it of items) if (it.id === null) it.id = next++;
-  const scene = { tank: { w: clamp(t.w, 0.2, 1.2), h: clamp(t.h, 0.1, 0.6), d: clamp(t.d, 0.15, 1.0) }, items };
-  // the solver block is an addition of this build (solver sizes and physics for batch jobs); old
-  // files without it load unchanged because unknown top-level keys are ignored
+  const tank = Object.fromEntries(Object.entries(TANK_LIMITS).map(([k, [lo, hi]]) => [k, clamp(t[k], lo, hi)]));
+  const scene = { tank, items };
+  // the solver block is an addition of this build (solver sizes and physics for batch jobs); files
+  // without it load unchanged because unknown top-level keys are ignored
   if (o.solver && typeof o.solver === 'object') scene.solver = { ...o.solver };
   return scene;
 }
@@ -179,6 +159,6 @@ const PRESETS = [
 ];
 
 module.exports = {
-  SHAPES, MATERIALS, SOURCE_MODES, makeObstacle, makeSource, makeBlock, makeDrain, setNextId,
-  sourceNozzle, sourceDir, shapeParams, boundRadius, obstacleRot, sceneFile, validScene, PRESETS,
+  SHAPES, MATERIALS, SOURCE_MODES, SCHEMA, makeObstacle, makeSource, makeBlock, makeDrain, setNextId, sceneFile,
+  validScene, PRESETS,
 };

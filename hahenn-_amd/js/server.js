@@ -1,12 +1,10 @@
 'use strict';
 // Serving loop for Node: scene-bake jobs (a scene JSON of the reference's schema + frame count)
 // admitted against a device-memory budget and stepped together on the GPU, one frame per tick
-// (sl_server_* through the N-API addon). A job's sources emit through the reference's emitters
-// (55-emit.js) called on the JS thread during tick; its results are the same as a plain FluidSim
-// run of the scene with one substep per frame.
-const { addon, packObjects, gridOf, QUALITY, FIELDS } = require('./fluid_sim');
-const { seedStream } = require('./util');
-const emit = require('./emit');
+// (sl_server_* through the N-API addon). The library packs a job's objects and fills its blocks at
+// admission and emits its sources every frame (sl_job.items), with the emitters a FluidSim uses, so
+// a job's results are the same as a plain FluidSim run of the scene with one substep per frame.
+const { addon, QUALITY, FIELDS } = require('./fluid_sim');
 
 class BakeServer {
   // opts: {budget (bytes), maxJobs, device, autotuneAfter, graphAfter, framesPerTick}
@@ -22,17 +20,9 @@ class BakeServer {
     const cfg = addon.configDefault(quality);
     cfg.tank_w = scene.tank.w; cfg.tank_h = scene.tank.h; cfg.tank_d = scene.tank.d;
     for (const [k, v] of Object.entries(scene.solver || {})) if (k in cfg) cfg[k] = typeof v === 'boolean' ? (v ? 1 : 0) : v;
-    // the emitters read the tank geometry and keep per-source accumulators, like a FluidSim
-    const g = gridOf(cfg);
-    const job = { h: g.h, origin: g.origin, vp: g.vp, sceneItems: scene.items, emitAcc: new Map() };
-    const seeds = seedStream(opts.seed === undefined ? 1 : opts.seed);
-    const seeded = (list) => list.map((e) => ({ ...e, seed: e.seed === undefined ? seeds() : e.seed }));
-    const { data } = packObjects(scene.items, new Map(), 0);
-    const hasSource = scene.items.some((it) => it.kind === 'source');
+    if (opts.seed !== undefined) cfg.seed = opts.seed;  // the seed stream of the job's emission records
     const id = addon.serverSubmit(this.handle, {
-      config: cfg, objects: data, fill: seeded(emit.blockEmits(job, scene.items)),
-      steps: opts.frames || 100, dt: opts.dt || 1 / 120, keep: opts.keep ? 1 : 0,
-      source: hasSource ? (dt, t) => seeded(emit.sourceEmits(job, dt, t, scene.items)) : undefined,
+      config: cfg, items: scene.items, steps: opts.frames || 100, dt: opts.dt || 1 / 120, keep: opts.keep ? 1 : 0,
       bakeDir: opts.bakeDir, bakeEvery: opts.bakeEvery || 0,
     });
     this.jobs.set(id, { scene, opts });

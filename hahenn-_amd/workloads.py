@@ -10,6 +10,10 @@ emitters), as configurations + object lists + fill records for the library.
 * tank_mix: independent tanks drawn per seed, grids 32^3 .. 128x64x64 (h = 1/96), fill fractions
   0.2 .. 0.7 as a dam-break block, 0 .. 3 random primitives, 40 .. 80 SOR iterations,
   dt = 1/120 s; whitewater on, ripple M = 256.
+* ripple_droplets: the droplet layer on a 128^3 tank (0.5 m cube, h = 1/256) filled 0.1 m deep with
+  a sphere standing out of the water; ripple M = 1024 over the 0.5 x 0.5 m surface with the fixed depth
+  0.1 m, sigma 0.072 N/m, rho 1000 kg/m^3; 4096 droplets per step, lognormal diameter (median 1.5 mm,
+  sigma_ln 0.4), speed 1 .. 4 m/s; whitewater when K = We Oh^-0.4 > 60; SOR omega 1.8 x 60, dt 1/240 s.
 
 Particles come from the reference's jittered half-cell lattice emitter (shape 2,
 src/js/31-wgsl-sim2.js:295-306) with recorded seeds; velocities start at zero.
@@ -112,23 +116,34 @@ def grid_of(cfg):
     return nx, ny, nz, hh, [-nx * hh / 2, 0.0, -nz * hh / 2]
 
 
+def ripple_droplets(steps=2000, R=512, droplets=4096):
+    """The droplet layer over a 128^3 tank filled 0.1 m deep (see the module docstring)."""
+    n = 128
+    h = 0.5 / n
+    tank = (0.5, 0.5, 0.5)
+    layers = int(round(0.1 / h))  # 26 layers = 0.1016 m of water
+    items = [scene.make_obstacle("sphere", pos=[0.1, 0.06, -0.08], dims={"r": 0.07})]
+    cfg = _cfg(tank, h, 60, R, n * layers * n * 8 + 200_000, 1_000_000, 0.95, 1.8, True)
+    cfg.update(ripple_depth=0.1, r_sigma_rho=0.072 / 1000.0, r_rho_kick=1000.0, splash_rho=1000.0,
+               splash_sigma=0.072, splash_mu=0.001, splash_k_jet=60.0, droplets=droplets, drop_seed=2024,
+               drop_d50=0.0015, drop_sigma_ln=0.4, drop_vmin=1.0, drop_vmax=4.0)
+    return dict(name="ripple-droplets-128^3", grid=(n, n, n), steps=steps, frame_dt=1.0 / 240, cfg=cfg,
+                items=items, blocks=[(0, n, 0, layers, 0, n)], seed=777, sources=False)
+
+
 def job_inputs(wl):
-    """(object rows, fill records, source emitter or None) of a workload, without a simulation."""
+    """(scene items, fill records) of a workload for a serving job: the job packs the items' obstacles
+    and drains at admission and emits their sources every frame (sl_job.items)."""
     nx, ny, nz, hh, origin = grid_of(wl["cfg"])
-    rows, _ = scene.ObjectPacker().pack(wl["items"], 0)
     fill = [scene.cell_block_emit(hh, origin, *blk, seed=wl["seed"] + i) for i, blk in enumerate(wl["blocks"])]
-    src = scene.Emitters(wl["items"], (hh / 2) ** 3, seed=wl["seed"] + 99) if wl["sources"] else None
-    return rows, fill, src
+    return wl["items"], fill
 
 
 def setup(sim, wl):
     """Objects and the initial fill of a workload on a Sim (or a tank of a Batch)."""
-    packer = scene.ObjectPacker()
-    rows, _ = packer.pack(wl["items"], 0)
-    sim.set_objects_rows(rows)
+    sim.set_items(wl["items"], 0)
     recs = [scene.cell_block_emit(sim.h, sim.origin, *blk, seed=wl["seed"] + i) for i, blk in enumerate(wl["blocks"])]
     sim.emit_now(recs)
-    return packer
 
 
 def step_bytes(nx, ny, nz, P, iters):

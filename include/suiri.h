@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define SL_ABI_VERSION 1
+#define SL_ABI_VERSION 2
 
 #define SL_OK 0
 #define SL_EARG -1     /* bad argument */
@@ -65,6 +65,23 @@ typedef struct sl_config {
                                     particles (63-wgsl-water.js:5-55, what the renderer draws for the
                                     next frame's ripple mask); 0: the map stays as uploaded (zeros, as
                                     in the reference's headless runs) */
+  uint32_t seed;                 /* seed of the stream of emission-record seeds (sl_scene_source_emits,
+                                    sl_scene_block_emits; the reference draws Math.random(), 50-sim.js:234) */
+  /* splash of a drop (splashCrown, 31-wgsl-sim2.js:119-148): liquid density, surface tension and
+     viscosity of K = We * Oh^-0.4 (998, 0.0728, 0.001 at :121-123); jets when K > splash_k_jet (700),
+     a crown ring of round(4 (K / splash_k_crown - 1)) droplets (2100, :125-126) */
+  double splash_rho, splash_sigma, splash_mu;
+  float splash_k_jet, splash_k_crown;
+  /* ripple depth D [m] of the dispersion relation: > 0 a fixed depth; 0 the mean depth of the
+     particle count, as the reference (50-sim.js:332) */
+  double ripple_depth;
+  /* sub-grid droplets deposited on the ripple layer each frame (no counterpart in the reference):
+     `droplets` per frame, diameter lognormal with median drop_d50 [m] and drop_sigma_ln (sigma of
+     ln D), speed uniform in [drop_vmin, drop_vmax] [m/s], impact points uniform over the tank
+     footprint at the water level; each deposits the impulse 2 m v (mass m from splash_rho) and
+     splashes as splashCrown. 0 = none. Not for the tanks of a batch. */
+  uint32_t droplets, drop_seed;
+  float drop_d50, drop_sigma_ln, drop_vmin, drop_vmax;
   uint32_t reserved[2];
 } sl_config;
 
@@ -132,10 +149,11 @@ enum sl_stage {
   SL_S_PROJECT = 5,    /* 30-wgsl-sim.js:299-341 */
   SL_S_G2P = 6,        /* 31-wgsl-sim2.js:149-251 */
   SL_S_COMPACT = 7,    /* COMPACT + COUNT(1), 31-wgsl-sim2.js:317-339 */
-  SL_S_WW = 8,         /* WW + COUNT(3) with dt = arg-th part... see sl_run_stage_f */
+  SL_S_WW = 8,         /* WW + COUNT(3), with the call's dt as the frame length */
   SL_S_RIPPLE = 9,     /* RippleLayer.encodeFrame, 64-ripple.js:222-253 */
   SL_S_SOR = 10,       /* the full SOR loop (iters x red, black) */
-  SL_S_COUNT_ = 11
+  SL_S_DROPLETS = 11,  /* the frame's sub-grid droplets (cfg.droplets) with the frame's ripple uniform */
+  SL_S_COUNT_ = 12
 };
 
 int sl_config_default(sl_config *cfg, const char *quality); /* "low" | "standard" | "high" */
@@ -152,6 +170,65 @@ int sl_grid_info(const sl_sim *sim, sl_grid_t *out);
    packing and the moving-object velocities (50-sim.js:171-201); an unchanged set is not re-uploaded
    and does not mark the geometry dirty. *changed (optional) receives 1 when it was. */
 int sl_set_objects(sl_sim *sim, const float *rows, uint32_t n, int *changed);
+
+/* ---- scene items (the scene model of src/js/10-scene.js:17-87 and its file schema) ----
+   One implementation of the host steps that turn a scene into device records, shared by every host
+   (Node addon, Python, the serving loop): object packing (FluidSim.setObjects, 50-sim.js:165-211) and
+   the emitters (sourceEmits / burstEmit / blockEmits / blockLiters, 55-emit.js:13-113). */
+enum { SL_ITEM_OBSTACLE = 0, SL_ITEM_DRAIN = 1, SL_ITEM_SOURCE = 2, SL_ITEM_BLOCK = 3 };
+/* obstacle shapes (10-scene.js:5-15) */
+enum { SL_SHAPE_BOX = 0, SL_SHAPE_PLATE, SL_SHAPE_SPHERE, SL_SHAPE_CYLINDER, SL_SHAPE_CONE, SL_SHAPE_TORUS,
+       SL_SHAPE_CUP, SL_SHAPE_CONTAINER, SL_SHAPE_GUTTER, SL_SHAPE_COUNT_ };
+/* source modes (10-scene.js:60-68) */
+enum { SL_MODE_STREAM = 0, SL_MODE_DROPS, SL_MODE_SHOWER, SL_MODE_BURST, SL_MODE_RAIN, SL_MODE_COUNT_ };
+typedef struct sl_item {
+  uint32_t id;       /* item id (object velocities and source progress are kept per id) */
+  uint32_t kind;     /* SL_ITEM_* */
+  uint32_t shape;    /* obstacle: SL_SHAPE_*; source: SL_MODE_* */
+  uint32_t enabled;  /* sources: 0 = off */
+  double pos[3];     /* [m], y up */
+  double rot[3];     /* Euler angles [deg], applied x, then y, then z (obstacles, drains) */
+  /* obstacle dims by shape: box (w, h, d), plate (w, d, t), sphere (r), cylinder and cone (r, h),
+     torus (R, r), cup (r, h, t), container (w, h, d, t), gutter (len, r, t); block (w, h, d);
+     drain (r) */
+  double dims[4];
+  /* sources: pitch / yaw [deg], flow [L/min], speed [m/s], drop diameter [mm], drops per second,
+     shower spread [deg], burst volume [L], burst period [s] (0: once), rain rate [mm/h], rain square
+     side [m], total litres (0: unlimited) */
+  double pitch, yaw, flow, speed, drop_d, drop_rate, spread, burst, burst_every, rain_rate, rain_size, total;
+} sl_item;
+
+/* Host-side scene state (source progress, previous object poses, seed stream) for a tank of this
+   configuration; no device is touched. A simulation keeps its own (sl_sim_scene): sl_save_water /
+   sl_restore_water save and restore it with the water, sl_clear_water resets the sources. */
+typedef struct sl_scene sl_scene;
+int sl_scene_create(const sl_config *cfg, sl_scene **out);
+void sl_scene_destroy(sl_scene *sc);
+const char *sl_scene_last_error(const sl_scene *sc);
+sl_scene *sl_sim_scene(sl_sim *sim);
+/* The Obj records of the obstacles and drains among `items` (at most SL_MAX_OBJ, in item order) into
+   rows[SL_MAX_OBJ * 28]: R^T rows, position + shape type, shape dimensions, linear velocity + bound
+   radius, angular velocity + flag. Velocities come from each id's pose at the previous call over
+   dt_move (none when dt_move <= 0 or the id is new), clamped to 4 m/s and 40 rad/s; *moving = 1 when
+   an object moved (50-sim.js:165-201). The poses are replaced by this call's. */
+int sl_scene_pack_objects(sl_scene *sc, const sl_item *items, uint32_t n, double dt_move, float *rows,
+                          uint32_t *n_rows, int *moving);
+/* FluidSim.setObjects(items, dtMove): sl_scene_pack_objects with the simulation's scene, then
+   sl_set_objects */
+int sl_set_items(sl_sim *sim, const sl_item *items, uint32_t n, double dt_move, int *moving);
+/* sourceEmits(sim, dt, t) — 55-emit.js:13-69: the emission records of the enabled sources among
+   `items` for one substep of length dt (stream and shower discs, drops, bursts, rain), at most `max`;
+   each record takes the next seed of the scene's stream */
+int sl_scene_source_emits(sl_scene *sc, const sl_item *items, uint32_t n, double dt, sl_emit *out, uint32_t max,
+                          uint32_t *n_out);
+/* blockEmits(sim, items, fill) — 55-emit.js:83-106: one jittered half-cell lattice per block,
+   clipped to the tank; `fill` pours skip cells that already hold water */
+int sl_scene_block_emits(sl_scene *sc, const sl_item *items, uint32_t n, int fill, sl_emit *out, uint32_t max,
+                         uint32_t *n_out);
+/* blockLiters(b) — 55-emit.js:108-113: the litres of a block that lie inside a tank of w x h x d */
+double sl_block_liters(const sl_item *block, double tank_w, double tank_h, double tank_d);
+/* FluidSim.step(frameDt, sourceEmits): a frame whose substeps emit the sources among `items` */
+int sl_step_items(sl_sim *sim, double frame_dt, const sl_item *items, uint32_t n);
 
 /* FluidSim.emitNow(list, fill) — src/js/50-sim.js:248-265 (writeEmits budget cap-count-64, `full`) */
 int sl_emit_now(sl_sim *sim, const sl_emit *list, uint32_t n, int fill, uint32_t *emitted);
@@ -223,7 +300,7 @@ int sl_pressure_solver_stats(sl_sim *sim, uint32_t *iters, int *breakdown, doubl
 /* Per-stage device timing with HIP events on the sim's stream. */
 enum sl_tstage {
   SL_T_EMIT = 0, SL_T_CLEAR, SL_T_GEOM, SL_T_P2G, SL_T_NORMFORCE, SL_T_DIVERGENCE, SL_T_SOR,
-  SL_T_PROJECT, SL_T_G2P, SL_T_COMPACT, SL_T_WW, SL_T_RIPPLE, SL_T_COUNT_
+  SL_T_PROJECT, SL_T_G2P, SL_T_COMPACT, SL_T_WW, SL_T_RIPPLE, SL_T_DROPLETS, SL_T_COUNT_
 };
 int sl_timing_enable(sl_sim *sim, int enable);
 /* ms[i] = accumulated milliseconds of stage i since the last reset, launches[i] = kernel launches */
@@ -330,7 +407,12 @@ typedef struct {
   uint32_t bake_every;      /* with bake_dir: sl_bake_frame at admission, every bake_every frames and at
                                the last frame (0: only at the last) */
   const char *bake_dir;     /* NULL: no bake output; copied at submission */
-  uint32_t reserved[2];
+  /* scene items (copied; instead of `objects`): at admission the obstacles and drains are packed
+     (sl_set_items) and the blocks fill the tank (sl_scene_block_emits, after `fill`); every frame the
+     sources emit (sl_scene_source_emits, after the records of `source`) */
+  const sl_item *items;
+  uint32_t n_items;
+  uint32_t reserved;
 } sl_job;
 enum { SL_JOB_QUEUED = 0, SL_JOB_RUNNING = 1, SL_JOB_DONE = 2, SL_JOB_CANCELLED = 3, SL_JOB_REFUSED = 4 };
 typedef struct {

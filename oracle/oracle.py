@@ -25,6 +25,10 @@ class OrQuality(ctypes.Structure):
         ("r_film", ctypes.c_float), ("r_hp_k", ctypes.c_float), ("r_rho_kick", ctypes.c_float),
         ("wmap", ctypes.c_uint32), ("reorder_every", ctypes.c_uint32), ("reorder_phase", ctypes.c_uint32),
         ("top_height", ctypes.c_uint32),
+        ("splash_rho", ctypes.c_double), ("splash_sigma", ctypes.c_double), ("splash_mu", ctypes.c_double),
+        ("splash_k_jet", ctypes.c_float), ("splash_k_crown", ctypes.c_float), ("ripple_depth", ctypes.c_double),
+        ("droplets", ctypes.c_uint32), ("drop_seed", ctypes.c_uint32), ("drop_d50", ctypes.c_float),
+        ("drop_sigma_ln", ctypes.c_float), ("drop_vmin", ctypes.c_float), ("drop_vmax", ctypes.c_float),
     ]
 
 
@@ -81,6 +85,8 @@ def lib():
         L.or_rmask.argtypes = [P, P, P]
         L.or_rextract.argtypes = [P, P]
         L.or_crown.argtypes = [P, ctypes.POINTER(ctypes.c_float), ctypes.c_float, ctypes.c_float, ctypes.c_uint32]
+        L.or_droplets.argtypes = [P, ctypes.c_double]
+        L.or_droplet_draws.argtypes = [P, P]
         L.or_hashU.restype = ctypes.c_uint32
         L.or_hashU.argtypes = [ctypes.c_uint32]
         L.or_scene_sdf.restype = ctypes.c_float

@@ -52,6 +52,7 @@ static inline float cr_powf(float x, float y) { return (float)pow((double)x, (do
 static inline float cr_sinf(float x) { return (float)sin((double)x); }
 static inline float cr_cosf(float x) { return (float)cos((double)x); }
 static inline float cr_sinhf(float x) { return (float)sinh((double)x); }
+static inline float cr_logf(float x) { return (float)log((double)x); }
 static inline int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
 static inline v3i v2i(v3 g) { v3i r = {f2i(g.x), f2i(g.y), f2i(g.z)}; return r; }
 static inline float ld4(const float *a, uint32_t n, int c) { return a[(size_t)n * 4 + c]; }
@@ -514,17 +515,21 @@ static void depositRipple(or_sim *s, float x, float z, float impulse, uint32_t R
   uint32_t j = f2u(clampw(rintf(w * (float)R), 0.0f, (float)R));
   __atomic_fetch_add(&s->rsrc[j * (R + 1u) + i], f2i(clampw(impulse, 0.0f, 2e-3f) * 1e9f), __ATOMIC_RELAXED);
 }
-/* splashCrown, 31-wgsl-sim2.js:119-148 */
+/* splashCrown, 31-wgsl-sim2.js:119-148. The reference's liquid constants (998, 0.0728, 0.001) and
+   thresholds (700, 2100) are the quality's splash_* values; each constant expression is folded in
+   double and rounded once, as the WGSL folds 3.14159265 * 998.0 and 998.0 * 0.0728 */
 static void splashCrown(or_sim *s, v3 x, float U, float m, uint32_t id, or_particle *ww) {
   if (s->P.tank[1] <= 0.0f) return;
-  const float PI998 = (float)(3.14159265 * 998.0);
-  const float RS = (float)(998.0 * 0.0728);
-  float D = cr_powf(6.0f * m / PI998, (float)(1.0 / 3.0));
-  float We = 998.0f * U * U * D / 0.0728f;
-  float Oh = 0.001f / sqrtf(RS * D);
+  const float PIRHO = (float)(3.14159265 * s->q.splash_rho);
+  const float RS = (float)(s->q.splash_rho * s->q.splash_sigma);
+  const float RHO = (float)s->q.splash_rho, SIGMA = (float)s->q.splash_sigma, MU = (float)s->q.splash_mu;
+  const float KJ = s->q.splash_k_jet, KC = s->q.splash_k_crown;
+  float D = cr_powf(6.0f * m / PIRHO, (float)(1.0 / 3.0));
+  float We = RHO * U * U * D / SIGMA;
+  float Oh = MU / sqrtf(RS * D);
   float K = We * cr_powf(Oh, -0.4f);
-  uint32_t n = f2u(clampw(rintf(4.0f * (K / 2100.0f - 1.0f)), 0.0f, 24.0f));
-  uint32_t nj = K > 700.0f ? (K > 2100.0f ? 2u : 1u) : 0u;
+  uint32_t n = f2u(clampw(rintf(4.0f * (K / KC - 1.0f)), 0.0f, 24.0f));
+  uint32_t nj = K > KJ ? (K > KC ? 2u : 1u) : 0u;
   if (n + nj == 0u) return;
   uint32_t st = or_hashU(id * 2654435761u + s->P.ext[3] * 97u + 3u);
   uint32_t base = __atomic_fetch_add(&s->C.wwCount, n + nj, __ATOMIC_RELAXED);
@@ -557,6 +562,38 @@ static void splashCrown(or_sim *s, v3 x, float U, float m, uint32_t id, or_parti
     q.vel[0] = a; q.vel[1] = sp; q.vel[2] = c; q.vel[3] = 0.0f;
     ww[k] = q;
   }
+}
+/* Sub-grid droplets of a frame (no counterpart in the reference; the droplet layer of the build).
+   Droplet d of frame f: hash stream seeded from (d, f, drop_seed); z = sqrt(-2 ln(1 - u1)) cos(2 pi u2)
+   (Box-Muller), D = d50 exp(sigma_ln z), v = vmin + (vmax - vmin) u5, impact point
+   (lx + (hx - lx) u3, lz + (hz - lz) u4) on the footprint; mass m = f32(pi/6 rho) D^3; it deposits
+   2 m v through depositRipple and splashes through splashCrown at the water level (origin y + the
+   frame's ripple depth). */
+static void dropletDraw(const or_sim *s, uint32_t d, float *x, float *z, float *D, float *v) {
+  uint32_t st = or_hashU(d * 0x9e3779b1u + s->P.ext[3] * 0x85ebca77u + s->q.drop_seed * 0xc2b2ae3du + 7u);
+  float u1 = rnd(&st), u2 = rnd(&st), u3 = rnd(&st), u4 = rnd(&st), u5 = rnd(&st);
+  float g = sqrtf(-2.0f * cr_logf(1.0f - u1)) * cr_cosf(6.2831853f * u2);
+  *D = s->q.drop_d50 * cr_expf(s->q.drop_sigma_ln * g);
+  *v = s->q.drop_vmin + (s->q.drop_vmax - s->q.drop_vmin) * u5;
+  float lx, lz, hx, hz;
+  fpMin(s, &lx, &lz); fpMax(s, &hx, &hz);
+  *x = lx + (hx - lx) * u3;
+  *z = lz + (hz - lz) * u4;
+}
+void or_droplets(or_sim *s, double depth) {
+  const float mc = (float)(3.141592653589793 / 6.0 * s->q.splash_rho);
+  for (uint32_t d = 0; d < s->q.droplets; d++) {
+    float x, z, D, v;
+    dropletDraw(s, d, &x, &z, &D, &v);
+    float m = mc * D * D * D;
+    if (s->R > 0u) depositRipple(s, x, z, 2.0f * m * v, s->R);
+    splashCrown(s, V(x, s->P.origin[1] + (float)depth, z), v, m, 0x80000000u + d, s->ww[s->ww_cur]);
+  }
+}
+/* test hook: the draws (x, z, D, v) of the current frame's droplets */
+void or_droplet_draws(const or_sim *s, float *out) {
+  for (uint32_t d = 0; d < s->q.droplets; d++)
+    dropletDraw(s, d, out + 4 * d, out + 4 * d + 1, out + 4 * d + 2, out + 4 * d + 3);
 }
 /* test hook: one splashCrown at x into the current whitewater buffer (appends at wwCount) */
 void or_crown(or_sim *s, const float *x, float U, float m, uint32_t id) {
@@ -1106,6 +1143,12 @@ void or_quality_default(or_quality *q, const char *name) {
   q->r_sigma_rho = (float)(0.0728 / 998.0); q->r_nu = 1.0e-6f; q->r_film = 0.3f;
   q->r_hp_k = (float)(2.0 * 3.141592653589793 / 0.12); q->r_rho_kick = 998.0f;
   q->wmap = 256; /* 65-render.js:161 */
+  /* splashCrown, 31-wgsl-sim2.js:121-126 */
+  q->splash_rho = 998.0; q->splash_sigma = 0.0728; q->splash_mu = 0.001;
+  q->splash_k_jet = 700.0f; q->splash_k_crown = 2100.0f;
+  q->ripple_depth = 0.0;
+  q->droplets = 0; q->drop_seed = 1;
+  q->drop_d50 = 0.0015f; q->drop_sigma_ln = 0.4f; q->drop_vmin = 1.0f; q->drop_vmax = 4.0f;
 }
 
 static void *xcalloc(size_t n, size_t sz) { void *p = calloc(n ? n : 1, sz); if (!p) abort(); return p; }
@@ -1297,12 +1340,14 @@ void or_frame_tail(or_sim *s, double frame_dt) {
     or_reorder(s);
     s->reorder_clock = 0;
   }
-  or_ww(s, (float)frame_dt);
-  s->ww_cur = 1 - s->ww_cur;
-  or_count(s, 3, 0);
   double liters = s->count * s->vp * 1000;
   double md = liters / 1000 / (s->nx * s->nz * s->h * s->h);
   md = md < 0.003 ? 0.003 : (md > s->rim ? s->rim : md);
+  if (s->q.ripple_depth > 0) md = s->q.ripple_depth;  /* a fixed depth instead of the count's */
+  if (s->R) or_droplets(s, md);  /* before WW: the same frame's whitewater pass moves their splashes */
+  or_ww(s, (float)frame_dt);
+  s->ww_cur = 1 - s->ww_cur;
+  or_count(s, 3, 0);
   or_ripple_frame(s, frame_dt, md);
   if (s->R && s->q.top_height) or_toph(s);  /* drawn after the step; the next frame's mask reads it */
 }
@@ -1385,4 +1430,6 @@ void or_set_scalar(or_sim *s, const char *name, double v) {
   else if (!strcmp(name, "cur")) s->cur = (int)v;
   else if (!strcmp(name, "ww_cur")) s->ww_cur = (int)v;
   else if (!strcmp(name, "full")) s->full = (int)v;
+  else if (!strcmp(name, "ww_count")) s->ww_count = (uint32_t)v;
+  else if (!strcmp(name, "impacts")) s->impacts = (uint32_t)v;
 }

@@ -67,6 +67,13 @@ typedef struct {
   uint32_t reorder_every; /* spatial re-ordering period in frames (0 = never) */
   uint32_t reorder_phase; /* frames already counted towards the first re-ordering */
   uint32_t top_height;    /* 1: produce the top-height map from the particles at each frame's end */
+  /* splashCrown liquid constants and K thresholds (31-wgsl-sim2.js:121-126: 998, 0.0728, 0.001, 700, 2100) */
+  double splash_rho, splash_sigma, splash_mu;
+  float splash_k_jet, splash_k_crown;
+  double ripple_depth;    /* > 0: fixed ripple depth; 0: the mean depth of the count (50-sim.js:332) */
+  /* sub-grid droplets per frame (0 = none): seed, lognormal median diameter and sigma of ln D, speeds */
+  uint32_t droplets, drop_seed;
+  float drop_d50, drop_sigma_ln, drop_vmin, drop_vmax;
 } or_quality;
 
 typedef struct or_sim {
@@ -134,6 +141,10 @@ void or_toph_blur(or_sim *s);
 void or_count(or_sim *s, uint32_t mode, uint32_t y); /* 31-wgsl-sim2.js:332-349 */
 void or_ww(or_sim *s, float dt);           /* 31-wgsl-sim2.js:361-409 */
 void or_crown(or_sim *s, const float *x, float U, float m, uint32_t id); /* splashCrown, 31-wgsl-sim2.js:119-148 */
+/* the frame's sub-grid droplets (q.droplets): impulses into rsrc, splashes into the whitewater; the
+   water level is origin y + depth */
+void or_droplets(or_sim *s, double depth);
+void or_droplet_draws(const or_sim *s, float *out);  /* per droplet (x, z, D, v) of the current frame */
 
 /* ripple kernels, src/js/64-ripple.js */
 void or_rsrc(or_sim *s, float *out);                                   /* :18-31 */

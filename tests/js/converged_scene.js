@@ -9,7 +9,7 @@ const frames = parseInt(process.argv[2], 10);
 const sc = S.PRESETS.find((p) => p.id === 'dam').make();
 const sim = new S.FluidSim(sc.tank, 'low', { seed: 1234, ripple_R: 64, cap: 200000, ww_cap: 20000 });
 sim.setObjects(sc.items, 0);
-sim.emitNow(S.blockEmits(sim, sc.items).map((e) => ({ ...e, seed: sim.seed() })));
+sim.emitNow(sim.blockEmits(sc.items));
 let unknown = '';
 try { sim.setPressureSolver('multigrid'); } catch (e) { unknown = e.message; }
 sim.setPressureSolver('converged', 1e-5, 40);

@@ -16,14 +16,14 @@ sim.sceneItems = sc.items;
 const trace = { preset, frames, opts, tank: sc.tank, emits: [], rows: null, substeps: [] };
 sim.setObjects(sc.items, 0);
 trace.rows = Array.from(sim.readback('objs').subarray(0, sim.objCount * 28));
-const blocks = S.blockEmits(sim, sc.items).map((e) => ({ ...e, seed: sim.seed() }));
+const blocks = sim.blockEmits(sc.items);
 trace.emits = blocks;
 sim.emitNow(blocks);
 for (let f = 0; f < frames; f++) {
   sim.setObjects(sc.items, 1 / 120);
   const subs = [];
-  sim.step(1 / 120, (s, dt, t) => {
-    const list = S.sourceEmits(s, dt, t, sc.items).map((e) => ({ ...e, seed: s.seed() }));
+  sim.step(1 / 120, (s, dt) => {
+    const list = s.sourceEmits(sc.items, dt);
     subs.push(list);
     return list;
   });

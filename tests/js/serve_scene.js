@@ -16,8 +16,8 @@ const sim = new S.FluidSim(sc.tank, 'low', { seed: 77, ...sc.solver });
 sim.sceneItems = sc.items;
 sim.setSubsteps(1);
 sim.setObjects(sc.items, 0);
-sim.emitNow(S.blockEmits(sim, sc.items));
-for (let f = 0; f < frames; f++) sim.step(dt, (s, d, t) => S.sourceEmits(s, d, t, sc.items));
+sim.emitNow(sim.blockEmits(sc.items));
+for (let f = 0; f < frames; f++) sim.step(dt, (s, d) => s.sourceEmits(sc.items, d));
 const plainCount = sim.stats().count;
 const plain = sim.readback('particles').subarray(0, plainCount * 8);
 sim.destroy();

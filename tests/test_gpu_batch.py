@@ -48,7 +48,7 @@ class Driver:
 
     def __init__(self, sim, sp, seed):
         self.sim, self.sp, self.seed = sim, sp, seed
-        self.packer = scene.ObjectPacker()
+        self.packer = scene.Scene()
         rows, _ = self.packer.pack(items_of(sp), 0)
         sim.set_objects_rows(rows)
         if not sp["mask"]:

@@ -19,7 +19,7 @@ def build(wl):
     workloads.setup(g, wl)
     g.reorder()
     g.set_substeps(1)
-    return g, scene.Emitters(wl["items"], g.vp, seed=99)
+    return g, scene.Emitters(wl["items"], g.scene())
 
 
 def closed_tank():

@@ -26,7 +26,7 @@ def fill_block(cfg, i1, j1, k1, seed):
 
 def obstacle_rows(x):
     items = [dict(scene.make_obstacle("sphere", pos=[x, 0.05, 0.0], dims={"r": 0.03}), id=1)]
-    rows, _ = scene.ObjectPacker().pack(items, 0)
+    rows, _ = scene.Scene().pack(items, 0)
     return rows
 
 

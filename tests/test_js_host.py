@@ -21,7 +21,7 @@ def test_js_host_cpu_logic():
 
 @pytest.mark.skipif(NODE is None, reason="node not installed")
 def test_js_addon_loads():
-    r = subprocess.run([NODE, "-e", "const a=require(process.argv[1]); if (a.version()!==1) process.exit(3); "
+    r = subprocess.run([NODE, "-e", "const a=require(process.argv[1]); if (a.version()!==2) process.exit(3); "
                         "const c=a.configDefault('high'); if (c.iters!==56 || c.cap!==700000) process.exit(4);",
                         os.path.join(ROOT, "hahenn-_amd", "suiri_napi.node")], capture_output=True, text=True,
                        timeout=120)

@@ -11,7 +11,8 @@ scene = importlib.import_module("hahenn-_amd.scene")
 
 
 def small_scene(tank=(0.5, 0.25, 0.25), h=1 / 64, iters=30, R=64, cap=60000, ww_cap=20000, flip=0.95,
-                omega=1.7, whitewater=True, obstacle=True, fill_cells=(12, 10, 16), reorder_every=0, top_height=0):
+                omega=1.7, whitewater=True, obstacle=True, fill_cells=(12, 10, 16), reorder_every=0, top_height=0,
+                extra=None):
     """A dam break with a box obstacle (the shape of the 64x32x32 dam-break workload, smaller)."""
     items = []
     if obstacle:
@@ -19,7 +20,7 @@ def small_scene(tank=(0.5, 0.25, 0.25), h=1 / 64, iters=30, R=64, cap=60000, ww_
         items.append(scene.make_obstacle("sphere", pos=[0.15, 0.04, 0.05], dims={"r": 0.03}))
     return dict(tank=tank, h=h, iters=iters, R=R, cap=cap, ww_cap=ww_cap, flip=flip, omega=omega,
                 whitewater=whitewater, items=items, fill_cells=fill_cells, reorder_every=reorder_every,
-                top_height=top_height)
+                top_height=top_height, extra=dict(extra or {}))
 
 
 def make_twin(sc, threads=4):
@@ -28,14 +29,16 @@ def make_twin(sc, threads=4):
                   top_height=sc.get("top_height", 0))
     if not sc["whitewater"]:
         q.ww_rate = 0.0
+    for k, v in sc.get("extra", {}).items():   # the same settings on both sides (same field names)
+        setattr(q, k, v)
     o = O.Oracle(sc["tank"], q, threads=threads)
     cfg = suiri.config("standard", tank_w=sc["tank"][0], tank_h=sc["tank"][1], tank_d=sc["tank"][2], h=sc["h"],
                        iters=sc["iters"], ripple_R=sc["R"], cap=sc["cap"], ww_cap=sc["ww_cap"], flip=sc["flip"],
                        omega=sc["omega"], whitewater=1 if sc["whitewater"] else 0,
-                       reorder_every=sc["reorder_every"], top_height=sc.get("top_height", 0))
+                       reorder_every=sc["reorder_every"], top_height=sc.get("top_height", 0),
+                       **sc.get("extra", {}))
     g = suiri.Sim(cfg)
-    packer = scene.ObjectPacker()
-    rows, _ = packer.pack(sc["items"], 0)
+    rows, _ = scene.Scene(cfg).pack(sc["items"], 0)
     o.set_objects(rows)
     g.set_objects_rows(rows)
     inf = o.info()
