@@ -53,12 +53,77 @@ def step(g, wl):
     g.step_items(wl["frame_dt"], wl["items"])
 
 
-def sor_bytes_per_launch(nx, ny, nz):
-    """Algorithmic bytes of one colour pass: 4C (p read) + 2C (p write, half) + 12N (face
-    fractions) + 2C (rhs, half) + 1C (flags) = 12N + 9C."""
+def sor_bytes_survey_model(nx, ny, nz):
+    """The per-pass byte model of a reference-style pass that re-reads every face fraction:
+    4C (p read) + 2C (p write, half) + 12N (fractions) + 2C (rhs, half) + 1C (flags) = 12N + 9C.
+    k_sor does not move these bytes (it reads stencil flags and re-reads fractions only in cut
+    cells); reported beside the kernel's own count for comparison."""
     N = (nx + 1) * (ny + 1) * (nz + 1)
     C = nx * ny * nz
     return 12 * N + 9 * C
+
+
+def sor_pass_bytes(g):
+    """Bytes one k_sor colour pass moves, counted as the kernel reads and writes them, from the tank's
+    state (cell counts and face fractions read back after the timed region): every slot of the colour
+    loads its u16 stencil flags (2 B); each liquid cell of the colour reads its pressure and
+    right-hand side and writes its pressure (12 B); the six neighbour loads of those cells touch
+    other-colour pressures (4 B each, counted once); cut cells (an open face with 0 < f < 1) re-read
+    their face fractions (4 B per distinct face). Returns (mean bytes of the two colours, parts)."""
+    nx, ny, nz = g.nx, g.ny, g.nz
+    slots = ((nx + 1) // 2) * ny * nz
+    liquid = g.readback(suiri.F_CELLCOUNT).reshape(nz, ny, nx) > 0
+    fr = g.readback(suiri.F_FRAC).reshape(nz + 1, ny + 1, nx + 1, 4)
+    k, j, i = np.indices((nz, ny, nx), sparse=True)
+    colour = (i + j + k) & 1
+    faces = [fr[:nz, :ny, :nx, 0], fr[:nz, :ny, 1:, 0], fr[:nz, :ny, :nx, 1], fr[:nz, 1:, :nx, 1],
+             fr[:nz, :ny, :nx, 2], fr[1:, :ny, :nx, 2]]
+    cut = np.zeros_like(liquid)
+    for f in faces:
+        cut |= (f > 0) & (f != 1)
+    cut &= liquid
+    out = []
+    for c in (0, 1):
+        lc = liquid & (colour == c)
+        near = np.zeros_like(lc)
+        near[1:] |= lc[:-1]; near[:-1] |= lc[1:]
+        near[:, 1:] |= lc[:, :-1]; near[:, :-1] |= lc[:, 1:]
+        near[:, :, 1:] |= lc[:, :, :-1]; near[:, :, :-1] |= lc[:, :, 1:]
+        cc = cut & (colour == c)
+        xf = np.zeros((nz, ny, nx + 1), bool); xf[:, :, :-1] |= cc; xf[:, :, 1:] |= cc
+        yf = np.zeros((nz, ny + 1, nx), bool); yf[:, :-1] |= cc; yf[:, 1:] |= cc
+        zf = np.zeros((nz + 1, ny, nx), bool); zf[:-1] |= cc; zf[1:] |= cc
+        parts = {"flags": 2 * slots, "liquid": 12 * int(lc.sum()), "neighbours": 4 * int(near.sum()),
+                 "cut_fractions": 4 * int(xf.sum() + yf.sum() + zf.sum())}
+        out.append(parts)
+    mean = {k2: (out[0][k2] + out[1][k2]) / 2 for k2 in out[0]}
+    return sum(mean.values()), {"per_pass_mean": mean, "per_colour": out, "liquid_cells": int(liquid.sum()),
+                                "cut_cells": int(cut.sum())}
+
+
+def calib_sor(passes):
+    """Counter calibration on k_sor's own access pattern: the headline tank after a few steps, then
+    `passes` times a 1 GiB device copy (which evicts the L2 and the Infinity Cache) followed by one
+    colour-0 pass, so that every byte the pass reads comes from HBM. Prints the pass's byte count from
+    sor_pass_bytes for rocprofv3's FETCH_SIZE / WRITE_SIZE of the same dispatches to be set against."""
+    import torch
+    wl = workloads.faucet()
+    g = build_sim(wl)
+    for _ in range(3):
+        step(g, wl)
+    g.sync()
+    _, det = sor_pass_bytes(g)
+    src = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    for _ in range(passes):
+        dst.copy_(src)
+        torch.cuda.synchronize()
+        g.run_stage(suiri.S_PRESSURE, 0, wl["frame_dt"])
+        g.sync()
+    c0 = det["per_colour"][0]
+    print(json.dumps({"passes": passes, "colour": 0, "bytes": sum(c0.values()), "parts": c0,
+                      "read_bytes": c0["flags"] + c0["liquid"] * 2 // 3 + c0["neighbours"] + c0["cut_fractions"],
+                      "write_bytes": c0["liquid"] // 3}))
 
 
 def cpu_baseline(wl, steps, threads):
@@ -186,6 +251,51 @@ def converged_leg(wl, steps, warmup, tol, max_iters):
             "ms_per_step": el / steps * 1e3, "steps_per_s": steps / el,
             "particle_steps_per_s": mean_p * steps / el, "iters_sampled": its,
             "last_solve": st, "divergence_after_projection": res}
+
+
+def droplet_leg(steps, warmup):
+    """The droplet layer (workloads.ripple_droplets): a 128^3 tank 0.1 m deep with 4096 sub-grid
+    droplets per step on a 1024 x 1024 ripple layer of fixed depth 0.1 m (sigma 0.072, rho 1000),
+    whitewater at K > 60, SOR omega 1.8 x 60, dt 1/240. Reports the coupled step, the droplet and
+    ripple stages per frame, and the ripple frame's fraction of the HBM peak at its algorithmic
+    176 M^2 + 16 (R+1)^2 bytes."""
+    wl = workloads.ripple_droplets()
+    g = build_sim(wl)
+    for _ in range(warmup):
+        step(g, wl)
+    tuned = g.autotune(reps=2)
+    g.timing(True)
+    g.timing_read(reset=True)
+    for _ in range(steps):
+        step(g, wl)
+    g.sync()
+    tm = g.timing_read(reset=True)
+    g.timing(False)
+    c0 = g.stats()["count"]
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        step(g, wl)
+    g.sync()
+    el = time.perf_counter() - t0
+    st = g.stats()
+    g.close()
+    R = wl["cfg"]["ripple_R"]
+    M = 2 * R
+    rb = 176 * M * M + 16 * (R + 1) ** 2
+    rip_ms = tm["ripple"][0] / steps
+    drop_ms = tm["droplets"][0] / steps
+    return {"workload": "128^3 tank 0.5 m, 0.1 m deep, sphere; ripple M = 1024 (fixed depth 0.1 m, sigma 0.072, "
+                        "rho 1000); 4096 droplets per step, lognormal D (median 1.5 mm, sigma_ln 0.4), v 1-4 m/s, "
+                        "whitewater at K > 60; SOR 1.8 x 60; dt 1/240",
+            "steps": steps, "ms_per_step": el / steps * 1e3, "steps_per_s": steps / el,
+            "particles": int((c0 + st["count"]) / 2), "whitewater": st["ww"],
+            "particle_steps_per_s": (c0 + st["count"]) / 2 * steps / el,
+            "droplets_per_s": wl["cfg"]["droplets"] * steps / el,
+            "stage_ms_per_step": {k: v[0] / steps for k, v in tm.items()},
+            "droplet_ms_per_frame": drop_ms, "ripple_ms_per_frame": rip_ms,
+            "ripple_roofline": {"bytes": rb, "GBps": rb / (rip_ms / 1e3) / 1e9,
+                                "frac": rb / (rip_ms / 1e3) / 1e9 / HBM_PEAK_GBS},
+            "tuned_variants": tuned, "gpu_errors": st["gpuErrors"]}
 
 
 def batched_leg(n_tanks, steps, warmup, seq_steps, graph_steps=50, seed=7):
@@ -320,6 +430,9 @@ def serve_leg(rate=20.0, window=3.0, limit=45.0, steps=(100, 2000), kinds=(0, 1,
     srv.close()
     done = [x for x in infos if x["status"] == "done"]
     lat = np.array([x["t_done"] - x["t_submit"] for x in done]) if done else np.zeros(1)
+    # every request: an unfinished one counts with the time it had waited at the cut-off, a lower bound
+    # of its latency (right-censored)
+    lat_all = np.array([(x["t_done"] if x["status"] == "done" else st["uptime"]) - x["t_submit"] for x in infos])
     # sequential plain solves of the same requests: per-kind time per frame measured alone on the plain
     # path (faucet and dam break at their sizes, batched-mix tanks on a sample), applied to each request
     def plain_rate(wl, q, k=8):
@@ -360,8 +473,11 @@ def serve_leg(rate=20.0, window=3.0, limit=45.0, steps=(100, 2000), kinds=(0, 1,
         "frames": st["frames"],
         "frame_latency_ms": {"p50": st["step_ms_p50"], "p99": st["step_ms_p99"]},
         "device_busy": st["step_ms_sum"] / 1e3 / elapsed,
-        "request_latency_s": {"p50": float(np.percentile(lat, 50)), "p99": float(np.percentile(lat, 99)),
-                              "of": "completed requests"},
+        "request_latency_s": {"p50": float(np.percentile(lat_all, 50)), "p99": float(np.percentile(lat_all, 99)),
+                              "of": "all requests; unfinished ones censored at the cut-off (lower bounds)",
+                              "censored": int(sum(x["status"] != "done" for x in infos)),
+                              "completed_p50": float(np.percentile(lat, 50)),
+                              "completed_p99": float(np.percentile(lat, 99))},
         "sequential_estimate": {"value": seq_ps / seq_t, "unit": "particle-steps/s",
                                 "method": "each request's frames at the per-frame time of its scene run alone "
                                           "on the plain path"},
@@ -380,11 +496,12 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ripple", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter runs for `traffic`")
-    ap.add_argument("--only", choices=["all", "headline", "batched", "serve", "converged"], default="all",
+    ap.add_argument("--only", choices=["all", "headline", "batched", "serve", "converged", "droplets"], default="all",
                     help="run one leg (its JSON object alone) instead of the full line")
     ap.add_argument("--conv-tol", type=float, default=1e-4, help="converged leg: residual tolerance (1/s)")
     ap.add_argument("--conv-iters", type=int, default=24, help="converged leg: CG iterations enqueued per solve")
     ap.add_argument("--conv-steps", type=int, default=60)
+    ap.add_argument("--drop-steps", type=int, default=100, help="droplet leg: timed steps")
     ap.add_argument("--batch-tanks", type=int, default=64)
     ap.add_argument("--batch-steps", type=int, default=500)
     ap.add_argument("--seq-steps", type=int, default=10)
@@ -394,6 +511,8 @@ def main():
     ap.add_argument("--serve-limit", type=float, default=45.0)
     ap.add_argument("--serve-kinds", default="0,1,3")
     ap.add_argument("--serve-fpt", type=int, default=4, help="serving: frames per tick")
+    ap.add_argument("--calib-sor", type=int, default=0,
+                    help="only run this many HBM-cold k_sor passes of the headline tank (counter calibration) and exit")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -407,6 +526,9 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"calib_copy_bytes_read": n * 4, "calib_copy_bytes_written": n * 4, "launches": 3}))
         return
+    if args.calib_sor:
+        calib_sor(args.calib_sor)
+        return
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU; multi-GPU sharding is design-only (see DESIGN.md)",
                           "n_gpus": args.gpus}))
@@ -418,6 +540,9 @@ def main():
         return
     if args.only == "batched":
         print(json.dumps(batched_leg(args.batch_tanks, args.batch_steps, 5, args.seq_steps, args.graph_steps)))
+        return
+    if args.only == "droplets":
+        print(json.dumps(droplet_leg(args.drop_steps, args.warmup)))
         return
     grid = tuple(int(x) for x in args.grid.split("x"))
     wl = workloads.faucet(grid=grid, R=0 if args.no_ripple else 512)
@@ -461,8 +586,23 @@ def main():
     iters = wl["cfg"]["iters"]
     sor_ms, sor_launches = tm["sor"]
     sor_launch_s = sor_ms / 1e3 / max(sor_launches, 1)
-    bpl = sor_bytes_per_launch(nx, ny, nz)
+    bpl, bpl_detail = sor_pass_bytes(g)
     achieved = bpl / sor_launch_s / 1e9
+    # per-stage fractions of the HBM peak with the stage byte counts of the step's roofline table
+    # (P2G 28P + 24N + 4C, its binning another 21P; G2P 64P + 24N + 4C; a ripple frame
+    # 176 M^2 + 16 (R+1)^2)
+    N = (nx + 1) * (ny + 1) * (nz + 1)
+    C = nx * ny * nz
+    M, R = 2 * wl["cfg"]["ripple_R"], wl["cfg"]["ripple_R"]
+    stage_bytes = {"p2g": 28 * mean_p + 24 * N + 4 * C, "g2p": 64 * mean_p + 24 * N + 4 * C,
+                   "ripple": 176 * M * M + 16 * (R + 1) ** 2}
+    stage_frac = {}
+    for k, b in stage_bytes.items():
+        ms = tm[k][0] / args.steps
+        stage_frac[k] = {"bytes": b, "ms_per_step": ms, "GBps": b / (ms / 1e3) / 1e9 if ms else None,
+                         "frac": b / (ms / 1e3) / 1e9 / HBM_PEAK_GBS if ms else None}
+    stage_frac["p2g"]["bytes_with_binning"] = stage_bytes["p2g"] + 21 * mean_p
+    stage_frac["sor"] = {"bytes_per_pass": bpl, "us_per_pass": sor_launch_s * 1e6, "frac": achieved / HBM_PEAK_GBS}
     out = {
         "metric": "particle-steps/s at 256x128x128; SOR HBM GB/s; batched p99 step latency",
         "value": value,
@@ -482,8 +622,11 @@ def main():
                    "whitewater": True},
         "roofline": {"bound": "hbm", "kernel": "k_sor (one colour pass)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None, "bytes_per_launch": bpl, "avg_launch_us": sor_launch_s * 1e6},
+                     "traffic": None, "bytes_per_launch": bpl, "bytes_detail": bpl_detail,
+                     "avg_launch_us": sor_launch_s * 1e6,
+                     "survey_model_bytes_per_launch": sor_bytes_survey_model(nx, ny, nz)},
         "sor_GBps": iters * 2 * bpl / (sor_ms / 1e3 / args.steps) / 1e9,
+        "stage_roofline": stage_frac,
         "stage_ms_per_step": {k: v[0] / args.steps for k, v in tm.items()},
         "step_algorithmic_GB": workloads.step_bytes(nx, ny, nz, mean_p, iters) / 1e9,
         "divergence_after_projection": resid,
@@ -495,6 +638,8 @@ def main():
     if args.only == "all":
         out["converged"] = converged_leg(wl, args.conv_steps, args.warmup, args.conv_tol, args.conv_iters)
         progress("converged solve done")
+        out["ripple_droplets"] = droplet_leg(args.drop_steps, args.warmup)
+        progress("droplet layer done")
         out["batched"] = batched_leg(args.batch_tanks, args.batch_steps, 5, args.seq_steps, args.graph_steps)
         out["batched_p99_step_ms"] = out["batched"]["step_latency_ms"]["p99"]
         progress("batched tanks done")
@@ -512,19 +657,22 @@ def main():
         if t:
             out["roofline"]["traffic"] = t["bytes"]
             out["roofline"]["traffic_detail"] = t
-            # the HBM bandwidth the pass actually draws (counter bytes over the same launch time); the
-            # algorithmic `achieved` exceeds the HBM peak because the red/black arrays stay in the
-            # 256 MB Infinity Cache between passes
+            # the bandwidth the pass draws past L2 (counter bytes over the same launch time)
             rl = out["roofline"]
             rl["traffic_GBps"] = t["bytes"] / (rl["avg_launch_us"] * 1e-6) / 1e9
             rl["traffic_frac"] = rl["traffic_GBps"] / HBM_PEAK_GBS
         progress("counter runs done")
     if not args.no_cpu:
-        threads = int(os.environ.get("OMP_NUM_THREADS") or min(os.cpu_count() or 1, 16))
+        # every core the process may use: the host's share for one GPU sets OMP_NUM_THREADS, else nproc
+        nproc = os.cpu_count() or 1
+        threads = int(os.environ.get("OMP_NUM_THREADS") or nproc)
         v, secs = cpu_baseline(wl, args.cpu_steps, threads)
-        out["cpu_baseline"] = {"value": v, "unit": "particle-steps/s", "cores": threads, "kind": "port",
+        out["cpu_baseline"] = {"value": v, "unit": "particle-steps/s", "cores": threads, "nproc": nproc,
+                               "kind": "port",
                                "sample": f"{args.cpu_steps} steps of the same workload after 1 untimed step "
-                                         f"({secs:.1f} s), oracle/suiri_oracle.c with OpenMP"}
+                                         f"({secs:.1f} s), oracle/suiri_oracle.c with OpenMP on {threads} threads "
+                                         f"(OMP_NUM_THREADS: the cores granted to this GPU's host share; nproc "
+                                         f"{nproc})"}
     print(json.dumps(out))
 
 

@@ -894,7 +894,7 @@ static int set_ripple_uniform(sl_sim *s, double frame_dt, double mean_depth) {
 static int encode_droplets(sl_sim *s) {
   if (!s->ht.drops) return SL_OK;
   StageScope sc(s, SL_T_DROPLETS, 1);
-  sl_k_droplets(launch_of(s, s->ht.drops));
+  sl_k_droplets(launch_of(s, s->ht.drops, 64));
   return check_launch(s, "droplets");
 }
 static int encode_ripple(sl_sim *s) {

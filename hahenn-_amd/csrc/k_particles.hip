@@ -156,20 +156,26 @@ __device__ __forceinline__ void ww_store(const SlTank &T, uint32_t k, f3 p, floa
   T.wv[wc][k] = make_float4(v.x, v.y, v.z, kind);
 }
 // splash crown, 31-wgsl-sim2.js:119-148, with the liquid's density, surface tension, viscosity and
-// the two K thresholds taken from the tank (the reference's constants by default)
-__device__ __forceinline__ void splash_crown(const SlTank &T, f3 x, float U, float m, uint32_t id) {
-  if (T.ww_rate <= 0.0f) return;
-  float D = sl_pow(6.0f * m / T.sp_pirho, (float)(1.0 / 3.0));
-  float We = T.sp_rho * U * U * D / T.sp_sigma;
-  float Oh = T.sp_mu / sqrtf(T.sp_rs * D);
+// the two K thresholds taken from the tank (the reference's constants by default). Split in two: the
+// drop's diameter and the ring and jet counts, then the records written from slot `base` on.
+struct SplashCount { float D; uint32_t n, nj; };
+__device__ __forceinline__ SplashCount splash_count(const SlTank &T, float U, float m) {
+  SplashCount c;
+  c.D = sl_pow(6.0f * m / T.sp_pirho, (float)(1.0 / 3.0));
+  float We = T.sp_rho * U * U * c.D / T.sp_sigma;
+  float Oh = T.sp_mu / sqrtf(T.sp_rs * c.D);
   float K = We * sl_pow(Oh, -0.4f);
-  uint32_t n = f2u(clampf(rintf(4.0f * (K / T.sp_kcrown - 1.0f)), 0.0f, 24.0f));
-  uint32_t nj = K > T.sp_kjet ? (K > T.sp_kcrown ? 2u : 1u) : 0u;
-  if (n + nj == 0u) return;
+  c.n = f2u(clampf(rintf(4.0f * (K / T.sp_kcrown - 1.0f)), 0.0f, 24.0f));
+  c.nj = K > T.sp_kjet ? (K > T.sp_kcrown ? 2u : 1u) : 0u;
+  if (T.ww_rate <= 0.0f) c.n = c.nj = 0u;
+  return c;
+}
+__device__ __forceinline__ void splash_write(const SlTank &T, f3 x, float U, const SplashCount &c, uint32_t id,
+                                             uint32_t base) {
+  const uint32_t n = c.n, nj = c.nj;
   uint32_t s = hashU(id * 2654435761u + T.frame * 97u + 3u);
-  uint32_t base = atomicAdd(&T.C->wwCount, n + nj);
   uint32_t cap = f2u(T.ww_cap_f);
-  float r0 = 0.6f * D;
+  float r0 = 0.6f * c.D;
   float ph0 = rndu(s) * 6.2831853f;
   for (uint32_t i = 0; i < n; i++) {
     uint32_t k = base + i;
@@ -189,9 +195,15 @@ __device__ __forceinline__ void splash_crown(const SlTank &T, f3 x, float U, flo
     float sp = U * (0.22f + 0.04f * (float)j) * (0.9f + 0.2f * rndu(s));
     f3 p = F3(x.x + 0.0f, x.y + 0.001f * (float)j, x.z + 0.0f);
     float a = (rndu(s) - 0.5f) * 0.05f;
-    float c = (rndu(s) - 0.5f) * 0.05f;
-    ww_store(T, k, p, 0.7f, F3(a, sp, c), 0.0f);
+    float c2 = (rndu(s) - 0.5f) * 0.05f;
+    ww_store(T, k, p, 0.7f, F3(a, sp, c2), 0.0f);
   }
+}
+// one splash with its own append (a rare event per particle in G2P)
+__device__ __forceinline__ void splash_crown(const SlTank &T, f3 x, float U, float m, uint32_t id) {
+  const SplashCount c = splash_count(T, U, m);
+  if (c.n + c.nj == 0u) return;
+  splash_write(T, x, U, c, id, atomicAdd(&T.C->wwCount, c.n + c.nj));
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -214,20 +226,40 @@ __device__ __forceinline__ void droplet_draw(const SlTank &T, uint32_t d, float 
   x = T.ox + (hx - T.ox) * u3;
   z = T.oz + (hz - T.oz) * u4;
 }
-__global__ __launch_bounds__(SL_WG) void k_droplets(SlWork W) {
+// One wavefront per workgroup (64 droplets) so that a frame's 4096 droplets spread over 64 CUs; the
+// wavefront reserves its splashes' whitewater slots with one atomic (lane prefix sums of the counts)
+// instead of one atomic per droplet on the same counter.
+__global__ __launch_bounds__(64) void k_droplets(SlWork W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
-  const uint32_t d = lb * SL_WG + threadIdx.x;
-  if (d >= T.drops) return;
-  float x, z, D, v;
-  droplet_draw(T, d, x, z, D, v);
-  const float m = T.drop_mc * D * D * D;
-  if (T.R > 0u) deposit_ripple(T, x, z, 2.0f * m * v);
-  splash_crown(T, F3(x, T.oy + T.rmd, z), v, m, 0x80000000u + d);
+  const uint32_t d = lb * 64u + threadIdx.x;
+  const bool on = d < T.drops;
+  float x = 0.0f, z = 0.0f, D = 0.0f, v = 0.0f, m = 0.0f;
+  SplashCount c = {0.0f, 0u, 0u};
+  if (on) {
+    droplet_draw(T, d, x, z, D, v);
+    m = T.drop_mc * D * D * D;
+    if (T.R > 0u) deposit_ripple(T, x, z, 2.0f * m * v);
+    c = splash_count(T, v, m);
+  }
+  // inclusive scan of the record counts over the wavefront
+  const uint32_t mine = c.n + c.nj;
+  uint32_t incl = mine;
+  const int lane = (int)threadIdx.x;
+#pragma unroll
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t y = (uint32_t)__shfl_up((int)incl, o, 64);
+    if (lane >= o) incl += y;
+  }
+  const uint32_t total = (uint32_t)__shfl((int)incl, 63, 64);
+  uint32_t base = 0;
+  if (lane == 63 && total) base = atomicAdd(&T.C->wwCount, total);
+  base = (uint32_t)__shfl((int)base, 63, 64);
+  if (mine) splash_write(T, F3(x, T.oy + T.rmd, z), v, c, 0x80000000u + d, base + incl - mine);
 }
 void sl_k_droplets(const SlLaunch &L) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_droplets, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  hipLaunchKernelGGL(k_droplets, dim3(L.w.total), dim3(64), 0, L.stream, L.w);
 }
 
 // ------------------------------------------------------------------------------------------------

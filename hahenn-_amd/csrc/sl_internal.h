@@ -402,7 +402,7 @@ void sl_k_compact_count(const SlLaunch &L);
 void sl_k_compact_scan(const SlLaunch &L, int ntanks);
 void sl_k_compact_move(const SlLaunch &L);
 void sl_k_ww(const SlLaunch &L, float dt);
-void sl_k_droplets(const SlLaunch &L);  // one thread per droplet (SL_WG per workgroup)
+void sl_k_droplets(const SlLaunch &L);  // one thread per droplet (64 per workgroup)
 void sl_k_clear_i32(const SlLaunch &L, int which);
 // spatial re-ordering (rocPRIM radix sort of cell-index keys)
 size_t sl_sort_temp_bytes(uint32_t n, int end_bit);

@@ -911,3 +911,249 @@ def test_top_height_blur_is_the_separable_normalised_gaussian():
     o.kernel("toph_blur")
     assert np.all(Hm[:, :8] == -1.0)
     assert np.abs(Hm[:, 8:] - 0.37).max() < 1e-6
+
+
+# ------------------------------------------------------------------------------------------------
+# Emission shapes, collisions, drains, solid face velocities, crown ring spacing: pinned by the
+# distributions and the geometry they must have.
+def _ks_uniform(x):
+    from scipy.stats import kstest
+    return kstest(np.asarray(x, np.float64), "uniform").pvalue
+
+
+def _emit(o, rec, n):
+    o.write_params(1 / 60)
+    o.field("counters")[0] = 0
+    o.set("count", 0)
+    o.emit_now([dict(rec, count=n)])
+    assert int(o.field("counters")[0]) == n
+    return o.field("parts")[:n].copy()
+
+
+def test_emit_disc_outlet_distribution():
+    """Shape 0 (stream / shower, 31-wgsl-sim2.js:276-292): positions uniform over the outlet disc
+    (r^2 and the angle uniform) and along the substep's travel; velocity along the axis within the
+    1% jitter, or inside the shower's cone with cos(angle) uniform on the cap."""
+    o = empty_sim(tank=(1.0, 0.5, 0.5), h=1 / 32, cap=40000)
+    axis = np.array([0.6, -0.64, 0.48]) / np.linalg.norm([0.6, -0.64, 0.48])
+    u = np.cross(axis, [0, 1, 0]); u /= np.linalg.norm(u)
+    w = np.cross(axis, u)
+    c = np.array([0.05, 0.3, -0.02])
+    R, speed, travel = 0.012, 1.5, 1.5 / 240
+    n = 20000
+    for spread in (0.0, 0.4):
+        rec = {"shape": 0, "a": list(c) + [R], "b": list(axis) + [speed], "c": list(u) + [spread],
+               "d": list(w) + [travel], "seed": 41}
+        p = _emit(o, rec, n).astype(np.float64)
+        rel = p[:, :3] - c
+        along = rel @ axis
+        radial = rel - np.outer(along, axis)
+        rr = np.linalg.norm(radial, axis=1)
+        assert along.min() >= -1e-6 and along.max() <= travel + 1e-6 and rr.max() <= R * (1 + 1e-5)
+        assert _ks_uniform(along / travel) > 1e-3
+        assert _ks_uniform((rr / R) ** 2) > 1e-3
+        assert _ks_uniform((np.arctan2(radial @ w, radial @ u) + np.pi) / (2 * np.pi)) > 1e-3
+        v = p[:, 4:7]
+        assert np.all(p[:, 3] == 1) and np.all(p[:, 7] == 0)
+        if spread == 0:
+            assert np.abs(v - axis * speed).max() <= 0.01 * speed + 1e-6
+        else:
+            cosang = (v / np.linalg.norm(v, axis=1, keepdims=True)) @ axis
+            assert cosang.min() >= math.cos(spread) - 0.02
+            # uniform on the spherical cap: cos uniform in [cos(spread), 1] (the jitter blurs it)
+            assert abs(cosang.mean() - (1 + math.cos(spread)) / 2) < 0.005
+
+
+def test_emit_sphere_and_rain_distributions():
+    """Shape 1 (drops, bursts, 31-wgsl-sim2.js:293-295): inside the ball, radius r = R u^(1/3), the
+    record's velocity, free-flight mark from b.w. Shape 3 (rain, :307-311): uniform over the square
+    and the substep's fall, velocity b, marked free-flying."""
+    o = empty_sim(tank=(1.0, 0.5, 0.5), h=1 / 32, cap=40000)
+    c = np.array([-0.1, 0.35, 0.05])
+    n = 20000
+    for mark in (0.0, 1.0):
+        rec = {"shape": 1, "a": list(c) + [0.03], "b": [0.2, -1.0, 0.1, mark], "seed": 9}
+        p = _emit(o, rec, n).astype(np.float64)
+        r = np.linalg.norm(p[:, :3] - c, axis=1) / 0.03
+        assert r.max() <= 1 + 1e-5
+        assert _ks_uniform(r ** (1 / 0.3333)) > 1e-3
+        assert np.allclose(p[:, 4:7], [0.2, -1.0, 0.1]) and np.all(p[:, 7] == mark)
+    rec = {"shape": 3, "a": [0.1, 0.45, -0.05, 0.2], "b": [0.0, -6.5, 0.0, 0.0], "d": [0, 0, 0, 6.5 / 240], "seed": 3}
+    p = _emit(o, rec, n).astype(np.float64)
+    assert _ks_uniform((p[:, 0] - (0.1 - 0.2)) / 0.4) > 1e-3
+    assert _ks_uniform((p[:, 2] - (-0.05 - 0.2)) / 0.4) > 1e-3
+    assert _ks_uniform((0.45 - p[:, 1]) / (6.5 / 240)) > 1e-3
+    assert np.allclose(p[:, 4:7], [0, -6.5, 0]) and np.all(p[:, 7] == 1) and np.all(p[:, 3] == 1)
+
+
+def test_emission_inside_an_obstacle_is_dead():
+    o = empty_sim(tank=(1.0, 0.5, 0.5), h=1 / 32, cap=40000)
+    c = np.array([0.1, 0.3, -0.05])
+    o.set_objects(obj_row(1, c, (0.06, 0, 0, 0), bound=0.062)[None])
+    rec = {"shape": 3, "a": [0.1, 0.36, -0.05, 0.1], "b": [0.0, -6.5, 0.0, 0.0], "d": [0, 0, 0, 0.12], "seed": 5}
+    p = _emit(o, rec, 8000).astype(np.float64)
+    d = np.linalg.norm(p[:, :3] - c, axis=1) - 0.06
+    far = np.abs(d) > 1e-5
+    assert (d < 0).sum() > 500
+    assert np.array_equal(p[far, 3] == 0, d[far] < 0)
+
+
+def test_g2p_obstacle_push_out_and_friction():
+    """A particle that ends inside the contact radius rc of an obstacle is moved out along the
+    surface normal to exactly rc from the surface; relative to the obstacle's surface velocity
+    v + w x r, an approaching normal component is removed and the tangential part keeps 1 - 0.03
+    (31-wgsl-sim2.js:170-178). Moving sphere above the tank, particles in free flight."""
+    o = empty_sim(tank=(0.5, 0.25, 0.25), h=1 / 32, cap=4000)
+    inf = o.info()
+    c = np.array([0.0, inf["rim"] + 0.12, 0.0])
+    r = 0.05
+    lin, ang = np.array([0.2, 0.0, -0.1]), np.array([0.0, 3.0, 1.0])
+    o.set_objects(obj_row(1, c, (r, 0, 0, 0), bound=r + 0.002, lin=lin, ang=ang)[None])
+    o.write_params(1 / 120)
+    rc = 0.32 * inf["h"]
+    rng = np.random.default_rng(4)
+    d = rng.normal(size=(300, 3))
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    d = d[d[:, 1] > -0.5]
+    x0 = c + d * (r + 0.3 * rc)
+    v0 = -d * 0.3 + rng.normal(scale=0.3, size=d.shape)
+    place_particles(o, x0, v0)
+    o.kernel("g2p")
+    n = len(d)
+    out = o.field("parts")[:n].astype(np.float64)
+    vb = v0.astype(np.float32).astype(np.float64).copy()
+    vb[:, 1] -= np.float32(9.81 * (1 / 120))                 # free flight outside the grid
+    x1 = x0.astype(np.float32) + vb * (1 / 120)
+    hit = np.linalg.norm(x1 - c, axis=1) - r < rc
+    assert hit.sum() > 100
+    dist = np.linalg.norm(out[:, :3] - c, axis=1)
+    assert np.abs(dist[hit] - (r + rc)).max() < 2e-5
+    nrm = (out[hit, :3] - c) / dist[hit, None]
+    vs = lin + np.cross(ang, out[hit, :3] - c)
+    rel = vb[hit] - vs
+    vn = np.sum(rel * nrm, axis=1)
+    want = np.where((vn < 0)[:, None], vs + (rel - nrm * vn[:, None]) * 0.97, vb[hit])
+    assert np.abs(out[hit, 4:7] - want).max() < 2e-3
+    assert (vn < 0).sum() > 50
+    # particles that stay clear of the contact radius keep their free flight
+    if (~hit).any():
+        assert np.abs(out[~hit, 4:7] - vb[~hit]).max() < 1e-6
+
+
+def test_drain_kills_inside_its_cylinder():
+    """A drain (flag 2) kills the particles that end strictly inside its cylinder of radius r and
+    half height 12 mm in its own frame (30-wgsl-sim.js:107-114, 31-wgsl-sim2.js:181)."""
+    from scipy.spatial.transform import Rotation
+    o = empty_sim(tank=(0.5, 0.25, 0.25), h=1 / 32, cap=8000)
+    inf = o.info()
+    c = np.array([0.02, 0.1, -0.01])
+    Rm = Rotation.from_euler("xyz", [35, -20, 50], degrees=True).as_matrix()
+    rad = 0.03
+    o.set_objects(obj_row(9, c, (rad, 0.012, 0, 0), R=Rm, flag=2, bound=rad + 0.02)[None])
+    o.write_params(1 / 120)
+    rng = np.random.default_rng(8)
+    pts = c + rng.uniform(-0.05, 0.05, (5000, 3))
+    place_particles(o, pts, np.zeros_like(pts))
+    o.field("velOld")[:, 3] = 0      # no face takes part: the grid leaves the velocity as it is
+    o.kernel("g2p")
+    alive = o.field("parts")[:5000, 3] == 1
+    loc = (pts.astype(np.float32) - c) @ Rm
+    radial = np.hypot(loc[:, 0], loc[:, 2])
+    inside = (radial < rad) & (np.abs(loc[:, 1]) < 0.012)
+    margin = np.minimum(np.abs(radial - rad), np.abs(np.abs(loc[:, 1]) - 0.012)) > 1e-5
+    assert inside.sum() > 100
+    assert np.array_equal(alive[margin], ~inside[margin])
+
+
+def test_geom_solid_face_velocity_is_rigid_motion():
+    """GEOM gives every face the velocity of the nearest solid's rigid motion at the face centre,
+    v + w x (p - c) (30-wgsl-sim.js:96-105, :143-154): x, y and z components at their own faces."""
+    o = empty_sim(tank=(0.25, 0.125, 0.125), h=1 / 32)
+    inf = o.info()
+    c = np.array([0.02, 0.05, -0.01])
+    lin, ang = np.array([0.3, -0.1, 0.2]), np.array([1.5, -2.0, 4.0])
+    o.set_objects(obj_row(1, c, (0.03, 0, 0, 0), bound=0.032, lin=lin, ang=ang)[None])
+    o.write_params(1 / 60)
+    o.kernel("geom")
+    nx, ny, nz, h = int(inf["nx"]), int(inf["ny"]), int(inf["nz"]), inf["h"]
+    org = np.array([inf["ox"], inf["oy"], inf["oz"]])
+    sv = o.field("svel").reshape(nz + 1, ny + 1, nx + 1, 4)
+    k, j, i = np.meshgrid(np.arange(nz + 1), np.arange(ny + 1), np.arange(nx + 1), indexing="ij")
+    for comp, off, valid in ((0, (0, 0.5, 0.5), (i > 0) & (i < nx) & (j < ny) & (k < nz)),
+                             (1, (0.5, 0, 0.5), (j > 0) & (j < ny) & (i < nx) & (k < nz)),
+                             (2, (0.5, 0.5, 0), (k > 0) & (k < nz) & (i < nx) & (j < ny))):
+        p = org + h * np.stack([i + off[0], j + off[1], k + off[2]], -1)
+        want = (lin + np.cross(ang, p - c))[..., comp]
+        got = sv[..., comp]
+        assert np.abs(got[valid] - want[valid]).max() < 1e-5, comp
+
+
+def test_crown_ring_spacing_and_ranges():
+    """The crown ring (31-wgsl-sim2.js:131-140): n droplets on the circle of radius 0.6 D around the
+    impact, successive phases 2 pi / n apart to within +-0.3 of that spacing, elevations 0.9-1.35 rad,
+    speeds U (0.15-0.35), lives 0.5-0.9; the jets rise from the centre 1 mm apart."""
+    q = O.quality("standard", h=1 / 32, iters=1, ripple_R=0, cap=1000, ww_cap=4000)
+    o = O.Oracle((0.25, 0.125, 0.125), q)
+    o.write_params(1 / 120)
+    x = np.array([0.01, 0.05, -0.02], np.float32)
+    U = 3.0
+    m = 998 * (0.004 ** 3)          # a drop of about 5 mm: K well above 2100
+    D = (6 * m / (math.pi * 998)) ** (1 / 3)
+    for trial in range(6):
+        o.field("counters")[2] = 0
+        o.L.or_crown(o.ptr, x.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), U, m, 1000 + trial)
+        cnt = int(o.field("counters")[2])
+        ww = o.field("ww")[:cnt].astype(np.float64)
+        jets = ww[np.abs(ww[:, 0] - x[0]) + np.abs(ww[:, 2] - x[2]) < 1e-6]
+        ring = ww[np.abs(ww[:, 0] - x[0]) + np.abs(ww[:, 2] - x[2]) >= 1e-6]
+        n = len(ring)
+        assert n >= 3 and len(jets) == 2
+        rel = ring[:, [0, 2]] - x[[0, 2]]
+        assert np.allclose(np.hypot(rel[:, 0], rel[:, 1]), 0.6 * D, rtol=1e-5)
+        ph = np.unwrap(np.arctan2(rel[:, 1], rel[:, 0]))
+        gaps = np.diff(ph) / (2 * np.pi / n)
+        assert gaps.min() >= 0.7 - 1e-4 and gaps.max() <= 1.3 + 1e-4
+        v = ring[:, 4:7]
+        sp = np.linalg.norm(v, axis=1)
+        el = np.arcsin(np.clip(v[:, 1] / sp, -1, 1))
+        assert el.min() >= 0.9 - 1e-4 and el.max() <= 1.35 + 1e-4
+        assert sp.min() >= 0.15 * U - 1e-5 and sp.max() <= 0.35 * U + 1e-5
+        assert ring[:, 3].min() >= 0.5 and ring[:, 3].max() <= 0.9
+        assert np.allclose(np.sort(jets[:, 1]) - x[1], [0.0, 0.001], atol=1e-6) and np.all(jets[:, 3] == np.float32(0.7))
+    o.close()
+
+
+def test_droplet_draws_impulses_and_splash_counts():
+    """The droplet layer's draws are what its definition says: diameters lognormal (ln D normal with
+    mean ln d50 and sigma_ln), speeds uniform on [vmin, vmax], impact points uniform over the
+    footprint; the impulse grid holds the sum of the truncated 2 m v x 1e9 of the droplets (mass
+    rho pi/6 D^3), and the whitewater count is the sum over droplets of the ring and jet counts of
+    K = We Oh^-0.4 (We = rho U^2 D / sigma, Oh = mu / sqrt(rho sigma D)) evaluated in float64."""
+    from scipy.stats import kstest
+    q = O.quality("standard", h=1 / 32, iters=1, ripple_R=64, cap=1000, ww_cap=400000, droplets=20000,
+                  drop_seed=5, splash_rho=1000.0, splash_sigma=0.072, splash_mu=0.001, splash_k_jet=60.0,
+                  ripple_depth=0.05)
+    o = O.Oracle((0.5, 0.25, 0.25), q)
+    o.write_params(1 / 240)
+    inf = o.info()
+    dr = np.zeros((20000, 4), np.float32)
+    o.L.or_droplet_draws(o.ptr, dr.ctypes.data)
+    x, z, D, v = (dr[:, k].astype(np.float64) for k in range(4))
+    assert kstest((np.log(D) - math.log(0.0015)) / 0.4, "norm").pvalue > 1e-3
+    assert _ks_uniform((v - 1.0) / 3.0) > 1e-3
+    W, Dp = inf["nx"] * inf["h"], inf["nz"] * inf["h"]
+    assert _ks_uniform((x - inf["ox"]) / W) > 1e-3 and _ks_uniform((z - inf["oz"]) / Dp) > 1e-3
+    o.L.or_droplets(o.ptr, 0.05)
+    m = 1000 * math.pi / 6 * D ** 3
+    units = np.floor(np.minimum(2 * m * v, 2e-3) * 1e9)
+    got = int(o.field("rsrc").astype(np.int64).sum())
+    assert abs(got - units.sum()) <= 0.001 * len(D)        # a truncation may differ by one unit
+    We = 1000 * v * v * D / 0.072
+    Oh = 0.001 / np.sqrt(1000 * 0.072 * D)
+    K = We * Oh ** -0.4
+    ring = np.clip(np.floor(4 * (K / 2100 - 1) + 0.5), 0, 24)
+    jets = np.where(K > 60, np.where(K > 2100, 2, 1), 0)
+    cnt = int(o.field("counters")[2])
+    want = int((ring + jets).sum())
+    assert abs(cnt - want) <= 0.001 * len(D) and cnt > len(D)
+    o.close()
