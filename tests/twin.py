@@ -75,3 +75,26 @@ def assert_close(name, got, want, tol):
     e = rel_err(got, want) if got.size else 0.0
     assert e <= tol, f"{name}: max relative error {e:.3e} > {tol:.1e}"
     return e
+
+
+def make_twin_workload(wl, quality="standard", threads=8):
+    """Oracle and GPU simulation of a bench workload (hahenn-_amd/workloads.py): the same
+    configuration, objects (packed by the library's host code), lattice fill and, when the bench
+    re-orders after the fill, the same re-ordering."""
+    c = dict(wl["cfg"])
+    qkeys = {f for f, _ in O.OrQuality._fields_}
+    q = O.quality(quality, **{k: v for k, v in c.items() if k in qkeys})
+    if not c.get("whitewater", 1):
+        q.ww_rate = 0.0
+    o = O.Oracle((c["tank_w"], c["tank_h"], c["tank_d"]), q, threads=threads)
+    g = suiri.Sim(suiri.config(quality, **c))
+    rows, _ = scene.Scene(g.cfg).pack(wl["items"], 0)
+    o.set_objects(rows)
+    g.set_objects_rows(rows)
+    inf = o.info()
+    recs = [scene.cell_block_emit(inf["h"], [inf["ox"], inf["oy"], inf["oz"]], *blk, seed=wl["seed"] + i)
+            for i, blk in enumerate(wl["blocks"])]
+    o.emit_now(recs)
+    g.emit_now(recs)
+    g.set_substeps(1)
+    return o, g
