@@ -94,7 +94,7 @@ EXPORTS = ["sl_config_default", "sl_create", "sl_destroy", "sl_last_error", "sl_
            "sl_server_info", "sl_server_job_sim", "sl_server_release", "sl_server_stats", "sl_bake_frame",
            "sl_write_exr_rgb", "sl_scene_create", "sl_scene_destroy", "sl_scene_last_error", "sl_sim_scene",
            "sl_scene_pack_objects", "sl_set_items", "sl_scene_source_emits", "sl_scene_block_emits",
-           "sl_block_liters", "sl_step_items"]
+           "sl_block_liters", "sl_step_items", "sl_ripple_frame_noalloc"]
 
 EMIT_FN = ctypes.CFUNCTYPE(ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                            ctypes.POINTER(SlEmit), ctypes.c_uint32)
@@ -226,6 +226,7 @@ def load():
         "sl_scene_block_emits": (I, [P, P, U, I, P, U, ctypes.POINTER(U)]),
         "sl_block_liters": (D, [P, D, D, D]),
         "sl_step_items": (I, [P, D, P, U]),
+        "sl_ripple_frame_noalloc": (I, [ctypes.POINTER(SlConfig), U, D, D, D, D, P, P, P, P]),
         "sl_version": (I, []),
         "sl_build_info": (ctypes.c_char_p, []),
     }
@@ -547,6 +548,17 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+def ripple_frame_noalloc(cfg, R, Lx, Lz, dt, depth, state_ptr, src_ptr, tex_ptr, stream=None):
+    """One unmasked ripple frame on caller-owned device buffers (addresses), allocating nothing
+    (sl_ripple_frame_noalloc)."""
+    L = load()
+    r = L.sl_ripple_frame_noalloc(ctypes.byref(cfg), int(R), float(Lx), float(Lz), float(dt), float(depth),
+                                  ctypes.c_void_p(state_ptr), ctypes.c_void_p(src_ptr), ctypes.c_void_p(tex_ptr),
+                                  ctypes.c_void_p(stream) if stream else None)
+    if r != SL_OK:
+        raise SuiriError(f"suiri error {r}: {L.sl_last_error(None).decode()}")
 
 
 def job_bytes(cfg):

@@ -16,6 +16,8 @@
 #include "sl_scene.h"
 
 static thread_local std::string g_err;
+// the message of sl_last_error(NULL) for failures of entry points without a simulation
+void sl_set_error(const char *msg) { g_err = msg ? msg : ""; }
 
 // dynamic per-substep fields of a tank, written on the device by k_update (kernel arguments are
 // captured at launch, so the host can change them freely while earlier work is in flight)

@@ -452,6 +452,20 @@ sl_sim *sl_server_job_sim(sl_server *srv, uint64_t id);
 int sl_server_release(sl_server *srv, uint64_t id);
 int sl_server_stats(const sl_server *srv, sl_server_stats_t *out);
 
+/* One unmasked ripple frame (RippleLayer.encodeFrame, src/js/64-ripple.js:222-253, the mask unbound) on
+   caller-owned device buffers, allocating no device memory. No counterpart in the reference: for
+   callers that cannot allocate (code under HIP graph capture, tight serving loops). The same kernels
+   as sl_run_stage(sim, SL_S_RIPPLE) with the mask off: `state` (M^2 float2, read and written), `src`
+   ((R+1)^2 i32 impulses at 1e9 per N s, read and cleared), `tex` ((R+1)^2 float4 height, slope x,
+   slope z, 0; written), on the surface Lx x Lz [m] with frame length dt [s] and depth [m] (at least
+   0.002 is used); the physics constants are cfg's ripple fields (r_*, gravity). M = 2R, a power of
+   two, at most SL_RIPPLE_NOALLOC_MAX_M. `stream` is a hipStream_t (NULL: the null stream), on the
+   current device. The tank record, the twiddles and the three M x M work fields live in one static
+   device array of 6.3 MB per GPU, kept until the process ends; calls that overlap on one GPU race. */
+#define SL_RIPPLE_NOALLOC_MAX_M 512
+int sl_ripple_frame_noalloc(const sl_config *cfg, uint32_t R, double Lx, double Lz, double dt, double depth,
+                            void *state, int32_t *src, void *tex, void *stream);
+
 /* ABI version + build description */
 int sl_version(void);
 const char *sl_build_info(void);
