@@ -94,7 +94,7 @@ EXPORTS = ["sl_config_default", "sl_create", "sl_destroy", "sl_last_error", "sl_
            "sl_server_info", "sl_server_job_sim", "sl_server_release", "sl_server_stats", "sl_bake_frame",
            "sl_write_exr_rgb", "sl_scene_create", "sl_scene_destroy", "sl_scene_last_error", "sl_sim_scene",
            "sl_scene_pack_objects", "sl_set_items", "sl_scene_source_emits", "sl_scene_block_emits",
-           "sl_block_liters", "sl_step_items", "sl_ripple_frame_noalloc"]
+           "sl_block_liters", "sl_step_items", "sl_ripple_frame_noalloc", "sl_wait_idle"]
 
 EMIT_FN = ctypes.CFUNCTYPE(ctypes.c_uint32, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_double, ctypes.c_double,
                            ctypes.POINTER(SlEmit), ctypes.c_uint32)
@@ -172,6 +172,7 @@ def load():
         "sl_poll_stats": (I, [P, ctypes.POINTER(I)]),
         "sl_stats": (I, [P, ctypes.POINTER(SlStats)]),
         "sl_sync": (I, [P]),
+        "sl_wait_idle": (I, [P]),
         "sl_set_ripple_mask": (I, [P, I]),
         "sl_field_bytes": (ctypes.c_size_t, [P, I]),
         "sl_upload": (I, [P, I, P, ctypes.c_size_t]),

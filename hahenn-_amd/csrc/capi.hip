@@ -1032,6 +1032,10 @@ extern "C" int sl_poll_stats(sl_sim *s, int *updated) {
   if (updated) *updated = 1;
   return SL_OK;
 }
+extern "C" int sl_wait_idle(const sl_sim *s) {
+  if (!s || !s->stream) return SL_EARG;
+  return hipStreamSynchronize(s->stream) == hipSuccess ? SL_OK : SL_EHIP;
+}
 extern "C" int sl_sync(sl_sim *s) {
   if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
   HIPCHK(s, hipStreamSynchronize(s->stream));

@@ -58,6 +58,8 @@ static void set_num(napi_env env, napi_value obj, const char *key, double v) {
 struct Handle {
   sl_sim *sim = nullptr;
   bool owned = true;  // false: a tank of a served job, destroyed with its server
+  int pending = 0;    // statsAsync waits in flight on worker threads
+  bool doomed = false;  // destroy() came while a wait was in flight: destroyed when the last one ends
 };
 static void finalize(napi_env, void *data, void *) {
   Handle *h = (Handle *)data;
@@ -66,7 +68,7 @@ static void finalize(napi_env, void *data, void *) {
 }
 static sl_sim *get_sim(napi_env env, napi_value v) {
   Handle *h = nullptr;
-  if (napi_get_value_external(env, v, (void **)&h) != napi_ok || !h || !h->sim) {
+  if (napi_get_value_external(env, v, (void **)&h) != napi_ok || !h || !h->sim || h->doomed) {
     napi_throw_error(env, nullptr, "suiri: invalid or destroyed simulation handle");
     return nullptr;
   }
@@ -142,6 +144,7 @@ static napi_value Destroy(napi_env env, napi_callback_info info) {
   if (!get_args(env, info, 1, argv)) return nullptr;
   Handle *h = nullptr;
   if (napi_get_value_external(env, argv[0], (void **)&h) == napi_ok && h && h->sim) {
+    if (h->pending) { h->doomed = true; return nullptr; }  // a worker still waits on its stream
     sl_destroy(h->sim);
     h->sim = nullptr;
   }
@@ -439,14 +442,7 @@ static napi_value PollStats(napi_env env, napi_callback_info info) {
   if (r) return throw_sl(env, s, r);
   return boolean(env, u != 0);
 }
-static napi_value Stats(napi_env env, napi_callback_info info) {
-  napi_value argv[1];
-  if (!get_args(env, info, 1, argv)) return nullptr;
-  sl_sim *s = get_sim(env, argv[0]);
-  if (!s) return nullptr;
-  sl_stats_t st;
-  int r = sl_stats(s, &st);
-  if (r) return throw_sl(env, s, r);
+static napi_value stats_js(napi_env env, const sl_stats_t &st) {
   napi_value o;
   napi_create_object(env, &o);
   set_num(env, o, "count", st.count); set_num(env, o, "ww", st.ww_count); set_num(env, o, "impacts", st.impacts);
@@ -459,6 +455,78 @@ static napi_value Stats(napi_env env, napi_callback_info info) {
   for (uint32_t i = 0; i < 3; i++) napi_set_element(env, grid, i, num(env, st.grid[i]));
   napi_set_named_property(env, o, "grid", grid);
   return o;
+}
+static napi_value Stats(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  sl_stats_t st;
+  int r = sl_stats(s, &st);
+  if (r) return throw_sl(env, s, r);
+  return stats_js(env, st);
+}
+
+// statsAsync(h) -> Promise of the stats object: a worker thread waits for the work submitted so far
+// (sl_wait_idle, which touches no simulation state), then the JS thread reads the counters
+// (suiriLab.stats() after device.queue.onSubmittedWorkDone(), 90-main.js:159-165, :196)
+struct StatsJob {
+  napi_async_work work = nullptr;
+  napi_deferred deferred = nullptr;
+  napi_ref handle_ref = nullptr;  // keeps the simulation alive while the worker waits
+  sl_sim *sim = nullptr;
+  int wait_rc = 0;
+};
+static void stats_execute(napi_env, void *data) {
+  StatsJob *j = (StatsJob *)data;
+  j->wait_rc = sl_wait_idle(j->sim);
+}
+static void stats_complete(napi_env env, napi_status status, void *data) {
+  StatsJob *j = (StatsJob *)data;
+  napi_value handle = nullptr;
+  napi_get_reference_value(env, j->handle_ref, &handle);
+  Handle *h = nullptr;
+  napi_get_value_external(env, handle, (void **)&h);
+  if (h) h->pending--;
+  if (h && h->doomed && !h->pending && h->sim) {  // destroy() was called while the worker waited
+    sl_destroy(h->sim);
+    h->sim = nullptr;
+  }
+  sl_stats_t st;
+  int r = j->wait_rc;
+  if (status != napi_ok) r = SL_ESTATE;
+  else if (!h || !h->sim) r = SL_ESTATE;
+  else if (!r) r = sl_stats(h->sim, &st);
+  if (r) {
+    std::string m = "suiri error " + std::to_string(r) + ": " + (h && h->sim ? sl_last_error(h->sim) : "simulation destroyed");
+    napi_value msg, err;
+    napi_create_string_utf8(env, m.c_str(), NAPI_AUTO_LENGTH, &msg);
+    napi_create_error(env, nullptr, msg, &err);
+    napi_reject_deferred(env, j->deferred, err);
+  } else {
+    napi_resolve_deferred(env, j->deferred, stats_js(env, st));
+  }
+  napi_delete_reference(env, j->handle_ref);
+  napi_delete_async_work(env, j->work);
+  delete j;
+}
+static napi_value StatsAsync(napi_env env, napi_callback_info info) {
+  napi_value argv[1];
+  if (!get_args(env, info, 1, argv)) return nullptr;
+  sl_sim *s = get_sim(env, argv[0]);
+  if (!s) return nullptr;
+  StatsJob *j = new StatsJob();
+  j->sim = s;
+  Handle *hh = nullptr;
+  napi_get_value_external(env, argv[0], (void **)&hh);
+  hh->pending++;
+  napi_value promise, name;
+  NAPI_CALL(env, napi_create_promise(env, &j->deferred, &promise));
+  NAPI_CALL(env, napi_create_reference(env, argv[0], 1, &j->handle_ref));
+  napi_create_string_utf8(env, "suiri.statsAsync", NAPI_AUTO_LENGTH, &name);
+  NAPI_CALL(env, napi_create_async_work(env, nullptr, name, stats_execute, stats_complete, j, &j->work));
+  NAPI_CALL(env, napi_queue_async_work(env, j->work));
+  return promise;
 }
 // hostState(h) -> {count, time, frame, vmax, full, impacts, wwCount}: no device wait
 static napi_value HostState(napi_env env, napi_callback_info info) {
@@ -1165,6 +1233,7 @@ static napi_value Init(napi_env env, napi_value exports) {
       {"restoreWater", 0, RestoreWater, 0, 0, 0, napi_default, 0},
       {"pollStats", 0, PollStats, 0, 0, 0, napi_default, 0},
       {"stats", 0, Stats, 0, 0, 0, napi_default, 0},
+      {"statsAsync", 0, StatsAsync, 0, 0, 0, napi_default, 0},
       {"hostState", 0, HostState, 0, 0, 0, napi_default, 0},
       {"setScalar", 0, SetScalar, 0, 0, 0, napi_default, 0},
       {"sync", 0, Sync, 0, 0, 0, napi_default, 0},

@@ -128,6 +128,12 @@ class FluidSim {
     return st;
   }
 
+  // a Promise of the same stats, resolved once the submitted work is done (the wait runs on a worker
+  // thread, so the JS thread stays free)
+  statsAsync() {
+    return addon.statsAsync(this.handle).then((st) => { this._pull(); return st; });
+  }
+
   idle() { addon.sync(this.handle); }
 
   // the library resets the sources' progress with the water, and saves / restores it with the copy

@@ -267,6 +267,10 @@ int sl_poll_stats(sl_sim *sim, int *updated);
 int sl_stats(sl_sim *sim, sl_stats_t *out);
 /* device.queue.onSubmittedWorkDone() */
 int sl_sync(sl_sim *sim);
+/* waits for the work submitted to the simulation so far without touching its state, so another
+   thread may wait while the owning thread keeps calling (the Node addon's statsAsync waits on a
+   worker thread); returns SL_OK or SL_EHIP */
+int sl_wait_idle(const sl_sim *sim);
 
 /* Ripple mask on/off: suiriLab.noRippleMask() — src/js/90-main.js:195 */
 int sl_set_ripple_mask(sl_sim *sim, int bound);

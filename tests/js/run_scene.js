@@ -37,4 +37,16 @@ fs.writeFileSync(path.join(outdir, 'trace.json'), JSON.stringify(trace));
 fs.writeFileSync(path.join(outdir, 'particles.bin'), Buffer.from(sim.readback('particles').buffer));
 fs.writeFileSync(path.join(outdir, 'pres.bin'), Buffer.from(sim.readback('pres').buffer));
 console.log(JSON.stringify({ count: st.count, frames: st.frames, vmax: st.vmax }));
-sim.destroy();
+// statsAsync (a Promise resolved once the submitted work is done) agrees with the blocking stats,
+// also with work enqueued after the call
+sim.step(1 / 120, null);
+const pending = sim.statsAsync();
+const stNow = sim.stats();
+pending.then((sa) => {
+  if (sa.count !== stNow.count || sa.frames !== stNow.frames || sa.ww !== stNow.ww) {
+    console.error('statsAsync mismatch', JSON.stringify(sa), JSON.stringify(stNow));
+    process.exit(5);
+  }
+  console.log(JSON.stringify({ statsAsync: 'ok' }));
+  sim.destroy();
+}, (e) => { console.error(e); process.exit(6); });

@@ -36,6 +36,7 @@ def test_js_fluidsim_dam_preset_matches_oracle(tmp_path):
     r = subprocess.run([NODE, os.path.join(ROOT, "tests", "js", "run_scene.js"), "dam", str(frames), str(tmp_path)],
                        capture_output=True, text=True, timeout=300)
     assert r.returncode == 0, r.stdout + r.stderr
+    assert '"statsAsync":"ok"' in r.stdout   # the Promise-based stats agree with the blocking ones
     tr = json.load(open(tmp_path / "trace.json"))
     opts = tr["opts"]
     q = O.quality("low", ripple_R=opts["ripple_R"], cap=opts["cap"], ww_cap=opts["ww_cap"])
