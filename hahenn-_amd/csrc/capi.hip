@@ -391,6 +391,7 @@ static int tank_buffers(sl_sim *s, SlTank &T, Take take) {
   if (s->ext_counters) T.C = s->ext_counters;
   else TAKE(T.C, sizeof(SlCounters));
   TAKE(T.args, 64);
+  TAKE(T.vslot, (size_t)SL_VSLOTS * SL_VSLOT_STRIDE * 4);
   TAKE(T.acc, N * 24);
   TAKE(T.cellCount, C * 4);
   TAKE(T.vel, N * 12); TAKE(T.velOld, N * 12);
@@ -1043,6 +1044,7 @@ extern "C" int sl_sync(sl_sim *s) {
 }
 extern "C" int sl_stats(sl_sim *s, sl_stats_t *o) {
   if (!s || !o) FAIL((sl_sim *)nullptr, SL_EARG, "sl_stats: null argument");
+  sl_k_count(launch_of(s, 1), 0, 0);  // G2P's speed slots into vmax
   HIPCHK(s, hipStreamSynchronize(s->stream));
   int r = sl_poll_stats(s, nullptr);
   if (r) return r;
@@ -1113,6 +1115,7 @@ extern "C" int sl_save_water(sl_sim *s, uint32_t *id) {
     HIPCHK(s, hipMemcpyAsync(v.ww, T.wp[w], wbytes / 2, hipMemcpyDeviceToDevice, s->stream));
     HIPCHK(s, hipMemcpyAsync((uint8_t *)v.ww + wbytes / 2, T.wv[w], wbytes / 2, hipMemcpyDeviceToDevice, s->stream));
   }
+  sl_k_count(launch_of(s, 1), 0, 0);  // G2P's speed slots into vmax before the counters are copied
   HIPCHK(s, hipMemcpyAsync(v.counters, T.C, 32, hipMemcpyDeviceToDevice, s->stream));
   if (rbytes) HIPCHK(s, hipMemcpyAsync(v.ripple, T.rstate, rbytes, hipMemcpyDeviceToDevice, s->stream));
   v.time = s->time; v.count = s->count; v.ww_count = s->ww_count; v.impacts = s->impacts; v.full = s->full;
@@ -1142,6 +1145,7 @@ extern "C" int sl_restore_water(sl_sim *s, uint32_t id) {
     HIPCHK(s, hipMemcpyAsync(T.wv[w], (uint8_t *)v.ww + wbytes / 2, wbytes / 2, hipMemcpyDeviceToDevice, s->stream));
   }
   HIPCHK(s, hipMemcpyAsync(T.C, v.counters, 32, hipMemcpyDeviceToDevice, s->stream));
+  sl_k_fill_u32(s->stream, T.vslot, 0, (size_t)SL_VSLOTS * SL_VSLOT_STRIDE);
   if (rbytes) HIPCHK(s, hipMemcpyAsync(T.rstate, v.ripple, rbytes, hipMemcpyDeviceToDevice, s->stream));
   s->time = v.time; s->count = v.count; s->ww_count = v.ww_count; s->impacts = v.impacts; s->full = v.full;
   s->scene.sources = v.sources;
@@ -1197,6 +1201,12 @@ static int d2h(sl_sim *s, void *dst, const void *src, size_t bytes) {
 static int h2d(sl_sim *s, void *dst, const void *src, size_t bytes) {
   HIPCHK(s, hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
   return SL_OK;
+}
+// G2P's speed slots folded into the device vmax, waited for
+static int fold_vmax(sl_sim *s) {
+  sl_k_count(launch_of(s, 1), 0, 0);
+  HIPCHK(s, hipStreamSynchronize(s->stream));
+  return check_launch(s, "vmax fold");
 }
 static inline uint32_t packed_slot(const sl_sim *s, uint32_t c) {
   uint32_t i = c % s->nx, j = (c / s->nx) % s->ny, k = c / (s->nx * s->ny);
@@ -1257,7 +1267,9 @@ extern "C" int sl_readback(sl_sim *s, int f, void *dst, size_t bytes) {
       for (size_t i = 0; i < W; i++) { memcpy(o + i * 8, &a[i * 4], 16); memcpy(o + i * 8 + 4, &b[i * 4], 16); }
       return SL_OK;
     }
-    case SL_F_COUNTERS: return d2h(s, dst, T.C, 32);
+    case SL_F_COUNTERS:
+      if ((r = fold_vmax(s))) return r;
+      return d2h(s, dst, T.C, 32);
     case SL_F_ACC: {  // device: 3 u64 planes (mom << 32 | wt); reference: 6 i32 per node
       std::vector<uint64_t> a(N * 3);
       if ((r = d2h(s, a.data(), T.acc, N * 24))) return r;
@@ -1353,6 +1365,8 @@ extern "C" int sl_upload(sl_sim *s, int f, const void *src, size_t bytes) {
     case SL_F_COUNTERS: {
       r = h2d(s, T.C, src, 32);
       if (r) return r;
+      sl_k_fill_u32(s->stream, T.vslot, 0, (size_t)SL_VSLOTS * SL_VSLOT_STRIDE);
+      HIPCHK(s, hipStreamSynchronize(s->stream));
       const SlCounters *cc = (const SlCounters *)src;
       s->p_ub = cc->count < s->cap ? cc->count : s->cap;
       return SL_OK;
@@ -1471,7 +1485,10 @@ extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
       if (s->solver == SL_SOLVER_MGPCG) sl_k_project_mg(s->stream, s->d_tank, s->nodes, s->mg.xs);
       else sl_k_project(launch_of(s, s->nodes));
       break;
-    case SL_S_G2P: sl_k_g2p(launch_of(s, s->p_ub), s->g2p_variant); break;
+    case SL_S_G2P:
+      sl_k_g2p(launch_of(s, s->p_ub), s->g2p_variant);
+      sl_k_count(launch_of(s, 1), 0, 0);  // the speed slots into vmax
+      break;
     case SL_S_COMPACT: return encode_compact(s);
     case SL_S_WW:
       sl_k_ww(launch_of(s, s->ww_cap), (float)dt);
@@ -1536,7 +1553,7 @@ extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
   const int c = s->cur;
   const size_t P = s->p_ub, C = s->cells;
   const size_t R1 = s->R ? (size_t)(s->R + 1) * (s->R + 1) : 0;
-  // snapshot of what P2G and G2P write: particles, counters, cell counts, ripple impulses
+  // snapshot of what P2G and G2P write: particles, counters, speed slots, cell counts, ripple impulses
   std::vector<void *> snap;
   bool ok = true;
   auto dup = [&](const void *src, size_t bytes) -> void * {
@@ -1551,6 +1568,7 @@ extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
   void *pcpy[7];
   for (int i = 0; i < 7; i++) pcpy[i] = dup(psrc[i], i < 6 ? P * 4 : P);
   void *ccpy = dup(T.C, sizeof(SlCounters));
+  void *vcpy = dup(T.vslot, (size_t)SL_VSLOTS * SL_VSLOT_STRIDE * 4);
   void *cccpy = dup(T.cellCount, C * 4);
   void *rcpy = R1 ? dup(T.rsrc, R1 * 4) : nullptr;
   if (!ok) {
@@ -1562,6 +1580,7 @@ extern "C" int sl_autotune(sl_sim *s, uint32_t reps, uint32_t *choice) {
     for (int i = 0; i < 7; i++)
       if (pcpy[i]) (void)hipMemcpyAsync(psrc[i], pcpy[i], i < 6 ? P * 4 : P, hipMemcpyDeviceToDevice, s->stream);
     if (ccpy) (void)hipMemcpyAsync(T.C, ccpy, sizeof(SlCounters), hipMemcpyDeviceToDevice, s->stream);
+    if (vcpy) (void)hipMemcpyAsync(T.vslot, vcpy, (size_t)SL_VSLOTS * SL_VSLOT_STRIDE * 4, hipMemcpyDeviceToDevice, s->stream);
     if (cccpy) (void)hipMemcpyAsync(T.cellCount, cccpy, C * 4, hipMemcpyDeviceToDevice, s->stream);
     if (rcpy) (void)hipMemcpyAsync(T.rsrc, rcpy, R1 * 4, hipMemcpyDeviceToDevice, s->stream);
   };
@@ -2356,10 +2375,11 @@ extern "C" int sl_batch_autotune(sl_batch *b, uint32_t reps, uint32_t *choice) {
     const SlTank &T = s->ht;
     const int c = s->cur;
     const size_t P = s->cap;
-    void *src[10] = {T.px[c], T.py[c], T.pz[c], T.vx[c], T.vy[c], T.vz[c], T.pf[c], T.C, T.cellCount, T.rsrc};
-    size_t by[10] = {P * 4, P * 4, P * 4, P * 4, P * 4, P * 4, P, sizeof(SlCounters), (size_t)s->cells * 4,
-                     s->R ? (size_t)(s->R + 1) * (s->R + 1) * 4 : 0};
-    for (int i = 0; i < 10; i++) {
+    void *src[11] = {T.px[c], T.py[c], T.pz[c], T.vx[c], T.vy[c], T.vz[c], T.pf[c], T.C, T.cellCount, T.rsrc,
+                     T.vslot};
+    size_t by[11] = {P * 4, P * 4, P * 4, P * 4, P * 4, P * 4, P, sizeof(SlCounters), (size_t)s->cells * 4,
+                     s->R ? (size_t)(s->R + 1) * (s->R + 1) * 4 : 0, (size_t)SL_VSLOTS * SL_VSLOT_STRIDE * 4};
+    for (int i = 0; i < 11; i++) {
       if (!by[i]) continue;
       void *p = nullptr;
       if (hipMalloc(&p, by[i]) != hipSuccess) { release(); return batch_fail(b, SL_ENOMEM, "batch autotune: snapshot"); }

@@ -371,8 +371,10 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
   return vm;
 }
 // vmax: atomicMax on the bit pattern of a non-negative float (31-wgsl-sim2.js:233), reduced over
-// the workgroup first so that one atomic per workgroup reaches the counter. MINW is the occupancy
-// target handed to the register allocator (waves per SIMD); the variants compute identical results.
+// the workgroup first; the workgroup's atomic goes to its slot (workgroup % SL_VSLOTS), which COUNT
+// folds into the counter: tens of thousands of workgroups on one address serialise there. MINW is
+// the occupancy target handed to the register allocator (waves per SIMD); the variants compute
+// identical results.
 template <int MINW>
 __global__ __launch_bounds__(SL_WG, MINW) void k_g2p(SlWork W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
@@ -386,7 +388,7 @@ __global__ __launch_bounds__(SL_WG, MINW) void k_g2p(SlWork W) {
   if (threadIdx.x == 0) {
     uint32_t m = red[0];
     for (int w = 1; w < SL_WG / 64; w++) m = max(m, red[w]);
-    if (m) atomicMax(&T.C->vmax, m);
+    if (m) atomicMax(&T.vslot[(blockIdx.x % SL_VSLOTS) * SL_VSLOT_STRIDE], m);
   }
 }
 void sl_k_g2p(const SlLaunch &L, int variant) {
@@ -482,17 +484,27 @@ void sl_k_emit(const SlLaunch &L, uint32_t reserve) {
 
 // ------------------------------------------------------------------------------------------------
 // COUNT, 31-wgsl-sim2.js:332-349 (the indirect arguments are not needed: kernels read the counters)
-// mode 1: count = newCount; 2: count = min(count + y, cap); 3: wwCount = min(wwNew, wcap);
-// 4: clear; 5: vmax = 0 (the writeBuffer of 50-sim.js:306); 6: flip the particle buffer;
-// 7: count += the emitted particles that fit below cap - 64 (a batch's emission, whose budget is
-// applied on the device count). One thread per tank (per listed slot when `list` is given).
+// mode 0: only fold the G2P speed slots into vmax; 1: count = newCount (and fold); 2: count =
+// min(count + y, cap); 3: wwCount = min(wwNew, wcap); 4: clear; 5: vmax = 0 and the slots cleared
+// (the writeBuffer of 50-sim.js:306); 6: flip the particle buffer; 7: count += the emitted
+// particles that fit below cap - 64 (a batch's emission, whose budget is applied on the device
+// count). One thread per tank (per listed slot when `list` is given).
 // ------------------------------------------------------------------------------------------------
+__device__ __forceinline__ void vmax_fold(const SlTank &T, bool clear) {
+  uint32_t v = T.C->vmax, q[SL_VSLOTS];
+#pragma unroll
+  for (uint32_t k = 0; k < SL_VSLOTS; k++) q[k] = T.vslot[k * SL_VSLOT_STRIDE];
+#pragma unroll
+  for (uint32_t k = 0; k < SL_VSLOTS; k++) { v = max(v, q[k]); T.vslot[k * SL_VSLOT_STRIDE] = 0u; }
+  T.C->vmax = clear ? 0u : v;
+}
 __global__ void k_count(SlTank *tanks, const uint32_t *list, int ntanks, uint32_t mode, uint32_t y) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= ntanks) return;
   SlTank &T = tanks[list ? list[i] : i];
   SlCounters *C = T.C;
   uint32_t cap = T.cap, wcap = f2u(T.ww_cap_f);
+  if (mode == 0 || mode == 1 || mode == 5) vmax_fold(T, mode == 5);
   if (mode == 1) {  // compaction is in place: the buffer stays current
     C->count = C->newCount;
     C->newCount = 0;
@@ -500,7 +512,6 @@ __global__ void k_count(SlTank *tanks, const uint32_t *list, int ntanks, uint32_
   if (mode == 2) { uint32_t yy = y ? y : T.emit_total; uint32_t s = C->count + yy; C->count = s < cap ? s : cap; }
   if (mode == 3) { C->wwCount = C->wwNew < wcap ? C->wwNew : wcap; C->wwNew = 0; T.ww_cur = 1 - T.ww_cur; }
   if (mode == 4) { C->count = 0; C->newCount = 0; C->wwCount = 0; C->wwNew = 0; C->impacts = 0; }
-  if (mode == 5) C->vmax = 0;
   if (mode == 6) T.cur = 1 - T.cur;
   if (mode == 7 && T.emit_total) {
     const uint32_t lim = cap > 64u ? cap - 64u : 0u, c = C->count;

@@ -35,6 +35,11 @@
 #define SF_SLOW (1u << 13)  // an open face has 0 < f < 1: the kernel re-reads the fractions
 
 struct SlCounters { uint32_t count, newCount, wwCount, wwNew, vmax, impacts, pad0, pad1; };
+// G2P's speed maxima go to SL_VSLOTS slots SL_VSLOT_STRIDE words apart (workgroup b to slot
+// b % SL_VSLOTS), so that the per-workgroup atomics of one launch spread over many addresses; COUNT
+// folds the slots into SlCounters::vmax
+#define SL_VSLOTS 64u
+#define SL_VSLOT_STRIDE 64u
 struct SlObj { float r0[4], r1[4], r2[4], pos[4], dims[4], vel[4], angv[4]; };
 struct SlEmitDev { float a[4], b[4], c[4], d[4]; uint32_t info[4]; };
 
@@ -68,6 +73,7 @@ struct SlTank {
   uint8_t *pf[2];
   float4 *wp[2], *wv[2];
   SlCounters *C;
+  uint32_t *vslot;  // SL_VSLOTS speed-maximum slots (bit patterns), folded into C->vmax by COUNT
   uint32_t *args;  // 16 u32 dispatch / draw-indirect arguments (COUNT)
   // grid
   int32_t *acc;
