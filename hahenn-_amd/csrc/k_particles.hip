@@ -26,7 +26,7 @@ __device__ __forceinline__ float2 ld_pair(const float *base, uint32_t n) {
 // grid or masked out are then dropped by select, so the sums take the reference's terms in the
 // reference's order.
 __device__ __forceinline__ f3 transfer1(const SlTank &T, f3 q, int comp, float vp) {
-  const int32_t bx = f2i(floorf(q.x)), by = f2i(floorf(q.y)), bz = f2i(floorf(q.z));
+  const int32_t bx = f2i_hw(floorf(q.x)), by = f2i_hw(floorf(q.y)), bz = f2i_hw(floorf(q.z));
   const float fx = q.x - (float)bx, fy = q.y - (float)by, fz = q.z - (float)bz;
   bool ix[2], iy[2], iz[2];
   int32_t cy[2], cz[2];
@@ -302,7 +302,7 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
   if (in_drain(T, x)) alive = false;
   if (!in_footprint(T, x) && x.y < 0.002f) alive = false;
   if (fabsf(x.x) > 20.0f || fabsf(x.z) > 20.0f || x.y < -1.0f || x.y > 30.0f) alive = false;
-  const i3 c0 = toi3(g);
+  const i3 c0 = toi3_hw(g);
   bool airborne = (pf & 2u) != 0u;
   if (ing) {
     uint32_t n0 = count_at(T, c0.x, c0.y, c0.z);
@@ -327,7 +327,7 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
     }
   }
   const f3 g1 = grid_pos(T, x);
-  const i3 c1 = toi3(g1);
+  const i3 c1 = toi3_hw(g1);
   if (airborne && v0.y < -0.3f && in_grid(T, g1)) {
     uint32_t others = count_at(T, c1.x, c1.y, c1.z) - ((c1.x == c0.x && c1.y == c0.y && c1.z == c0.z) ? 1u : 0u);
     if (others >= 3u) {

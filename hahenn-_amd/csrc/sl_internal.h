@@ -144,6 +144,15 @@ __device__ __forceinline__ uint32_t f2u(float f) {
   if (f >= 4294967296.0f) return UINT32_MAX;
   return (uint32_t)f;
 }
+// f2i by the hardware conversion: v_cvt_i32_f32 truncates toward zero, saturates values beyond the
+// int32 range (infinities included) and maps NaN to 0 -- f2i's cases, in one instruction instead of
+// the compiler's three branches (checked on gfx950 over 4 Mi random bit patterns and the edge cases)
+__device__ __forceinline__ int32_t f2i_hw(float f) {
+  int32_t r;
+  asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(f));
+  return r;
+}
+__device__ __forceinline__ i3 toi3_hw(f3 g) { i3 r; r.x = f2i_hw(g.x); r.y = f2i_hw(g.y); r.z = f2i_hw(g.z); return r; }
 __device__ __forceinline__ int32_t wadd(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
 __device__ __forceinline__ i3 toi3(f3 g) { i3 r; r.x = f2i(g.x); r.y = f2i(g.y); r.z = f2i(g.z); return r; }
 __device__ __forceinline__ uint32_t hashU(uint32_t x) {
@@ -307,9 +316,12 @@ __device__ __forceinline__ f3 solid_vel_at(const SlTank &T, f3 p) {
 }
 // The host's cull radius of object o (row pad r0.w, see cull_radius in capi.hip) is farther than
 // `reach` from p, with slack for the rotation rows' rounding: then o's signed distance at p exceeds
-// `reach` (for a drain, p is outside its cylinder).
+// `reach` (for a drain, p is outside its cylinder). Compared squared: |d|^2 > c^2 in f32 gives
+// |d| > c (1 - 2^-21), far inside the 2e-4 slack; a negative c (radius -inf) culls everything.
 __device__ __forceinline__ bool beyond_cull(const SlObj &o, f3 p, float reach) {
-  return len3(sub3(p, obj_pos(o))) > (o.r0[3] + reach) * 1.0002f + 1e-4f;
+  const f3 d = sub3(p, obj_pos(o));
+  const float c = (o.r0[3] + reach) * 1.0002f + 1e-4f;
+  return c < 0.0f || d.x * d.x + d.y * d.y + d.z * d.z > c * c;
 }
 // some obstacle could be within `reach` of p (otherwise scene_sdf(p) > reach)
 __device__ __forceinline__ bool obstacle_within(const SlTank &T, f3 p, float reach) {
