@@ -11,8 +11,8 @@ A step = one FluidSim.step (emission, P2G, normalise+forces, divergence, 60 SOR 
 projection, G2P, compaction, whitewater, ripple frame). Inputs are resident in HBM before timing.
 Prints one JSON line: particle-steps/s, the SOR kernel's HBM roofline and a CPU baseline (the fp32
 C restatement of the reference, OpenMP on the host cores, on a bounded sample of the same workload).
-Beside the headline: the projection residual the reference's 60 sweeps leave, the same workload with
-the converged pressure solve (multigrid-preconditioned CG to a residual tolerance), the batched tanks
+Beside the headline: the projection residual the reference's 60 sweeps leave, the same workload over
+its stated 1000 steps (timed in windows of 200), the same workload with the converged pressure solve (multigrid-preconditioned CG to a residual tolerance), the batched tanks
 and the serving loop.
 """
 import argparse
@@ -219,6 +219,34 @@ def projection_residual(g, dt):
     out["f64"]["above_f32_floor"] = stats(np.maximum(d - fl, 0.0))
     out["f64"]["f32_floor_max"] = float(fl.max()) if fl.size else 0.0
     return out
+
+
+def flow_leg(wl, steps=1000, window=200, warmup=5):
+    """The faucet over its stated length (1000 steps after `warmup` untimed ones and the tuner), timed
+    in windows of `window` steps: the step time grows as the stream fills the tank and spreads."""
+    g = build_sim(wl)
+    for _ in range(warmup):
+        step(g, wl)
+    g.autotune(reps=2)
+    g.sync()
+    wins = []
+    t_all = time.perf_counter()
+    for w0 in range(0, steps, window):
+        n0 = g.stats()["count"]
+        t0 = time.perf_counter()
+        for _ in range(min(window, steps - w0)):
+            step(g, wl)
+        g.sync()
+        dt = time.perf_counter() - t0
+        k = min(window, steps - w0)
+        n1 = g.stats()["count"]
+        wins.append({"steps": [w0 + 1, w0 + k], "steps_per_s": k / dt, "particles": [n0, n1],
+                     "particle_steps_per_s": (n0 + n1) / 2 * k / dt})
+    total = time.perf_counter() - t_all
+    st = g.stats()
+    g.close()
+    return {"steps": steps, "warmup": warmup, "seconds": total, "steps_per_s": steps / total,
+            "gpu_errors": st["gpuErrors"], "windows": wins}
 
 
 def converged_leg(wl, steps, warmup, tol, max_iters):
@@ -496,7 +524,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-ripple", action="store_true")
     ap.add_argument("--no-pmc", action="store_true", help="skip the rocprofv3 counter runs for `traffic`")
-    ap.add_argument("--only", choices=["all", "headline", "batched", "serve", "converged", "droplets"], default="all",
+    ap.add_argument("--only", choices=["all", "headline", "batched", "serve", "converged", "droplets", "flow"], default="all",
                     help="run one leg (its JSON object alone) instead of the full line")
     ap.add_argument("--conv-tol", type=float, default=1e-4, help="converged leg: residual tolerance (1/s)")
     ap.add_argument("--conv-iters", type=int, default=24, help="converged leg: CG iterations enqueued per solve")
@@ -546,6 +574,9 @@ def main():
         return
     grid = tuple(int(x) for x in args.grid.split("x"))
     wl = workloads.faucet(grid=grid, R=0 if args.no_ripple else 512)
+    if args.only == "flow":
+        print(json.dumps(flow_leg(wl)))
+        return
     if args.only == "converged":
         print(json.dumps(converged_leg(wl, args.conv_steps, args.warmup, args.conv_tol, args.conv_iters)))
         return
@@ -636,6 +667,8 @@ def main():
     g.close()
     progress(f"headline done: {out['steps_per_s']:.1f} steps/s")
     if args.only == "all":
+        out["faucet_1000_steps"] = flow_leg(wl)
+        progress("1000-step faucet done")
         out["converged"] = converged_leg(wl, args.conv_steps, args.warmup, args.conv_tol, args.conv_iters)
         progress("converged solve done")
         out["ripple_droplets"] = droplet_leg(args.drop_steps, args.warmup)
