@@ -395,7 +395,6 @@ static int tank_buffers(sl_sim *s, SlTank &T, Take take) {
   TAKE(T.acc, N * 24);
   TAKE(T.cellCount, C * 4);
   TAKE(T.vel, N * 12); TAKE(T.velOld, N * 12);
-  TAKE(T.vmask, N + 8);  // + 8: G2P reads mask bytes as 8-byte words from their 4-byte-aligned base
   TAKE(T.frac, N * 12); TAKE(T.svel, N * 12);
   TAKE(T.divg, C * 4);
   {
@@ -590,6 +589,8 @@ extern "C" int sl_grid_info(const sl_sim *s, sl_grid_t *o) {
 // objects, 50-sim.js:194-210
 // ------------------------------------------------------------------------------------------------
 static bool same_f(float a, float b) { return a == b || (a != a && b != b); }
+static uint32_t f32_bits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+static float bits_f32(uint32_t u) { float f; memcpy(&f, &u, 4); return f; }
 // Radius of a sphere about an object's position that contains its solid, from the shape table of
 // shape_sdf (sl_internal.h), stored in the unused row pad r0.w of the device copy. For the shapes
 // below the signed distance is at least the distance to that sphere (the exact distance of a solid
@@ -1293,11 +1294,19 @@ extern "C" int sl_readback(sl_sim *s, int f, void *dst, size_t bytes) {
     case SL_F_VEL: case SL_F_VELOLD: case SL_F_FRAC: case SL_F_SVEL: {
       const float *src = f == SL_F_VEL ? T.vel : f == SL_F_VELOLD ? T.velOld : f == SL_F_FRAC ? T.frac : T.svel;
       std::vector<float> a(N * 3);
-      std::vector<uint8_t> m(N, 0);
       if ((r = d2h(s, a.data(), src, N * 12))) return r;
-      if (f == SL_F_VELOLD && (r = d2h(s, m.data(), T.vmask, N))) return r;
       for (size_t n = 0; n < N; n++) {
-        o[n * 4] = a[n]; o[n * 4 + 1] = a[N + n]; o[n * 4 + 2] = a[2 * N + n]; o[n * 4 + 3] = (float)m[n];
+        uint32_t m = 0;
+        for (int c = 0; c < 3; c++) {
+          float v = a[c * N + n];
+          if (f == SL_F_VELOLD) {  // the mask bits out of velOld (which reads 0 where the mask is off)
+            const bool on = f32_bits(v) != SL_MASKED_OUT;
+            m |= (on ? 1u : 0u) << c;
+            if (!on) v = 0.0f;
+          }
+          o[n * 4 + c] = v;
+        }
+        o[n * 4 + 3] = (float)m;
       }
       return SL_OK;
     }
@@ -1383,14 +1392,19 @@ extern "C" int sl_upload(sl_sim *s, int f, const void *src, size_t bytes) {
     case SL_F_VEL: case SL_F_VELOLD: case SL_F_FRAC: case SL_F_SVEL: {
       float *d = f == SL_F_VEL ? T.vel : f == SL_F_VELOLD ? T.velOld : f == SL_F_FRAC ? T.frac : T.svel;
       std::vector<float> a(N * 3);
-      std::vector<uint8_t> m(N);
       for (size_t n = 0; n < N; n++) {
         a[n] = in[n * 4]; a[N + n] = in[n * 4 + 1]; a[2 * N + n] = in[n * 4 + 2];
-        float w = in[n * 4 + 3];
-        m[n] = (uint8_t)(w > 0 ? (uint32_t)w & 7u : 0u);
+        if (f != SL_F_VELOLD) continue;
+        // velOld: the mask bits (w: bit c for component c) go in as the masked-out pattern
+        const float w = in[n * 4 + 3];
+        const uint32_t m = w > 0 ? (uint32_t)w & 7u : 0u;
+        for (int c = 0; c < 3; c++) {
+          float &v = a[c * N + n];
+          if (!((m >> c) & 1u)) v = bits_f32(SL_MASKED_OUT);
+          else if (f32_bits(v) == SL_MASKED_OUT) v = bits_f32(0x7fc00000u);  // a NaN, not a mask
+        }
       }
       if ((r = h2d(s, d, a.data(), N * 12))) return r;
-      if (f == SL_F_VELOLD) return h2d(s, T.vmask, m.data(), N);
       if (f == SL_F_FRAC || f == SL_F_SVEL) s->geom_dirty = false;  // explicit geometry replaces GEOM
       return SL_OK;
     }

@@ -610,18 +610,17 @@ __global__ __launch_bounds__(SL_WG) void k_normforce(SlWork W) {
   if (n >= T.nodes) return;
   const size_t N = T.nodes;
   float v[3] = {0.0f, 0.0f, 0.0f};
-  uint32_t m = 0;
   unsigned long long *acc = (unsigned long long *)T.acc;
 #pragma unroll
   for (int c = 0; c < 3; c++) {
     unsigned long long a = acc[c * N + n];
     int32_t mo = (int32_t)(uint32_t)(a >> 32), wi = (int32_t)(uint32_t)a;
     float wt = (float)wi;
-    if (wt > 1.0f) { v[c] = (float)mo / wt; m |= 1u << c; }
-    T.velOld[c * N + n] = v[c];
+    float old = __uint_as_float(SL_MASKED_OUT);  // velOld: 0 and masked out, kept as the mask pattern
+    if (wt > 1.0f) { v[c] = (float)mo / wt; old = v[c]; }
+    T.velOld[c * N + n] = old;
     acc[c * N + n] = 0ull;
   }
-  T.vmask[n] = (uint8_t)m;
   v[1] -= T.gdt;
 #pragma unroll
   for (int c = 0; c < 3; c++) {

@@ -19,56 +19,60 @@ __device__ __forceinline__ float2 ld_pair(const float *base, uint32_t n) {
   __builtin_memcpy(&r, __builtin_assume_aligned((const char *)base + n * 4u, 4), 8);
   return r;
 }
-// masked PIC/FLIP gather, 31-wgsl-sim2.js:9-36 -> (PIC, FLIP, found). The mask bytes and both
-// velocities of all 8 corners are loaded up front: per row (y, z clamped into the grid) the two
-// x-corners are the pair of nodes at b0 = clamp(bx, 0, nx - 1), so the corner at i = nx is the
-// pair's second node and the corner at i = 0 reached from bx = -1 its first; corners outside the
-// grid or masked out are then dropped by select, so the sums take the reference's terms in the
-// reference's order.
-__device__ __forceinline__ f3 transfer1(const SlTank &T, f3 q, int comp, float vp) {
+// masked PIC/FLIP gather, 31-wgsl-sim2.js:9-36, in two parts so that G2P can issue the loads of all
+// three components before it sums any: tgather finds the 8 corners of q on component `comp`'s grid
+// and loads vel and velOld (whose bit pattern carries the mask) of each row (y, z clamped into the
+// grid) as the node pair at b0 = clamp(bx, 0, nx - 1), whose second node is the corner at i = nx and
+// whose first is the corner at i = 0 reached from bx = -1; tsum drops the corners outside the grid
+// or masked out by select, so the sums take the reference's terms in the reference's order, and
+// returns (PIC, FLIP, found).
+struct TGather {
+  float fx, fy, fz;
+  bool ix0, ix1, iy0, iy1, iz0, iz1, hi0, lo1;
+  float2 a[4], b[4];  // rows (y0,z0), (y1,z0), (y0,z1), (y1,z1): vel and velOld of nodes b0, b0 + 1
+};
+__device__ __forceinline__ void tgather(const SlTank &T, f3 q, int comp, TGather &G) {
   const int32_t bx = f2i_hw(floorf(q.x)), by = f2i_hw(floorf(q.y)), bz = f2i_hw(floorf(q.z));
-  const float fx = q.x - (float)bx, fy = q.y - (float)by, fz = q.z - (float)bz;
-  bool ix[2], iy[2], iz[2];
-  int32_t cy[2], cz[2];
-#pragma unroll
-  for (int d = 0; d < 2; d++) {
-    const int32_t i = bx + d, j = by + d, k = bz + d;
-    ix[d] = i >= 0 && i <= T.nx;
-    iy[d] = j >= 0 && j <= T.ny; cy[d] = min(max(j, 0), T.ny);
-    iz[d] = k >= 0 && k <= T.nz; cz[d] = min(max(k, 0), T.nz);
-  }
+  G.fx = q.x - (float)bx; G.fy = q.y - (float)by; G.fz = q.z - (float)bz;
+  G.ix0 = bx >= 0 && bx <= T.nx; G.ix1 = bx + 1 >= 0 && bx + 1 <= T.nx;
+  G.iy0 = by >= 0 && by <= T.ny; G.iy1 = by + 1 >= 0 && by + 1 <= T.ny;
+  G.iz0 = bz >= 0 && bz <= T.nz; G.iz1 = bz + 1 >= 0 && bz + 1 <= T.nz;
+  const int32_t cy[2] = {min(max(by, 0), T.ny), min(max(by + 1, 0), T.ny)};
+  const int32_t cz[2] = {min(max(bz, 0), T.nz), min(max(bz + 1, 0), T.nz)};
   const int32_t b0 = min(max(bx, 0), T.nx - 1);
-  const bool hi0 = bx == T.nx, lo1 = bx == -1;
+  G.hi0 = bx == T.nx; G.lo1 = bx == -1;
   const float *vn = T.vel + comp * (size_t)T.nodes, *vo = T.velOld + comp * (size_t)T.nodes;
-  uint32_t m[8];
-  float a[8], b[8];
 #pragma unroll
   for (int r = 0; r < 4; r++) {
     const uint32_t n = node_idx(T, b0, cy[r & 1], cz[r >> 1]);
-    // mask bytes n and n + 1 from the 8 bytes at the 4-byte-aligned base below n (the mask array
-    // has 8 bytes of padding, so the word stays inside it)
-    uint2 mw;
-    __builtin_memcpy(&mw, __builtin_assume_aligned(T.vmask + (n & ~3u), 4), 8);
-    const uint64_t mq = ((uint64_t)mw.y << 32 | mw.x) >> (8u * (n & 3u));
-    const uint32_t m0 = (uint32_t)mq & 0xffu, m1 = (uint32_t)(mq >> 8) & 0xffu;
-    const float2 pa = ld_pair(vn, n), pb = ld_pair(vo, n);
-    m[2 * r] = hi0 ? m1 : m0; m[2 * r + 1] = lo1 ? m0 : m1;
-    a[2 * r] = hi0 ? pa.y : pa.x; a[2 * r + 1] = lo1 ? pa.x : pa.y;
-    b[2 * r] = hi0 ? pb.y : pb.x; b[2 * r + 1] = lo1 ? pb.x : pb.y;
+    G.a[r] = ld_pair(vn, n);
+    G.b[r] = ld_pair(vo, n);
   }
-  const uint32_t bit = 1u << comp;
+}
+__device__ __forceinline__ f3 tsum(const TGather &G, float vp) {
+  bool mk[8];
+  float a[8], b[8];
+#pragma unroll
+  for (int r = 0; r < 4; r++) {
+    const float2 pa = G.a[r], pb = G.b[r];
+    a[2 * r] = G.hi0 ? pa.y : pa.x; b[2 * r] = G.hi0 ? pb.y : pb.x;
+    a[2 * r + 1] = G.lo1 ? pa.x : pa.y; b[2 * r + 1] = G.lo1 ? pb.x : pb.y;
+    mk[2 * r] = __float_as_uint(b[2 * r]) != SL_MASKED_OUT;
+    mk[2 * r + 1] = __float_as_uint(b[2 * r + 1]) != SL_MASKED_OUT;
+  }
+  const bool ix[2] = {G.ix0, G.ix1}, iy[2] = {G.iy0, G.iy1}, iz[2] = {G.iz0, G.iz1};
   float sn = 0.0f, so = 0.0f, sw = 0.0f;
 #pragma unroll
   for (int dz = 0; dz < 2; dz++) {
-    const float wz = dz == 1 ? fz : 1.0f - fz;
+    const float wz = dz == 1 ? G.fz : 1.0f - G.fz;
 #pragma unroll
     for (int dy = 0; dy < 2; dy++) {
-      const float wy = dy == 1 ? fy : 1.0f - fy;
+      const float wy = dy == 1 ? G.fy : 1.0f - G.fy;
 #pragma unroll
       for (int dx = 0; dx < 2; dx++) {
         const int c = dz * 4 + dy * 2 + dx;
-        const bool on = iz[dz] && iy[dy] && ix[dx] && (m[c] & bit) != 0u;
-        const float w = (dx == 1 ? fx : 1.0f - fx) * wy * wz;
+        const bool on = iz[dz] && iy[dy] && ix[dx] && mk[c];
+        const float w = (dx == 1 ? G.fx : 1.0f - G.fx) * wy * wz;
         sn = on ? sn + w * a[c] : sn;
         so = on ? so + w * b[c] : so;
         sw = on ? sw + w : sw;
@@ -77,6 +81,11 @@ __device__ __forceinline__ f3 transfer1(const SlTank &T, f3 q, int comp, float v
   }
   if (sw < 1e-6f) return F3(vp, vp, 0.0f);
   return F3(sn / sw, vp + (sn - so) / sw, 1.0f);
+}
+__device__ __forceinline__ f3 transfer1(const SlTank &T, f3 q, int comp, float vp) {
+  TGather G;
+  tgather(T, q, comp, G);
+  return tsum(G, vp);
 }
 // unmasked clamped sample, 31-wgsl-sim2.js:39-59
 __device__ __forceinline__ float sample1(const SlTank &T, f3 q, int comp) {
@@ -278,9 +287,11 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
   const f3 g = grid_pos(T, x0);
   const bool ing = in_grid(T, g);
   if (ing) {
-    f3 tx = transfer1(T, sub3(g, F3(0.0f, 0.5f, 0.5f)), 0, v.x);
-    f3 ty = transfer1(T, sub3(g, F3(0.5f, 0.0f, 0.5f)), 1, v.y);
-    f3 tz = transfer1(T, sub3(g, F3(0.5f, 0.5f, 0.0f)), 2, v.z);
+    TGather gx, gy, gz;
+    tgather(T, sub3(g, F3(0.0f, 0.5f, 0.5f)), 0, gx);
+    tgather(T, sub3(g, F3(0.5f, 0.0f, 0.5f)), 1, gy);
+    tgather(T, sub3(g, F3(0.5f, 0.5f, 0.0f)), 2, gz);
+    const f3 tx = tsum(gx, v.x), ty = tsum(gy, v.y), tz = tsum(gz, v.z);
     v = F3(mixf(tx.x, tx.y, T.flip), mixf(ty.x, ty.y, T.flip), mixf(tz.x, tz.y, T.flip));
   } else {
     v.y -= T.g * dt;

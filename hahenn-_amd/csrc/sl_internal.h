@@ -4,8 +4,8 @@
 //   particles : px,py,pz,vx,vy,vz (f32) + pf (u8: bit0 alive, bit1 free-flight mark), ping-pong x2
 //   whitewater: wp (float4 pos.xyz, life), wv (float4 vel.xyz, kind), ping-pong x2
 //   MAC nodes : N = (nx+1)(ny+1)(nz+1), x-fastest; per component planes of N floats:
-//               acc (3N u64: momentum sum << 32 | weight sum, per axis), vel (3N), velOld (3N) + vmask (N u8),
-//               frac (3N), svel (3N)
+//               acc (3N u64: momentum sum << 32 | weight sum, per axis), vel (3N), velOld (3N, the P2G
+//               mask as a bit pattern), frac (3N), svel (3N)
 //   cells     : C = nx*ny*nz, x-fastest: cellCount (u32), divg (f32)
 //   SOR arrays: red/black split ("packed"): cell (i,j,k) has colour (i+j+k)&1 and slot
 //               (i>>1) + hw*(j + ny*k) with hw = (nx+1)/2, in array `colour`; pres, rhs (f32) and
@@ -39,6 +39,9 @@ struct SlCounters { uint32_t count, newCount, wwCount, wwNew, vmax, impacts, pad
 // b % SL_VSLOTS), so that the per-workgroup atomics of one launch spread over many addresses; COUNT
 // folds the slots into SlCounters::vmax
 #define SL_VSLOTS 64u
+// a signalling-NaN pattern that no float operation produces (their NaNs are quiet): velOld at a node
+// without weight
+#define SL_MASKED_OUT 0x7fa5a5a5u
 #define SL_VSLOT_STRIDE 64u
 struct SlObj { float r0[4], r1[4], r2[4], pos[4], dims[4], vel[4], angv[4]; };
 struct SlEmitDev { float a[4], b[4], c[4], d[4]; uint32_t info[4]; };
@@ -78,8 +81,9 @@ struct SlTank {
   // grid
   int32_t *acc;
   uint32_t *cellCount;
+  // velOld carries P2G's mask: bits SL_MASKED_OUT where the node had no weight (NORMALIZE sets velOld
+  // to 0 there and transfer1 skips the node), so G2P takes a corner's mask from its velOld load
   float *vel, *velOld, *frac, *svel;
-  uint8_t *vmask;
   float *divg, *pres, *rhs;
   uint16_t *sflag;
   SlObj *objs;

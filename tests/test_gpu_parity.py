@@ -167,6 +167,53 @@ def test_stage_by_stage_from_identical_state():
     assert np.array_equal(gp[:, 3], op[:, 3])
 
 
+def test_g2p_from_uploaded_grid():
+    """G2P from grid velocities uploaded through the C ABI (vel, and velOld with its mask in the 4th
+    column). The GPU keeps the mask in velOld's place as a bit pattern, so a masked-out node's velOld
+    is dropped (the reference's transfer1 never reads it, 31-wgsl-sim2.js:18-19; NORMALIZE leaves it
+    0) and reads back as 0. Values planted there must not change the particles."""
+    o, g = twin.make_twin(twin.small_scene())
+    dt = 1 / 120
+    for _ in range(5):
+        twin.oracle_step(o, dt)
+    o.L.or_step_begin(o.ptr)
+    o.set("frame", o.info()["frame"] + 1)
+    o.write_params(dt)
+    o.field("acc")[:] = 0
+    o.field("cellCount")[:] = 0
+    for k in ("p2g", "normalize", "forces", "divergence"):
+        o.kernel(k)
+    for _ in range(int(o.info()["iters"])):
+        o.kernel("pressure", 0)
+        o.kernel("pressure", 1)
+    o.kernel("project")
+    vo = o.field("velOld")
+    mask = vo[:, 3].astype(np.uint32)
+    off = [(mask >> c) & 1 == 0 for c in range(3)]
+    assert all(o_.any() for o_ in off) and all((~o_).any() for o_ in off)
+    rng = np.random.default_rng(5)
+    for c in range(3):   # planted at masked-out nodes: transfer1 skips them
+        vo[off[c], c] = rng.uniform(-3, 3, int(off[c].sum())).astype(np.float32)
+    n = int(o.field("counters")[0])
+    g.upload(suiri.F_PARTICLES, o.field("parts"))
+    g.upload(suiri.F_COUNTERS, o.field("counters"))
+    g.upload(suiri.F_CELLCOUNT, o.field("cellCount"))
+    g.upload(suiri.F_VELOLD, vo)
+    g.upload(suiri.F_VEL, o.field("vel"))
+    g.set_scalar("frame", o.info()["frame"])
+    g.run_stage(suiri.S_GEOM, 0, dt)
+    want_vo = vo.copy()
+    for c in range(3):
+        want_vo[off[c], c] = 0
+    assert np.array_equal(g.readback(suiri.F_VELOLD), want_vo)
+    assert np.array_equal(g.readback(suiri.F_VEL)[:, :3], o.field("vel")[:, :3])
+    print(f"G2P from uploaded grid: {n} particles", flush=True)
+    o.kernel("g2p")
+    g.run_stage(suiri.S_G2P, 0, dt)
+    gp, op = g.readback(suiri.F_PARTICLES)[:n], o.field("parts")[:n]
+    assert np.array_equal(gp.view(np.uint32), op.view(np.uint32))
+
+
 def test_reorder_is_bitwise():
     """The spatial re-ordering (stable sort by cell index) produces the oracle's permutation."""
     o, g = twin.make_twin(twin.small_scene())
