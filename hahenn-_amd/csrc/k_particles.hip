@@ -286,6 +286,9 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
   f3 v = v0;
   const f3 g = grid_pos(T, x0);
   const bool ing = in_grid(T, g);
+  // the particle's own cell count goes out with the gathers (the free-flight test below reads it)
+  const i3 c0 = toi3_hw(g);
+  const uint32_t n0 = ing ? count_at(T, c0.x, c0.y, c0.z) : 0u;
   if (ing) {
     TGather gx, gy, gz;
     tgather(T, sub3(g, F3(0.0f, 0.5f, 0.5f)), 0, gx);
@@ -313,10 +316,8 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
   if (in_drain(T, x)) alive = false;
   if (!in_footprint(T, x) && x.y < 0.002f) alive = false;
   if (fabsf(x.x) > 20.0f || fabsf(x.z) > 20.0f || x.y < -1.0f || x.y > 30.0f) alive = false;
-  const i3 c0 = toi3_hw(g);
   bool airborne = (pf & 2u) != 0u;
   if (ing) {
-    uint32_t n0 = count_at(T, c0.x, c0.y, c0.z);
     if (n0 >= 4u) airborne = false;
     else if (n0 <= 2u && wadd(c0.y, 2) < T.ny) {
       // 3x3 columns of rows y-1 .. y+2 (rows y+1 and y+2 lie in the grid here, row y-1 when y > 0):
