@@ -15,6 +15,10 @@
 
 #define FFT_LDS_COMPLEX 4096  // complex values per LDS buffer (two buffers = 64 KiB)
 #define FFT_LOADS (FFT_LDS_COMPLEX / 256)  // loads per thread of a 256-thread FFT workgroup, at most
+// the column kernel's workgroup: 1024 threads, so that its two workgroups per CU (72 KB of LDS each)
+// keep 8 waves per SIMD (256 threads: ripple frame 0.118 ms, 1024: 0.107 ms at M = 1024)
+#define FFT_COL_THREADS 1024
+#define FFT_COL_LOADS (FFT_LDS_COMPLEX / FFT_COL_THREADS)
 
 enum { IN_BUF = 0, IN_SRC = 1, IN_MASK = 2 };
 enum { OUT_BUF = 0, OUT_SCALE = 1 };
@@ -124,7 +128,8 @@ __global__ __launch_bounds__(256) void k_fft_rows(SlWork W, int which_in, int wh
 }
 
 // all vertical passes of a tile of CW = 4096/M adjacent columns
-__global__ __launch_bounds__(256) void k_fft_cols(SlWork W, int which_in, int which_out, int omode, float sign) {
+__global__ __launch_bounds__(FFT_COL_THREADS) void k_fft_cols(SlWork W, int which_in, int which_out, int omode,
+                                                              float sign) {
   // adjacent column tiles share the 128-byte lines of every row: XCD-aware order keeps them in one L2
   SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
   const SlTank &T = W.tanks[t];
@@ -141,23 +146,23 @@ __global__ __launch_bounds__(256) void k_fft_cols(SlWork W, int which_in, int wh
   float2 *out = obufs[which_out];
   const uint32_t n = CW * M;
   // LDS layout: column c at offset c, row y at y*CW (stride CW); every load of the thread is issued
-  // before the first LDS write (n <= FFT_LDS_COMPLEX = 16 * 256)
+  // before the first LDS write (n <= FFT_LDS_COMPLEX = FFT_COL_LOADS * FFT_COL_THREADS)
   {
-    float2 v[FFT_LOADS];
+    float2 v[FFT_COL_LOADS];
 #pragma unroll
-    for (int k = 0; k < FFT_LOADS; k++) {
-      const uint32_t q = threadIdx.x + 256u * k;
+    for (int k = 0; k < FFT_COL_LOADS; k++) {
+      const uint32_t q = threadIdx.x + FFT_COL_THREADS * k;
       if (q < n) v[k] = in[(q >> lCW) * M + x0 + (q & (CW - 1u))];
     }
-    float2 w[FFT_LOADS];
+    float2 w[FFT_COL_LOADS];
 #pragma unroll
-    for (int k = 0; k < FFT_LOADS; k++) {
-      const uint32_t x = threadIdx.x + 256u * k;
+    for (int k = 0; k < FFT_COL_LOADS; k++) {
+      const uint32_t x = threadIdx.x + FFT_COL_THREADS * k;
       if (x < M) w[k] = T.tw[x];
     }
 #pragma unroll
-    for (int k = 0; k < FFT_LOADS; k++) {
-      const uint32_t q = threadIdx.x + 256u * k;
+    for (int k = 0; k < FFT_COL_LOADS; k++) {
+      const uint32_t q = threadIdx.x + FFT_COL_THREADS * k;
       if (q < n) lds[0][q] = v[k];
       if (q < M) stw[q] = w[k];
     }
@@ -286,19 +291,19 @@ void sl_k_ripple_frame_ex(const SlWork &rows, const SlWork &cols, const SlWork &
   const size_t cl = (size_t)maxM * sizeof(float2);      // column LDS beside the static buffers: the twiddles
   // forward transform of the mirrored source -> hat (64-ripple.js:229-232)
   hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), rl, s, rows, 0, 1, (int)IN_SRC, -1.0f, maxM);
-  hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), cl, s, cols, 1, 3, (int)OUT_BUF, -1.0f);
+  hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(FFT_COL_THREADS), cl, s, cols, 1, 3, (int)OUT_BUF, -1.0f);
   // evolve the modes and add the kick (64-ripple.js:233)
   hipLaunchKernelGGL(k_revolve, dim3(elems.total), dim3(256), 0, s, elems);
   // inverse transform of the state -> b (64-ripple.js:234-235)
   hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), rl, s, rows, 0, 1, (int)IN_BUF, 1.0f, maxM);
-  hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), cl, s, cols, 1, 2, (int)OUT_BUF, 1.0f);
+  hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(FFT_COL_THREADS), cl, s, cols, 1, 2, (int)OUT_BUF, 1.0f);
   // height and slopes (64-ripple.js:236-238)
   hipLaunchKernelGGL(k_rextract, dim3(tex.total), dim3(256), 0, s, tex);
   if (mask) {
     // absorb: mask the spatial field, transform back into the state, rescale (64-ripple.js:239-250)
     hipLaunchKernelGGL(k_rmaskbuild, dim3(tex.total), dim3(256), 0, s, tex);
     hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), rl, s, rows, 2, 1, (int)IN_MASK, -1.0f, maxM);
-    hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(256), cl, s, cols, 1, 0, (int)OUT_SCALE, -1.0f);
+    hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(FFT_COL_THREADS), cl, s, cols, 1, 0, (int)OUT_SCALE, -1.0f);
   }
   hipLaunchKernelGGL(k_clear_rsrc, dim3(tex.total), dim3(256), 0, s, tex);  // 64-ripple.js:252
 }
