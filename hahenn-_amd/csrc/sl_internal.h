@@ -150,7 +150,8 @@ __device__ __forceinline__ uint32_t f2u(float f) {
 }
 // f2i by the hardware conversion: v_cvt_i32_f32 truncates toward zero, saturates values beyond the
 // int32 range (infinities included) and maps NaN to 0 -- f2i's cases, in one instruction instead of
-// the compiler's three branches (checked on gfx950 over 4 Mi random bit patterns and the edge cases)
+// the compiler's three branches (tests/probes/cvt_probe.hip checks it on the GPU over 4 Mi random
+// bit patterns and the edge cases)
 __device__ __forceinline__ int32_t f2i_hw(float f) {
   int32_t r;
   asm("v_cvt_i32_f32 %0, %1" : "=v"(r) : "v"(f));
