@@ -1743,6 +1743,12 @@ struct sl_batch {
   std::vector<hipEvent_t> tev;
   size_t tev_used = 0;
   std::vector<void *> allocs;
+  // the re-orderings due in one frame run as one sort: keys and values (2 x ro_cap each) and the
+  // sort's temporary storage, grown on demand
+  uint32_t *ro_keys = nullptr, *ro_vals = nullptr;
+  void *ro_temp = nullptr;
+  size_t ro_temp_bytes = 0;
+  uint64_t ro_cap = 0;
   std::string err;
 };
 
@@ -1952,6 +1958,9 @@ extern "C" void sl_batch_destroy(sl_batch *b) {
   drop_graph(b);
   for (sl_sim *s : b->sims) if (s) sl_destroy(s);
   for (void *p : b->allocs) (void)hipFree(p);
+  if (b->ro_keys) (void)hipFree(b->ro_keys);
+  if (b->ro_vals) (void)hipFree(b->ro_vals);
+  if (b->ro_temp) (void)hipFree(b->ro_temp);
   ring_free(b->maps);
   ring_free(b->emit);
   if (b->h_counters) (void)hipHostFree(b->h_counters);
@@ -2219,8 +2228,63 @@ static int batch_prepare(sl_batch *b) {
   return e == hipSuccess ? SL_OK : batch_fail(b, SL_EHIP, hipGetErrorString(e));
 }
 
+// The re-orderings due this frame as one stable radix sort: each tank's keys carry its rank in the
+// list above the cell-index bits, so the tanks' particles come out in list order, and each tank's in
+// exactly the order of its own sort (stable, same cell keys, the same sentinel outside the grid). One
+// sort instead of one per tank: a radix sort is a dozen or so launches whatever its length.
+static int batch_reorder(sl_batch *b, const std::vector<uint32_t> &due) {
+  if (due.empty()) return SL_OK;
+  uint64_t ntot = 0;
+  int cbits = 1;
+  for (uint32_t t : due) {
+    ntot += b->sims[t]->p_ub;
+    cbits = std::max(cbits, b->sims[t]->sort_end_bit);
+  }
+  const int rbits = due.size() > 1 ? 32 - __builtin_clz((uint32_t)due.size() - 1u) : 0;
+  if (due.size() == 1 || cbits + rbits > 32 || ntot >= (1ull << 31)) {  // one tank, or keys too wide
+    for (uint32_t t : due) {
+      int r = encode_reorder(b->sims[t]);
+      if (r) return batch_fail(b, r, b->sims[t]->err);
+    }
+    return SL_OK;
+  }
+  const int end_bit = cbits + rbits;
+  if (ntot > b->ro_cap) {  // grown with a quarter of headroom, so that steady frames allocate nothing
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, SL_EHIP, "re-ordering: sync");
+    (void)hipFree(b->ro_keys); (void)hipFree(b->ro_vals); (void)hipFree(b->ro_temp);
+    b->ro_keys = b->ro_vals = nullptr; b->ro_temp = nullptr; b->ro_cap = 0; b->ro_temp_bytes = 0;
+    const uint64_t cap = std::min<uint64_t>(ntot + ntot / 4, (1ull << 31) - 1);
+    const size_t tb = sl_sort_temp_bytes((uint32_t)cap, 32);
+    if (hipMalloc((void **)&b->ro_keys, cap * 8) != hipSuccess || hipMalloc((void **)&b->ro_vals, cap * 8) != hipSuccess ||
+        hipMalloc(&b->ro_temp, tb ? tb : 256) != hipSuccess)
+      return batch_fail(b, SL_ENOMEM, "re-ordering: no device memory for the batch sort");
+    b->ro_cap = cap;
+    b->ro_temp_bytes = tb ? tb : 256;
+  }
+  uint32_t off = 0, rank = 0;
+  for (uint32_t t : due) {
+    sl_sim *s = b->sims[t];
+    sl_k_sort_keys(launch_of(s, 1), s->p_ub, b->ro_keys, b->ro_vals, rank << cbits, off);
+    off += s->p_ub;
+    rank++;
+  }
+  int e = sl_sort_pairs(b->ro_temp, b->ro_temp_bytes, b->ro_keys, b->ro_vals, (uint32_t)ntot, end_bit, b->stream);
+  if (e) return batch_fail(b, SL_EHIP, std::string("re-ordering sort: ") + hipGetErrorString((hipError_t)e));
+  off = 0;
+  for (uint32_t t : due) {
+    sl_sim *s = b->sims[t];
+    sl_k_sort_gather(launch_of(s, 1), s->p_ub, b->ro_vals + ntot, off);
+    sl_k_count(launch_of(s, 1), 6, 0);  // the gathered buffer becomes current
+    s->reorder_clock = 0;
+    off += s->p_ub;
+  }
+  hipError_t le = hipGetLastError();
+  return le == hipSuccess ? SL_OK : batch_fail(b, SL_EHIP, std::string("re-ordering: ") + hipGetErrorString(le));
+}
+
 // host mirror of one frame of every stepped tank, plus each tank's re-ordering cadence
 static int batch_host_frame(sl_batch *b, double dt) {
+  std::vector<uint32_t> due;
   for (uint32_t t = 0; t < b->cap; t++) {
     sl_sim *s = b->sims[t];
     if (s) b->copy_valid[t] = 1;  // the frame's counters copy covers every tank in the table
@@ -2233,12 +2297,9 @@ static int batch_host_frame(sl_batch *b, double dt) {
     s->frames++;
     s->substeps_last = 1;
     s->emitted_at_copy = s->emitted_total;
-    if (s->cfg.reorder_every && ++s->reorder_clock >= s->cfg.reorder_every) {
-      int r = encode_reorder(s);
-      if (r) return batch_fail(b, r, s->err);
-    }
+    if (s->cfg.reorder_every && ++s->reorder_clock >= s->cfg.reorder_every) due.push_back(t);
   }
-  return SL_OK;
+  return batch_reorder(b, due);
 }
 
 // dt > 0: every stepped tank advances by dt; dt = 0: each by its own frame length (sl_batch_set_dt)
@@ -2282,9 +2343,10 @@ extern "C" int sl_batch_step(sl_batch *b, double dt) {
   } else if ((r = batch_encode_frame(b, dt))) {
     return r;
   }
-  if (t1) (void)hipEventRecord(t1, b->stream);
   b->steps++;
-  return batch_host_frame(b, dt);
+  const int hr = batch_host_frame(b, dt);  // enqueues the frame's re-orderings
+  if (t1) (void)hipEventRecord(t1, b->stream);
+  return hr;
 }
 
 // Records one frame of every stepped tank into a hipGraph; sl_batch_step with the same dt replays it

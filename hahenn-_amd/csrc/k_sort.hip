@@ -16,7 +16,10 @@
   uint32_t lb = (b);                                          \
   if ((W).ntanks > 1) { t = tank_of((W).start, (W).ntanks, b); lb = (b) - (W).start[t]; }
 
-__global__ __launch_bounds__(SL_WG) void k_sort_keys(SlWork W, uint32_t *keys, uint32_t *vals, uint32_t n_ub) {
+// key: the cell index (the sentinel outside the grid) plus `key_hi`; value: the index plus `val_off`
+// (a batch sorts several tanks in one pass, each tank's rank in the high key bits)
+__global__ __launch_bounds__(SL_WG) void k_sort_keys(SlWork W, uint32_t *keys, uint32_t *vals, uint32_t n_ub,
+                                                      uint32_t key_hi, uint32_t val_off) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
   uint32_t id = lb * SL_WG + threadIdx.x;
@@ -30,17 +33,17 @@ __global__ __launch_bounds__(SL_WG) void k_sort_keys(SlWork W, uint32_t *keys, u
       key = cell_idx(T, c.x, c.y, c.z);
     }
   }
-  keys[id] = key;
-  vals[id] = id;
+  keys[id] = key_hi | key;
+  vals[id] = val_off + id;
 }
 
-__global__ __launch_bounds__(SL_WG) void k_sort_gather(SlWork W, const uint32_t *vals) {
+__global__ __launch_bounds__(SL_WG) void k_sort_gather(SlWork W, const uint32_t *vals, uint32_t val_off) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
   uint32_t d = lb * SL_WG + threadIdx.x;
   if (d >= T.C->count) return;
   const int cur = T.cur, oth = 1 - cur;
-  uint32_t s = vals[d];
+  uint32_t s = vals[d] - val_off;
   T.px[oth][d] = T.px[cur][s]; T.py[oth][d] = T.py[cur][s]; T.pz[oth][d] = T.pz[cur][s];
   T.vx[oth][d] = T.vx[cur][s]; T.vy[oth][d] = T.vy[cur][s]; T.vz[oth][d] = T.vz[cur][s];
   T.pf[oth][d] = T.pf[cur][s];
@@ -60,11 +63,34 @@ int sl_k_reorder(const SlLaunch &L, uint32_t n_ub, uint32_t *keys, uint32_t *val
   if (!n_ub) return 0;
   SlLaunch K = L;
   K.w.total = (n_ub + SL_WG - 1) / SL_WG;
-  hipLaunchKernelGGL(k_sort_keys, dim3(K.w.total), dim3(SL_WG), 0, L.stream, K.w, keys, vals, n_ub);
+  hipLaunchKernelGGL(k_sort_keys, dim3(K.w.total), dim3(SL_WG), 0, L.stream, K.w, keys, vals, n_ub, 0u, 0u);
   size_t tb = temp_bytes;
   hipError_t e = rocprim::radix_sort_pairs(temp, tb, keys, keys + n_ub, vals, vals + n_ub, n_ub, 0, end_bit,
                                            L.stream);
   if (e != hipSuccess) return (int)e;
-  hipLaunchKernelGGL(k_sort_gather, dim3(K.w.total), dim3(SL_WG), 0, L.stream, K.w, vals + n_ub);
+  hipLaunchKernelGGL(k_sort_gather, dim3(K.w.total), dim3(SL_WG), 0, L.stream, K.w, vals + n_ub, 0u);
   return 0;
+}
+
+// the three parts of a re-ordering for a batch that sorts the particles of several tanks together:
+// keys and values of one tank at its offset, one stable sort of the n keys (values and keys in the
+// first halves of their buffers, results in the second halves), the gather of one tank
+void sl_k_sort_keys(const SlLaunch &L, uint32_t n_ub, uint32_t *keys, uint32_t *vals, uint32_t key_hi,
+                    uint32_t val_off) {
+  if (!n_ub) return;
+  SlLaunch K = L;
+  K.w.total = (n_ub + SL_WG - 1) / SL_WG;
+  hipLaunchKernelGGL(k_sort_keys, dim3(K.w.total), dim3(SL_WG), 0, L.stream, K.w, keys + val_off, vals + val_off,
+                     n_ub, key_hi, val_off);
+}
+int sl_sort_pairs(void *temp, size_t temp_bytes, uint32_t *keys, uint32_t *vals, uint32_t n, int end_bit,
+                  hipStream_t stream) {
+  size_t tb = temp_bytes;
+  return (int)rocprim::radix_sort_pairs(temp, tb, keys, keys + n, vals, vals + n, n, 0, end_bit, stream);
+}
+void sl_k_sort_gather(const SlLaunch &L, uint32_t n_ub, const uint32_t *sorted_vals, uint32_t val_off) {
+  if (!n_ub) return;
+  SlLaunch K = L;
+  K.w.total = (n_ub + SL_WG - 1) / SL_WG;
+  hipLaunchKernelGGL(k_sort_gather, dim3(K.w.total), dim3(SL_WG), 0, L.stream, K.w, sorted_vals + val_off, val_off);
 }

@@ -429,6 +429,11 @@ void sl_k_droplets(const SlLaunch &L);  // one thread per droplet (64 per workgr
 void sl_k_clear_i32(const SlLaunch &L, int which);
 // spatial re-ordering (rocPRIM radix sort of cell-index keys)
 size_t sl_sort_temp_bytes(uint32_t n, int end_bit);
+void sl_k_sort_keys(const SlLaunch &L, uint32_t n_ub, uint32_t *keys, uint32_t *vals, uint32_t key_hi,
+                    uint32_t val_off);
+int sl_sort_pairs(void *temp, size_t temp_bytes, uint32_t *keys, uint32_t *vals, uint32_t n, int end_bit,
+                  hipStream_t stream);
+void sl_k_sort_gather(const SlLaunch &L, uint32_t n_ub, const uint32_t *sorted_vals, uint32_t val_off);
 int sl_k_reorder(const SlLaunch &L, uint32_t n_ub, uint32_t *keys, uint32_t *vals, void *temp, size_t temp_bytes,
                  int end_bit);
 // ripple: rows = M workgroups per tank, cols = M/CW, elems = M*M/256, tex = (R+1)^2/256
