@@ -936,7 +936,6 @@ static int encode_reorder(sl_sim *s) {
                        s->sort_end_bit);
   if (e) { s->gpu_errors++; FAIL(s, SL_EHIP, "particle re-ordering failed: %s", hipGetErrorString((hipError_t)e)); }
   sl_k_count(launch_of(s, 1), 6, 0);
-  s->reorder_clock = 0;
   return check_launch(s, "reorder");
 }
 
@@ -951,6 +950,8 @@ extern "C" int sl_step_end(sl_sim *s, double frame_dt) {
   int r = encode_compact(s);
   if (r) return r;
   if (s->cfg.reorder_every && ++s->reorder_clock >= s->cfg.reorder_every) {
+    // the cadence restarts here only: an explicit sl_reorder leaves it (as the oracle's step does)
+    s->reorder_clock = 0;
     r = encode_reorder(s);
     if (r) return r;
   }
@@ -2243,17 +2244,23 @@ static int batch_reorder(sl_batch *b, const std::vector<uint32_t> &due) {
   const int rbits = due.size() > 1 ? 32 - __builtin_clz((uint32_t)due.size() - 1u) : 0;
   if (due.size() == 1 || cbits + rbits > 32 || ntot >= (1ull << 31)) {  // one tank, or keys too wide
     for (uint32_t t : due) {
+      b->sims[t]->reorder_clock = 0;
       int r = encode_reorder(b->sims[t]);
       if (r) return batch_fail(b, r, b->sims[t]->err);
     }
     return SL_OK;
   }
   const int end_bit = cbits + rbits;
-  if (ntot > b->ro_cap) {  // grown with a quarter of headroom, so that steady frames allocate nothing
+  if (ntot > b->ro_cap) {
+    // sized for every stepped tank at once (the most that can be due together), so that no later
+    // frame allocates or waits until the membership grows
+    uint64_t all = 0;
+    for (uint32_t t = 0; t < b->cap; t++)
+      if (b->sims[t] && b->stepped[t]) all += b->sims[t]->cap;
     if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, SL_EHIP, "re-ordering: sync");
     (void)hipFree(b->ro_keys); (void)hipFree(b->ro_vals); (void)hipFree(b->ro_temp);
     b->ro_keys = b->ro_vals = nullptr; b->ro_temp = nullptr; b->ro_cap = 0; b->ro_temp_bytes = 0;
-    const uint64_t cap = std::min<uint64_t>(ntot + ntot / 4, (1ull << 31) - 1);
+    const uint64_t cap = std::min<uint64_t>(std::max(ntot, all), (1ull << 31) - 1);
     const size_t tb = sl_sort_temp_bytes((uint32_t)cap, 32);
     if (hipMalloc((void **)&b->ro_keys, cap * 8) != hipSuccess || hipMalloc((void **)&b->ro_vals, cap * 8) != hipSuccess ||
         hipMalloc(&b->ro_temp, tb ? tb : 256) != hipSuccess)

@@ -239,6 +239,25 @@ def test_steps_with_periodic_reorder():
     assert np.array_equal(gp[:, 3], op[:, 3])
 
 
+def test_explicit_reorder_keeps_the_cadence():
+    """An explicit re-ordering between frames leaves the periodic cadence where it was (the oracle
+    restarts its clock only when the period elapses), so the automatic re-orderings of later frames
+    fall on the same frames on both sides."""
+    o, g = twin.make_twin(twin.small_scene(reorder_every=3, whitewater=False))
+    dt = 1 / 120
+    twin.oracle_step(o, dt)
+    twin.gpu_step(g, dt)
+    o.kernel("reorder")
+    g.reorder()
+    for f in range(5):
+        twin.oracle_step(o, dt)
+        twin.gpu_step(g, dt)
+        n = int(o.field("counters")[0])
+        assert g.stats()["count"] == n, f
+        gp, op = g.readback(suiri.F_PARTICLES)[:n], o.field("parts")[:n]
+        assert np.array_equal(gp.view(np.uint32), op.view(np.uint32)), f
+
+
 def test_autotune_keeps_results_bitwise():
     """The tuner times every kernel variant on the live state and restores it; the variants compute
     identical results, so the run continues exactly like the oracle."""
