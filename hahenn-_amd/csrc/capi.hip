@@ -423,11 +423,14 @@ static int tank_buffers(sl_sim *s, SlTank &T, Take take) {
     TAKE(T.topH, wm * wm * 4);
     if (s->cfg.top_height) TAKE(T.topB, wm * wm * 4);
   }
-  // re-ordering keys: cell indices 0 .. C-1 and C for particles outside the grid
+  // re-ordering keys: cell indices 0 .. C-1 and C for particles outside the grid (the tanks of a
+  // batch sort in the batch's buffers)
   {
     int bits = 0;
     while ((1ull << bits) <= (uint64_t)s->cells) bits++;
     s->sort_end_bit = bits;
+  }
+  if (!s->ext_tank) {
     TAKE(s->sort_keys, (size_t)s->cap * 8);
     TAKE(s->sort_vals, (size_t)s->cap * 8);
     s->sort_temp_bytes = sl_sort_temp_bytes(s->cap, s->sort_end_bit);
@@ -931,6 +934,7 @@ static int encode_stats_copy(sl_sim *s) {
 }
 
 static int encode_reorder(sl_sim *s) {
+  if (!s->sort_keys) FAIL(s, SL_EARG, "re-ordering: the tanks of a batch re-order with sl_batch_reorder");
   StageScope sc(s, SL_T_COMPACT, 5);
   int e = sl_k_reorder(launch_of(s, 1), s->p_ub, s->sort_keys, s->sort_vals, s->sort_temp, s->sort_temp_bytes,
                        s->sort_end_bit);
@@ -2229,31 +2233,19 @@ static int batch_prepare(sl_batch *b) {
   return e == hipSuccess ? SL_OK : batch_fail(b, SL_EHIP, hipGetErrorString(e));
 }
 
-// The re-orderings due this frame as one stable radix sort: each tank's keys carry its rank in the
-// list above the cell-index bits, so the tanks' particles come out in list order, and each tank's in
-// exactly the order of its own sort (stable, same cell keys, the same sentinel outside the grid). One
-// sort instead of one per tank: a radix sort is a dozen or so launches whatever its length.
-static int batch_reorder(sl_batch *b, const std::vector<uint32_t> &due) {
-  if (due.empty()) return SL_OK;
-  uint64_t ntot = 0;
-  int cbits = 1;
-  for (uint32_t t : due) {
-    ntot += b->sims[t]->p_ub;
-    cbits = std::max(cbits, b->sims[t]->sort_end_bit);
-  }
-  const int rbits = due.size() > 1 ? 32 - __builtin_clz((uint32_t)due.size() - 1u) : 0;
-  if (due.size() == 1 || cbits + rbits > 32 || ntot >= (1ull << 31)) {  // one tank, or keys too wide
-    for (uint32_t t : due) {
-      b->sims[t]->reorder_clock = 0;
-      int r = encode_reorder(b->sims[t]);
-      if (r) return batch_fail(b, r, b->sims[t]->err);
-    }
-    return SL_OK;
-  }
+// Re-orderings of batch tanks as one stable radix sort per group: each tank's keys carry its rank
+// in the group above the cell-index bits, so the tanks' particles come out in group order, and each
+// tank's in exactly the order of its own sort (stable, same cell keys, the same sentinel outside the
+// grid). One sort instead of one per tank: a radix sort is a dozen or so launches whatever its
+// length. A group takes tanks while the key fits 32 bits and the group stays below 2^31 particles;
+// a single tank always fits (fewer than 2^30 cells and particles). The tanks of a batch hold no sort
+// buffers of their own: the batch's serve them all (sized for every stepped tank at once, so that
+// no later frame allocates or waits until the membership grows). `restart` restarts the tanks'
+// cadence (the periodic re-orderings; an explicit one leaves it).
+static int batch_sort_group(sl_batch *b, const uint32_t *ts, size_t k, uint64_t ntot, int cbits, bool restart) {
+  const int rbits = k > 1 ? 32 - __builtin_clz((uint32_t)k - 1u) : 0;
   const int end_bit = cbits + rbits;
   if (ntot > b->ro_cap) {
-    // sized for every stepped tank at once (the most that can be due together), so that no later
-    // frame allocates or waits until the membership grows
     uint64_t all = 0;
     for (uint32_t t = 0; t < b->cap; t++)
       if (b->sims[t] && b->stepped[t]) all += b->sims[t]->cap;
@@ -2268,25 +2260,46 @@ static int batch_reorder(sl_batch *b, const std::vector<uint32_t> &due) {
     b->ro_cap = cap;
     b->ro_temp_bytes = tb ? tb : 256;
   }
-  uint32_t off = 0, rank = 0;
-  for (uint32_t t : due) {
-    sl_sim *s = b->sims[t];
-    sl_k_sort_keys(launch_of(s, 1), s->p_ub, b->ro_keys, b->ro_vals, rank << cbits, off);
+  uint32_t off = 0;
+  for (size_t r = 0; r < k; r++) {
+    sl_sim *s = b->sims[ts[r]];
+    sl_k_sort_keys(launch_of(s, 1), s->p_ub, b->ro_keys, b->ro_vals, (uint32_t)r << cbits, off);
     off += s->p_ub;
-    rank++;
   }
   int e = sl_sort_pairs(b->ro_temp, b->ro_temp_bytes, b->ro_keys, b->ro_vals, (uint32_t)ntot, end_bit, b->stream);
   if (e) return batch_fail(b, SL_EHIP, std::string("re-ordering sort: ") + hipGetErrorString((hipError_t)e));
   off = 0;
-  for (uint32_t t : due) {
-    sl_sim *s = b->sims[t];
+  for (size_t r = 0; r < k; r++) {
+    sl_sim *s = b->sims[ts[r]];
     sl_k_sort_gather(launch_of(s, 1), s->p_ub, b->ro_vals + ntot, off);
     sl_k_count(launch_of(s, 1), 6, 0);  // the gathered buffer becomes current
-    s->reorder_clock = 0;
+    if (restart) s->reorder_clock = 0;
     off += s->p_ub;
   }
   hipError_t le = hipGetLastError();
   return le == hipSuccess ? SL_OK : batch_fail(b, SL_EHIP, std::string("re-ordering: ") + hipGetErrorString(le));
+}
+static int batch_reorder(sl_batch *b, const std::vector<uint32_t> &due, bool restart) {
+  size_t i = 0;
+  while (i < due.size()) {
+    size_t j = i;
+    uint64_t n = 0;
+    int cbits = 1;
+    while (j < due.size()) {
+      const sl_sim *s = b->sims[due[j]];
+      const int cb = std::max(cbits, s->sort_end_bit);
+      const size_t k = j - i + 1;
+      const int rb = k > 1 ? 32 - __builtin_clz((uint32_t)k - 1u) : 0;
+      if (j > i && (cb + rb > 32 || n + s->p_ub >= (1ull << 31))) break;
+      cbits = cb;
+      n += s->p_ub;
+      j++;
+    }
+    int r = batch_sort_group(b, due.data() + i, j - i, n, cbits, restart);
+    if (r) return r;
+    i = j;
+  }
+  return SL_OK;
 }
 
 // host mirror of one frame of every stepped tank, plus each tank's re-ordering cadence
@@ -2306,7 +2319,7 @@ static int batch_host_frame(sl_batch *b, double dt) {
     s->emitted_at_copy = s->emitted_total;
     if (s->cfg.reorder_every && ++s->reorder_clock >= s->cfg.reorder_every) due.push_back(t);
   }
-  return batch_reorder(b, due);
+  return batch_reorder(b, due, true);
 }
 
 // dt > 0: every stepped tank advances by dt; dt = 0: each by its own frame length (sl_batch_set_dt)
@@ -2409,12 +2422,10 @@ extern "C" int sl_batch_sync(sl_batch *b) {
 
 extern "C" int sl_batch_reorder(sl_batch *b) {
   if (!b) return batch_fail(nullptr, SL_EARG, "null batch");
-  for (uint32_t t = 0; t < b->cap; t++) {
-    if (!live(b, t)) continue;
-    int r = encode_reorder(b->sims[t]);
-    if (r) return batch_fail(b, r, b->sims[t]->err);
-  }
-  return SL_OK;
+  std::vector<uint32_t> all;
+  for (uint32_t t = 0; t < b->cap; t++)
+    if (live(b, t)) all.push_back(t);
+  return batch_reorder(b, all, false);
 }
 
 extern "C" int sl_batch_timing_enable(sl_batch *b, int enable) {
