@@ -2243,6 +2243,11 @@ static int batch_prepare(sl_batch *b) {
 // no later frame allocates or waits until the membership grows). `restart` restarts the tanks'
 // cadence (the periodic re-orderings; an explicit one leaves it).
 static int batch_sort_group(sl_batch *b, const uint32_t *ts, size_t k, uint64_t ntot, int cbits, bool restart) {
+  if (!ntot) {  // nothing to sort (empty tanks)
+    if (restart)
+      for (size_t r = 0; r < k; r++) b->sims[ts[r]]->reorder_clock = 0;
+    return SL_OK;
+  }
   const int rbits = k > 1 ? 32 - __builtin_clz((uint32_t)k - 1u) : 0;
   const int end_bit = cbits + rbits;
   if (ntot > b->ro_cap) {
