@@ -2243,11 +2243,6 @@ static int batch_prepare(sl_batch *b) {
 // no later frame allocates or waits until the membership grows). `restart` restarts the tanks'
 // cadence (the periodic re-orderings; an explicit one leaves it).
 static int batch_sort_group(sl_batch *b, const uint32_t *ts, size_t k, uint64_t ntot, int cbits, bool restart) {
-  if (!ntot) {  // nothing to sort (empty tanks)
-    if (restart)
-      for (size_t r = 0; r < k; r++) b->sims[ts[r]]->reorder_clock = 0;
-    return SL_OK;
-  }
   const int rbits = k > 1 ? 32 - __builtin_clz((uint32_t)k - 1u) : 0;
   const int end_bit = cbits + rbits;
   if (ntot > b->ro_cap) {
@@ -2271,7 +2266,8 @@ static int batch_sort_group(sl_batch *b, const uint32_t *ts, size_t k, uint64_t 
     sl_k_sort_keys(launch_of(s, 1), s->p_ub, b->ro_keys, b->ro_vals, (uint32_t)r << cbits, off);
     off += s->p_ub;
   }
-  int e = sl_sort_pairs(b->ro_temp, b->ro_temp_bytes, b->ro_keys, b->ro_vals, (uint32_t)ntot, end_bit, b->stream);
+  // (empty tanks sort nothing but still swap their buffers, as a tank alone does)
+  int e = ntot ? sl_sort_pairs(b->ro_temp, b->ro_temp_bytes, b->ro_keys, b->ro_vals, (uint32_t)ntot, end_bit, b->stream) : 0;
   if (e) return batch_fail(b, SL_EHIP, std::string("re-ordering sort: ") + hipGetErrorString((hipError_t)e));
   off = 0;
   for (size_t r = 0; r < k; r++) {
