@@ -12,8 +12,9 @@ projection, G2P, compaction, whitewater, ripple frame). Inputs are resident in H
 Prints one JSON line: particle-steps/s, the SOR kernel's HBM roofline and a CPU baseline (the fp32
 C restatement of the reference, OpenMP on the host cores, on a bounded sample of the same workload).
 Beside the headline: the projection residual the reference's 60 sweeps leave, the same workload over
-its stated 1000 steps (timed in windows of 200), the same workload with the converged pressure solve (multigrid-preconditioned CG to a residual tolerance), the batched tanks
-and the serving loop.
+its stated 1000 steps (timed in windows of 200), the same workload with the converged pressure solve
+(multigrid-preconditioned CG to a residual tolerance), the ripple-and-droplet workload, the batched
+tanks and the serving loop.
 """
 import argparse
 import importlib
@@ -152,10 +153,14 @@ def cpu_baseline(wl, steps, threads):
     return psteps / dt, dt
 
 
+TRAFFIC_KERNELS = ("k_sor", "k_g2p", "k_p2g_tiles", "k_bin_count", "k_bin_scatter", "k_normforce")
+
+
 def pmc_traffic(out_dir, timeout=150):
-    """HBM bytes of one k_sor launch from two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE) of a short
-    bench, each scaled by what the same counter reads for a plain 256 MiB device copy. Child processes;
-    returns None when rocprofv3 is missing or a pass fails."""
+    """HBM bytes per launch of k_sor (and of the G2P, P2G and NORMALIZE kernels, under "kernels") from
+    two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE) of a short bench, each scaled by what the same
+    counter reads for a plain 256 MiB device copy. Child processes; returns None when rocprofv3 is
+    missing or a pass fails."""
     import csv
     import shutil
     import statistics
@@ -167,7 +172,8 @@ def pmc_traffic(out_dir, timeout=150):
     short = [sys.executable, os.path.join(ROOT, "bench.py"), "--steps", "3", "--warmup", "1", "--no-cpu", "--no-pmc",
              "--only", "headline"]
     calib = [sys.executable, os.path.join(ROOT, "bench.py"), "--calib-copy-mb", "256"]
-    runs = {"fetch": ("FETCH_SIZE", short, "k_sor"), "write": ("WRITE_SIZE", short, "k_sor"),
+    regex = "|".join(TRAFFIC_KERNELS)
+    runs = {"fetch": ("FETCH_SIZE", short, regex), "write": ("WRITE_SIZE", short, regex),
             "cfetch": ("FETCH_SIZE", calib, None), "cwrite": ("WRITE_SIZE", calib, None)}
     vals = {}
     for key, (counter, cmd, regex) in runs.items():
@@ -181,17 +187,30 @@ def pmc_traffic(out_dir, timeout=150):
         except Exception:
             return None
         if regex:
-            v = [float(r["Counter_Value"]) for r in rows if r["Kernel_Name"].startswith("k_sor(")]
+            per = {}
+            for r in rows:
+                name = r["Kernel_Name"]
+                for k in TRAFFIC_KERNELS:   # every k_g2p<MINW> variant moves the same bytes
+                    if name.startswith(k + "(") or name.startswith("void " + k + "<"):
+                        per.setdefault(k, []).append(float(r["Counter_Value"]))
+            if "k_sor" not in per:
+                return None
+            vals[key] = {k: statistics.mean(v) for k, v in per.items()}
         else:
             v = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
-        if not v:
-            return None
-        vals[key] = statistics.mean(v)
+            if not v:
+                return None
+            vals[key] = statistics.mean(v)
     kib = 256 * 1024
-    fetch = vals["fetch"] * 1024 * kib / vals["cfetch"]
-    write = vals["write"] * 1024 * kib / vals["cwrite"]
-    return {"bytes": fetch + write, "fetch_bytes": fetch, "write_bytes": write,
-            "fetch_factor": kib / vals["cfetch"], "write_factor": kib / vals["cwrite"]}
+    ff, wf = kib / vals["cfetch"], kib / vals["cwrite"]
+    kern = {}
+    for k in vals["fetch"]:
+        if k in vals["write"]:
+            fb, wb = vals["fetch"][k] * 1024 * ff, vals["write"][k] * 1024 * wf
+            kern[k] = {"bytes": fb + wb, "fetch_bytes": fb, "write_bytes": wb}
+    s0 = kern["k_sor"]
+    return {"bytes": s0["bytes"], "fetch_bytes": s0["fetch_bytes"], "write_bytes": s0["write_bytes"],
+            "fetch_factor": ff, "write_factor": wf, "kernels": kern}
 
 
 def projection_residual(g, dt):
@@ -694,6 +713,15 @@ def main():
             rl = out["roofline"]
             rl["traffic_GBps"] = t["bytes"] / (rl["avg_launch_us"] * 1e-6) / 1e9
             rl["traffic_frac"] = rl["traffic_GBps"] / HBM_PEAK_GBS
+            # HBM bytes per launch of the P2G and G2P kernels beside their algorithmic bytes
+            kk = t.get("kernels", {})
+            sr = out["stage_roofline"]
+            if "k_g2p" in kk:
+                sr["g2p"]["traffic"] = kk["k_g2p"]["bytes"]
+            p2g = [kk[k]["bytes"] for k in ("k_bin_count", "k_bin_scatter", "k_p2g_tiles") if k in kk]
+            if len(p2g) == 3:
+                sr["p2g"]["traffic"] = sum(p2g)
+                sr["p2g"]["traffic_detail"] = {k: kk[k]["bytes"] for k in ("k_bin_count", "k_bin_scatter", "k_p2g_tiles")}
         progress("counter runs done")
     if not args.no_cpu:
         # every core the process may use: the host's share for one GPU sets OMP_NUM_THREADS, else nproc
