@@ -190,12 +190,14 @@ def pmc_traffic(out_dir, timeout=150):
             per = {}
             for r in rows:
                 name = r["Kernel_Name"]
-                for k in TRAFFIC_KERNELS:   # every k_g2p<MINW> variant moves the same bytes
+                for k in TRAFFIC_KERNELS:
                     if name.startswith(k + "(") or name.startswith("void " + k + "<"):
-                        per.setdefault(k, []).append(float(r["Counter_Value"]))
+                        per.setdefault(k, {}).setdefault(name, []).append(float(r["Counter_Value"]))
             if "k_sor" not in per:
                 return None
-            vals[key] = {k: statistics.mean(v) for k, v in per.items()}
+            # of several template variants (the tuner's trials of k_g2p<MINW>), the one the timed steps
+            # ran: the most dispatches (a spilling trial variant writes scratch on top of the step's bytes)
+            vals[key] = {k: statistics.mean(max(v.values(), key=len)) for k, v in per.items()}
         else:
             v = [float(r["Counter_Value"]) for r in rows if "copyBuffer" in r["Kernel_Name"]]
             if not v:
