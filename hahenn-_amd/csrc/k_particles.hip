@@ -559,10 +559,17 @@ __global__ __launch_bounds__(SL_WG) void k_compact_count(SlWork W) {
   const uint32_t lim = min(T.C->count, T.cap);
   const uint32_t base = lb * SL_COMPACT_ITEMS;
   const uint8_t *pf = T.pf[T.cur];
+  static_assert(SL_COMPACT_ITEMS == 4 * SL_WG, "a thread counts four consecutive flags");
+  // four flags per thread as one word where all four lie below the count (the word then lies inside
+  // the flag array, whose length is the capacity)
+  const uint32_t id = base + 4u * threadIdx.x;
   uint32_t c = 0;
-  for (uint32_t q = 0; q < SL_COMPACT_ITEMS / SL_WG; q++) {
-    uint32_t id = base + q * SL_WG + threadIdx.x;
-    if (id < lim && (pf[id] & 1u)) c++;
+  if (id + 4u <= lim) {
+    const uint32_t w = ld32((const uint32_t *)pf, id / 4u) & 0x01010101u;
+    c = __builtin_popcount(w);
+  } else {
+    for (uint32_t q = 0; q < 4u; q++)
+      if (id + q < lim && (pf[id + q] & 1u)) c++;
   }
   __shared__ uint32_t red[SL_WG / 64];
   for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
@@ -654,6 +661,11 @@ __global__ __launch_bounds__(SL_WG) void k_compact_holes(SlWork W) {
   const uint32_t lim = min(T.C->count, T.cap), n1 = T.C->newCount;
   const uint32_t base = lb * SL_COMPACT_ITEMS;
   if (n1 >= lim || base > n1) return;  // nothing died, or no slot below n' here
+  // a full workgroup below n' whose particles all live has no hole (the one that holds n' runs: it
+  // records the survivors below n')
+  const uint32_t nb = (lim + SL_COMPACT_ITEMS - 1) / SL_COMPACT_ITEMS;
+  if (lb + 1u < nb && base + SL_COMPACT_ITEMS <= n1 && T.scanBlocks[lb + 1u] - T.scanBlocks[lb] == SL_COMPACT_ITEMS)
+    return;
   constexpr int Q = SL_COMPACT_ITEMS / SL_WG;
   uint8_t f[Q];
   uint32_t rank[Q];
