@@ -19,8 +19,8 @@ __device__ __forceinline__ float2 ld_pair(const float *base, uint32_t n) {
   __builtin_memcpy(&r, __builtin_assume_aligned((const char *)base + n * 4u, 4), 8);
   return r;
 }
-// masked PIC/FLIP gather, 31-wgsl-sim2.js:9-36, in two parts so that G2P can issue the loads of all
-// three components before it sums any: tgather finds the 8 corners of q on component `comp`'s grid
+// masked PIC/FLIP gather, 31-wgsl-sim2.js:9-36, in two parts so that G2P can issue the loads of one
+// component before it sums another: tgather finds the 8 corners of q on component `comp`'s grid
 // and loads vel and velOld (whose bit pattern carries the mask) of each row (y, z clamped into the
 // grid) as the node pair at b0 = clamp(bx, 0, nx - 1), whose second node is the corner at i = nx and
 // whose first is the corner at i = 0 reached from bx = -1; tsum drops the corners outside the grid
@@ -290,11 +290,15 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
   const i3 c0 = toi3_hw(g);
   const uint32_t n0 = ing ? count_at(T, c0.x, c0.y, c0.z) : 0u;
   if (ing) {
+    // two components' loads in flight, the third's issued while the first is summed: all three up
+    // front need 87 VGPRs (5 waves per SIMD), this order 79 (6 waves without spills; G2P 0.587 ->
+    // 0.525 ms at 256x128x128)
     TGather gx, gy, gz;
     tgather(T, sub3(g, F3(0.0f, 0.5f, 0.5f)), 0, gx);
     tgather(T, sub3(g, F3(0.5f, 0.0f, 0.5f)), 1, gy);
+    const f3 tx = tsum(gx, v.x);
     tgather(T, sub3(g, F3(0.5f, 0.5f, 0.0f)), 2, gz);
-    const f3 tx = tsum(gx, v.x), ty = tsum(gy, v.y), tz = tsum(gz, v.z);
+    const f3 ty = tsum(gy, v.y), tz = tsum(gz, v.z);
     v = F3(mixf(tx.x, tx.y, T.flip), mixf(ty.x, ty.y, T.flip), mixf(tz.x, tz.y, T.flip));
   } else {
     v.y -= T.g * dt;
