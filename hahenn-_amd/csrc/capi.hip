@@ -64,7 +64,7 @@ struct sl_sim {
   double h = 0, origin[3] = {0, 0, 0}, rim = 0, vp = 0, radius = 0, time = 0, vmax = 1.0;
   uint32_t nodes = 0, cells = 0, hw = 0, hcells = 0, cap = 0, ww_cap = 0, R = 0, M = 0, iters = 0;
   uint32_t frame = 0, obj_count = 0, count = 0, ww_count = 0, impacts = 0, frames = 0;
-  bool full = false, geom_dirty = true;
+  bool full = false, geom_dirty = true, mask_dirty = true;  // mask_dirty: objMask out of date
   int cur = 0, ww_cur = 0;
   uint32_t forced_substeps = 0, substeps_last = 0;
   // spatial re-ordering
@@ -406,6 +406,7 @@ static int tank_buffers(sl_sim *s, SlTank &T, Take take) {
   }
   TAKE(T.rhs, H * 8); TAKE(T.sflag, H * 4);
   TAKE(T.objs, sizeof(SlObj) * SL_MAX_OBJ);
+  TAKE(T.objMask, (size_t)cdiv(C, 64) * 8 + 64);
   if (s->ext_emits) T.emits = s->ext_emits;
   else TAKE(T.emits, sizeof(SlEmitDev) * SL_MAX_EMIT);
   TAKE(T.scanBlocks, (size_t)cdiv(s->cap, SL_COMPACT_ITEMS) * 4 + 64);
@@ -640,6 +641,7 @@ extern "C" int sl_set_objects(sl_sim *s, const float *rows, uint32_t n, int *cha
   if (n) memcpy(s->objs, rows, (size_t)n * 28 * 4);
   s->obj_count = n;
   s->geom_dirty = true;
+  s->mask_dirty = true;
   float dev[SL_MAX_OBJ * 28];
   memcpy(dev, s->objs, sizeof dev);
   for (uint32_t i = 0; i < n; i++) dev[i * 28 + 3] = cull_radius(dev + i * 28);
@@ -846,6 +848,13 @@ static void encode_p2g(sl_sim *s, int variant) {
     sl_k_p2g(launch_of(s, pb, sl_p2g_chunk(variant)), 0, variant);
 }
 
+// G2P's object mask, after objects changed (the objects and the count the substep's parameters hold)
+static void encode_objmask(sl_sim *s) {
+  if (!s->mask_dirty) return;
+  sl_k_objmask(launch_of(s, s->cells));
+  s->mask_dirty = false;
+}
+
 static int encode_substep(sl_sim *s) {
   const uint32_t pb = s->p_ub;
   if (s->geom_dirty) {
@@ -861,6 +870,7 @@ static int encode_substep(sl_sim *s) {
   { StageScope sc(s, SL_T_NORMFORCE, 1); sl_k_normforce(launch_of(s, s->nodes)); }
   { StageScope sc(s, SL_T_DIVERGENCE, 1); sl_k_divsetup(launch_of(s, s->cells)); }
   encode_pressure(s);
+  encode_objmask(s);
   { StageScope sc(s, SL_T_G2P, 1); sl_k_g2p(launch_of(s, pb), s->g2p_variant); }
   return check_launch(s, "substep");
 }
@@ -1505,6 +1515,7 @@ extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
       else sl_k_project(launch_of(s, s->nodes));
       break;
     case SL_S_G2P:
+      encode_objmask(s);
       sl_k_g2p(launch_of(s, s->p_ub), s->g2p_variant);
       sl_k_count(launch_of(s, 1), 0, 0);  // the speed slots into vmax
       break;
@@ -2218,12 +2229,13 @@ static int batch_encode_frame(sl_batch *b, double dt) {
 static int batch_prepare(sl_batch *b) {
   for (uint32_t t = 0; t < b->cap; t++) {
     sl_sim *s = b->sims[t];
-    if (!s || !b->stepped[t] || !s->geom_dirty) continue;
+    if (!s || !b->stepped[t] || !(s->geom_dirty || s->mask_dirty)) continue;
     s->ht.obj_count = s->obj_count;  // this frame's parameters, as the single path writes before GEOM
     int r = push_dyn(s);
     if (r) return batch_fail(b, r, s->err);
-    sl_k_geom(launch_of(s, s->nodes));
+    if (s->geom_dirty) sl_k_geom(launch_of(s, s->nodes));
     s->geom_dirty = false;
+    encode_objmask(s);
   }
   if (b->maps_dirty) {
     int r = batch_rebuild(b);

@@ -75,6 +75,50 @@ void sl_k_geom(const SlLaunch &L) {
   hipLaunchKernelGGL(k_geom, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
 }
 
+// Object mask for G2P's culling (no counterpart in the reference; it only skips work whose outcome
+// it fixes). Bit c is set when some obstacle's cull sphere grown by the contact radius rc, or some
+// drain's cull sphere, comes within 1 mm of cell c's box grown by 2 h + 2 rc on every side. G2P reads
+// the bit of the cell a particle starts in; when the particle started inside the footprint and moved
+// at most h along every axis, the obstacle test point (its advected position) lies within h of that
+// box and the drain test point (after the wall clamp, which then moves it by less than rc along each
+// axis) within h + rc, so with the bit clear every object fails G2P's own cull test
+// (beyond_cull, which the margins cover with room for rounding) and both loops would find nothing.
+// Types other than obstacle (1) and drain (2) never matter to G2P; a +inf cull radius (never culled)
+// sets every bit, a negative one none.
+__global__ __launch_bounds__(SL_WG) void k_objmask(SlWork W) {
+  SL_TANK_OF(W, blockIdx.x, t, lb);
+  const SlTank &T = W.tanks[t];
+  const uint32_t c = lb * SL_WG + threadIdx.x;
+  const bool in = c < T.cells;
+  bool near = false;
+  if (in) {
+    const uint32_t nx = (uint32_t)T.nx;
+    const uint32_t kq = udiv_mag(c, T.nxy_mag), rq = c - kq * (nx * (uint32_t)T.ny), jq = udiv_mag(rq, T.nx_mag);
+    const float h = T.h, pad = 2.0f * h + 2.0f * T.rc;
+    const f3 lo = F3(T.ox + (float)(rq - jq * nx) * h - pad, T.oy + (float)jq * h - pad, T.oz + (float)kq * h - pad);
+    const f3 hi = F3(lo.x + h + 2.0f * pad, lo.y + h + 2.0f * pad, lo.z + h + 2.0f * pad);
+    for (uint32_t q = 0; q < T.obj_count && !near; q++) {
+      const SlObj &o = T.objs[q];
+      if (o.angv[3] != 1.0f && o.angv[3] != 2.0f) continue;
+      const float cr = (o.r0[3] + (o.angv[3] == 1.0f ? T.rc : 0.0f)) * 1.0002f + 1e-4f;
+      if (cr < 0.0f) continue;
+      const float dx = fmaxf(fmaxf(lo.x - o.pos[0], o.pos[0] - hi.x), 0.0f);
+      const float dy = fmaxf(fmaxf(lo.y - o.pos[1], o.pos[1] - hi.y), 0.0f);
+      const float dz = fmaxf(fmaxf(lo.z - o.pos[2], o.pos[2] - hi.z), 0.0f);
+      const float rr = cr + 1e-3f;
+      near = dx * dx + dy * dy + dz * dz <= rr * rr;
+    }
+  }
+  // the wave's 64 cells start at a multiple of 64: lanes 0 and 32 store its two words
+  const uint64_t m = __ballot(near);
+  const int lane = __lane_id();
+  if ((lane & 31) == 0 && in) T.objMask[c >> 5] = (uint32_t)(m >> lane);
+}
+void sl_k_objmask(const SlLaunch &L) {
+  if (!L.w.total) return;
+  hipLaunchKernelGGL(k_objmask, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+}
+
 // ------------------------------------------------------------------------------------------------
 // P2G: trilinear splat of each velocity component to its 8 face nodes in 2^18 fixed point,
 // 30-wgsl-sim.js:159-200. Integer atomics make the sums independent of the particle order.

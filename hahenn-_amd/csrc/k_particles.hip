@@ -286,9 +286,12 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
   f3 v = v0;
   const f3 g = grid_pos(T, x0);
   const bool ing = in_grid(T, g);
-  // the particle's own cell count goes out with the gathers (the free-flight test below reads it)
+  // the particle's own cell count and its cell's object-mask word go out with the gathers (the
+  // free-flight test and the object culling below read them)
   const i3 c0 = toi3_hw(g);
-  const uint32_t n0 = ing ? count_at(T, c0.x, c0.y, c0.z) : 0u;
+  const uint32_t ci0 = cell_idx(T, c0.x, c0.y, c0.z);
+  const uint32_t n0 = ing ? ld32(T.cellCount, ci0) : 0u;
+  const uint32_t mword = ing ? ld32(T.objMask, ci0 >> 5) : 0xffffffffu;
   if (ing) {
     // two components' loads in flight, the third's issued while the first is summed: all three up
     // front need 87 VGPRs (5 waves per SIMD), this order 79 (6 waves without spills; G2P 0.587 ->
@@ -305,8 +308,12 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
   }
   f3 x = add3(x0, mul3(v, dt));
   const float rc = T.rc;
+  // object loops only where the mask of the start cell allows an object within reach (it covers a
+  // particle from inside the footprint that moved at most h along every axis; others take the loops)
+  const bool objs = !ing || !in_footprint(T, x0) || !(fabsf(x.x - x0.x) <= h && fabsf(x.y - x0.y) <= h &&
+                                                      fabsf(x.z - x0.z) <= h) || ((mword >> (ci0 & 31u)) & 1u);
   // only d < rc matters; no obstacle within reach means scene_sdf(x) > rc
-  const float d = obstacle_within(T, x, rc) ? scene_sdf(T, x) : rc;
+  const float d = objs && obstacle_within(T, x, rc) ? scene_sdf(T, x) : rc;
   if (d < rc) {
     f3 nrm = scene_normal(T, x);
     x = add3(x, mul3(nrm, rc - d));
@@ -317,7 +324,7 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
   }
   tank_clamp(T, x0, x, v, rc);
   bool alive = true;
-  if (in_drain(T, x)) alive = false;
+  if (objs && in_drain(T, x)) alive = false;
   if (!in_footprint(T, x) && x.y < 0.002f) alive = false;
   if (fabsf(x.x) > 20.0f || fabsf(x.z) > 20.0f || x.y < -1.0f || x.y > 30.0f) alive = false;
   bool airborne = (pf & 2u) != 0u;

@@ -87,6 +87,9 @@ struct SlTank {
   float *divg, *pres, *rhs;
   uint16_t *sflag;
   SlObj *objs;
+  // one bit per cell (x fastest): some object's cull sphere may reach a particle that starts in the
+  // cell and moves at most one cell in a substep (k_objmask); G2P skips its object loops elsewhere
+  uint32_t *objMask;
   SlEmitDev *emits;
   uint32_t *scanBlocks;   // compaction per-workgroup counts / offsets
   // binned P2G: per particle its cell and rank in the cell, the particle indices in cell order,
@@ -356,6 +359,7 @@ struct SlLaunch {
 // grid stages (work map in units of SL_WG-wide workgroups over the stage's index space)
 void sl_k_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 void sl_k_geom(const SlLaunch &L);
+void sl_k_objmask(const SlLaunch &L);  // over the cells
 void sl_k_p2g(const SlLaunch &L, int count_only, int variant);
 int sl_p2g_chunk(int variant);
 // binned P2G: particles (SL_WG per workgroup), scan blocks (one workgroup each), tanks (one workgroup

@@ -338,6 +338,32 @@ def test_every_obstacle_shape_and_drain_bitwise():
     assert n < n0, "the drain should take particles"
 
 
+def test_moving_obstacle_and_drain_bitwise():
+    """A sphere sweeping through the water and a drain moving along the floor, both moved every frame
+    (G2P's object mask, which lets particles far from every object skip the object loops, is rebuilt
+    after each move): the oracle's particles bit for bit, frame after frame, with collisions and drain
+    kills happening."""
+    sc = twin.small_scene(fill_cells=(24, 12, 16))
+    o, g = twin.make_twin(sc)
+    sc_g = scene.Scene(g.cfg)
+    n0 = int(o.field("counters")[0])
+    items = [scene.make_obstacle("sphere", dims={"r": 0.035}), scene.make_drain(r=0.025)]
+    for f in range(14):
+        items[0]["pos"] = [-0.2 + 0.025 * f, 0.05, 0.01 * f - 0.05]   # the same ids: moving, with velocities
+        items[1]["pos"] = [0.2 - 0.02 * f, 0.0, -0.06 + 0.008 * f]
+        rows, _ = sc_g.pack(items, 1 / 120)
+        o.set_objects(rows)
+        g.set_objects_rows(rows)
+        twin.oracle_step(o, 1 / 120)
+        twin.gpu_step(g, 1 / 120)
+        n = int(o.field("counters")[0])
+        assert g.stats()["count"] == n, f
+        gp, op = g.readback(suiri.F_PARTICLES)[:n], o.field("parts")[:n]
+        assert np.array_equal(gp.view(np.uint32), op.view(np.uint32)), f
+    assert n < n0, "the drain should take particles"
+    assert np.any(o.field("parts")[:n, 4:7] != 0)
+
+
 def test_multi_step_dam_break():
     """500 steps of a dam break with two obstacles (60 SOR iterations, omega 1.7, dt 1/120): every 50
     steps the fluid volume (live particles x particle volume) and the kinetic energy track the oracle
