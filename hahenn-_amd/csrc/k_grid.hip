@@ -76,15 +76,18 @@ void sl_k_geom(const SlLaunch &L) {
 }
 
 // Object mask for G2P's culling (no counterpart in the reference; it only skips work whose outcome
-// it fixes). Bit c is set when some obstacle's cull sphere grown by the contact radius rc, or some
-// drain's cull sphere, comes within 1 mm of cell c's box grown by 2 h + 2 rc on every side. G2P reads
-// the bit of the cell a particle starts in; when the particle started inside the footprint and moved
-// at most h along every axis, the obstacle test point (its advected position) lies within h of that
-// box and the drain test point (after the wall clamp, which then moves it by less than rc along each
-// axis) within h + rc, so with the bit clear every object fails G2P's own cull test
-// (beyond_cull, which the margins cover with room for rounding) and both loops would find nothing.
-// Types other than obstacle (1) and drain (2) never matter to G2P; a +inf cull radius (never culled)
-// sets every bit, a negative one none.
+// it fixes). G2P reads the bit of the cell a particle starts in; when the particle started inside the
+// footprint and moved at most h along every axis, its obstacle test point (the advected position) and
+// its drain test point (after the wall clamp, which then moves it by less than rc along each axis)
+// lie in cell c's box grown by 2 h + 2 rc on every side. Bit c is clear only when every object fails
+// there: an obstacle or drain whose cull sphere (grown by rc for an obstacle) stays more than 1 mm
+// from the grown box fails G2P's own cull test (beyond_cull, which the margin covers with room for
+// rounding); one whose cull sphere reaches the box fails when its distance function at the box centre
+// exceeds rc (obstacle; 0 for a drain's cylinder) by the box's half-diagonal and 1 mm -- every shape's
+// distance changes by at most the distance moved (exact distances, or the max of such), with a rotation
+// the host verified orthonormal, which is also when it gives a finite cull radius -- so G2P's collision
+// test d < rc and its drain test sd < 0 fail at every test point. An object without a finite cull
+// radius sets every bit. Types other than obstacle (1) and drain (2) never matter to G2P.
 __global__ __launch_bounds__(SL_WG) void k_objmask(SlWork W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
@@ -94,19 +97,24 @@ __global__ __launch_bounds__(SL_WG) void k_objmask(SlWork W) {
   if (in) {
     const uint32_t nx = (uint32_t)T.nx;
     const uint32_t kq = udiv_mag(c, T.nxy_mag), rq = c - kq * (nx * (uint32_t)T.ny), jq = udiv_mag(rq, T.nx_mag);
-    const float h = T.h, pad = 2.0f * h + 2.0f * T.rc;
-    const f3 lo = F3(T.ox + (float)(rq - jq * nx) * h - pad, T.oy + (float)jq * h - pad, T.oz + (float)kq * h - pad);
-    const f3 hi = F3(lo.x + h + 2.0f * pad, lo.y + h + 2.0f * pad, lo.z + h + 2.0f * pad);
+    const float h = T.h, pad = 2.0f * h + 2.0f * T.rc, half = 0.5f * h + pad;
+    const f3 ctr = F3(T.ox + ((float)(rq - jq * nx) + 0.5f) * h, T.oy + ((float)jq + 0.5f) * h, T.oz + ((float)kq + 0.5f) * h);
+    const float diag = 1.7320508f * half;  // half-diagonal of the grown box
     for (uint32_t q = 0; q < T.obj_count && !near; q++) {
       const SlObj &o = T.objs[q];
-      if (o.angv[3] != 1.0f && o.angv[3] != 2.0f) continue;
-      const float cr = (o.r0[3] + (o.angv[3] == 1.0f ? T.rc : 0.0f)) * 1.0002f + 1e-4f;
+      const bool obst = o.angv[3] == 1.0f;
+      if (!obst && o.angv[3] != 2.0f) continue;
+      if (!(o.r0[3] < INFINITY)) { near = true; break; }
+      const float reach = obst ? T.rc : 0.0f;
+      const float cr = (o.r0[3] + reach) * 1.0002f + 1e-4f;
       if (cr < 0.0f) continue;
-      const float dx = fmaxf(fmaxf(lo.x - o.pos[0], o.pos[0] - hi.x), 0.0f);
-      const float dy = fmaxf(fmaxf(lo.y - o.pos[1], o.pos[1] - hi.y), 0.0f);
-      const float dz = fmaxf(fmaxf(lo.z - o.pos[2], o.pos[2] - hi.z), 0.0f);
+      const float dx = fmaxf(fabsf(ctr.x - o.pos[0]) - half, 0.0f);
+      const float dy = fmaxf(fabsf(ctr.y - o.pos[1]) - half, 0.0f);
+      const float dz = fmaxf(fabsf(ctr.z - o.pos[2]) - half, 0.0f);
       const float rr = cr + 1e-3f;
-      near = dx * dx + dy * dy + dz * dz <= rr * rr;
+      if (dx * dx + dy * dy + dz * dz > rr * rr) continue;
+      const float sd = obst ? shape_sdf(o, ctr) : sd_cyl(to_local(o, ctr), o.dims[0], o.dims[1]);
+      near = !(sd > reach + diag + 1e-3f);
     }
   }
   // the wave's 64 cells start at a multiple of 64: lanes 0 and 32 store its two words
