@@ -1706,9 +1706,7 @@ __global__ void k_batch_advance(SlTank *tanks, const uint32_t *list, int n, doub
   }
 }
 __global__ __launch_bounds__(SL_WG) void k_batch_clear_cells(SlWork W) {
-  int t = 0;
-  uint32_t lb = blockIdx.x;
-  if (W.ntanks > 1) { t = tank_of(W.start, W.ntanks, lb); lb -= W.start[t]; }
+  SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = W.tanks[t];
   uint32_t c = lb * SL_WG + threadIdx.x;
   if (c < T.cells) T.cellCount[c] = 0u;
@@ -1742,6 +1740,11 @@ struct sl_batch {
   } maps, emit;
   // maps block: BM_COUNT prefix tables of cap+1 entries, then the stepped-slot list (cap entries)
   std::vector<uint32_t> maps_host;
+  // the maps resolved per workgroup (k_wtab_fill after every maps upload): map w's entries start at
+  // wtab + wtab_off[w]
+  uint2 *wtab = nullptr;
+  size_t wtab_cap = 0;
+  uint64_t wtab_off[BM_COUNT] = {};
   uint32_t total[BM_COUNT] = {};
   int map_variant[BM_COUNT];
   uint32_t n_stepped = 0, max_iters = 0;
@@ -1819,6 +1822,7 @@ static SlLaunch batch_launch(const sl_batch *b, int which) {
   L.w.ntanks = (int32_t)b->cap;
   L.w.total = b->total[which];
   L.w.list = nullptr;
+  L.w.wtab = b->total[which] ? b->wtab + b->wtab_off[which] : nullptr;
   L.stream = b->stream;
   return L;
 }
@@ -1849,10 +1853,41 @@ static int put_map(sl_batch *b, int which, Items items, uint32_t per_wg, Inc inc
   b->total[which] = (uint32_t)acc;
   return SL_OK;
 }
+// workgroup b of a map: its table slot and its index among the slot's workgroups
+__global__ __launch_bounds__(SL_WG) void k_wtab_fill(const uint32_t *start, int ntanks, uint2 *out, uint32_t total) {
+  const uint32_t b = blockIdx.x * SL_WG + threadIdx.x;
+  if (b >= total) return;
+  const int t = tank_of(start, ntanks, b);
+  out[b] = make_uint2((uint32_t)t, b - start[t]);
+}
+static void drop_graph(sl_batch *b);
+// the maps block, then every map resolved per workgroup (both stream-ordered after the kernels of
+// earlier frames, which keep reading the previous contents)
 static int maps_upload(sl_batch *b) {
   const size_t bytes = b->maps_host.size() * 4;
   memcpy(ring_host(b->maps), b->maps_host.data(), bytes);
-  return ring_push(b, b->maps, bytes);
+  int r = ring_push(b, b->maps, bytes);
+  if (r) return r;
+  uint64_t need = 0;
+  for (int w = 0; w < BM_COUNT; w++) { b->wtab_off[w] = need; need += b->total[w]; }
+  if (need > b->wtab_cap) {
+    // a larger table: the kernels still in flight read the old one, and a captured frame holds it
+    if (hipStreamSynchronize(b->stream) != hipSuccess) return batch_fail(b, SL_EHIP, "batch: work tables");
+    drop_graph(b);
+    if (b->wtab) (void)hipFree(b->wtab);
+    b->wtab = nullptr;
+    b->wtab_cap = 0;
+    const size_t cap = (size_t)(need + need / 4 + 4096);
+    if (hipMalloc((void **)&b->wtab, cap * sizeof(uint2)) != hipSuccess)
+      return batch_fail(b, SL_ENOMEM, "batch: no device memory for the work tables");
+    b->wtab_cap = cap;
+  }
+  for (int w = 0; w < BM_COUNT; w++)
+    if (b->total[w])
+      hipLaunchKernelGGL(k_wtab_fill, dim3(cdiv(b->total[w], SL_WG)), dim3(SL_WG), 0, b->stream, map_dev(b, w),
+                         (int)b->cap, b->wtab + b->wtab_off[w], b->total[w]);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? SL_OK : batch_fail(b, SL_EHIP, std::string("batch: work tables: ") + hipGetErrorString(e));
 }
 static void drop_graph(sl_batch *b) {
   if (b->exec) { (void)hipGraphExecDestroy(b->exec); b->exec = nullptr; }
@@ -1977,6 +2012,7 @@ extern "C" void sl_batch_destroy(sl_batch *b) {
   if (b->ro_keys) (void)hipFree(b->ro_keys);
   if (b->ro_vals) (void)hipFree(b->ro_vals);
   if (b->ro_temp) (void)hipFree(b->ro_temp);
+  if (b->wtab) (void)hipFree(b->wtab);
   ring_free(b->maps);
   ring_free(b->emit);
   if (b->h_counters) (void)hipHostFree(b->h_counters);

@@ -5,10 +5,6 @@
 #include "sl_internal.h"
 
 // Resolve (tank, local workgroup) of workgroup b of a launch.
-#define SL_TANK_OF(W, b, t, lb)                               \
-  int t = 0;                                                  \
-  uint32_t lb = (b);                                          \
-  if ((W).ntanks > 1) { t = tank_of((W).start, (W).ntanks, b); lb = (b) - (W).start[t]; }
 
 __global__ void k_fill_u32(uint32_t *p, uint32_t v, size_t n) {
   size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;

@@ -2,10 +2,6 @@
 // whitewater spawning (G2P), emission (EMIT), counter commits (COUNT), compaction, whitewater.
 #include "sl_internal.h"
 
-#define SL_TANK_OF(W, b, t, lb)                               \
-  int t = 0;                                                  \
-  uint32_t lb = (b);                                          \
-  if ((W).ntanks > 1) { t = tank_of((W).start, (W).ntanks, b); lb = (b) - (W).start[t]; }
 
 // ------------------------------------------------------------------------------------------------
 // particle count of an in-grid cell

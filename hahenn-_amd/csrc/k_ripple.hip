@@ -8,10 +8,6 @@
 // folded into the loads and stores of the transforms they feed.
 #include "sl_internal.h"
 
-#define SL_TANK_OF(W, b, t, lb)                               \
-  int t = 0;                                                  \
-  uint32_t lb = (b);                                          \
-  if ((W).ntanks > 1) { t = tank_of((W).start, (W).ntanks, b); lb = (b) - (W).start[t]; }
 
 #define FFT_LDS_COMPLEX 4096  // complex values per LDS buffer (two buffers = 64 KiB)
 #define FFT_LOADS (FFT_LDS_COMPLEX / 256)  // loads per thread of a 256-thread FFT workgroup, at most

@@ -11,10 +11,6 @@
 #include <rocprim/rocprim.hpp>
 #include "sl_internal.h"
 
-#define SL_TANK_OF(W, b, t, lb)                               \
-  int t = 0;                                                  \
-  uint32_t lb = (b);                                          \
-  if ((W).ntanks > 1) { t = tank_of((W).start, (W).ntanks, b); lb = (b) - (W).start[t]; }
 
 // key: the cell index (the sentinel outside the grid) plus `key_hi`; value: the index plus `val_off`
 // (a batch sorts several tanks in one pass, each tank's rank in the high key bits)

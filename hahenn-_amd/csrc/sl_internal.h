@@ -49,14 +49,21 @@ struct SlEmitDev { float a[4], b[4], c[4], d[4]; uint32_t info[4]; };
 struct SlTank {
   // sizes
   int32_t nx, ny, nz, hw;
-  uint32_t nodes, cells, hcells;  // hcells = hw*ny*nz
-  uint32_t cap, ww_cap, R, M, logM, iters, obj_count, frame, emit_k, emit_total, cur, ww_cur, wmap;
+  uint32_t nodes, cells, hcells, iters;  // hcells = hw*ny*nz
+  // division (udiv_mag) by hw and hw*ny
+  uint32_t hw_mag[3], rowl_mag[3];
+  // the rest of what a SOR pass reads: with the sizes above, the record's first 88 bytes (two
+  // scalar-cache lines; a batch's passes read one record per tank)
+  float omega, omega1;
+  float *pres, *rhs;
+  uint16_t *sflag;
+  uint32_t cap, ww_cap, R, M, logM, obj_count, frame, emit_k, emit_total, cur, ww_cur, wmap;
   int32_t mask_bound, whitewater_on, top_height;
-  // division (udiv_mag) by hw, hw*ny, nx, nx*ny, nx+1, (nx+1)*(ny+1)
-  uint32_t hw_mag[3], rowl_mag[3], nx_mag[3], nxy_mag[3], nx1_mag[3], nxy1_mag[3];
+  // division (udiv_mag) by nx, nx*ny, nx+1, (nx+1)*(ny+1)
+  uint32_t nx_mag[3], nxy_mag[3], nx1_mag[3], nxy1_mag[3];
   float tankH;  // scene tank height (the renderer's F.tank.y)
   // params as the reference uniform (f32)
-  float ox, oy, oz, h, invh, hh, dt, g, gdt, flip, ppc, drift, omega, omega1, rc, rim;
+  float ox, oy, oz, h, invh, hh, dt, g, gdt, flip, ppc, drift, rc, rim;
   float ww_rate, ww_cap_f, friction, time_f, radius, R_f;
   // ripple uniform (64-ripple.js:223-226)
   float rLx, rLz, rdt, rmd, rsr, rnu, rfilm, rhpk, rrho;
@@ -84,8 +91,7 @@ struct SlTank {
   // velOld carries P2G's mask: bits SL_MASKED_OUT where the node had no weight (NORMALIZE sets velOld
   // to 0 there and transfer1 skips the node), so G2P takes a corner's mask from its velOld load
   float *vel, *velOld, *frac, *svel;
-  float *divg, *pres, *rhs;
-  uint16_t *sflag;
+  float *divg;
   SlObj *objs;
   // one bit per cell (x fastest): some object's cull sphere may reach a particle that starts in the
   // cell and moves at most one cell in a substep (k_objmask); G2P skips its object loops elsewhere
@@ -111,6 +117,9 @@ struct SlWork {
   // per-tank kernels (one thread or workgroup per tank): the table slots to process, in order;
   // nullptr means slots 0 .. ntanks-1
   const uint32_t *list = nullptr;
+  // per workgroup of the launch: (table slot, workgroup index within the slot), the prefix table
+  // resolved once per map on the device (nullptr: resolve by binary search over `start`)
+  const uint2 *wtab = nullptr;
 };
 
 #ifdef __HIPCC__
@@ -233,6 +242,20 @@ __device__ __forceinline__ int tank_of(const uint32_t *start, int ntanks, uint32
   }
   return lo;
 }
+// The table slot t of workgroup b of a launch over work map W, and b's index lb among t's workgroups:
+// one scalar load from the map's resolved table, else a binary search over the prefix table.
+#define SL_TANK_OF(W, b, t, lb)                                         \
+  int t = 0;                                                            \
+  uint32_t lb = (b);                                                    \
+  if ((W).wtab) {                                                       \
+    const uint2 e_ = (W).wtab[b];                                       \
+    t = (int)e_.x;                                                      \
+    lb = e_.y;                                                          \
+  } else if ((W).ntanks > 1) {                                          \
+    t = tank_of((W).start, (W).ntanks, b);                              \
+    lb = (b) - (W).start[t];                                            \
+  }
+
 // Workgroup i is dispatched to XCD i % 8. Re-number workgroups so that each XCD works on one
 // contiguous range (its L2 then holds the halos its neighbours in that range re-read).
 __device__ __forceinline__ uint32_t xcd_remap(uint32_t i, uint32_t G) {
