@@ -13,7 +13,7 @@ import numpy as np
 import pytest
 
 import twin
-from twin import assert_close, suiri
+from twin import assert_close, suiri, ww_multiset
 
 workloads = importlib.import_module("hahenn-_amd.workloads")
 
@@ -25,11 +25,6 @@ DT = 1 / 240
 LAYER = dict(ripple_depth=0.1, r_sigma_rho=0.072 / 1000.0, r_rho_kick=1000.0, splash_rho=1000.0,
              splash_sigma=0.072, splash_mu=0.001, splash_k_jet=60.0, droplets=4096, drop_seed=2024,
              drop_d50=0.0015, drop_sigma_ln=0.4, drop_vmin=1.0, drop_vmax=4.0)
-
-
-def ww_multiset(a):
-    a = np.ascontiguousarray(a, np.float32)
-    return a[np.lexsort(a.view(np.uint32).T[::-1])].view(np.uint32)
 
 
 def per_component(name, got, want, tol):

@@ -9,7 +9,7 @@ import numpy as np
 import pytest
 
 import twin
-from twin import assert_close, scene, suiri
+from twin import assert_close, scene, suiri, ww_multiset
 
 pytestmark = pytest.mark.gpu
 
@@ -18,12 +18,6 @@ TOL = 1e-5
 
 def live(parts, n):
     return parts[: int(n)]
-
-
-def ww_multiset(a):
-    """Whitewater records sorted by their bit patterns (a multiset compared bit for bit)."""
-    a = np.ascontiguousarray(a, np.float32)
-    return a[np.lexsort(a.view(np.uint32).T[::-1])].view(np.uint32)
 
 
 def test_emission_is_bitwise():

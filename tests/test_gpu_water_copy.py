@@ -14,18 +14,13 @@ import numpy as np
 import pytest
 
 import twin
-from twin import assert_close, suiri
+from twin import assert_close, suiri, ww_multiset
 
 scene = importlib.import_module("hahenn-_amd.scene")
 
 pytestmark = pytest.mark.gpu
 
 DT = 1 / 120
-
-
-def ww_multiset(a):
-    a = np.ascontiguousarray(a, np.float32)
-    return a[np.lexsort(a.view(np.uint32).T[::-1])].view(np.uint32)
 
 
 def source_items(o):

@@ -105,3 +105,61 @@ def test_faucet_256x128x128_two_steps_against_the_oracle():
     o.kernel("p2g")
     assert np.array_equal(g.readback(suiri.F_ACC), o.field("acc"))
     assert np.array_equal(g.readback(suiri.F_CELLCOUNT), o.field("cellCount"))
+
+
+def test_batch_64_tank_mix_against_the_oracle():
+    """The batched bench's 64 tanks (12.3 M cells, 46 M particles, grids 32^3 .. 128x64x64, 40 .. 80
+    SOR iterations, ripple M = 256 and whitewater on, re-orderings staggered over 16 frames: the
+    tanks at phases 13 .. 15 re-order inside the three frames) stepped as one batch, each tank against
+    its own oracle twin, bit for bit: counts, cell counts, particles, grid velocities, pressure,
+    ripple state and the whitewater multiset. The oracle polls its count after the fill, as the
+    batch's ripple depth reads the device count at frame start."""
+    mix = workloads.tank_mix(64, seed=7)
+    dt = 1.0 / 120
+    frames = 3
+    print("building the 64-tank batch", flush=True)
+    b = suiri.Batch([suiri.config("standard", **wl["cfg"]) for wl in mix])
+    for t, wl in zip(b.tanks, mix):
+        workloads.setup(t, wl)
+    b.reorder()
+    for f in range(frames):
+        print(f"batch step {f}", flush=True)
+        b.step(dt)
+    b.sync()
+    qkeys = {f for f, _ in twin.O.OrQuality._fields_}
+
+    def u32(a):
+        return np.ascontiguousarray(a).view(np.uint32)
+
+    total = 0
+    for i, (t, wl) in enumerate(zip(b.tanks, mix)):
+        c = wl["cfg"]
+        o = twin.O.Oracle((c["tank_w"], c["tank_h"], c["tank_d"]),
+                          twin.O.quality("standard", **{k: v for k, v in c.items() if k in qkeys}), threads=16)
+        rows, _ = twin.scene.Scene(t.cfg).pack(wl["items"], 0)
+        o.set_objects(rows)
+        inf = o.info()
+        assert (inf["nx"], inf["ny"], inf["nz"]) == (t.nx, t.ny, t.nz) == wl["grid"]
+        o.emit_now([twin.scene.cell_block_emit(inf["h"], [inf["ox"], inf["oy"], inf["oz"]], *blk, seed=wl["seed"] + k)
+                    for k, blk in enumerate(wl["blocks"])])
+        o.L.or_poll_stats(o.ptr)
+        o.L.or_reorder(o.ptr)
+        for f in range(frames):
+            twin.oracle_step(o, dt)
+        label = f"tank {i} {wl['name']}"
+        st = t.stats()
+        n = int(o.field("counters")[0])
+        assert st["count"] == n and st["frames"] == frames, label
+        assert np.array_equal(t.readback(suiri.F_CELLCOUNT), o.field("cellCount")), label
+        assert np.array_equal(u32(t.readback(suiri.F_PARTICLES)[:n]), u32(o.field("parts")[:n])), f"{label}: particles"
+        assert np.array_equal(u32(t.readback(suiri.F_VEL)[:, :3]), u32(o.field("vel")[:, :3])), f"{label}: vel"
+        assert np.array_equal(u32(t.readback(suiri.F_PRES)), u32(o.field("pres"))), f"{label}: pres"
+        assert np.array_equal(u32(t.readback(suiri.F_RSTATE)), u32(o.field("rstate"))), f"{label}: rstate"
+        w = int(o.info()["ww_count"])
+        assert st["ww"] == w, label
+        assert np.array_equal(twin.ww_multiset(t.readback(suiri.F_WW)[:w]), twin.ww_multiset(o.field("ww")[:w])), label
+        total += n
+        o.close()
+    print(f"64 tanks, {total} particles after {frames} frames: every tank bit-identical to its oracle twin",
+          flush=True)
+    b.close()

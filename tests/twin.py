@@ -61,6 +61,13 @@ def gpu_step(g, frame_dt, records=()):
     g.step(frame_dt, (lambda dt, t: list(records)) if records else None)
 
 
+def ww_multiset(a):
+    """Whitewater records sorted by their bit patterns (a multiset compared bit for bit: survivors are
+    appended with atomics, so their order is not fixed)."""
+    a = np.ascontiguousarray(a, np.float32)
+    return a[np.lexsort(a.view(np.uint32).T[::-1])].view(np.uint32)
+
+
 def rel_err(a, b):
     a = np.asarray(a, np.float64)
     b = np.asarray(b, np.float64)
