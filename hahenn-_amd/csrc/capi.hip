@@ -814,8 +814,8 @@ static void encode_pressure(sl_sim *s) {
   }
   {
     StageScope sc(s, SL_T_SOR, 2 * s->iters);
-    SlLaunch L = launch_of(s, s->hcells, SL_SOR_WG);
-    for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor(L, 0, 2 * i); sl_k_sor(L, 1, 2 * i + 1); }
+    const SlSorRec R = sl_sor_rec(s->ht);
+    for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor_rec(s->stream, R, 0); sl_k_sor_rec(s->stream, R, 1); }
   }
   StageScope sc(s, SL_T_PROJECT, 1);
   sl_k_project(launch_of(s, s->nodes));
@@ -1498,7 +1498,7 @@ extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
       break;
     case SL_S_NORMFORCE: sl_k_normforce(launch_of(s, s->nodes)); break;
     case SL_S_DIVERGENCE: sl_k_divsetup(launch_of(s, s->cells)); break;
-    case SL_S_PRESSURE: sl_k_sor(launch_of(s, s->hcells, SL_SOR_WG), arg & 1u, 0); break;
+    case SL_S_PRESSURE: sl_k_sor_rec(s->stream, sl_sor_rec(s->ht), arg & 1u); break;
     case SL_S_SOR:
       if (s->solver == SL_SOLVER_MGPCG) {
         sl_mg_solve(s->stream, s->d_tank, s->mg, s->mg_seed ? 1 : 0);
@@ -1506,8 +1506,8 @@ extern "C" int sl_run_stage(sl_sim *s, int stage, uint32_t arg, double dt) {
         break;
       }
       {
-        SlLaunch L = launch_of(s, s->hcells, SL_SOR_WG);
-        for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor(L, 0, 2 * i); sl_k_sor(L, 1, 2 * i + 1); }
+        const SlSorRec R = sl_sor_rec(s->ht);
+        for (uint32_t i = 0; i < s->iters; i++) { sl_k_sor_rec(s->stream, R, 0); sl_k_sor_rec(s->stream, R, 1); }
       }
       break;
     case SL_S_PROJECT:

@@ -398,6 +398,19 @@ void sl_k_p2g_binned(const SlLaunch &parts, const SlLaunch &blocks, const SlLaun
 void sl_k_normforce(const SlLaunch &L);
 void sl_k_divsetup(const SlLaunch &L);
 void sl_k_sor(const SlLaunch &L, uint32_t parity, uint32_t pass);
+// what a SOR colour pass of one tank reads, passed by value (the single-tank path)
+struct SlSorRec {
+  float *pres;
+  const float *rhs;
+  const uint16_t *sflag;
+  const float *frac;
+  int32_t nx, ny, hw;
+  uint32_t nodes, hcells;
+  uint32_t hw_mag[3], rowl_mag[3];
+  float omega, omega1;
+};
+SlSorRec sl_sor_rec(const SlTank &T);  // from the host mirror (its pointers are device addresses)
+void sl_k_sor_rec(hipStream_t stream, const SlSorRec &R, uint32_t parity);
 void sl_k_project(const SlLaunch &L);
 
 // Converged pressure solve (k_mgpcg.hip): multigrid-preconditioned conjugate gradients on the system
