@@ -219,6 +219,14 @@ static SlLaunch launch_of(sl_sim *s, uint64_t items, uint32_t per_wg = SL_WG) {
   L.w.total = cdiv(items, per_wg);
   L.w.list = nullptr;
   L.stream = s->stream;
+  // a simulation of its own: the host mirror equals the device record (push_dyn writes both, and
+  // the host flips the buffer indices where it enqueues their flips); a tank of a batch is advanced
+  // by the batch's kernels on the device, so its launches read the table
+  if (!s->ext_tank) {
+    s->ht.cur = s->cur;
+    s->ht.ww_cur = s->ww_cur;
+    L.one = &s->ht;
+  }
   return L;
 }
 
@@ -242,12 +250,16 @@ static int push_dyn(sl_sim *s, uint32_t emit_k = 0, uint32_t emit_total = 0) {
   d.iters = s->iters; d.mask_bound = s->mask_bound;
   d.time_d = s->time;
   d.dt_frame = s->batch_dt;
+  // the host mirror keeps the same values: single-simulation launches pass it by value
+  s->ht.frame = d.frame; s->ht.emit_k = emit_k; s->ht.emit_total = emit_total;
+  s->ht.iters = d.iters; s->ht.mask_bound = d.mask_bound; s->ht.time_d = d.time_d; s->ht.dt_frame = d.dt_frame;
   hipLaunchKernelGGL(k_update, dim3(1), dim3(64), 0, s->stream, s->d_tank, d);
   return check_launch(s, "update");
 }
 
-// The particle buffer index lives on the device (compaction flips it only when a particle died):
-// host code that copies particle buffers reads it at a synchronisation point.
+// The particle buffer index flips on the device when a re-ordering's gather makes the other buffer
+// current; the host flips its copy where it enqueues that flip. Code that copies particle buffers
+// reads the device copy at a synchronisation point.
 static int dev_cur(sl_sim *s) {
   int32_t c = 0;
   HIPCHK(s, hipStreamSynchronize(s->stream));
@@ -950,6 +962,7 @@ static int encode_reorder(sl_sim *s) {
                        s->sort_end_bit);
   if (e) { s->gpu_errors++; FAIL(s, SL_EHIP, "particle re-ordering failed: %s", hipGetErrorString((hipError_t)e)); }
   sl_k_count(launch_of(s, 1), 6, 0);
+  s->cur ^= 1;  // the gathered buffer becomes current
   return check_launch(s, "reorder");
 }
 
@@ -2322,6 +2335,7 @@ static int batch_sort_group(sl_batch *b, const uint32_t *ts, size_t k, uint64_t 
     sl_sim *s = b->sims[ts[r]];
     sl_k_sort_gather(launch_of(s, 1), s->p_ub, b->ro_vals + ntot, off);
     sl_k_count(launch_of(s, 1), 6, 0);  // the gathered buffer becomes current
+    s->cur ^= 1;
     if (restart) s->reorder_clock = 0;
     off += s->p_ub;
   }

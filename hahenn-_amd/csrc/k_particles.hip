@@ -394,10 +394,10 @@ __device__ __forceinline__ uint32_t g2p_one(const SlTank &T, uint32_t id) {
 // folds into the counter: tens of thousands of workgroups on one address serialise there. MINW is
 // the occupancy target handed to the register allocator (waves per SIMD); the variants compute
 // identical results.
-template <int MINW>
-__global__ __launch_bounds__(SL_WG, MINW) void k_g2p(SlWork W) {
+template <int MINW, class A>
+__global__ __launch_bounds__(SL_WG, MINW) void k_g2p(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   uint32_t id = lb * SL_WG + threadIdx.x;
   uint32_t vm = id < T.C->count ? g2p_one(T, id) : 0u;
   for (int o = 32; o > 0; o >>= 1) vm = max(vm, (uint32_t)__shfl_xor((int)vm, o, 64));
@@ -410,13 +410,24 @@ __global__ __launch_bounds__(SL_WG, MINW) void k_g2p(SlWork W) {
     if (m) atomicMax(&T.vslot[(blockIdx.x % SL_VSLOTS) * SL_VSLOT_STRIDE], m);
   }
 }
+template <int MINW>
+static void launch_g2p(const SlLaunch &L) {
+  if (L.one) {
+    SlOne o;
+    o.T = *L.one;
+    o.total = L.w.total;
+    hipLaunchKernelGGL((k_g2p<MINW, SlOne>), dim3(L.w.total), dim3(SL_WG), 0, L.stream, o);
+  } else {
+    hipLaunchKernelGGL((k_g2p<MINW, SlWork>), dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  }
+}
 void sl_k_g2p(const SlLaunch &L, int variant) {
   if (!L.w.total) return;
   switch (variant & 3) {
-    case 0: hipLaunchKernelGGL(k_g2p<1>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w); break;
-    case 1: hipLaunchKernelGGL(k_g2p<5>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w); break;
-    case 2: hipLaunchKernelGGL(k_g2p<6>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w); break;
-    default: hipLaunchKernelGGL(k_g2p<8>, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w); break;
+    case 0: launch_g2p<1>(L); break;
+    case 1: launch_g2p<5>(L); break;
+    case 2: launch_g2p<6>(L); break;
+    default: launch_g2p<8>(L); break;
   }
 }
 

@@ -122,7 +122,21 @@ struct SlWork {
   const uint2 *wtab = nullptr;
 };
 
+// One tank by value in the kernel arguments (a launch for a single simulation): a kernel reads its
+// fields one scalar-load level from the kernel arguments, against two (the table pointer, then the
+// record) for a tank of the batch table. The members SL_TANK_OF reads are constants, so the lookup
+// folds away.
+struct SlOne {
+  SlTank T;
+  uint32_t total;
+  static constexpr const uint2 *wtab = nullptr;
+  static constexpr const uint32_t *start = nullptr;
+  static constexpr int32_t ntanks = 1;
+};
+
 #ifdef __HIPCC__
+__device__ __forceinline__ const SlTank &sl_tank(const SlOne &W, int) { return W.T; }
+__device__ __forceinline__ const SlTank &sl_tank(const SlWork &W, int t) { return W.tanks[t]; }
 // ----------------------------------------------------------------------------------------------
 // device helpers with WGSL semantics
 // ----------------------------------------------------------------------------------------------
@@ -378,7 +392,23 @@ __device__ __forceinline__ bool in_drain(const SlTank &T, f3 p) {
 struct SlLaunch {
   SlWork w;         // tanks + work map
   hipStream_t stream;
+  // a single simulation's host mirror of its record, equal to the device record for every launch
+  // made through this SlLaunch: the kernels that SL_LAUNCH starts take it by value (SlOne)
+  const SlTank *one = nullptr;
 };
+// launches kernel template K (a template over its work argument: SlWork or SlOne) over L, with the
+// tank by value when L carries one
+#define SL_LAUNCH(K, grid, block, lds, L, ...)                                                   \
+  do {                                                                                            \
+    if ((L).one) {                                                                                \
+      SlOne o_;                                                                                   \
+      o_.T = *(L).one;                                                                            \
+      o_.total = (L).w.total;                                                                     \
+      hipLaunchKernelGGL(K<SlOne>, grid, block, lds, (L).stream, o_, ##__VA_ARGS__);              \
+    } else {                                                                                      \
+      hipLaunchKernelGGL(K<SlWork>, grid, block, lds, (L).stream, (L).w, ##__VA_ARGS__);          \
+    }                                                                                             \
+  } while (0)
 // grid stages (work map in units of SL_WG-wide workgroups over the stage's index space)
 void sl_k_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
 void sl_k_geom(const SlLaunch &L);
