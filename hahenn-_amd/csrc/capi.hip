@@ -931,8 +931,9 @@ static int encode_ripple(sl_sim *s) {
   StageScope sc(s, SL_T_RIPPLE, s->mask_bound ? 10 : 6);
   const uint32_t M = s->M, R1 = s->R + 1;
   uint32_t CW = 4096 / M; if (CW > M) CW = M; if (CW < 1) CW = 1;
-  sl_k_ripple_frame_ex(launch_of(s, M, 1).w, launch_of(s, M, CW).w, launch_of(s, (uint64_t)M * M).w,
-                       launch_of(s, (uint64_t)R1 * R1).w, s->stream, s->mask_bound, M);
+  const SlLaunch rows = launch_of(s, M, 1);
+  sl_k_ripple_frame_ex(rows.w, launch_of(s, M, CW).w, launch_of(s, (uint64_t)M * M).w,
+                       launch_of(s, (uint64_t)R1 * R1).w, s->stream, s->mask_bound, M, rows.one);
   return check_launch(s, "ripple");
 }
 
@@ -996,7 +997,8 @@ extern "C" int sl_step_end(sl_sim *s, double frame_dt) {
   if (s->ht.top_height) {  // drawn after the step, read by the next frame's ripple mask
     StageScope sc(s, SL_T_RIPPLE, 6);
     const uint32_t wm = s->ht.wmap;
-    sl_k_toph(launch_of(s, s->p_ub).w, launch_of(s, (uint64_t)wm * wm).w, s->stream);
+    const SlLaunch parts = launch_of(s, s->p_ub);
+    sl_k_toph(parts.w, launch_of(s, (uint64_t)wm * wm).w, s->stream, parts.one);
     if ((r = check_launch(s, "top height"))) return r;
   }
   s->frames++;

@@ -38,9 +38,10 @@ __device__ float face_open(const SlTank &T, f3 c, int axis) {
   return clampf(pos / tot, 0.0f, 1.0f);
 }
 
-__global__ __launch_bounds__(SL_WG) void k_geom(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_geom(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   uint32_t n = lb * SL_WG + threadIdx.x;
   if (n >= T.nodes) return;
   uint32_t sx = T.nx + 1, sy = T.ny + 1;
@@ -68,7 +69,7 @@ __global__ __launch_bounds__(SL_WG) void k_geom(SlWork W) {
 }
 void sl_k_geom(const SlLaunch &L) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_geom, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  SL_LAUNCH(k_geom, dim3(L.w.total), dim3(SL_WG), 0, L);
 }
 
 // Object mask for G2P's culling (no counterpart in the reference; it only skips work whose outcome
@@ -84,9 +85,10 @@ void sl_k_geom(const SlLaunch &L) {
 // the host verified orthonormal, which is also when it gives a finite cull radius -- so G2P's collision
 // test d < rc and its drain test sd < 0 fail at every test point. An object without a finite cull
 // radius sets every bit. Types other than obstacle (1) and drain (2) never matter to G2P.
-__global__ __launch_bounds__(SL_WG) void k_objmask(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_objmask(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t c = lb * SL_WG + threadIdx.x;
   const bool in = c < T.cells;
   bool near = false;
@@ -120,7 +122,7 @@ __global__ __launch_bounds__(SL_WG) void k_objmask(SlWork W) {
 }
 void sl_k_objmask(const SlLaunch &L) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_objmask, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  SL_LAUNCH(k_objmask, dim3(L.w.total), dim3(SL_WG), 0, L);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -208,12 +210,12 @@ __device__ __forceinline__ void splat_box(void *boxp, int32_t vol, int32_t i0, i
 
 #define P2G_LDS_BYTES (60 * 1024)
 
-template <int COUNT_ONLY, int CHUNK, bool LDS64, bool STRIDED>
-__global__ __launch_bounds__(SL_WG) void k_p2g(SlWork W) {
+template <int COUNT_ONLY, int CHUNK, bool LDS64, bool STRIDED, class A>
+__global__ __launch_bounds__(SL_WG) void k_p2g(A W) {
   constexpr int PER = CHUNK / SL_WG;
   constexpr int LDS_BYTES = CHUNK <= 512 ? 44 * 1024 : P2G_LDS_BYTES;
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   __shared__ unsigned long long box[LDS_BYTES / 8];
   __shared__ int32_t bb[6];
   const uint32_t cnt = T.C->count;
@@ -313,9 +315,20 @@ __global__ __launch_bounds__(SL_WG) void k_p2g(SlWork W) {
 // variant table: id -> (chunk, u64 LDS, strided)
 static const int kP2gChunk[8] = {512, 512, 1024, 1024, 2048, 2048, 1024, 512};
 int sl_p2g_chunk(int variant) { return kP2gChunk[variant & 7]; }
+template <int C, int CH, bool U, bool S>
+static void launch_p2g(const SlLaunch &L) {
+  if (L.one) {
+    SlOne o;
+    o.T = *L.one;
+    o.total = L.w.total;
+    hipLaunchKernelGGL((k_p2g<C, CH, U, S, SlOne>), dim3(L.w.total), dim3(SL_WG), 0, L.stream, o);
+  } else {
+    hipLaunchKernelGGL((k_p2g<C, CH, U, S, SlWork>), dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  }
+}
 void sl_k_p2g(const SlLaunch &L, int count_only, int variant) {
   if (!L.w.total) return;
-#define P2G_LAUNCH(C, CH, U, S) hipLaunchKernelGGL((k_p2g<C, CH, U, S>), dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w)
+#define P2G_LAUNCH(C, CH, U, S) launch_p2g<C, CH, U, S>(L)
   if (count_only) { P2G_LAUNCH(1, 512, false, false); return; }
   switch (variant & 7) {
     case 0: P2G_LAUNCH(0, 512, false, false); break;
@@ -350,9 +363,10 @@ void sl_k_p2g(const SlLaunch &L, int count_only, int variant) {
 // particle array) take their ranks with one atomic. A wavefront takes SL_BIN_CHUNKS consecutive runs
 // of 64 particles: their flag and position loads are issued together, and so are their atomics, so
 // the kernel waits on memory a few times per wavefront rather than a few times per 64 particles.
-__global__ __launch_bounds__(SL_WG) void k_bin_count(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_bin_count(A W) {
   SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t n = min(T.C->count, T.cap);
   const int cur = T.cur;
   const int lane = __lane_id();
@@ -403,9 +417,10 @@ __global__ __launch_bounds__(SL_WG) void k_bin_count(SlWork W) {
 
 // exclusive prefix of the cell counts inside each block of BIN_SCAN cells; the block total goes to
 // binTop[block] (256 threads x 16 consecutive cells)
-__global__ __launch_bounds__(256) void k_bin_scan(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(256) void k_bin_scan(A W) {
   SL_TANK_OF(W, blockIdx.x, t, blk);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t C = T.cells, base = blk * BIN_SCAN + threadIdx.x * 16u;
   uint32_t v[16], s = 0;
   if (base + 16u <= C) {
@@ -456,10 +471,11 @@ __global__ __launch_bounds__(256) void k_bin_scan(SlWork W) {
 }
 
 // exclusive scan of the block totals of one tank, in place (one workgroup per tank)
-__global__ __launch_bounds__(1024) void k_bin_top(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(1024) void k_bin_top(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   (void)lb;
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t nb = (T.cells + BIN_SCAN - 1) / BIN_SCAN;
   __shared__ uint32_t wsum[16];
   __shared__ uint32_t carry;
@@ -495,9 +511,10 @@ __global__ __launch_bounds__(1024) void k_bin_top(SlWork W) {
 
 // particle index -> its slot in cell order
 // SL_BIN_CHUNKS runs of 64 particles per wavefront, as k_bin_count
-__global__ __launch_bounds__(SL_WG) void k_bin_scatter(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_bin_scatter(A W) {
   SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t n = min(T.C->count, T.cap);
   const uint32_t base_id = lb * SL_BIN_PER_WG + (threadIdx.x >> 6) * (64u * SL_BIN_CHUNKS) + (uint32_t)__lane_id();
   uint32_t cell[SL_BIN_CHUNKS], rank[SL_BIN_CHUNKS];
@@ -580,9 +597,10 @@ __device__ __forceinline__ void p2g_cell(const SlTank &T, int cur, uint32_t star
 #define P2G_TILE_THREADS (BIN_TX * BIN_TY * BIN_TZ)  // one thread per cell of the tile
 static_assert(BIN_TX == 8 && BIN_TY == 8, "the tile kernel decodes a thread as 8 x 8 cells per z layer");
 
-__global__ __launch_bounds__(P2G_TILE_THREADS) void k_p2g_tiles(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(P2G_TILE_THREADS) void k_p2g_tiles(A W) {
   SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   __shared__ unsigned long long wmw[3 * WIN];
   __shared__ uint32_t qstart[P2G_QUEUE], qinfo[P2G_QUEUE], qn;
   for (int e = threadIdx.x; e < 3 * WIN; e += P2G_TILE_THREADS) wmw[e] = 0ull;
@@ -640,20 +658,21 @@ uint32_t sl_bin_tiles(int nx, int ny, int nz) {
 uint32_t sl_bin_blocks(uint32_t cells) { return (cells + BIN_SCAN - 1) / BIN_SCAN; }
 void sl_k_p2g_binned(const SlLaunch &parts, const SlLaunch &blocks, const SlLaunch &tanks, const SlLaunch &tiles) {
   if (!parts.w.total || !tiles.w.total) return;
-  hipLaunchKernelGGL(k_bin_count, dim3(parts.w.total), dim3(SL_WG), 0, parts.stream, parts.w);
-  hipLaunchKernelGGL(k_bin_scan, dim3(blocks.w.total), dim3(256), 0, parts.stream, blocks.w);
-  hipLaunchKernelGGL(k_bin_top, dim3(tanks.w.total), dim3(1024), 0, parts.stream, tanks.w);
-  hipLaunchKernelGGL(k_bin_scatter, dim3(parts.w.total), dim3(SL_WG), 0, parts.stream, parts.w);
-  hipLaunchKernelGGL(k_p2g_tiles, dim3(tiles.w.total), dim3(P2G_TILE_THREADS), 0, parts.stream, tiles.w);
+  SL_LAUNCH(k_bin_count, dim3(parts.w.total), dim3(SL_WG), 0, parts);
+  SL_LAUNCH(k_bin_scan, dim3(blocks.w.total), dim3(256), 0, blocks);
+  SL_LAUNCH(k_bin_top, dim3(tanks.w.total), dim3(1024), 0, tanks);
+  SL_LAUNCH(k_bin_scatter, dim3(parts.w.total), dim3(SL_WG), 0, parts);
+  SL_LAUNCH(k_p2g_tiles, dim3(tiles.w.total), dim3(P2G_TILE_THREADS), 0, tiles);
 }
 
 // ------------------------------------------------------------------------------------------------
 // NORMALIZE (30-wgsl-sim.js:203-222) fused with FORCES (30-wgsl-sim.js:224-238), per node.
 // The accumulators are cleared after they are read, ready for the next substep's P2G.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(SL_WG) void k_normforce(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_normforce(A W) {
   SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   uint32_t n = lb * SL_WG + threadIdx.x;
   if (n >= T.nodes) return;
   const size_t N = T.nodes;
@@ -678,7 +697,7 @@ __global__ __launch_bounds__(SL_WG) void k_normforce(SlWork W) {
 }
 void sl_k_normforce(const SlLaunch &L) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_normforce, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  SL_LAUNCH(k_normforce, dim3(L.w.total), dim3(SL_WG), 0, L);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -689,9 +708,10 @@ void sl_k_normforce(const SlLaunch &L) {
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ float flux(float f, float v, float s) { return f * v + (1.0f - f) * s; }
 
-__global__ __launch_bounds__(SL_WG) void k_divsetup(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_divsetup(A W) {
   SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   uint32_t c = lb * SL_WG + threadIdx.x;
   if (c >= T.cells) return;
   const int32_t nx = T.nx, ny = T.ny, nz = T.nz;
@@ -737,7 +757,7 @@ __global__ __launch_bounds__(SL_WG) void k_divsetup(SlWork W) {
 }
 void sl_k_divsetup(const SlLaunch &L) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_divsetup, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  SL_LAUNCH(k_divsetup, dim3(L.w.total), dim3(SL_WG), 0, L);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -795,7 +815,7 @@ __device__ __forceinline__ void sor_slot(const R &T, uint32_t lb, uint32_t parit
 // a batch: each workgroup finds its tank in the table (passes beyond a tank's own count return)
 __global__ __launch_bounds__(SL_SOR_WG) void k_sor(SlWork W, uint32_t parity, uint32_t pass) {
   SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   if (pass >= 2u * T.iters) return;
   sor_slot(T, lb, parity);
 }
@@ -834,9 +854,10 @@ __device__ __forceinline__ float cpres(const SlTank &T, int32_t i, int32_t j, in
   if (ct != 1u) return 0.0f;
   return T.pres[colour_base(T, i, j, k) + packed_idx(T, i, j, k)];
 }
-__global__ __launch_bounds__(SL_WG) void k_project(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_project(A W) {
   SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   uint32_t n = lb * SL_WG + threadIdx.x;
   if (n >= T.nodes) return;
   const uint32_t sxn = T.nx + 1, syn = T.ny + 1;
@@ -876,5 +897,5 @@ __global__ __launch_bounds__(SL_WG) void k_project(SlWork W) {
 }
 void sl_k_project(const SlLaunch &L) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_project, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  SL_LAUNCH(k_project, dim3(L.w.total), dim3(SL_WG), 0, L);
 }

@@ -234,9 +234,10 @@ __device__ __forceinline__ void droplet_draw(const SlTank &T, uint32_t d, float 
 // One wavefront per workgroup (64 droplets) so that a frame's 4096 droplets spread over 64 CUs; the
 // wavefront reserves its splashes' whitewater slots with one atomic (lane prefix sums of the counts)
 // instead of one atomic per droplet on the same counter.
-__global__ __launch_bounds__(64) void k_droplets(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(64) void k_droplets(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t d = lb * 64u + threadIdx.x;
   const bool on = d < T.drops;
   float x = 0.0f, z = 0.0f, D = 0.0f, v = 0.0f, m = 0.0f;
@@ -264,7 +265,7 @@ __global__ __launch_bounds__(64) void k_droplets(SlWork W) {
 }
 void sl_k_droplets(const SlLaunch &L) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_droplets, dim3(L.w.total), dim3(64), 0, L.stream, L.w);
+  SL_LAUNCH(k_droplets, dim3(L.w.total), dim3(64), 0, L);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -436,9 +437,10 @@ void sl_k_g2p(const SlLaunch &L, int variant) {
 // ------------------------------------------------------------------------------------------------
 // `reserve`: slots kept free below the capacity (0 here; a batch passes the 64 that the host-side
 // budget of writeEmits keeps, 50-sim.js:226, so that the budget is applied on the device count)
-__global__ __launch_bounds__(SL_WG) void k_emit(SlWork W, uint32_t reserve) {
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_emit(A W, uint32_t reserve) {
   SL_TANK_OF(W, blockIdx.x, tk, lb);
-  const SlTank &T = W.tanks[tk];
+  const SlTank &T = sl_tank(W, tk);
   uint32_t t = lb * SL_WG + threadIdx.x;
   if (t >= T.emit_total) return;
   uint32_t e = 0;
@@ -509,7 +511,7 @@ __global__ __launch_bounds__(SL_WG) void k_emit(SlWork W, uint32_t reserve) {
 }
 void sl_k_emit(const SlLaunch &L, uint32_t reserve) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_emit, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w, reserve);
+  SL_LAUNCH(k_emit, dim3(L.w.total), dim3(SL_WG), 0, L, reserve);
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -571,9 +573,10 @@ void sl_k_count(const SlLaunch &L, uint32_t mode, uint32_t y) {
 // particle list, unused at this point), (4) the survivors at or above n' into those slots. (3) and
 // (4) return at once when nothing died.
 // ------------------------------------------------------------------------------------------------
-__global__ __launch_bounds__(SL_WG) void k_compact_count(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_compact_count(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t lim = min(T.C->count, T.cap);
   const uint32_t base = lb * SL_COMPACT_ITEMS;
   const uint8_t *pf = T.pf[T.cur];
@@ -601,12 +604,13 @@ __global__ __launch_bounds__(SL_WG) void k_compact_count(SlWork W) {
 }
 void sl_k_compact_count(const SlLaunch &L) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_compact_count, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  SL_LAUNCH(k_compact_count, dim3(L.w.total), dim3(SL_WG), 0, L);
 }
 // one workgroup per tank scans its per-workgroup counts in place (exclusive) and sets newCount
-__global__ __launch_bounds__(1024) void k_compact_scan(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(1024) void k_compact_scan(A W) {
   const int t = W.list ? (int)W.list[blockIdx.x] : (int)blockIdx.x;
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t lim = min(T.C->count, T.cap);
   const uint32_t nb = (lim + SL_COMPACT_ITEMS - 1) / SL_COMPACT_ITEMS;
   __shared__ uint32_t wsum[16];
@@ -644,7 +648,7 @@ __global__ __launch_bounds__(1024) void k_compact_scan(SlWork W) {
 }
 void sl_k_compact_scan(const SlLaunch &L, int ntanks) {
   if (ntanks <= 0) return;
-  hipLaunchKernelGGL(k_compact_scan, dim3(ntanks), dim3(1024), 0, L.stream, L.w);
+  SL_LAUNCH(k_compact_scan, dim3(ntanks), dim3(1024), 0, L);
 }
 // live ranks inside a compaction workgroup: for each of its Q items per thread, the number of live
 // particles before it in the workgroup (positions base + q * SL_WG + thread, in position order)
@@ -673,9 +677,10 @@ __device__ __forceinline__ uint32_t compact_slot_L(const SlTank &T) {
   return (T.cap + SL_COMPACT_ITEMS - 1u) / SL_COMPACT_ITEMS;
 }
 // (3): dead slot p < n' is the (p - live before p)-th hole
-__global__ __launch_bounds__(SL_WG) void k_compact_holes(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_compact_holes(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t lim = min(T.C->count, T.cap), n1 = T.C->newCount;
   const uint32_t base = lb * SL_COMPACT_ITEMS;
   if (n1 >= lim || base > n1) return;  // nothing died, or no slot below n' here
@@ -702,9 +707,10 @@ __global__ __launch_bounds__(SL_WG) void k_compact_holes(SlWork W) {
   }
 }
 // (4): survivor p >= n' with r survivors in [n', p) moves into the r-th hole
-__global__ __launch_bounds__(SL_WG) void k_compact_fill(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_compact_fill(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t lim = min(T.C->count, T.cap), n1 = T.C->newCount;
   const uint32_t base = lb * SL_COMPACT_ITEMS;
   if (n1 >= lim || base >= lim || base + SL_COMPACT_ITEMS <= n1) return;
@@ -732,17 +738,18 @@ __global__ __launch_bounds__(SL_WG) void k_compact_fill(SlWork W) {
 }
 void sl_k_compact_move(const SlLaunch &L) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_compact_holes, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
-  hipLaunchKernelGGL(k_compact_fill, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w);
+  SL_LAUNCH(k_compact_holes, dim3(L.w.total), dim3(SL_WG), 0, L);
+  SL_LAUNCH(k_compact_fill, dim3(L.w.total), dim3(SL_WG), 0, L);
 }
 
 // ------------------------------------------------------------------------------------------------
 // WW, 31-wgsl-sim2.js:361-409: spray / foam / bubbles; survivors appended to the other buffer
 // ------------------------------------------------------------------------------------------------
 // dt = 0: the tank's substep dt (batched steps, where every tank runs one substep per frame)
-__global__ __launch_bounds__(SL_WG) void k_ww(SlWork W, float dt) {
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_ww(A W, float dt) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   if (!(dt > 0.0f)) dt = T.dt;
   const uint32_t wcap = f2u(T.ww_cap_f);
   uint32_t id = lb * SL_WG + threadIdx.x;
@@ -797,5 +804,5 @@ __global__ __launch_bounds__(SL_WG) void k_ww(SlWork W, float dt) {
 }
 void sl_k_ww(const SlLaunch &L, float dt) {
   if (!L.w.total) return;
-  hipLaunchKernelGGL(k_ww, dim3(L.w.total), dim3(SL_WG), 0, L.stream, L.w, dt);
+  SL_LAUNCH(k_ww, dim3(L.w.total), dim3(SL_WG), 0, L, dt);
 }

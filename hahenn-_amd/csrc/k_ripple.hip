@@ -76,10 +76,11 @@ __device__ __forceinline__ float2 fft_input(const SlTank &T, int mode, const flo
 }
 
 // all horizontal passes of one row; dynamic LDS of two buffers of the largest M in the launch
-__global__ __launch_bounds__(256) void k_fft_rows(SlWork W, int which_in, int which_out, int mode, float sign,
+template <class A>
+__global__ __launch_bounds__(256) void k_fft_rows(A W, int which_in, int which_out, int mode, float sign,
                                                   uint32_t ldsM) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t M = T.M, lM = T.logM, y = lb;
   if (y >= M) return;
   if (mode == IN_MASK && !T.mask_bound) return;  // tanks of a batch without the absorbing mask
@@ -124,11 +125,12 @@ __global__ __launch_bounds__(256) void k_fft_rows(SlWork W, int which_in, int wh
 }
 
 // all vertical passes of a tile of CW = 4096/M adjacent columns
-__global__ __launch_bounds__(FFT_COL_THREADS) void k_fft_cols(SlWork W, int which_in, int which_out, int omode,
+template <class A>
+__global__ __launch_bounds__(FFT_COL_THREADS) void k_fft_cols(A W, int which_in, int which_out, int omode,
                                                               float sign) {
   // adjacent column tiles share the 128-byte lines of every row: XCD-aware order keeps them in one L2
   SL_TANK_OF(W, xcd_remap(blockIdx.x, gridDim.x), t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t M = T.M, lM = T.logM;
   const uint32_t lCW = lM <= 6u ? lM : 12u - lM, CW = 1u << lCW;  // FFT_LDS_COMPLEX / M, at most M
   const uint32_t x0 = lb * CW;
@@ -192,9 +194,10 @@ __global__ __launch_bounds__(FFT_COL_THREADS) void k_fft_cols(SlWork W, int whic
 }
 
 // REVOLVE, 64-ripple.js:58-90
-__global__ __launch_bounds__(256) void k_revolve(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(256) void k_revolve(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t M = T.M, R = T.R;
   uint32_t id = lb * 256 + threadIdx.x;
   if (id >= M * M) return;
@@ -229,9 +232,10 @@ __global__ __launch_bounds__(256) void k_revolve(SlWork W) {
 }
 
 // REXTRACT, 64-ripple.js:134-147 (f32 values; the f16 texture is produced on readback)
-__global__ __launch_bounds__(256) void k_rextract(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(256) void k_rextract(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t M = T.M, R = T.R, R1 = R + 1;
   uint32_t id = lb * 256 + threadIdx.x;
   if (id >= R1 * R1) return;
@@ -246,9 +250,10 @@ __global__ __launch_bounds__(256) void k_rextract(SlWork W) {
 }
 
 // RMASKBUILD, 64-ripple.js:106-126
-__global__ __launch_bounds__(256) void k_rmaskbuild(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(256) void k_rmaskbuild(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   if (!T.mask_bound) return;
   const uint32_t R = T.R, R1 = R + 1;
   uint32_t id = lb * 256 + threadIdx.x;
@@ -271,37 +276,47 @@ __global__ __launch_bounds__(256) void k_rmaskbuild(SlWork W) {
   T.rmask[id] = m;
 }
 
-__global__ void k_clear_rsrc(SlWork W) {
+template <class A>
+__global__ void k_clear_rsrc(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   uint32_t id = lb * 256 + threadIdx.x;
   uint32_t R1 = T.R + 1;
   if (id < R1 * R1) T.rsrc[id] = 0;
 }
 
+static SlLaunch launch_over(const SlWork &w, hipStream_t s, const SlTank *one) {
+  SlLaunch L;
+  L.w = w;
+  L.stream = s;
+  L.one = one;
+  return L;
+}
 // buffers: 0 state, 1 a, 2 b, 3 hat
-void sl_k_ripple_frame_ex(const SlWork &rows, const SlWork &cols, const SlWork &elems, const SlWork &tex,
-                          hipStream_t s, int mask, uint32_t maxM) {
-  if (!rows.total) return;
+void sl_k_ripple_frame_ex(const SlWork &rows_w, const SlWork &cols_w, const SlWork &elems_w, const SlWork &tex_w,
+                          hipStream_t s, int mask, uint32_t maxM, const SlTank *one) {
+  if (!rows_w.total) return;
+  const SlLaunch rows = launch_over(rows_w, s, one), cols = launch_over(cols_w, s, one);
+  const SlLaunch elems = launch_over(elems_w, s, one), tex = launch_over(tex_w, s, one);
   const size_t rl = (size_t)3 * maxM * sizeof(float2);  // row LDS: two buffers of M points and the twiddles
   const size_t cl = (size_t)maxM * sizeof(float2);      // column LDS beside the static buffers: the twiddles
   // forward transform of the mirrored source -> hat (64-ripple.js:229-232)
-  hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), rl, s, rows, 0, 1, (int)IN_SRC, -1.0f, maxM);
-  hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(FFT_COL_THREADS), cl, s, cols, 1, 3, (int)OUT_BUF, -1.0f);
+  SL_LAUNCH(k_fft_rows, dim3(rows_w.total), dim3(256), rl, rows, 0, 1, (int)IN_SRC, -1.0f, maxM);
+  SL_LAUNCH(k_fft_cols, dim3(cols_w.total), dim3(FFT_COL_THREADS), cl, cols, 1, 3, (int)OUT_BUF, -1.0f);
   // evolve the modes and add the kick (64-ripple.js:233)
-  hipLaunchKernelGGL(k_revolve, dim3(elems.total), dim3(256), 0, s, elems);
+  SL_LAUNCH(k_revolve, dim3(elems_w.total), dim3(256), 0, elems);
   // inverse transform of the state -> b (64-ripple.js:234-235)
-  hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), rl, s, rows, 0, 1, (int)IN_BUF, 1.0f, maxM);
-  hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(FFT_COL_THREADS), cl, s, cols, 1, 2, (int)OUT_BUF, 1.0f);
+  SL_LAUNCH(k_fft_rows, dim3(rows_w.total), dim3(256), rl, rows, 0, 1, (int)IN_BUF, 1.0f, maxM);
+  SL_LAUNCH(k_fft_cols, dim3(cols_w.total), dim3(FFT_COL_THREADS), cl, cols, 1, 2, (int)OUT_BUF, 1.0f);
   // height and slopes (64-ripple.js:236-238)
-  hipLaunchKernelGGL(k_rextract, dim3(tex.total), dim3(256), 0, s, tex);
+  SL_LAUNCH(k_rextract, dim3(tex_w.total), dim3(256), 0, tex);
   if (mask) {
     // absorb: mask the spatial field, transform back into the state, rescale (64-ripple.js:239-250)
-    hipLaunchKernelGGL(k_rmaskbuild, dim3(tex.total), dim3(256), 0, s, tex);
-    hipLaunchKernelGGL(k_fft_rows, dim3(rows.total), dim3(256), rl, s, rows, 2, 1, (int)IN_MASK, -1.0f, maxM);
-    hipLaunchKernelGGL(k_fft_cols, dim3(cols.total), dim3(FFT_COL_THREADS), cl, s, cols, 1, 0, (int)OUT_SCALE, -1.0f);
+    SL_LAUNCH(k_rmaskbuild, dim3(tex_w.total), dim3(256), 0, tex);
+    SL_LAUNCH(k_fft_rows, dim3(rows_w.total), dim3(256), rl, rows, 2, 1, (int)IN_MASK, -1.0f, maxM);
+    SL_LAUNCH(k_fft_cols, dim3(cols_w.total), dim3(FFT_COL_THREADS), cl, cols, 1, 0, (int)OUT_SCALE, -1.0f);
   }
-  hipLaunchKernelGGL(k_clear_rsrc, dim3(tex.total), dim3(256), 0, s, tex);  // 64-ripple.js:252
+  SL_LAUNCH(k_clear_rsrc, dim3(tex_w.total), dim3(256), 0, tex);  // 64-ripple.js:252
 }
 
 // ------------------------------------------------------------------------------------------------
@@ -320,16 +335,18 @@ __device__ __forceinline__ uint32_t ord_key(float f) {
 __device__ __forceinline__ float ord_val(uint32_t k) {
   return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
 }
-__global__ __launch_bounds__(256) void k_toph_init(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(256) void k_toph_init(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t n = T.wmap * T.wmap, i = lb * 256 + threadIdx.x;
   if (!T.top_height || i >= n) return;
   ((uint32_t *)T.topH)[i] = ord_key(-1.0f);
 }
-__global__ __launch_bounds__(256) void k_toph_splat(SlWork W) {
+template <class A>
+__global__ __launch_bounds__(256) void k_toph_splat(A W) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const uint32_t id = lb * 256 + threadIdx.x;
   if (!T.top_height || id >= min(T.C->count, T.cap)) return;
   const int cur = T.cur;
@@ -356,9 +373,10 @@ __global__ __launch_bounds__(256) void k_toph_splat(SlWork W) {
     }
 }
 // one blur pass along (dx, dy) (63-wgsl-water.js:40-55); pass 0 reads the splat's integer image
-__global__ __launch_bounds__(256) void k_toph_blur(SlWork W, int pass) {
+template <class A>
+__global__ __launch_bounds__(256) void k_toph_blur(A W, int pass) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   const int Wm = (int)T.wmap;
   const uint32_t i = lb * 256 + threadIdx.x;
   if (!T.top_height || i >= (uint32_t)(Wm * Wm)) return;
@@ -383,9 +401,10 @@ __global__ __launch_bounds__(256) void k_toph_blur(SlWork W, int pass) {
   }
   dst[i] = sum / wsum;
 }
-void sl_k_toph(const SlWork &parts, const SlWork &texels, hipStream_t s) {
-  if (!texels.total) return;
-  hipLaunchKernelGGL(k_toph_init, dim3(texels.total), dim3(256), 0, s, texels);
-  if (parts.total) hipLaunchKernelGGL(k_toph_splat, dim3(parts.total), dim3(256), 0, s, parts);
-  for (int pass = 0; pass < 4; pass++) hipLaunchKernelGGL(k_toph_blur, dim3(texels.total), dim3(256), 0, s, texels, pass);
+void sl_k_toph(const SlWork &parts_w, const SlWork &texels_w, hipStream_t s, const SlTank *one) {
+  if (!texels_w.total) return;
+  const SlLaunch parts = launch_over(parts_w, s, one), texels = launch_over(texels_w, s, one);
+  SL_LAUNCH(k_toph_init, dim3(texels_w.total), dim3(256), 0, texels);
+  if (parts_w.total) SL_LAUNCH(k_toph_splat, dim3(parts_w.total), dim3(256), 0, parts);
+  for (int pass = 0; pass < 4; pass++) SL_LAUNCH(k_toph_blur, dim3(texels_w.total), dim3(256), 0, texels, pass);
 }

@@ -14,10 +14,11 @@
 
 // key: the cell index (the sentinel outside the grid) plus `key_hi`; value: the index plus `val_off`
 // (a batch sorts several tanks in one pass, each tank's rank in the high key bits)
-__global__ __launch_bounds__(SL_WG) void k_sort_keys(SlWork W, uint32_t *keys, uint32_t *vals, uint32_t n_ub,
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_sort_keys(A W, uint32_t *keys, uint32_t *vals, uint32_t n_ub,
                                                       uint32_t key_hi, uint32_t val_off) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   uint32_t id = lb * SL_WG + threadIdx.x;
   if (id >= n_ub) return;
   uint32_t key = sl_key_sentinel(T);
@@ -33,9 +34,10 @@ __global__ __launch_bounds__(SL_WG) void k_sort_keys(SlWork W, uint32_t *keys, u
   vals[id] = val_off + id;
 }
 
-__global__ __launch_bounds__(SL_WG) void k_sort_gather(SlWork W, const uint32_t *vals, uint32_t val_off) {
+template <class A>
+__global__ __launch_bounds__(SL_WG) void k_sort_gather(A W, const uint32_t *vals, uint32_t val_off) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = W.tanks[t];
+  const SlTank &T = sl_tank(W, t);
   uint32_t d = lb * SL_WG + threadIdx.x;
   if (d >= T.C->count) return;
   const int cur = T.cur, oth = 1 - cur;
@@ -59,12 +61,12 @@ int sl_k_reorder(const SlLaunch &L, uint32_t n_ub, uint32_t *keys, uint32_t *val
   if (!n_ub) return 0;
   SlLaunch K = L;
   K.w.total = (n_ub + SL_WG - 1) / SL_WG;
-  hipLaunchKernelGGL(k_sort_keys, dim3(K.w.total), dim3(SL_WG), 0, L.stream, K.w, keys, vals, n_ub, 0u, 0u);
+  SL_LAUNCH(k_sort_keys, dim3(K.w.total), dim3(SL_WG), 0, K, keys, vals, n_ub, 0u, 0u);
   size_t tb = temp_bytes;
   hipError_t e = rocprim::radix_sort_pairs(temp, tb, keys, keys + n_ub, vals, vals + n_ub, n_ub, 0, end_bit,
                                            L.stream);
   if (e != hipSuccess) return (int)e;
-  hipLaunchKernelGGL(k_sort_gather, dim3(K.w.total), dim3(SL_WG), 0, L.stream, K.w, vals + n_ub, 0u);
+  SL_LAUNCH(k_sort_gather, dim3(K.w.total), dim3(SL_WG), 0, K, vals + n_ub, 0u);
   return 0;
 }
 
@@ -76,7 +78,7 @@ void sl_k_sort_keys(const SlLaunch &L, uint32_t n_ub, uint32_t *keys, uint32_t *
   if (!n_ub) return;
   SlLaunch K = L;
   K.w.total = (n_ub + SL_WG - 1) / SL_WG;
-  hipLaunchKernelGGL(k_sort_keys, dim3(K.w.total), dim3(SL_WG), 0, L.stream, K.w, keys + val_off, vals + val_off,
+  SL_LAUNCH(k_sort_keys, dim3(K.w.total), dim3(SL_WG), 0, K, keys + val_off, vals + val_off,
                      n_ub, key_hi, val_off);
 }
 int sl_sort_pairs(void *temp, size_t temp_bytes, uint32_t *keys, uint32_t *vals, uint32_t n, int end_bit,
@@ -88,5 +90,5 @@ void sl_k_sort_gather(const SlLaunch &L, uint32_t n_ub, const uint32_t *sorted_v
   if (!n_ub) return;
   SlLaunch K = L;
   K.w.total = (n_ub + SL_WG - 1) / SL_WG;
-  hipLaunchKernelGGL(k_sort_gather, dim3(K.w.total), dim3(SL_WG), 0, L.stream, K.w, sorted_vals + val_off, val_off);
+  SL_LAUNCH(k_sort_gather, dim3(K.w.total), dim3(SL_WG), 0, K, sorted_vals + val_off, val_off);
 }

@@ -131,6 +131,7 @@ struct SlOne {
   uint32_t total;
   static constexpr const uint2 *wtab = nullptr;
   static constexpr const uint32_t *start = nullptr;
+  static constexpr const uint32_t *list = nullptr;
   static constexpr int32_t ntanks = 1;
 };
 
@@ -509,9 +510,9 @@ int sl_k_reorder(const SlLaunch &L, uint32_t n_ub, uint32_t *keys, uint32_t *val
 // ripple: rows = M workgroups per tank, cols = M/CW, elems = M*M/256, tex = (R+1)^2/256
 // top-surface height map of the tanks with top_height set: particles (256 per workgroup), texels
 // (wmap^2, 256 per workgroup)
-void sl_k_toph(const SlWork &parts, const SlWork &texels, hipStream_t s);
+void sl_k_toph(const SlWork &parts, const SlWork &texels, hipStream_t s, const SlTank *one = nullptr);
 // maxM: the largest ripple M among the tanks of the launch (sizes the row transforms' LDS)
 void sl_k_ripple_frame_ex(const SlWork &rows, const SlWork &cols, const SlWork &elems, const SlWork &tex,
-                          hipStream_t s, int mask, uint32_t maxM);
+                          hipStream_t s, int mask, uint32_t maxM, const SlTank *one = nullptr);
 // workgroups of compaction per tank (each compaction workgroup covers SL_COMPACT_ITEMS particles)
 #define SL_COMPACT_ITEMS 1024
