@@ -57,8 +57,8 @@ def step(g, wl):
 def sor_bytes_survey_model(nx, ny, nz):
     """The per-pass byte model of a reference-style pass that re-reads every face fraction:
     4C (p read) + 2C (p write, half) + 12N (fractions) + 2C (rhs, half) + 1C (flags) = 12N + 9C.
-    k_sor does not move these bytes (it reads stencil flags and re-reads fractions only in cut
-    cells); reported beside the kernel's own count for comparison."""
+    k_sor does not move these bytes (it reads stencil flags, and a cut cell's record of its
+    fractions); reported beside the kernel's own count for comparison."""
     N = (nx + 1) * (ny + 1) * (nz + 1)
     C = nx * ny * nz
     return 12 * N + 9 * C
@@ -69,8 +69,9 @@ def sor_pass_bytes(g):
     state (cell counts and face fractions read back after the timed region): every slot of the colour
     loads its u16 stencil flags (2 B); each liquid cell of the colour reads its pressure and
     right-hand side and writes its pressure (12 B); the six neighbour loads of those cells touch
-    other-colour pressures (4 B each, counted once); cut cells (an open face with 0 < f < 1) re-read
-    their face fractions (4 B per distinct face). Returns (mean bytes of the two colours, parts)."""
+    other-colour pressures (4 B each, counted once); cut cells (an open face with 0 < f < 1) read
+    their 32-byte record of face fractions and their sum. Returns (mean bytes of the two colours,
+    parts)."""
     nx, ny, nz = g.nx, g.ny, g.nz
     slots = ((nx + 1) // 2) * ny * nz
     liquid = g.readback(suiri.F_CELLCOUNT).reshape(nz, ny, nx) > 0
@@ -91,11 +92,8 @@ def sor_pass_bytes(g):
         near[:, 1:] |= lc[:, :-1]; near[:, :-1] |= lc[:, 1:]
         near[:, :, 1:] |= lc[:, :, :-1]; near[:, :, :-1] |= lc[:, :, 1:]
         cc = cut & (colour == c)
-        xf = np.zeros((nz, ny, nx + 1), bool); xf[:, :, :-1] |= cc; xf[:, :, 1:] |= cc
-        yf = np.zeros((nz, ny + 1, nx), bool); yf[:, :-1] |= cc; yf[:, 1:] |= cc
-        zf = np.zeros((nz + 1, ny, nx), bool); zf[:-1] |= cc; zf[1:] |= cc
         parts = {"flags": 2 * slots, "liquid": 12 * int(lc.sum()), "neighbours": 4 * int(near.sum()),
-                 "cut_fractions": 4 * int(xf.sum() + yf.sum() + zf.sum())}
+                 "cut_records": 32 * int(cc.sum())}
         out.append(parts)
     mean = {k2: (out[0][k2] + out[1][k2]) / 2 for k2 in out[0]}
     return sum(mean.values()), {"per_pass_mean": mean, "per_colour": out, "liquid_cells": int(liquid.sum()),
@@ -123,7 +121,7 @@ def calib_sor(passes):
         g.sync()
     c0 = det["per_colour"][0]
     print(json.dumps({"passes": passes, "colour": 0, "bytes": sum(c0.values()), "parts": c0,
-                      "read_bytes": c0["flags"] + c0["liquid"] * 2 // 3 + c0["neighbours"] + c0["cut_fractions"],
+                      "read_bytes": c0["flags"] + c0["liquid"] * 2 // 3 + c0["neighbours"] + c0["cut_records"],
                       "write_bytes": c0["liquid"] // 3}))
 
 

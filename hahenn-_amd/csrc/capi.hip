@@ -416,7 +416,7 @@ static int tank_buffers(sl_sim *s, SlTank &T, Take take) {
     TAKE(T.pres, (H * 2 + 2 * G) * 4);
     if (T.pres) T.pres += G;
   }
-  TAKE(T.rhs, H * 8); TAKE(T.sflag, H * 4);
+  TAKE(T.rhs, H * 8); TAKE(T.sflag, H * 4); TAKE(T.scut, H * 64);
   TAKE(T.objs, sizeof(SlObj) * SL_MAX_OBJ);
   TAKE(T.objMask, (size_t)cdiv(C, 64) * 8 + 64);
   if (s->ext_emits) T.emits = s->ext_emits;

@@ -746,14 +746,20 @@ __global__ __launch_bounds__(SL_WG) void k_divsetup(A W) {
   const bool ok[6] = {i > 0, i < nx - 1, j > 0, j < ny - 1, k > 0, k < nz - 1};
   const int32_t off[6] = {-1, 1, -nx, nx, -nx * ny, nx * ny};
   uint32_t fl = SF_FLUID;
+  float den = 0.0f;
 #pragma unroll
   for (int q = 0; q < 6; q++) {
     if (f[q] <= 0.0f) continue;
+    den += f[q];
     fl |= 1u << (SF_OPEN0 + q);
     if (f[q] != 1.0f) fl |= SF_SLOW;
     if (ok[q] && T.cellCount[(int64_t)c + off[q]] > 0u) fl |= 1u << (SF_NUM0 + q);
   }
   T.sflag[slot] = (uint16_t)fl;
+  if (fl & SF_SLOW) {
+    T.scut[2 * (size_t)slot] = make_float4(f0x, f1x, f0y, f1y);
+    T.scut[2 * (size_t)slot + 1] = make_float4(f0z, f1z, den, 0.0f);
+  }
 }
 void sl_k_divsetup(const SlLaunch &L) {
   if (!L.w.total) return;
@@ -796,16 +802,15 @@ __device__ __forceinline__ void sor_slot(const R &T, uint32_t lb, uint32_t parit
       if (fl & (1u << (SF_NUM0 + q))) num += p_n[q];
     den = (float)__builtin_popcount(fl & 0x3fu);
   } else {
-    const size_t N = T.nodes;
-    const uint32_t n0 = (uint32_t)(i + ((int32_t)j + (int32_t)k * (T.ny + 1)) * (T.nx + 1));
-    const float f[6] = {T.frac[n0], T.frac[n0 + 1], T.frac[N + n0], T.frac[N + n0 + (uint32_t)(T.nx + 1)],
-                        T.frac[2 * N + n0], T.frac[2 * N + n0 + (uint32_t)((T.nx + 1) * (T.ny + 1))]};
+    // the cut cell's record: fractions in face order and their sum over the open faces, summed in
+    // that order by the divergence set-up
+    const float4 *rec = T.scut + 2 * ((size_t)parity * H + s);
+    const float4 a = rec[0], c = rec[1];
+    const float f[6] = {a.x, a.y, a.z, a.w, c.x, c.y};
 #pragma unroll
-    for (int q = 0; q < 6; q++) {
-      if (f[q] <= 0.0f) continue;
-      den += f[q];
+    for (int q = 0; q < 6; q++)  // a numerator bit is set only on an open face
       if (fl & (1u << (SF_NUM0 + q))) num += f[q] * p_n[q];
-    }
+    den = c.z;
   }
   if (den > 0.0f) {
     const float pn = (num - b) / den;
@@ -830,7 +835,7 @@ void sl_k_sor(const SlLaunch &L, uint32_t parity, uint32_t pass) {
 }
 SlSorRec sl_sor_rec(const SlTank &T) {
   SlSorRec R;
-  R.pres = T.pres; R.rhs = T.rhs; R.sflag = T.sflag; R.frac = T.frac;
+  R.pres = T.pres; R.rhs = T.rhs; R.sflag = T.sflag; R.scut = T.scut;
   R.nx = T.nx; R.ny = T.ny; R.hw = T.hw; R.nodes = T.nodes; R.hcells = T.hcells;
   for (int q = 0; q < 3; q++) { R.hw_mag[q] = T.hw_mag[q]; R.rowl_mag[q] = T.rowl_mag[q]; }
   R.omega = T.omega; R.omega1 = T.omega1;

@@ -92,6 +92,10 @@ struct SlTank {
   // to 0 there and transfer1 skips the node), so G2P takes a corner's mask from its velOld load
   float *vel, *velOld, *frac, *svel;
   float *divg;
+  // per SOR slot of a cut cell (SF_SLOW), written by the divergence set-up: its six face fractions in
+  // face order and their sum over the open faces, (f0, f1, f2, f3), (f4, f5, den, 0), so a pass
+  // reads one 32-byte record instead of six fractions from three node arrays
+  float4 *scut;
   SlObj *objs;
   // one bit per cell (x fastest): some object's cull sphere may reach a particle that starts in the
   // cell and moves at most one cell in a substep (k_objmask); G2P skips its object loops elsewhere
@@ -434,7 +438,7 @@ struct SlSorRec {
   float *pres;
   const float *rhs;
   const uint16_t *sflag;
-  const float *frac;
+  const float4 *scut;
   int32_t nx, ny, hw;
   uint32_t nodes, hcells;
   uint32_t hw_mag[3], rowl_mag[3];
