@@ -57,7 +57,7 @@ def step(g, wl):
 def sor_bytes_survey_model(nx, ny, nz):
     """The per-pass byte model of a reference-style pass that re-reads every face fraction:
     4C (p read) + 2C (p write, half) + 12N (fractions) + 2C (rhs, half) + 1C (flags) = 12N + 9C.
-    k_sor does not move these bytes (it reads stencil flags, and a cut cell's record of its
+    the colour pass does not move these bytes (it reads stencil flags, and a cut cell's record of its
     fractions); reported beside the kernel's own count for comparison."""
     N = (nx + 1) * (ny + 1) * (nz + 1)
     C = nx * ny * nz
@@ -65,7 +65,7 @@ def sor_bytes_survey_model(nx, ny, nz):
 
 
 def sor_pass_bytes(g):
-    """Bytes one k_sor colour pass moves, counted as the kernel reads and writes them, from the tank's
+    """Bytes one SOR colour pass (k_sor_rec) moves, counted as the kernel reads and writes them, from the tank's
     state (cell counts and face fractions read back after the timed region): every slot of the colour
     loads its u16 stencil flags (2 B); each liquid cell of the colour reads its pressure and
     right-hand side and writes its pressure (12 B); the six neighbour loads of those cells touch
@@ -101,7 +101,7 @@ def sor_pass_bytes(g):
 
 
 def calib_sor(passes):
-    """Counter calibration on k_sor's own access pattern: the headline tank after a few steps, then
+    """Counter calibration on the SOR colour pass's own access pattern: the headline tank after a few steps, then
     `passes` times a 1 GiB device copy (which evicts the L2 and the Infinity Cache) followed by one
     colour-0 pass, so that every byte the pass reads comes from HBM. Prints the pass's byte count from
     sor_pass_bytes for rocprofv3's FETCH_SIZE / WRITE_SIZE of the same dispatches to be set against."""
@@ -151,13 +151,13 @@ def cpu_baseline(wl, steps, threads):
     return psteps / dt, dt
 
 
-TRAFFIC_KERNELS = ("k_sor", "k_g2p", "k_p2g_tiles", "k_bin_count", "k_bin_scatter", "k_normforce")
+TRAFFIC_KERNELS = ("k_sor_rec", "k_g2p", "k_p2g_tiles", "k_bin_count", "k_bin_scatter", "k_normforce")
 
 
 def pmc_traffic(out_dir, timeout=150):
-    """HBM bytes per launch of k_sor (and of the G2P, P2G and NORMALIZE kernels, under "kernels") from
-    two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE) of a short bench, each scaled by what the same
-    counter reads for a plain 256 MiB device copy. Child processes; returns None when rocprofv3 is
+    """HBM bytes per launch of the SOR colour pass k_sor_rec (and of the G2P, P2G and NORMALIZE kernels,
+    under "kernels") from two rocprofv3 counter runs (FETCH_SIZE, WRITE_SIZE) of a short bench, each
+    scaled by what the same counter reads for a plain 256 MiB device copy. Child processes; returns None when rocprofv3 is
     missing or a pass fails."""
     import csv
     import shutil
@@ -191,7 +191,7 @@ def pmc_traffic(out_dir, timeout=150):
                 for k in TRAFFIC_KERNELS:
                     if name.startswith(k + "(") or name.startswith("void " + k + "<"):
                         per.setdefault(k, {}).setdefault(name, []).append(float(r["Counter_Value"]))
-            if "k_sor" not in per:
+            if "k_sor_rec" not in per:
                 return None
             # of several template variants (the tuner's trials of k_g2p<MINW>), the one the timed steps
             # ran: the most dispatches (a spilling trial variant writes scratch on top of the step's bytes)
@@ -208,7 +208,7 @@ def pmc_traffic(out_dir, timeout=150):
         if k in vals["write"]:
             fb, wb = vals["fetch"][k] * 1024 * ff, vals["write"][k] * 1024 * wf
             kern[k] = {"bytes": fb + wb, "fetch_bytes": fb, "write_bytes": wb}
-    s0 = kern["k_sor"]
+    s0 = kern["k_sor_rec"]
     return {"bytes": s0["bytes"], "fetch_bytes": s0["fetch_bytes"], "write_bytes": s0["write_bytes"],
             "fetch_factor": ff, "write_factor": wf, "kernels": kern}
 
@@ -559,7 +559,7 @@ def main():
     ap.add_argument("--serve-kinds", default="0,1,3")
     ap.add_argument("--serve-fpt", type=int, default=4, help="serving: frames per tick")
     ap.add_argument("--calib-sor", type=int, default=0,
-                    help="only run this many HBM-cold k_sor passes of the headline tank (counter calibration) and exit")
+                    help="only run this many HBM-cold SOR colour passes of the headline tank (counter calibration) and exit")
     ap.add_argument("--calib-copy-mb", type=int, default=0,
                     help="only run a plain device copy of this many MiB (counter calibration) and exit")
     args = ap.parse_args()
@@ -670,7 +670,7 @@ def main():
         "config": {"workload": wl["name"], "grid": list(grid), "particles": int(mean_p), "sor_iters": iters,
                    "omega": 1.8, "dt": wl["frame_dt"], "ripple_M": 2 * wl["cfg"]["ripple_R"],
                    "whitewater": True},
-        "roofline": {"bound": "hbm", "kernel": "k_sor (one colour pass)", "achieved": achieved,
+        "roofline": {"bound": "hbm", "kernel": "k_sor_rec (one colour pass)", "achieved": achieved,
                      "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": None, "bytes_per_launch": bpl, "bytes_detail": bpl_detail,
                      "avg_launch_us": sor_launch_s * 1e6,
