@@ -795,23 +795,22 @@ __device__ __forceinline__ void sor_slot(const R &T, uint32_t lb, uint32_t parit
   const float p_n[6] = {ld32(pb, g + r - 1u), ld32(pb, g + r), ld32(pb, g - hw),
                         ld32(pb, g + hw), ld32(pb, g - rowl), ld32(pb, g + rowl)};
   const float b = ld32(T.rhs + parity * H, s), p = ld32((const float *)pm, s);
-  float num = 0.0f, den = 0.0f;
-  if (!(fl & SF_SLOW)) {
-#pragma unroll
-    for (int q = 0; q < 6; q++)
-      if (fl & (1u << (SF_NUM0 + q))) num += p_n[q];
-    den = (float)__builtin_popcount(fl & 0x3fu);
-  } else {
-    // the cut cell's record: fractions in face order and their sum over the open faces, summed in
-    // that order by the divergence set-up
+  // one path for cut and uncut cells: an uncut cell weighs its contributing faces by 1 (1 * p = p,
+  // so its sum is the count-weighted one bit for bit) and divides by its number of open faces; a cut
+  // cell takes its face fractions and their sum from its 32-byte record (written by the divergence
+  // set-up in face order). A numerator bit is set only on an open face.
+  float f[6] = {1.0f, 1.0f, 1.0f, 1.0f, 1.0f, 1.0f};
+  float den = (float)__builtin_popcount(fl & 0x3fu);
+  if (fl & SF_SLOW) {
     const float4 *rec = T.scut + 2 * ((size_t)parity * H + s);
     const float4 a = rec[0], c = rec[1];
-    const float f[6] = {a.x, a.y, a.z, a.w, c.x, c.y};
-#pragma unroll
-    for (int q = 0; q < 6; q++)  // a numerator bit is set only on an open face
-      if (fl & (1u << (SF_NUM0 + q))) num += f[q] * p_n[q];
+    f[0] = a.x; f[1] = a.y; f[2] = a.z; f[3] = a.w; f[4] = c.x; f[5] = c.y;
     den = c.z;
   }
+  float num = 0.0f;
+#pragma unroll
+  for (int q = 0; q < 6; q++)
+    if (fl & (1u << (SF_NUM0 + q))) num += f[q] * p_n[q];
   if (den > 0.0f) {
     const float pn = (num - b) / den;
     st32(pm, s, p * T.omega1 + pn * T.omega);
