@@ -201,13 +201,17 @@ __device__ __forceinline__ float rndu(uint32_t &s) { s = hashU(s); return (float
 
 // element i of an array of fewer than 2^32 bytes at a wave-uniform base: a 32-bit byte offset, so
 // the load takes the scalar-base + vector-offset form without 64-bit address arithmetic
+// Every base is a device allocation of a tank: the global address space makes the access a global
+// (not flat) instruction also where the compiler cannot prove it, as for pointers read from a batch's
+// tank table -- a flat access counts on the LDS wait counter too, so it also waits out LDS work.
+#define SL_GLOBAL __attribute__((address_space(1)))
 template <class E>
 __device__ __forceinline__ E ld32(const E *base, uint32_t i) {
-  return *(const E *)((const char *)base + i * (uint32_t)sizeof(E));
+  return *(const SL_GLOBAL E *)((const SL_GLOBAL char *)base + i * (uint32_t)sizeof(E));
 }
 template <class E>
 __device__ __forceinline__ void st32(E *base, uint32_t i, E v) {
-  *(E *)((char *)base + i * (uint32_t)sizeof(E)) = v;
+  *(SL_GLOBAL E *)((SL_GLOBAL char *)base + i * (uint32_t)sizeof(E)) = v;
 }
 // n / d from the triple (m, s1, s2) that sl_magic_u32 computes for d: Granlund & Montgomery,
 // "Division by Invariant Integers using Multiplication" (PLDI 1994), fig. 4.1, exact for every
