@@ -614,35 +614,56 @@ __global__ __launch_bounds__(1024) void k_compact_scan(A W) {
   const uint32_t lim = min(T.C->count, T.cap);
   const uint32_t nb = (lim + SL_COMPACT_ITEMS - 1) / SL_COMPACT_ITEMS;
   __shared__ uint32_t wsum[16];
-  __shared__ uint32_t carry;
-  if (threadIdx.x == 0) carry = 0;
-  __syncthreads();
-  for (uint32_t base = 0; base < nb; base += 1024) {
-    uint32_t i = base + threadIdx.x;
-    uint32_t v = i < nb ? T.scanBlocks[i] : 0u;
-    // inclusive wave scan
-    uint32_t x = v;
-    const int lane = threadIdx.x & 63;
+  __shared__ uint32_t carry_s;
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t carry = 0;
+  // 16 consecutive counts per thread (four 16-byte loads, all in flight together; the allocation
+  // has 64 bytes beyond the last count, and counts at or beyond nb are taken as 0), so 16 Ki counts
+  // take one pass with two barriers instead of sixteen rounds of loads and barriers
+  for (uint32_t base = 0; base < nb; base += 16384u) {
+    const uint32_t i0 = base + 16u * threadIdx.x;
+    uint32_t v[16];
+    if (i0 < nb) {
+      const uint4 *src = (const uint4 *)(T.scanBlocks + i0);
+#pragma unroll
+      for (int q = 0; q < 4; q++) {
+        const uint4 u = src[q];
+        v[4 * q] = u.x; v[4 * q + 1] = u.y; v[4 * q + 2] = u.z; v[4 * q + 3] = u.w;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+      if (i0 + (uint32_t)q >= nb) v[q] = 0u;
+    uint32_t tot = 0;
+#pragma unroll
+    for (int q = 0; q < 16; q++) tot += v[q];
+    uint32_t x = tot;  // inclusive wave scan of the thread totals
     for (int o = 1; o < 64; o <<= 1) {
-      uint32_t y = __shfl_up(x, o, 64);
+      const uint32_t y = __shfl_up(x, o, 64);
       if (lane >= o) x += y;
     }
-    if (lane == 63) wsum[threadIdx.x >> 6] = x;
+    if (lane == 63) wsum[wv] = x;
     __syncthreads();
     if (threadIdx.x < 16) {
-      uint32_t ws = wsum[threadIdx.x], acc = ws;
+      const uint32_t ws = wsum[threadIdx.x];
+      uint32_t acc = ws;
       for (int o = 1; o < 16; o <<= 1) {
-        uint32_t y = __shfl_up(acc, o, 16);
+        const uint32_t y = __shfl_up(acc, o, 16);
         if ((int)threadIdx.x >= o) acc += y;
       }
-      wsum[threadIdx.x] = acc - ws;  // exclusive prefix of wave sums
+      wsum[threadIdx.x] = acc - ws;  // exclusive prefix of the wave totals
     }
     __syncthreads();
-    uint32_t excl = carry + wsum[threadIdx.x >> 6] + x - v;
-    if (i < nb) T.scanBlocks[i] = excl;
+    uint32_t run = carry + wsum[wv] + x - tot;  // exclusive prefix of this thread's first count
+#pragma unroll
+    for (int q = 0; q < 16; q++) {
+      if (i0 + (uint32_t)q < nb) T.scanBlocks[i0 + q] = run;
+      run += v[q];
+    }
+    if (threadIdx.x == 1023) carry_s = run;  // carry + every count of the chunk
     __syncthreads();
-    if (threadIdx.x == 1023) carry = excl + v;
-    __syncthreads();
+    carry = carry_s;
+    __syncthreads();  // wsum and carry_s are rewritten by the next chunk
   }
   if (threadIdx.x == 0) T.C->newCount = carry;
 }
