@@ -253,6 +253,10 @@ static int push_dyn(sl_sim *s, uint32_t emit_k = 0, uint32_t emit_total = 0) {
   // the host mirror keeps the same values: single-simulation launches pass it by value
   s->ht.frame = d.frame; s->ht.emit_k = emit_k; s->ht.emit_total = emit_total;
   s->ht.iters = d.iters; s->ht.mask_bound = d.mask_bound; s->ht.time_d = d.time_d; s->ht.dt_frame = d.dt_frame;
+  // A simulation of its own reads these fields only from its by-value record: the kernels that take
+  // its device record (COUNT with an explicit emission total, the converged solve) read none of them,
+  // so only a tank of a batch, whose kernels read the table, needs the device copy updated.
+  if (!s->ext_tank) return SL_OK;
   hipLaunchKernelGGL(k_update, dim3(1), dim3(64), 0, s->stream, s->d_tank, d);
   return check_launch(s, "update");
 }
