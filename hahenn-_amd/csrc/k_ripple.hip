@@ -232,13 +232,8 @@ __global__ __launch_bounds__(256) void k_revolve(A W) {
 }
 
 // REXTRACT, 64-ripple.js:134-147 (f32 values; the f16 texture is produced on readback)
-template <class A>
-__global__ __launch_bounds__(256) void k_rextract(A W) {
-  SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = sl_tank(W, t);
+__device__ __forceinline__ void rextract_one(const SlTank &T, uint32_t id) {
   const uint32_t M = T.M, R = T.R, R1 = R + 1;
-  uint32_t id = lb * 256 + threadIdx.x;
-  if (id >= R1 * R1) return;
   uint32_t i = id % R1, j = id / R1;
   const float2 *f = T.rb;
   const float s = 1.0f / (float)(M * M);
@@ -250,14 +245,8 @@ __global__ __launch_bounds__(256) void k_rextract(A W) {
 }
 
 // RMASKBUILD, 64-ripple.js:106-126
-template <class A>
-__global__ __launch_bounds__(256) void k_rmaskbuild(A W) {
-  SL_TANK_OF(W, blockIdx.x, t, lb);
-  const SlTank &T = sl_tank(W, t);
-  if (!T.mask_bound) return;
+__device__ __forceinline__ void rmaskbuild_one(const SlTank &T, uint32_t id) {
   const uint32_t R = T.R, R1 = R + 1;
-  uint32_t id = lb * 256 + threadIdx.x;
-  if (id >= R1 * R1) return;
   uint32_t gx = id % R1, gy = id / R1;
   float u = (float)gx / (float)R, w = (float)gy / (float)R;
   float hix = T.ox + (float)T.nx * T.h, hiz = T.oz + (float)T.nz * T.h;
@@ -276,13 +265,20 @@ __global__ __launch_bounds__(256) void k_rmaskbuild(A W) {
   T.rmask[id] = m;
 }
 
+// The frame's texel-space tail in one launch over the (R+1)^2 texels: REXTRACT (64-ripple.js:134-147),
+// RMASKBUILD where the absorbing mask is on (:106-126; the masked transform that follows reads it), and
+// the source field cleared for the next frame's deposits (:252; this frame's forward transform has
+// read it already)
 template <class A>
-__global__ void k_clear_rsrc(A W) {
+__global__ __launch_bounds__(256) void k_rtail(A W, int mask) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = sl_tank(W, t);
-  uint32_t id = lb * 256 + threadIdx.x;
-  uint32_t R1 = T.R + 1;
-  if (id < R1 * R1) T.rsrc[id] = 0;
+  const uint32_t R1 = T.R + 1;
+  const uint32_t id = lb * 256 + threadIdx.x;
+  if (id >= R1 * R1) return;
+  rextract_one(T, id);
+  if (mask && T.mask_bound) rmaskbuild_one(T, id);
+  T.rsrc[id] = 0;
 }
 
 static SlLaunch launch_over(const SlWork &w, hipStream_t s, const SlTank *one) {
@@ -308,15 +304,13 @@ void sl_k_ripple_frame_ex(const SlWork &rows_w, const SlWork &cols_w, const SlWo
   // inverse transform of the state -> b (64-ripple.js:234-235)
   SL_LAUNCH(k_fft_rows, dim3(rows_w.total), dim3(256), rl, rows, 0, 1, (int)IN_BUF, 1.0f, maxM);
   SL_LAUNCH(k_fft_cols, dim3(cols_w.total), dim3(FFT_COL_THREADS), cl, cols, 1, 2, (int)OUT_BUF, 1.0f);
-  // height and slopes (64-ripple.js:236-238)
-  SL_LAUNCH(k_rextract, dim3(tex_w.total), dim3(256), 0, tex);
+  // height and slopes (64-ripple.js:236-238), the mask, the source cleared (64-ripple.js:252)
+  SL_LAUNCH(k_rtail, dim3(tex_w.total), dim3(256), 0, tex, mask);
   if (mask) {
     // absorb: mask the spatial field, transform back into the state, rescale (64-ripple.js:239-250)
-    SL_LAUNCH(k_rmaskbuild, dim3(tex_w.total), dim3(256), 0, tex);
     SL_LAUNCH(k_fft_rows, dim3(rows_w.total), dim3(256), rl, rows, 2, 1, (int)IN_MASK, -1.0f, maxM);
     SL_LAUNCH(k_fft_cols, dim3(cols_w.total), dim3(FFT_COL_THREADS), cl, cols, 1, 0, (int)OUT_SCALE, -1.0f);
   }
-  SL_LAUNCH(k_clear_rsrc, dim3(tex_w.total), dim3(256), 0, tex);  // 64-ripple.js:252
 }
 
 // ------------------------------------------------------------------------------------------------
