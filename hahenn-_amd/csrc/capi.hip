@@ -941,13 +941,15 @@ static int encode_ripple(sl_sim *s) {
   return check_launch(s, "ripple");
 }
 
-static int encode_compact(sl_sim *s) {
-  StageScope sc(s, SL_T_COMPACT, 4);
+// commit = false leaves the count commit (count = newCount) to the whitewater commit of the same
+// frame (COUNT mode 8): the kernels between them read only the whitewater counters
+static int encode_compact(sl_sim *s, bool commit = true) {
+  StageScope sc(s, SL_T_COMPACT, commit ? 4 : 3);
   SlLaunch L = launch_of(s, s->p_ub, SL_COMPACT_ITEMS);
   sl_k_compact_count(L);
   sl_k_compact_scan(L, 1);
   sl_k_compact_move(L);
-  sl_k_count(launch_of(s, 1), 1, 0);
+  if (commit) sl_k_count(launch_of(s, 1), 1, 0);
   return check_launch(s, "compact");
 }
 
@@ -979,7 +981,9 @@ extern "C" int sl_reorder(sl_sim *s) {
 extern "C" int sl_step_end(sl_sim *s, double frame_dt) {
   if (!s) FAIL((sl_sim *)nullptr, SL_EARG, "null sim");
   if (!(frame_dt > 0)) return SL_OK;
-  int r = encode_compact(s);
+  // a re-ordering reads the committed count; without one the commit joins the whitewater commit
+  const bool reorder_due = s->cfg.reorder_every && s->reorder_clock + 1 >= s->cfg.reorder_every;
+  int r = encode_compact(s, reorder_due);
   if (r) return r;
   if (s->cfg.reorder_every && ++s->reorder_clock >= s->cfg.reorder_every) {
     // the cadence restarts here only: an explicit sl_reorder leaves it (as the oracle's step does)
@@ -993,7 +997,7 @@ extern "C" int sl_step_end(sl_sim *s, double frame_dt) {
   {
     StageScope sc(s, SL_T_WW, 2);
     sl_k_ww(launch_of(s, s->ww_cap), (float)frame_dt);
-    sl_k_count(launch_of(s, 1), 3, 0);
+    sl_k_count(launch_of(s, 1), reorder_due ? 3 : 8, 0);
     s->ww_cur ^= 1;
   }
   r = encode_ripple(s);
@@ -2266,9 +2270,8 @@ static int batch_encode_frame(sl_batch *b, double dt) {
   Lscan.w.list = list_dev(b);
   sl_k_compact_scan(Lscan, n);
   sl_k_compact_move(Lc);
-  sl_k_count(P, 1, 0);
   sl_k_ww(batch_launch(b, BM_WW), (float)dt);  // dt = 0: each tank's own (its substep dt)
-  sl_k_count(P, 3, 0);
+  sl_k_count(P, 8, 0);  // count = newCount and the whitewater commit (WW reads neither count)
   if (b->any_ripple)  // the mask stages return for tanks whose mask is off
     sl_k_ripple_frame_ex(batch_launch(b, BM_ROWS).w, batch_launch(b, BM_COLS).w, batch_launch(b, BM_ELEMS).w,
                          batch_launch(b, BM_TEX).w, b->stream, 1, b->max_M);

@@ -520,7 +520,9 @@ void sl_k_emit(const SlLaunch &L, uint32_t reserve) {
 // min(count + y, cap); 3: wwCount = min(wwNew, wcap); 4: clear; 5: vmax = 0 and the slots cleared
 // (the writeBuffer of 50-sim.js:306); 6: flip the particle buffer; 7: count += the emitted
 // particles that fit below cap - 64 (a batch's emission, whose budget is applied on the device
-// count). One thread per tank (per listed slot when `list` is given).
+// count); 8: modes 1 and 3 in one launch (the compaction's commit deferred to the whitewater
+// commit, when no kernel between them reads the particle count). One thread per tank (per listed
+// slot when `list` is given).
 // ------------------------------------------------------------------------------------------------
 __device__ __forceinline__ void vmax_fold(const SlTank &T, bool clear) {
   uint32_t v = T.C->vmax, q[SL_VSLOTS];
@@ -536,13 +538,13 @@ __global__ void k_count(SlTank *tanks, const uint32_t *list, int ntanks, uint32_
   SlTank &T = tanks[list ? list[i] : i];
   SlCounters *C = T.C;
   uint32_t cap = T.cap, wcap = f2u(T.ww_cap_f);
-  if (mode == 0 || mode == 1 || mode == 5) vmax_fold(T, mode == 5);
-  if (mode == 1) {  // compaction is in place: the buffer stays current
+  if (mode == 0 || mode == 1 || mode == 5 || mode == 8) vmax_fold(T, mode == 5);
+  if (mode == 1 || mode == 8) {  // compaction is in place: the buffer stays current
     C->count = C->newCount;
     C->newCount = 0;
   }
   if (mode == 2) { uint32_t yy = y ? y : T.emit_total; uint32_t s = C->count + yy; C->count = s < cap ? s : cap; }
-  if (mode == 3) { C->wwCount = C->wwNew < wcap ? C->wwNew : wcap; C->wwNew = 0; T.ww_cur = 1 - T.ww_cur; }
+  if (mode == 3 || mode == 8) { C->wwCount = C->wwNew < wcap ? C->wwNew : wcap; C->wwNew = 0; T.ww_cur = 1 - T.ww_cur; }
   if (mode == 4) { C->count = 0; C->newCount = 0; C->wwCount = 0; C->wwNew = 0; C->impacts = 0; }
   if (mode == 6) T.cur = 1 - T.cur;
   if (mode == 7 && T.emit_total) {
