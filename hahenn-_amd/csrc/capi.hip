@@ -67,6 +67,7 @@ struct sl_sim {
   bool full = false, geom_dirty = true, mask_dirty = true;  // mask_dirty: objMask out of date
   int cur = 0, ww_cur = 0;
   uint32_t forced_substeps = 0, substeps_last = 0;
+  uint32_t pending_add = 0;  // particles emitted in this substep, committed by its cell-count clear
   // spatial re-ordering
   uint32_t *sort_keys = nullptr, *sort_vals = nullptr;
   void *sort_temp = nullptr;
@@ -704,13 +705,16 @@ static int write_emits(sl_sim *s, const sl_emit *list, uint32_t n, uint32_t *tot
   }
   return SL_OK;
 }
-static int encode_emit(sl_sim *s, uint32_t k, uint32_t total) {
+// commit = false (inside a substep) leaves count += total to the substep's cell-count clear, the
+// next kernel, which reads nothing that needs it (the launch is saved)
+static int encode_emit(sl_sim *s, uint32_t k, uint32_t total, bool commit = true) {
   int r = push_dyn(s, k, total);
   if (r) return r;
   {
-    StageScope sc(s, SL_T_EMIT, 2);
+    StageScope sc(s, SL_T_EMIT, commit ? 2 : 1);
     sl_k_emit(launch_of(s, total));
-    sl_k_count(launch_of(s, 1), 2, total);
+    if (commit) sl_k_count(launch_of(s, 1), 2, total);
+    else s->pending_add += total;
   }
   s->emitted_total += total;
   uint64_t ub = (uint64_t)s->p_ub + total;
@@ -880,7 +884,8 @@ static int encode_substep(sl_sim *s) {
   }
   {
     StageScope sc(s, SL_T_P2G, 2);
-    sl_k_fill_u32(s->stream, s->ht.cellCount, 0, s->cells);
+    sl_k_clear_cells_commit(s->stream, s->ht.cellCount, s->cells, s->ht.C, s->pending_add, s->cap);
+    s->pending_add = 0;
     encode_p2g(s, s->p2g_variant);
   }
   { StageScope sc(s, SL_T_NORMFORCE, 1); sl_k_normforce(launch_of(s, s->nodes)); }
@@ -899,7 +904,7 @@ extern "C" int sl_substep(sl_sim *s, double dt, const sl_emit *list, uint32_t n,
   uint32_t total = 0, k = 0;
   int r = write_emits(s, list, n, &total, &k);
   if (r) return r;
-  if (total > 0) { r = encode_emit(s, k, total); if (r) return r; }
+  if (total > 0) { r = encode_emit(s, k, total, false); if (r) return r; }
   r = push_dyn(s);
   if (r) return r;
   r = encode_substep(s);

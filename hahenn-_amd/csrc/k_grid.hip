@@ -17,6 +17,23 @@ void sl_k_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n) {
   if (blocks > 8192) blocks = 8192;
   hipLaunchKernelGGL(k_fill_u32, dim3((unsigned)blocks), dim3(SL_WG), 0, s, p, v, n);
 }
+// the cell counts cleared for P2G; the first thread also adds a substep's emitted particles to the
+// count, as COUNT mode 2 would right after the emission (no kernel between them reads the count)
+__global__ void k_clear_cells_commit(uint32_t *cc, size_t n, SlCounters *C, uint32_t add, uint32_t cap) {
+  size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0 && add) {
+    const uint32_t s = C->count + add;
+    C->count = s < cap ? s : cap;
+  }
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (; i < n; i += stride) cc[i] = 0u;
+}
+void sl_k_clear_cells_commit(hipStream_t s, uint32_t *cc, size_t n, SlCounters *C, uint32_t add, uint32_t cap) {
+  size_t blocks = (n + SL_WG - 1) / SL_WG;
+  if (blocks > 8192) blocks = 8192;
+  if (blocks < 1) blocks = 1;
+  hipLaunchKernelGGL(k_clear_cells_commit, dim3((unsigned)blocks), dim3(SL_WG), 0, s, cc, n, C, add, cap);
+}
 
 // ------------------------------------------------------------------------------------------------
 // GEOM: open fraction and solid velocity of every staggered face, 30-wgsl-sim.js:118-157

@@ -422,6 +422,8 @@ struct SlLaunch {
   } while (0)
 // grid stages (work map in units of SL_WG-wide workgroups over the stage's index space)
 void sl_k_fill_u32(hipStream_t s, uint32_t *p, uint32_t v, size_t n);
+// cell counts cleared, and count = min(count + add, cap) (a substep's deferred emission commit)
+void sl_k_clear_cells_commit(hipStream_t s, uint32_t *cc, size_t n, SlCounters *C, uint32_t add, uint32_t cap);
 void sl_k_geom(const SlLaunch &L);
 void sl_k_objmask(const SlLaunch &L);  // over the cells
 void sl_k_p2g(const SlLaunch &L, int count_only, int variant);
