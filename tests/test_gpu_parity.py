@@ -304,6 +304,42 @@ def test_g2p_variants_with_stream_source_bitwise():
     assert np.array_equal(ww_multiset(g.readback(suiri.F_WW)[:wo]), ww_multiset(o.field("ww")[:wo]))
 
 
+def test_stream_source_over_three_substeps_bitwise():
+    """Three substeps per frame with a stream source emitting in every substep (each substep's emission
+    is committed to the count by the substep's own cell-count clear): the oracle's particles, cell
+    counts and counters bit for bit after every frame, the substeps' records drawn as FluidSim.step
+    draws them (src/js/50-sim.js:300-317)."""
+    sc = twin.small_scene(fill_cells=(12, 8, 16))
+    o, g = twin.make_twin(sc)
+    inf = o.info()
+    top = inf["oy"] + 0.2
+    src = scene.make_source(pos=[inf["ox"] + 0.28, top, inf["oz"] + 0.12], mode="stream", flow=90.0,
+                            speed=1.0, pitch=-90, yaw=0)
+    em = scene.Emitters([src], scene.Scene(g.cfg))
+    g.set_substeps(3)
+    frame_dt = 1 / 120
+    sub = frame_dt / 3   # the library's substep: frame_dt / n in double
+    emitted = 0
+    for f in range(10):
+        subs = [em(sub, (3 * f + k) * sub) for k in range(3)]
+        emitted += sum(r["count"] for recs in subs for r in recs)
+        o.L.or_step_begin(o.ptr)
+        for recs in subs:
+            o.step_substep(sub, recs)
+        o.L.or_frame_tail(o.ptr, frame_dt)
+        o.L.or_poll_stats(o.ptr)
+        it = iter(subs)
+        assert g.step(frame_dt, lambda dt, t: next(it)) == 3
+        n = int(o.field("counters")[0])
+        assert g.stats()["count"] == n, f
+        gp, op = g.readback(suiri.F_PARTICLES)[:n], o.field("parts")[:n]
+        assert np.array_equal(gp.view(np.uint32), op.view(np.uint32)), f
+        assert np.array_equal(g.readback(suiri.F_CELLCOUNT), o.field("cellCount")), f
+        gc, oc = g.readback(suiri.F_COUNTERS), o.field("counters")
+        assert [gc[0], gc[4], gc[5]] == [oc[0], oc[4], oc[5]], f   # count, vmax bits, impacts
+    assert emitted > 0 and n > 12 * 8 * 16 * 8, "the source should have added particles"
+
+
 def test_every_obstacle_shape_and_drain_bitwise():
     """G2P skips the signed distances of obstacles (and the drain test of drains) whose bounding
     sphere is beyond the contact radius; a dam break through one obstacle of every shape type (one
