@@ -940,7 +940,7 @@ static int encode_ripple(sl_sim *s) {
   StageScope sc(s, SL_T_RIPPLE, s->mask_bound ? 10 : 6);
   const uint32_t M = s->M, R1 = s->R + 1;
   uint32_t CW = 4096 / M; if (CW > M) CW = M; if (CW < 1) CW = 1;
-  const SlLaunch rows = launch_of(s, M, 1);
+  const SlLaunch rows = launch_of(s, M / sl_fft_rows_per_wg(M), 1);
   sl_k_ripple_frame_ex(rows.w, launch_of(s, M, CW).w, launch_of(s, (uint64_t)M * M).w,
                        launch_of(s, (uint64_t)R1 * R1).w, s->stream, s->mask_bound, M, rows.one);
   return check_launch(s, "ripple");
@@ -1967,7 +1967,7 @@ static int batch_rebuild(sl_batch *b) {
       (r = put_map(b, BM_WW, [](const sl_sim *s) { return (uint64_t)s->ww_cap; }, SL_WG, all)) ||
       (r = put_map(b, BM_ELEMS, [](const sl_sim *s) { return (uint64_t)s->M * s->M; }, SL_WG, all)) ||
       (r = put_map(b, BM_TEX, [](const sl_sim *s) { return s->R ? (uint64_t)(s->R + 1) * (s->R + 1) : 0; }, SL_WG, all)) ||
-      (r = put_map(b, BM_ROWS, [](const sl_sim *s) { return (uint64_t)s->M; }, 1, all)) ||
+      (r = put_map(b, BM_ROWS, [](const sl_sim *s) { return (uint64_t)(s->M / sl_fft_rows_per_wg(s->M)); }, 1, all)) ||
       (r = put_map(b, BM_COLS, [](const sl_sim *s) { return (uint64_t)cols_of(s); }, 1, all)))
     return r;
   auto toph = [&](uint32_t t) { return b->sims[t]->ht.top_height != 0; };

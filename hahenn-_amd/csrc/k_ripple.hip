@@ -75,53 +75,64 @@ __device__ __forceinline__ float2 fft_input(const SlTank &T, int mode, const flo
   return make_float2(f.x * m, f.y * m);
 }
 
-// all horizontal passes of one row; dynamic LDS of two buffers of the largest M in the launch
+// all horizontal passes of sl_fft_rows_per_wg(M) consecutive rows (one row for M >= 1024, else
+// about 1024 points, so that small fields keep every thread of the workgroup busy); dynamic LDS of
+// two buffers of ldsP points (the most points of a workgroup in the launch) and the M twiddles
 template <class A>
 __global__ __launch_bounds__(256) void k_fft_rows(A W, int which_in, int which_out, int mode, float sign,
-                                                  uint32_t ldsM) {
+                                                  uint32_t ldsP) {
   SL_TANK_OF(W, blockIdx.x, t, lb);
   const SlTank &T = sl_tank(W, t);
-  const uint32_t M = T.M, lM = T.logM, y = lb;
-  if (y >= M) return;
+  const uint32_t M = T.M, lM = T.logM, y0 = lb * sl_fft_rows_per_wg(M);
+  if (y0 >= M) return;
   if (mode == IN_MASK && !T.mask_bound) return;  // tanks of a batch without the absorbing mask
-  extern __shared__ float2 lds[];  // buffer b at lds + b * ldsM, then the M twiddles at lds + 2 * ldsM
-  float2 *stw = lds + 2u * ldsM;
+  const uint32_t n = sl_fft_rows_per_wg(M) * M;  // points of the workgroup: rows y0 .. y0 + n/M - 1
+  extern __shared__ float2 lds[];  // buffer b at lds + b * ldsP, then the M twiddles at lds + 2 * ldsP
+  float2 *stw = lds + 2u * ldsP;
   const float2 *bufs[5] = {T.rstate, T.ra, T.rb, T.rhat, nullptr};
   float2 *obufs[5] = {T.rstate, T.ra, T.rb, T.rhat, nullptr};
   const float2 *in = bufs[which_in];
   float2 *out = obufs[which_out];
-  // every load of the thread is issued before the first LDS write (M <= FFT_LDS_COMPLEX = 16 * 256)
+  // every load of the thread is issued before the first LDS write (n <= FFT_LDS_COMPLEX = 16 * 256)
   {
     float2 v[FFT_LOADS];
     float2 w[FFT_LOADS];
 #pragma unroll
     for (int k = 0; k < FFT_LOADS; k++) {
-      const uint32_t x = threadIdx.x + 256u * k;
-      if (x < M) { v[k] = fft_input(T, mode, in, x, y); w[k] = T.tw[x]; }
+      const uint32_t q = threadIdx.x + 256u * k;
+      if (q < n) v[k] = fft_input(T, mode, in, q & (M - 1u), y0 + (q >> lM));
+      if (q < M) w[k] = T.tw[q];
     }
 #pragma unroll
     for (int k = 0; k < FFT_LOADS; k++) {
-      const uint32_t x = threadIdx.x + 256u * k;
-      if (x < M) { lds[x] = v[k]; stw[x] = w[k]; }
+      const uint32_t q = threadIdx.x + 256u * k;
+      if (q < n) lds[q] = v[k];
+      if (q < M) stw[q] = w[k];
     }
   }
   __syncthreads();
   uint32_t cur = 0, lns = 1;
-  for (; lns + 1u <= lM; lns += 2u) {
-    const float2 *src = lds + cur * ldsM;
-    float2 *dst = lds + (cur ^ 1u) * ldsM;
-    for (uint32_t g = threadIdx.x; g < (M >> 2); g += blockDim.x) bfly4(src, dst, g, lns, lM, 1u, stw, sign);
+  for (; lns + 1u <= lM; lns += 2u) {  // row r of the workgroup at offset r * M of each buffer
+    const float2 *src = lds + cur * ldsP;
+    float2 *dst = lds + (cur ^ 1u) * ldsP;
+    for (uint32_t g = threadIdx.x; g < (n >> 2); g += blockDim.x) {
+      const uint32_t o = (g >> (lM - 2u)) << lM;
+      bfly4(src + o, dst + o, g & ((M >> 2) - 1u), lns, lM, 1u, stw, sign);
+    }
     __syncthreads();
     cur ^= 1u;
   }
   if (lns == lM) {  // odd log2 M: the last pass alone
-    const float2 *src = lds + cur * ldsM;
-    float2 *dst = lds + (cur ^ 1u) * ldsM;
-    for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) dst[x] = bfly(src, x, lns, lM, 1, stw, sign);
+    const float2 *src = lds + cur * ldsP;
+    float2 *dst = lds + (cur ^ 1u) * ldsP;
+    for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) {
+      const uint32_t o = (q >> lM) << lM;
+      dst[q] = bfly(src + o, q & (M - 1u), lns, lM, 1, stw, sign);
+    }
     __syncthreads();
     cur ^= 1u;
   }
-  for (uint32_t x = threadIdx.x; x < M; x += blockDim.x) out[y * M + x] = lds[cur * ldsM + x];
+  for (uint32_t q = threadIdx.x; q < n; q += blockDim.x) out[(size_t)y0 * M + q] = lds[cur * ldsP + q];
 }
 
 // all vertical passes of a tile of CW = 4096/M adjacent columns
@@ -294,21 +305,24 @@ void sl_k_ripple_frame_ex(const SlWork &rows_w, const SlWork &cols_w, const SlWo
   if (!rows_w.total) return;
   const SlLaunch rows = launch_over(rows_w, s, one), cols = launch_over(cols_w, s, one);
   const SlLaunch elems = launch_over(elems_w, s, one), tex = launch_over(tex_w, s, one);
-  const size_t rl = (size_t)3 * maxM * sizeof(float2);  // row LDS: two buffers of M points and the twiddles
+  // row LDS: two buffers of the most points a workgroup takes (M for M >= 1024, else M times the rows
+  // per workgroup, at most 1024) and the twiddles
+  const uint32_t ldsP = maxM > 1024u ? maxM : 1024u;
+  const size_t rl = ((size_t)2 * ldsP + maxM) * sizeof(float2);
   const size_t cl = (size_t)maxM * sizeof(float2);      // column LDS beside the static buffers: the twiddles
   // forward transform of the mirrored source -> hat (64-ripple.js:229-232)
-  SL_LAUNCH(k_fft_rows, dim3(rows_w.total), dim3(256), rl, rows, 0, 1, (int)IN_SRC, -1.0f, maxM);
+  SL_LAUNCH(k_fft_rows, dim3(rows_w.total), dim3(256), rl, rows, 0, 1, (int)IN_SRC, -1.0f, ldsP);
   SL_LAUNCH(k_fft_cols, dim3(cols_w.total), dim3(FFT_COL_THREADS), cl, cols, 1, 3, (int)OUT_BUF, -1.0f);
   // evolve the modes and add the kick (64-ripple.js:233)
   SL_LAUNCH(k_revolve, dim3(elems_w.total), dim3(256), 0, elems);
   // inverse transform of the state -> b (64-ripple.js:234-235)
-  SL_LAUNCH(k_fft_rows, dim3(rows_w.total), dim3(256), rl, rows, 0, 1, (int)IN_BUF, 1.0f, maxM);
+  SL_LAUNCH(k_fft_rows, dim3(rows_w.total), dim3(256), rl, rows, 0, 1, (int)IN_BUF, 1.0f, ldsP);
   SL_LAUNCH(k_fft_cols, dim3(cols_w.total), dim3(FFT_COL_THREADS), cl, cols, 1, 2, (int)OUT_BUF, 1.0f);
   // height and slopes (64-ripple.js:236-238), the mask, the source cleared (64-ripple.js:252)
   SL_LAUNCH(k_rtail, dim3(tex_w.total), dim3(256), 0, tex, mask);
   if (mask) {
     // absorb: mask the spatial field, transform back into the state, rescale (64-ripple.js:239-250)
-    SL_LAUNCH(k_fft_rows, dim3(rows_w.total), dim3(256), rl, rows, 2, 1, (int)IN_MASK, -1.0f, maxM);
+    SL_LAUNCH(k_fft_rows, dim3(rows_w.total), dim3(256), rl, rows, 2, 1, (int)IN_MASK, -1.0f, ldsP);
     SL_LAUNCH(k_fft_cols, dim3(cols_w.total), dim3(FFT_COL_THREADS), cl, cols, 1, 0, (int)OUT_SCALE, -1.0f);
   }
 }

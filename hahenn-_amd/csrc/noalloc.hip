@@ -110,12 +110,12 @@ extern "C" int sl_ripple_frame_noalloc(const sl_config *cfg, uint32_t R, double 
   hipLaunchKernelGGL(k_put_record, dim3(1), dim3(64), 0, st, drec, rec);
   if (hipMemcpyAsync(base + NA_TW, tw, (size_t)M * 8, hipMemcpyHostToDevice, st) != hipSuccess)
     return fail(SL_EHIP, "sl_ripple_frame_noalloc: twiddle copy failed");
-  // the work maps of encode_ripple (capi.hip): rows, column tiles of 4096 / M, elements, texels
+  // the work maps of encode_ripple (capi.hip): row groups, column tiles of 4096 / M, elements, texels
   uint32_t CW = 4096 / M;
   if (CW > M) CW = M;
   if (CW < 1) CW = 1;
   const uint32_t R1 = R + 1;
-  sl_k_ripple_frame_ex(work_of(drec, M, 1), work_of(drec, M, CW), work_of(drec, (uint64_t)M * M, SL_WG),
+  sl_k_ripple_frame_ex(work_of(drec, M / sl_fft_rows_per_wg(M), 1), work_of(drec, M, CW), work_of(drec, (uint64_t)M * M, SL_WG),
                        work_of(drec, (uint64_t)R1 * R1, SL_WG), st, 0, M);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess) return fail(SL_EHIP, hipGetErrorString(e));

@@ -213,6 +213,14 @@ template <class E>
 __device__ __forceinline__ void st32(E *base, uint32_t i, E v) {
   *(SL_GLOBAL E *)((SL_GLOBAL char *)base + i * (uint32_t)sizeof(E)) = v;
 }
+// rows of an M x M ripple field (M a power of two) per workgroup of the row transform: one for
+// M >= 1024, else 1024 / M (at most M), so that a workgroup of 256 threads holds about 1024 points;
+// the row work map of a tank is M / sl_fft_rows_per_wg(M) workgroups
+__host__ __device__ __forceinline__ uint32_t sl_fft_rows_per_wg(uint32_t M) {
+  if (M >= 1024u || M == 0u) return 1u;
+  const uint32_t r = 1024u / M;
+  return r < M ? r : M;
+}
 // n / d from the triple (m, s1, s2) that sl_magic_u32 computes for d: Granlund & Montgomery,
 // "Division by Invariant Integers using Multiplication" (PLDI 1994), fig. 4.1, exact for every
 // 32-bit n and 1 <= d < 2^31
